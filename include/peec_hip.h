@@ -1,0 +1,150 @@
+/*
+ * libpeec_hip -- C ABI of the MI355X (gfx950) voxel-PEEC hot path.
+ *
+ * Every entry point replaces one function of the reference's FFT solver
+ * (Draco112358/Solver, Julia, src/lib/solverFFT/).  The reference has no FFI:
+ * a Julia shim included after the reference files redefines each function
+ * name with the same positional signature and forwards here with ccall.
+ *
+ * Conventions
+ *  - Host pointers in and out unless a name says d_ (device).  The library
+ *    never keeps a host pointer after a call returns.
+ *  - Arrays follow Julia's column-major layout; an N x 3 matrix of centres
+ *    is x[0..N), y[0..N), z[0..N).
+ *  - Sparse matrices are Julia SparseMatrixCSC{Tv,Int64}: colptr[ncol+1] and
+ *    rowval[nnz] 1-based Int64, nzval[nnz].
+ *  - Complex numbers are interleaved {re, im} doubles (bit-compatible with
+ *    Julia ComplexF64).
+ *  - Return value: PEEC_OK, PEEC_STOPPED (the registered stop flag was set;
+ *    the shim returns the reference's stop sentinel), or a negative error
+ *    code; peec_last_error() then describes it.
+ *  - One context = one HIP stream.  Calls on distinct contexts may run from
+ *    distinct threads; calls on one context must not overlap.
+ */
+#ifndef PEEC_HIP_H
+#define PEEC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct peec_ctx peec_ctx;
+
+typedef struct {
+    double re, im;
+} peec_c128;
+
+enum {
+    PEEC_OK = 0,
+    PEEC_STOPPED = 1,
+    PEEC_ERR_ARG = -1,
+    PEEC_ERR_HIP = -2,
+    PEEC_ERR_STATE = -3,
+    PEEC_ERR_NUMERIC = -4,
+    PEEC_ERR_NOMEM = -5
+};
+
+/* ---- context ----------------------------------------------------------- */
+int peec_ctx_create(int device, peec_ctx **out);
+int peec_ctx_destroy(peec_ctx *ctx);
+const char *peec_last_error(void);
+/* Cooperative stop (is_stop_requested, solver_start.jl:370-374): the
+ * library polls *flag between GMRES iterations and integral tiles. */
+int peec_set_stop_flag(peec_ctx *ctx, volatile const int32_t *flag);
+
+/* ---- partial-element rows (FP64) --------------------------------------- */
+/* compute_Lp_Voxels(centri_m, centri_n, sx, sy, sz, sx2, sy2, sz2, dc,
+ *                   is_sym=false, id)            compute_Lp_Voxels.jl:1-63
+ * centri_m: 3 doubles (one source centre); centri_n: N x 3; out: N. */
+int peec_lp_row(peec_ctx *ctx, const double *centri_m, const double *centri_n, int64_t N,
+                double sx, double sy, double sz, double sx2, double sy2, double sz2,
+                int dc, double *out);
+
+/* compute_row_P_sup(centers, centers2, sx, sy, sz, caser, caser2, id)
+ *                                                compute_row_P_sup.jl:1-104 */
+int peec_p_row(peec_ctx *ctx, const double *centers, const double *centers2, int64_t N,
+               double sx, double sy, double sz, int caser, int caser2, double *out);
+
+/* Lp_self(l, W, T)                               compute_diagonals.jl:136-155 */
+int peec_lp_self(double l, double W, double T, double *out);
+
+/* ---- circulant kernels -------------------------------------------------- */
+/* compute_FFT_mutual_coupling_mats(circulant_centers, escalings, Nx, Ny, Nz,
+ *   QS_Rcc_FW=1, id, chan)             compute_FFT_mutual_coupling_mats.jl:1-33
+ * The circulant centres are generated on the device from the voxel grid
+ * (build_center_P_Voxels.jl:1-111, build_centers_Lp_with_air.jl:1-60,
+ * build_center_air_Voxels.jl:1-26).  The 3 Lp and 6 P rows are embedded and
+ * transformed into device-resident real spectra owned by the context. */
+int peec_kernels_build(peec_ctx *ctx, const double origin[3], int Nx, int Ny, int Nz,
+                       double sx, double sy, double sz, double esc_Lp, double esc_P);
+/* Copy one scaled kernel row back (kind 0..2 = Lp x,y,z; 3..8 = P 11, 22,
+ * 33, 12, 13, 23).  out holds the compact grid (x fastest); *n receives its
+ * length.  Pass out = NULL to query the length. */
+int peec_kernel_row(peec_ctx *ctx, int kind, double *out, int64_t *n);
+
+/* ---- operator (per geometry) -------------------------------------------- */
+/* The sparse data of ComputeMatrixVectorFTT (compute_Matrix_vector_fft.jl:1-81):
+ *   A       m x n   (incidence_selection["A"], +-1)
+ *   Gamma   n x ns  (incidence_selection["Gamma"], 0/1)
+ *   lp_map[d] (d = 0..2)  N_ind[d] x m_d   (expansions["mat_map_Lp"][d,1])
+ *   p_map[3*j + i] (i, j = 0..2) rows(grid) x ns  (expansions["exp_P"][i,j])
+ *   diag_P  ns      (diagonals["P"]; invP = 1 ./ diag_P)
+ * All maps are 0/1 with one entry per column (asserted). */
+typedef struct {
+    const int64_t *colptr;
+    const int64_t *rowval;
+    const double *nzval;
+    int64_t nrow, ncol;
+} peec_csc_f64;
+
+typedef struct {
+    const int64_t *colptr;
+    const int64_t *rowval;
+    const peec_c128 *nzval;
+    int64_t nrow, ncol;
+} peec_csc_c128;
+
+int peec_operator_set(peec_ctx *ctx, int64_t mx, int64_t my, int64_t mz,
+                      const peec_csc_f64 *A, const peec_csc_f64 *Gamma,
+                      const peec_csc_f64 lp_map[3], const peec_csc_f64 p_map[9],
+                      const double *diag_P);
+
+/* Per-frequency data of FFT_solver_QS_S_type.jl:105-119: w (scaled angular
+ * frequency), DZ (m), Yle (n x n CSC). */
+int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, const peec_csc_c128 *Yle);
+
+/* LU factors of SS = Yle + A' Z^-1 A + j w Gamma P^-1 Gamma'
+ * (F = lu(SS), FFT_solver_QS_S_type.jl:118-119) in UMFPACK's convention,
+ * which Julia exposes as F.L, F.U, F.p, F.q, F.Rs:
+ *     L * U = (Rs .* SS)[p, q]
+ * L lower and U upper triangular (n x n CSC, diagonals stored), p and q
+ * 0-based permutations, Rs the row scaling (NULL = no scaling).  The
+ * triangular solves run on the GPU (sync-free, one wavefront per row). */
+int peec_lu_set_factors(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L, const peec_csc_c128 *U,
+                        const int64_t *p, const int64_t *q, const double *Rs);
+
+/* ComputeMatrixVectorFTT(x, w, ...) including the precond_3_3_vector step
+ * (compute_Matrix_vector_fft.jl:1-146): y = M^-1 K x, x and y of length
+ * m + ns + n. */
+int peec_matvec(peec_ctx *ctx, const peec_c128 *x, peec_c128 *y);
+
+/* K x alone, without M^-1 (the first part of ComputeMatrixVectorFTT,
+ * compute_Matrix_vector_fft.jl:1-77): exposes the operator for checks. */
+int peec_apply_K(peec_ctx *ctx, const peec_c128 *x, peec_c128 *y);
+
+/* precond_3_3_Kt_fft!(F, invZ, invP, A, Gamma, m, ns, is, tn, resProd)
+ *                                         FFT_solver_QS_S_type.jl:237-262 */
+int peec_precond_rhs(peec_ctx *ctx, const double *is, peec_c128 *tn);
+
+/* gmres_custom(b, restarted=false, tol, maxit, x, ...)  gmres_custom.jl:1-331
+ * x is the warm start on input and the solution on output.  resvec must
+ * hold maxit + 1 doubles; *nres receives its used length. */
+int peec_gmres(peec_ctx *ctx, const peec_c128 *b, double tol, int maxit, peec_c128 *x,
+               int *flag, double *relres, int64_t iter[2], double *resvec, int64_t *nres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PEEC_HIP_H */

@@ -1,0 +1,75 @@
+"""Loader of libpeec_hip.so (ctypes).  Fails loudly when the library is
+missing: there is no CPU fallback for any entry point."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpeec_hip.so")
+
+PEEC_OK = 0
+PEEC_STOPPED = 1
+
+_lib = None
+
+i64 = ctypes.c_int64
+dp = ctypes.POINTER(ctypes.c_double)
+i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class C128(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+c128p = ctypes.POINTER(C128)
+
+
+class CscF64(ctypes.Structure):
+    _fields_ = [("colptr", i64p), ("rowval", i64p), ("nzval", dp), ("nrow", i64), ("ncol", i64)]
+
+
+class CscC128(ctypes.Structure):
+    _fields_ = [("colptr", i64p), ("rowval", i64p), ("nzval", c128p), ("nrow", i64), ("ncol", i64)]
+
+
+class PeecError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PeecError("libpeec_hip.so is not built (%s); run python solver_amd/build.py" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.peec_last_error.restype = ctypes.c_char_p
+        L.peec_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.peec_ctx_destroy.argtypes = [vp]
+        L.peec_set_stop_flag.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+        L.peec_lp_row.argtypes = [vp, dp, dp, i64] + [ctypes.c_double] * 6 + [ctypes.c_int, dp]
+        L.peec_p_row.argtypes = [vp, dp, dp, i64] + [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_int, dp]
+        L.peec_lp_self.argtypes = [ctypes.c_double] * 3 + [dp]
+        for name in ("peec_kernels_build", "peec_kernel_row", "peec_operator_set", "peec_frequency_set",
+                     "peec_lu_set_factors", "peec_matvec", "peec_precond_rhs", "peec_gmres"):
+            if hasattr(L, name):
+                getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc == PEEC_OK or rc == PEEC_STOPPED:
+        return rc
+    msg = lib().peec_last_error()
+    raise PeecError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+def as_dp(a):
+    return a.ctypes.data_as(dp)
+
+
+def as_c128p(a):
+    assert a.dtype == np.complex128 and a.flags.c_contiguous
+    return a.ctypes.data_as(c128p)

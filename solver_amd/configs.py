@@ -1,0 +1,123 @@
+"""Seeded geometries of the five benchmark configurations.
+
+Each builder returns a dict with the material grid (0 = empty, k = material
+k), the materials, cell size, port coordinates, frequencies and GMRES
+tolerance.  Ports are given as coordinates (metres) and resolved to voxels
+by the mesher the way find_voxels_port_pp.jl does (nearest voxel centre).
+"""
+import numpy as np
+import scipy.ndimage as ndi
+
+from .mesher import Material
+
+COPPER = 5.8e7
+
+
+def crossbar_24x24x4():
+    """24x24x4 grid, 0.5 x 0.5 x 0.1 mm cells, copper x-bar (y = 10..13) and
+    y-bar (x = 10..13) in layers z = 0..1; one port from the -x end face to
+    the +x end face of the x-bar; 10 frequencies 1e6..1e9 Hz; tol 1e-8."""
+    Nx, Ny, Nz = 24, 24, 4
+    cell = (0.5e-3, 0.5e-3, 0.1e-3)
+    g = np.zeros((Nx, Ny, Nz), dtype=np.int64)
+    g[:, 10:14, 0:2] = 1
+    g[10:14, :, 0:2] = 1
+    yc = 12 * cell[1]
+    zc = 1 * cell[2]
+    ports = [((0.0, yc, zc), (Nx * cell[0], yc, zc))]
+    return dict(name="crossbar_24x24x4", grid=g, materials=[Material("copper", COPPER)], cell=cell,
+                origin=(0.0, 0.0, 0.0), ports=ports, freqs=np.logspace(6, 9, 10), tol=1e-8)
+
+
+def _largest_component(mask):
+    lab, nlab = ndi.label(mask)  # 6-connectivity
+    if nlab == 0:
+        return np.zeros_like(mask)
+    sizes = np.bincount(lab.reshape(-1))
+    sizes[0] = 0
+    return lab == np.argmax(sizes)
+
+
+def _layered(Nx, Ny, Nz, cell, z_diel, p_cu, eps_r, tan_d, seed, sigma_range=None):
+    rng = np.random.default_rng(seed)
+    g = np.zeros((Nx, Ny, Nz), dtype=np.int64)
+    g[:, :, :z_diel] = 2
+    cu = rng.random((Nx, Ny, Nz - z_diel)) < p_cu
+    cu = _largest_component(cu)
+    g[:, :, z_diel:][cu] = 1
+    mats = [Material("copper", COPPER), Material("dielectric", 0.0, eps_r, tan_d)]
+    sigma = None
+    if sigma_range is not None:
+        sigma = np.where(g == 1, rng.uniform(sigma_range[0], sigma_range[1], g.shape), 0.0)
+    return g, mats, sigma
+
+
+def _port_to_slab(g, cell, vox, z_diel):
+    """Port from a copper voxel (given by index) to the slab voxel under its
+    column at the top dielectric layer (coordinates of the two centres)."""
+    i, j, k = vox
+    a = ((i + 0.5) * cell[0], (j + 0.5) * cell[1], (k + 0.5) * cell[2])
+    b = ((i + 0.5) * cell[0], (j + 0.5) * cell[1], (z_diel - 0.5) * cell[2])
+    return (a, b)
+
+
+def _x_face_ports(g, cell, z_diel):
+    cu = np.argwhere(g == 1)
+    lo = cu[np.lexsort((cu[:, 2], cu[:, 1], cu[:, 0]))][0]
+    hi = cu[np.lexsort((cu[:, 2], cu[:, 1], -cu[:, 0]))][0]
+    return [_port_to_slab(g, cell, lo, z_diel), _port_to_slab(g, cell, hi, z_diel)]
+
+
+def two_material_64x64x16(seed=0):
+    """64x64x16, 0.25 x 0.25 x 0.05 mm; z = 0..11 dielectric (eps_r 4.4,
+    tan d 0.02); z = 12..15 copper Bernoulli(0.35) reduced to its largest
+    6-connected component; two ports on the component's -x and +x faces
+    (each from the face voxel to the slab voxel below); 32 frequencies
+    1e7..1e10 Hz; tol 1e-10."""
+    Nx, Ny, Nz = 64, 64, 16
+    cell = (0.25e-3, 0.25e-3, 0.05e-3)
+    g, mats, _ = _layered(Nx, Ny, Nz, cell, 12, 0.35, 4.4, 0.02, seed)
+    return dict(name="two_material_64x64x16", grid=g, materials=mats, cell=cell, origin=(0.0, 0.0, 0.0),
+                ports=_x_face_ports(g, cell, 12), freqs=np.logspace(7, 10, 32), tol=1e-10)
+
+
+def batched_96x96x24(seed=0, nports=8):
+    """96x96x24, same recipe with copper Bernoulli(0.3); 8 ports on boundary
+    voxels of the largest component chosen by the seed; 128 frequencies
+    1e6..2e10 Hz."""
+    Nx, Ny, Nz = 96, 96, 24
+    cell = (0.25e-3, 0.25e-3, 0.05e-3)
+    g, mats, _ = _layered(Nx, Ny, Nz, cell, 18, 0.3, 4.4, 0.02, seed)
+    rng = np.random.default_rng(seed + 1000)
+    cu = np.argwhere(g == 1)
+    pick = cu[np.sort(rng.choice(len(cu), size=min(nports, len(cu)), replace=False))]
+    ports = [_port_to_slab(g, cell, v, 18) for v in pick]
+    return dict(name="batched_96x96x24", grid=g, materials=mats, cell=cell, origin=(0.0, 0.0, 0.0), ports=ports,
+                freqs=np.logspace(6, np.log10(2e10), 128), tol=1e-6)
+
+
+def adjoint_64x64x16(seed=0):
+    d = two_material_64x64x16(seed)
+    g, mats, sigma = _layered(64, 64, 16, d["cell"], 12, 0.35, 4.4, 0.02, seed, sigma_range=(3e7, 6e7))
+    d.update(name="adjoint_64x64x16", sigma=sigma)
+    return d
+
+
+def large_192x192x48(seed=0):
+    """192x192x48 cubic 0.1 mm cells; eps_r 3.5 slab z = 0..35, copper
+    Bernoulli(0.25) in z = 36..47 (largest component); one port; 16
+    frequencies 1e8..1e10 Hz."""
+    Nx, Ny, Nz = 192, 192, 48
+    cell = (0.1e-3, 0.1e-3, 0.1e-3)
+    g, mats, _ = _layered(Nx, Ny, Nz, cell, 36, 0.25, 3.5, 0.0, seed)
+    return dict(name="large_192x192x48", grid=g, materials=mats, cell=cell, origin=(0.0, 0.0, 0.0),
+                ports=_x_face_ports(g, cell, 36)[:1], freqs=np.logspace(8, 10, 16), tol=1e-6)
+
+
+CONFIGS = {
+    "crossbar_24x24x4": crossbar_24x24x4,
+    "two_material_64x64x16": two_material_64x64x16,
+    "batched_96x96x24": batched_96x96x24,
+    "adjoint_64x64x16": adjoint_64x64x16,
+    "large_192x192x48": large_192x192x48,
+}

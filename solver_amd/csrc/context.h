@@ -1,0 +1,138 @@
+// Context object behind the opaque peec_ctx handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <vector>
+
+#include "peec_internal.h"
+
+// A device array that frees itself.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    bool alloc(size_t count)
+    {
+        if (count == n && p) return true;
+        reset();
+        if (count == 0) return true;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        n = count;
+        return true;
+    }
+};
+
+struct double2_t {
+    double x, y;
+};
+
+// One circulant kernel (3 Lp + 6 P): compact dims, embedded dims and its
+// real spectrum (already divided by Lx*Ly*Lz) in FP64 and FP32.
+struct Kernel {
+    int n[3] = {0, 0, 0};
+    int L[3] = {0, 0, 0};
+    DevBuf<double> row;      // compact row (scaled), kept for inspection
+    DevBuf<double> spec64;   // L0*L1*L2
+    DevBuf<float> spec32;
+    int64_t compact() const { return (int64_t)n[0] * n[1] * n[2]; }
+    int64_t embedded() const { return (int64_t)L[0] * L[1] * L[2]; }
+};
+
+// One FFT-convolution application of the matvec: scatter unknowns of one
+// section of x into the kernel's compact grid, convolve, gather into a
+// section of y.  map_in/map_out: compact position -> unknown (or -1).
+struct ConvApp {
+    int kernel = -1;
+    int64_t in_off = 0, out_off = 0;  // section offsets inside x / y
+    DevBuf<int32_t> map_in, map_out;
+    int64_t nnz_in = 0, nnz_out = 0;
+};
+
+// Sparse matrix in CSR (0-based, int32 indices) with real or complex values.
+struct CsrR {
+    int64_t nrow = 0, ncol = 0;
+    DevBuf<int32_t> ptr, idx;
+    DevBuf<double> val;
+};
+struct CsrC {
+    int64_t nrow = 0, ncol = 0;
+    DevBuf<int32_t> ptr, idx;
+    DevBuf<double2_t> val;
+};
+
+// Triangular factor prepared for the sync-free solve: rows in solve order,
+// each row's off-diagonal entries plus its diagonal.
+struct TriFactor {
+    int64_t n = 0;
+    DevBuf<int32_t> order;     // row solved by wave w
+    DevBuf<int32_t> ptr, idx;  // off-diagonal entries per row (by row id)
+    DevBuf<double2_t> val;
+    DevBuf<double2_t> diag;    // diagonal per row (1 for unit)
+};
+
+struct peec_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    volatile const int32_t *stop_flag = nullptr;
+
+    static constexpr int kScratchSlots = 8;
+    void *scratch_ptr[kScratchSlots] = {};
+    size_t scratch_bytes[kScratchSlots] = {};
+    void *scratch(size_t bytes, int slot);
+    bool stop_requested() const;
+    void release_all();
+
+    // ---- kernels
+    bool kernels_ready = false;
+    Kernel kern[9];
+    void release_kernels();
+
+    // ---- FFT work buffers (complex, sized for the largest application)
+    DevBuf<double2_t> w1, w2;
+    DevBuf<double2_t> twiddles;             // concatenated exp(-2 pi i t / L) tables
+    std::map<int, int64_t> twiddle_off;     // L -> offset into twiddles
+    int64_t twiddle_offset(int L);          // builds the table on first use
+
+    // ---- operator
+    bool op_ready = false;
+    int64_t m = 0, ns = 0, n = 0, mx = 0, my = 0, mz = 0;
+    ConvApp apps[12];
+    int napps = 0;
+    CsrR A, At, G, Gt;           // A (m x n), A' (n x m), Gamma (n x ns), Gamma' (ns x n)
+    DevBuf<double> invP;         // ns
+    void release_operator();
+
+    // ---- frequency
+    bool freq_ready = false;
+    double w = 0.0;
+    DevBuf<double2_t> DZ, invZ;  // m
+    CsrC Yle;                    // n x n
+
+    // ---- LU
+    bool lu_ready = false;
+    TriFactor Lf, Uf;
+    DevBuf<int32_t> perm_r, perm_c;  // inverse row / column permutations (0-based)
+    DevBuf<double> Rs;               // optional row scaling of SS
+    bool has_Rs = false;
+    DevBuf<uint32_t> flags;          // completion flags (generation stamped)
+    DevBuf<int> err;                 // set when a dependency wait hits its spin limit
+    uint32_t generation = 0;
+
+    // ---- vectors
+    DevBuf<double2_t> vx, vy, vt1, vt2, vt3;  // N-sized work vectors
+};

@@ -1,0 +1,662 @@
+// LDS-staged mixed-radix FFT passes and the circulant convolution pipeline.
+//
+// The reference convolves every Toeplitz block with FFTW over a 2x
+// zero-padded grid (compute_Matrix_vector_fft.jl:14-65, 149-161) and builds
+// the kernel spectra with fft(store_circulant_fft(row)) (compute_FFT_mutual_
+// coupling_mats.jl:57-125, 130-155).  Here:
+//   - each axis uses its own smooth length L >= 2n-1 (2^a 3^b 5^c); any such
+//     length gives the same linear convolution as the reference's 2n;
+//   - the even-symmetric embedding has a real spectrum, stored as a real
+//     array (FP64 and FP32 copies) with 1/(Lx Ly Lz) folded in;
+//   - one convolution = five passes over HBM: x-FFT (fused with the 0/1
+//     scatter from the unknown vector, zero padding pruned), y-FFT, a fused
+//     z-FFT * spectrum * z-IFFT, y-IFFT (output pruned to the compact rows)
+//     and x-IFFT fused with the 0/1 gather back into the result vector.
+//
+// A pass loads a tile of TB lines into LDS (coalesced along whichever axis
+// is contiguous), runs a Stockham autosort FFT with TPL threads per line
+// (radix 8/4/2/3/5 butterflies held in registers between barriers), and
+// stores the tile back.
+#include "context.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+// ------------------------------------------------------------------ plans
+int fft_smooth_length(int nmin)
+{
+    if (nmin <= 1) return 1;
+    for (int L = nmin;; L++) {
+        int r = L;
+        while (r % 2 == 0) r /= 2;
+        while (r % 3 == 0) r /= 3;
+        while (r % 5 == 0) r /= 5;
+        if (r == 1) return L;
+    }
+}
+
+bool fft_make_plan(int L, FftPlan1D *p)
+{
+    p->L = L;
+    p->nstages = 0;
+    int r = L;
+    while (r > 1) {
+        int R;
+        if (r % 8 == 0) R = 8;
+        else if (r % 4 == 0) R = 4;
+        else if (r % 2 == 0) R = 2;
+        else if (r % 3 == 0) R = 3;
+        else if (r % 5 == 0) R = 5;
+        else return false;
+        if (p->nstages >= 16) return false;
+        p->radix[p->nstages++] = R;
+        r /= R;
+    }
+    return true;
+}
+
+int64_t peec_ctx::twiddle_offset(int L)
+{
+    auto it = twiddle_off.find(L);
+    if (it != twiddle_off.end()) return it->second;
+    // rebuild the concatenated table with the new length appended
+    std::vector<int> lens;
+    for (auto &kv : twiddle_off) lens.push_back(kv.first);
+    lens.push_back(L);
+    int64_t total = 0;
+    for (int l : lens) total += l;
+    std::vector<double2_t> h((size_t)total);
+    std::map<int, int64_t> offs;
+    int64_t o = 0;
+    for (int l : lens) {
+        offs[l] = o;
+        for (int t = 0; t < l; t++) {
+            long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)l;
+            h[(size_t)(o + t)] = {(double)cosl(a), (double)sinl(a)};
+        }
+        o += l;
+    }
+    DevBuf<double2_t> nb;
+    if (!nb.alloc((size_t)total)) return -1;
+    if (hipMemcpy(nb.p, h.data(), sizeof(double2_t) * (size_t)total, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    std::swap(twiddles.p, nb.p);
+    std::swap(twiddles.n, nb.n);
+    twiddle_off = offs;
+    return twiddle_off[L];
+}
+
+void peec_ctx::release_kernels()
+{
+    for (auto &k : kern) {
+        k.row.reset();
+        k.spec64.reset();
+        k.spec32.reset();
+        k.n[0] = k.n[1] = k.n[2] = 0;
+    }
+    kernels_ready = false;
+}
+
+// ------------------------------------------------------------------ device
+namespace {
+
+template <typename T>
+struct C2 {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename T>
+__device__ __forceinline__ C2<T> cscale(C2<T> a, T s) { return {a.x * s, a.y * s}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename T, int DIR>
+__device__ __forceinline__ C2<T> rot(C2<T> a) { return DIR < 0 ? C2<T>{a.y, -a.x} : C2<T>{-a.y, a.x}; }
+
+template <typename T, int DIR>
+__device__ __forceinline__ void dft2(C2<T> *v)
+{
+    C2<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void dft4(C2<T> *v)
+{
+    C2<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    C2<T> s13 = cadd(v[1], v[3]), d13 = rot<T, DIR>(csub(v[1], v[3]));
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void dft8(C2<T> *v)
+{
+    C2<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<T, DIR>(e);
+    dft4<T, DIR>(o);
+    const T h = (T)0.70710678118654752440084436210485;
+    // o[k] *= exp(DIR * 2 pi i k / 8)
+    C2<T> w1 = DIR < 0 ? C2<T>{h, -h} : C2<T>{h, h};
+    C2<T> w3 = DIR < 0 ? C2<T>{-h, -h} : C2<T>{-h, h};
+    o[1] = cmul(o[1], w1);
+    o[2] = rot<T, DIR>(o[2]);
+    o[3] = cmul(o[3], w3);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        v[k] = cadd(e[k], o[k]);
+        v[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void dft3(C2<T> *v)
+{
+    const T s = (T)0.86602540378443864676372317075294;
+    C2<T> b = cadd(v[1], v[2]);
+    C2<T> d = rot<T, DIR>(cscale(csub(v[1], v[2]), s));
+    C2<T> t = csub(v[0], cscale(b, (T)0.5));
+    v[0] = cadd(v[0], b);
+    v[1] = cadd(t, d);
+    v[2] = csub(t, d);
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void dft5(C2<T> *v)
+{
+    const T c1 = (T)0.30901699437494742410229341718282, c2 = (T)-0.80901699437494742410229341718282;
+    const T s1 = (T)0.95105651629515357211643933337938, s2 = (T)0.58778525229247312916870595463907;
+    C2<T> b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+    C2<T> d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+    C2<T> a0 = v[0];
+    C2<T> r1 = cadd(a0, cadd(cscale(b1, c1), cscale(b2, c2)));
+    C2<T> r2 = cadd(a0, cadd(cscale(b1, c2), cscale(b2, c1)));
+    C2<T> i1 = rot<T, DIR>(cadd(cscale(d1, s1), cscale(d2, s2)));
+    C2<T> i2 = rot<T, DIR>(csub(cscale(d1, s2), cscale(d2, s1)));
+    v[0] = cadd(a0, cadd(b1, b2));
+    v[1] = cadd(r1, i1);
+    v[4] = csub(r1, i1);
+    v[2] = cadd(r2, i2);
+    v[3] = csub(r2, i2);
+}
+
+template <typename T, int DIR, int R>
+__device__ __forceinline__ void dftR(C2<T> *v)
+{
+    if (R == 2) dft2<T, DIR>(v);
+    else if (R == 3) dft3<T, DIR>(v);
+    else if (R == 4) dft4<T, DIR>(v);
+    else if (R == 5) dft5<T, DIR>(v);
+    else dft8<T, DIR>(v);
+}
+
+// One Stockham stage on a line of length L held in LDS (buf).  Each of the
+// TPL lanes of the line owns butterflies k = lane + q*TPL.
+template <typename T, int DIR, int R, int TPL, int LMAX>
+__device__ __forceinline__ void stockham_stage(C2<T> *buf, int L, int Ns, int lane, const double2_t *__restrict__ tw)
+{
+    constexpr int MAXB = (LMAX + R * TPL - 1) / (R * TPL);
+    const int nb = L / R;
+    const int stride = L / (Ns * R);
+    C2<T> v[MAXB][R];
+#pragma unroll
+    for (int q = 0; q < MAXB; q++) {
+        int k = lane + q * TPL;
+        if (k < nb) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[q][r] = buf[k + r * nb];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; q++) {
+        int k = lane + q * TPL;
+        if (k < nb) {
+            int kk = k % Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    double2_t w = tw[kk * r * stride];
+                    C2<T> wt = {(T)w.x, (T)(DIR < 0 ? w.y : -w.y)};
+                    v[q][r] = cmul(v[q][r], wt);
+                }
+            }
+            dftR<T, DIR, R>(v[q]);
+            int base = (k - kk) * R + kk;
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[base + r * Ns] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+struct PassDesc {
+    int L, nin, nout;
+    int nstage;
+    int radix[16];
+    int64_t tw_off;
+    int nb, n2;                          // lines: b (consecutive, tiled) x l2
+    int64_t in_sb, in_s2, in_sj;
+    int64_t out_sb, out_s2, out_sj;
+    int64_t in_rhs, out_rhs;             // stride between right-hand sides
+    int nrhs;
+    const int32_t *map_in;               // optional 0/1 scatter (compact position -> unknown)
+    const double2_t *vec_in;
+    int64_t vec_in_ld;                   // stride between right-hand sides in vec_in
+    const int32_t *map_out;              // optional 0/1 gather
+    double2_t *vec_out;
+    int64_t vec_out_ld;
+    const void *spec;                    // optional real spectrum (T) for the fused z pass
+    int64_t spec_sb, spec_s2, spec_sj;
+};
+
+template <typename T, int DIR, int TPL, int LMAX>
+__device__ void run_fft(C2<T> *line, int L, const PassDesc &d, int lane, const double2_t *tw)
+{
+    int Ns = 1;
+    for (int s = 0; s < d.nstage; s++) {
+        int R = d.radix[s];
+        switch (R) {
+        case 8: stockham_stage<T, DIR, 8, TPL, LMAX>(line, L, Ns, lane, tw); break;
+        case 4: stockham_stage<T, DIR, 4, TPL, LMAX>(line, L, Ns, lane, tw); break;
+        case 2: stockham_stage<T, DIR, 2, TPL, LMAX>(line, L, Ns, lane, tw); break;
+        case 3: stockham_stage<T, DIR, 3, TPL, LMAX>(line, L, Ns, lane, tw); break;
+        default: stockham_stage<T, DIR, 5, TPL, LMAX>(line, L, Ns, lane, tw); break;
+        }
+        Ns *= R;
+    }
+}
+
+// MODE: 0 forward, 1 inverse, 2 forward * spectrum * inverse.
+// src/dst are grid buffers of T2 (ignored when map_in / map_out are used).
+template <typename T, int MODE, int TPL, int LMAX>
+__global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, C2<T> *dst,
+                                                  const double2_t *__restrict__ twall)
+{
+    constexpr int TB = 256 / TPL;
+    extern __shared__ char smem_raw[];
+    C2<T> *tile = reinterpret_cast<C2<T> *>(smem_raw);
+    const int L = d.L;
+    const int tid = threadIdx.x;
+    const int b0 = blockIdx.x * TB;
+    const int l2 = blockIdx.y;
+    const int rhs = blockIdx.z;
+    const int nbt = min(TB, d.nb - b0);
+    const double2_t *tw = twall + d.tw_off;
+    const bool contig = (d.in_sj == 1);
+
+    // ---- load (zero padding beyond nin)
+    const int64_t ibase = (int64_t)b0 * d.in_sb + (int64_t)l2 * d.in_s2 + (int64_t)rhs * d.in_rhs;
+    if (contig) {
+        for (int b = 0; b < TB; b++) {
+            for (int j = tid; j < L; j += 256) {
+                C2<T> v = {(T)0, (T)0};
+                if (b < nbt && j < d.nin) {
+                    int64_t p = ibase + (int64_t)b * d.in_sb + j;
+                    if (d.map_in) {
+                        int u = d.map_in[p - (int64_t)rhs * d.in_rhs];
+                        if (u >= 0) {
+                            double2_t x = d.vec_in[(int64_t)u + (int64_t)rhs * d.vec_in_ld];
+                            v = {(T)x.x, (T)x.y};
+                        }
+                    } else {
+                        v = src[p];
+                    }
+                }
+                tile[b * L + j] = v;
+            }
+        }
+    } else {
+        for (int e = tid; e < TB * L; e += 256) {
+            int b = e % TB, j = e / TB;
+            C2<T> v = {(T)0, (T)0};
+            if (b < nbt && j < d.nin) v = src[ibase + (int64_t)b * d.in_sb + (int64_t)j * d.in_sj];
+            tile[b * L + j] = v;
+        }
+    }
+    __syncthreads();
+
+    // ---- transform
+    const int lb = tid / TPL, lane = tid % TPL;
+    C2<T> *line = tile + lb * L;
+    if (MODE == 1) {
+        run_fft<T, 1, TPL, LMAX>(line, L, d, lane, tw);
+    } else {
+        run_fft<T, -1, TPL, LMAX>(line, L, d, lane, tw);
+        if (MODE == 2) {
+            const T *sp = reinterpret_cast<const T *>(d.spec);
+            const int64_t sbase = (int64_t)b0 * d.spec_sb + (int64_t)l2 * d.spec_s2;
+            for (int e = tid; e < TB * L; e += 256) {
+                int b = e % TB, j = e / TB;
+                if (b < nbt) {
+                    T s = sp[sbase + (int64_t)b * d.spec_sb + (int64_t)j * d.spec_sj];
+                    tile[b * L + j] = cscale(tile[b * L + j], s);
+                }
+            }
+            __syncthreads();
+            run_fft<T, 1, TPL, LMAX>(line, L, d, lane, tw);
+        }
+    }
+
+    // ---- store (first nout elements)
+    const int64_t obase = (int64_t)b0 * d.out_sb + (int64_t)l2 * d.out_s2 + (int64_t)rhs * d.out_rhs;
+    if (d.out_sj == 1) {
+        for (int b = 0; b < nbt; b++) {
+            for (int j = tid; j < d.nout; j += 256) {
+                C2<T> v = tile[b * L + j];
+                int64_t p = obase + (int64_t)b * d.out_sb + j;
+                if (d.map_out) {
+                    int u = d.map_out[p - (int64_t)rhs * d.out_rhs];
+                    if (u >= 0) {
+                        double2_t *o = d.vec_out + (int64_t)u + (int64_t)rhs * d.vec_out_ld;
+                        double2_t a = *o;
+                        a.x += (double)v.x;
+                        a.y += (double)v.y;
+                        *o = a;
+                    }
+                } else {
+                    dst[p] = v;
+                }
+            }
+        }
+    } else {
+        for (int e = tid; e < TB * d.nout; e += 256) {
+            int b = e % TB, j = e / TB;
+            if (b < nbt) dst[obase + (int64_t)b * d.out_sb + (int64_t)j * d.out_sj] = tile[b * L + j];
+        }
+    }
+}
+
+// Even-symmetric embedding of a compact row into an L grid
+// (store_circulant_fft, compute_FFT_mutual_coupling_mats.jl:130-155, at
+// lengths L >= 2n - 1 instead of 2n).
+__global__ void k_embed(const double *__restrict__ row, int nx, int ny, int nz, int Lx, int Ly, int Lz,
+                        double2_t *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t G = (int64_t)Lx * Ly * Lz;
+    if (i >= G) return;
+    int x = (int)(i % Lx);
+    int64_t r = i / Lx;
+    int y = (int)(r % Ly);
+    int z = (int)(r / Ly);
+    int sx = x < nx ? x : (Lx - x < nx ? Lx - x : -1);
+    int sy = y < ny ? y : (Ly - y < ny ? Ly - y : -1);
+    int sz = z < nz ? z : (Lz - z < nz ? Lz - z : -1);
+    double v = 0.0;
+    if (sx >= 0 && sy >= 0 && sz >= 0) v = row[sx + (int64_t)nx * (sy + (int64_t)ny * sz)];
+    out[i] = {v, 0.0};
+}
+
+__global__ void k_spec_extract(const double2_t *__restrict__ g, int64_t G, double scale, double *__restrict__ s64,
+                               float *__restrict__ s32)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G) return;
+    double v = g[i].x * scale;
+    s64[i] = v;
+    s32[i] = (float)v;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host side
+struct PassGeom {
+    int L, nin, nout, nb, n2;
+    int64_t in_sb, in_s2, in_sj, out_sb, out_s2, out_sj;
+};
+
+template <typename T, int MODE>
+static int launch_pass_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst)
+{
+    const int L = d.L;
+    int TPL = L <= 128 ? 16 : (L <= 512 ? 32 : 64);
+    int TB = 256 / TPL;
+    size_t lds = (size_t)TB * L * sizeof(C2<T>);
+    dim3 grid((unsigned)((d.nb + TB - 1) / TB), (unsigned)d.n2, (unsigned)d.nrhs);
+    const double2_t *tw = ctx->twiddles.p;
+    const C2<T> *s = (const C2<T> *)src;
+    C2<T> *o = (C2<T> *)dst;
+    if (TPL == 16)
+        hipLaunchKernelGGL((k_fft_pass<T, MODE, 16, 128>), grid, dim3(256), lds, ctx->stream, d, s, o, tw);
+    else if (TPL == 32)
+        hipLaunchKernelGGL((k_fft_pass<T, MODE, 32, 512>), grid, dim3(256), lds, ctx->stream, d, s, o, tw);
+    else
+        hipLaunchKernelGGL((k_fft_pass<T, MODE, 64, 1024>), grid, dim3(256), lds, ctx->stream, d, s, o, tw);
+    PEEC_HIP_TRY(hipGetLastError());
+    return PEEC_OK;
+}
+
+static int make_desc(peec_ctx *ctx, const PassGeom &g, PassDesc *d)
+{
+    if (g.L > 1024) {
+        peec_set_error("FFT length %d exceeds the 1024 supported per axis", g.L);
+        return PEEC_ERR_ARG;
+    }
+    FftPlan1D p;
+    if (!fft_make_plan(g.L, &p)) {
+        peec_set_error("FFT length %d is not 2^a 3^b 5^c", g.L);
+        return PEEC_ERR_ARG;
+    }
+    memset(d, 0, sizeof(*d));
+    d->L = g.L;
+    d->nin = g.nin;
+    d->nout = g.nout;
+    d->nstage = p.nstages;
+    for (int i = 0; i < p.nstages; i++) d->radix[i] = p.radix[i];
+    int64_t off = ctx->twiddle_offset(g.L);
+    if (off < 0) {
+        peec_set_error("twiddle table allocation failed");
+        return PEEC_ERR_NOMEM;
+    }
+    d->tw_off = off;
+    d->nb = g.nb;
+    d->n2 = g.n2;
+    d->in_sb = g.in_sb; d->in_s2 = g.in_s2; d->in_sj = g.in_sj;
+    d->out_sb = g.out_sb; d->out_s2 = g.out_s2; d->out_sj = g.out_sj;
+    d->nrhs = 1;
+    return PEEC_OK;
+}
+
+static int launch_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const void *src, void *dst)
+{
+    if (d.nb <= 0 || d.n2 <= 0 || d.nrhs <= 0) return PEEC_OK;
+    if (fp32) {
+        if (mode == 0) return launch_pass_t<float, 0>(ctx, d, src, dst);
+        if (mode == 1) return launch_pass_t<float, 1>(ctx, d, src, dst);
+        return launch_pass_t<float, 2>(ctx, d, src, dst);
+    }
+    if (mode == 0) return launch_pass_t<double, 0>(ctx, d, src, dst);
+    if (mode == 1) return launch_pass_t<double, 1>(ctx, d, src, dst);
+    return launch_pass_t<double, 2>(ctx, d, src, dst);
+}
+
+// compact dims of each circulant kernel
+static void kernel_dims(int kind, int Nx, int Ny, int Nz, int d[3])
+{
+    switch (kind) {
+    case 0: d[0] = Nx - 1; d[1] = Ny; d[2] = Nz; break;
+    case 1: d[0] = Nx; d[1] = Ny - 1; d[2] = Nz; break;
+    case 2: d[0] = Nx; d[1] = Ny; d[2] = Nz - 1; break;
+    case 3: d[0] = Nx; d[1] = Ny + 1; d[2] = Nz; break;
+    case 4: d[0] = Nx + 1; d[1] = Ny; d[2] = Nz; break;
+    case 5: d[0] = Nx; d[1] = Ny; d[2] = Nz + 1; break;
+    case 6: d[0] = 2 * (Nx + 1) - 1; d[1] = 2 * (Ny + 1) - 1; d[2] = Nz; break;
+    case 7: d[0] = Nx; d[1] = 2 * (Ny + 1) - 1; d[2] = 2 * (Nz + 1) - 1; break;
+    default: d[0] = 2 * (Nx + 1) - 1; d[1] = Ny; d[2] = 2 * (Nz + 1) - 1; break;
+    }
+}
+
+// Forward 3-D FFT in place over a full L grid (spectrum construction).
+static int fft3_forward_inplace(peec_ctx *ctx, double2_t *g, const int L[3])
+{
+    int64_t Lx = L[0], Ly = L[1], Lz = L[2];
+    PassDesc d;
+    int rc;
+    // x: lines contiguous, b = y, l2 = z
+    PassGeom gx = {L[0], L[0], L[0], L[1], L[2], Lx, Lx * Ly, 1, Lx, Lx * Ly, 1};
+    if ((rc = make_desc(ctx, gx, &d))) return rc;
+    if (L[0] > 1 && (rc = launch_pass(ctx, false, 0, d, g, g))) return rc;
+    PassGeom gy = {L[1], L[1], L[1], L[0], L[2], 1, Lx * Ly, Lx, 1, Lx * Ly, Lx};
+    if ((rc = make_desc(ctx, gy, &d))) return rc;
+    if (L[1] > 1 && (rc = launch_pass(ctx, false, 0, d, g, g))) return rc;
+    PassGeom gz = {L[2], L[2], L[2], L[0], L[1], 1, Lx, Lx * Ly, 1, Lx, Lx * Ly};
+    if ((rc = make_desc(ctx, gz, &d))) return rc;
+    if (L[2] > 1 && (rc = launch_pass(ctx, false, 0, d, g, g))) return rc;
+    return PEEC_OK;
+}
+
+int peec_kernels_build_impl(peec_ctx *ctx, const double origin[3], int Nx, int Ny, int Nz, double sx, double sy,
+                            double sz, double esc_Lp, double esc_P)
+{
+    ctx->release_kernels();
+    int64_t Gmax = 0;
+    for (int k = 0; k < 9; k++) {
+        Kernel &K = ctx->kern[k];
+        kernel_dims(k, Nx, Ny, Nz, K.n);
+        for (int a = 0; a < 3; a++) K.L[a] = K.n[a] > 0 ? fft_smooth_length(2 * K.n[a] - 1) : 0;
+        if (K.compact() > 0) Gmax = std::max(Gmax, K.embedded());
+    }
+    // register every FFT length before the first launch: the twiddle table
+    // is reallocated when a new length appears
+    for (int k = 0; k < 9; k++)
+        for (int a = 0; a < 3; a++)
+            if (ctx->kern[k].L[a] > 0 && ctx->twiddle_offset(ctx->kern[k].L[a]) < 0) {
+                peec_set_error("kernels_build: twiddle table allocation failed");
+                return PEEC_ERR_NOMEM;
+            }
+    DevBuf<double2_t> grid;
+    if (!grid.alloc((size_t)Gmax)) {
+        peec_set_error("kernels_build: cannot allocate %lld-point work grid", (long long)Gmax);
+        return PEEC_ERR_NOMEM;
+    }
+    for (int k = 0; k < 9; k++) {
+        if (ctx->stop_requested()) return PEEC_STOPPED;
+        Kernel &K = ctx->kern[k];
+        int64_t g = K.compact();
+        if (g <= 0) continue;
+        if (!K.row.alloc((size_t)g) || !K.spec64.alloc((size_t)K.embedded()) || !K.spec32.alloc((size_t)K.embedded())) {
+            peec_set_error("kernels_build: out of device memory");
+            return PEEC_ERR_NOMEM;
+        }
+        int64_t n;
+        int dims[3];
+        int rc = peec_grid_row_device(ctx->stream, k, origin, Nx, Ny, Nz, sx, sy, sz, k <= 2 ? esc_Lp : esc_P, K.row.p,
+                                      &n, dims);
+        if (rc) return rc;
+        int64_t G = K.embedded();
+        hipLaunchKernelGGL(k_embed, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, ctx->stream, K.row.p, K.n[0],
+                           K.n[1], K.n[2], K.L[0], K.L[1], K.L[2], grid.p);
+        PEEC_HIP_TRY(hipGetLastError());
+        if ((rc = fft3_forward_inplace(ctx, grid.p, K.L))) return rc;
+        hipLaunchKernelGGL(k_spec_extract, dim3((unsigned)((G + 255) / 256)), dim3(256), 0, ctx->stream, grid.p, G,
+                           1.0 / (double)G, K.spec64.p, K.spec32.p);
+        PEEC_HIP_TRY(hipGetLastError());
+    }
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->kernels_ready = true;
+    return PEEC_OK;
+}
+
+// One convolution application: vec_out[section] += gather(conv(scatter(vec_in)))
+// for nrhs right-hand sides (vectors at stride ld).  w1/w2 are grid work
+// buffers sized by conv_work_elems.
+int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *vec_in, double2_t *vec_out,
+               int nrhs, int64_t ld, void *w1, void *w2)
+{
+    const Kernel &K = ctx->kern[app.kernel];
+    const int nx = K.n[0], ny = K.n[1], nz = K.n[2];
+    const int64_t Lx = K.L[0], Ly = K.L[1], Lz = K.L[2];
+    if ((int64_t)nx * ny * nz <= 0 || app.nnz_in == 0 || app.nnz_out == 0) return PEEC_OK;
+    const int64_t s1 = Lx * ny * nz, s2 = Lx * Ly * nz;  // per-rhs grid sizes
+    PassDesc d;
+    int rc;
+    // 1: x forward with scatter; lines (y, z)
+    PassGeom g1 = {(int)Lx, nx, (int)Lx, ny, nz, nx, (int64_t)nx * ny, 1, Lx, Lx * ny, 1};
+    if ((rc = make_desc(ctx, g1, &d))) return rc;
+    d.map_in = app.map_in.p;
+    d.vec_in = vec_in + app.in_off;
+    d.vec_in_ld = ld;
+    d.nrhs = nrhs;
+    d.in_rhs = (int64_t)nx * ny * nz;  // only used to strip the rhs offset from map positions
+    d.out_rhs = s1;
+    if ((rc = launch_pass(ctx, fp32, 0, d, nullptr, w1))) return rc;
+    // 2: y forward; lines (x', z)
+    PassGeom g2 = {(int)Ly, ny, (int)Ly, (int)Lx, nz, 1, Lx * ny, Lx, 1, Lx * Ly, Lx};
+    if ((rc = make_desc(ctx, g2, &d))) return rc;
+    d.nrhs = nrhs;
+    d.in_rhs = s1;
+    d.out_rhs = s2;
+    if ((rc = launch_pass(ctx, fp32, 0, d, w1, w2))) return rc;
+    // 3: z forward * spectrum * z inverse, in place; lines (x', y')
+    PassGeom g3 = {(int)Lz, nz, nz, (int)Lx, (int)Ly, 1, Lx, Lx * Ly, 1, Lx, Lx * Ly};
+    if ((rc = make_desc(ctx, g3, &d))) return rc;
+    d.nrhs = nrhs;
+    d.in_rhs = s2;
+    d.out_rhs = s2;
+    d.spec = fp32 ? (const void *)K.spec32.p : (const void *)K.spec64.p;
+    d.spec_sb = 1;
+    d.spec_s2 = Lx;
+    d.spec_sj = Lx * Ly;
+    if ((rc = launch_pass(ctx, fp32, 2, d, w2, w2))) return rc;
+    // 4: y inverse, keep ny rows; lines (x', z)
+    PassGeom g4 = {(int)Ly, (int)Ly, ny, (int)Lx, nz, 1, Lx * Ly, Lx, 1, Lx * ny, Lx};
+    if ((rc = make_desc(ctx, g4, &d))) return rc;
+    d.nrhs = nrhs;
+    d.in_rhs = s2;
+    d.out_rhs = s1;
+    if ((rc = launch_pass(ctx, fp32, 1, d, w2, w1))) return rc;
+    // 5: x inverse with gather; lines (y, z)
+    PassGeom g5 = {(int)Lx, (int)Lx, nx, ny, nz, Lx, Lx * ny, 1, nx, (int64_t)nx * ny, 1};
+    if ((rc = make_desc(ctx, g5, &d))) return rc;
+    d.nrhs = nrhs;
+    d.in_rhs = s1;
+    d.out_rhs = (int64_t)nx * ny * nz;
+    d.map_out = app.map_out.p;
+    d.vec_out = vec_out + app.out_off;
+    d.vec_out_ld = ld;
+    if ((rc = launch_pass(ctx, fp32, 1, d, w1, nullptr))) return rc;
+    return PEEC_OK;
+}
+
+int64_t conv_work_elems(const peec_ctx *ctx, int kernel)
+{
+    const Kernel &K = ctx->kern[kernel];
+    return (int64_t)K.L[0] * K.L[1] * K.n[2];
+}
+
+extern "C" int peec_kernels_build(peec_ctx *ctx, const double origin[3], int Nx, int Ny, int Nz, double sx, double sy,
+                                  double sz, double esc_Lp, double esc_P)
+{
+    if (!ctx || !origin || Nx < 1 || Ny < 1 || Nz < 1 || !(sx > 0) || !(sy > 0) || !(sz > 0)) {
+        peec_set_error("peec_kernels_build: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    return peec_kernels_build_impl(ctx, origin, Nx, Ny, Nz, sx, sy, sz, esc_Lp, esc_P);
+}
+
+extern "C" int peec_kernel_row(peec_ctx *ctx, int kind, double *out, int64_t *n)
+{
+    if (!ctx || kind < 0 || kind > 8 || !n) {
+        peec_set_error("peec_kernel_row: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->kernels_ready) {
+        peec_set_error("peec_kernel_row: call peec_kernels_build first");
+        return PEEC_ERR_STATE;
+    }
+    const Kernel &K = ctx->kern[kind];
+    *n = K.compact();
+    if (!out || *n <= 0) return PEEC_OK;
+    PEEC_HIP_TRY(hipMemcpyAsync(out, K.row.p, sizeof(double) * (size_t)*n, hipMemcpyDeviceToHost, ctx->stream));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PEEC_OK;
+}

@@ -1,0 +1,720 @@
+// The preconditioned PEEC operator y = M^-1 K x on the GPU.
+//
+// K x follows ComputeMatrixVectorFTT (compute_Matrix_vector_fft.jl:1-81):
+//   Y1 = j w (Lp I) + DZ .* I + A Phi
+//   Y2 = P Q - Gamma' Phi
+//   Y3 = -A' I + Yle Phi + j w Gamma Q
+// with Lp I and P Q as 12 FFT convolutions (fft.hip).  M^-1 follows
+// precond_3_3_vector (:84-146).  The reference runs three LU solves
+// (M2 = F\(A' Z^-1 X1), M4 = F\(Gamma P^-1 X2), M5 = F\X3) and only ever uses
+// the combination M2 - j w M4 + M5, so by linearity one solve of
+//   SS M = A' Z^-1 X1 - j w Gamma P^-1 X2 + X3
+// gives the same vector; then
+//   Y1 = Z^-1 (X1 - A M),  Y2 = P^-1 (X2 + Gamma' M),  Y3 = M.
+//
+// The LU factors of SS come from the host; the two triangular solves run
+// sync-free: one wavefront per row in dependency order, each row waiting on
+// the completion flags of the rows it reads (bounded spin, error flag on
+// timeout).  Rows are dispatched in blockIdx order and only wait on rows
+// with smaller blockIdx, which the in-order dispatch has already placed on
+// the GPU, so every wait ends.
+#include "context.h"
+
+#include <algorithm>
+#include <cstring>
+
+void peec_ctx::release_operator()
+{
+    for (auto &a : apps) {
+        a.map_in.reset();
+        a.map_out.reset();
+        a.kernel = -1;
+        a.nnz_in = a.nnz_out = 0;
+    }
+    napps = 0;
+    op_ready = false;
+    freq_ready = false;
+    lu_ready = false;
+}
+
+int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *vec_in, double2_t *vec_out, int nrhs,
+               int64_t ld, void *w1, void *w2);
+int64_t conv_work_elems(const peec_ctx *ctx, int kernel);
+
+namespace {
+
+typedef double2_t cd;
+
+__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cd csub(cd a, cd b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cd cscal(cd a, double s) { return {a.x * s, a.y * s}; }
+__device__ __forceinline__ cd jw(cd a, double w) { return {-w * a.y, w * a.x}; }  // (j w) * a
+__device__ __forceinline__ cd cdiv(cd a, cd b)
+{
+    double d = b.x * b.x + b.y * b.y;
+    return {(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+}
+
+__global__ void k_zero(cd *y, int64_t n, int nrhs, int64_t ld)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int r = 0; r < nrhs; r++) y[i + r * ld] = {0.0, 0.0};
+}
+
+// Y1 = j w Y1 + DZ .* I + A Phi           (compute_Matrix_vector_fft.jl:29-31)
+__global__ void k_y1(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const int32_t *ai, const double *av,
+                     double w, int64_t m, int64_t phi0, int nrhs, int64_t ld)
+{
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    for (int r = 0; r < nrhs; r++) {
+        const cd *xr = x + r * ld;
+        cd s = {0.0, 0.0};
+        for (int k = ap[e]; k < ap[e + 1]; k++) s = cadd(s, cscal(xr[phi0 + ai[k]], av[k]));
+        cd v = jw(y[e + r * ld], w);
+        v = cadd(v, cmul(DZ[e], xr[e]));
+        y[e + r * ld] = cadd(v, s);
+    }
+}
+
+// Y2 = Y2 - Gamma' Phi                    (:66-68)
+__global__ void k_y2(cd *y, const cd *x, const int32_t *gp, const int32_t *gi, const double *gv, int64_t m,
+                     int64_t ns, int64_t phi0, int nrhs, int64_t ld)
+{
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    for (int r = 0; r < nrhs; r++) {
+        const cd *xr = x + r * ld;
+        cd a = {0.0, 0.0};
+        for (int k = gp[s]; k < gp[s + 1]; k++) a = cadd(a, cscal(xr[phi0 + gi[k]], gv[k]));
+        y[m + s + r * ld] = csub(y[m + s + r * ld], a);
+    }
+}
+
+// Y3 = -A' I + Yle Phi + j w Gamma Q      (:69-77)
+__global__ void k_y3(cd *y, const cd *x, const int32_t *atp, const int32_t *ati, const double *atv, const int32_t *yp,
+                     const int32_t *yi, const cd *yv, const int32_t *gp, const int32_t *gi, const double *gv, double w,
+                     int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int64_t phi0 = m + ns;
+    for (int r = 0; r < nrhs; r++) {
+        const cd *xr = x + r * ld;
+        cd a = {0.0, 0.0};
+        for (int k = atp[v]; k < atp[v + 1]; k++) a = cadd(a, cscal(xr[ati[k]], atv[k]));
+        cd b = {0.0, 0.0};
+        for (int k = yp[v]; k < yp[v + 1]; k++) b = cadd(b, cmul(yv[k], xr[phi0 + yi[k]]));
+        cd c = {0.0, 0.0};
+        for (int k = gp[v]; k < gp[v + 1]; k++) c = cadd(c, cscal(xr[m + gi[k]], gv[k]));
+        cd t = {-a.x, -a.y};
+        t = cadd(t, b);
+        y[phi0 + v + r * ld] = cadd(t, jw(c, w));
+    }
+}
+
+// t = A' (Z^-1 X1) - j w Gamma (P^-1 X2) + X3, scattered through the row
+// permutation and row scaling of the LU: b[i] = Rs[p[i]] t[p[i]]
+__global__ void k_pre(const cd *y, const cd *invZ, const double *invP, const int32_t *atp, const int32_t *ati,
+                      const double *atv, const int32_t *gp, const int32_t *gi, const double *gv, const int32_t *pinv,
+                      const double *Rs, cd *b, double w, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int64_t phi0 = m + ns;
+    for (int r = 0; r < nrhs; r++) {
+        const cd *yr = y + r * ld;
+        cd a = {0.0, 0.0};
+        for (int k = atp[v]; k < atp[v + 1]; k++) a = cadd(a, cscal(cmul(invZ[ati[k]], yr[ati[k]]), atv[k]));
+        cd c = {0.0, 0.0};
+        for (int k = gp[v]; k < gp[v + 1]; k++) c = cadd(c, cscal(yr[m + gi[k]], invP[gi[k]] * gv[k]));
+        cd t = csub(a, jw(c, w));
+        t = cadd(t, yr[phi0 + v]);
+        if (Rs) t = cscal(t, Rs[v]);
+        b[pinv[v] + r * n] = t;
+    }
+}
+
+// Y1 = Z^-1 (X1 - A M), Y2 = P^-1 (X2 + Gamma' M), Y3 = M (M = Q z, i.e.
+// M[q[j]] = z[j]; read through the inverse column permutation)
+__global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ, const double *invP, const int32_t *ap,
+                       const int32_t *ai, const double *av, const int32_t *gtp, const int32_t *gti, const double *gtv,
+                       int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = m + ns + n;
+    if (i >= N) return;
+    for (int r = 0; r < nrhs; r++) {
+        const cd *zr = z + r * n;
+        cd *yr = y + r * ld;
+        if (i < m) {
+            cd s = {0.0, 0.0};
+            for (int k = ap[i]; k < ap[i + 1]; k++) s = cadd(s, cscal(zr[qinv[ai[k]]], av[k]));
+            yr[i] = cmul(invZ[i], csub(yr[i], s));
+        } else if (i < m + ns) {
+            int64_t s_ = i - m;
+            cd s = {0.0, 0.0};
+            for (int k = gtp[s_]; k < gtp[s_ + 1]; k++) s = cadd(s, cscal(zr[qinv[gti[k]]], gtv[k]));
+            yr[i] = cscal(cadd(yr[i], s), invP[s_]);
+        } else {
+            yr[i] = zr[qinv[i - m - ns]];
+        }
+    }
+}
+
+// Sync-free triangular solve: wave w solves row order[w]:
+//   x[i] = (b[i] - sum_j T[i,j] x[j]) / diag[i]
+// waiting on flag[j] == gen for every j it reads.  nrhs vectors at stride n.
+constexpr int kSpinLimit = 1 << 24;
+
+__global__ void __launch_bounds__(256) k_trsv(const int32_t *order, const int32_t *ptr, const int32_t *idx,
+                                              const cd *val, const cd *diag, const cd *b, cd *x, uint32_t *flag,
+                                              uint32_t gen, int64_t n, int nrhs, int *err)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= n) return;
+    const int i = order[w];
+    const int k0 = ptr[i], k1 = ptr[i + 1];
+    for (int r = 0; r < nrhs; r++) {
+        cd acc = {0.0, 0.0};
+        for (int k = k0 + lane; k < k1; k += 64) {
+            int j = idx[k];
+            if (r == 0) {
+                int spins = 0;
+                while (__hip_atomic_load(&flag[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen) {
+                    if (++spins > kSpinLimit) {
+                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            cd xj;
+            xj.x = __hip_atomic_load(&x[j + r * n].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            xj.y = __hip_atomic_load(&x[j + r * n].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            acc = cadd(acc, cmul(val[k], xj));
+        }
+        // wave reduction
+        for (int off = 32; off > 0; off >>= 1) {
+            acc.x += __shfl_xor(acc.x, off, 64);
+            acc.y += __shfl_xor(acc.y, off, 64);
+        }
+        if (lane == 0) {
+            cd v = cdiv(csub(b[i + r * n], acc), diag[i]);
+            __hip_atomic_store(&x[i + r * n].x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&x[i + r * n].y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (lane == 0) __hip_atomic_store(&flag[i], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host helpers
+static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// Julia CSC (1-based Int64) -> device CSR (0-based int32) of the matrix or of
+// its transpose.
+static int csc_to_csr(const peec_csc_f64 *a, bool transpose, CsrR *out, const char *what)
+{
+    if (!a || !a->colptr || (a->colptr[a->ncol] - 1 > 0 && (!a->rowval || !a->nzval))) {
+        peec_set_error("operator_set: %s missing", what);
+        return PEEC_ERR_ARG;
+    }
+    int64_t nr = transpose ? a->ncol : a->nrow, nc = transpose ? a->nrow : a->ncol;
+    int64_t nnz = a->colptr[a->ncol] - 1;
+    std::vector<int32_t> ptr((size_t)nr + 1, 0), idx((size_t)std::max<int64_t>(nnz, 1));
+    std::vector<double> val((size_t)std::max<int64_t>(nnz, 1));
+    for (int64_t c = 0; c < a->ncol; c++)
+        for (int64_t k = a->colptr[c] - 1; k < a->colptr[c + 1] - 1; k++) {
+            int64_t r = a->rowval[k] - 1;
+            if (r < 0 || r >= a->nrow) {
+                peec_set_error("operator_set: %s row index out of range", what);
+                return PEEC_ERR_ARG;
+            }
+            ptr[(size_t)(transpose ? c : r) + 1]++;
+        }
+    for (int64_t i = 0; i < nr; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
+    std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
+    for (int64_t c = 0; c < a->ncol; c++)
+        for (int64_t k = a->colptr[c] - 1; k < a->colptr[c + 1] - 1; k++) {
+            int64_t r = a->rowval[k] - 1;
+            int64_t row = transpose ? c : r, col = transpose ? r : c;
+            int32_t p = pos[(size_t)row]++;
+            idx[(size_t)p] = (int32_t)col;
+            val[(size_t)p] = a->nzval[k];
+        }
+    out->nrow = nr;
+    out->ncol = nc;
+    if (!out->ptr.alloc(ptr.size()) || !out->idx.alloc(idx.size()) || !out->val.alloc(val.size())) {
+        peec_set_error("operator_set: out of device memory");
+        return PEEC_ERR_NOMEM;
+    }
+    PEEC_HIP_TRY(hipMemcpy(out->ptr.p, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(out->idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(out->val.p, val.data(), val.size() * 8, hipMemcpyHostToDevice));
+    return PEEC_OK;
+}
+
+// 0/1 expansion matrix (rows = compact kernel grid, one entry per column) ->
+// position map; columns are local unknown indices [col0, col0 + ncols).
+static int map_from_csc(const peec_csc_f64 *a, int64_t grid, int64_t col0, int64_t ncols, DevBuf<int32_t> *out,
+                        int64_t *nnz, const char *what)
+{
+    if (a->nrow != grid) {
+        peec_set_error("operator_set: %s has %lld rows, kernel grid has %lld", what, (long long)a->nrow,
+                       (long long)grid);
+        return PEEC_ERR_ARG;
+    }
+    std::vector<int32_t> map((size_t)std::max<int64_t>(grid, 1), -1);
+    int64_t cnt = 0;
+    for (int64_t c = 0; c < a->ncol; c++) {
+        int64_t k0 = a->colptr[c] - 1, k1 = a->colptr[c + 1] - 1;
+        if (k1 - k0 > 1) {
+            peec_set_error("operator_set: %s column %lld has %lld entries (expected <= 1)", what, (long long)c + 1,
+                           (long long)(k1 - k0));
+            return PEEC_ERR_ARG;
+        }
+        for (int64_t k = k0; k < k1; k++) {
+            int64_t r = a->rowval[k] - 1;
+            if (a->nzval[k] != 1.0) {
+                peec_set_error("operator_set: %s is not a 0/1 selection", what);
+                return PEEC_ERR_ARG;
+            }
+            if (c < col0 || c >= col0 + ncols) {
+                peec_set_error("operator_set: %s selects column %lld outside its section", what, (long long)c + 1);
+                return PEEC_ERR_ARG;
+            }
+            if (map[(size_t)r] >= 0) {
+                peec_set_error("operator_set: %s maps two unknowns to grid position %lld", what, (long long)r + 1);
+                return PEEC_ERR_ARG;
+            }
+            map[(size_t)r] = (int32_t)(c - col0);
+            cnt++;
+        }
+    }
+    if (!out->alloc(map.size())) {
+        peec_set_error("operator_set: out of device memory");
+        return PEEC_ERR_NOMEM;
+    }
+    PEEC_HIP_TRY(hipMemcpy(out->p, map.data(), map.size() * 4, hipMemcpyHostToDevice));
+    *nnz = cnt;
+    return PEEC_OK;
+}
+
+extern "C" int peec_operator_set(peec_ctx *ctx, int64_t mx, int64_t my, int64_t mz, const peec_csc_f64 *A,
+                                 const peec_csc_f64 *Gamma, const peec_csc_f64 lp_map[3], const peec_csc_f64 p_map[9],
+                                 const double *diag_P)
+{
+    if (!ctx || !A || !Gamma || !lp_map || !p_map || !diag_P) {
+        peec_set_error("peec_operator_set: null argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->kernels_ready) {
+        peec_set_error("peec_operator_set: call peec_kernels_build first");
+        return PEEC_ERR_STATE;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->release_operator();
+    const int64_t m = A->nrow, n = A->ncol, ns = Gamma->ncol;
+    if (mx + my + mz != m || Gamma->nrow != n) {
+        peec_set_error("peec_operator_set: inconsistent sizes (m=%lld, mx+my+mz=%lld, Gamma %lldx%lld, n=%lld)",
+                       (long long)m, (long long)(mx + my + mz), (long long)Gamma->nrow, (long long)ns, (long long)n);
+        return PEEC_ERR_ARG;
+    }
+    if (m + ns + n > INT32_MAX) {
+        peec_set_error("peec_operator_set: %lld unknowns exceed the int32 index range", (long long)(m + ns + n));
+        return PEEC_ERR_ARG;
+    }
+    ctx->m = m;
+    ctx->n = n;
+    ctx->ns = ns;
+    ctx->mx = mx;
+    ctx->my = my;
+    ctx->mz = mz;
+    int rc;
+    if ((rc = csc_to_csr(A, false, &ctx->A, "A")) || (rc = csc_to_csr(A, true, &ctx->At, "A'")) ||
+        (rc = csc_to_csr(Gamma, false, &ctx->G, "Gamma")) || (rc = csc_to_csr(Gamma, true, &ctx->Gt, "Gamma'")))
+        return rc;
+    std::vector<double> ip((size_t)std::max<int64_t>(ns, 1), 0.0);
+    for (int64_t s = 0; s < ns; s++) ip[(size_t)s] = 1.0 / diag_P[s];
+    if (!ctx->invP.alloc(ip.size())) return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemcpy(ctx->invP.p, ip.data(), ip.size() * 8, hipMemcpyHostToDevice));
+
+    // Lp applications (compute_Matrix_vector_fft.jl:14-28)
+    const int64_t moff[3] = {0, mx, mx + my}, msz[3] = {mx, my, mz};
+    int na = 0;
+    int64_t wmax = 1;
+    for (int d = 0; d < 3; d++) {
+        if (msz[d] == 0) continue;
+        ConvApp &a = ctx->apps[na];
+        a.kernel = d;
+        a.in_off = moff[d];
+        a.out_off = moff[d];
+        const char *nm[3] = {"mat_map_Lp[1,1]", "mat_map_Lp[2,1]", "mat_map_Lp[3,1]"};
+        if (lp_map[d].ncol != msz[d]) {
+            peec_set_error("peec_operator_set: %s has %lld columns, expected %lld", nm[d], (long long)lp_map[d].ncol,
+                           (long long)msz[d]);
+            return PEEC_ERR_ARG;
+        }
+        if ((rc = map_from_csc(&lp_map[d], ctx->kern[d].compact(), 0, msz[d], &a.map_in, &a.nnz_in, nm[d]))) return rc;
+        if ((rc = map_from_csc(&lp_map[d], ctx->kern[d].compact(), 0, msz[d], &a.map_out, &a.nnz_out, nm[d]))) return rc;
+        wmax = std::max(wmax, conv_work_elems(ctx, d));
+        na++;
+    }
+    // P applications (:35-65): pairs (c1, c2), c2 >= c1; off-diagonal pairs twice
+    const int pk[3][3] = {{3, 6, 7}, {6, 4, 8}, {7, 8, 5}};
+    for (int c1 = 0; c1 < 3; c1++)
+        for (int c2 = c1; c2 < 3; c2++) {
+            int k = pk[c1][c2];
+            for (int pass = 0; pass < (c1 == c2 ? 1 : 2); pass++) {
+                int i_in = pass == 0 ? c1 : c2, j_in = pass == 0 ? c2 : c1;
+                const peec_csc_f64 *min_ = &p_map[3 * j_in + i_in];  // exp_P[i_in, j_in]
+                const peec_csc_f64 *mout = &p_map[3 * i_in + j_in];  // exp_P[j_in, i_in]
+                ConvApp &a = ctx->apps[na];
+                a.kernel = k;
+                a.in_off = m;
+                a.out_off = m;
+                if ((rc = map_from_csc(min_, ctx->kern[k].compact(), 0, ns, &a.map_in, &a.nnz_in, "exp_P (scatter)")))
+                    return rc;
+                if ((rc = map_from_csc(mout, ctx->kern[k].compact(), 0, ns, &a.map_out, &a.nnz_out, "exp_P (gather)")))
+                    return rc;
+                wmax = std::max(wmax, conv_work_elems(ctx, k));
+                na++;
+            }
+        }
+    ctx->napps = na;
+    // work buffers: two grids (one rhs) plus vectors
+    if (!ctx->w1.alloc((size_t)wmax) || !ctx->w2.alloc((size_t)wmax)) {
+        peec_set_error("peec_operator_set: out of device memory for FFT work grids");
+        return PEEC_ERR_NOMEM;
+    }
+    const int64_t N = m + ns + n;
+    if (!ctx->vx.alloc((size_t)N) || !ctx->vy.alloc((size_t)N) || !ctx->vt1.alloc((size_t)std::max<int64_t>(n, 1)) ||
+        !ctx->vt2.alloc((size_t)std::max<int64_t>(n, 1))) {
+        peec_set_error("peec_operator_set: out of device memory");
+        return PEEC_ERR_NOMEM;
+    }
+    ctx->op_ready = true;
+    return PEEC_OK;
+}
+
+extern "C" int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, const peec_csc_c128 *Yle)
+{
+    if (!ctx || !DZ || !Yle) {
+        peec_set_error("peec_frequency_set: null argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->op_ready) {
+        peec_set_error("peec_frequency_set: call peec_operator_set first");
+        return PEEC_ERR_STATE;
+    }
+    if (Yle->nrow != ctx->n || Yle->ncol != ctx->n) {
+        peec_set_error("peec_frequency_set: Yle must be n x n");
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int64_t m = ctx->m, n = ctx->n;
+    ctx->w = w;
+    std::vector<double2_t> dz((size_t)m), iz((size_t)m);
+    for (int64_t e = 0; e < m; e++) {
+        dz[(size_t)e] = {DZ[e].re, DZ[e].im};
+        double d = DZ[e].re * DZ[e].re + DZ[e].im * DZ[e].im;
+        if (d == 0.0) {
+            peec_set_error("peec_frequency_set: DZ[%lld] is zero", (long long)e + 1);
+            return PEEC_ERR_NUMERIC;
+        }
+        // 1 ./ DZ (FFT_solver_QS_S_type.jl:115)
+        iz[(size_t)e] = {DZ[e].re / d, -DZ[e].im / d};
+    }
+    if (!ctx->DZ.alloc((size_t)std::max<int64_t>(m, 1)) || !ctx->invZ.alloc((size_t)std::max<int64_t>(m, 1)))
+        return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemcpy(ctx->DZ.p, dz.data(), (size_t)m * 16, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(ctx->invZ.p, iz.data(), (size_t)m * 16, hipMemcpyHostToDevice));
+    // Yle CSC -> CSR
+    int64_t nnz = Yle->colptr[n] - 1;
+    std::vector<int32_t> ptr((size_t)n + 1, 0), idx((size_t)std::max<int64_t>(nnz, 1));
+    std::vector<double2_t> val((size_t)std::max<int64_t>(nnz, 1));
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t k = Yle->colptr[c] - 1; k < Yle->colptr[c + 1] - 1; k++) ptr[(size_t)Yle->rowval[k]]++;
+    for (int64_t i = 0; i < n; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
+    std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t k = Yle->colptr[c] - 1; k < Yle->colptr[c + 1] - 1; k++) {
+            int64_t r = Yle->rowval[k] - 1;
+            int32_t p = pos[(size_t)r]++;
+            idx[(size_t)p] = (int32_t)c;
+            val[(size_t)p] = {Yle->nzval[k].re, Yle->nzval[k].im};
+        }
+    ctx->Yle.nrow = ctx->Yle.ncol = n;
+    if (!ctx->Yle.ptr.alloc(ptr.size()) || !ctx->Yle.idx.alloc(idx.size()) || !ctx->Yle.val.alloc(val.size()))
+        return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemcpy(ctx->Yle.ptr.p, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(ctx->Yle.idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(ctx->Yle.val.p, val.data(), val.size() * 16, hipMemcpyHostToDevice));
+    ctx->freq_ready = true;
+    ctx->lu_ready = false;
+    return PEEC_OK;
+}
+
+// Split a CSC triangle into row-ordered off-diagonal CSR plus the diagonal.
+static int tri_from_csc(const peec_csc_c128 *T, bool lower, TriFactor *f, const char *what)
+{
+    const int64_t n = T->ncol;
+    if (T->nrow != n) {
+        peec_set_error("lu_set_factors: %s is not square", what);
+        return PEEC_ERR_ARG;
+    }
+    std::vector<double2_t> diag((size_t)n, {0.0, 0.0});
+    std::vector<char> has((size_t)n, 0);
+    std::vector<int32_t> ptr((size_t)n + 1, 0);
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t k = T->colptr[c] - 1; k < T->colptr[c + 1] - 1; k++) {
+            int64_t r = T->rowval[k] - 1;
+            if (r == c) continue;
+            if ((lower && r < c) || (!lower && r > c)) {
+                if (T->nzval[k].re != 0.0 || T->nzval[k].im != 0.0) {
+                    peec_set_error("lu_set_factors: %s is not %s triangular", what, lower ? "lower" : "upper");
+                    return PEEC_ERR_ARG;
+                }
+                continue;
+            }
+            ptr[(size_t)r + 1]++;
+        }
+    for (int64_t i = 0; i < n; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
+    int64_t nnz = ptr[(size_t)n];
+    std::vector<int32_t> idx((size_t)std::max<int64_t>(nnz, 1)), pos(ptr.begin(), ptr.end() - 1);
+    std::vector<double2_t> val((size_t)std::max<int64_t>(nnz, 1));
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t k = T->colptr[c] - 1; k < T->colptr[c + 1] - 1; k++) {
+            int64_t r = T->rowval[k] - 1;
+            if (r == c) {
+                diag[(size_t)c] = {T->nzval[k].re, T->nzval[k].im};
+                has[(size_t)c] = 1;
+                continue;
+            }
+            if ((lower && r < c) || (!lower && r > c)) continue;
+            int32_t p = pos[(size_t)r]++;
+            idx[(size_t)p] = (int32_t)c;
+            val[(size_t)p] = {T->nzval[k].re, T->nzval[k].im};
+        }
+    for (int64_t i = 0; i < n; i++)
+        if (!has[(size_t)i] || (diag[(size_t)i].x == 0.0 && diag[(size_t)i].y == 0.0)) {
+            peec_set_error("lu_set_factors: %s has a zero diagonal at %lld", what, (long long)i + 1);
+            return PEEC_ERR_NUMERIC;
+        }
+    std::vector<int32_t> order((size_t)n);
+    for (int64_t w = 0; w < n; w++) order[(size_t)w] = (int32_t)(lower ? w : n - 1 - w);
+    f->n = n;
+    if (!f->order.alloc((size_t)n) || !f->ptr.alloc(ptr.size()) || !f->idx.alloc(idx.size()) ||
+        !f->val.alloc(val.size()) || !f->diag.alloc((size_t)n))
+        return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemcpy(f->order.p, order.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->ptr.p, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->val.p, val.data(), val.size() * 16, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->diag.p, diag.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+    return PEEC_OK;
+}
+
+extern "C" int peec_lu_set_factors(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L, const peec_csc_c128 *U,
+                                   const int64_t *p, const int64_t *q, const double *Rs)
+{
+    if (!ctx || !L || !U || !p || !q) {
+        peec_set_error("peec_lu_set_factors: null argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->freq_ready) {
+        peec_set_error("peec_lu_set_factors: call peec_frequency_set first");
+        return PEEC_ERR_STATE;
+    }
+    if (n != ctx->n || L->ncol != n || U->ncol != n) {
+        peec_set_error("peec_lu_set_factors: factors must be n x n with n = %lld", (long long)ctx->n);
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int rc;
+    if ((rc = tri_from_csc(L, true, &ctx->Lf, "L")) || (rc = tri_from_csc(U, false, &ctx->Uf, "U"))) return rc;
+    // pinv[p[i]] = i (scatter of the permuted rhs), qinv[q[j]] = j
+    std::vector<int32_t> pinv((size_t)n, -1), qinv((size_t)n, -1);
+    for (int64_t i = 0; i < n; i++) {
+        if (p[i] < 0 || p[i] >= n || pinv[(size_t)p[i]] >= 0 || q[i] < 0 || q[i] >= n || qinv[(size_t)q[i]] >= 0) {
+            peec_set_error("peec_lu_set_factors: p and q must be 0-based permutations");
+            return PEEC_ERR_ARG;
+        }
+        pinv[(size_t)p[i]] = (int32_t)i;
+        qinv[(size_t)q[i]] = (int32_t)i;
+    }
+    if (!ctx->perm_r.alloc((size_t)n) || !ctx->perm_c.alloc((size_t)n)) return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemcpy(ctx->perm_r.p, pinv.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(ctx->perm_c.p, qinv.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    if (Rs) {
+        if (!ctx->Rs.alloc((size_t)n)) return PEEC_ERR_NOMEM;
+        PEEC_HIP_TRY(hipMemcpy(ctx->Rs.p, Rs, (size_t)n * 8, hipMemcpyHostToDevice));
+        ctx->has_Rs = true;
+    } else {
+        ctx->has_Rs = false;
+    }
+    if (!ctx->flags.alloc((size_t)n) || !ctx->err.alloc(1)) return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemsetAsync(ctx->flags.p, 0, (size_t)n * 4, ctx->stream));
+    PEEC_HIP_TRY(hipMemsetAsync(ctx->err.p, 0, 4, ctx->stream));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->generation = 0;
+    ctx->lu_ready = true;
+    return PEEC_OK;
+}
+
+// ------------------------------------------------------------------ apply
+static int trsv(peec_ctx *ctx, const TriFactor &f, const double2_t *b, double2_t *x, int nrhs)
+{
+    if (ctx->generation >= 0xFFFFFFF0u) {
+        PEEC_HIP_TRY(hipMemsetAsync(ctx->flags.p, 0, (size_t)f.n * 4, ctx->stream));
+        ctx->generation = 0;
+    }
+    uint32_t gen = ++ctx->generation;
+    hipLaunchKernelGGL(k_trsv, dim3(nblk(f.n, 4)), dim3(256), 0, ctx->stream, f.order.p, f.ptr.p, f.idx.p, f.val.p,
+                       f.diag.p, b, x, ctx->flags.p, gen, f.n, nrhs, ctx->err.p);
+    PEEC_HIP_TRY(hipGetLastError());
+    return PEEC_OK;
+}
+
+// In place on y (length N per rhs, stride ld): y <- M^-1 y
+int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld)
+{
+    const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n;
+    if (nrhs * n > (int64_t)ctx->vt1.n) {
+        if (!ctx->vt1.alloc((size_t)(nrhs * n)) || !ctx->vt2.alloc((size_t)(nrhs * n))) return PEEC_ERR_NOMEM;
+    }
+    double2_t *b = ctx->vt1.p, *z = ctx->vt2.p;
+    hipLaunchKernelGGL(k_pre, dim3(nblk(n)), dim3(256), 0, ctx->stream, y, ctx->invZ.p, ctx->invP.p, ctx->At.ptr.p,
+                       ctx->At.idx.p, ctx->At.val.p, ctx->G.ptr.p, ctx->G.idx.p, ctx->G.val.p, ctx->perm_r.p,
+                       ctx->has_Rs ? ctx->Rs.p : nullptr, b, ctx->w, m, ns, n, nrhs, ld);
+    PEEC_HIP_TRY(hipGetLastError());
+    int rc;
+    if ((rc = trsv(ctx, ctx->Lf, b, z, nrhs))) return rc;   // L c = b   (c in z)
+    if ((rc = trsv(ctx, ctx->Uf, z, b, nrhs))) return rc;   // U z' = c  (z' in b)
+    hipLaunchKernelGGL(k_post, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, b, ctx->perm_c.p, ctx->invZ.p,
+                       ctx->invP.p, ctx->A.ptr.p, ctx->A.idx.p, ctx->A.val.p, ctx->Gt.ptr.p, ctx->Gt.idx.p,
+                       ctx->Gt.val.p, m, ns, n, nrhs, ld);
+    PEEC_HIP_TRY(hipGetLastError());
+    return PEEC_OK;
+}
+
+// y = K x (before the preconditioner), device vectors
+int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32)
+{
+    const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n, N = m + ns + n;
+    hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns)), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
+    PEEC_HIP_TRY(hipGetLastError());
+    size_t need = 0;
+    for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx, ctx->apps[a].kernel));
+    need *= (size_t)nrhs;
+    if (need > ctx->w1.n) {
+        if (!ctx->w1.alloc(need) || !ctx->w2.alloc(need)) {
+            peec_set_error("out of device memory for %d-rhs FFT work grids", nrhs);
+            return PEEC_ERR_NOMEM;
+        }
+    }
+    for (int a = 0; a < ctx->napps; a++) {
+        int rc = conv_apply(ctx, ctx->apps[a], fp32, x, y, nrhs, ld, ctx->w1.p, ctx->w2.p);
+        if (rc) return rc;
+    }
+    const int64_t phi0 = m + ns;
+    hipLaunchKernelGGL(k_y1, dim3(nblk(m)), dim3(256), 0, ctx->stream, y, x, ctx->DZ.p, ctx->A.ptr.p, ctx->A.idx.p,
+                       ctx->A.val.p, ctx->w, m, phi0, nrhs, ld);
+    hipLaunchKernelGGL(k_y2, dim3(nblk(ns)), dim3(256), 0, ctx->stream, y, x, ctx->Gt.ptr.p, ctx->Gt.idx.p,
+                       ctx->Gt.val.p, m, ns, phi0, nrhs, ld);
+    hipLaunchKernelGGL(k_y3, dim3(nblk(n)), dim3(256), 0, ctx->stream, y, x, ctx->At.ptr.p, ctx->At.idx.p,
+                       ctx->At.val.p, ctx->Yle.ptr.p, ctx->Yle.idx.p, ctx->Yle.val.p, ctx->G.ptr.p, ctx->G.idx.p,
+                       ctx->G.val.p, ctx->w, m, ns, n, nrhs, ld);
+    PEEC_HIP_TRY(hipGetLastError());
+    (void)N;
+    return PEEC_OK;
+}
+
+int check_trsv_error(peec_ctx *ctx)
+{
+    int e = 0;
+    PEEC_HIP_TRY(hipMemcpyAsync(&e, ctx->err.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (e) {
+        peec_set_error("triangular solve: a dependency wait exceeded its spin limit");
+        return PEEC_ERR_NUMERIC;
+    }
+    return PEEC_OK;
+}
+
+// device-vector matvec used by GMRES: y = M^-1 K x
+int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32)
+{
+    int rc = apply_K(ctx, x, y, nrhs, ld, fp32);
+    if (rc) return rc;
+    return apply_precond(ctx, y, nrhs, ld);
+}
+
+static int ready(peec_ctx *ctx, const char *what)
+{
+    if (!ctx || !ctx->op_ready || !ctx->freq_ready || !ctx->lu_ready) {
+        peec_set_error("%s: operator, frequency and LU factors must be set first", what);
+        return PEEC_ERR_STATE;
+    }
+    return PEEC_OK;
+}
+
+extern "C" int peec_matvec(peec_ctx *ctx, const peec_c128 *x, peec_c128 *y)
+{
+    int rc = ready(ctx, "peec_matvec");
+    if (rc) return rc;
+    if (!x || !y) {
+        peec_set_error("peec_matvec: null vector");
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t N = ctx->m + ctx->ns + ctx->n;
+    PEEC_HIP_TRY(hipMemcpyAsync(ctx->vx.p, x, (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = matvec_device(ctx, ctx->vx.p, ctx->vy.p, 1, N, false))) return rc;
+    PEEC_HIP_TRY(hipMemcpyAsync(y, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
+    return check_trsv_error(ctx);
+}
+
+// K x only (no preconditioner); exported for parity tests of the operator
+extern "C" int peec_apply_K(peec_ctx *ctx, const peec_c128 *x, peec_c128 *y)
+{
+    if (!ctx || !ctx->op_ready || !ctx->freq_ready) {
+        peec_set_error("peec_apply_K: operator and frequency must be set first");
+        return PEEC_ERR_STATE;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t N = ctx->m + ctx->ns + ctx->n;
+    PEEC_HIP_TRY(hipMemcpyAsync(ctx->vx.p, x, (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
+    int rc = apply_K(ctx, ctx->vx.p, ctx->vy.p, 1, N, false);
+    if (rc) return rc;
+    PEEC_HIP_TRY(hipMemcpyAsync(y, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PEEC_OK;
+}
+
+extern "C" int peec_precond_rhs(peec_ctx *ctx, const double *is, peec_c128 *tn)
+{
+    int rc = ready(ctx, "peec_precond_rhs");
+    if (rc) return rc;
+    if (!is || !tn) {
+        peec_set_error("peec_precond_rhs: null vector");
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n, N = m + ns + n;
+    // precond_3_3_Kt_fft! is M^-1 applied to [0; 0; is]
+    std::vector<double2_t> h((size_t)N, {0.0, 0.0});
+    for (int64_t v = 0; v < n; v++) h[(size_t)(m + ns + v)] = {is[v], 0.0};
+    PEEC_HIP_TRY(hipMemcpyAsync(ctx->vy.p, h.data(), (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = apply_precond(ctx, ctx->vy.p, 1, N))) return rc;
+    PEEC_HIP_TRY(hipMemcpyAsync(tn, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
+    return check_trsv_error(ctx);
+}

@@ -1,0 +1,253 @@
+"""Host-side mirror of the reference's FFT-solver functions over libpeec_hip.
+
+Each function keeps the name, argument order and meaning of the Julia
+function it replaces (Draco112358/Solver, src/lib/solverFFT/) and forwards
+to the C ABI declared in include/peec_hip.h.  All arithmetic of the hot path
+runs on the GPU; the host only moves arguments and results.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Context:
+    """One device context (one HIP stream), the analogue of one simulation
+    task of the reference (solver_start.jl:187)."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        L.check(L.lib().peec_ctx_create(int(device), ctypes.byref(self._h)), "peec_ctx_create")
+        self._stop = ctypes.c_int32(0)
+        L.check(L.lib().peec_set_stop_flag(self._h, ctypes.byref(self._stop)), "peec_set_stop_flag")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def request_stop(self, on=True):
+        """stopComputation[id][] = true (solver_start.jl:345-367)."""
+        self._stop.value = 1 if on else 0
+
+    def close(self):
+        if self._h:
+            L.lib().peec_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _centers_colmajor(c):
+    c = np.asarray(c, dtype=np.float64).reshape(-1, 3)
+    return np.ascontiguousarray(c.T).reshape(-1), c.shape[0]
+
+
+def compute_Lp_Voxels(centri_m, centri_n, sx, sy, sz, sx2, sy2, sz2, dc, is_sym=False, id=None, ctx=None):
+    """compute_Lp_Voxels.jl:1-63 with M = 1 (the hot-path call).
+
+    Returns a 1 x N matrix, or None when a stop was requested (the
+    reference's `return nothing`, :58)."""
+    if is_sym:
+        raise ValueError("is_sym=true is not on the FFT hot path (compute_FFT_mutual_coupling_mats.jl:46)")
+    ctx = ctx or default_context()
+    src = np.ascontiguousarray(np.asarray(centri_m, dtype=np.float64).reshape(-1)[:3])
+    cf, N = _centers_colmajor(centri_n)
+    out = np.empty(N, dtype=np.float64)
+    rc = L.check(L.lib().peec_lp_row(ctx.handle, L.as_dp(src), L.as_dp(cf), N, sx, sy, sz, sx2, sy2, sz2,
+                                     int(dc), L.as_dp(out)), "peec_lp_row")
+    if rc == L.PEEC_STOPPED:
+        return None
+    return out.reshape(1, N)
+
+
+def compute_row_P_sup(centers, centers2, sx, sy, sz, caser, caser2, id=None, ctx=None):
+    """compute_row_P_sup.jl:1-104.  Returns an N x 1 matrix, or NaN when a
+    stop was requested (the reference's `return NaN`, :59)."""
+    ctx = ctx or default_context()
+    src = np.ascontiguousarray(np.asarray(centers, dtype=np.float64).reshape(-1)[:3])
+    cf, N = _centers_colmajor(centers2)
+    out = np.empty(N, dtype=np.float64)
+    rc = L.check(L.lib().peec_p_row(ctx.handle, L.as_dp(src), L.as_dp(cf), N, sx, sy, sz, int(caser), int(caser2),
+                                    L.as_dp(out)), "peec_p_row")
+    if rc == L.PEEC_STOPPED:
+        return float("nan")
+    return out.reshape(N, 1)
+
+
+def Lp_self(l, W, T):
+    """compute_diagonals.jl:136-155."""
+    out = ctypes.c_double()
+    L.check(L.lib().peec_lp_self(l, W, T, ctypes.byref(out)), "peec_lp_self")
+    return out.value
+
+
+# ---------------------------------------------------------------- operator
+def _csc_f64(A):
+    """Julia-layout CSC (1-based Int64) view of a scipy matrix; the arrays
+    are returned too so they stay alive during the call."""
+    A = A.tocsc()
+    A.sort_indices()
+    colptr = np.ascontiguousarray(A.indptr.astype(np.int64) + 1)
+    rowval = np.ascontiguousarray(A.indices.astype(np.int64) + 1)
+    nz = np.ascontiguousarray(A.data.astype(np.float64))
+    s = L.CscF64(colptr.ctypes.data_as(L.i64p), rowval.ctypes.data_as(L.i64p), nz.ctypes.data_as(L.dp),
+                 A.shape[0], A.shape[1])
+    return s, (colptr, rowval, nz)
+
+
+def _csc_c128(A):
+    A = A.tocsc()
+    A.sort_indices()
+    colptr = np.ascontiguousarray(A.indptr.astype(np.int64) + 1)
+    rowval = np.ascontiguousarray(A.indices.astype(np.int64) + 1)
+    nz = np.ascontiguousarray(A.data.astype(np.complex128))
+    s = L.CscC128(colptr.ctypes.data_as(L.i64p), rowval.ctypes.data_as(L.i64p), nz.ctypes.data_as(L.c128p),
+                  A.shape[0], A.shape[1])
+    return s, (colptr, rowval, nz)
+
+
+def compute_FFT_mutual_coupling_mats(origin, Nx, Ny, Nz, sx, sy, sz, escalings, ctx=None):
+    """compute_FFT_mutual_coupling_mats.jl:1-33 (QS): the 9 kernel spectra,
+    kept on the device inside ctx.  Returns False when a stop was requested
+    (the reference returns nothing)."""
+    ctx = ctx or default_context()
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    rc = L.check(L.lib().peec_kernels_build(ctx.handle, L.as_dp(o), int(Nx), int(Ny), int(Nz), sx, sy, sz,
+                                            float(escalings["Lp"]), float(escalings["P"])), "peec_kernels_build")
+    return rc != L.PEEC_STOPPED
+
+
+def kernel_row(kind, ctx=None):
+    ctx = ctx or default_context()
+    n = ctypes.c_int64()
+    L.check(L.lib().peec_kernel_row(ctx.handle, int(kind), None, ctypes.byref(n)), "peec_kernel_row")
+    out = np.empty(n.value)
+    if n.value:
+        L.check(L.lib().peec_kernel_row(ctx.handle, int(kind), L.as_dp(out), ctypes.byref(n)), "peec_kernel_row")
+    return out
+
+
+def operator_set(mesh, ctx=None):
+    """Upload A, Gamma, the expansion maps and 1 ./ diag(P) of one geometry."""
+    ctx = ctx or default_context()
+    keep = []
+    A, k = _csc_f64(mesh.A)
+    keep.append(k)
+    G, k = _csc_f64(mesh.Gamma)
+    keep.append(k)
+    lp = (L.CscF64 * 3)()
+    for d in range(3):
+        s, k = _csc_f64(mesh.mat_map_Lp[d])
+        lp[d] = s
+        keep.append(k)
+    pm = (L.CscF64 * 9)()
+    for i in range(3):
+        for j in range(3):
+            s, k = _csc_f64(mesh.exp_P[i][j])
+            pm[3 * j + i] = s
+            keep.append(k)
+    dP = np.ascontiguousarray(mesh.diag_P, dtype=np.float64)
+    L.check(L.lib().peec_operator_set(ctx.handle, int(mesh.mx), int(mesh.my), int(mesh.mz), ctypes.byref(A),
+                                      ctypes.byref(G), lp, pm, L.as_dp(dP)), "peec_operator_set")
+
+
+def frequency_set(w, DZ, Yle, ctx=None):
+    ctx = ctx or default_context()
+    dz = np.ascontiguousarray(DZ, dtype=np.complex128)
+    Y, keep = _csc_c128(Yle)
+    L.check(L.lib().peec_frequency_set(ctx.handle, float(w), L.as_c128p(dz), ctypes.byref(Y)), "peec_frequency_set")
+
+
+def lu_set_factors(Lf, Uf, p, q, Rs=None, ctx=None):
+    """F = lu(SS) in Julia's convention L*U = (Rs .* SS)[p, q] (0-based p, q)."""
+    ctx = ctx or default_context()
+    Ls, k1 = _csc_c128(Lf)
+    Us, k2 = _csc_c128(Uf)
+    pp = np.ascontiguousarray(p, dtype=np.int64)
+    qq = np.ascontiguousarray(q, dtype=np.int64)
+    rs = None if Rs is None else np.ascontiguousarray(Rs, dtype=np.float64)
+    L.check(L.lib().peec_lu_set_factors(ctx.handle, int(Lf.shape[0]), ctypes.byref(Ls), ctypes.byref(Us),
+                                        pp.ctypes.data_as(L.i64p), qq.ctypes.data_as(L.i64p),
+                                        None if rs is None else L.as_dp(rs)), "peec_lu_set_factors")
+
+
+def ComputeMatrixVectorFTT(x, ctx=None):
+    """compute_Matrix_vector_fft.jl:1-81 (M^-1 K x) for the operator,
+    frequency and LU factors loaded into ctx."""
+    ctx = ctx or default_context()
+    xx = np.ascontiguousarray(x, dtype=np.complex128)
+    y = np.empty_like(xx)
+    L.check(L.lib().peec_matvec(ctx.handle, L.as_c128p(xx), L.as_c128p(y)), "peec_matvec")
+    return y
+
+
+def apply_K(x, ctx=None):
+    ctx = ctx or default_context()
+    xx = np.ascontiguousarray(x, dtype=np.complex128)
+    y = np.empty_like(xx)
+    L.check(L.lib().peec_apply_K(ctx.handle, L.as_c128p(xx), L.as_c128p(y)), "peec_apply_K")
+    return y
+
+
+def precond_3_3_Kt_fft(is_, ctx=None):
+    """FFT_solver_QS_S_type.jl:237-262: M^-1 [0; 0; is]."""
+    ctx = ctx or default_context()
+    v = np.ascontiguousarray(is_, dtype=np.float64)
+    N = ctx_size(ctx)
+    tn = np.empty(N, dtype=np.complex128)
+    L.check(L.lib().peec_precond_rhs(ctx.handle, L.as_dp(v), L.as_c128p(tn)), "peec_precond_rhs")
+    return tn
+
+
+def gmres_custom(b, restarted, tol, maxit, x, ctx=None):
+    """gmres_custom.jl:1-331.  Returns (x, flag, relres, iter, resvec);
+    flag 99 means a stop was requested."""
+    if restarted:
+        raise ValueError("the restarted branch of gmres_custom is dead code in the reference (gmres_custom.jl:4-10)")
+    ctx = ctx or default_context()
+    bb = np.ascontiguousarray(b, dtype=np.complex128).reshape(-1)
+    n = bb.size
+    xx = np.ascontiguousarray(x if x is not None and len(x) else np.zeros(n), dtype=np.complex128).reshape(-1).copy()
+    flag = ctypes.c_int()
+    relres = ctypes.c_double()
+    it = (ctypes.c_int64 * 2)()
+    resvec = np.zeros(min(maxit, n) + 1)
+    nres = ctypes.c_int64()
+    rc = L.check(L.lib().peec_gmres(ctx.handle, L.as_c128p(bb), float(tol), int(maxit), L.as_c128p(xx),
+                                    ctypes.byref(flag), ctypes.byref(relres), it, L.as_dp(resvec), ctypes.byref(nres)),
+                 "peec_gmres")
+    if rc == L.PEEC_STOPPED:
+        return xx, 99, 0.0, [it[0], it[1]], 0
+    return xx, flag.value, relres.value, [it[0], it[1]], resvec[:nres.value]
+
+
+_sizes = {}
+
+
+def ctx_size(ctx):
+    return _sizes[id(ctx)]
+
+
+def set_ctx_size(ctx, N):
+    _sizes[id(ctx)] = N

@@ -1,0 +1,196 @@
+"""FFT_solver_QS_S_type on the GPU path (FFT_solver_QS_S_type.jl:1-285).
+
+The host runs the per-frequency bookkeeping of the reference -- Yle
+(build_Yle_FFT.jl), Z_self (compute_Z_self.jl), the skin-effect switch and
+DZ (FFT_solver_QS_S_type.jl:105-114) -- and the numeric LU of SS with
+SuperLU (the reference uses UMFPACK on the host too, :118-119).  The kernel
+spectra, the FFT matvec, the preconditioner's triangular solves and GMRES
+run on the GPU through libpeec_hip.
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from . import peec
+
+
+def lumped_admittance(typ, R, L, C, w):
+    """get_lumped_elements_admittance_fft (build_Yle_FFT.jl:128-209)."""
+    jw = 1j * w
+    with np.errstate(divide="ignore", invalid="ignore"):
+        def ok(y):
+            return not (np.isnan(y) or np.isinf(y) or abs(y) < 1e-16)
+
+        if typ == 1:
+            z = R + jw * L + 1 / (jw * C)
+            if not (not (np.isnan(z) or np.isinf(z)) and abs(z) >= 1e-16):
+                z = R + jw * L
+            y = 1 / z
+        else:
+            chains = {
+                2: [lambda: 1 / (R + jw * L) + jw * C, lambda: 1 / R + jw * C, lambda: 1 / (R + jw * L),
+                    lambda: 1 / R, lambda: 1 / (jw * L) + jw * C, lambda: jw * C],
+                3: [lambda: 1 / (R + 1 / (jw * C)) + 1 / (jw * L), lambda: 1 / (R + 1 / (jw * C)),
+                    lambda: 1 / R + 1 / (jw * L), lambda: 1 / R, lambda: 1 / (1 / (jw * C)) + 1 / (jw * L),
+                    lambda: 1 / (jw * L), lambda: 1 / (1 / (jw * C))],
+                4: [lambda: 1 / R + jw * C + 1 / (jw * L), lambda: 1 / R + jw * C, lambda: jw * C + 1 / (jw * L),
+                    lambda: jw * C],
+                5: [lambda: 1 / R + 1 / (jw * L + 1 / (jw * C)), lambda: 1 / R + 1 / (1 / (jw * C)),
+                    lambda: 1 / R + 1 / (jw * L), lambda: 1 / R, lambda: 1 / (jw * L + 1 / (jw * C)),
+                    lambda: 1 / (jw * L), lambda: 1 / (1 / (jw * C))],
+            }[typ]
+            y = chains[0]()
+            for f in chains[1:]:
+                if not ok(y):
+                    y = f()
+        if np.isnan(y) or np.isinf(y):
+            y = 0.0
+    return complex(y)
+
+
+def build_Yle_FFT(lumped_elements, port_nodes, escalings, n, w, val_chiusura):
+    """build_Yle_FFT.jl:1-126 (w unscaled, values scaled by escalings['Yle'])."""
+    acc = {}
+    for le in lumped_elements:
+        n1, n2 = le["nodes"]
+        y = lumped_admittance(le["type"], le["R"], le["L"], le["C"], w)
+        for (a, b), s in (((n1, n1), 1), ((n2, n2), 1), ((n1, n2), -1), ((n2, n1), -1)):
+            acc[(a, b)] = acc.get((a, b), 0.0) + s * y
+    for n1, n2 in port_nodes:
+        for (a, b), s in (((n1, n1), 1), ((n2, n2), 1), ((n1, n2), -1), ((n2, n1), -1)):
+            acc[(a, b)] = acc.get((a, b), 0.0) + s / val_chiusura
+    if not acc:
+        return sp.csc_matrix((n, n), dtype=np.complex128)
+    ks = list(acc)
+    return sp.csc_matrix(([acc[k] * escalings["Yle"] for k in ks], ([k[0] for k in ks], [k[1] for k in ks])),
+                         shape=(n, n), dtype=np.complex128)
+
+
+def compute_Z_self(R, Cd, w):
+    """compute_Z_self.jl:1-15."""
+    Z = np.zeros(R.shape[0], dtype=np.complex128)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for a in range(4):
+            r, c = R[:, a], Cd[:, a]
+            both = (r != 0) & (c != 0)
+            only_r = (r != 0) & (c == 0)
+            only_c = (r == 0) & (c != 0)
+            Z[both] += 1 / (1 / r[both] + 1j * w * c[both])
+            Z[only_r] += r[only_r]
+            Z[only_c] += 1 / (1j * w * c[only_c])
+    return Z
+
+
+def DZ_of(mesh, w):
+    """FFT_solver_QS_S_type.jl:106-114 with the scaled angular frequency w."""
+    esc = mesh.escalings
+    Z_self = compute_Z_self(mesh.diag_R, mesh.diag_Cd, w)
+    Zs = esc["R"] * (mesh.Zs * np.sqrt(w / esc["freq"]))
+    se = mesh.surface_edges
+    d = np.real(Zs[se]) - np.real(Z_self[se])
+    a, b = se[d > 0], se[d < 0]
+    Z_self[a] = 1j * np.imag(Z_self[a])
+    Zs[b] = 1j * np.imag(Zs[b])
+    return Z_self + np.real(Zs) + 1j * w * mesh.diag_Lp
+
+
+def SS_of(mesh, w, DZ, Yle):
+    """SS = Yle + A' Z^-1 A + j w Gamma P^-1 Gamma' (FFT_solver_QS_S_type.jl:118)."""
+    invZ = sp.diags(1.0 / DZ)
+    invP = sp.diags(1.0 / mesh.diag_P)
+    A = mesh.A.tocsr()
+    G = mesh.Gamma.tocsr()
+    return (Yle + A.T @ (invZ @ A) + 1j * w * (G @ invP) @ G.T).tocsc()
+
+
+def lu_factor(SS):
+    """Host LU of SS (the reference's lu(SS), :119) with SuperLU: minimum
+    degree on A'+A, no pivoting (SS is complex symmetric and diagonally
+    weighted), returned in Julia's convention L*U = SS[p, q]."""
+    F = spla.splu(SS, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0, options=dict(SymmetricMode=True))
+    p = np.argsort(F.perm_r)
+    q = np.argsort(F.perm_c)
+    return F.L, F.U, p, q
+
+
+class Solver:
+    """Holds one geometry on one device context."""
+
+    def __init__(self, mesh, ctx=None):
+        self.mesh = mesh
+        self.ctx = ctx or peec.default_context()
+        ok = peec.compute_FFT_mutual_coupling_mats(mesh.origin, mesh.Nx, mesh.Ny, mesh.Nz, mesh.sx, mesh.sy,
+                                                   mesh.sz, mesh.escalings, ctx=self.ctx)
+        if not ok:
+            raise RuntimeError("stopped")
+        peec.operator_set(mesh, ctx=self.ctx)
+        peec.set_ctx_size(self.ctx, mesh.m + mesh.ns + mesh.n)
+
+    def set_frequency(self, w, R0=50.0):
+        m = self.mesh
+        esc = m.escalings
+        Yle = build_Yle_FFT(m.lumped_elements, m.port_nodes, esc, m.n, w / esc["freq"], R0)
+        DZ = DZ_of(m, w)
+        peec.frequency_set(w, DZ, Yle, ctx=self.ctx)
+        Lf, Uf, p, q = lu_factor(SS_of(m, w, DZ, Yle))
+        peec.lu_set_factors(Lf, Uf, p, q, ctx=self.ctx)
+
+    def matvec(self, x):
+        return peec.ComputeMatrixVectorFTT(x, ctx=self.ctx)
+
+
+def FFT_solver_QS_S_type(freq, mesh, inner_iter, tol, ports_scatter_value=50.0, solver=None, record=None):
+    """FFT_solver_QS_S_type.jl:1-234 for QS_Rcc_FW = 1.  Returns
+    dict(S, Z, Y, f), or None when a stop was requested."""
+    solver = solver or Solver(mesh)
+    esc = mesh.escalings
+    freq = np.asarray(freq, dtype=np.float64) * esc["freq"]
+    tol = np.broadcast_to(np.asarray(tol, dtype=np.float64), freq.shape)
+    m, n, ns = mesh.m, mesh.n, mesh.ns
+    w = 2 * np.pi * freq
+    P = mesh.port_nodes.shape[0]
+    R0 = ports_scatter_value
+    S = np.zeros((P, P, len(freq)), dtype=np.complex128)
+    Vrest = np.zeros((m + n + ns, P), dtype=np.complex128)
+    is_ = np.zeros(n)
+    for k in range(len(freq)):
+        if solver.ctx._stop.value:
+            return None
+        solver.set_frequency(w[k], R0)
+        for c1 in range(P):
+            n1, n2 = mesh.port_nodes[c1]
+            is_[n1] = esc["Is"]
+            is_[n2] = -1 * esc["Is"]
+            tn = peec.precond_3_3_Kt_fft(is_, ctx=solver.ctx)
+            V, flag, relres, it, resvec = peec.gmres_custom(tn, False, tol[k], inner_iter, Vrest[:, c1],
+                                                            ctx=solver.ctx)
+            if flag == 99:
+                return None
+            if record is not None:
+                record.append(dict(k=k, port=c1, flag=flag, iter=list(it), relres=relres, nres=len(resvec)))
+            Vrest[:, c1] = V
+            is_[n1] = 0
+            is_[n2] = 0
+            for c2 in range(c1, P):
+                n3, n4 = mesh.port_nodes[c2]
+                if c1 == c2:
+                    S[c1, c2, k] = (2 * (V[m + ns + n3] - V[m + ns + n4]) - R0) / R0
+                else:
+                    S[c1, c2, k] = (2 * (V[m + ns + n3] - V[m + ns + n4])) / R0
+                    S[c2, c1, k] = S[c1, c2, k]
+    return dict(S=S, Z=s2z_fft(S, R0), Y=s2y_fft(S, R0), f=freq / esc["freq"])
+
+
+def s2z_fft(S, Zo):
+    """FFT_solver_QS_S_type.jl:265-274."""
+    P = S.shape[0]
+    I = np.eye(P)
+    return np.stack([Zo * np.linalg.solve(I - S[:, :, k], I + S[:, :, k]) for k in range(S.shape[2])], axis=2)
+
+
+def s2y_fft(S, Zo):
+    """FFT_solver_QS_S_type.jl:276-285, reproduced as written (it multiplies
+    by Zo where an admittance would divide)."""
+    P = S.shape[0]
+    I = np.eye(P)
+    return np.stack([Zo * np.linalg.solve(I + S[:, :, k], I - S[:, :, k]) for k in range(S.shape[2])], axis=2)
