@@ -12,14 +12,20 @@
  * Julia arithmetic details reproduced on purpose:
  *   x^2, x^3 -> x*x, x*x*x   (Base.literal_pow)
  *   x^4      -> compensated power-by-squaring (Base.Math.pow_body)
- *   real(log(complex(a))) -> 0.5*log(a*a) (+log1p branch near |a|=1), i.e.
+ *   real(or_det_log(complex(a))) -> 0.5*or_det_log(a*a) (+log1p branch near |a|=1), i.e.
  *             Julia's Complex log real part for an imaginary part of zero
  *   round(x*10^14)/10^14 with ties-to-even (rint)
  *   atan(y, x) -> atan2
+ * log, log1p and atan2 are the deterministic fdlibm-algorithm versions in
+ * detmath_oracle.h (the GPU kernels evaluate the same operation sequence),
+ * so far entries -- sums of corner terms that cancel by many orders of
+ * magnitude -- come out identical on both sides.
  * Built with -ffp-contract=off so that no multiply-add is fused.
  */
 #include <math.h>
 #include <stdint.h>
+
+#include "detmath_oracle.h"
 
 #define ISBAD(t) (isnan(t) || isinf(t))
 
@@ -39,7 +45,7 @@ static double jl_pow4(double x)
     return x4;
 }
 
-/* real(log(complex(a))) for real a (Base log(::Complex{Float64})). */
+/* real(or_det_log(complex(a))) for real a (Base or_det_log(::Complex{Float64})). */
 static double jl_clog_re(double a)
 {
     double b = fabs(a);
@@ -48,11 +54,11 @@ static double jl_clog_re(double a)
     if (isinf(b))
         return INFINITY;
     if (b < 1e-140 || b > 1e140)
-        return log(b);
+        return or_det_log(b);
     double rho = a * a;
     if (0.5 < b * b && (b <= 1.25 || rho < 3.0))
-        return log1p((b - 1.0) * (b + 1.0)) / 2.0;
-    return log(rho) / 2.0;
+        return or_det_log1p((b - 1.0) * (b + 1.0)) / 2.0;
+    return or_det_log(rho) / 2.0;
 }
 
 /* round_ud_fft(v, 14), compute_Lp_Voxels.jl:807-814 */
@@ -78,9 +84,9 @@ static double lp_line_point(const double *x1v, double y3, double z3,
         x1 = x1 + 1e-8;
     double R1 = sqrt((x1 - x3) * (x1 - x3) + (y1 - y3) * (y1 - y3) + (z1 - z3) * (z1 - z3));
     double R2 = sqrt((x1 - x4) * (x1 - x4) + (y1 - y3) * (y1 - y3) + (z1 - z3) * (z1 - z3));
-    double Ip = log((x1 - x3) + R1) - log((x1 - x4) + R2);
+    double Ip = or_det_log((x1 - x3) + R1) - or_det_log((x1 - x4) + R2);
     if (ISBAD(Ip))
-        Ip = (x1 - x3) / fabs(x1 - x3) * log(fabs(x1 - x3)) - (x1 - x4) / fabs(x1 - x4) * log(fabs(x1 - x4));
+        Ip = (x1 - x3) / fabs(x1 - x3) * or_det_log(fabs(x1 - x3)) - (x1 - x4) / fabs(x1 - x4) * or_det_log(fabs(x1 - x4));
     return Ip;
 }
 
@@ -98,7 +104,7 @@ static double point_sup(double x1, double y1, double z1,
             if (ISBAD(t1)) t1 = 0.0;
             double t2 = (y1 - y2) * jl_clog_re((x1 - x2) + R);
             if (ISBAD(t2)) t2 = 0.0;
-            double t3 = -fabs(z1 - z2) * atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
+            double t3 = -fabs(z1 - z2) * or_det_atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
             sol = sol + sgnpow(c1 + c2) * (t1 + t2 + t3);
         }
     }
@@ -174,7 +180,7 @@ static double line_line_ortho_xy(const double *x1v, double y1, double z1,
             if (ISBAD(t1)) t1 = 0.0;
             double t2 = (y1 - y2) * jl_clog_re((x1 - x2) + R);
             if (ISBAD(t2)) t2 = 0.0;
-            double t3 = -fabs(z1 - z2) * atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
+            double t3 = -fabs(z1 - z2) * or_det_atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
             sol = sol + sgnpow(c1 + c2 + 1) * (t1 + t2 + t3);
         }
     }
@@ -199,9 +205,9 @@ static double lp_point_vol(double x1, double y1, double z1,
                 if (ISBAD(t2)) t2 = 0.0;
                 double t3 = (y1 - y2) * (x1 - x2) * jl_clog_re((z1 - z2) + R);
                 if (ISBAD(t3)) t3 = 0.0;
-                double t4 = -1.0 / 2 * fabs(z1 - z2) * (z1 - z2) * atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
-                double t5 = -1.0 / 2 * fabs(y1 - y2) * (y1 - y2) * atan2((x1 - x2) * (z1 - z2), (fabs(y1 - y2) * R));
-                double t6 = -1.0 / 2 * fabs(x1 - x2) * (x1 - x2) * atan2((y1 - y2) * (z1 - z2), (fabs(x1 - x2) * R));
+                double t4 = -1.0 / 2 * fabs(z1 - z2) * (z1 - z2) * or_det_atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
+                double t5 = -1.0 / 2 * fabs(y1 - y2) * (y1 - y2) * or_det_atan2((x1 - x2) * (z1 - z2), (fabs(y1 - y2) * R));
+                double t6 = -1.0 / 2 * fabs(x1 - x2) * (x1 - x2) * or_det_atan2((y1 - y2) * (z1 - z2), (fabs(x1 - x2) * R));
                 sol = sol + sgnpow(c1 + c2 + c3 + 1) * (t1 + t2 + t3 + t4 + t5 + t6);
             }
         }
@@ -226,7 +232,7 @@ static double line_surf_ortho(const double *x1v, double y1, double z1,
                 if (ISBAD(t1)) t1 = 0.0;
                 double t2 = (x1 - x2) * (z1 - z2) * jl_clog_re((y1 - y2) + R);
                 if (ISBAD(t2)) t2 = 0.0;
-                double t3 = -fabs(z1 - z2) * atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
+                double t3 = -fabs(z1 - z2) * or_det_atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
                 sol = sol + sgnpow(c1 + c2 + c3) * (t1 + t2 + t3);
             }
         }
@@ -253,9 +259,9 @@ static double p_line_surf_ortho(const double *x1v, double y1, double z1,
                 if (ISBAD(t2)) t2 = 0.0;
                 double t3 = (y1 - y2) * (x1 - x2) * jl_clog_re((z1 - z2) + R);
                 if (ISBAD(t3)) t3 = 0.0;
-                double t4 = -1.0 / 2 * fabs(z1 - z2) * (z1 - z2) * atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
-                double t5 = -1.0 / 2 * fabs(y1 - y2) * (y1 - y2) * atan2((x1 - x2) * (z1 - z2), (fabs(y1 - y2) * R));
-                double t6 = -1.0 / 2 * fabs(x1 - x2) * (x1 - x2) * atan2((y1 - y2) * (z1 - z2), (fabs(x1 - x2) * R));
+                double t4 = -1.0 / 2 * fabs(z1 - z2) * (z1 - z2) * or_det_atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
+                double t5 = -1.0 / 2 * fabs(y1 - y2) * (y1 - y2) * or_det_atan2((x1 - x2) * (z1 - z2), (fabs(y1 - y2) * R));
+                double t6 = -1.0 / 2 * fabs(x1 - x2) * (x1 - x2) * or_det_atan2((y1 - y2) * (z1 - z2), (fabs(x1 - x2) * R));
                 sol = sol + sgnpow(c1 + c2 + c3) * (t1 + t2 + t3 + t4 + t5 + t6);
             }
         }
@@ -279,7 +285,7 @@ static double line_surf_para(const double *x1v, const double *y1v, double z1,
                 if (ISBAD(t1)) t1 = 0.0;
                 double t2 = ((x1 - x2) * (x1 - x2) - (z1 - z2) * (z1 - z2)) / 2 * jl_clog_re((y1 - y2) + R);
                 if (ISBAD(t2)) t2 = 0.0;
-                double t3 = -(x1 - x2) * fabs(z1 - z2) * atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
+                double t3 = -(x1 - x2) * fabs(z1 - z2) * or_det_atan2((x1 - x2) * (y1 - y2), (fabs(z1 - z2) * R));
                 double t4 = -(y1 - y2) / 2 * R;
                 sol = sol + sgnpow(c1 + c2 + c3 + 1) * (t1 + t2 + t3 + t4);
             }
@@ -304,9 +310,9 @@ static double lp_line_vol(const double *x1v, const double *y1v, const double *z1
                     double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
                     double R = sqrt(dx * dx + dy * dy + dz * dz);
                     double t1 = -1.0 / 3 * dy * dz * R;
-                    double t2 = -1.0 / 2 * dx * fabs(dz) * dz * atan2(dx * dy, (fabs(dz) * R));
-                    double t3 = -1.0 / 2 * dx * fabs(dy) * dy * atan2(dx * dz, (fabs(dy) * R));
-                    double t4 = -1.0 / 6 * (fabs(dx) * fabs(dx) * fabs(dx)) * atan2(dy * dz, (fabs(dx) * R));
+                    double t2 = -1.0 / 2 * dx * fabs(dz) * dz * or_det_atan2(dx * dy, (fabs(dz) * R));
+                    double t3 = -1.0 / 2 * dx * fabs(dy) * dy * or_det_atan2(dx * dz, (fabs(dy) * R));
+                    double t4 = -1.0 / 6 * (fabs(dx) * fabs(dx) * fabs(dx)) * or_det_atan2(dy * dz, (fabs(dx) * R));
                     double t5 = dx * dy * dz * jl_clog_re(dx + R);
                     if (ISBAD(t5)) t5 = 0.0;
                     double t6 = (1.0 / 2 * (dx * dx) - 1.0 / 6 * (dz * dz)) * dz * jl_clog_re(dy + R);
@@ -337,7 +343,7 @@ static double lp_surf_surf_para(const double *x1v, const double *y1v, double z1,
                     double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
                     double R = sqrt(dx * dx + dy * dy + dz * dz);
                     double t1 = -1.0 / 6 * (dx * dx + dy * dy - 2 * (dz * dz)) * R;
-                    double t2 = -dx * dy * fabs(dz) * atan2(dx * dy, (fabs(dz) * R));
+                    double t2 = -dx * dy * fabs(dz) * or_det_atan2(dx * dy, (fabs(dz) * R));
                     double t3 = 1.0 / 2 * (dx * dx - dz * dz) * dy * jl_clog_re(dy + R);
                     if (ISBAD(t3)) t3 = 0.0;
                     double t4 = 1.0 / 2 * (dy * dy - dz * dz) * dx * jl_clog_re(dx + R);
@@ -366,10 +372,10 @@ static double qs_surf_surf_para(const double *x1v, const double *y1v, double z1,
                     double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
                     double R = sqrt(dx * dx + dy * dy + dz * dz);
                     double t1 = -1.0 / 6 * (dx * dx + dy * dy - 2 * (dz * dz)) * R;
-                    double t2 = -dx * dy * fabs(dz) * atan2(dx * dy, fabs(dz) * R);
-                    double t3 = 1.0 / 2 * (dx * dx - dz * dz) * dy * log(dy + R);
+                    double t2 = -dx * dy * fabs(dz) * or_det_atan2(dx * dy, fabs(dz) * R);
+                    double t3 = 1.0 / 2 * (dx * dx - dz * dz) * dy * or_det_log(dy + R);
                     if (ISBAD(t3)) t3 = 0.0;
-                    double t4 = 1.0 / 2 * (dy * dy - dz * dz) * dx * log(dx + R);
+                    double t4 = 1.0 / 2 * (dy * dy - dz * dz) * dx * or_det_log(dx + R);
                     if (ISBAD(t4)) t4 = 0.0;
                     sol += sgnpow(c1 + c2 + c3 + c4) * (t1 + t2 + t3 + t4);
                 }
@@ -395,7 +401,7 @@ static double lp_surf_surf_ortho(const double *x1v, const double *y1v, double z1
                     double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
                     double R = sqrt(dx * dx + dy * dy + dz * dz);
                     double t1 = -1.0 / 3 * dy * dz * R;
-                    double t2 = -1.0 / 2 * dx * fabs(dz) * dz * atan2(dx * dy, (fabs(dz) * R));
+                    double t2 = -1.0 / 2 * dx * fabs(dz) * dz * or_det_atan2(dx * dy, (fabs(dz) * R));
                     double t3 = 1.0 / 2 * (dx * dx - dz * dz) * dy * jl_clog_re(dy + R);
                     if (ISBAD(t3)) t3 = 0.0;
                     double t4 = 1.0 / 2 * (dx * dx - dy * dy) * dx * jl_clog_re(dx + R);
@@ -424,9 +430,9 @@ static double p_surf_surf_ortho(const double *x1v, const double *y1v, double z1,
                     double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
                     double R = sqrt(dx * dx + dy * dy + dz * dz);
                     double t1 = -1.0 / 3 * dy * dz * R;
-                    double t2 = -1.0 / 2 * dx * fabs(dz) * dz * atan2(dx * dy, fabs(dz) * R);
-                    double t3 = -1.0 / 2 * dx * fabs(dy) * dy * atan2(dx * dz, fabs(dy) * R);
-                    double t4 = -1.0 / 6 * (fabs(dx) * fabs(dx) * fabs(dx)) * atan2(dy * dz, fabs(dx) * R);
+                    double t2 = -1.0 / 2 * dx * fabs(dz) * dz * or_det_atan2(dx * dy, fabs(dz) * R);
+                    double t3 = -1.0 / 2 * dx * fabs(dy) * dy * or_det_atan2(dx * dz, fabs(dy) * R);
+                    double t4 = -1.0 / 6 * (fabs(dx) * fabs(dx) * fabs(dx)) * or_det_atan2(dy * dz, fabs(dx) * R);
                     double t5 = dx * dy * dz * jl_clog_re(dx + R);
                     if (ISBAD(t5)) t5 = 0.0;
                     double t6 = (1.0 / 2 * (dx * dx) - 1.0 / 6 * (dz * dz)) * dz * jl_clog_re(dy + R);
@@ -465,9 +471,9 @@ static double lp_vol_surf(const double *x1v, const double *y1v, const double *z1
                         if (ISBAD(t3)) t3 = 0.0;
                         double t4 = (-jl_pow4(dx) / 24 - jl_pow4(dy) / 24 + ((dy * dy) * (dx * dx)) / 4) * jl_clog_re(dz + R);
                         if (ISBAD(t4)) t4 = 0.0;
-                        double t5 = -(fabs(dx) * fabs(dx) * fabs(dx)) * dy / 6 * atan2(dy * dz, (fabs(dx) * R));
-                        double t6 = -dx * (fabs(dy) * fabs(dy) * fabs(dy)) / 6 * atan2(dx * dz, (fabs(dy) * R));
-                        double t7 = -dx * dy * fabs(dz) * dz / 2 * atan2(dx * dy, (fabs(dz) * R));
+                        double t5 = -(fabs(dx) * fabs(dx) * fabs(dx)) * dy / 6 * or_det_atan2(dy * dz, (fabs(dx) * R));
+                        double t6 = -dx * (fabs(dy) * fabs(dy) * fabs(dy)) / 6 * or_det_atan2(dx * dz, (fabs(dy) * R));
+                        double t7 = -dx * dy * fabs(dz) * dz / 2 * or_det_atan2(dx * dy, (fabs(dz) * R));
                         sol = sol + sgnpow(c1 + c2 + c3 + c4 + c5) * (t1 + t2 + t3 + t4 + t5 + t6 + t7);
                     }
                 }
@@ -504,9 +510,9 @@ static double lp_vol_vol(const double *x1v, const double *y1v, const double *z1v
                             if (ISBAD(t3)) t3 = 0.0;
                             double t4 = ((dz * dz) * (dx * dx) / 4 - jl_pow4(dz) / 24 - jl_pow4(dx) / 24) * dy * jl_clog_re(dy + R);
                             if (ISBAD(t4)) t4 = 0.0;
-                            double t5 = -(fabs(dx) * fabs(dx) * fabs(dx)) * dy * dz / 6 * atan2(dy * dz, (fabs(dx) * R));
-                            double t6 = -dx * (fabs(dy) * fabs(dy) * fabs(dy)) * dz / 6 * atan2(dx * dz, (fabs(dy) * R));
-                            double t7 = -dx * dy * (fabs(dz) * fabs(dz) * fabs(dz)) / 6 * atan2(dx * dy, (fabs(dz) * R));
+                            double t5 = -(fabs(dx) * fabs(dx) * fabs(dx)) * dy * dz / 6 * or_det_atan2(dy * dz, (fabs(dx) * R));
+                            double t6 = -dx * (fabs(dy) * fabs(dy) * fabs(dy)) * dz / 6 * or_det_atan2(dx * dz, (fabs(dy) * R));
+                            double t7 = -dx * dy * (fabs(dz) * fabs(dz) * fabs(dz)) / 6 * or_det_atan2(dx * dy, (fabs(dz) * R));
                             sol = sol + sgnpow(c1 + c2 + c3 + c4 + c5 + c6 + 1) * (t1 + t2 + t3 + t4 + t5 + t6 + t7);
                         }
                     }

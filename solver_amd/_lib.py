@@ -51,10 +51,21 @@ def lib():
         L.peec_lp_row.argtypes = [vp, dp, dp, i64] + [ctypes.c_double] * 6 + [ctypes.c_int, dp]
         L.peec_p_row.argtypes = [vp, dp, dp, i64] + [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_int, dp]
         L.peec_lp_self.argtypes = [ctypes.c_double] * 3 + [dp]
-        for name in ("peec_kernels_build", "peec_kernel_row", "peec_operator_set", "peec_frequency_set",
-                     "peec_lu_set_factors", "peec_matvec", "peec_precond_rhs", "peec_gmres"):
-            if hasattr(L, name):
-                getattr(L, name).restype = ctypes.c_int
+        d = ctypes.c_double
+        L.peec_kernels_build.argtypes = [vp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, d, d, d, d, d]
+        L.peec_kernel_row.argtypes = [vp, ctypes.c_int, dp, i64p]
+        L.peec_operator_set.argtypes = [vp, i64, i64, i64, ctypes.POINTER(CscF64), ctypes.POINTER(CscF64),
+                                        ctypes.POINTER(CscF64), ctypes.POINTER(CscF64), dp]
+        L.peec_frequency_set.argtypes = [vp, d, c128p, ctypes.POINTER(CscC128)]
+        L.peec_lu_set_factors.argtypes = [vp, i64, ctypes.POINTER(CscC128), ctypes.POINTER(CscC128), i64p, i64p, dp]
+        L.peec_matvec.argtypes = [vp, c128p, c128p]
+        L.peec_apply_K.argtypes = [vp, c128p, c128p]
+        L.peec_precond_rhs.argtypes = [vp, dp, c128p]
+        L.peec_gmres.argtypes = [vp, c128p, d, ctypes.c_int, c128p, ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(d), i64p, dp, i64p]
+        L.peec_test_lp_row_host.argtypes = [dp, dp, i64] + [d] * 6 + [ctypes.c_int, dp]
+        L.peec_test_p_row_host.argtypes = [dp, dp, i64] + [d] * 3 + [ctypes.c_int, ctypes.c_int, dp]
+        L.peec_test_probe.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, i64, dp]
         _lib = L
     return _lib
 

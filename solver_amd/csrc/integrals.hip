@@ -11,10 +11,14 @@
 // Arithmetic matches the reference's Julia semantics: x^2 = x*x, x^3 =
 // x*x*x, x^4 = compensated square-of-square (Base.Math.pow_body), the real
 // part of a complex log of a real number = 0.5*log(a*a), ties-to-even
-// coordinate rounding to 1e-14 m, NaN/Inf terms zeroed.  This file is built
+// coordinate rounding to 1e-14 m, NaN/Inf terms zeroed.  log, log1p and
+// atan2 are the deterministic fdlibm-algorithm versions of detmath.h, so
+// the cancelling corner sums come out identical on the CPU and the GPU.
+// This file is built
 // with -ffp-contract=off so no multiply-add is fused behind the formulas'
 // backs.
 #include "peec_internal.h"
+#include "detmath.h"
 
 #include <cmath>
 
@@ -42,10 +46,10 @@ HD inline double clog_re(double a)
     double b = fabs(a);
     if (b == 0.0) return -INFINITY;
     if (__builtin_isinf(b)) return INFINITY;
-    if (b < 1e-140 || b > 1e140) return log(b);
+    if (b < 1e-140 || b > 1e140) return det_log(b);
     double rho = a * a;
-    if (0.5 < b * b && (b <= 1.25 || rho < 3.0)) return log1p((b - 1.0) * (b + 1.0)) / 2.0;
-    return log(rho) / 2.0;
+    if (0.5 < b * b && (b <= 1.25 || rho < 3.0)) return det_log1p((b - 1.0) * (b + 1.0)) / 2.0;
+    return det_log(rho) / 2.0;
 }
 
 HD inline double round14(double v) { return rint(v * 1e14) / 1e14; }
@@ -86,9 +90,9 @@ HD double f_vol_vol(const double *x1v, const double *y1v, const double *z1v,
         if (bad(t3)) t3 = 0.0;
         double t4 = (dz2 * dx2 / 4 - dz4 / 24 - dx4 / 24) * dy * clog_re(dy + R);
         if (bad(t4)) t4 = 0.0;
-        double t5 = -cube_abs(dx) * dy * dz / 6 * atan2(dy * dz, (fabs(dx) * R));
-        double t6 = -dx * cube_abs(dy) * dz / 6 * atan2(dx * dz, (fabs(dy) * R));
-        double t7 = -dx * dy * cube_abs(dz) / 6 * atan2(dx * dy, (fabs(dz) * R));
+        double t5 = -cube_abs(dx) * dy * dz / 6 * det_atan2(dy * dz, (fabs(dx) * R));
+        double t6 = -dx * cube_abs(dy) * dz / 6 * det_atan2(dx * dz, (fabs(dy) * R));
+        double t7 = -dx * dy * cube_abs(dz) / 6 * det_atan2(dx * dy, (fabs(dz) * R));
         sol = sol + sgn(c1 + c2 + c3 + c4 + c5 + c6 + 1) * (t1 + t2 + t3 + t4 + t5 + t6 + t7);
     }
     return sol;
@@ -111,9 +115,9 @@ HD double f_vol_surf(const double *x1v, const double *y1v, const double *z1v,
         if (bad(t3)) t3 = 0.0;
         double t4 = (-pow4(dx) / 24 - pow4(dy) / 24 + ((dy * dy) * (dx * dx)) / 4) * clog_re(dz + R);
         if (bad(t4)) t4 = 0.0;
-        double t5 = -cube_abs(dx) * dy / 6 * atan2(dy * dz, (fabs(dx) * R));
-        double t6 = -dx * cube_abs(dy) / 6 * atan2(dx * dz, (fabs(dy) * R));
-        double t7 = -dx * dy * fabs(dz) * dz / 2 * atan2(dx * dy, (fabs(dz) * R));
+        double t5 = -cube_abs(dx) * dy / 6 * det_atan2(dy * dz, (fabs(dx) * R));
+        double t6 = -dx * cube_abs(dy) / 6 * det_atan2(dx * dz, (fabs(dy) * R));
+        double t7 = -dx * dy * fabs(dz) * dz / 2 * det_atan2(dx * dy, (fabs(dz) * R));
         sol = sol + sgn(c1 + c2 + c3 + c4 + c5 + 1) * (t1 + t2 + t3 + t4 + t5 + t6 + t7);
     }
     return sol;
@@ -132,10 +136,10 @@ HD double f_surf_surf_para(const double *x1v, const double *y1v, double z1,
         double dx = x1v[c1] - x2v[c4], dy = y1v[c2] - y2v[c3], dz = z1 - z2;
         double R = sqrt(dx * dx + dy * dy + dz * dz);
         double t1 = -1.0 / 6 * (dx * dx + dy * dy - 2 * (dz * dz)) * R;
-        double t2 = -dx * dy * fabs(dz) * atan2(dx * dy, (fabs(dz) * R));
-        double t3 = 1.0 / 2 * (dx * dx - dz * dz) * dy * (REAL_LOG ? log(dy + R) : clog_re(dy + R));
+        double t2 = -dx * dy * fabs(dz) * det_atan2(dx * dy, (fabs(dz) * R));
+        double t3 = 1.0 / 2 * (dx * dx - dz * dz) * dy * (REAL_LOG ? det_log(dy + R) : clog_re(dy + R));
         if (bad(t3)) t3 = 0.0;
-        double t4 = 1.0 / 2 * (dy * dy - dz * dz) * dx * (REAL_LOG ? log(dx + R) : clog_re(dx + R));
+        double t4 = 1.0 / 2 * (dy * dy - dz * dz) * dx * (REAL_LOG ? det_log(dx + R) : clog_re(dx + R));
         if (bad(t4)) t4 = 0.0;
         sol = sol + sgn(c1 + c2 + c3 + c4) * (t1 + t2 + t3 + t4);
     }
@@ -155,11 +159,11 @@ HD double f_surf_surf_ortho(const double *x1v, const double *y1v, double z1,
         double dx = x1v[c1] - x2v[c4], dy = y1v[c2] - y2, dz = z1 - z2v[c3];
         double R = sqrt(dx * dx + dy * dy + dz * dz);
         double t1 = -1.0 / 3 * dy * dz * R;
-        double t2 = -1.0 / 2 * dx * fabs(dz) * dz * atan2(dx * dy, (fabs(dz) * R));
+        double t2 = -1.0 / 2 * dx * fabs(dz) * dz * det_atan2(dx * dy, (fabs(dz) * R));
         double acc;
         if (P_COPY) {
-            double t3 = -1.0 / 2 * dx * fabs(dy) * dy * atan2(dx * dz, fabs(dy) * R);
-            double t4 = -1.0 / 6 * cube_abs(dx) * atan2(dy * dz, fabs(dx) * R);
+            double t3 = -1.0 / 2 * dx * fabs(dy) * dy * det_atan2(dx * dz, fabs(dy) * R);
+            double t4 = -1.0 / 6 * cube_abs(dx) * det_atan2(dy * dz, fabs(dx) * R);
             double t5 = dx * dy * dz * clog_re(dx + R);
             if (bad(t5)) t5 = 0.0;
             double t6 = (1.0 / 2 * (dx * dx) - 1.0 / 6 * (dz * dz)) * dz * clog_re(dy + R);
@@ -190,9 +194,9 @@ HD double f_line_vol(const double *x1v, const double *y1v, const double *z1v,
         double dx = x1v[c1] - x2v[c4], dy = y1v[c2] - y2, dz = z1v[c3] - z2;
         double R = sqrt(dx * dx + dy * dy + dz * dz);
         double t1 = -1.0 / 3 * dy * dz * R;
-        double t2 = -1.0 / 2 * dx * fabs(dz) * dz * atan2(dx * dy, (fabs(dz) * R));
-        double t3 = -1.0 / 2 * dx * fabs(dy) * dy * atan2(dx * dz, (fabs(dy) * R));
-        double t4 = -1.0 / 6 * cube_abs(dx) * atan2(dy * dz, (fabs(dx) * R));
+        double t2 = -1.0 / 2 * dx * fabs(dz) * dz * det_atan2(dx * dy, (fabs(dz) * R));
+        double t3 = -1.0 / 2 * dx * fabs(dy) * dy * det_atan2(dx * dz, (fabs(dy) * R));
+        double t4 = -1.0 / 6 * cube_abs(dx) * det_atan2(dy * dz, (fabs(dx) * R));
         double t5 = dx * dy * dz * clog_re(dx + R);
         if (bad(t5)) t5 = 0.0;
         double t6 = (1.0 / 2 * (dx * dx) - 1.0 / 6 * (dz * dz)) * dz * clog_re(dy + R);
@@ -218,7 +222,7 @@ HD double f_line_surf_para(const double *x1v, const double *y1v, double z1,
         if (bad(t1)) t1 = 0.0;
         double t2 = (dx * dx - dz * dz) / 2 * clog_re(dy + R);
         if (bad(t2)) t2 = 0.0;
-        double t3 = -dx * fabs(dz) * atan2(dx * dy, (fabs(dz) * R));
+        double t3 = -dx * fabs(dz) * det_atan2(dx * dy, (fabs(dz) * R));
         double t4 = -dy / 2 * R;
         sol = sol + sgn(c1 + c2 + c3) * (t1 + t2 + t3 + t4);
     }
@@ -245,12 +249,12 @@ HD double f_line_surf_ortho(const double *x1v, double y1, double z1,
         if (P_COPY) {
             double t3 = dy * dx * clog_re(dz + R);
             if (bad(t3)) t3 = 0.0;
-            double t4 = -1.0 / 2 * fabs(dz) * dz * atan2(dx * dy, (fabs(dz) * R));
-            double t5 = -1.0 / 2 * fabs(dy) * dy * atan2(dx * dz, (fabs(dy) * R));
-            double t6 = -1.0 / 2 * fabs(dx) * dx * atan2(dy * dz, (fabs(dx) * R));
+            double t4 = -1.0 / 2 * fabs(dz) * dz * det_atan2(dx * dy, (fabs(dz) * R));
+            double t5 = -1.0 / 2 * fabs(dy) * dy * det_atan2(dx * dz, (fabs(dy) * R));
+            double t6 = -1.0 / 2 * fabs(dx) * dx * det_atan2(dy * dz, (fabs(dx) * R));
             acc = t1 + t2 + t3 + t4 + t5 + t6;
         } else {
-            double t3 = -fabs(dz) * atan2(dx * dy, (fabs(dz) * R));
+            double t3 = -fabs(dz) * det_atan2(dx * dy, (fabs(dz) * R));
             acc = t1 + t2 + t3;
         }
         sol = sol + sgn(c1 + c2 + c3 + 1) * acc;
@@ -274,9 +278,9 @@ HD double f_point_vol(double x1, double y1, double z1,
         if (bad(t2)) t2 = 0.0;
         double t3 = dy * dx * clog_re(dz + R);
         if (bad(t3)) t3 = 0.0;
-        double t4 = -1.0 / 2 * fabs(dz) * dz * atan2(dx * dy, (fabs(dz) * R));
-        double t5 = -1.0 / 2 * fabs(dy) * dy * atan2(dx * dz, (fabs(dy) * R));
-        double t6 = -1.0 / 2 * fabs(dx) * dx * atan2(dy * dz, (fabs(dx) * R));
+        double t4 = -1.0 / 2 * fabs(dz) * dz * det_atan2(dx * dy, (fabs(dz) * R));
+        double t5 = -1.0 / 2 * fabs(dy) * dy * det_atan2(dx * dz, (fabs(dy) * R));
+        double t6 = -1.0 / 2 * fabs(dx) * dx * det_atan2(dy * dz, (fabs(dx) * R));
         sol = sol + sgn(c1 + c2 + c3) * (t1 + t2 + t3 + t4 + t5 + t6);
     }
     return sol;
@@ -299,7 +303,7 @@ HD double f_two_axis(const double *xv, double xfix, const double *yv, double yfi
         if (bad(t1)) t1 = 0.0;
         double t2 = dy * clog_re(dx + R);
         if (bad(t2)) t2 = 0.0;
-        double t3 = -fabs(dz) * atan2(dx * dy, (fabs(dz) * R));
+        double t3 = -fabs(dz) * det_atan2(dx * dy, (fabs(dz) * R));
         sol = sol + sgn(c1 + c2 + so) * (t1 + t2 + t3);
     }
     return sol;
@@ -347,9 +351,9 @@ HD double f_line_point(const double *x1v, double y3, double z3, double x1, doubl
     if (__builtin_isnan(check) || (__builtin_isinf(check) && x1 > 0)) x1 = x1 + 1e-8;
     double R1 = sqrt((x1 - x3) * (x1 - x3) + (y1 - y3) * (y1 - y3) + (z1 - z3) * (z1 - z3));
     double R2 = sqrt((x1 - x4) * (x1 - x4) + (y1 - y3) * (y1 - y3) + (z1 - z3) * (z1 - z3));
-    double Ip = log((x1 - x3) + R1) - log((x1 - x4) + R2);
+    double Ip = det_log((x1 - x3) + R1) - det_log((x1 - x4) + R2);
     if (bad(Ip))
-        Ip = (x1 - x3) / fabs(x1 - x3) * log(fabs(x1 - x3)) - (x1 - x4) / fabs(x1 - x4) * log(fabs(x1 - x4));
+        Ip = (x1 - x3) / fabs(x1 - x3) * det_log(fabs(x1 - x3)) - (x1 - x4) / fabs(x1 - x4) * det_log(fabs(x1 - x4));
     return Ip;
 }
 
@@ -459,8 +463,9 @@ HD void classify(int s1, int s2, const Box &b1, const Box &b2, bool is_p, Call *
         int a1 = (~s1) & 7, a2 = (~s2) & 7;
         a1 = a1 == 1 ? 0 : a1 == 2 ? 1 : 2;
         a2 = a2 == 1 ? 0 : a2 == 2 ? 1 : 2;
-        double f1 = 1.0, f2 = 1.0;
-        for (int a = 0; a < 3; a++) { if (a != a1) f1 *= L1[a]; if (a != a2) f2 *= L2[a]; }
+        // ((l1 * l1') * l2) * l2', each box's two cross lengths in axis order
+        const int u1 = a1 == 0 ? 1 : 0, v1 = a1 == 2 ? 1 : 2;
+        const int u2 = a2 == 0 ? 1 : 0, v2 = a2 == 2 ? 1 : 2;
         if (a1 == a2) {
             // parallel: form axes (a1, the two others in increasing order)
             int o = a1 == 0 ? 1 : 0;
@@ -469,7 +474,7 @@ HD void classify(int s1, int s2, const Box &b1, const Box &b2, bool is_p, Call *
             // orthogonal: form x = a1, form y = a2
             *k = Call{LLO, perm_id(a1, a2), false};
         }
-        *pre = f1 * f2;
+        *pre = L1[u1] * L1[v1] * L2[u2] * L2[v2];
         return;
     }
     if (sum == 3) {
@@ -487,9 +492,8 @@ HD void classify(int s1, int s2, const Box &b1, const Box &b2, bool is_p, Call *
         const double *Lsu = v1surf ? L1 : L2, *Lln = v1surf ? L2 : L1;
         int nrm = (ss & 1) ? 0 : (ss & 2) ? 1 : 2;
         int lax = (ls & 1) == 0 ? 0 : (ls & 2) == 0 ? 1 : 2;
-        double fl = 1.0;
-        for (int a = 0; a < 3; a++) if (a != lax) fl *= Lln[a];
-        *pre = Lsu[nrm] * fl;
+        const int lu = lax == 0 ? 1 : 0, lv = lax == 2 ? 1 : 2;
+        *pre = Lsu[nrm] * Lln[lu] * Lln[lv];
         if (lax == nrm) {
             // line orthogonal to the surface: form (lax, then the surface axes)
             int o = lax == 0 ? 1 : 0;
@@ -672,7 +676,7 @@ HD Box p_face(int caser, double cx, double cy, double cz, double sx, double sy, 
     Box b;
     double c[3] = {cx, cy, cz};
     double s[3] = {sx, sy, sz};
-    int nrm = (caser - 1) >> 1;
+    int nrm = caser <= 2 ? 1 : (caser <= 4 ? 0 : 2);  // caser 1,2: y-faces; 3,4: x-faces; 5,6: z-faces
     double sg = (caser & 1) ? -1.0 : 1.0;
     c[nrm] = c[nrm] + sg * (s[nrm] / 2);
     s[nrm] = 0.0;
@@ -811,8 +815,8 @@ __global__ void k_probe(int fn, const double *x, const double *y, int64_t n, dou
     double r = 0.0;
     switch (fn) {
     case 0: r = clog_re(a); break;
-    case 1: r = log(a); break;
-    case 2: r = atan2(a, b); break;
+    case 1: r = det_log(a); break;
+    case 2: r = det_atan2(a, b); break;
     case 3: r = sqrt(a); break;
     case 4: r = pow4(a); break;
     }
@@ -823,8 +827,8 @@ HD double probe_host(int fn, double a, double b)
 {
     switch (fn) {
     case 0: return clog_re(a);
-    case 1: return log(a);
-    case 2: return atan2(a, b);
+    case 1: return det_log(a);
+    case 2: return det_atan2(a, b);
     case 3: return sqrt(a);
     case 4: return pow4(a);
     }
@@ -871,7 +875,7 @@ static Box host_p_face(int caser, const double *cc, double sx, double sy, double
 {
     Box b;
     double c[3] = {cc[0], cc[1], cc[2]}, s[3] = {sx, sy, sz};
-    int nrm = (caser - 1) >> 1;
+    int nrm = caser <= 2 ? 1 : (caser <= 4 ? 0 : 2);  // caser 1,2: y-faces; 3,4: x-faces; 5,6: z-faces
     double sg = (caser & 1) ? -1.0 : 1.0;
     c[nrm] = c[nrm] + sg * (s[nrm] / 2);
     s[nrm] = 0.0;

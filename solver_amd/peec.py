@@ -170,6 +170,7 @@ def operator_set(mesh, ctx=None):
     dP = np.ascontiguousarray(mesh.diag_P, dtype=np.float64)
     L.check(L.lib().peec_operator_set(ctx.handle, int(mesh.mx), int(mesh.my), int(mesh.mz), ctypes.byref(A),
                                       ctypes.byref(G), lp, pm, L.as_dp(dP)), "peec_operator_set")
+    set_ctx_size(ctx, int(mesh.m + mesh.ns + mesh.n))
 
 
 def frequency_set(w, DZ, Yle, ctx=None):
@@ -190,6 +191,7 @@ def lu_set_factors(Lf, Uf, p, q, Rs=None, ctx=None):
     L.check(L.lib().peec_lu_set_factors(ctx.handle, int(Lf.shape[0]), ctypes.byref(Ls), ctypes.byref(Us),
                                         pp.ctypes.data_as(L.i64p), qq.ctypes.data_as(L.i64p),
                                         None if rs is None else L.as_dp(rs)), "peec_lu_set_factors")
+    del k1, k2
 
 
 def ComputeMatrixVectorFTT(x, ctx=None):
