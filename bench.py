@@ -1,0 +1,193 @@
+"""Benchmark: frequency-point solves per second of the voxel-PEEC FFT solver.
+
+One step = one frequency point of the sweep: the host assembles SS and
+factors it (as the reference does), then every port right-hand side is
+solved by GMRES on the GPU (FFT convolutions, sync-free triangular solves,
+Householder GMRES), warm-started from the previous frequency like
+FFT_solver_QS_S_type.jl:140,196.  Inputs are resident on the device before
+the timed region; the one-time geometry work (integrals, spectra, operator
+upload) is reported separately as setup_s.
+
+Prints one JSON line (rank 0, N = 1).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_mesh(cfg, p_self):
+    from solver_amd import mesher
+    return mesher.mesh(cfg["grid"], cfg["materials"], cfg["cell"], cfg["origin"], cfg["ports"], p_self_fn=p_self)
+
+
+def gpu_p_self(sx, sy, sz):
+    from solver_amd import peec
+    c = np.zeros((1, 3))
+    return [peec.compute_row_P_sup(c[0], c, sx, sy, sz, cs, cs)[0, 0] for cs in (1, 3, 5)]
+
+
+def profile_read(ctx, enable):
+    import ctypes
+    from solver_amd import _lib as L
+    nk = len(KIND_NAMES)
+    ms = (ctypes.c_double * nk)()
+    by = (ctypes.c_double * nk)()
+    cnt = (ctypes.c_int64 * nk)()
+    L.check(L.lib().peec_profile(ctx.handle, int(enable), ms, by, cnt, nk), "peec_profile")
+    return [dict(name=KIND_NAMES[k], ms=ms[k], bytes=by[k], count=cnt[k]) for k in range(nk)]
+
+
+def cpu_baseline(mesh, cfg, gpu_iters, sample_matvecs):
+    """The oracle (numpy/scipy restatement) on the host: one LU of SS plus
+    `sample_matvecs` preconditioned matvecs, extrapolated to one frequency
+    point with the GPU run's average GMRES matvec count per frequency."""
+    from oracle import fft_solver as OF
+    t0 = time.perf_counter()
+    rows = OF.kernel_rows(mesh.origin, mesh.Nx, mesh.Ny, mesh.Nz, mesh.sx, mesh.sy, mesh.sz)
+    spec = OF.spectra(rows, mesh.Nx, mesh.Ny, mesh.Nz)
+    t_setup = time.perf_counter() - t0
+    f = cfg["freqs"][len(cfg["freqs"]) // 2]
+    w = 2 * np.pi * f * mesh.escalings["freq"]
+    t0 = time.perf_counter()
+    Yle = OF.build_Yle(mesh.lumped_elements, mesh.port_nodes, mesh.escalings["Yle"], mesh.n, w / mesh.escalings["freq"], 50.0)
+    op = OF.Operator(mesh, spec)
+    op.set_frequency(w, OF.frequency_terms(mesh, w), Yle)
+    t_lu = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+    N = mesh.m + mesh.ns + mesh.n
+    x = rng.normal(size=N) + 1j * rng.normal(size=N)
+    t0 = time.perf_counter()
+    for _ in range(sample_matvecs):
+        x = op.matvec(x)
+        x /= np.linalg.norm(x)
+    t_mv = (time.perf_counter() - t0) / sample_matvecs
+    per_freq = t_lu + gpu_iters * t_mv
+    return dict(value=1.0 / per_freq, unit="freq-point solves/s", cores=1, kind="port",
+                sample="oracle on %s: one SS assembly + LU (%.2f s) and %d preconditioned FFT matvecs "
+                       "(%.3f s each), extrapolated to %.1f matvecs per frequency point (the GPU run's GMRES "
+                       "average); one-time integrals + spectra %.1f s not included"
+                       % (cfg["name"], t_lu, sample_matvecs, t_mv, gpu_iters, t_setup))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="two_material_64x64x16")
+    ap.add_argument("--inner", type=int, default=400)
+    ap.add_argument("--cpu-matvecs", type=int, default=12)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the sweep shards "
+              "by frequency across GPUs (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from solver_amd import configs, sweep
+    cfg = configs.CONFIGS[args.workload]()
+    t0 = time.perf_counter()
+    mesh = build_mesh(cfg, gpu_p_self)
+    t_mesh = time.perf_counter() - t0
+    log("mesh %s: n=%d m=%d ns=%d N=%d ports=%d (%.1f s)" % (cfg["name"], mesh.n, mesh.m, mesh.ns,
+                                                              mesh.m + mesh.ns + mesh.n, len(mesh.port_nodes), t_mesh))
+    t0 = time.perf_counter()
+    solver = sweep.Solver(mesh)
+    t_setup = time.perf_counter() - t0
+    log("setup (integrals, spectra, operator upload): %.2f s" % t_setup)
+
+    esc = mesh.escalings
+    freqs = np.asarray(cfg["freqs"]) * esc["freq"]
+    P = mesh.port_nodes.shape[0]
+    N = mesh.m + mesh.ns + mesh.n
+    Vrest = np.zeros((N, P), dtype=np.complex128)
+    from solver_amd import peec
+    iters = []
+
+    def step(k, record):
+        w = 2 * np.pi * freqs[k % len(freqs)]
+        solver.set_frequency(w)
+        is_ = np.zeros(mesh.n)
+        for c1 in range(P):
+            n1, n2 = mesh.port_nodes[c1]
+            is_[:] = 0
+            is_[n1] = esc["Is"]
+            is_[n2] = -esc["Is"]
+            tn = peec.precond_3_3_Kt_fft(is_, ctx=solver.ctx)
+            V, flag, relres, it, resvec = peec.gmres_custom(tn, False, cfg["tol"], args.inner, Vrest[:, c1],
+                                                            ctx=solver.ctx)
+            Vrest[:, c1] = V
+            if record:
+                iters.append(len(resvec) - 1)
+                log("  f=%.3e Hz port %d flag %d iter %s relres %.2e" % (freqs[k % len(freqs)] / esc["freq"], c1,
+                                                                       flag, it, relres))
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+
+    for k in range(args.warmup):
+        step(k, False)
+    profile_read(solver.ctx, True)
+    sync()
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        step(k, True)
+        log("step %d done" % k)
+    sync()
+    dt = time.perf_counter() - t0
+    prof = profile_read(solver.ctx, False)
+    value = args.steps / dt
+    dom = max(prof, key=lambda r: r["ms"])
+    ach = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else 0.0
+    for r in prof:
+        log("  kernel %-18s launches %6d  time %8.2f ms  bytes %.3e  %.1f GB/s" % (
+            r["name"], r["count"], r["ms"], r["bytes"], r["bytes"] / max(r["ms"], 1e-9) / 1e6))
+    gpu_ms = sum(r["ms"] for r in prof)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        avg_iters = float(np.mean(iters)) * P if iters else 50.0
+        cpu = cpu_baseline(mesh, cfg, avg_iters + 2 * P, args.cpu_matvecs)
+
+    out = {
+        "metric": "freq-point solves/sec at 1e-6 rel. impedance err; FFT-matvec HBM GB/s",
+        "value": value,
+        "unit": "freq-point solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (seeded geometry, %s)" % cfg["name"],
+        "config": {"workload": cfg["name"], "grid": list(cfg["grid"].shape), "unknowns": int(N),
+                   "ports": int(P), "gmres_tol": cfg["tol"], "path": "plain FP64",
+                   "setup_s": round(t_setup, 3), "gpu_kernel_ms_per_step": gpu_ms / args.steps},
+        "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": ach / PEAK_HBM_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

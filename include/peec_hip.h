@@ -144,6 +144,15 @@ int peec_precond_rhs(peec_ctx *ctx, const double *is, peec_c128 *tn);
 int peec_gmres(peec_ctx *ctx, const peec_c128 *b, double tol, int maxit, peec_c128 *x,
                int *flag, double *relres, int64_t iter[2], double *resvec, int64_t *nres);
 
+/* Kernel timing (no counterpart in the reference, which only prints @time,
+ * do_solving_fft.jl:133).  Reads and clears the per-kind totals recorded
+ * since the previous call -- kernel milliseconds from HIP events on the
+ * context's stream, algorithmic bytes and launch counts -- and switches
+ * recording on (enable = 1) or off.  Kinds: 0..4 the five FFT-convolution
+ * passes (x fwd + scatter, y fwd, z fwd * spectrum * z inv, y inv, x inv +
+ * gather), 5 triangular solves. */
+int peec_profile(peec_ctx *ctx, int enable, double *ms, double *bytes, int64_t *count, int nkinds);
+
 #ifdef __cplusplus
 }
 #endif

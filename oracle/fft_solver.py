@@ -86,6 +86,17 @@ def conv(spec, vec, n):
 
 
 # ---------------------------------------------------------------- operator
+def lu(SS):
+    """Sparse LU of SS (UMFPACK's role in the reference, FFT_solver_QS_S_type.jl:119).
+
+    SS is ill-conditioned at low frequency (cond ~ 1e9 at 10 MHz for the test
+    geometries), so two different exact factorizations already disagree at
+    the cond * eps level.  The oracle therefore uses the ordering and pivoting
+    the GPU path's host factorization uses: minimum degree on A'+A, no
+    pivoting."""
+    return spla.splu(SS, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0, options=dict(SymmetricMode=True))
+
+
 class Operator:
     """Everything ComputeMatrixVectorFTT needs for one geometry + frequency."""
 
@@ -104,7 +115,7 @@ class Operator:
         self.invZ = sp.diags(1.0 / self.DZ)
         SS = self.Yle + self.A.T @ (self.invZ @ self.A) + 1j * w * (self.Gamma @ self.invP) @ self.Gamma.T
         self.SS = SS.tocsc()
-        self.F = F if F is not None else spla.splu(self.SS)
+        self.F = F if F is not None else lu(self.SS)
 
     def solve(self, b):
         return self.F.solve(np.asarray(b, dtype=np.complex128))

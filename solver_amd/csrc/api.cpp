@@ -82,8 +82,67 @@ void *peec_ctx::scratch(size_t bytes, int slot)
     return scratch_ptr[slot];
 }
 
+int peec_ctx::prof_begin(int kind, double bytes)
+{
+    if (!prof_on) return -1;
+    ProfRec r;
+    r.kind = kind;
+    r.bytes = bytes;
+    for (hipEvent_t *e : {&r.a, &r.b}) {
+        if (!event_pool.empty()) {
+            *e = event_pool.back();
+            event_pool.pop_back();
+        } else if (hipEventCreate(e) != hipSuccess) {
+            return -1;
+        }
+    }
+    (void)hipEventRecord(r.a, stream);
+    prof.push_back(r);
+    return (int)prof.size() - 1;
+}
+
+void peec_ctx::prof_end(int idx)
+{
+    if (idx < 0) return;
+    (void)hipEventRecord(prof[(size_t)idx].b, stream);
+}
+
+extern "C" int peec_profile(peec_ctx *ctx, int enable, double *ms, double *bytes, int64_t *count, int nkinds)
+{
+    if (!ctx) {
+        peec_set_error("peec_profile: null context");
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < nkinds; k++) {
+        if (ms) ms[k] = 0.0;
+        if (bytes) bytes[k] = 0.0;
+        if (count) count[k] = 0;
+    }
+    for (auto &r : ctx->prof) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess && r.kind < nkinds) {
+            if (ms) ms[r.kind] += t;
+            if (bytes) bytes[r.kind] += r.bytes;
+            if (count) count[r.kind] += 1;
+        }
+        ctx->event_pool.push_back(r.a);
+        ctx->event_pool.push_back(r.b);
+    }
+    ctx->prof.clear();
+    ctx->prof_on = enable != 0;
+    return PEEC_OK;
+}
+
 void peec_ctx::release_all()
 {
+    for (auto &r : prof) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    prof.clear();
+    for (auto e : event_pool) (void)hipEventDestroy(e);
+    event_pool.clear();
     for (int i = 0; i < kScratchSlots; i++) {
         if (scratch_ptr[i]) (void)hipFree(scratch_ptr[i]);
         scratch_ptr[i] = nullptr;

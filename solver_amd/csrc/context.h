@@ -85,6 +85,16 @@ struct TriFactor {
     DevBuf<double2_t> diag;    // diagonal per row (1 for unit)
 };
 
+// Per-kernel timing with HIP events on the context's stream.
+struct ProfRec {
+    int kind;
+    double bytes;
+    hipEvent_t a, b;
+};
+enum ProfKind {
+    PK_FFT_X_FWD = 0, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV, PK_TRSV, PK_SPMV, PK_COUNT
+};
+
 struct peec_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -135,4 +145,11 @@ struct peec_ctx {
 
     // ---- vectors
     DevBuf<double2_t> vx, vy, vt1, vt2, vt3;  // N-sized work vectors
+
+    // ---- profiling
+    bool prof_on = false;
+    std::vector<ProfRec> prof;
+    std::vector<hipEvent_t> event_pool;
+    int prof_begin(int kind, double bytes);
+    void prof_end(int idx);
 };

@@ -587,14 +587,23 @@ int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *ve
     d.nrhs = nrhs;
     d.in_rhs = (int64_t)nx * ny * nz;  // only used to strip the rhs offset from map positions
     d.out_rhs = s1;
+    const double e = fp32 ? 8.0 : 16.0, es = fp32 ? 4.0 : 8.0;
+    const double g = (double)nx * ny * nz, G1 = (double)s1 * nrhs, G2 = (double)s2 * nrhs;
+    // algorithmic bytes: every pass reads its input once and writes its
+    // output once; the scatter/gather passes also read the 0/1 position map
+    // and the unknowns it selects (the gather read-modify-writes them)
+    int pi = ctx->prof_begin(PK_FFT_X_FWD, 4.0 * g + 16.0 * app.nnz_in * nrhs + e * G1);
     if ((rc = launch_pass(ctx, fp32, 0, d, nullptr, w1))) return rc;
+    ctx->prof_end(pi);
     // 2: y forward; lines (x', z)
     PassGeom g2 = {(int)Ly, ny, (int)Ly, (int)Lx, nz, 1, Lx * ny, Lx, 1, Lx * Ly, Lx};
     if ((rc = make_desc(ctx, g2, &d))) return rc;
     d.nrhs = nrhs;
     d.in_rhs = s1;
     d.out_rhs = s2;
+    pi = ctx->prof_begin(PK_FFT_Y_FWD, e * (G1 + G2));
     if ((rc = launch_pass(ctx, fp32, 0, d, w1, w2))) return rc;
+    ctx->prof_end(pi);
     // 3: z forward * spectrum * z inverse, in place; lines (x', y')
     PassGeom g3 = {(int)Lz, nz, nz, (int)Lx, (int)Ly, 1, Lx, Lx * Ly, 1, Lx, Lx * Ly};
     if ((rc = make_desc(ctx, g3, &d))) return rc;
@@ -605,14 +614,18 @@ int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *ve
     d.spec_sb = 1;
     d.spec_s2 = Lx;
     d.spec_sj = Lx * Ly;
+    pi = ctx->prof_begin(PK_FFT_Z_CONV, 2.0 * e * G2 + es * (double)K.embedded());
     if ((rc = launch_pass(ctx, fp32, 2, d, w2, w2))) return rc;
+    ctx->prof_end(pi);
     // 4: y inverse, keep ny rows; lines (x', z)
     PassGeom g4 = {(int)Ly, (int)Ly, ny, (int)Lx, nz, 1, Lx * Ly, Lx, 1, Lx * ny, Lx};
     if ((rc = make_desc(ctx, g4, &d))) return rc;
     d.nrhs = nrhs;
     d.in_rhs = s2;
     d.out_rhs = s1;
+    pi = ctx->prof_begin(PK_FFT_Y_INV, e * (G1 + G2));
     if ((rc = launch_pass(ctx, fp32, 1, d, w2, w1))) return rc;
+    ctx->prof_end(pi);
     // 5: x inverse with gather; lines (y, z)
     PassGeom g5 = {(int)Lx, (int)Lx, nx, ny, nz, Lx, Lx * ny, 1, nx, (int64_t)nx * ny, 1};
     if ((rc = make_desc(ctx, g5, &d))) return rc;
@@ -622,7 +635,9 @@ int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *ve
     d.map_out = app.map_out.p;
     d.vec_out = vec_out + app.out_off;
     d.vec_out_ld = ld;
+    pi = ctx->prof_begin(PK_FFT_X_INV, e * G1 + 4.0 * g + 32.0 * app.nnz_out * nrhs);
     if ((rc = launch_pass(ctx, fp32, 1, d, w1, nullptr))) return rc;
+    ctx->prof_end(pi);
     return PEEC_OK;
 }
 

@@ -578,9 +578,12 @@ static int trsv(peec_ctx *ctx, const TriFactor &f, const double2_t *b, double2_t
         ctx->generation = 0;
     }
     uint32_t gen = ++ctx->generation;
+    // bytes: the factor's indices and values, the right-hand side and the result
+    int pi = ctx->prof_begin(PK_TRSV, (double)f.idx.n * 4.0 + (double)f.val.n * 16.0 + (double)f.n * (24.0 + 32.0 * nrhs));
     hipLaunchKernelGGL(k_trsv, dim3(nblk(f.n, 4)), dim3(256), 0, ctx->stream, f.order.p, f.ptr.p, f.idx.p, f.val.p,
                        f.diag.p, b, x, ctx->flags.p, gen, f.n, nrhs, ctx->err.p);
     PEEC_HIP_TRY(hipGetLastError());
+    ctx->prof_end(pi);
     return PEEC_OK;
 }
 
