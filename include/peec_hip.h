@@ -121,9 +121,18 @@ int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, const peec_
  *     L * U = (Rs .* SS)[p, q]
  * L lower and U upper triangular (n x n CSC, diagonals stored), p and q
  * 0-based permutations, Rs the row scaling (NULL = no scaling).  The
- * triangular solves run on the GPU (sync-free, one wavefront per row). */
+ * triangular solves run on the GPU. */
 int peec_lu_set_factors(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L, const peec_csc_c128 *U,
                         const int64_t *p, const int64_t *q, const double *Rs);
+/* Same, with the factor order cut into nblocks contiguous row blocks
+ * (block_ptr[0] = 0 ... block_ptr[nblocks] = n, at most 4096 rows each),
+ * e.g. the leaves and separators of a nested-dissection ordering.  Blocks
+ * are solved one workgroup each, all blocks of a dependency level in one
+ * launch.  peec_lu_set_factors derives the blocks from chains of the
+ * elimination tree of L. */
+int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L, const peec_csc_c128 *U,
+                                const int64_t *p, const int64_t *q, const double *Rs, int64_t nblocks,
+                                const int64_t *block_ptr);
 
 /* ComputeMatrixVectorFTT(x, w, ...) including the precond_3_3_vector step
  * (compute_Matrix_vector_fft.jl:1-146): y = M^-1 K x, x and y of length

@@ -86,14 +86,31 @@ def conv(spec, vec, n):
 
 
 # ---------------------------------------------------------------- operator
-def lu(SS):
+class _OrderedLU:
+    """F.solve(b) for F the LU of SS[order, order]."""
+
+    def __init__(self, SS, order):
+        self.order = np.asarray(order)
+        self.F = spla.splu(SS[self.order][:, self.order].tocsc(), permc_spec="NATURAL", diag_pivot_thresh=0.0,
+                           options=dict(SymmetricMode=True))
+
+    def solve(self, b):
+        x = np.empty_like(b)
+        x[self.order] = self.F.solve(b[self.order])
+        return x
+
+
+def lu(SS, order=None):
     """Sparse LU of SS (UMFPACK's role in the reference, FFT_solver_QS_S_type.jl:119).
 
     SS is ill-conditioned at low frequency (cond ~ 1e9 at 10 MHz for the test
     geometries), so two different exact factorizations already disagree at
-    the cond * eps level.  The oracle therefore uses the ordering and pivoting
-    the GPU path's host factorization uses: minimum degree on A'+A, no
-    pivoting."""
+    the cond * eps level.  The oracle therefore takes the elimination order
+    as an input and factors without pivoting, like the GPU path's host
+    factorization: minimum degree on A'+A when no order is given, else
+    SS[order, order] as it stands."""
+    if order is not None:
+        return _OrderedLU(SS, order)
     return spla.splu(SS, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0, options=dict(SymmetricMode=True))
 
 
@@ -108,14 +125,14 @@ class Operator:
         self.Gamma = mesh.Gamma.tocsr()
         self.invP = sp.diags(1.0 / mesh.diag_P)
 
-    def set_frequency(self, w, DZ, Yle, F=None):
+    def set_frequency(self, w, DZ, Yle, F=None, order=None):
         self.w = w
         self.DZ = np.asarray(DZ).reshape(-1)
         self.Yle = Yle.tocsr()
         self.invZ = sp.diags(1.0 / self.DZ)
         SS = self.Yle + self.A.T @ (self.invZ @ self.A) + 1j * w * (self.Gamma @ self.invP) @ self.Gamma.T
         self.SS = SS.tocsc()
-        self.F = F if F is not None else lu(self.SS)
+        self.F = F if F is not None else lu(self.SS, order)
 
     def solve(self, b):
         return self.F.solve(np.asarray(b, dtype=np.complex128))
@@ -492,8 +509,9 @@ def s2y(S, Zo):
     return Y
 
 
-def fft_solver_qs(freq, mesh, spec, inner_iter, tol, R0=50.0, stop=lambda: False, record=None):
-    """FFT_solver_QS_S_type.jl:1-234 (QS_Rcc_FW = 1)."""
+def fft_solver_qs(freq, mesh, spec, inner_iter, tol, R0=50.0, stop=lambda: False, record=None, order=None):
+    """FFT_solver_QS_S_type.jl:1-234 (QS_Rcc_FW = 1).  `order`: elimination
+    order for the LU of SS (see lu)."""
     esc = mesh.escalings
     freq = np.asarray(freq, dtype=np.float64) * esc["freq"]
     tol = np.broadcast_to(np.asarray(tol, dtype=np.float64), freq.shape)
@@ -509,7 +527,7 @@ def fft_solver_qs(freq, mesh, spec, inner_iter, tol, R0=50.0, stop=lambda: False
             return None
         Yle = build_Yle(mesh.lumped_elements, mesh.port_nodes, esc["Yle"], n, w[k] / esc["freq"], R0)
         DZ = frequency_terms(mesh, w[k])
-        op.set_frequency(w[k], DZ, Yle)
+        op.set_frequency(w[k], DZ, Yle, order=order)
         for c1 in range(P):
             n1, n2 = mesh.port_nodes[c1]
             is_[n1] = esc["Is"]

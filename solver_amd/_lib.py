@@ -58,6 +58,8 @@ def lib():
                                         ctypes.POINTER(CscF64), ctypes.POINTER(CscF64), dp]
         L.peec_frequency_set.argtypes = [vp, d, c128p, ctypes.POINTER(CscC128)]
         L.peec_lu_set_factors.argtypes = [vp, i64, ctypes.POINTER(CscC128), ctypes.POINTER(CscC128), i64p, i64p, dp]
+        L.peec_lu_set_factors_blocked.argtypes = [vp, i64, ctypes.POINTER(CscC128), ctypes.POINTER(CscC128), i64p,
+                                                  i64p, dp, i64, i64p]
         L.peec_matvec.argtypes = [vp, c128p, c128p]
         L.peec_apply_K.argtypes = [vp, c128p, c128p]
         L.peec_precond_rhs.argtypes = [vp, dp, c128p]

@@ -79,10 +79,15 @@ struct CsrC {
 // each row's off-diagonal entries plus its diagonal.
 struct TriFactor {
     int64_t n = 0;
-    DevBuf<int32_t> order;     // row solved by wave w
-    DevBuf<int32_t> ptr, idx;  // off-diagonal entries per row (by row id)
-    DevBuf<double2_t> val;
-    DevBuf<double2_t> diag;    // diagonal per row (1 for unit)
+    int nblocks = 0, max_block = 0;
+    double nnz = 0.0;
+    std::vector<int32_t> level_ptr;     // host: blocks of level l are level_blocks[level_ptr[l] .. level_ptr[l+1])
+    DevBuf<int32_t> blk_ptr, level_blocks;
+    DevBuf<int32_t> ob_ptr, ob_idx;     // per row: entries reaching into other blocks
+    DevBuf<double2_t> ob_val;
+    DevBuf<int32_t> ib_ptr, ib_idx;     // per column: entries inside its own block
+    DevBuf<double2_t> ib_val;
+    DevBuf<double2_t> diag;
 };
 
 // Per-kernel timing with HIP events on the context's stream.
@@ -139,9 +144,6 @@ struct peec_ctx {
     DevBuf<int32_t> perm_r, perm_c;  // inverse row / column permutations (0-based)
     DevBuf<double> Rs;               // optional row scaling of SS
     bool has_Rs = false;
-    DevBuf<uint32_t> flags;          // completion flags (generation stamped)
-    DevBuf<int> err;                 // set when a dependency wait hits its spin limit
-    uint32_t generation = 0;
 
     // ---- vectors
     DevBuf<double2_t> vx, vy, vt1, vt2, vt3;  // N-sized work vectors

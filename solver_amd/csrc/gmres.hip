@@ -20,7 +20,7 @@
 typedef std::complex<double> zc;
 
 int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32);
-int check_trsv_error(peec_ctx *ctx);
+int sync_stream(peec_ctx *ctx);
 
 namespace {
 
@@ -281,7 +281,7 @@ extern "C" int peec_gmres(peec_ctx *ctx, const peec_c128 *b_h, double tol, int m
     hipLaunchKernelGGL(k_sub, dim3(nb256(n)), dim3(256), 0, ctx->stream, W.r.p, W.b.p, W.v.p, n);
     double normr;
     if ((rc = W.norm(W.r.p, 0, &normr))) return rc;
-    if ((rc = check_trsv_error(ctx))) return rc;
+    if ((rc = sync_stream(ctx))) return rc;
     if (normr <= tolb) {
         // :50-57
         if ((rc = finish_copy_x(W.xmin.p))) return rc;
@@ -580,7 +580,7 @@ extern "C" int peec_gmres(peec_ctx *ctx, const peec_c128 *b_h, double tol, int m
         iter1 = jmin;
         relres = normr_act / n2minv_b;
     }
-    if ((rc = check_trsv_error(ctx))) return rc;
+    if ((rc = sync_stream(ctx))) return rc;
     if ((rc = finish_copy_x(xout))) return rc;
     int64_t nres = (int64_t)std::max(outitercount - 1, 0) * inner + initercount + 1;
     for (int64_t i = 0; i < nres; i++) resvec_out[i] = resvec[(size_t)i];

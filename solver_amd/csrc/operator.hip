@@ -12,16 +12,21 @@
 // gives the same vector; then
 //   Y1 = Z^-1 (X1 - A M),  Y2 = P^-1 (X2 + Gamma' M),  Y3 = M.
 //
-// The LU factors of SS come from the host; the two triangular solves run
-// sync-free: one wavefront per row in dependency order, each row waiting on
-// the completion flags of the rows it reads (bounded spin, error flag on
-// timeout).  Rows are dispatched in blockIdx order and only wait on rows
-// with smaller blockIdx, which the in-order dispatch has already placed on
-// the GPU, so every wait ends.
+// The LU factors of SS come from the host.  The triangular solves are
+// blocked: the factor order is cut into contiguous row blocks (the
+// separators and leaves of a nested-dissection ordering when the host
+// provides them), block levels follow the block dependency graph, and one
+// launch per level solves all its blocks in parallel, one workgroup per
+// block.  A workgroup first gathers the contributions of earlier blocks
+// (one wavefront per row), then runs the in-block substitution in LDS,
+// column by column with one barrier per column.  Kernel boundaries order
+// the levels, so no cross-workgroup waiting is needed.
 #include "context.h"
 
 #include <algorithm>
 #include <cstring>
+
+constexpr int kMaxBlock = 4096;  // rows x rhs of one LDS tile (64 KiB)
 
 void peec_ctx::release_operator()
 {
@@ -164,51 +169,60 @@ __global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ, 
     }
 }
 
-// Sync-free triangular solve: wave w solves row order[w]:
-//   x[i] = (b[i] - sum_j T[i,j] x[j]) / diag[i]
-// waiting on flag[j] == gen for every j it reads.  nrhs vectors at stride n.
-constexpr int kSpinLimit = 1 << 24;
-
-__global__ void __launch_bounds__(256) k_trsv(const int32_t *order, const int32_t *ptr, const int32_t *idx,
-                                              const cd *val, const cd *diag, const cd *b, cd *x, uint32_t *flag,
-                                              uint32_t gen, int64_t n, int nrhs, int *err)
+// One level of a blocked triangular solve.  LOWER: forward substitution
+// with L (rows of a block depend on earlier blocks through ob_* entries
+// and on earlier rows of the block through the column lists ib_*);
+// otherwise backward substitution with U.  nrhs vectors at stride n,
+// processed rc at a time (s * rc complex values fit the LDS tile).
+template <bool LOWER>
+__global__ void __launch_bounds__(256) k_block_trsv(const int32_t *level_blocks, const int32_t *blk_ptr,
+                                                    const int32_t *ob_ptr, const int32_t *ob_idx, const cd *ob_val,
+                                                    const int32_t *ib_ptr, const int32_t *ib_idx, const cd *ib_val,
+                                                    const cd *diag, const cd *b, cd *x, int64_t n, int nrhs, int rcmax)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= n) return;
-    const int i = order[w];
-    const int k0 = ptr[i], k1 = ptr[i + 1];
-    for (int r = 0; r < nrhs; r++) {
-        cd acc = {0.0, 0.0};
-        for (int k = k0 + lane; k < k1; k += 64) {
-            int j = idx[k];
-            if (r == 0) {
-                int spins = 0;
-                while (__hip_atomic_load(&flag[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen) {
-                    if (++spins > kSpinLimit) {
-                        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
+    extern __shared__ char smem_raw[];
+    cd *acc = reinterpret_cast<cd *>(smem_raw);
+    const int B = level_blocks[blockIdx.x];
+    const int r0 = blk_ptr[B], s = blk_ptr[B + 1] - r0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int rb = 0; rb < nrhs; rb += rcmax) {
+        const int rc = min(rcmax, nrhs - rb);
+        // gather: acc[t][r] = b - sum over earlier blocks
+        for (int t = wave; t < s; t += 4) {
+            const int row = r0 + t;
+            for (int r = 0; r < rc; r++) {
+                const int64_t off = (int64_t)(rb + r) * n;
+                cd a = {0.0, 0.0};
+                for (int k = ob_ptr[row] + lane; k < ob_ptr[row + 1]; k += 64) a = cadd(a, cmul(ob_val[k], x[ob_idx[k] + off]));
+                for (int o = 32; o > 0; o >>= 1) {
+                    a.x += __shfl_xor(a.x, o, 64);
+                    a.y += __shfl_xor(a.y, o, 64);
                 }
+                if (lane == 0) acc[t * rc + r] = csub(b[row + off], a);
             }
-            cd xj;
-            xj.x = __hip_atomic_load(&x[j + r * n].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            xj.y = __hip_atomic_load(&x[j + r * n].y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            acc = cadd(acc, cmul(val[k], xj));
         }
-        // wave reduction
-        for (int off = 32; off > 0; off >>= 1) {
-            acc.x += __shfl_xor(acc.x, off, 64);
-            acc.y += __shfl_xor(acc.y, off, 64);
+        __syncthreads();
+        // in-block substitution, one column per step
+        for (int q = 0; q < s; q++) {
+            const int j = LOWER ? q : s - 1 - q;
+            const int col = r0 + j;
+            const int k0 = ib_ptr[col], k1 = ib_ptr[col + 1];
+            const int work = (k1 - k0) * rc;
+            const cd dj = diag[col];
+            for (int e = threadIdx.x; e < work; e += 256) {
+                const int k = k0 + e / rc, r = e % rc;
+                const cd y = cdiv(acc[j * rc + r], dj);
+                const int t = ib_idx[k] - r0;
+                acc[t * rc + r] = csub(acc[t * rc + r], cmul(ib_val[k], y));
+            }
+            __syncthreads();
         }
-        if (lane == 0) {
-            cd v = cdiv(csub(b[i + r * n], acc), diag[i]);
-            __hip_atomic_store(&x[i + r * n].x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&x[i + r * n].y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int e = threadIdx.x; e < s * rc; e += 256) {
+            const int t = e / rc, r = e % rc;
+            x[r0 + t + (int64_t)(rb + r) * n] = cdiv(acc[e], diag[r0 + t]);
         }
+        __syncthreads();
     }
-    if (lane == 0) __hip_atomic_store(&flag[i], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace
@@ -462,68 +476,138 @@ extern "C" int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, 
     return PEEC_OK;
 }
 
-// Split a CSC triangle into row-ordered off-diagonal CSR plus the diagonal.
-static int tri_from_csc(const peec_csc_c128 *T, bool lower, TriFactor *f, const char *what)
+// Split a CSC triangle into the blocked layout: per row the entries that
+// reach into other blocks (CSR, ob_*), per column the entries inside its own
+// block (CSC, ib_*), and the diagonal; then the block levels.
+static int tri_from_csc(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f,
+                        const char *what)
 {
     const int64_t n = T->ncol;
     if (T->nrow != n) {
         peec_set_error("lu_set_factors: %s is not square", what);
         return PEEC_ERR_ARG;
     }
+    const int nb = (int)blk.size() - 1;
+    std::vector<int32_t> bof((size_t)n);
+    for (int B = 0; B < nb; B++)
+        for (int i = blk[(size_t)B]; i < blk[(size_t)B + 1]; i++) bof[(size_t)i] = B;
     std::vector<double2_t> diag((size_t)n, {0.0, 0.0});
     std::vector<char> has((size_t)n, 0);
-    std::vector<int32_t> ptr((size_t)n + 1, 0);
+    std::vector<int32_t> optr((size_t)n + 1, 0), iptr((size_t)n + 1, 0);
     for (int64_t c = 0; c < n; c++)
         for (int64_t k = T->colptr[c] - 1; k < T->colptr[c + 1] - 1; k++) {
             int64_t r = T->rowval[k] - 1;
             if (r == c) continue;
-            if ((lower && r < c) || (!lower && r > c)) {
+            bool tri_ok = lower ? r > c : r < c;
+            if (!tri_ok) {
                 if (T->nzval[k].re != 0.0 || T->nzval[k].im != 0.0) {
                     peec_set_error("lu_set_factors: %s is not %s triangular", what, lower ? "lower" : "upper");
                     return PEEC_ERR_ARG;
                 }
                 continue;
             }
-            ptr[(size_t)r + 1]++;
+            if (bof[(size_t)r] == bof[(size_t)c]) iptr[(size_t)c + 1]++;
+            else optr[(size_t)r + 1]++;
         }
-    for (int64_t i = 0; i < n; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
-    int64_t nnz = ptr[(size_t)n];
-    std::vector<int32_t> idx((size_t)std::max<int64_t>(nnz, 1)), pos(ptr.begin(), ptr.end() - 1);
-    std::vector<double2_t> val((size_t)std::max<int64_t>(nnz, 1));
+    for (int64_t i = 0; i < n; i++) {
+        optr[(size_t)i + 1] += optr[(size_t)i];
+        iptr[(size_t)i + 1] += iptr[(size_t)i];
+    }
+    std::vector<int32_t> oidx((size_t)std::max<int32_t>(optr[(size_t)n], 1)), iidx((size_t)std::max<int32_t>(iptr[(size_t)n], 1));
+    std::vector<double2_t> oval(oidx.size()), ival(iidx.size());
+    std::vector<int32_t> opos(optr.begin(), optr.end() - 1), ipos(iptr.begin(), iptr.end() - 1);
+    std::vector<int32_t> level((size_t)nb, 0);
     for (int64_t c = 0; c < n; c++)
         for (int64_t k = T->colptr[c] - 1; k < T->colptr[c + 1] - 1; k++) {
             int64_t r = T->rowval[k] - 1;
+            double2_t v = {T->nzval[k].re, T->nzval[k].im};
             if (r == c) {
-                diag[(size_t)c] = {T->nzval[k].re, T->nzval[k].im};
+                diag[(size_t)c] = v;
                 has[(size_t)c] = 1;
                 continue;
             }
-            if ((lower && r < c) || (!lower && r > c)) continue;
-            int32_t p = pos[(size_t)r]++;
-            idx[(size_t)p] = (int32_t)c;
-            val[(size_t)p] = {T->nzval[k].re, T->nzval[k].im};
+            if (lower ? r < c : r > c) continue;
+            if (bof[(size_t)r] == bof[(size_t)c]) {
+                int32_t p = ipos[(size_t)c]++;
+                iidx[(size_t)p] = (int32_t)r;
+                ival[(size_t)p] = v;
+            } else {
+                int32_t p = opos[(size_t)r]++;
+                oidx[(size_t)p] = (int32_t)c;
+                oval[(size_t)p] = v;
+            }
         }
     for (int64_t i = 0; i < n; i++)
         if (!has[(size_t)i] || (diag[(size_t)i].x == 0.0 && diag[(size_t)i].y == 0.0)) {
             peec_set_error("lu_set_factors: %s has a zero diagonal at %lld", what, (long long)i + 1);
             return PEEC_ERR_NUMERIC;
         }
-    std::vector<int32_t> order((size_t)n);
-    for (int64_t w = 0; w < n; w++) order[(size_t)w] = (int32_t)(lower ? w : n - 1 - w);
+    // levels: a block follows every block its rows read
+    for (int step = 0; step < nb; step++) {
+        const int B = lower ? step : nb - 1 - step;
+        int lv = 0;
+        for (int i = blk[(size_t)B]; i < blk[(size_t)B + 1]; i++)
+            for (int k = optr[(size_t)i]; k < optr[(size_t)i + 1]; k++)
+                lv = std::max(lv, level[(size_t)bof[(size_t)oidx[(size_t)k]]] + 1);
+        level[(size_t)B] = lv;
+    }
+    int nlev = 0;
+    for (int B = 0; B < nb; B++) nlev = std::max(nlev, level[(size_t)B] + 1);
+    std::vector<int32_t> lptr((size_t)nlev + 1, 0), lblk((size_t)nb);
+    for (int B = 0; B < nb; B++) lptr[(size_t)level[(size_t)B] + 1]++;
+    for (int l = 0; l < nlev; l++) lptr[(size_t)l + 1] += lptr[(size_t)l];
+    std::vector<int32_t> lpos(lptr.begin(), lptr.end() - 1);
+    for (int B = 0; B < nb; B++) lblk[(size_t)lpos[(size_t)level[(size_t)B]]++] = B;
     f->n = n;
-    if (!f->order.alloc((size_t)n) || !f->ptr.alloc(ptr.size()) || !f->idx.alloc(idx.size()) ||
-        !f->val.alloc(val.size()) || !f->diag.alloc((size_t)n))
+    f->nblocks = nb;
+    f->level_ptr = lptr;
+    f->max_block = 0;
+    for (int B = 0; B < nb; B++) f->max_block = std::max(f->max_block, blk[(size_t)B + 1] - blk[(size_t)B]);
+    if (!f->blk_ptr.alloc(blk.size()) || !f->level_blocks.alloc(lblk.size()) || !f->ob_ptr.alloc(optr.size()) ||
+        !f->ob_idx.alloc(oidx.size()) || !f->ob_val.alloc(oval.size()) || !f->ib_ptr.alloc(iptr.size()) ||
+        !f->ib_idx.alloc(iidx.size()) || !f->ib_val.alloc(ival.size()) || !f->diag.alloc((size_t)n))
         return PEEC_ERR_NOMEM;
-    PEEC_HIP_TRY(hipMemcpy(f->order.p, order.data(), (size_t)n * 4, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(f->ptr.p, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(f->idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(f->val.p, val.data(), val.size() * 16, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->blk_ptr.p, blk.data(), blk.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->level_blocks.p, lblk.data(), lblk.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->ob_ptr.p, optr.data(), optr.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->ob_idx.p, oidx.data(), oidx.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->ob_val.p, oval.data(), oval.size() * 16, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->ib_ptr.p, iptr.data(), iptr.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->ib_idx.p, iidx.data(), iidx.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->ib_val.p, ival.data(), ival.size() * 16, hipMemcpyHostToDevice));
     PEEC_HIP_TRY(hipMemcpy(f->diag.p, diag.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+    f->nnz = (double)oidx.size() + (double)iidx.size();
     return PEEC_OK;
+}
+
+// Blocks when the caller gives none: chains of the elimination tree of L
+// (column j + 1 is the parent of column j), capped in size.
+static std::vector<int32_t> chain_blocks(const peec_csc_c128 *L, int cap)
+{
+    const int64_t n = L->ncol;
+    std::vector<int32_t> blk{0};
+    for (int64_t j = 0; j < n; j++) {
+        int64_t parent = n;
+        for (int64_t k = L->colptr[j] - 1; k < L->colptr[j + 1] - 1; k++) {
+            int64_t r = L->rowval[k] - 1;
+            if (r > j && r < parent && (L->nzval[k].re != 0.0 || L->nzval[k].im != 0.0)) parent = r;
+        }
+        bool cont = parent == j + 1 && (j + 1 - blk.back()) < cap;
+        if (!cont) blk.push_back((int32_t)(j + 1));
+    }
+    if (blk.back() != n) blk.push_back((int32_t)n);
+    return blk;
 }
 
 extern "C" int peec_lu_set_factors(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L, const peec_csc_c128 *U,
                                    const int64_t *p, const int64_t *q, const double *Rs)
+{
+    return peec_lu_set_factors_blocked(ctx, n, L, U, p, q, Rs, 0, nullptr);
+}
+
+extern "C" int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L, const peec_csc_c128 *U,
+                                           const int64_t *p, const int64_t *q, const double *Rs, int64_t nblocks,
+                                           const int64_t *block_ptr)
 {
     if (!ctx || !L || !U || !p || !q) {
         peec_set_error("peec_lu_set_factors: null argument");
@@ -539,8 +623,35 @@ extern "C" int peec_lu_set_factors(peec_ctx *ctx, int64_t n, const peec_csc_c128
     }
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n > INT32_MAX) {
+        peec_set_error("peec_lu_set_factors: n exceeds the int32 index range");
+        return PEEC_ERR_ARG;
+    }
+    std::vector<int32_t> blk;
+    if (nblocks > 0 && block_ptr) {
+        if (block_ptr[0] != 0 || block_ptr[nblocks] != n) {
+            peec_set_error("peec_lu_set_factors: block_ptr must run from 0 to n");
+            return PEEC_ERR_ARG;
+        }
+        blk.resize((size_t)nblocks + 1);
+        for (int64_t B = 0; B <= nblocks; B++) {
+            if (B > 0 && block_ptr[B] <= block_ptr[B - 1]) {
+                peec_set_error("peec_lu_set_factors: empty or decreasing block");
+                return PEEC_ERR_ARG;
+            }
+            blk[(size_t)B] = (int32_t)block_ptr[B];
+        }
+    } else {
+        blk = chain_blocks(L, 1024);
+    }
+    int max_blk = 0;
+    for (size_t B = 0; B + 1 < blk.size(); B++) max_blk = std::max(max_blk, blk[B + 1] - blk[B]);
+    if (max_blk > kMaxBlock) {
+        peec_set_error("peec_lu_set_factors: block of %d rows exceeds the %d-row LDS tile", max_blk, kMaxBlock);
+        return PEEC_ERR_ARG;
+    }
     int rc;
-    if ((rc = tri_from_csc(L, true, &ctx->Lf, "L")) || (rc = tri_from_csc(U, false, &ctx->Uf, "U"))) return rc;
+    if ((rc = tri_from_csc(L, true, blk, &ctx->Lf, "L")) || (rc = tri_from_csc(U, false, blk, &ctx->Uf, "U"))) return rc;
     // pinv[p[i]] = i (scatter of the permuted rhs), qinv[q[j]] = j
     std::vector<int32_t> pinv((size_t)n, -1), qinv((size_t)n, -1);
     for (int64_t i = 0; i < n; i++) {
@@ -561,27 +672,30 @@ extern "C" int peec_lu_set_factors(peec_ctx *ctx, int64_t n, const peec_csc_c128
     } else {
         ctx->has_Rs = false;
     }
-    if (!ctx->flags.alloc((size_t)n) || !ctx->err.alloc(1)) return PEEC_ERR_NOMEM;
-    PEEC_HIP_TRY(hipMemsetAsync(ctx->flags.p, 0, (size_t)n * 4, ctx->stream));
-    PEEC_HIP_TRY(hipMemsetAsync(ctx->err.p, 0, 4, ctx->stream));
-    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->generation = 0;
     ctx->lu_ready = true;
     return PEEC_OK;
 }
 
 // ------------------------------------------------------------------ apply
-static int trsv(peec_ctx *ctx, const TriFactor &f, const double2_t *b, double2_t *x, int nrhs)
+static int trsv(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs)
 {
-    if (ctx->generation >= 0xFFFFFFF0u) {
-        PEEC_HIP_TRY(hipMemsetAsync(ctx->flags.p, 0, (size_t)f.n * 4, ctx->stream));
-        ctx->generation = 0;
+    // bytes: every factor entry (index + value) once, the diagonal, the
+    // right-hand sides read and the solutions written
+    int pi = ctx->prof_begin(PK_TRSV, f.nnz * 20.0 + (double)f.n * (16.0 + 32.0 * nrhs));
+    const int rcmax = std::max(1, std::min(nrhs, kMaxBlock / std::max(f.max_block, 1)));
+    const size_t lds = (size_t)f.max_block * rcmax * 16;
+    const int nlev = (int)f.level_ptr.size() - 1;
+    for (int l = 0; l < nlev; l++) {
+        // levels are numbered in solve order (forward for L, backward for U)
+        const int nbl = f.level_ptr[(size_t)l + 1] - f.level_ptr[(size_t)l];
+        const int32_t *lb = f.level_blocks.p + f.level_ptr[(size_t)l];
+        if (lower)
+            hipLaunchKernelGGL(k_block_trsv<true>, dim3(nbl), dim3(256), lds, ctx->stream, lb, f.blk_ptr.p, f.ob_ptr.p,
+                               f.ob_idx.p, f.ob_val.p, f.ib_ptr.p, f.ib_idx.p, f.ib_val.p, f.diag.p, b, x, f.n, nrhs, rcmax);
+        else
+            hipLaunchKernelGGL(k_block_trsv<false>, dim3(nbl), dim3(256), lds, ctx->stream, lb, f.blk_ptr.p, f.ob_ptr.p,
+                               f.ob_idx.p, f.ob_val.p, f.ib_ptr.p, f.ib_idx.p, f.ib_val.p, f.diag.p, b, x, f.n, nrhs, rcmax);
     }
-    uint32_t gen = ++ctx->generation;
-    // bytes: the factor's indices and values, the right-hand side and the result
-    int pi = ctx->prof_begin(PK_TRSV, (double)f.idx.n * 4.0 + (double)f.val.n * 16.0 + (double)f.n * (24.0 + 32.0 * nrhs));
-    hipLaunchKernelGGL(k_trsv, dim3(nblk(f.n, 4)), dim3(256), 0, ctx->stream, f.order.p, f.ptr.p, f.idx.p, f.val.p,
-                       f.diag.p, b, x, ctx->flags.p, gen, f.n, nrhs, ctx->err.p);
     PEEC_HIP_TRY(hipGetLastError());
     ctx->prof_end(pi);
     return PEEC_OK;
@@ -600,8 +714,8 @@ int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld)
                        ctx->has_Rs ? ctx->Rs.p : nullptr, b, ctx->w, m, ns, n, nrhs, ld);
     PEEC_HIP_TRY(hipGetLastError());
     int rc;
-    if ((rc = trsv(ctx, ctx->Lf, b, z, nrhs))) return rc;   // L c = b   (c in z)
-    if ((rc = trsv(ctx, ctx->Uf, z, b, nrhs))) return rc;   // U z' = c  (z' in b)
+    if ((rc = trsv(ctx, ctx->Lf, true, b, z, nrhs))) return rc;   // L c = b   (c in z)
+    if ((rc = trsv(ctx, ctx->Uf, false, z, b, nrhs))) return rc;  // U z' = c  (z' in b)
     hipLaunchKernelGGL(k_post, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, b, ctx->perm_c.p, ctx->invZ.p,
                        ctx->invP.p, ctx->A.ptr.p, ctx->A.idx.p, ctx->A.val.p, ctx->Gt.ptr.p, ctx->Gt.idx.p,
                        ctx->Gt.val.p, m, ns, n, nrhs, ld);
@@ -641,15 +755,9 @@ int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t l
     return PEEC_OK;
 }
 
-int check_trsv_error(peec_ctx *ctx)
+int sync_stream(peec_ctx *ctx)
 {
-    int e = 0;
-    PEEC_HIP_TRY(hipMemcpyAsync(&e, ctx->err.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (e) {
-        peec_set_error("triangular solve: a dependency wait exceeded its spin limit");
-        return PEEC_ERR_NUMERIC;
-    }
     return PEEC_OK;
 }
 
@@ -683,7 +791,7 @@ extern "C" int peec_matvec(peec_ctx *ctx, const peec_c128 *x, peec_c128 *y)
     PEEC_HIP_TRY(hipMemcpyAsync(ctx->vx.p, x, (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
     if ((rc = matvec_device(ctx, ctx->vx.p, ctx->vy.p, 1, N, false))) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(y, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
-    return check_trsv_error(ctx);
+    return sync_stream(ctx);
 }
 
 // K x only (no preconditioner); exported for parity tests of the operator
@@ -719,5 +827,5 @@ extern "C" int peec_precond_rhs(peec_ctx *ctx, const double *is, peec_c128 *tn)
     PEEC_HIP_TRY(hipMemcpyAsync(ctx->vy.p, h.data(), (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
     if ((rc = apply_precond(ctx, ctx->vy.p, 1, N))) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(tn, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
-    return check_trsv_error(ctx);
+    return sync_stream(ctx);
 }

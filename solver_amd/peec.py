@@ -180,17 +180,27 @@ def frequency_set(w, DZ, Yle, ctx=None):
     L.check(L.lib().peec_frequency_set(ctx.handle, float(w), L.as_c128p(dz), ctypes.byref(Y)), "peec_frequency_set")
 
 
-def lu_set_factors(Lf, Uf, p, q, Rs=None, ctx=None):
-    """F = lu(SS) in Julia's convention L*U = (Rs .* SS)[p, q] (0-based p, q)."""
+def lu_set_factors(Lf, Uf, p, q, Rs=None, ctx=None, blocks=None):
+    """F = lu(SS) in Julia's convention L*U = (Rs .* SS)[p, q] (0-based p, q).
+    blocks: optional block_ptr cutting the factor order into row blocks
+    for the triangular solves (peec_lu_set_factors_blocked)."""
     ctx = ctx or default_context()
     Ls, k1 = _csc_c128(Lf)
     Us, k2 = _csc_c128(Uf)
     pp = np.ascontiguousarray(p, dtype=np.int64)
     qq = np.ascontiguousarray(q, dtype=np.int64)
     rs = None if Rs is None else np.ascontiguousarray(Rs, dtype=np.float64)
-    L.check(L.lib().peec_lu_set_factors(ctx.handle, int(Lf.shape[0]), ctypes.byref(Ls), ctypes.byref(Us),
-                                        pp.ctypes.data_as(L.i64p), qq.ctypes.data_as(L.i64p),
-                                        None if rs is None else L.as_dp(rs)), "peec_lu_set_factors")
+    rsp = None if rs is None else L.as_dp(rs)
+    if blocks is None:
+        L.check(L.lib().peec_lu_set_factors(ctx.handle, int(Lf.shape[0]), ctypes.byref(Ls), ctypes.byref(Us),
+                                            pp.ctypes.data_as(L.i64p), qq.ctypes.data_as(L.i64p), rsp),
+                "peec_lu_set_factors")
+    else:
+        bp = np.ascontiguousarray(blocks, dtype=np.int64)
+        L.check(L.lib().peec_lu_set_factors_blocked(ctx.handle, int(Lf.shape[0]), ctypes.byref(Ls),
+                                                    ctypes.byref(Us), pp.ctypes.data_as(L.i64p),
+                                                    qq.ctypes.data_as(L.i64p), rsp, int(bp.size - 1),
+                                                    bp.ctypes.data_as(L.i64p)), "peec_lu_set_factors_blocked")
     del k1, k2
 
 

@@ -11,7 +11,7 @@ import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from . import peec
+from . import ordering, peec
 
 
 def lumped_admittance(typ, R, L, C, w):
@@ -103,22 +103,27 @@ def SS_of(mesh, w, DZ, Yle):
     return (Yle + A.T @ (invZ @ A) + 1j * w * (G @ invP) @ G.T).tocsc()
 
 
-def lu_factor(SS):
-    """Host LU of SS (the reference's lu(SS), :119) with SuperLU: minimum
-    degree on A'+A, no pivoting (SS is complex symmetric and diagonally
-    weighted), returned in Julia's convention L*U = SS[p, q]."""
-    F = spla.splu(SS, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0, options=dict(SymmetricMode=True))
-    p = np.argsort(F.perm_r)
-    q = np.argsort(F.perm_c)
-    return F.L, F.U, p, q
+def lu_factor(SS, order=None):
+    """Host LU of SS (the reference's lu(SS), :119) with SuperLU, no pivoting
+    (SS is complex symmetric and diagonally weighted), returned in Julia's
+    convention L*U = SS[p, q].  With `order` (a nested-dissection order,
+    ordering.mesh_order) SS[order, order] is factored as it stands;
+    otherwise SuperLU orders by minimum degree on A'+A."""
+    if order is None:
+        F = spla.splu(SS, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0, options=dict(SymmetricMode=True))
+        return F.L, F.U, np.argsort(F.perm_r), np.argsort(F.perm_c)
+    F = spla.splu(SS[order][:, order].tocsc(), permc_spec="NATURAL", diag_pivot_thresh=0.0,
+                  options=dict(SymmetricMode=True))
+    return F.L, F.U, order[np.argsort(F.perm_r)], order[np.argsort(F.perm_c)]
 
 
 class Solver:
     """Holds one geometry on one device context."""
 
-    def __init__(self, mesh, ctx=None):
+    def __init__(self, mesh, ctx=None, nested_dissection=True):
         self.mesh = mesh
         self.ctx = ctx or peec.default_context()
+        self.order, self.blocks = ordering.mesh_order(mesh) if nested_dissection else (None, None)
         ok = peec.compute_FFT_mutual_coupling_mats(mesh.origin, mesh.Nx, mesh.Ny, mesh.Nz, mesh.sx, mesh.sy,
                                                    mesh.sz, mesh.escalings, ctx=self.ctx)
         if not ok:
@@ -132,8 +137,12 @@ class Solver:
         Yle = build_Yle_FFT(m.lumped_elements, m.port_nodes, esc, m.n, w / esc["freq"], R0)
         DZ = DZ_of(m, w)
         peec.frequency_set(w, DZ, Yle, ctx=self.ctx)
-        Lf, Uf, p, q = lu_factor(SS_of(m, w, DZ, Yle))
-        peec.lu_set_factors(Lf, Uf, p, q, ctx=self.ctx)
+        Lf, Uf, p, q = lu_factor(SS_of(m, w, DZ, Yle), self.order)
+        # the blocks hold when SuperLU kept the order (no pivoting, no
+        # column postorder); otherwise the library finds its own
+        blocks = self.blocks if self.order is not None and np.array_equal(p, self.order) and \
+            np.array_equal(q, self.order) else None
+        peec.lu_set_factors(Lf, Uf, p, q, ctx=self.ctx, blocks=blocks)
 
     def matvec(self, x):
         return peec.ComputeMatrixVectorFTT(x, ctx=self.ctx)

@@ -74,16 +74,17 @@ def _setfreq(M, spec, S, f):
     S.set_frequency(w)
     op = OF.Operator(M, spec)
     Yle = OF.build_Yle(M.lumped_elements, M.port_nodes, M.escalings["Yle"], M.n, w / M.escalings["freq"], 50.0)
-    op.set_frequency(w, OF.frequency_terms(M, w), Yle)
+    op.set_frequency(w, OF.frequency_terms(M, w), Yle, order=S.order)
     return op
 
 
 @pytest.mark.parametrize("f", [1e7, 3e8, 5e9])
 def test_K_and_matvec_match_oracle(small, f):
     """K x to 1e-12.  M^-1 K x to 1e-12 or, where SS is ill-conditioned
-    (low frequency), to 100 * cond(SS) * eps: both sides factor SS the same
-    way, but the oracle runs the reference's three LU solves and the GPU one
-    combined solve with sync-free triangular sweeps, and the rounding of
+    (low frequency), to cond(SS) * eps, the forward-error scale of a
+    backward-stable solve: both sides factor SS in the same elimination
+    order, but the oracle runs the reference's three LU solves and the GPU
+    one combined solve with blocked triangular sweeps, and the rounding of
     either is amplified by cond(SS)."""
     from solver_amd import peec
     M, rows, spec, S = small
@@ -98,7 +99,7 @@ def test_K_and_matvec_match_oracle(small, f):
     y_gpu = peec.ComputeMatrixVectorFTT(x, ctx=S.ctx)
     y_ora = op.matvec(x)
     cond = np.linalg.cond(op.SS.toarray())
-    tol = max(1e-12, 100 * cond * np.finfo(float).eps)
+    tol = max(1e-12, cond * np.finfo(float).eps)
     print("M^-1 K", f, rel(y_gpu, y_ora), "cond(SS) %.2e tol %.1e" % (cond, tol), flush=True)
     assert rel(y_gpu, y_ora) < tol
 
@@ -135,18 +136,30 @@ def test_gmres_stop_flag(small):
 
 def test_crossbar_sweep_matches_oracle():
     """The 24x24x4 crossing-bar configuration, 10 frequencies 1e6..1e9 Hz,
-    tol 1e-8: port impedance to 1e-12 relative, same GMRES iterations."""
-    from solver_amd import sweep
+    tol 1e-8: same GMRES flags and iteration counts, and port impedance to
+    1e-12 relative or, at the low frequencies where SS is ill-conditioned,
+    to the same cond(SS) * eps bound as the preconditioned matvec (the
+    impedance inherits the preconditioner's rounding)."""
+    from solver_amd import ordering, sweep
     M, cfg = crossbar_mesh()
     rows = OF.kernel_rows(M.origin, M.Nx, M.Ny, M.Nz, M.sx, M.sy, M.sz)
     spec = OF.spectra(rows, M.Nx, M.Ny, M.Nz)
     rec_o, rec_g = [], []
-    out_o = OF.fft_solver_qs(cfg["freqs"], M, spec, 300, cfg["tol"], record=rec_o)
+    out_o = OF.fft_solver_qs(cfg["freqs"], M, spec, 300, cfg["tol"], record=rec_o,
+                             order=ordering.mesh_order(M)[0])
     out_g = sweep.FFT_solver_QS_S_type(cfg["freqs"], M, 300, cfg["tol"], record=rec_g)
     for a, b in zip(rec_g, rec_o):
         assert a["flag"] == b["flag"] and a["iter"] == b["iter"], (a, b)
     Zg, Zo = out_g["Z"], out_o["Z"]
-    err = np.max(np.abs(Zg - Zo) / np.abs(Zo))
-    print("crossbar Z max rel err", err, flush=True)
-    assert err < 1e-12
-    assert np.allclose(out_g["Y"], out_o["Y"], rtol=1e-11, atol=0)
+    for k, f in enumerate(cfg["freqs"]):
+        w = 2 * np.pi * f * M.escalings["freq"]
+        Yle = OF.build_Yle(M.lumped_elements, M.port_nodes, M.escalings["Yle"], M.n, f, 50.0)
+        op = OF.Operator(M, spec)
+        op.set_frequency(w, OF.frequency_terms(M, w), Yle, F=False)
+        cond = np.linalg.cond(op.SS.toarray())
+        tol = max(1e-12, cond * np.finfo(float).eps)
+        err = np.max(np.abs(Zg[..., k] - Zo[..., k]) / np.abs(Zo[..., k]))
+        print("crossbar f %.3e Z rel err %.2e cond(SS) %.2e tol %.1e" % (f, err, cond, tol), flush=True)
+        assert err < tol
+        erry = np.max(np.abs(out_g["Y"][..., k] - out_o["Y"][..., k]) / np.abs(out_o["Y"][..., k]))
+        assert erry < 10 * tol
