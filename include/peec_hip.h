@@ -125,11 +125,12 @@ int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, const peec_
 int peec_lu_set_factors(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L, const peec_csc_c128 *U,
                         const int64_t *p, const int64_t *q, const double *Rs);
 /* Same, with the factor order cut into nblocks contiguous row blocks
- * (block_ptr[0] = 0 ... block_ptr[nblocks] = n, at most 4096 rows each),
- * e.g. the leaves and separators of a nested-dissection ordering.  Blocks
- * are solved one workgroup each, all blocks of a dependency level in one
- * launch.  peec_lu_set_factors derives the blocks from chains of the
- * elimination tree of L. */
+ * (block_ptr[0] = 0 ... block_ptr[nblocks] = n, at most 64 rows each),
+ * e.g. the leaves and separator chunks of a nested-dissection ordering.
+ * All blocks of one dependency level are solved together, one workgroup
+ * per block.  No counterpart in the reference (UMFPACK solves on the host);
+ * peec_lu_set_factors derives the blocks from chains of the elimination
+ * tree of L instead. */
 int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L, const peec_csc_c128 *U,
                                 const int64_t *p, const int64_t *q, const double *Rs, int64_t nblocks,
                                 const int64_t *block_ptr);
@@ -159,7 +160,8 @@ int peec_gmres(peec_ctx *ctx, const peec_c128 *b, double tol, int maxit, peec_c1
  * context's stream, algorithmic bytes and launch counts -- and switches
  * recording on (enable = 1) or off.  Kinds: 0..4 the five FFT-convolution
  * passes (x fwd + scatter, y fwd, z fwd * spectrum * z inv, y inv, x inv +
- * gather), 5 triangular solves. */
+ * gather), 5 triangular-solve gathers, 6 triangular-solve block
+ * substitutions (one record per launch). */
 int peec_profile(peec_ctx *ctx, int enable, double *ms, double *bytes, int64_t *count, int nkinds);
 
 #ifdef __cplusplus

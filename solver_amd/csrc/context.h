@@ -77,17 +77,30 @@ struct CsrC {
 
 // Triangular factor prepared for the sync-free solve: rows in solve order,
 // each row's off-diagonal entries plus its diagonal.
+constexpr int kTrsvMaxBlock = 64;   // rows of one block (one per lane)
+constexpr int kTrsvSeg = 256;       // entries summed by one wavefront (4 per lane)
+
+// One triangular factor laid out for the blocked solve (trsv.hip).  The
+// factor order is cut into row blocks of at most 64 rows; blocks are
+// grouped into dependency levels (solve order: forward for L, backward for
+// U).  Entries that reach into other blocks are kept per row (CSR, ob_*)
+// and summed in segments of at most kTrsvSeg entries, one wavefront per
+// segment; the entries inside a block are kept as a dense s x s
+// column-major tile.
 struct TriFactor {
     int64_t n = 0;
     int nblocks = 0, max_block = 0;
-    double nnz = 0.0;
-    std::vector<int32_t> level_ptr;     // host: blocks of level l are level_blocks[level_ptr[l] .. level_ptr[l+1])
-    DevBuf<int32_t> blk_ptr, level_blocks;
-    DevBuf<int32_t> ob_ptr, ob_idx;     // per row: entries reaching into other blocks
+    int64_t nseg = 0;
+    double nnz = 0.0;                        // stored off-diagonal entries + diagonal
+    std::vector<int32_t> level_ptr;          // host: blocks of level l are binfo entries level_ptr[l] ..
+    std::vector<int64_t> seg_level_ptr;      // host: segments of level l
+    std::vector<double> lev_ob, lev_ib, lev_rows;  // host, per level: off-block and in-block entries, rows
+    DevBuf<int32_t> binfo;                   // per block, in level order: r0, rows, dense offset (2 x 32 bit)
+    DevBuf<int64_t> seg_k0, seg_k1;          // entry range of each segment
+    DevBuf<int32_t> row_seg;                 // per row: first and end segment
+    DevBuf<int32_t> ob_idx;
     DevBuf<double2_t> ob_val;
-    DevBuf<int32_t> ib_ptr, ib_idx;     // per column: entries inside its own block
-    DevBuf<double2_t> ib_val;
-    DevBuf<double2_t> diag;
+    DevBuf<double2_t> dense, diag;           // diag holds 1 / T[i, i]
 };
 
 // Per-kernel timing with HIP events on the context's stream.
@@ -97,7 +110,7 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 enum ProfKind {
-    PK_FFT_X_FWD = 0, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV, PK_TRSV, PK_SPMV, PK_COUNT
+    PK_FFT_X_FWD = 0, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV, PK_TRSV_GATHER, PK_TRSV_BLOCK, PK_COUNT
 };
 
 struct peec_ctx {
@@ -147,6 +160,7 @@ struct peec_ctx {
 
     // ---- vectors
     DevBuf<double2_t> vx, vy, vt1, vt2, vt3;  // N-sized work vectors
+    DevBuf<double2_t> trsv_part;              // segment partial sums of the triangular solves
 
     // ---- profiling
     bool prof_on = false;
@@ -155,3 +169,8 @@ struct peec_ctx {
     int prof_begin(int kind, double bytes);
     void prof_end(int idx);
 };
+
+// trsv.hip
+std::vector<int32_t> tri_chain_blocks(const peec_csc_c128 *L, int cap);
+int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f, const char *what);
+int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs);

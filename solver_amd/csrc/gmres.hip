@@ -43,11 +43,13 @@ __global__ void __launch_bounds__(256) k_norm2_part(const cd *v, int64_t from, i
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// part[b * ncol + k] = sum over the rows of block b of conj(U[i,k]) v[i]
+// part[w * ncol + k] = sum over the rows of wavefront w of conj(U[i,k]) v[i]
+// (4 wavefronts per block, reduced by lane shuffles: no barriers)
 __global__ void __launch_bounds__(256) k_gemvh_part(const cd *U, int64_t ld, int ncol, const cd *v, int64_t n,
                                                     cd *part)
 {
-    __shared__ double shx[256], shy[256];
+    const int lane = threadIdx.x & 63;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     for (int k = 0; k < ncol; k++) {
         const cd *u = U + (int64_t)k * ld;
         double sx = 0.0, sy = 0.0;
@@ -56,31 +58,34 @@ __global__ void __launch_bounds__(256) k_gemvh_part(const cd *U, int64_t ld, int
             sx += a.x * b.x + a.y * b.y;
             sy += a.x * b.y - a.y * b.x;
         }
-        shx[threadIdx.x] = sx;
-        shy[threadIdx.x] = sy;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (threadIdx.x < o) {
-                shx[threadIdx.x] += shx[threadIdx.x + o];
-                shy[threadIdx.x] += shy[threadIdx.x + o];
-            }
-            __syncthreads();
+        for (int o = 32; o > 0; o >>= 1) {
+            sx += __shfl_xor(sx, o, 64);
+            sy += __shfl_xor(sy, o, 64);
         }
-        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * ncol + k] = {shx[0], shy[0]};
-        __syncthreads();
+        if (lane == 0) part[w * ncol + k] = {sx, sy};
     }
 }
 
-__global__ void k_gemvh_final(const cd *part, int nblk, int ncol, cd *out)
+// out[k] = sum over the npart partials of column k (one block per column)
+__global__ void __launch_bounds__(256) k_gemvh_final(const cd *part, int npart, int ncol, cd *out)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ncol) return;
+    __shared__ double shx[4], shy[4];
+    const int k = blockIdx.x;
     double sx = 0.0, sy = 0.0;
-    for (int b = 0; b < nblk; b++) {
+    for (int b = threadIdx.x; b < npart; b += 256) {
         sx += part[(int64_t)b * ncol + k].x;
         sy += part[(int64_t)b * ncol + k].y;
     }
-    out[k] = {sx, sy};
+    for (int o = 32; o > 0; o >>= 1) {
+        sx += __shfl_xor(sx, o, 64);
+        sy += __shfl_xor(sy, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        shx[threadIdx.x >> 6] = sx;
+        shy[threadIdx.x >> 6] = sy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) out[k] = {(shx[0] + shx[1]) + (shx[2] + shx[3]), (shy[0] + shy[1]) + (shy[2] + shy[3])};
 }
 
 // out = base - U[:, 0:ncol) y, base = in (or 0) + s * e_idx
@@ -151,7 +156,7 @@ struct GmresWork {
     {
         if (!U.alloc((size_t)N * (inner + 1)) || !b.alloc(N) || !x.alloc(N) || !r.alloc(N) || !v.alloc(N) ||
             !u.alloc(N) || !xm.alloc(N) || !xmin.alloc(N) || !add.alloc(N) ||
-            !part.alloc((size_t)kRedBlocks * (inner + 1)) || !small.alloc((size_t)inner + 1)) {
+            !part.alloc((size_t)kRedBlocks * 4 * (inner + 1)) || !small.alloc((size_t)inner + 1)) {
             peec_set_error("gmres: out of device memory for a %lld x %d Krylov basis", (long long)N, inner + 1);
             return PEEC_ERR_NOMEM;
         }
@@ -185,7 +190,7 @@ struct GmresWork {
         if (ncol == 0) return PEEC_OK;
         int nbk = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (N + 255) / 256));
         hipLaunchKernelGGL(k_gemvh_part, dim3(nbk), dim3(256), 0, ctx->stream, U.p, N, ncol, vec, N, part.p);
-        hipLaunchKernelGGL(k_gemvh_final, dim3((ncol + 63) / 64), dim3(64), 0, ctx->stream, part.p, nbk, ncol, small.p);
+        hipLaunchKernelGGL(k_gemvh_final, dim3(ncol), dim3(256), 0, ctx->stream, part.p, nbk * 4, ncol, small.p);
         PEEC_HIP_TRY(hipGetLastError());
         PEEC_HIP_TRY(hipMemcpyAsync(hsmall.data(), small.p, sizeof(cd) * ncol, hipMemcpyDeviceToHost, ctx->stream));
         int rc = sync();

@@ -12,21 +12,13 @@
 // gives the same vector; then
 //   Y1 = Z^-1 (X1 - A M),  Y2 = P^-1 (X2 + Gamma' M),  Y3 = M.
 //
-// The LU factors of SS come from the host.  The triangular solves are
-// blocked: the factor order is cut into contiguous row blocks (the
-// separators and leaves of a nested-dissection ordering when the host
-// provides them), block levels follow the block dependency graph, and one
-// launch per level solves all its blocks in parallel, one workgroup per
-// block.  A workgroup first gathers the contributions of earlier blocks
-// (one wavefront per row), then runs the in-block substitution in LDS,
-// column by column with one barrier per column.  Kernel boundaries order
-// the levels, so no cross-workgroup waiting is needed.
+// The LU factors of SS come from the host; the triangular solves are the
+// blocked, level-scheduled ones of trsv.hip.
 #include "context.h"
 
 #include <algorithm>
 #include <cstring>
 
-constexpr int kMaxBlock = 4096;  // rows x rhs of one LDS tile (64 KiB)
 
 void peec_ctx::release_operator()
 {
@@ -166,62 +158,6 @@ __global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ, 
         } else {
             yr[i] = zr[qinv[i - m - ns]];
         }
-    }
-}
-
-// One level of a blocked triangular solve.  LOWER: forward substitution
-// with L (rows of a block depend on earlier blocks through ob_* entries
-// and on earlier rows of the block through the column lists ib_*);
-// otherwise backward substitution with U.  nrhs vectors at stride n,
-// processed rc at a time (s * rc complex values fit the LDS tile).
-template <bool LOWER>
-__global__ void __launch_bounds__(256) k_block_trsv(const int32_t *level_blocks, const int32_t *blk_ptr,
-                                                    const int32_t *ob_ptr, const int32_t *ob_idx, const cd *ob_val,
-                                                    const int32_t *ib_ptr, const int32_t *ib_idx, const cd *ib_val,
-                                                    const cd *diag, const cd *b, cd *x, int64_t n, int nrhs, int rcmax)
-{
-    extern __shared__ char smem_raw[];
-    cd *acc = reinterpret_cast<cd *>(smem_raw);
-    const int B = level_blocks[blockIdx.x];
-    const int r0 = blk_ptr[B], s = blk_ptr[B + 1] - r0;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int rb = 0; rb < nrhs; rb += rcmax) {
-        const int rc = min(rcmax, nrhs - rb);
-        // gather: acc[t][r] = b - sum over earlier blocks
-        for (int t = wave; t < s; t += 4) {
-            const int row = r0 + t;
-            for (int r = 0; r < rc; r++) {
-                const int64_t off = (int64_t)(rb + r) * n;
-                cd a = {0.0, 0.0};
-                for (int k = ob_ptr[row] + lane; k < ob_ptr[row + 1]; k += 64) a = cadd(a, cmul(ob_val[k], x[ob_idx[k] + off]));
-                for (int o = 32; o > 0; o >>= 1) {
-                    a.x += __shfl_xor(a.x, o, 64);
-                    a.y += __shfl_xor(a.y, o, 64);
-                }
-                if (lane == 0) acc[t * rc + r] = csub(b[row + off], a);
-            }
-        }
-        __syncthreads();
-        // in-block substitution, one column per step
-        for (int q = 0; q < s; q++) {
-            const int j = LOWER ? q : s - 1 - q;
-            const int col = r0 + j;
-            const int k0 = ib_ptr[col], k1 = ib_ptr[col + 1];
-            const int work = (k1 - k0) * rc;
-            const cd dj = diag[col];
-            for (int e = threadIdx.x; e < work; e += 256) {
-                const int k = k0 + e / rc, r = e % rc;
-                const cd y = cdiv(acc[j * rc + r], dj);
-                const int t = ib_idx[k] - r0;
-                acc[t * rc + r] = csub(acc[t * rc + r], cmul(ib_val[k], y));
-            }
-            __syncthreads();
-        }
-        for (int e = threadIdx.x; e < s * rc; e += 256) {
-            const int t = e / rc, r = e % rc;
-            x[r0 + t + (int64_t)(rb + r) * n] = cdiv(acc[e], diag[r0 + t]);
-        }
-        __syncthreads();
     }
 }
 
@@ -476,129 +412,6 @@ extern "C" int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, 
     return PEEC_OK;
 }
 
-// Split a CSC triangle into the blocked layout: per row the entries that
-// reach into other blocks (CSR, ob_*), per column the entries inside its own
-// block (CSC, ib_*), and the diagonal; then the block levels.
-static int tri_from_csc(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f,
-                        const char *what)
-{
-    const int64_t n = T->ncol;
-    if (T->nrow != n) {
-        peec_set_error("lu_set_factors: %s is not square", what);
-        return PEEC_ERR_ARG;
-    }
-    const int nb = (int)blk.size() - 1;
-    std::vector<int32_t> bof((size_t)n);
-    for (int B = 0; B < nb; B++)
-        for (int i = blk[(size_t)B]; i < blk[(size_t)B + 1]; i++) bof[(size_t)i] = B;
-    std::vector<double2_t> diag((size_t)n, {0.0, 0.0});
-    std::vector<char> has((size_t)n, 0);
-    std::vector<int32_t> optr((size_t)n + 1, 0), iptr((size_t)n + 1, 0);
-    for (int64_t c = 0; c < n; c++)
-        for (int64_t k = T->colptr[c] - 1; k < T->colptr[c + 1] - 1; k++) {
-            int64_t r = T->rowval[k] - 1;
-            if (r == c) continue;
-            bool tri_ok = lower ? r > c : r < c;
-            if (!tri_ok) {
-                if (T->nzval[k].re != 0.0 || T->nzval[k].im != 0.0) {
-                    peec_set_error("lu_set_factors: %s is not %s triangular", what, lower ? "lower" : "upper");
-                    return PEEC_ERR_ARG;
-                }
-                continue;
-            }
-            if (bof[(size_t)r] == bof[(size_t)c]) iptr[(size_t)c + 1]++;
-            else optr[(size_t)r + 1]++;
-        }
-    for (int64_t i = 0; i < n; i++) {
-        optr[(size_t)i + 1] += optr[(size_t)i];
-        iptr[(size_t)i + 1] += iptr[(size_t)i];
-    }
-    std::vector<int32_t> oidx((size_t)std::max<int32_t>(optr[(size_t)n], 1)), iidx((size_t)std::max<int32_t>(iptr[(size_t)n], 1));
-    std::vector<double2_t> oval(oidx.size()), ival(iidx.size());
-    std::vector<int32_t> opos(optr.begin(), optr.end() - 1), ipos(iptr.begin(), iptr.end() - 1);
-    std::vector<int32_t> level((size_t)nb, 0);
-    for (int64_t c = 0; c < n; c++)
-        for (int64_t k = T->colptr[c] - 1; k < T->colptr[c + 1] - 1; k++) {
-            int64_t r = T->rowval[k] - 1;
-            double2_t v = {T->nzval[k].re, T->nzval[k].im};
-            if (r == c) {
-                diag[(size_t)c] = v;
-                has[(size_t)c] = 1;
-                continue;
-            }
-            if (lower ? r < c : r > c) continue;
-            if (bof[(size_t)r] == bof[(size_t)c]) {
-                int32_t p = ipos[(size_t)c]++;
-                iidx[(size_t)p] = (int32_t)r;
-                ival[(size_t)p] = v;
-            } else {
-                int32_t p = opos[(size_t)r]++;
-                oidx[(size_t)p] = (int32_t)c;
-                oval[(size_t)p] = v;
-            }
-        }
-    for (int64_t i = 0; i < n; i++)
-        if (!has[(size_t)i] || (diag[(size_t)i].x == 0.0 && diag[(size_t)i].y == 0.0)) {
-            peec_set_error("lu_set_factors: %s has a zero diagonal at %lld", what, (long long)i + 1);
-            return PEEC_ERR_NUMERIC;
-        }
-    // levels: a block follows every block its rows read
-    for (int step = 0; step < nb; step++) {
-        const int B = lower ? step : nb - 1 - step;
-        int lv = 0;
-        for (int i = blk[(size_t)B]; i < blk[(size_t)B + 1]; i++)
-            for (int k = optr[(size_t)i]; k < optr[(size_t)i + 1]; k++)
-                lv = std::max(lv, level[(size_t)bof[(size_t)oidx[(size_t)k]]] + 1);
-        level[(size_t)B] = lv;
-    }
-    int nlev = 0;
-    for (int B = 0; B < nb; B++) nlev = std::max(nlev, level[(size_t)B] + 1);
-    std::vector<int32_t> lptr((size_t)nlev + 1, 0), lblk((size_t)nb);
-    for (int B = 0; B < nb; B++) lptr[(size_t)level[(size_t)B] + 1]++;
-    for (int l = 0; l < nlev; l++) lptr[(size_t)l + 1] += lptr[(size_t)l];
-    std::vector<int32_t> lpos(lptr.begin(), lptr.end() - 1);
-    for (int B = 0; B < nb; B++) lblk[(size_t)lpos[(size_t)level[(size_t)B]]++] = B;
-    f->n = n;
-    f->nblocks = nb;
-    f->level_ptr = lptr;
-    f->max_block = 0;
-    for (int B = 0; B < nb; B++) f->max_block = std::max(f->max_block, blk[(size_t)B + 1] - blk[(size_t)B]);
-    if (!f->blk_ptr.alloc(blk.size()) || !f->level_blocks.alloc(lblk.size()) || !f->ob_ptr.alloc(optr.size()) ||
-        !f->ob_idx.alloc(oidx.size()) || !f->ob_val.alloc(oval.size()) || !f->ib_ptr.alloc(iptr.size()) ||
-        !f->ib_idx.alloc(iidx.size()) || !f->ib_val.alloc(ival.size()) || !f->diag.alloc((size_t)n))
-        return PEEC_ERR_NOMEM;
-    PEEC_HIP_TRY(hipMemcpy(f->blk_ptr.p, blk.data(), blk.size() * 4, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(f->level_blocks.p, lblk.data(), lblk.size() * 4, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(f->ob_ptr.p, optr.data(), optr.size() * 4, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(f->ob_idx.p, oidx.data(), oidx.size() * 4, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(f->ob_val.p, oval.data(), oval.size() * 16, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(f->ib_ptr.p, iptr.data(), iptr.size() * 4, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(f->ib_idx.p, iidx.data(), iidx.size() * 4, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(f->ib_val.p, ival.data(), ival.size() * 16, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(f->diag.p, diag.data(), (size_t)n * 16, hipMemcpyHostToDevice));
-    f->nnz = (double)oidx.size() + (double)iidx.size();
-    return PEEC_OK;
-}
-
-// Blocks when the caller gives none: chains of the elimination tree of L
-// (column j + 1 is the parent of column j), capped in size.
-static std::vector<int32_t> chain_blocks(const peec_csc_c128 *L, int cap)
-{
-    const int64_t n = L->ncol;
-    std::vector<int32_t> blk{0};
-    for (int64_t j = 0; j < n; j++) {
-        int64_t parent = n;
-        for (int64_t k = L->colptr[j] - 1; k < L->colptr[j + 1] - 1; k++) {
-            int64_t r = L->rowval[k] - 1;
-            if (r > j && r < parent && (L->nzval[k].re != 0.0 || L->nzval[k].im != 0.0)) parent = r;
-        }
-        bool cont = parent == j + 1 && (j + 1 - blk.back()) < cap;
-        if (!cont) blk.push_back((int32_t)(j + 1));
-    }
-    if (blk.back() != n) blk.push_back((int32_t)n);
-    return blk;
-}
-
 extern "C" int peec_lu_set_factors(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L, const peec_csc_c128 *U,
                                    const int64_t *p, const int64_t *q, const double *Rs)
 {
@@ -642,16 +455,10 @@ extern "C" int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_
             blk[(size_t)B] = (int32_t)block_ptr[B];
         }
     } else {
-        blk = chain_blocks(L, 1024);
-    }
-    int max_blk = 0;
-    for (size_t B = 0; B + 1 < blk.size(); B++) max_blk = std::max(max_blk, blk[B + 1] - blk[B]);
-    if (max_blk > kMaxBlock) {
-        peec_set_error("peec_lu_set_factors: block of %d rows exceeds the %d-row LDS tile", max_blk, kMaxBlock);
-        return PEEC_ERR_ARG;
+        blk = tri_chain_blocks(L, kTrsvMaxBlock);
     }
     int rc;
-    if ((rc = tri_from_csc(L, true, blk, &ctx->Lf, "L")) || (rc = tri_from_csc(U, false, blk, &ctx->Uf, "U"))) return rc;
+    if ((rc = tri_build(L, true, blk, &ctx->Lf, "L")) || (rc = tri_build(U, false, blk, &ctx->Uf, "U"))) return rc;
     // pinv[p[i]] = i (scatter of the permuted rhs), qinv[q[j]] = j
     std::vector<int32_t> pinv((size_t)n, -1), qinv((size_t)n, -1);
     for (int64_t i = 0; i < n; i++) {
@@ -677,30 +484,6 @@ extern "C" int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_
 }
 
 // ------------------------------------------------------------------ apply
-static int trsv(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs)
-{
-    // bytes: every factor entry (index + value) once, the diagonal, the
-    // right-hand sides read and the solutions written
-    int pi = ctx->prof_begin(PK_TRSV, f.nnz * 20.0 + (double)f.n * (16.0 + 32.0 * nrhs));
-    const int rcmax = std::max(1, std::min(nrhs, kMaxBlock / std::max(f.max_block, 1)));
-    const size_t lds = (size_t)f.max_block * rcmax * 16;
-    const int nlev = (int)f.level_ptr.size() - 1;
-    for (int l = 0; l < nlev; l++) {
-        // levels are numbered in solve order (forward for L, backward for U)
-        const int nbl = f.level_ptr[(size_t)l + 1] - f.level_ptr[(size_t)l];
-        const int32_t *lb = f.level_blocks.p + f.level_ptr[(size_t)l];
-        if (lower)
-            hipLaunchKernelGGL(k_block_trsv<true>, dim3(nbl), dim3(256), lds, ctx->stream, lb, f.blk_ptr.p, f.ob_ptr.p,
-                               f.ob_idx.p, f.ob_val.p, f.ib_ptr.p, f.ib_idx.p, f.ib_val.p, f.diag.p, b, x, f.n, nrhs, rcmax);
-        else
-            hipLaunchKernelGGL(k_block_trsv<false>, dim3(nbl), dim3(256), lds, ctx->stream, lb, f.blk_ptr.p, f.ob_ptr.p,
-                               f.ob_idx.p, f.ob_val.p, f.ib_ptr.p, f.ib_idx.p, f.ib_val.p, f.diag.p, b, x, f.n, nrhs, rcmax);
-    }
-    PEEC_HIP_TRY(hipGetLastError());
-    ctx->prof_end(pi);
-    return PEEC_OK;
-}
-
 // In place on y (length N per rhs, stride ld): y <- M^-1 y
 int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld)
 {
@@ -714,8 +497,8 @@ int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld)
                        ctx->has_Rs ? ctx->Rs.p : nullptr, b, ctx->w, m, ns, n, nrhs, ld);
     PEEC_HIP_TRY(hipGetLastError());
     int rc;
-    if ((rc = trsv(ctx, ctx->Lf, true, b, z, nrhs))) return rc;   // L c = b   (c in z)
-    if ((rc = trsv(ctx, ctx->Uf, false, z, b, nrhs))) return rc;  // U z' = c  (z' in b)
+    if ((rc = trsv_solve(ctx, ctx->Lf, true, b, z, nrhs))) return rc;   // L c = b   (c in z)
+    if ((rc = trsv_solve(ctx, ctx->Uf, false, z, b, nrhs))) return rc;  // U z' = c  (z' in b)
     hipLaunchKernelGGL(k_post, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, b, ctx->perm_c.p, ctx->invZ.p,
                        ctx->invP.p, ctx->A.ptr.p, ctx->A.idx.p, ctx->A.val.p, ctx->Gt.ptr.p, ctx->Gt.idx.p,
                        ctx->Gt.val.p, m, ns, n, nrhs, ld);

@@ -1,0 +1,375 @@
+// Blocked sparse triangular solves with the LU factors of SS.
+//
+// The preconditioner solves SS M = r once per matvec (precond_3_3_vector,
+// compute_Matrix_vector_fft.jl:84-146, whose three F\ solves the operator
+// combines into one), i.e. one forward sweep with L and one backward sweep
+// with U.  The factor order is cut into row blocks of at most 64 rows, the
+// leaves and separator chunks of a nested dissection when the host gives
+// them.  A block depends on the blocks its rows read; blocks with no
+// pending dependency form a level, and the levels are solved one after the
+// other, each in two launches:
+//
+//   gather  -- every row of the level subtracts its entries in other
+//              (already solved) blocks; long rows are cut into segments of
+//              at most kTrsvSeg entries, one wavefront per segment, so a
+//              separator row with thousands of entries is spread over many
+//              CUs.  Each segment writes one partial sum per right-hand side.
+//   block   -- one workgroup per block loads the block's dense s x s
+//              triangle into LDS; a wavefront per right-hand side holds one
+//              row per lane, sums the row's partials and substitutes column
+//              by column, broadcasting each solved value by a lane shuffle.
+//
+// Kernel boundaries order the levels; there is no waiting between
+// workgroups.  The sums are taken in a fixed order, so results are
+// reproducible run to run.
+#include "context.h"
+
+#include <algorithm>
+
+namespace {
+
+typedef double2_t cd;
+
+__device__ __forceinline__ cd tcmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cd tcdiv(cd a, cd b)
+{
+    double d = b.x * b.x + b.y * b.y;
+    return {(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+}
+
+// part[s + r * pstride] = sum over the entries of segment s of T[row, c] * x[c]
+__global__ void __launch_bounds__(256) k_trsv_gather(const int64_t *seg_k0, const int64_t *seg_k1, int64_t s0,
+                                                     int64_t nseg, const int32_t *idx, const cd *val, const cd *x,
+                                                     cd *part, int64_t pstride, int64_t n, int nrhs)
+{
+    const int64_t sg = s0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (sg >= s0 + nseg) return;  // whole wavefront: the kernel has no barrier
+    const int lane = threadIdx.x & 63;
+    const int64_t k0 = seg_k0[sg], k1 = seg_k1[sg];
+    // a segment holds at most kTrsvSeg = 4 * 64 entries: 4 per lane
+    int c[4];
+    cd v[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int64_t k = k0 + lane + 64 * i;
+        c[i] = k < k1 ? idx[k] : -1;
+        v[i] = k < k1 ? val[k] : cd{0.0, 0.0};
+    }
+    for (int r = 0; r < nrhs; r++) {
+        const cd *xr = x + (int64_t)r * n;
+        double ax = 0.0, ay = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (c[i] >= 0) {
+                const cd xv = xr[c[i]];
+                ax += v[i].x * xv.x - v[i].y * xv.y;
+                ay += v[i].x * xv.y + v[i].y * xv.x;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            ax += __shfl_xor(ax, o, 64);
+            ay += __shfl_xor(ay, o, 64);
+        }
+        if (lane == 0) part[sg + (int64_t)r * pstride] = {ax, ay};
+    }
+}
+
+// Value of v in lane l (l uniform across the wavefront).
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// One workgroup per block of the level (binfo: the level's blocks in
+// launch order).  inv_diag holds 1 / T[i, i].
+template <bool LOWER>
+__global__ void __launch_bounds__(256) k_trsv_block(const int4 *binfo, const cd *dense, const cd *inv_diag,
+                                                    const int2 *row_seg, const cd *part, int64_t pstride, const cd *b,
+                                                    cd *x, int64_t n, int nrhs)
+{
+    __shared__ cd tile[64 * 64];
+    const int4 bi = binfo[blockIdx.x];  // r0, s, dense offset (lo, hi)
+    const int r0 = bi.x, s = bi.y;
+    const cd *T = dense + (int64_t)(((uint64_t)(uint32_t)bi.w << 32) | (uint32_t)bi.z);
+    const int ss = s * s;
+    {
+        // all 16 loads in flight before the first LDS store
+        cd t[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int e = threadIdx.x + 256 * i;
+            t[i] = e < ss ? T[e] : cd{0.0, 0.0};
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < ss) tile[e] = t[i];
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool mine = lane < s;
+    const int row = r0 + (mine ? lane : 0);
+    const cd dinv = mine ? inv_diag[row] : cd{1.0, 0.0};
+    const int2 gs = mine ? row_seg[row] : int2{0, 0};
+    const int g0 = gs.x, g1 = gs.y;
+    __syncthreads();
+    for (int r = (int)blockIdx.y * 4 + wave; r < nrhs; r += (int)gridDim.y * 4) {
+        cd acc = {0.0, 0.0};
+        if (mine) {
+            acc = b[row + (int64_t)r * n];
+            const cd *pr = part + (int64_t)r * pstride;
+            int g = g0;
+            for (; g + 4 <= g1; g += 4) {
+                const cd p0 = pr[g], p1 = pr[g + 1], p2 = pr[g + 2], p3 = pr[g + 3];
+                acc.x -= ((p0.x + p1.x) + (p2.x + p3.x));
+                acc.y -= ((p0.y + p1.y) + (p2.y + p3.y));
+            }
+            for (; g < g1; g++) {
+                acc.x -= pr[g].x;
+                acc.y -= pr[g].y;
+            }
+        }
+        for (int q = 0; q < s; q++) {
+            const int j = LOWER ? q : s - 1 - q;
+            const cd t = tile[j * s + lane < 64 * 64 ? j * s + lane : 0];
+            const double ax = lane_value(acc.x, j), ay = lane_value(acc.y, j);
+            const double dx = lane_value(dinv.x, j), dy = lane_value(dinv.y, j);
+            const double yx = ax * dx - ay * dy, yy = ax * dy + ay * dx;
+            if (mine && (LOWER ? lane > j : lane < j)) {
+                acc.x -= t.x * yx - t.y * yy;
+                acc.y -= t.x * yy + t.y * yx;
+            }
+        }
+        if (mine) x[row + (int64_t)r * n] = {acc.x * dinv.x - acc.y * dinv.y, acc.x * dinv.y + acc.y * dinv.x};
+    }
+}
+
+}  // namespace
+
+// Blocks when the caller gives none: chains of the elimination tree of L
+// (column j + 1 is the parent of column j), at most `cap` rows each.
+std::vector<int32_t> tri_chain_blocks(const peec_csc_c128 *L, int cap)
+{
+    const int64_t n = L->ncol;
+    std::vector<int32_t> blk{0};
+    for (int64_t j = 0; j < n; j++) {
+        int64_t parent = n;
+        for (int64_t k = L->colptr[j] - 1; k < L->colptr[j + 1] - 1; k++) {
+            int64_t r = L->rowval[k] - 1;
+            if (r > j && r < parent && (L->nzval[k].re != 0.0 || L->nzval[k].im != 0.0)) parent = r;
+        }
+        bool cont = parent == j + 1 && (j + 1 - blk.back()) < cap;
+        if (!cont) blk.push_back((int32_t)(j + 1));
+    }
+    if (blk.back() != n) blk.push_back((int32_t)n);
+    return blk;
+}
+
+int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f, const char *what)
+{
+    const int64_t n = T->ncol;
+    if (T->nrow != n) {
+        peec_set_error("lu_set_factors: %s is not square", what);
+        return PEEC_ERR_ARG;
+    }
+    const int nb = (int)blk.size() - 1;
+    std::vector<int32_t> bof((size_t)n);
+    std::vector<int64_t> doff((size_t)nb + 1, 0);
+    for (int B = 0; B < nb; B++) {
+        const int s = blk[(size_t)B + 1] - blk[(size_t)B];
+        if (s < 1 || s > kTrsvMaxBlock) {
+            peec_set_error("lu_set_factors: block %d has %d rows (1..%d allowed)", B, s, kTrsvMaxBlock);
+            return PEEC_ERR_ARG;
+        }
+        for (int i = blk[(size_t)B]; i < blk[(size_t)B + 1]; i++) bof[(size_t)i] = B;
+        doff[(size_t)B + 1] = doff[(size_t)B] + (int64_t)s * s;
+    }
+    std::vector<double2_t> diag((size_t)n, {0.0, 0.0});
+    std::vector<char> has((size_t)n, 0);
+    std::vector<double2_t> dense((size_t)doff[(size_t)nb], {0.0, 0.0});
+    std::vector<int64_t> optr((size_t)n + 1, 0);
+    int64_t nnz = 0;
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t k = T->colptr[c] - 1; k < T->colptr[c + 1] - 1; k++) {
+            const int64_t r = T->rowval[k] - 1;
+            if (r < 0 || r >= n) {
+                peec_set_error("lu_set_factors: %s row index out of range", what);
+                return PEEC_ERR_ARG;
+            }
+            const double2_t v = {T->nzval[k].re, T->nzval[k].im};
+            if (r == c) {
+                diag[(size_t)c] = v;
+                has[(size_t)c] = 1;
+                continue;
+            }
+            if (lower ? r < c : r > c) {
+                if (v.x != 0.0 || v.y != 0.0) {
+                    peec_set_error("lu_set_factors: %s is not %s triangular", what, lower ? "lower" : "upper");
+                    return PEEC_ERR_ARG;
+                }
+                continue;
+            }
+            nnz++;
+            const int Br = bof[(size_t)r], Bc = bof[(size_t)c];
+            if (Br == Bc) {
+                const int r0 = blk[(size_t)Br], s = blk[(size_t)Br + 1] - r0;
+                dense[(size_t)(doff[(size_t)Br] + (int64_t)(c - r0) * s + (r - r0))] = v;
+            } else {
+                optr[(size_t)r + 1]++;
+            }
+        }
+    for (int64_t i = 0; i < n; i++) optr[(size_t)i + 1] += optr[(size_t)i];
+    const int64_t nob = optr[(size_t)n];
+    std::vector<int32_t> oidx((size_t)std::max<int64_t>(nob, 1));
+    std::vector<double2_t> oval(oidx.size());
+    {
+        std::vector<int64_t> opos(optr.begin(), optr.end() - 1);
+        for (int64_t c = 0; c < n; c++)
+            for (int64_t k = T->colptr[c] - 1; k < T->colptr[c + 1] - 1; k++) {
+                const int64_t r = T->rowval[k] - 1;
+                if (r == c || (lower ? r < c : r > c) || bof[(size_t)r] == bof[(size_t)c]) continue;
+                const int64_t p = opos[(size_t)r]++;
+                oidx[(size_t)p] = (int32_t)c;
+                oval[(size_t)p] = {T->nzval[k].re, T->nzval[k].im};
+            }
+    }
+    for (int64_t i = 0; i < n; i++) {
+        const double2_t d = diag[(size_t)i];
+        if (!has[(size_t)i] || (d.x == 0.0 && d.y == 0.0)) {
+            peec_set_error("lu_set_factors: %s has a zero diagonal at %lld", what, (long long)i + 1);
+            return PEEC_ERR_NUMERIC;
+        }
+        const double m2 = d.x * d.x + d.y * d.y;
+        diag[(size_t)i] = {d.x / m2, -d.y / m2};  // stored as 1 / d
+    }
+    // levels in solve order: a block follows every block its rows read
+    std::vector<int32_t> level((size_t)nb, 0);
+    for (int step = 0; step < nb; step++) {
+        const int B = lower ? step : nb - 1 - step;
+        int lv = 0;
+        for (int i = blk[(size_t)B]; i < blk[(size_t)B + 1]; i++)
+            for (int64_t k = optr[(size_t)i]; k < optr[(size_t)i + 1]; k++)
+                lv = std::max(lv, level[(size_t)bof[(size_t)oidx[(size_t)k]]] + 1);
+        level[(size_t)B] = lv;
+    }
+    int nlev = 0;
+    for (int B = 0; B < nb; B++) nlev = std::max(nlev, level[(size_t)B] + 1);
+    std::vector<int32_t> lptr((size_t)nlev + 1, 0), lblk((size_t)std::max(nb, 1));
+    for (int B = 0; B < nb; B++) lptr[(size_t)level[(size_t)B] + 1]++;
+    for (int l = 0; l < nlev; l++) lptr[(size_t)l + 1] += lptr[(size_t)l];
+    {
+        std::vector<int32_t> lpos(lptr.begin(), lptr.end() - 1);
+        for (int B = 0; B < nb; B++) lblk[(size_t)lpos[(size_t)level[(size_t)B]]++] = B;
+    }
+    // in-block entries per block (for the byte counts of the profile)
+    std::vector<double> ibn((size_t)nb, 0.0);
+    for (int B = 0; B < nb; B++) {
+        const int64_t s = blk[(size_t)B + 1] - blk[(size_t)B];
+        for (int64_t e = 0; e < s * s; e++) {
+            const double2_t v = dense[(size_t)(doff[(size_t)B] + e)];
+            if (v.x != 0.0 || v.y != 0.0) ibn[(size_t)B] += 1.0;
+        }
+    }
+    f->lev_ob.assign((size_t)nlev, 0.0);
+    f->lev_ib.assign((size_t)nlev, 0.0);
+    f->lev_rows.assign((size_t)nlev, 0.0);
+    // segments, numbered level by level
+    std::vector<int64_t> sk0, sk1, slev((size_t)nlev + 1, 0);
+    std::vector<int32_t> rs((size_t)n * 2, 0);
+    std::vector<int32_t> binfo((size_t)std::max(nb, 1) * 4, 0);
+    for (int l = 0; l < nlev; l++) {
+        for (int e = lptr[(size_t)l]; e < lptr[(size_t)l + 1]; e++) {
+            const int B = lblk[(size_t)e];
+            binfo[(size_t)e * 4 + 0] = blk[(size_t)B];
+            binfo[(size_t)e * 4 + 1] = blk[(size_t)B + 1] - blk[(size_t)B];
+            binfo[(size_t)e * 4 + 2] = (int32_t)(uint32_t)(uint64_t)doff[(size_t)B];
+            binfo[(size_t)e * 4 + 3] = (int32_t)(uint32_t)((uint64_t)doff[(size_t)B] >> 32);
+            f->lev_ib[(size_t)l] += ibn[(size_t)B];
+            f->lev_rows[(size_t)l] += blk[(size_t)B + 1] - blk[(size_t)B];
+            for (int i = blk[(size_t)B]; i < blk[(size_t)B + 1]; i++) {
+                f->lev_ob[(size_t)l] += (double)(optr[(size_t)i + 1] - optr[(size_t)i]);
+                rs[(size_t)i * 2] = (int32_t)sk0.size();
+                for (int64_t k = optr[(size_t)i]; k < optr[(size_t)i + 1]; k += kTrsvSeg) {
+                    sk0.push_back(k);
+                    sk1.push_back(std::min<int64_t>(k + kTrsvSeg, optr[(size_t)i + 1]));
+                }
+                rs[(size_t)i * 2 + 1] = (int32_t)sk0.size();
+            }
+        }
+        slev[(size_t)l + 1] = (int64_t)sk0.size();
+    }
+    if (sk0.size() > (size_t)INT32_MAX) {
+        peec_set_error("lu_set_factors: %s needs too many segments", what);
+        return PEEC_ERR_ARG;
+    }
+    f->n = n;
+    f->nblocks = nb;
+    f->level_ptr = lptr;
+    f->seg_level_ptr = slev;
+    f->nseg = (int64_t)sk0.size();
+    f->max_block = 0;
+    for (int B = 0; B < nb; B++) f->max_block = std::max(f->max_block, blk[(size_t)B + 1] - blk[(size_t)B]);
+    f->nnz = (double)nnz + (double)n;
+    if (sk0.empty()) {
+        sk0.push_back(0);
+        sk1.push_back(0);
+    }
+    if (!f->binfo.alloc(binfo.size()) || !f->seg_k0.alloc(sk0.size()) || !f->seg_k1.alloc(sk1.size()) ||
+        !f->row_seg.alloc(rs.size()) || !f->ob_idx.alloc(oidx.size()) || !f->ob_val.alloc(oval.size()) ||
+        !f->dense.alloc(std::max<size_t>(dense.size(), 1)) || !f->diag.alloc((size_t)n)) {
+        peec_set_error("lu_set_factors: out of device memory for %s", what);
+        return PEEC_ERR_NOMEM;
+    }
+    PEEC_HIP_TRY(hipMemcpy(f->binfo.p, binfo.data(), binfo.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->seg_k0.p, sk0.data(), sk0.size() * 8, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->seg_k1.p, sk1.data(), sk1.size() * 8, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->row_seg.p, rs.data(), rs.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->ob_idx.p, oidx.data(), oidx.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->ob_val.p, oval.data(), oval.size() * 16, hipMemcpyHostToDevice));
+    if (!dense.empty()) PEEC_HIP_TRY(hipMemcpy(f->dense.p, dense.data(), dense.size() * 16, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->diag.p, diag.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+    return PEEC_OK;
+}
+
+int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs)
+{
+    const int64_t pstride = std::max<int64_t>(f.nseg, 1);
+    if ((size_t)(pstride * nrhs) > ctx->trsv_part.n) {
+        if (!ctx->trsv_part.alloc((size_t)(pstride * nrhs))) {
+            peec_set_error("out of device memory for triangular-solve partial sums");
+            return PEEC_ERR_NOMEM;
+        }
+    }
+    cd *part = ctx->trsv_part.p;
+    const int ry = std::min((nrhs + 3) / 4, 64);
+    const int nlev = (int)f.level_ptr.size() - 1;
+    for (int l = 0; l < nlev; l++) {
+        const int64_t s0 = f.seg_level_ptr[(size_t)l], ns = f.seg_level_ptr[(size_t)l + 1] - s0;
+        // algorithmic bytes of a gather: its entries (value + index) and the
+        // partial sums written; of a block solve: the in-block entries, the
+        // diagonal, b and x, and the partial sums read
+        if (ns > 0) {
+            int pi = ctx->prof_begin(PK_TRSV_GATHER, f.lev_ob[(size_t)l] * 20.0 + (double)ns * 16.0 * nrhs);
+            hipLaunchKernelGGL(k_trsv_gather, dim3((unsigned)((ns + 3) / 4)), dim3(256), 0, ctx->stream, f.seg_k0.p,
+                               f.seg_k1.p, s0, ns, f.ob_idx.p, f.ob_val.p, (const cd *)x, part, pstride, f.n, nrhs);
+            ctx->prof_end(pi);
+        }
+        int pi = ctx->prof_begin(PK_TRSV_BLOCK, f.lev_ib[(size_t)l] * 16.0 + f.lev_rows[(size_t)l] * (16.0 + 32.0 * nrhs) +
+                                                    (double)ns * 16.0 * nrhs);
+        const int nbl = f.level_ptr[(size_t)l + 1] - f.level_ptr[(size_t)l];
+        const int4 *bi = reinterpret_cast<const int4 *>(f.binfo.p) + f.level_ptr[(size_t)l];
+        const int2 *rsg = reinterpret_cast<const int2 *>(f.row_seg.p);
+        if (lower)
+            hipLaunchKernelGGL(k_trsv_block<true>, dim3(nbl, ry), dim3(256), 0, ctx->stream, bi, f.dense.p, f.diag.p,
+                               rsg, (const cd *)part, pstride, b, x, f.n, nrhs);
+        else
+            hipLaunchKernelGGL(k_trsv_block<false>, dim3(nbl, ry), dim3(256), 0, ctx->stream, bi, f.dense.p, f.diag.p,
+                               rsg, (const cd *)part, pstride, b, x, f.n, nrhs);
+        ctx->prof_end(pi);
+    }
+    PEEC_HIP_TRY(hipGetLastError());
+    return PEEC_OK;
+}

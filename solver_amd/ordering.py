@@ -23,8 +23,9 @@ def _split(idx, ijk, leaf, cap, out):
     c = ijk[idx]
     lo, hi = c.min(0), c.max(0)
     ax = int(np.argmax(hi - lo))
-    if hi[ax] == lo[ax]:
-        out.append(idx)
+    if hi[ax] == lo[ax]:  # coincident nodes: no plane separates them
+        for s in range(0, idx.size, leaf):
+            out.append(idx[s:s + leaf])
         return
     vals = c[:, ax]
     m = int(np.floor(np.median(vals)))
@@ -37,7 +38,7 @@ def _split(idx, ijk, leaf, cap, out):
         out.append(sep[s:s + cap])
 
 
-def nested_dissection(ijk, special=(), leaf=64, cap=1024):
+def nested_dissection(ijk, special=(), leaf=64, cap=64):
     """Order the nodes with integer lattice coordinates ijk (n x 3).
 
     Nodes in `special` (ports, lumped elements: coupled through Yle to
@@ -62,7 +63,7 @@ def nested_dissection(ijk, special=(), leaf=64, cap=1024):
     return order, block_ptr
 
 
-def mesh_order(mesh, leaf=64, cap=1024):
+def mesh_order(mesh, leaf=64, cap=64):
     """nested_dissection for a voxel mesh: nodes are voxel centres; port
     and lumped-element nodes are special."""
     ijk = np.rint((np.asarray(mesh.vox_centers) - np.asarray(mesh.origin)) /
