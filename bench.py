@@ -27,7 +27,15 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv_gather",
-              "trsv_block"]
+              "trsv_block_L", "trsv_block_U"]
+# kernel symbol behind each profile kind (FFT passes are instantiated per
+# mode -- 0 forward, 1 inverse, 2 forward * spectrum * inverse -- and per
+# line length class; the z pass of every kernel at the two-material
+# workload is k_fft_pass<double, 2, 16, 128>)
+KERNEL_SYMBOLS = {"fft_x_fwd_scatter": "k_fft_pass<double, 0, TPL, LMAX>", "fft_y_fwd": "k_fft_pass<double, 0, TPL, LMAX>",
+                  "fft_z_conv": "k_fft_pass<double, 2, TPL, LMAX>", "fft_y_inv": "k_fft_pass<double, 1, TPL, LMAX>",
+                  "fft_x_inv_gather": "k_fft_pass<double, 1, TPL, LMAX>", "trsv_gather": "k_trsv_gather",
+                  "trsv_block_L": "k_trsv_block<true>", "trsv_block_U": "k_trsv_block<false>"}
 
 
 def log(*a):
@@ -98,6 +106,9 @@ def main():
     ap.add_argument("--cpu-matvecs", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--prof-matvecs", type=int, default=20)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB device to device (a known-size "
+                         "reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X, and the sweep shards "
@@ -200,7 +211,7 @@ def main():
                    "setup_s": round(t_setup, 3), "matvec_kernel_ms": round(mv_ms, 4),
                    "fft_matvec_ms": round(fft_ms, 4), "fft_matvec_GBs": round(fft_bytes / fft_ms / 1e6, 1),
                    "gmres_matvecs_per_step": float(np.sum(iters)) / args.steps if iters else None},
-        "roofline": {"bound": "hbm", "kernel": dom["name"], "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": dom["name"], "symbol": KERNEL_SYMBOLS[dom["name"]], "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": ach / PEAK_HBM_GBS, "traffic": None, "avg_launch_us": dom["avg_us"],
                      "bytes_per_launch": dom["bytes"] / max(dom["count"], 1)},
         "kernels": {r["name"]: {"launches_per_matvec": r["count"] / args.prof_matvecs, "avg_us": round(r["avg_us"], 3),
@@ -208,6 +219,14 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_mib > 0:
+        import torch
+        a = torch.empty(args.copy_mib << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied %d MiB device to device 3 times" % args.copy_mib)
 
 
 if __name__ == "__main__":

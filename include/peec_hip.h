@@ -160,8 +160,8 @@ int peec_gmres(peec_ctx *ctx, const peec_c128 *b, double tol, int maxit, peec_c1
  * context's stream, algorithmic bytes and launch counts -- and switches
  * recording on (enable = 1) or off.  Kinds: 0..4 the five FFT-convolution
  * passes (x fwd + scatter, y fwd, z fwd * spectrum * z inv, y inv, x inv +
- * gather), 5 triangular-solve gathers, 6 triangular-solve block
- * substitutions (one record per launch). */
+ * gather), 5 triangular-solve gathers, 6 and 7 triangular-solve block
+ * substitutions with L and with U (one record per launch). */
 int peec_profile(peec_ctx *ctx, int enable, double *ms, double *bytes, int64_t *count, int nkinds);
 
 #ifdef __cplusplus

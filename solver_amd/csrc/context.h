@@ -110,7 +110,7 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 enum ProfKind {
-    PK_FFT_X_FWD = 0, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV, PK_TRSV_GATHER, PK_TRSV_BLOCK, PK_COUNT
+    PK_FFT_X_FWD = 0, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV, PK_TRSV_GATHER, PK_TRSV_BLOCK_L, PK_TRSV_BLOCK_U, PK_COUNT
 };
 
 struct peec_ctx {

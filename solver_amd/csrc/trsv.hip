@@ -357,7 +357,7 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
                                f.seg_k1.p, s0, ns, f.ob_idx.p, f.ob_val.p, (const cd *)x, part, pstride, f.n, nrhs);
             ctx->prof_end(pi);
         }
-        int pi = ctx->prof_begin(PK_TRSV_BLOCK, f.lev_ib[(size_t)l] * 16.0 + f.lev_rows[(size_t)l] * (16.0 + 32.0 * nrhs) +
+        int pi = ctx->prof_begin(lower ? PK_TRSV_BLOCK_L : PK_TRSV_BLOCK_U, f.lev_ib[(size_t)l] * 16.0 + f.lev_rows[(size_t)l] * (16.0 + 32.0 * nrhs) +
                                                     (double)ns * 16.0 * nrhs);
         const int nbl = f.level_ptr[(size_t)l + 1] - f.level_ptr[(size_t)l];
         const int4 *bi = reinterpret_cast<const int4 *>(f.binfo.p) + f.level_ptr[(size_t)l];
