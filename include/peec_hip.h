@@ -135,6 +135,30 @@ int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L
                                 const int64_t *p, const int64_t *q, const double *Rs, int64_t nblocks,
                                 const int64_t *block_ptr);
 
+/* Device factorization of SS (no counterpart in the reference, which
+ * calls UMFPACK's lu(SS) on the host at every frequency,
+ * FFT_solver_QS_S_type.jl:118-119).  peec_lu_analyze does the symbolic work
+ * once per geometry, after peec_operator_set: order is the elimination
+ * order (0-based node ids, n), cut into nfront fronts
+ * order[front_ptr[f] .. front_ptr[f+1]) that form an assembly tree
+ * (front_parent[f] > f, or -1 for a root) -- e.g. the leaves and separators
+ * of a nested dissection, with the port and lumped-element nodes (the only
+ * ones Yle couples) in the root front; block_ptr cuts the order into blocks
+ * of at most 64 rows inside single fronts for the triangular solves.
+ * peec_lu_factor then factors SS at the frequency of the last
+ * peec_frequency_set entirely on the device (no pivoting, like the host
+ * path) and makes it the preconditioner's factorization.  A later
+ * peec_lu_set_factors replaces it; peec_lu_analyze must then run again. */
+int peec_lu_analyze(peec_ctx *ctx, int64_t n, const int64_t *order, int64_t nfront, const int64_t *front_ptr,
+                    const int64_t *front_parent, int64_t nblocks, const int64_t *block_ptr);
+int peec_lu_factor(peec_ctx *ctx);
+/* Front K of the last device factorization, f x f column-major (tests),
+ * and the factor position of each of its rows/columns (positions, f).
+ * out and positions may be NULL to query f, s (pivots) and first (first
+ * pivot position). */
+int peec_lu_front(peec_ctx *ctx, int64_t K, int64_t *f, int64_t *s, int64_t *first, peec_c128 *out,
+                  int64_t *positions);
+
 /* ComputeMatrixVectorFTT(x, w, ...) including the precond_3_3_vector step
  * (compute_Matrix_vector_fft.jl:1-146): y = M^-1 K x, x and y of length
  * m + ns + n. */
@@ -161,7 +185,8 @@ int peec_gmres(peec_ctx *ctx, const peec_c128 *b, double tol, int maxit, peec_c1
  * recording on (enable = 1) or off.  Kinds: 0..4 the five FFT-convolution
  * passes (x fwd + scatter, y fwd, z fwd * spectrum * z inv, y inv, x inv +
  * gather), 5 triangular-solve gathers, 6 and 7 triangular-solve block
- * substitutions with L and with U (one record per launch). */
+ * substitutions with L and with U (one record per launch), 8 device
+ * factorizations (peec_lu_factor, one record per call). */
 int peec_profile(peec_ctx *ctx, int enable, double *ms, double *bytes, int64_t *count, int nkinds);
 
 #ifdef __cplusplus

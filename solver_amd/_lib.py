@@ -60,6 +60,9 @@ def lib():
         L.peec_lu_set_factors.argtypes = [vp, i64, ctypes.POINTER(CscC128), ctypes.POINTER(CscC128), i64p, i64p, dp]
         L.peec_lu_set_factors_blocked.argtypes = [vp, i64, ctypes.POINTER(CscC128), ctypes.POINTER(CscC128), i64p,
                                                   i64p, dp, i64, i64p]
+        L.peec_lu_analyze.argtypes = [vp, i64, i64p, i64, i64p, i64p, i64, i64p]
+        L.peec_lu_factor.argtypes = [vp]
+        L.peec_lu_front.argtypes = [vp, i64, i64p, i64p, i64p, c128p, i64p]
         L.peec_matvec.argtypes = [vp, c128p, c128p]
         L.peec_apply_K.argtypes = [vp, c128p, c128p]
         L.peec_precond_rhs.argtypes = [vp, dp, c128p]

@@ -63,6 +63,58 @@ struct ConvApp {
     int64_t nnz_in = 0, nnz_out = 0;
 };
 
+// One front of the multifrontal factorization (lu.hip): an f x f
+// column-major dense matrix at pool offset `off`; its first s rows/columns
+// are the pivots (factor positions first .. first + s - 1), the rest its
+// boundary.
+struct LuFrontInfo {
+    int64_t off;
+    int32_t f, s, first, parent;
+};
+
+// A complex GEMM tile C(m x n) = A(m x k) B(k x n) (mode 0) or
+// C -= A B (mode 1); offsets into the pool, column-major, m, n <= 64.
+struct LuGemmItem {
+    int64_t a, b, c;
+    int32_t lda, ldb, ldc, m, n, k, mode, pad;
+};
+
+// Symbolic factorization of SS for the device LU (peec_lu_analyze) and the
+// launch schedule of its numeric factorization (peec_lu_factor).
+struct LuPlan {
+    bool ready = false;
+    int64_t n = 0;
+    std::vector<int32_t> pos;                  // node -> factor position
+    std::vector<LuFrontInfo> fronts;           // host copy
+    std::vector<int32_t> front_of;             // factor position -> front
+    std::vector<int64_t> bnd_ptr;              // boundary positions of front K: bnd[bnd_ptr[K] ..)
+    std::vector<int32_t> bnd;
+    int64_t pool_fronts = 0, pool_total = 0;   // complex entries
+    DevBuf<double2_t> pool;
+    DevBuf<LuFrontInfo> d_fronts;
+    // SS = Yle + A' Z^-1 A + j w Gamma P^-1 Gamma': per stored entry its pool
+    // target, its A' Z^-1 A terms (edge, coefficient) and its Gamma term
+    int64_t nss = 0;
+    DevBuf<int64_t> ss_tgt;
+    DevBuf<int32_t> ss_cptr, ss_cedge;
+    DevBuf<double> ss_ccoef, ss_gsum;
+    // schedule
+    struct Launch {
+        int kind;               // 0 extend-add, 1 panel factor, 2 GEMM tiles
+        int64_t first, count;   // item range
+    };
+    std::vector<Launch> sched;
+    DevBuf<int2> ea_items;      // (child, column of its update matrix)
+    DevBuf<int64_t> rel_off;    // per child: offset of its boundary -> parent map
+    DevBuf<int32_t> rel;
+    DevBuf<int4> diag_items;    // (front, scratch slot, first pivot, -)
+    DevBuf<LuGemmItem> gemm_items;
+    int64_t scratch0 = 0;       // pool offset of the per-slot 64 x 64 inverse pairs
+    // values of the triangular-solve layouts, gathered from the pool
+    DevBuf<int64_t> l_ob_src, l_dense_src, u_ob_src, u_dense_src, u_diag_src;
+    double flops = 0.0;
+};
+
 // Sparse matrix in CSR (0-based, int32 indices) with real or complex values.
 struct CsrR {
     int64_t nrow = 0, ncol = 0;
@@ -75,8 +127,6 @@ struct CsrC {
     DevBuf<double2_t> val;
 };
 
-// Triangular factor prepared for the sync-free solve: rows in solve order,
-// each row's off-diagonal entries plus its diagonal.
 constexpr int kTrsvMaxBlock = 64;   // rows of one block (one per lane)
 constexpr int kTrsvSeg = 256;       // entries summed by one wavefront (4 per lane)
 
@@ -110,7 +160,7 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 enum ProfKind {
-    PK_FFT_X_FWD = 0, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV, PK_TRSV_GATHER, PK_TRSV_BLOCK_L, PK_TRSV_BLOCK_U, PK_COUNT
+    PK_FFT_X_FWD = 0, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV, PK_TRSV_GATHER, PK_TRSV_BLOCK_L, PK_TRSV_BLOCK_U, PK_LU, PK_COUNT
 };
 
 struct peec_ctx {
@@ -143,6 +193,10 @@ struct peec_ctx {
     int napps = 0;
     CsrR A, At, G, Gt;           // A (m x n), A' (n x m), Gamma (n x ns), Gamma' (ns x n)
     DevBuf<double> invP;         // ns
+    // host copies for the device LU: A by edge (CSR), Gamma by surface, 1/P
+    std::vector<int64_t> hA_ptr, hG_ptr;
+    std::vector<int32_t> hA_idx, hG_idx;
+    std::vector<double> hA_val, hG_val, h_invP;
     void release_operator();
 
     // ---- frequency
@@ -150,6 +204,8 @@ struct peec_ctx {
     double w = 0.0;
     DevBuf<double2_t> DZ, invZ;  // m
     CsrC Yle;                    // n x n
+    std::vector<int32_t> hYle_ptr, hYle_idx;  // host copy (CSR)
+    std::vector<double2_t> hYle_val;
 
     // ---- LU
     bool lu_ready = false;
@@ -157,6 +213,8 @@ struct peec_ctx {
     DevBuf<int32_t> perm_r, perm_c;  // inverse row / column permutations (0-based)
     DevBuf<double> Rs;               // optional row scaling of SS
     bool has_Rs = false;
+    LuPlan lu;                       // device factorization (lu.hip)
+    bool lu_layout_from_plan = false;  // Lf/Uf hold the plan's layouts
 
     // ---- vectors
     DevBuf<double2_t> vx, vy, vt1, vt2, vt3;  // N-sized work vectors
@@ -171,6 +229,18 @@ struct peec_ctx {
 };
 
 // trsv.hip
+// An off-diagonal entry of a triangular factor (factor order) and where its
+// value comes from (an index into the caller's value array).
+struct TriEntry {
+    int32_t r, c;
+    int64_t src;
+};
+// Source index of every value slot of a TriFactor (-1: structural zero).
+struct TriSources {
+    std::vector<int64_t> ob, dense, diag;
+};
+int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std::vector<TriEntry> &ent,
+               const std::vector<int64_t> &diag_src, TriFactor *f, TriSources *src, const char *what);
 std::vector<int32_t> tri_chain_blocks(const peec_csc_c128 *L, int cap);
 int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f, const char *what);
 int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs);

@@ -292,6 +292,34 @@ extern "C" int peec_operator_set(peec_ctx *ctx, int64_t mx, int64_t my, int64_t 
         return rc;
     std::vector<double> ip((size_t)std::max<int64_t>(ns, 1), 0.0);
     for (int64_t s = 0; s < ns; s++) ip[(size_t)s] = 1.0 / diag_P[s];
+    // host copies for the device LU (lu.hip)
+    ctx->hA_ptr.assign((size_t)m + 1, 0);
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t k = A->colptr[c] - 1; k < A->colptr[c + 1] - 1; k++) ctx->hA_ptr[(size_t)A->rowval[k]]++;
+    for (int64_t e = 0; e < m; e++) ctx->hA_ptr[(size_t)e + 1] += ctx->hA_ptr[(size_t)e];
+    ctx->hA_idx.assign((size_t)ctx->hA_ptr[(size_t)m], 0);
+    ctx->hA_val.assign((size_t)ctx->hA_ptr[(size_t)m], 0.0);
+    {
+        std::vector<int64_t> at(ctx->hA_ptr.begin(), ctx->hA_ptr.end() - 1);
+        for (int64_t c = 0; c < n; c++)
+            for (int64_t k = A->colptr[c] - 1; k < A->colptr[c + 1] - 1; k++) {
+                const int64_t p = at[(size_t)(A->rowval[k] - 1)]++;
+                ctx->hA_idx[(size_t)p] = (int32_t)c;
+                ctx->hA_val[(size_t)p] = A->nzval[k];
+            }
+    }
+    ctx->hG_ptr.assign((size_t)ns + 1, 0);
+    ctx->hG_idx.clear();
+    ctx->hG_val.clear();
+    for (int64_t c = 0; c < ns; c++) {
+        for (int64_t k = Gamma->colptr[c] - 1; k < Gamma->colptr[c + 1] - 1; k++) {
+            ctx->hG_idx.push_back((int32_t)(Gamma->rowval[k] - 1));
+            ctx->hG_val.push_back(Gamma->nzval[k]);
+        }
+        ctx->hG_ptr[(size_t)c + 1] = (int64_t)ctx->hG_idx.size();
+    }
+    ctx->h_invP.assign(ip.begin(), ip.begin() + ns);
+    ctx->lu.ready = false;
     if (!ctx->invP.alloc(ip.size())) return PEEC_ERR_NOMEM;
     PEEC_HIP_TRY(hipMemcpy(ctx->invP.p, ip.data(), ip.size() * 8, hipMemcpyHostToDevice));
 
@@ -401,6 +429,9 @@ extern "C" int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, 
             idx[(size_t)p] = (int32_t)c;
             val[(size_t)p] = {Yle->nzval[k].re, Yle->nzval[k].im};
         }
+    ctx->hYle_ptr = ptr;
+    ctx->hYle_idx = idx;
+    ctx->hYle_val = val;
     ctx->Yle.nrow = ctx->Yle.ncol = n;
     if (!ctx->Yle.ptr.alloc(ptr.size()) || !ctx->Yle.idx.alloc(idx.size()) || !ctx->Yle.val.alloc(val.size()))
         return PEEC_ERR_NOMEM;
@@ -480,6 +511,7 @@ extern "C" int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_
         ctx->has_Rs = false;
     }
     ctx->lu_ready = true;
+    ctx->lu_layout_from_plan = false;
     return PEEC_OK;
 }
 

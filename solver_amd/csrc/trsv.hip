@@ -168,85 +168,70 @@ std::vector<int32_t> tri_chain_blocks(const peec_csc_c128 *L, int cap)
     return blk;
 }
 
-int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f, const char *what)
+int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std::vector<TriEntry> &ent,
+               const std::vector<int64_t> &diag_src, TriFactor *f, TriSources *src, const char *what)
 {
-    const int64_t n = T->ncol;
-    if (T->nrow != n) {
-        peec_set_error("lu_set_factors: %s is not square", what);
+    const int nb = (int)blk.size() - 1;
+    if (nb < 0 || blk.front() != 0 || blk.back() != n) {
+        peec_set_error("%s: blocks do not cover the factor", what);
         return PEEC_ERR_ARG;
     }
-    const int nb = (int)blk.size() - 1;
     std::vector<int32_t> bof((size_t)n);
     std::vector<int64_t> doff((size_t)nb + 1, 0);
     for (int B = 0; B < nb; B++) {
         const int s = blk[(size_t)B + 1] - blk[(size_t)B];
         if (s < 1 || s > kTrsvMaxBlock) {
-            peec_set_error("lu_set_factors: block %d has %d rows (1..%d allowed)", B, s, kTrsvMaxBlock);
+            peec_set_error("%s: block %d has %d rows (1..%d allowed)", what, B, s, kTrsvMaxBlock);
             return PEEC_ERR_ARG;
         }
         for (int i = blk[(size_t)B]; i < blk[(size_t)B + 1]; i++) bof[(size_t)i] = B;
         doff[(size_t)B + 1] = doff[(size_t)B] + (int64_t)s * s;
     }
-    std::vector<double2_t> diag((size_t)n, {0.0, 0.0});
-    std::vector<char> has((size_t)n, 0);
-    std::vector<double2_t> dense((size_t)doff[(size_t)nb], {0.0, 0.0});
+    src->dense.assign((size_t)doff[(size_t)nb], -1);
+    src->diag = diag_src;
     std::vector<int64_t> optr((size_t)n + 1, 0);
-    int64_t nnz = 0;
-    for (int64_t c = 0; c < n; c++)
-        for (int64_t k = T->colptr[c] - 1; k < T->colptr[c + 1] - 1; k++) {
-            const int64_t r = T->rowval[k] - 1;
-            if (r < 0 || r >= n) {
-                peec_set_error("lu_set_factors: %s row index out of range", what);
-                return PEEC_ERR_ARG;
-            }
-            const double2_t v = {T->nzval[k].re, T->nzval[k].im};
-            if (r == c) {
-                diag[(size_t)c] = v;
-                has[(size_t)c] = 1;
-                continue;
-            }
-            if (lower ? r < c : r > c) {
-                if (v.x != 0.0 || v.y != 0.0) {
-                    peec_set_error("lu_set_factors: %s is not %s triangular", what, lower ? "lower" : "upper");
-                    return PEEC_ERR_ARG;
-                }
-                continue;
-            }
-            nnz++;
-            const int Br = bof[(size_t)r], Bc = bof[(size_t)c];
-            if (Br == Bc) {
-                const int r0 = blk[(size_t)Br], s = blk[(size_t)Br + 1] - r0;
-                dense[(size_t)(doff[(size_t)Br] + (int64_t)(c - r0) * s + (r - r0))] = v;
-            } else {
-                optr[(size_t)r + 1]++;
-            }
+    std::vector<double> ibn((size_t)std::max(nb, 1), 0.0);
+    for (const TriEntry &e : ent) {
+        if (e.r < 0 || e.r >= n || e.c < 0 || e.c >= n || (lower ? e.r <= e.c : e.r >= e.c)) {
+            peec_set_error("%s: entry (%d, %d) outside the %s triangle", what, e.r + 1, e.c + 1, lower ? "lower" : "upper");
+            return PEEC_ERR_ARG;
         }
+        const int Br = bof[(size_t)e.r], Bc = bof[(size_t)e.c];
+        if (Br == Bc) {
+            const int r0 = blk[(size_t)Br], s = blk[(size_t)Br + 1] - r0;
+            src->dense[(size_t)(doff[(size_t)Br] + (int64_t)(e.c - r0) * s + (e.r - r0))] = e.src;
+            ibn[(size_t)Br] += 1.0;
+        } else {
+            optr[(size_t)e.r + 1]++;
+        }
+    }
     for (int64_t i = 0; i < n; i++) optr[(size_t)i + 1] += optr[(size_t)i];
     const int64_t nob = optr[(size_t)n];
-    std::vector<int32_t> oidx((size_t)std::max<int64_t>(nob, 1));
-    std::vector<double2_t> oval(oidx.size());
+    std::vector<int32_t> oidx((size_t)std::max<int64_t>(nob, 1), 0);
+    src->ob.assign(oidx.size(), -1);
     {
+        // entries of a row in increasing column order
         std::vector<int64_t> opos(optr.begin(), optr.end() - 1);
-        for (int64_t c = 0; c < n; c++)
-            for (int64_t k = T->colptr[c] - 1; k < T->colptr[c + 1] - 1; k++) {
-                const int64_t r = T->rowval[k] - 1;
-                if (r == c || (lower ? r < c : r > c) || bof[(size_t)r] == bof[(size_t)c]) continue;
-                const int64_t p = opos[(size_t)r]++;
-                oidx[(size_t)p] = (int32_t)c;
-                oval[(size_t)p] = {T->nzval[k].re, T->nzval[k].im};
-            }
-    }
-    for (int64_t i = 0; i < n; i++) {
-        const double2_t d = diag[(size_t)i];
-        if (!has[(size_t)i] || (d.x == 0.0 && d.y == 0.0)) {
-            peec_set_error("lu_set_factors: %s has a zero diagonal at %lld", what, (long long)i + 1);
-            return PEEC_ERR_NUMERIC;
+        for (const TriEntry &e : ent) {
+            if (bof[(size_t)e.r] == bof[(size_t)e.c]) continue;
+            const int64_t p = opos[(size_t)e.r]++;
+            oidx[(size_t)p] = e.c;
+            src->ob[(size_t)p] = e.src;
         }
-        const double m2 = d.x * d.x + d.y * d.y;
-        diag[(size_t)i] = {d.x / m2, -d.y / m2};  // stored as 1 / d
+        for (int64_t i = 0; i < n; i++) {
+            const int64_t a = optr[(size_t)i], b = optr[(size_t)i + 1];
+            if (b - a < 2) continue;
+            std::vector<std::pair<int32_t, int64_t>> row((size_t)(b - a));
+            for (int64_t k = a; k < b; k++) row[(size_t)(k - a)] = {oidx[(size_t)k], src->ob[(size_t)k]};
+            std::sort(row.begin(), row.end());
+            for (int64_t k = a; k < b; k++) {
+                oidx[(size_t)k] = row[(size_t)(k - a)].first;
+                src->ob[(size_t)k] = row[(size_t)(k - a)].second;
+            }
+        }
     }
     // levels in solve order: a block follows every block its rows read
-    std::vector<int32_t> level((size_t)nb, 0);
+    std::vector<int32_t> level((size_t)std::max(nb, 1), 0);
     for (int step = 0; step < nb; step++) {
         const int B = lower ? step : nb - 1 - step;
         int lv = 0;
@@ -263,15 +248,6 @@ int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &bl
     {
         std::vector<int32_t> lpos(lptr.begin(), lptr.end() - 1);
         for (int B = 0; B < nb; B++) lblk[(size_t)lpos[(size_t)level[(size_t)B]]++] = B;
-    }
-    // in-block entries per block (for the byte counts of the profile)
-    std::vector<double> ibn((size_t)nb, 0.0);
-    for (int B = 0; B < nb; B++) {
-        const int64_t s = blk[(size_t)B + 1] - blk[(size_t)B];
-        for (int64_t e = 0; e < s * s; e++) {
-            const double2_t v = dense[(size_t)(doff[(size_t)B] + e)];
-            if (v.x != 0.0 || v.y != 0.0) ibn[(size_t)B] += 1.0;
-        }
     }
     f->lev_ob.assign((size_t)nlev, 0.0);
     f->lev_ib.assign((size_t)nlev, 0.0);
@@ -302,7 +278,7 @@ int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &bl
         slev[(size_t)l + 1] = (int64_t)sk0.size();
     }
     if (sk0.size() > (size_t)INT32_MAX) {
-        peec_set_error("lu_set_factors: %s needs too many segments", what);
+        peec_set_error("%s: too many segments", what);
         return PEEC_ERR_ARG;
     }
     f->n = n;
@@ -312,15 +288,15 @@ int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &bl
     f->nseg = (int64_t)sk0.size();
     f->max_block = 0;
     for (int B = 0; B < nb; B++) f->max_block = std::max(f->max_block, blk[(size_t)B + 1] - blk[(size_t)B]);
-    f->nnz = (double)nnz + (double)n;
+    f->nnz = (double)ent.size() + (double)n;
     if (sk0.empty()) {
         sk0.push_back(0);
         sk1.push_back(0);
     }
     if (!f->binfo.alloc(binfo.size()) || !f->seg_k0.alloc(sk0.size()) || !f->seg_k1.alloc(sk1.size()) ||
-        !f->row_seg.alloc(rs.size()) || !f->ob_idx.alloc(oidx.size()) || !f->ob_val.alloc(oval.size()) ||
-        !f->dense.alloc(std::max<size_t>(dense.size(), 1)) || !f->diag.alloc((size_t)n)) {
-        peec_set_error("lu_set_factors: out of device memory for %s", what);
+        !f->row_seg.alloc(rs.size()) || !f->ob_idx.alloc(oidx.size()) || !f->ob_val.alloc(oidx.size()) ||
+        !f->dense.alloc(std::max<size_t>(src->dense.size(), 1)) || !f->diag.alloc((size_t)std::max<int64_t>(n, 1))) {
+        peec_set_error("%s: out of device memory", what);
         return PEEC_ERR_NOMEM;
     }
     PEEC_HIP_TRY(hipMemcpy(f->binfo.p, binfo.data(), binfo.size() * 4, hipMemcpyHostToDevice));
@@ -328,8 +304,63 @@ int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &bl
     PEEC_HIP_TRY(hipMemcpy(f->seg_k1.p, sk1.data(), sk1.size() * 8, hipMemcpyHostToDevice));
     PEEC_HIP_TRY(hipMemcpy(f->row_seg.p, rs.data(), rs.size() * 4, hipMemcpyHostToDevice));
     PEEC_HIP_TRY(hipMemcpy(f->ob_idx.p, oidx.data(), oidx.size() * 4, hipMemcpyHostToDevice));
+    return PEEC_OK;
+}
+
+int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f, const char *what)
+{
+    const int64_t n = T->ncol;
+    if (T->nrow != n) {
+        peec_set_error("lu_set_factors: %s is not square", what);
+        return PEEC_ERR_ARG;
+    }
+    std::vector<TriEntry> ent;
+    std::vector<int64_t> dsrc((size_t)n, -1);
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t k = T->colptr[c] - 1; k < T->colptr[c + 1] - 1; k++) {
+            const int64_t r = T->rowval[k] - 1;
+            if (r < 0 || r >= n) {
+                peec_set_error("lu_set_factors: %s row index out of range", what);
+                return PEEC_ERR_ARG;
+            }
+            if (r == c) {
+                dsrc[(size_t)c] = k;
+                continue;
+            }
+            if (lower ? r < c : r > c) {
+                if (T->nzval[k].re != 0.0 || T->nzval[k].im != 0.0) {
+                    peec_set_error("lu_set_factors: %s is not %s triangular", what, lower ? "lower" : "upper");
+                    return PEEC_ERR_ARG;
+                }
+                continue;
+            }
+            ent.push_back({(int32_t)r, (int32_t)c, k});
+        }
+    std::vector<double2_t> diag((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t k = dsrc[(size_t)i];
+        const double2_t d = k >= 0 ? double2_t{T->nzval[k].re, T->nzval[k].im} : double2_t{0.0, 0.0};
+        if (d.x == 0.0 && d.y == 0.0) {
+            peec_set_error("lu_set_factors: %s has a zero diagonal at %lld", what, (long long)i + 1);
+            return PEEC_ERR_NUMERIC;
+        }
+        const double m2 = d.x * d.x + d.y * d.y;
+        diag[(size_t)i] = {d.x / m2, -d.y / m2};  // stored as 1 / d
+    }
+    TriSources src;
+    int rc = tri_layout(n, lower, blk, ent, dsrc, f, &src, "lu_set_factors");
+    if (rc) return rc;
+    std::vector<double2_t> oval(src.ob.size()), dense(std::max<size_t>(src.dense.size(), 1), double2_t{0.0, 0.0});
+    for (size_t k = 0; k < src.ob.size(); k++) {
+        const int64_t s = src.ob[k];
+        oval[k] = s >= 0 ? double2_t{T->nzval[s].re, T->nzval[s].im} : double2_t{0.0, 0.0};
+    }
+    for (size_t e = 0; e < src.dense.size(); e++) {
+        const int64_t s = src.dense[e];
+        if (s >= 0) dense[e] = {T->nzval[s].re, T->nzval[s].im};
+    }
     PEEC_HIP_TRY(hipMemcpy(f->ob_val.p, oval.data(), oval.size() * 16, hipMemcpyHostToDevice));
-    if (!dense.empty()) PEEC_HIP_TRY(hipMemcpy(f->dense.p, dense.data(), dense.size() * 16, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->dense.p, dense.data(), dense.size() * 16, hipMemcpyHostToDevice));
     PEEC_HIP_TRY(hipMemcpy(f->diag.p, diag.data(), (size_t)n * 16, hipMemcpyHostToDevice));
     return PEEC_OK;
 }

@@ -16,58 +16,111 @@ The order is a function of the geometry only and is computed once per mesh.
 import numpy as np
 
 
-def _split(idx, ijk, leaf, cap, out):
+def _split(idx, ijk, leaf, fronts):
+    """Append the fronts of a dissection of idx (postorder: children before
+    parents) and return the index of its top front."""
     if idx.size <= leaf:
-        out.append(idx)
-        return
+        fronts.append([idx, []])
+        return len(fronts) - 1
     c = ijk[idx]
     lo, hi = c.min(0), c.max(0)
     ax = int(np.argmax(hi - lo))
     if hi[ax] == lo[ax]:  # coincident nodes: no plane separates them
-        for s in range(0, idx.size, leaf):
-            out.append(idx[s:s + leaf])
-        return
+        fronts.append([idx, []])
+        return len(fronts) - 1
     vals = c[:, ax]
     m = int(np.floor(np.median(vals)))
     m = min(max(m, int(lo[ax]) + 1), int(hi[ax]))
     left, sep, right = idx[vals < m], idx[vals == m], idx[vals > m]
-    _split(left, ijk, leaf, cap, out)
+    kids = [_split(left, ijk, leaf, fronts)]
     if right.size:
-        _split(right, ijk, leaf, cap, out)
-    for s in range(0, sep.size, cap):
-        out.append(sep[s:s + cap])
+        kids.append(_split(right, ijk, leaf, fronts))
+    fronts.append([sep, kids])
+    return len(fronts) - 1
 
 
-def nested_dissection(ijk, special=(), leaf=64, cap=64):
-    """Order the nodes with integer lattice coordinates ijk (n x 3).
+class Dissection:
+    """A nested-dissection elimination order with its assembly tree.
 
-    Nodes in `special` (ports, lumped elements: coupled through Yle to
-    arbitrary nodes) are placed last.  Returns (order, block_ptr): order[k]
-    is the node eliminated k-th; block_ptr (nblocks + 1, from 0 to n) cuts
-    the order into leaves of at most `leaf` nodes and separator chunks of
-    at most `cap` nodes."""
+    order[k]      node eliminated k-th
+    front_ptr     fronts are order[front_ptr[f] : front_ptr[f+1]]: leaves
+                  of the dissection, separators and (last) the special
+                  nodes; every front follows its descendants
+    front_parent  parent front (-1 for the root)
+    block_ptr     the order cut into blocks of at most `cap` rows, each
+                  inside one front (rows of the triangular solves)
+    """
+
+    def __init__(self, order, front_ptr, front_parent, block_ptr):
+        self.order, self.front_ptr, self.front_parent, self.block_ptr = order, front_ptr, front_parent, block_ptr
+
+
+def dissect(ijk, special=(), leaf=64, cap=64):
+    """Nested dissection of the nodes with integer lattice coordinates ijk
+    (n x 3).  Nodes in `special` (ports, lumped elements: coupled through
+    Yle to arbitrary nodes) form the root front."""
     ijk = np.asarray(ijk, dtype=np.int64)
     n = ijk.shape[0]
     special = np.unique(np.asarray(special, dtype=np.int64).reshape(-1))
     rest = np.setdiff1d(np.arange(n, dtype=np.int64), special)
-    parts = []
-    if rest.size:
-        _split(rest, ijk, leaf, cap, parts)
-    for s in range(0, special.size, cap):
-        parts.append(special[s:s + cap])
-    parts = [p for p in parts if p.size]
-    order = np.concatenate(parts) if parts else np.zeros(0, dtype=np.int64)
-    block_ptr = np.zeros(len(parts) + 1, dtype=np.int64)
-    block_ptr[1:] = np.cumsum([p.size for p in parts])
+    fronts = []
+    tops = [_split(rest, ijk, leaf, fronts)] if rest.size else []
+    if special.size or not tops:
+        fronts.append([special, tops])
+    # drop empty fronts (a separator can be empty when a side is empty)
+    parent = [-1] * len(fronts)
+    for f, (_, kids) in enumerate(fronts):
+        for k in kids:
+            parent[k] = f
+    keep = [f for f in range(len(fronts)) if fronts[f][0].size]
+    remap = {}
+    for f in keep:
+        remap[f] = len(remap)
+
+    def up(f):
+        while f >= 0 and f not in remap:
+            f = parent[f]
+        return f
+
+    front_parent = np.array([remap[up(parent[f])] if up(parent[f]) >= 0 else -1 for f in keep], dtype=np.int64)
+    sizes = [fronts[f][0].size for f in keep]
+    order = np.concatenate([fronts[f][0] for f in keep]) if keep else np.zeros(0, dtype=np.int64)
+    front_ptr = np.zeros(len(keep) + 1, dtype=np.int64)
+    front_ptr[1:] = np.cumsum(sizes)
+    bp = [0]
+    for f in range(len(keep)):
+        a, b = int(front_ptr[f]), int(front_ptr[f + 1])
+        bp.extend(range(a + cap, b, cap))
+        bp.append(b)
     assert order.size == n and np.array_equal(np.sort(order), np.arange(n))
-    return order, block_ptr
+    return Dissection(order, front_ptr, front_parent, np.array(bp, dtype=np.int64))
+
+
+def nested_dissection(ijk, special=(), leaf=64, cap=64):
+    """(order, block_ptr) of dissect(): the elimination order and its cut
+    into blocks of at most `cap` rows."""
+    d = dissect(ijk, special, leaf, cap)
+    return d.order, d.block_ptr
+
+
+def mesh_ijk(mesh):
+    return np.rint((np.asarray(mesh.vox_centers) - np.asarray(mesh.origin)) /
+                   np.array([mesh.sx, mesh.sy, mesh.sz]) - 0.5).astype(np.int64)
+
+
+def mesh_special(mesh):
+    special = [np.asarray(mesh.port_nodes, dtype=np.int64).reshape(-1)]
+    special += [np.asarray(le["nodes"], dtype=np.int64).reshape(-1) for le in mesh.lumped_elements]
+    return np.concatenate(special)
+
+
+def mesh_dissection(mesh, leaf=64, cap=64):
+    """dissect() for a voxel mesh: nodes are voxel centres; port and
+    lumped-element nodes are special."""
+    return dissect(mesh_ijk(mesh), mesh_special(mesh), leaf, cap)
 
 
 def mesh_order(mesh, leaf=64, cap=64):
-    """nested_dissection for a voxel mesh: nodes are voxel centres; port
-    and lumped-element nodes are special."""
-    ijk = np.rint((np.asarray(mesh.vox_centers) - np.asarray(mesh.origin)) /
-                  np.array([mesh.sx, mesh.sy, mesh.sz]) - 0.5).astype(np.int64)
-    special = [np.asarray(mesh.port_nodes, dtype=np.int64).reshape(-1)]
-    special += [np.asarray(le["nodes"], dtype=np.int64).reshape(-1) for le in mesh.lumped_elements]
-    return nested_dissection(ijk, np.concatenate(special), leaf, cap)
+    """nested_dissection for a voxel mesh (see mesh_dissection)."""
+    d = mesh_dissection(mesh, leaf, cap)
+    return d.order, d.block_ptr

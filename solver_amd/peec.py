@@ -204,6 +204,42 @@ def lu_set_factors(Lf, Uf, p, q, Rs=None, ctx=None, blocks=None):
     del k1, k2
 
 
+def lu_analyze(dissection, ctx=None):
+    """Symbolic factorization of SS for the device LU (peec_lu_analyze):
+    `dissection` has order, front_ptr, front_parent and block_ptr
+    (ordering.Dissection).  Once per geometry, after operator_set."""
+    ctx = ctx or default_context()
+    arrs = [np.ascontiguousarray(getattr(dissection, k), dtype=np.int64)
+            for k in ("order", "front_ptr", "front_parent", "block_ptr")]
+    order, fptr, fpar, bptr = arrs
+    L.check(L.lib().peec_lu_analyze(ctx.handle, int(order.size), order.ctypes.data_as(L.i64p), int(fptr.size - 1),
+                                    fptr.ctypes.data_as(L.i64p), fpar.ctypes.data_as(L.i64p), int(bptr.size - 1),
+                                    bptr.ctypes.data_as(L.i64p)), "peec_lu_analyze")
+
+
+def lu_factor(ctx=None):
+    """Device LU of SS at the frequency of the last frequency_set
+    (peec_lu_factor); replaces the host lu(SS) of FFT_solver_QS_S_type.jl:119."""
+    ctx = ctx or default_context()
+    L.check(L.lib().peec_lu_factor(ctx.handle), "peec_lu_factor")
+
+
+def lu_front(K, ctx=None):
+    """(s, positions, F): front K of the last device factorization -- its
+    pivot count, the factor position of each row/column, and the f x f
+    matrix (L below and U on/above the diagonal in the first s columns /
+    rows, the update matrix in the rest)."""
+    ctx = ctx or default_context()
+    f, s, first = (ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64())
+    L.check(L.lib().peec_lu_front(ctx.handle, int(K), ctypes.byref(f), ctypes.byref(s), ctypes.byref(first), None,
+                                  None), "peec_lu_front")
+    out = np.zeros(f.value * f.value, dtype=np.complex128)
+    pos = np.zeros(f.value, dtype=np.int64)
+    L.check(L.lib().peec_lu_front(ctx.handle, int(K), None, None, None, L.as_c128p(out), pos.ctypes.data_as(L.i64p)),
+            "peec_lu_front")
+    return s.value, pos, out.reshape(f.value, f.value, order="F")
+
+
 def ComputeMatrixVectorFTT(x, ctx=None):
     """compute_Matrix_vector_fft.jl:1-81 (M^-1 K x) for the operator,
     frequency and LU factors loaded into ctx."""

@@ -120,16 +120,23 @@ def lu_factor(SS, order=None):
 class Solver:
     """Holds one geometry on one device context."""
 
-    def __init__(self, mesh, ctx=None, nested_dissection=True):
+    def __init__(self, mesh, ctx=None, nested_dissection=True, device_lu=False):
+        """device_lu: factor SS on the GPU (peec_lu_factor) instead of with
+        SuperLU on the host; both use the nested-dissection order."""
         self.mesh = mesh
         self.ctx = ctx or peec.default_context()
-        self.order, self.blocks = ordering.mesh_order(mesh) if nested_dissection else (None, None)
+        self.device_lu = device_lu
+        self.dissection = ordering.mesh_dissection(mesh) if (nested_dissection or device_lu) else None
+        self.order, self.blocks = ((self.dissection.order, self.dissection.block_ptr) if self.dissection is not None
+                                   else (None, None))
         ok = peec.compute_FFT_mutual_coupling_mats(mesh.origin, mesh.Nx, mesh.Ny, mesh.Nz, mesh.sx, mesh.sy,
                                                    mesh.sz, mesh.escalings, ctx=self.ctx)
         if not ok:
             raise RuntimeError("stopped")
         peec.operator_set(mesh, ctx=self.ctx)
         peec.set_ctx_size(self.ctx, mesh.m + mesh.ns + mesh.n)
+        if device_lu:
+            peec.lu_analyze(self.dissection, ctx=self.ctx)
 
     def set_frequency(self, w, R0=50.0):
         m = self.mesh
@@ -137,6 +144,9 @@ class Solver:
         Yle = build_Yle_FFT(m.lumped_elements, m.port_nodes, esc, m.n, w / esc["freq"], R0)
         DZ = DZ_of(m, w)
         peec.frequency_set(w, DZ, Yle, ctx=self.ctx)
+        if self.device_lu:
+            peec.lu_factor(ctx=self.ctx)
+            return
         Lf, Uf, p, q = lu_factor(SS_of(m, w, DZ, Yle), self.order)
         # the blocks hold when SuperLU kept the order (no pivoting, no
         # column postorder); otherwise the library finds its own

@@ -1,0 +1,101 @@
+"""Device factorization of SS (peec_lu_analyze / peec_lu_factor) against
+SuperLU in the same elimination order, and the solver built on it against
+the oracle.
+
+Both factor SS[order, order] without pivoting, so the factors agree to
+rounding (checked entry-wise, 1e-10 relative to the largest entry of each
+factor column block); the preconditioned matvec and the sweep are held to
+the same max(1e-12, cond(SS) * eps) bar as the host-factor path."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import fft_solver as OF
+from solver_amd import ordering, sweep
+
+from test_gpu_operator import crossbar_mesh, rel, small_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_lu_matches_superlu_small():
+    from solver_amd import peec
+    M = small_mesh()
+    S = sweep.Solver(M, device_lu=True)
+    w = 2 * np.pi * 3e8 * M.escalings["freq"]
+    S.set_frequency(w)
+    Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+    SS = sweep.SS_of(M, w, sweep.DZ_of(M, w), Yle)
+    D = S.dissection
+    Lf, Uf, p, q = sweep.lu_factor(SS, D.order)
+    Ld, Ud = Lf.toarray(), Uf.toarray()
+    n = M.n
+    for K in range(len(D.front_ptr) - 1):
+        s, pos, F = peec.lu_front(K, ctx=S.ctx)
+        first = D.front_ptr[K]
+        assert s == D.front_ptr[K + 1] - first and np.array_equal(pos[:s], np.arange(first, first + s))
+        # L: unit lower in the pivot block, then the boundary rows; U: the
+        # pivot rows over all columns of the front
+        Lk = np.tril(F[:, :s], -1)
+        Lk[:s, :s] += np.eye(s)
+        Uk = np.triu(F[:s, :])
+        ref_L = Ld[np.ix_(pos, pos[:s])]
+        ref_U = Ud[np.ix_(pos[:s], pos)]
+        eL = np.abs(Lk - ref_L).max() / max(np.abs(ref_L).max(), 1e-300)
+        eU = np.abs(Uk - ref_U).max() / max(np.abs(ref_U).max(), 1e-300)
+        print("front", K, "f", F.shape[0], "s", s, "L err %.2e U err %.2e" % (eL, eU), flush=True)
+        assert eL <= 1e-10 and eU <= 1e-10
+    # nothing of SuperLU's factors lies outside the fronts
+    covered = np.zeros((n, n), dtype=bool)
+    for K in range(len(D.front_ptr) - 1):
+        s, pos, F = peec.lu_front(K, ctx=S.ctx)
+        covered[np.ix_(pos, pos[:s])] = True
+        covered[np.ix_(pos[:s], pos)] = True
+    assert not np.any((np.abs(Ld) + np.abs(Ud) > 0) & ~covered)
+    assert n == D.order.size
+
+
+@pytest.mark.parametrize("f", [1e7, 3e8, 5e9])
+def test_device_lu_matvec_matches_oracle(f):
+    from solver_amd import peec
+    M = small_mesh()
+    rows = OF.kernel_rows(M.origin, M.Nx, M.Ny, M.Nz, M.sx, M.sy, M.sz)
+    spec = OF.spectra(rows, M.Nx, M.Ny, M.Nz)
+    S = sweep.Solver(M, device_lu=True)
+    w = 2 * np.pi * f * M.escalings["freq"]
+    S.set_frequency(w)
+    op = OF.Operator(M, spec)
+    Yle = OF.build_Yle(M.lumped_elements, M.port_nodes, M.escalings["Yle"], M.n, w / M.escalings["freq"], 50.0)
+    op.set_frequency(w, OF.frequency_terms(M, w), Yle, order=S.order)
+    rng = np.random.default_rng(7)
+    N = M.m + M.ns + M.n
+    x = rng.normal(size=N) + 1j * rng.normal(size=N)
+    y_gpu = peec.ComputeMatrixVectorFTT(x, ctx=S.ctx)
+    y_ora = op.matvec(x)
+    cond = np.linalg.cond(op.SS.toarray())
+    tol = max(1e-12, cond * np.finfo(float).eps)
+    print("device LU: M^-1 K", f, rel(y_gpu, y_ora), "cond(SS) %.2e tol %.1e" % (cond, tol), flush=True)
+    assert rel(y_gpu, y_ora) < tol
+
+
+def test_device_lu_crossbar_sweep_matches_oracle():
+    M, cfg = crossbar_mesh()
+    rows = OF.kernel_rows(M.origin, M.Nx, M.Ny, M.Nz, M.sx, M.sy, M.sz)
+    spec = OF.spectra(rows, M.Nx, M.Ny, M.Nz)
+    order = ordering.mesh_order(M)[0]
+    rec_o, rec_g = [], []
+    out_o = OF.fft_solver_qs(cfg["freqs"], M, spec, 300, cfg["tol"], record=rec_o, order=order)
+    S = sweep.Solver(M, device_lu=True)
+    out_g = sweep.FFT_solver_QS_S_type(cfg["freqs"], M, 300, cfg["tol"], solver=S, record=rec_g)
+    for a, b in zip(rec_g, rec_o):
+        assert a["flag"] == b["flag"] and a["iter"] == b["iter"], (a, b)
+    for k, f in enumerate(cfg["freqs"]):
+        w = 2 * np.pi * f * M.escalings["freq"]
+        Yle = OF.build_Yle(M.lumped_elements, M.port_nodes, M.escalings["Yle"], M.n, f, 50.0)
+        op = OF.Operator(M, spec)
+        op.set_frequency(w, OF.frequency_terms(M, w), Yle, F=False)
+        cond = np.linalg.cond(op.SS.toarray())
+        tol = max(1e-12, cond * np.finfo(float).eps)
+        err = np.max(np.abs(out_g["Z"][..., k] - out_o["Z"][..., k]) / np.abs(out_o["Z"][..., k]))
+        print("device LU crossbar f %.3e Z rel err %.2e tol %.1e" % (f, err, tol), flush=True)
+        assert err < tol
