@@ -115,6 +115,19 @@ int peec_operator_set(peec_ctx *ctx, int64_t mx, int64_t my, int64_t mz,
  * frequency), DZ (m), Yle (n x n CSC). */
 int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, const peec_csc_c128 *Yle);
 
+/* Frequency slots for batched solves (no counterpart in the reference,
+ * whose sweep loop handles one frequency at a time,
+ * FFT_solver_QS_S_type.jl:100-232).  peec_frequency_slots allocates
+ * nslots slots (clearing all frequency and factor state; more than one
+ * slot needs the device factorization, peec_lu_analyze);
+ * peec_frequency_set_slot fills one, with the arguments of
+ * peec_frequency_set; peec_lu_factor_slot factors its SS on the device.
+ * Right-hand side r of a batched call uses slot r / rhs_per_slot.
+ * peec_frequency_set and peec_lu_factor are the one-slot forms (slot 0). */
+int peec_frequency_slots(peec_ctx *ctx, int nslots);
+int peec_frequency_set_slot(peec_ctx *ctx, int slot, double w, const peec_c128 *DZ, const peec_csc_c128 *Yle);
+int peec_lu_factor_slot(peec_ctx *ctx, int slot);
+
 /* LU factors of SS = Yle + A' Z^-1 A + j w Gamma P^-1 Gamma'
  * (F = lu(SS), FFT_solver_QS_S_type.jl:118-119) in UMFPACK's convention,
  * which Julia exposes as F.L, F.U, F.p, F.q, F.Rs:
@@ -177,6 +190,28 @@ int peec_precond_rhs(peec_ctx *ctx, const double *is, peec_c128 *tn);
  * hold maxit + 1 doubles; *nres receives its used length. */
 int peec_gmres(peec_ctx *ctx, const peec_c128 *b, double tol, int maxit, peec_c128 *x,
                int *flag, double *relres, int64_t iter[2], double *resvec, int64_t *nres);
+
+/* Batched solves for a frequency sweep (no counterpart in the reference,
+ * which calls gmres_custom once per port and frequency,
+ * FFT_solver_QS_S_type.jl:120-206).  nrhs right-hand sides b (N x nrhs,
+ * column-major, N = m + ns + n), right-hand side r against slot
+ * r / rhs_per_slot, each solved for (M^-1 K) x = b by restarted GMRES
+ * (restart basis vectors, classical Gram-Schmidt twice) inside iterative
+ * refinement until the FP64 residual satisfies ||b - M^-1 K x|| <=
+ * tol ||b|| or max_cycles refinement cycles ran.  mixed = 1 applies K with
+ * complex-FP32 FFT convolutions inside the cycles (FP64 everywhere else);
+ * inner_tol is the residual reduction a mixed cycle aims for (0: 1e-5).
+ * x holds the warm starts on input and the solutions on output.  Per
+ * right-hand side: flag (0 converged, 1 not within max_cycles, 3 the
+ * refinement stalled), relres (final FP64 relative residual), iters
+ * (inner iterations) and cycles. */
+int peec_gmres_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const peec_c128 *b, double tol, int restart,
+                       int max_cycles, int mixed, double inner_tol, peec_c128 *x, int *flag, double *relres, int *iters,
+                       int *cycles);
+/* tn_r = M^-1 [0; 0; is_r] for nrhs right-hand sides (is: n x nrhs real,
+ * column-major; slot r / rhs_per_slot): precond_3_3_Kt_fft!
+ * (FFT_solver_QS_S_type.jl:237-262) batched. */
+int peec_precond_rhs_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const double *is, peec_c128 *tn);
 
 /* Kernel timing (no counterpart in the reference, which only prints @time,
  * do_solving_fft.jl:133).  Reads and clears the per-kind totals recorded

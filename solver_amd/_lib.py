@@ -63,6 +63,13 @@ def lib():
         L.peec_lu_analyze.argtypes = [vp, i64, i64p, i64, i64p, i64p, i64, i64p]
         L.peec_lu_factor.argtypes = [vp]
         L.peec_lu_front.argtypes = [vp, i64, i64p, i64p, i64p, c128p, i64p]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.peec_frequency_slots.argtypes = [vp, ctypes.c_int]
+        L.peec_frequency_set_slot.argtypes = [vp, ctypes.c_int, ctypes.c_double, c128p, ctypes.POINTER(CscC128)]
+        L.peec_lu_factor_slot.argtypes = [vp, ctypes.c_int]
+        L.peec_gmres_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, c128p, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_double, c128p, ip, dp, ip, ip]
+        L.peec_precond_rhs_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, c128p]
         L.peec_matvec.argtypes = [vp, c128p, c128p]
         L.peec_apply_K.argtypes = [vp, c128p, c128p]
         L.peec_precond_rhs.argtypes = [vp, dp, c128p]

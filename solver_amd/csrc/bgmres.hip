@@ -1,0 +1,422 @@
+// Batched GMRES for a frequency sweep: many right-hand sides -- ports x
+// frequency slots -- solved in lockstep, each against its own slot's
+// preconditioned operator M^-1 K, with one batched matvec per iteration.
+//
+// The reference solves one port at one frequency at a time with
+// unrestarted Householder GMRES (gmres_custom.jl, called from
+// FFT_solver_QS_S_type.jl:140).  Here every right-hand side runs restarted
+// GMRES with classical Gram-Schmidt applied twice (CGS2: two passes of
+// V' w and w - V h over the basis, so the orthogonalisation is two
+// streaming passes instead of j dependent ones), wrapped in iterative
+// refinement: each cycle solves M^-1 K d = r to a relative tolerance, adds
+// d to x, and recomputes the true residual r = M^-1 (b - K x) with the FP64
+// operator.  With mixed = 1 the inner cycles apply K with the FFT
+// convolutions in complex FP32 (FP32 spectra, FP32 grids); the triangular
+// solves, the sparse terms, the orthogonalisation, the Givens least-squares
+// and the refinement residual stay in FP64, so the converged x satisfies
+// the FP64 residual test ||r|| <= tol ||b|| like the plain path.
+//
+// Right-hand side r uses frequency slot r / rhs_per_slot.  All basis
+// vectors and iterates live in HBM; the host keeps only the small
+// Hessenberg matrices, the rotations and the convergence flags.
+#include "context.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+typedef std::complex<double> zc;
+
+int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps);
+int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, int rps);
+
+namespace {
+
+typedef double2_t cd;
+
+constexpr int kGx = 128;  // row blocks per right-hand side in the reductions
+
+// part[(r * gx + bx) * 4 + wave] = partial sum of |v_r|^2
+__global__ void __launch_bounds__(256) k_bnorm_part(const cd *v, int64_t ld, int64_t n, double *part)
+{
+    const int r = blockIdx.y;
+    const cd *vr = v + (int64_t)r * ld;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        s += vr[i].x * vr[i].x + vr[i].y * vr[i].y;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) part[((int64_t)r * gridDim.x + blockIdx.x) * 4 + (threadIdx.x >> 6)] = s;
+}
+
+// out[r] = sqrt(sum of the np partials of r)
+__global__ void __launch_bounds__(256) k_bnorm_final(const double *part, int np, double *out)
+{
+    __shared__ double sh[4];
+    const int r = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) s += part[(int64_t)r * np + i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[r] = sqrt((sh[0] + sh[1]) + (sh[2] + sh[3]));
+}
+
+// part[((r * gx + bx) * 4 + wave) * ncol + k] = partial of conj(V_r[:, k]) . w_r
+__global__ void __launch_bounds__(256) k_bdot_part(const cd *V, int64_t vstride, int ncol, const cd *w, int64_t wld,
+                                                   int64_t n, cd *part)
+{
+    const int r = blockIdx.y, lane = threadIdx.x & 63;
+    const int64_t slot = ((int64_t)r * gridDim.x + blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const cd *wr = w + (int64_t)r * wld;
+    for (int k = 0; k < ncol; k++) {
+        const cd *u = V + (int64_t)r * vstride + (int64_t)k * n;
+        double sx = 0.0, sy = 0.0;
+        for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+            const cd a = u[i], b = wr[i];
+            sx += a.x * b.x + a.y * b.y;
+            sy += a.x * b.y - a.y * b.x;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            sx += __shfl_xor(sx, o, 64);
+            sy += __shfl_xor(sy, o, 64);
+        }
+        if (lane == 0) part[slot * ncol + k] = {sx, sy};
+    }
+}
+
+// h[r * hs + k] = sum of the np partials of (r, k); grid (ncol, nrhs)
+__global__ void __launch_bounds__(256) k_bdot_final(const cd *part, int np, int ncol, cd *h, int hs)
+{
+    __shared__ double shx[4], shy[4];
+    const int k = blockIdx.x, r = blockIdx.y;
+    double sx = 0.0, sy = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) {
+        const cd p = part[((int64_t)r * np + i) * ncol + k];
+        sx += p.x;
+        sy += p.y;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sx += __shfl_xor(sx, o, 64);
+        sy += __shfl_xor(sy, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        shx[threadIdx.x >> 6] = sx;
+        shy[threadIdx.x >> 6] = sy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) h[(int64_t)r * hs + k] = {(shx[0] + shx[1]) + (shx[2] + shx[3]), (shy[0] + shy[1]) + (shy[2] + shy[3])};
+}
+
+// w_r += sign * V_r[:, 0:ncol) h_r
+__global__ void k_bgemvn(cd *w, int64_t wld, const cd *V, int64_t vstride, int ncol, const cd *h, int hs, int64_t n,
+                         double sign)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n) return;
+    const cd *Vr = V + (int64_t)r * vstride;
+    const cd *hr = h + (int64_t)r * hs;
+    double ax = 0.0, ay = 0.0;
+    for (int k = 0; k < ncol; k++) {
+        const cd u = Vr[i + (int64_t)k * n], c = hr[k];
+        ax += u.x * c.x - u.y * c.y;
+        ay += u.x * c.y + u.y * c.x;
+    }
+    cd *p = w + (int64_t)r * wld + i;
+    *p = {p->x + sign * ax, p->y + sign * ay};
+}
+
+// dst_r = src_r / nrm_r (0 where nrm_r = 0)
+__global__ void k_bscale(cd *dst, int64_t dld, const cd *src, int64_t sld, const double *nrm, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n) return;
+    const double s = nrm[r] > 0.0 ? 1.0 / nrm[r] : 0.0;
+    const cd a = src[(int64_t)r * sld + i];
+    dst[(int64_t)r * dld + i] = {a.x * s, a.y * s};
+}
+
+// dst_r = src_r (column copy with strides)
+__global__ void k_bcopy(cd *dst, int64_t dld, const cd *src, int64_t sld, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i < n) dst[(int64_t)r * dld + i] = src[(int64_t)r * sld + i];
+}
+
+// r = b - y
+__global__ void k_bsub(cd *r, const cd *b, const cd *y, int64_t total)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) r[i] = {b[i].x - y[i].x, b[i].y - y[i].y};
+}
+
+inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+namespace {
+
+struct BWork {
+    peec_ctx *ctx;
+    int nrhs, rps, m;
+    int64_t N;
+    DevBuf<cd> V, B, X, R, W, T, hbuf, part;
+    DevBuf<double> nrm, npart;
+    int alloc()
+    {
+        const size_t nr = (size_t)nrhs, n = (size_t)N;
+        if (!V.alloc(nr * (size_t)(m + 1) * n) || !B.alloc(nr * n) || !X.alloc(nr * n) || !R.alloc(nr * n) ||
+            !W.alloc(nr * n) || !T.alloc(nr * n) || !hbuf.alloc(nr * 2 * (size_t)(m + 1)) ||
+            !part.alloc(nr * kGx * 4 * (size_t)(m + 1)) || !nrm.alloc(nr) || !npart.alloc(nr * kGx * 4)) {
+            peec_set_error("peec_gmres_batched: out of device memory for %d right-hand sides x %d basis vectors", nrhs,
+                           m + 1);
+            return PEEC_ERR_NOMEM;
+        }
+        return PEEC_OK;
+    }
+    // nrm[r] = ||v_r||, v with stride ld; copies to host
+    int norms(const cd *v, int64_t ld, std::vector<double> &out)
+    {
+        hipLaunchKernelGGL(k_bnorm_part, dim3(kGx, nrhs), dim3(256), 0, ctx->stream, v, ld, N, npart.p);
+        hipLaunchKernelGGL(k_bnorm_final, dim3(nrhs), dim3(256), 0, ctx->stream, npart.p, kGx * 4, nrm.p);
+        PEEC_HIP_TRY(hipGetLastError());
+        out.resize((size_t)nrhs);
+        PEEC_HIP_TRY(hipMemcpyAsync(out.data(), nrm.p, (size_t)nrhs * 8, hipMemcpyDeviceToHost, ctx->stream));
+        PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return PEEC_OK;
+    }
+    // h = V[:, 0:ncol)' w into hbuf at column offset hoff (stride hs = 2(m+1))
+    void dot(int ncol, const cd *w, int64_t wld, int hoff)
+    {
+        hipLaunchKernelGGL(k_bdot_part, dim3(kGx, nrhs), dim3(256), 0, ctx->stream, V.p, (int64_t)(m + 1) * N, ncol, w,
+                           wld, N, part.p);
+        hipLaunchKernelGGL(k_bdot_final, dim3(ncol, nrhs), dim3(256), 0, ctx->stream, part.p, kGx * 4, ncol,
+                           hbuf.p + hoff, 2 * (m + 1));
+    }
+    void gemv(cd *w, int64_t wld, int ncol, const cd *h, int hs, double sign)
+    {
+        hipLaunchKernelGGL(k_bgemvn, dim3(nb256(N), nrhs), dim3(256), 0, ctx->stream, w, wld, V.p,
+                           (int64_t)(m + 1) * N, ncol, h, hs, N, sign);
+    }
+    // W = A x (x with stride N), in FP64 or with FP32 convolutions
+    int matvec(const cd *x, bool fp32) { return matvec_device(ctx, x, W.p, nrhs, N, fp32, rps); }
+};
+
+}  // namespace
+
+extern "C" int peec_gmres_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const peec_c128 *b, double tol,
+                                  int restart, int max_cycles, int mixed, double inner_tol, peec_c128 *x, int *flag,
+                                  double *relres, int *iters, int *cycles)
+{
+    if (!ctx || nrhs < 1 || rhs_per_slot < 1 || !b || !x || restart < 1 || max_cycles < 1) {
+        peec_set_error("peec_gmres_batched: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->op_ready || !ctx->freq_ready || !ctx->lu_ready) {
+        peec_set_error("peec_gmres_batched: operator, frequency slots and factors must be set first");
+        return PEEC_ERR_STATE;
+    }
+    if ((nrhs + rhs_per_slot - 1) / rhs_per_slot > ctx->nslots) {
+        peec_set_error("peec_gmres_batched: %d right-hand sides at %d per slot need more than the %d slots", nrhs,
+                       rhs_per_slot, ctx->nslots);
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    BWork W;
+    W.ctx = ctx;
+    W.nrhs = nrhs;
+    W.rps = rhs_per_slot;
+    W.m = restart;
+    W.N = ctx->m + ctx->ns + ctx->n;
+    const int64_t N = W.N;
+    int rc = W.alloc();
+    if (rc) return rc;
+    const size_t bytes = (size_t)nrhs * (size_t)N * 16;
+    PEEC_HIP_TRY(hipMemcpyAsync(W.B.p, b, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PEEC_HIP_TRY(hipMemcpyAsync(W.X.p, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<double> nb, beta;
+    if ((rc = W.norms(W.B.p, N, nb))) return rc;
+    // r = b - A x
+    auto residual = [&]() -> int {
+        int e = W.matvec(W.X.p, false);
+        if (e) return e;
+        hipLaunchKernelGGL(k_bsub, dim3(nb256((int64_t)nrhs * N)), dim3(256), 0, ctx->stream, W.R.p, W.B.p, W.W.p,
+                           (int64_t)nrhs * N);
+        return W.norms(W.R.p, N, beta);
+    };
+    if ((rc = residual())) return rc;
+    std::vector<char> active((size_t)nrhs);
+    std::vector<int> it((size_t)nrhs, 0), cyc((size_t)nrhs, 0), fl((size_t)nrhs, 1), stall((size_t)nrhs, 0);
+    for (int r = 0; r < nrhs; r++) {
+        active[(size_t)r] = nb[(size_t)r] > 0.0 && beta[(size_t)r] > tol * nb[(size_t)r];
+        if (!active[(size_t)r]) fl[(size_t)r] = 0;
+    }
+    const int m = restart, hs = 2 * (m + 1);
+    std::vector<cd> hh((size_t)nrhs * hs);
+    std::vector<double> hn;
+    std::vector<std::vector<zc>> H((size_t)nrhs), G((size_t)nrhs);  // Hessenberg (col-major m+1 x m), rhs g
+    std::vector<std::vector<zc>> cs((size_t)nrhs), sn((size_t)nrhs);
+    std::vector<cd> y((size_t)nrhs * hs);
+    int ncycle = 0;
+    while (std::any_of(active.begin(), active.end(), [](char c) { return c != 0; }) && ncycle < max_cycles) {
+        if (ctx->stop_requested()) return PEEC_STOPPED;
+        ncycle++;
+        // V[:, 0] = r / beta
+        PEEC_HIP_TRY(hipMemcpyAsync(W.nrm.p, beta.data(), (size_t)nrhs * 8, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_bscale, dim3(nb256(N), nrhs), dim3(256), 0, ctx->stream, W.V.p, (int64_t)(m + 1) * N,
+                           W.R.p, N, W.nrm.p, N);
+        std::vector<char> inner((size_t)nrhs);
+        std::vector<int> kdone((size_t)nrhs, 0);
+        std::vector<double> eta((size_t)nrhs);
+        for (int r = 0; r < nrhs; r++) {
+            inner[(size_t)r] = active[(size_t)r];
+            H[(size_t)r].assign((size_t)(m + 1) * m, zc(0, 0));
+            G[(size_t)r].assign((size_t)m + 1, zc(0, 0));
+            G[(size_t)r][0] = beta[(size_t)r];
+            cs[(size_t)r].assign((size_t)m, zc(0, 0));
+            sn[(size_t)r].assign((size_t)m, zc(0, 0));
+            // inner target: the final tolerance, or (mixed) at most 5 digits per cycle
+            const double target = tol * nb[(size_t)r] / std::max(beta[(size_t)r], 1e-300);
+            eta[(size_t)r] = mixed ? std::max(target, inner_tol > 0 ? inner_tol : 1e-5) : std::max(target, 1e-15);
+        }
+        for (int j = 0; j < m; j++) {
+            // w = A_in v_j
+            hipLaunchKernelGGL(k_bcopy, dim3(nb256(N), nrhs), dim3(256), 0, ctx->stream, W.T.p, N,
+                               W.V.p + (int64_t)j * N, (int64_t)(m + 1) * N, N);
+            if ((rc = W.matvec(W.T.p, mixed != 0))) return rc;
+            // CGS2: two passes of h = V' w, w -= V h
+            W.dot(j + 1, W.W.p, N, 0);
+            W.gemv(W.W.p, N, j + 1, W.hbuf.p, hs, -1.0);
+            W.dot(j + 1, W.W.p, N, m + 1);
+            W.gemv(W.W.p, N, j + 1, W.hbuf.p + (m + 1), hs, -1.0);
+            PEEC_HIP_TRY(hipGetLastError());
+            PEEC_HIP_TRY(hipMemcpyAsync(hh.data(), W.hbuf.p, hh.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
+            if ((rc = W.norms(W.W.p, N, hn))) return rc;
+            hipLaunchKernelGGL(k_bscale, dim3(nb256(N), nrhs), dim3(256), 0, ctx->stream,
+                               W.V.p + (int64_t)(j + 1) * N, (int64_t)(m + 1) * N, W.W.p, N, W.nrm.p, N);
+            bool any = false;
+            for (int r = 0; r < nrhs; r++) {
+                if (!inner[(size_t)r]) continue;
+                std::vector<zc> &Hr = H[(size_t)r];
+                zc *col = Hr.data() + (size_t)j * (m + 1);
+                for (int k = 0; k <= j; k++) {
+                    const cd a = hh[(size_t)r * hs + k], c = hh[(size_t)r * hs + (m + 1) + k];
+                    col[k] = zc(a.x + c.x, a.y + c.y);
+                }
+                col[j + 1] = hn[(size_t)r];
+                // previous rotations, then a new one zeroing col[j + 1]
+                for (int k = 0; k < j; k++) {
+                    const zc t = cs[(size_t)r][k] * col[k] + sn[(size_t)r][k] * col[k + 1];
+                    col[k + 1] = -std::conj(sn[(size_t)r][k]) * col[k] + std::conj(cs[(size_t)r][k]) * col[k + 1];
+                    col[k] = t;
+                }
+                const double a = std::abs(col[j]), bb = std::abs(col[j + 1]);
+                zc c(1, 0), s(0, 0);
+                if (bb > 0) {
+                    const double rr = std::hypot(a, bb);
+                    if (a == 0) {
+                        c = 0;
+                        s = std::conj(col[j + 1]) / bb;
+                    } else {
+                        c = zc(a / rr, 0);
+                        s = (col[j] / a) * std::conj(col[j + 1]) / rr;
+                    }
+                }
+                cs[(size_t)r][j] = c;
+                sn[(size_t)r][j] = s;
+                col[j] = c * col[j] + s * col[j + 1];
+                col[j + 1] = 0;
+                G[(size_t)r][j + 1] = -std::conj(s) * G[(size_t)r][j];
+                G[(size_t)r][j] = c * G[(size_t)r][j];
+                kdone[(size_t)r] = j + 1;
+                it[(size_t)r]++;
+                if (std::abs(G[(size_t)r][j + 1]) <= eta[(size_t)r] * beta[(size_t)r] || hn[(size_t)r] == 0.0)
+                    inner[(size_t)r] = 0;
+                else
+                    any = true;
+            }
+            if (!any) break;
+        }
+        // x += V y, y = R \ g per right-hand side (zero where inactive)
+        std::fill(y.begin(), y.end(), cd{0.0, 0.0});
+        for (int r = 0; r < nrhs; r++) {
+            const int k = kdone[(size_t)r];
+            if (!active[(size_t)r] || k == 0) continue;
+            std::vector<zc> yy((size_t)k);
+            const std::vector<zc> &Hr = H[(size_t)r];
+            for (int i = k - 1; i >= 0; i--) {
+                zc t = G[(size_t)r][i];
+                for (int l = i + 1; l < k; l++) t -= Hr[(size_t)l * (m + 1) + i] * yy[(size_t)l];
+                yy[(size_t)i] = t / Hr[(size_t)i * (m + 1) + i];
+            }
+            for (int i = 0; i < k; i++) y[(size_t)r * hs + i] = {yy[(size_t)i].real(), yy[(size_t)i].imag()};
+        }
+        // only basis columns 0 .. kmax-1 were written this cycle (for every
+        // right-hand side, converged ones included)
+        const int kmax = *std::max_element(kdone.begin(), kdone.end());
+        PEEC_HIP_TRY(hipMemcpyAsync(W.hbuf.p, y.data(), y.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+        if (kmax > 0) W.gemv(W.X.p, N, kmax, W.hbuf.p, hs, 1.0);
+        PEEC_HIP_TRY(hipGetLastError());
+        const std::vector<double> beta_old = beta;
+        if ((rc = residual())) return rc;
+        for (int r = 0; r < nrhs; r++) {
+            if (!active[(size_t)r]) continue;
+            cyc[(size_t)r]++;
+            if (beta[(size_t)r] <= tol * nb[(size_t)r]) {
+                active[(size_t)r] = 0;
+                fl[(size_t)r] = 0;
+            } else if (beta[(size_t)r] > 0.5 * beta_old[(size_t)r]) {
+                // refinement no longer pays: two cycles without halving the residual
+                if (++stall[(size_t)r] >= 2) {
+                    active[(size_t)r] = 0;
+                    fl[(size_t)r] = 3;
+                }
+            } else {
+                stall[(size_t)r] = 0;
+            }
+        }
+    }
+    PEEC_HIP_TRY(hipMemcpyAsync(x, W.X.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int r = 0; r < nrhs; r++) {
+        if (flag) flag[r] = fl[(size_t)r];
+        if (relres) relres[r] = nb[(size_t)r] > 0 ? beta[(size_t)r] / nb[(size_t)r] : 0.0;
+        if (iters) iters[r] = it[(size_t)r];
+        if (cycles) cycles[r] = cyc[(size_t)r];
+    }
+    return PEEC_OK;
+}
+
+// tn_r = M^-1 [0; 0; is_r] for each right-hand side (precond_3_3_Kt_fft!,
+// FFT_solver_QS_S_type.jl:237-262, batched over slots)
+extern "C" int peec_precond_rhs_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const double *is, peec_c128 *tn)
+{
+    if (!ctx || nrhs < 1 || rhs_per_slot < 1 || !is || !tn) {
+        peec_set_error("peec_precond_rhs_batched: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->op_ready || !ctx->freq_ready || !ctx->lu_ready) {
+        peec_set_error("peec_precond_rhs_batched: operator, frequency slots and factors must be set first");
+        return PEEC_ERR_STATE;
+    }
+    if ((nrhs + rhs_per_slot - 1) / rhs_per_slot > ctx->nslots) {
+        peec_set_error("peec_precond_rhs_batched: not enough frequency slots");
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n, N = m + ns + n;
+    std::vector<cd> h((size_t)(N * nrhs), cd{0.0, 0.0});
+    for (int r = 0; r < nrhs; r++)
+        for (int64_t v = 0; v < n; v++) h[(size_t)(r * N + m + ns + v)] = {is[r * n + v], 0.0};
+    DevBuf<cd> Y;
+    if (!Y.alloc(h.size())) return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemcpyAsync(Y.p, h.data(), h.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+    int rc = apply_precond(ctx, Y.p, nrhs, N, rhs_per_slot);
+    if (rc) return rc;
+    PEEC_HIP_TRY(hipMemcpyAsync(tn, Y.p, h.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PEEC_OK;
+}

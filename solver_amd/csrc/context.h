@@ -140,6 +140,8 @@ constexpr int kTrsvSeg = 256;       // entries summed by one wavefront (4 per la
 struct TriFactor {
     int64_t n = 0;
     int nblocks = 0, max_block = 0;
+    int nslots = 1;                          // value sets (one per frequency slot)
+    int64_t ob_n = 0, dense_n = 0;           // values per slot in ob_val / dense (diag: n)
     int64_t nseg = 0;
     double nnz = 0.0;                        // stored off-diagonal entries + diagonal
     std::vector<int32_t> level_ptr;          // host: blocks of level l are binfo entries level_ptr[l] ..
@@ -149,8 +151,8 @@ struct TriFactor {
     DevBuf<int64_t> seg_k0, seg_k1;          // entry range of each segment
     DevBuf<int32_t> row_seg;                 // per row: first and end segment
     DevBuf<int32_t> ob_idx;
-    DevBuf<double2_t> ob_val;
-    DevBuf<double2_t> dense, diag;           // diag holds 1 / T[i, i]
+    DevBuf<double2_t> ob_val;                // nslots x ob_n
+    DevBuf<double2_t> dense, diag;           // nslots x dense_n, nslots x n; diag holds 1 / T[i, i]
 };
 
 // Per-kernel timing with HIP events on the context's stream.
@@ -199,16 +201,24 @@ struct peec_ctx {
     std::vector<double> hA_val, hG_val, h_invP;
     void release_operator();
 
-    // ---- frequency
-    bool freq_ready = false;
-    double w = 0.0;
-    DevBuf<double2_t> DZ, invZ;  // m
-    CsrC Yle;                    // n x n
-    std::vector<int32_t> hYle_ptr, hYle_idx;  // host copy (CSR)
-    std::vector<double2_t> hYle_val;
-
+    // ---- frequency: one or more slots; right-hand side r of a batched call
+    // uses slot r / rhs_per_slot
+    bool freq_ready = false;        // every slot set
+    int nslots = 1;
+    double w = 0.0;                 // slot 0
+    std::vector<double> hw;         // per slot
+    DevBuf<double> wv;              // per slot
+    DevBuf<double2_t> DZ, invZ;     // nslots x m
+    CsrC Yle;                       // nslots consecutive CSRs: ptr nslots x (n + 1), offsets into idx / val
+    struct HostCsr {
+        std::vector<int32_t> ptr, idx;
+        std::vector<double2_t> val;
+    };
+    std::vector<HostCsr> hYle;      // per slot
+    std::vector<char> slot_set, slot_lu;
+    int upload_yle();
     // ---- LU
-    bool lu_ready = false;
+    bool lu_ready = false;          // every slot factored
     TriFactor Lf, Uf;
     DevBuf<int32_t> perm_r, perm_c;  // inverse row / column permutations (0-based)
     DevBuf<double> Rs;               // optional row scaling of SS
@@ -241,6 +251,7 @@ struct TriSources {
 };
 int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std::vector<TriEntry> &ent,
                const std::vector<int64_t> &diag_src, TriFactor *f, TriSources *src, const char *what);
+int tri_set_slots(TriFactor *f, int nslots);  // reallocates the value sets (contents undefined)
 std::vector<int32_t> tri_chain_blocks(const peec_csc_c128 *L, int cap);
 int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f, const char *what);
-int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs);
+int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs, int rps);

@@ -19,7 +19,7 @@
 
 typedef std::complex<double> zc;
 
-int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32);
+int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps);
 int sync_stream(peec_ctx *ctx);
 
 namespace {
@@ -250,7 +250,7 @@ extern "C" int peec_gmres(peec_ctx *ctx, const peec_c128 *b_h, double tol, int m
     GmresWork W{ctx, n, inner};
     int rc = W.alloc();
     if (rc) return rc;
-    auto matvec = [&](const cd *xin, cd *yout) { return matvec_device(ctx, xin, yout, 1, n, false); };
+    auto matvec = [&](const cd *xin, cd *yout) { return matvec_device(ctx, xin, yout, 1, n, false, 1); };
 
     PEEC_HIP_TRY(hipMemcpyAsync(W.b.p, b_h, sizeof(cd) * n, hipMemcpyHostToDevice, ctx->stream));
     PEEC_HIP_TRY(hipMemcpyAsync(W.x.p, x_h, sizeof(cd) * n, hipMemcpyHostToDevice, ctx->stream));

@@ -646,10 +646,12 @@ extern "C" int peec_lu_analyze(peec_ctx *ctx, int64_t n, const int64_t *order, i
         }
         if ((rc = tri_layout(n, false, blk, ent, dsrc, &ctx->Uf, &us, "peec_lu_analyze (U)"))) return rc;
     }
-    // L is unit lower triangular: its inverse diagonal is 1 for good
+    // one value set per frequency slot; L is unit lower triangular, so its
+    // inverse diagonal is 1 in every slot for good
+    if ((rc = tri_set_slots(&ctx->Lf, ctx->nslots)) || (rc = tri_set_slots(&ctx->Uf, ctx->nslots))) return rc;
     {
-        std::vector<double2_t> ones((size_t)n, double2_t{1.0, 0.0});
-        PEEC_HIP_TRY(hipMemcpy(ctx->Lf.diag.p, ones.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+        std::vector<double2_t> ones((size_t)(n * ctx->nslots), double2_t{1.0, 0.0});
+        PEEC_HIP_TRY(hipMemcpy(ctx->Lf.diag.p, ones.data(), ones.size() * 16, hipMemcpyHostToDevice));
     }
     // ---- uploads
     if ((rc = upload(P.l_ob_src, ls.ob)) || (rc = upload(P.l_dense_src, ls.dense)) || (rc = upload(P.u_ob_src, us.ob)) ||
@@ -669,12 +671,13 @@ extern "C" int peec_lu_analyze(peec_ctx *ctx, int64_t n, const int64_t *order, i
     PEEC_HIP_TRY(hipMemcpy(ctx->perm_c.p, P.pos.data(), (size_t)n * 4, hipMemcpyHostToDevice));
     ctx->has_Rs = false;
     ctx->lu_ready = false;
+    ctx->slot_lu.assign((size_t)ctx->nslots, 0);
     ctx->lu_layout_from_plan = true;
     P.ready = true;
     return PEEC_OK;
 }
 
-extern "C" int peec_lu_factor(peec_ctx *ctx)
+extern "C" int peec_lu_factor_slot(peec_ctx *ctx, int slot)
 {
     if (!ctx) {
         peec_set_error("peec_lu_factor: null context");
@@ -686,34 +689,36 @@ extern "C" int peec_lu_factor(peec_ctx *ctx)
                        "peec_lu_set_factors)");
         return PEEC_ERR_STATE;
     }
-    if (!ctx->freq_ready) {
-        peec_set_error("peec_lu_factor: call peec_frequency_set first");
+    if (slot < 0 || slot >= ctx->nslots || (int)ctx->slot_set.size() != ctx->nslots || !ctx->slot_set[(size_t)slot]) {
+        peec_set_error("peec_lu_factor: frequency slot %d is not set (peec_frequency_set / _slot)", slot);
         return PEEC_ERR_STATE;
     }
+    const peec_ctx::HostCsr &Y = ctx->hYle[(size_t)slot];
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
     // Yle entries -> pool targets (host; a handful of port / lumped entries)
     std::vector<int64_t> ytgt;
     std::vector<double2_t> yval;
     for (int64_t i = 0; i < P.n; i++)
-        for (int32_t k = ctx->hYle_ptr[(size_t)i]; k < ctx->hYle_ptr[(size_t)i + 1]; k++) {
-            const int32_t pi = P.pos[(size_t)i], pj = P.pos[(size_t)ctx->hYle_idx[(size_t)k]];
+        for (int32_t k = Y.ptr[(size_t)i]; k < Y.ptr[(size_t)i + 1]; k++) {
+            const int32_t pi = P.pos[(size_t)i], pj = P.pos[(size_t)Y.idx[(size_t)k]];
             const int K = P.front_of[(size_t)std::min(pi, pj)];
             const int li = lu_local(P, K, pi), lj = lu_local(P, K, pj);
             if (li < 0 || lj < 0) {
                 peec_set_error("peec_lu_factor: Yle couples nodes %lld and %d that share no front; put port and "
                                "lumped-element nodes in the root front",
-                               (long long)i + 1, ctx->hYle_idx[(size_t)k] + 1);
+                               (long long)i + 1, Y.idx[(size_t)k] + 1);
                 return PEEC_ERR_ARG;
             }
             const LuFrontInfo &F = P.fronts[(size_t)K];
             ytgt.push_back(F.off + (int64_t)lj * F.f + li);
-            yval.push_back(ctx->hYle_val[(size_t)k]);
+            yval.push_back(Y.val[(size_t)k]);
         }
     int pi = ctx->prof_begin(PK_LU, 0.0);
     cd *pool = P.pool.p;
     hipLaunchKernelGGL(k_lu_zero, dim3(nblk256(P.pool_fronts)), dim3(256), 0, ctx->stream, pool, P.pool_fronts);
     hipLaunchKernelGGL(k_lu_ss, dim3(nblk256(P.nss)), dim3(256), 0, ctx->stream, P.ss_tgt.p, P.ss_cptr.p, P.ss_cedge.p,
-                       P.ss_ccoef.p, P.ss_gsum.p, ctx->invZ.p, ctx->w, pool, P.nss);
+                       P.ss_ccoef.p, P.ss_gsum.p, ctx->invZ.p + (int64_t)slot * ctx->m, ctx->hw[(size_t)slot], pool,
+                       P.nss);
     DevBuf<int64_t> dyt;
     DevBuf<double2_t> dyv;
     if (!ytgt.empty()) {
@@ -736,25 +741,32 @@ extern "C" int peec_lu_factor(peec_ctx *ctx)
                                pool);
     }
     PEEC_HIP_TRY(hipGetLastError());
-    // gather into the triangular-solve layouts
+    // gather into the slot's value sets of the triangular-solve layouts
     TriFactor &Lf = ctx->Lf, &Uf = ctx->Uf;
-    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256((int64_t)Lf.ob_val.n)), dim3(256), 0, ctx->stream, P.l_ob_src.p,
-                       Lf.ob_val.p, (int64_t)std::min(Lf.ob_val.n, P.l_ob_src.n), pool, 0);
-    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256((int64_t)Lf.dense.n)), dim3(256), 0, ctx->stream, P.l_dense_src.p,
-                       Lf.dense.p, (int64_t)std::min(Lf.dense.n, P.l_dense_src.n), pool, 0);
-    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256((int64_t)Uf.ob_val.n)), dim3(256), 0, ctx->stream, P.u_ob_src.p,
-                       Uf.ob_val.p, (int64_t)std::min(Uf.ob_val.n, P.u_ob_src.n), pool, 0);
-    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256((int64_t)Uf.dense.n)), dim3(256), 0, ctx->stream, P.u_dense_src.p,
-                       Uf.dense.p, (int64_t)std::min(Uf.dense.n, P.u_dense_src.n), pool, 0);
-    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(P.n)), dim3(256), 0, ctx->stream, P.u_diag_src.p, Uf.diag.p, P.n,
-                       pool, 1);
+    if (Lf.nslots <= slot || Uf.nslots <= slot) {
+        peec_set_error("peec_lu_factor: factor layouts hold %d value sets, slot %d requested", Lf.nslots, slot);
+        return PEEC_ERR_STATE;
+    }
+    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Lf.ob_n)), dim3(256), 0, ctx->stream, P.l_ob_src.p,
+                       Lf.ob_val.p + slot * Lf.ob_n, Lf.ob_n, pool, 0);
+    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Lf.dense_n)), dim3(256), 0, ctx->stream, P.l_dense_src.p,
+                       Lf.dense.p + slot * Lf.dense_n, Lf.dense_n, pool, 0);
+    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Uf.ob_n)), dim3(256), 0, ctx->stream, P.u_ob_src.p,
+                       Uf.ob_val.p + slot * Uf.ob_n, Uf.ob_n, pool, 0);
+    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Uf.dense_n)), dim3(256), 0, ctx->stream, P.u_dense_src.p,
+                       Uf.dense.p + slot * Uf.dense_n, Uf.dense_n, pool, 0);
+    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(P.n)), dim3(256), 0, ctx->stream, P.u_diag_src.p,
+                       Uf.diag.p + slot * P.n, P.n, pool, 1);
     PEEC_HIP_TRY(hipGetLastError());
     ctx->prof_end(pi);
     // the Yle staging buffers are freed on return: wait for their use
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->lu_ready = true;
+    ctx->slot_lu[(size_t)slot] = 1;
+    ctx->lu_ready = std::all_of(ctx->slot_lu.begin(), ctx->slot_lu.end(), [](char c) { return c != 0; });
     return PEEC_OK;
 }
+
+extern "C" int peec_lu_factor(peec_ctx *ctx) { return peec_lu_factor_slot(ctx, 0); }
 
 // Read back one front of the last factorization (tests): f x f column-major,
 // and the factor position of each of its f rows/columns.

@@ -61,17 +61,19 @@ __global__ void k_zero(cd *y, int64_t n, int nrhs, int64_t ld)
 }
 
 // Y1 = j w Y1 + DZ .* I + A Phi           (compute_Matrix_vector_fft.jl:29-31)
+// Right-hand side r takes w and DZ of frequency slot r / rps.
 __global__ void k_y1(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const int32_t *ai, const double *av,
-                     double w, int64_t m, int64_t phi0, int nrhs, int64_t ld)
+                     const double *wv, int64_t m, int64_t phi0, int nrhs, int64_t ld, int rps)
 {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m) return;
     for (int r = 0; r < nrhs; r++) {
+        const int sl = r / rps;
         const cd *xr = x + r * ld;
         cd s = {0.0, 0.0};
         for (int k = ap[e]; k < ap[e + 1]; k++) s = cadd(s, cscal(xr[phi0 + ai[k]], av[k]));
-        cd v = jw(y[e + r * ld], w);
-        v = cadd(v, cmul(DZ[e], xr[e]));
+        cd v = jw(y[e + r * ld], wv[sl]);
+        v = cadd(v, cmul(DZ[e + sl * m], xr[e]));
         y[e + r * ld] = cadd(v, s);
     }
 }
@@ -91,14 +93,17 @@ __global__ void k_y2(cd *y, const cd *x, const int32_t *gp, const int32_t *gi, c
 }
 
 // Y3 = -A' I + Yle Phi + j w Gamma Q      (:69-77)
-__global__ void k_y3(cd *y, const cd *x, const int32_t *atp, const int32_t *ati, const double *atv, const int32_t *yp,
-                     const int32_t *yi, const cd *yv, const int32_t *gp, const int32_t *gi, const double *gv, double w,
-                     int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld)
+__global__ void k_y3(cd *y, const cd *x, const int32_t *atp, const int32_t *ati, const double *atv, const int32_t *yp0,
+                     const int32_t *yi, const cd *yv, const int32_t *gp, const int32_t *gi, const double *gv,
+                     const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, int rps)
 {
     int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
     for (int r = 0; r < nrhs; r++) {
+        const int sl = r / rps;
+        const int32_t *yp = yp0 + sl * (n + 1);
+        const double w = wv[sl];
         const cd *xr = x + r * ld;
         cd a = {0.0, 0.0};
         for (int k = atp[v]; k < atp[v + 1]; k++) a = cadd(a, cscal(xr[ati[k]], atv[k]));
@@ -114,14 +119,18 @@ __global__ void k_y3(cd *y, const cd *x, const int32_t *atp, const int32_t *ati,
 
 // t = A' (Z^-1 X1) - j w Gamma (P^-1 X2) + X3, scattered through the row
 // permutation and row scaling of the LU: b[i] = Rs[p[i]] t[p[i]]
-__global__ void k_pre(const cd *y, const cd *invZ, const double *invP, const int32_t *atp, const int32_t *ati,
+__global__ void k_pre(const cd *y, const cd *invZ0, const double *invP, const int32_t *atp, const int32_t *ati,
                       const double *atv, const int32_t *gp, const int32_t *gi, const double *gv, const int32_t *pinv,
-                      const double *Rs, cd *b, double w, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld)
+                      const double *Rs, cd *b, const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs,
+                      int64_t ld, int rps)
 {
     int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
     for (int r = 0; r < nrhs; r++) {
+        const int sl = r / rps;
+        const cd *invZ = invZ0 + sl * m;
+        const double w = wv[sl];
         const cd *yr = y + r * ld;
         cd a = {0.0, 0.0};
         for (int k = atp[v]; k < atp[v + 1]; k++) a = cadd(a, cscal(cmul(invZ[ati[k]], yr[ati[k]]), atv[k]));
@@ -136,14 +145,15 @@ __global__ void k_pre(const cd *y, const cd *invZ, const double *invP, const int
 
 // Y1 = Z^-1 (X1 - A M), Y2 = P^-1 (X2 + Gamma' M), Y3 = M (M = Q z, i.e.
 // M[q[j]] = z[j]; read through the inverse column permutation)
-__global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ, const double *invP, const int32_t *ap,
+__global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ0, const double *invP, const int32_t *ap,
                        const int32_t *ai, const double *av, const int32_t *gtp, const int32_t *gti, const double *gtv,
-                       int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld)
+                       int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, int rps)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t N = m + ns + n;
     if (i >= N) return;
     for (int r = 0; r < nrhs; r++) {
+        const cd *invZ = invZ0 + (r / rps) * m;
         const cd *zr = z + r * n;
         cd *yr = y + r * ld;
         if (i < m) {
@@ -381,7 +391,73 @@ extern "C" int peec_operator_set(peec_ctx *ctx, int64_t mx, int64_t my, int64_t 
     return PEEC_OK;
 }
 
-extern "C" int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, const peec_csc_c128 *Yle)
+// All slots' Yle as consecutive CSRs (ptr offsets absolute).
+int peec_ctx::upload_yle()
+{
+    std::vector<int32_t> ptr, idx;
+    std::vector<double2_t> val;
+    for (int sl = 0; sl < nslots; sl++) {
+        const HostCsr &h = hYle[(size_t)sl];
+        const int32_t base = (int32_t)idx.size();
+        for (int64_t i = 0; i <= n; i++) ptr.push_back(base + (h.ptr.empty() ? 0 : h.ptr[(size_t)i]));
+        if (!h.ptr.empty()) {
+            idx.insert(idx.end(), h.idx.begin(), h.idx.begin() + h.ptr[(size_t)n]);
+            val.insert(val.end(), h.val.begin(), h.val.begin() + h.ptr[(size_t)n]);
+        }
+    }
+    if (idx.empty()) {
+        idx.push_back(0);
+        val.push_back({0.0, 0.0});
+    }
+    Yle.nrow = Yle.ncol = n;
+    if (!Yle.ptr.alloc(ptr.size()) || !Yle.idx.alloc(idx.size()) || !Yle.val.alloc(val.size())) return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemcpy(Yle.ptr.p, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(Yle.idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(Yle.val.p, val.data(), val.size() * 16, hipMemcpyHostToDevice));
+    return PEEC_OK;
+}
+
+extern "C" int peec_frequency_slots(peec_ctx *ctx, int nslots)
+{
+    if (!ctx || nslots < 1) {
+        peec_set_error("peec_frequency_slots: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->op_ready) {
+        peec_set_error("peec_frequency_slots: call peec_operator_set first");
+        return PEEC_ERR_STATE;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int64_t m = std::max<int64_t>(ctx->m, 1);
+    if (nslots > 1 && !ctx->lu_layout_from_plan) {
+        peec_set_error("peec_frequency_slots: several slots need the device factorization (peec_lu_analyze)");
+        return PEEC_ERR_STATE;
+    }
+    if (!ctx->wv.alloc((size_t)nslots) || !ctx->DZ.alloc((size_t)(m * nslots)) ||
+        !ctx->invZ.alloc((size_t)(m * nslots))) {
+        peec_set_error("peec_frequency_slots: out of device memory for %d slots", nslots);
+        return PEEC_ERR_NOMEM;
+    }
+    ctx->nslots = nslots;
+    ctx->hw.assign((size_t)nslots, 0.0);
+    ctx->hYle.assign((size_t)nslots, peec_ctx::HostCsr());
+    ctx->slot_set.assign((size_t)nslots, 0);
+    ctx->slot_lu.assign((size_t)nslots, 0);
+    ctx->freq_ready = false;
+    ctx->lu_ready = false;
+    int rc;
+    if (ctx->lu_layout_from_plan) {
+        if ((rc = tri_set_slots(&ctx->Lf, nslots)) || (rc = tri_set_slots(&ctx->Uf, nslots))) return rc;
+        // L is unit lower triangular: 1 / L[i, i] = 1 in every slot
+        std::vector<double2_t> ones((size_t)(ctx->n * nslots), double2_t{1.0, 0.0});
+        PEEC_HIP_TRY(hipMemcpy(ctx->Lf.diag.p, ones.data(), ones.size() * 16, hipMemcpyHostToDevice));
+    }
+    return PEEC_OK;
+}
+
+extern "C" int peec_frequency_set_slot(peec_ctx *ctx, int slot, double w, const peec_c128 *DZ,
+                                       const peec_csc_c128 *Yle)
 {
     if (!ctx || !DZ || !Yle) {
         peec_set_error("peec_frequency_set: null argument");
@@ -391,6 +467,10 @@ extern "C" int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, 
         peec_set_error("peec_frequency_set: call peec_operator_set first");
         return PEEC_ERR_STATE;
     }
+    if (slot < 0 || slot >= ctx->nslots || (int)ctx->slot_set.size() != ctx->nslots) {
+        peec_set_error("peec_frequency_set: slot %d outside the %d slots of peec_frequency_slots", slot, ctx->nslots);
+        return PEEC_ERR_ARG;
+    }
     if (Yle->nrow != ctx->n || Yle->ncol != ctx->n) {
         peec_set_error("peec_frequency_set: Yle must be n x n");
         return PEEC_ERR_ARG;
@@ -398,7 +478,6 @@ extern "C" int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, 
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
     const int64_t m = ctx->m, n = ctx->n;
-    ctx->w = w;
     std::vector<double2_t> dz((size_t)m), iz((size_t)m);
     for (int64_t e = 0; e < m; e++) {
         dz[(size_t)e] = {DZ[e].re, DZ[e].im};
@@ -410,10 +489,11 @@ extern "C" int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, 
         // 1 ./ DZ (FFT_solver_QS_S_type.jl:115)
         iz[(size_t)e] = {DZ[e].re / d, -DZ[e].im / d};
     }
-    if (!ctx->DZ.alloc((size_t)std::max<int64_t>(m, 1)) || !ctx->invZ.alloc((size_t)std::max<int64_t>(m, 1)))
-        return PEEC_ERR_NOMEM;
-    PEEC_HIP_TRY(hipMemcpy(ctx->DZ.p, dz.data(), (size_t)m * 16, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(ctx->invZ.p, iz.data(), (size_t)m * 16, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(ctx->DZ.p + (int64_t)slot * m, dz.data(), (size_t)m * 16, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(ctx->invZ.p + (int64_t)slot * m, iz.data(), (size_t)m * 16, hipMemcpyHostToDevice));
+    ctx->hw[(size_t)slot] = w;
+    if (slot == 0) ctx->w = w;
+    PEEC_HIP_TRY(hipMemcpy(ctx->wv.p, ctx->hw.data(), ctx->hw.size() * 8, hipMemcpyHostToDevice));
     // Yle CSC -> CSR
     int64_t nnz = Yle->colptr[n] - 1;
     std::vector<int32_t> ptr((size_t)n + 1, 0), idx((size_t)std::max<int64_t>(nnz, 1));
@@ -429,18 +509,30 @@ extern "C" int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, 
             idx[(size_t)p] = (int32_t)c;
             val[(size_t)p] = {Yle->nzval[k].re, Yle->nzval[k].im};
         }
-    ctx->hYle_ptr = ptr;
-    ctx->hYle_idx = idx;
-    ctx->hYle_val = val;
-    ctx->Yle.nrow = ctx->Yle.ncol = n;
-    if (!ctx->Yle.ptr.alloc(ptr.size()) || !ctx->Yle.idx.alloc(idx.size()) || !ctx->Yle.val.alloc(val.size()))
-        return PEEC_ERR_NOMEM;
-    PEEC_HIP_TRY(hipMemcpy(ctx->Yle.ptr.p, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(ctx->Yle.idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
-    PEEC_HIP_TRY(hipMemcpy(ctx->Yle.val.p, val.data(), val.size() * 16, hipMemcpyHostToDevice));
-    ctx->freq_ready = true;
+    peec_ctx::HostCsr &h = ctx->hYle[(size_t)slot];
+    h.ptr = std::move(ptr);
+    h.idx = std::move(idx);
+    h.val = std::move(val);
+    int rc = ctx->upload_yle();
+    if (rc) return rc;
+    ctx->slot_set[(size_t)slot] = 1;
+    ctx->slot_lu[(size_t)slot] = 0;
+    ctx->freq_ready = std::all_of(ctx->slot_set.begin(), ctx->slot_set.end(), [](char c) { return c != 0; });
     ctx->lu_ready = false;
     return PEEC_OK;
+}
+
+extern "C" int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, const peec_csc_c128 *Yle)
+{
+    if (!ctx) {
+        peec_set_error("peec_frequency_set: null context");
+        return PEEC_ERR_ARG;
+    }
+    if (ctx->nslots != 1 || (int)ctx->slot_set.size() != 1) {
+        int rc = peec_frequency_slots(ctx, 1);
+        if (rc) return rc;
+    }
+    return peec_frequency_set_slot(ctx, 0, w, DZ, Yle);
 }
 
 extern "C" int peec_lu_set_factors(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L, const peec_csc_c128 *U,
@@ -470,6 +562,10 @@ extern "C" int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_
     if (n > INT32_MAX) {
         peec_set_error("peec_lu_set_factors: n exceeds the int32 index range");
         return PEEC_ERR_ARG;
+    }
+    if (ctx->nslots != 1) {
+        peec_set_error("peec_lu_set_factors: host factors serve one frequency slot (%d set)", ctx->nslots);
+        return PEEC_ERR_STATE;
     }
     std::vector<int32_t> blk;
     if (nblocks > 0 && block_ptr) {
@@ -510,6 +606,7 @@ extern "C" int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_
     } else {
         ctx->has_Rs = false;
     }
+    ctx->slot_lu.assign(1, 1);
     ctx->lu_ready = true;
     ctx->lu_layout_from_plan = false;
     return PEEC_OK;
@@ -517,7 +614,7 @@ extern "C" int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_
 
 // ------------------------------------------------------------------ apply
 // In place on y (length N per rhs, stride ld): y <- M^-1 y
-int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld)
+int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, int rps)
 {
     const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n;
     if (nrhs * n > (int64_t)ctx->vt1.n) {
@@ -526,20 +623,20 @@ int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld)
     double2_t *b = ctx->vt1.p, *z = ctx->vt2.p;
     hipLaunchKernelGGL(k_pre, dim3(nblk(n)), dim3(256), 0, ctx->stream, y, ctx->invZ.p, ctx->invP.p, ctx->At.ptr.p,
                        ctx->At.idx.p, ctx->At.val.p, ctx->G.ptr.p, ctx->G.idx.p, ctx->G.val.p, ctx->perm_r.p,
-                       ctx->has_Rs ? ctx->Rs.p : nullptr, b, ctx->w, m, ns, n, nrhs, ld);
+                       ctx->has_Rs ? ctx->Rs.p : nullptr, b, ctx->wv.p, m, ns, n, nrhs, ld, rps);
     PEEC_HIP_TRY(hipGetLastError());
     int rc;
-    if ((rc = trsv_solve(ctx, ctx->Lf, true, b, z, nrhs))) return rc;   // L c = b   (c in z)
-    if ((rc = trsv_solve(ctx, ctx->Uf, false, z, b, nrhs))) return rc;  // U z' = c  (z' in b)
+    if ((rc = trsv_solve(ctx, ctx->Lf, true, b, z, nrhs, rps))) return rc;   // L c = b   (c in z)
+    if ((rc = trsv_solve(ctx, ctx->Uf, false, z, b, nrhs, rps))) return rc;  // U z' = c  (z' in b)
     hipLaunchKernelGGL(k_post, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, b, ctx->perm_c.p, ctx->invZ.p,
                        ctx->invP.p, ctx->A.ptr.p, ctx->A.idx.p, ctx->A.val.p, ctx->Gt.ptr.p, ctx->Gt.idx.p,
-                       ctx->Gt.val.p, m, ns, n, nrhs, ld);
+                       ctx->Gt.val.p, m, ns, n, nrhs, ld, rps);
     PEEC_HIP_TRY(hipGetLastError());
     return PEEC_OK;
 }
 
 // y = K x (before the preconditioner), device vectors
-int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32)
+int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps)
 {
     const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n, N = m + ns + n;
     hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns)), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
@@ -559,12 +656,12 @@ int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t l
     }
     const int64_t phi0 = m + ns;
     hipLaunchKernelGGL(k_y1, dim3(nblk(m)), dim3(256), 0, ctx->stream, y, x, ctx->DZ.p, ctx->A.ptr.p, ctx->A.idx.p,
-                       ctx->A.val.p, ctx->w, m, phi0, nrhs, ld);
+                       ctx->A.val.p, ctx->wv.p, m, phi0, nrhs, ld, rps);
     hipLaunchKernelGGL(k_y2, dim3(nblk(ns)), dim3(256), 0, ctx->stream, y, x, ctx->Gt.ptr.p, ctx->Gt.idx.p,
                        ctx->Gt.val.p, m, ns, phi0, nrhs, ld);
     hipLaunchKernelGGL(k_y3, dim3(nblk(n)), dim3(256), 0, ctx->stream, y, x, ctx->At.ptr.p, ctx->At.idx.p,
                        ctx->At.val.p, ctx->Yle.ptr.p, ctx->Yle.idx.p, ctx->Yle.val.p, ctx->G.ptr.p, ctx->G.idx.p,
-                       ctx->G.val.p, ctx->w, m, ns, n, nrhs, ld);
+                       ctx->G.val.p, ctx->wv.p, m, ns, n, nrhs, ld, rps);
     PEEC_HIP_TRY(hipGetLastError());
     (void)N;
     return PEEC_OK;
@@ -577,11 +674,11 @@ int sync_stream(peec_ctx *ctx)
 }
 
 // device-vector matvec used by GMRES: y = M^-1 K x
-int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32)
+int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps)
 {
-    int rc = apply_K(ctx, x, y, nrhs, ld, fp32);
+    int rc = apply_K(ctx, x, y, nrhs, ld, fp32, rps);
     if (rc) return rc;
-    return apply_precond(ctx, y, nrhs, ld);
+    return apply_precond(ctx, y, nrhs, ld, rps);
 }
 
 static int ready(peec_ctx *ctx, const char *what)
@@ -604,7 +701,7 @@ extern "C" int peec_matvec(peec_ctx *ctx, const peec_c128 *x, peec_c128 *y)
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
     const int64_t N = ctx->m + ctx->ns + ctx->n;
     PEEC_HIP_TRY(hipMemcpyAsync(ctx->vx.p, x, (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
-    if ((rc = matvec_device(ctx, ctx->vx.p, ctx->vy.p, 1, N, false))) return rc;
+    if ((rc = matvec_device(ctx, ctx->vx.p, ctx->vy.p, 1, N, false, 1))) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(y, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
     return sync_stream(ctx);
 }
@@ -619,7 +716,7 @@ extern "C" int peec_apply_K(peec_ctx *ctx, const peec_c128 *x, peec_c128 *y)
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
     const int64_t N = ctx->m + ctx->ns + ctx->n;
     PEEC_HIP_TRY(hipMemcpyAsync(ctx->vx.p, x, (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
-    int rc = apply_K(ctx, ctx->vx.p, ctx->vy.p, 1, N, false);
+    int rc = apply_K(ctx, ctx->vx.p, ctx->vy.p, 1, N, false, 1);
     if (rc) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(y, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -640,7 +737,7 @@ extern "C" int peec_precond_rhs(peec_ctx *ctx, const double *is, peec_c128 *tn)
     std::vector<double2_t> h((size_t)N, {0.0, 0.0});
     for (int64_t v = 0; v < n; v++) h[(size_t)(m + ns + v)] = {is[v], 0.0};
     PEEC_HIP_TRY(hipMemcpyAsync(ctx->vy.p, h.data(), (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
-    if ((rc = apply_precond(ctx, ctx->vy.p, 1, N))) return rc;
+    if ((rc = apply_precond(ctx, ctx->vy.p, 1, N, 1))) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(tn, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
     return sync_stream(ctx);
 }

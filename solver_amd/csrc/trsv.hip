@@ -38,9 +38,11 @@ __device__ __forceinline__ cd tcdiv(cd a, cd b)
 }
 
 // part[s + r * pstride] = sum over the entries of segment s of T[row, c] * x[c]
+// (right-hand side r uses the value set of slot r / rps, ob_n values each)
 __global__ void __launch_bounds__(256) k_trsv_gather(const int64_t *seg_k0, const int64_t *seg_k1, int64_t s0,
-                                                     int64_t nseg, const int32_t *idx, const cd *val, const cd *x,
-                                                     cd *part, int64_t pstride, int64_t n, int nrhs)
+                                                     int64_t nseg, const int32_t *idx, const cd *val, int64_t ob_n,
+                                                     const cd *x, cd *part, int64_t pstride, int64_t n, int nrhs,
+                                                     int rps)
 {
     const int64_t sg = s0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (sg >= s0 + nseg) return;  // whole wavefront: the kernel has no barrier
@@ -48,14 +50,20 @@ __global__ void __launch_bounds__(256) k_trsv_gather(const int64_t *seg_k0, cons
     const int64_t k0 = seg_k0[sg], k1 = seg_k1[sg];
     // a segment holds at most kTrsvSeg = 4 * 64 entries: 4 per lane
     int c[4];
-    cd v[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int64_t k = k0 + lane + 64 * i;
         c[i] = k < k1 ? idx[k] : -1;
-        v[i] = k < k1 ? val[k] : cd{0.0, 0.0};
     }
+    cd v[4];
+    int slot = -1;
     for (int r = 0; r < nrhs; r++) {
+        if (r / rps != slot) {
+            slot = r / rps;
+            const cd *vs = val + (int64_t)slot * ob_n;
+#pragma unroll
+            for (int i = 0; i < 4; i++) v[i] = c[i] >= 0 ? vs[k0 + lane + 64 * i] : cd{0.0, 0.0};
+        }
         const cd *xr = x + (int64_t)r * n;
         double ax = 0.0, ay = 0.0;
 #pragma unroll
@@ -83,17 +91,23 @@ __device__ __forceinline__ double lane_value(double v, int l)
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// One workgroup per block of the level (binfo: the level's blocks in
-// launch order).  inv_diag holds 1 / T[i, i].
+// One workgroup per (block of the level, value set, group of right-hand
+// sides): binfo lists the level's blocks in launch order; blockIdx.y =
+// slot * gps + g, and the workgroup solves the right-hand sides
+// slot * rps + g * 4 + wave (stride 4 gps) of that slot.  inv_diag holds
+// 1 / T[i, i].
 template <bool LOWER>
-__global__ void __launch_bounds__(256) k_trsv_block(const int4 *binfo, const cd *dense, const cd *inv_diag,
-                                                    const int2 *row_seg, const cd *part, int64_t pstride, const cd *b,
-                                                    cd *x, int64_t n, int nrhs)
+__global__ void __launch_bounds__(256) k_trsv_block(const int4 *binfo, const cd *dense, int64_t dense_n,
+                                                    const cd *inv_diag, const int2 *row_seg, const cd *part,
+                                                    int64_t pstride, const cd *b, cd *x, int64_t n, int nrhs, int rps,
+                                                    int gps)
 {
     __shared__ cd tile[64 * 64];
     const int4 bi = binfo[blockIdx.x];  // r0, s, dense offset (lo, hi)
     const int r0 = bi.x, s = bi.y;
-    const cd *T = dense + (int64_t)(((uint64_t)(uint32_t)bi.w << 32) | (uint32_t)bi.z);
+    const int slot = (int)blockIdx.y / gps, g = (int)blockIdx.y % gps;
+    const cd *T = dense + (int64_t)slot * dense_n + (int64_t)(((uint64_t)(uint32_t)bi.w << 32) | (uint32_t)bi.z);
+    inv_diag += (int64_t)slot * n;
     const int ss = s * s;
     {
         // all 16 loads in flight before the first LDS store
@@ -116,7 +130,8 @@ __global__ void __launch_bounds__(256) k_trsv_block(const int4 *binfo, const cd 
     const int2 gs = mine ? row_seg[row] : int2{0, 0};
     const int g0 = gs.x, g1 = gs.y;
     __syncthreads();
-    for (int r = (int)blockIdx.y * 4 + wave; r < nrhs; r += (int)gridDim.y * 4) {
+    const int rend = min(nrhs, (slot + 1) * rps);
+    for (int r = slot * rps + g * 4 + wave; r < rend; r += gps * 4) {
         cd acc = {0.0, 0.0};
         if (mine) {
             acc = b[row + (int64_t)r * n];
@@ -293,9 +308,12 @@ int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std
         sk0.push_back(0);
         sk1.push_back(0);
     }
+    f->nslots = 1;
+    f->ob_n = (int64_t)oidx.size();
+    f->dense_n = (int64_t)std::max<size_t>(src->dense.size(), 1);
     if (!f->binfo.alloc(binfo.size()) || !f->seg_k0.alloc(sk0.size()) || !f->seg_k1.alloc(sk1.size()) ||
-        !f->row_seg.alloc(rs.size()) || !f->ob_idx.alloc(oidx.size()) || !f->ob_val.alloc(oidx.size()) ||
-        !f->dense.alloc(std::max<size_t>(src->dense.size(), 1)) || !f->diag.alloc((size_t)std::max<int64_t>(n, 1))) {
+        !f->row_seg.alloc(rs.size()) || !f->ob_idx.alloc(oidx.size()) || !f->ob_val.alloc((size_t)f->ob_n) ||
+        !f->dense.alloc((size_t)f->dense_n) || !f->diag.alloc((size_t)std::max<int64_t>(n, 1))) {
         peec_set_error("%s: out of device memory", what);
         return PEEC_ERR_NOMEM;
     }
@@ -304,6 +322,18 @@ int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std
     PEEC_HIP_TRY(hipMemcpy(f->seg_k1.p, sk1.data(), sk1.size() * 8, hipMemcpyHostToDevice));
     PEEC_HIP_TRY(hipMemcpy(f->row_seg.p, rs.data(), rs.size() * 4, hipMemcpyHostToDevice));
     PEEC_HIP_TRY(hipMemcpy(f->ob_idx.p, oidx.data(), oidx.size() * 4, hipMemcpyHostToDevice));
+    return PEEC_OK;
+}
+
+int tri_set_slots(TriFactor *f, int nslots)
+{
+    if (nslots == f->nslots && f->ob_val.p) return PEEC_OK;
+    if (!f->ob_val.alloc((size_t)(f->ob_n * nslots)) || !f->dense.alloc((size_t)(f->dense_n * nslots)) ||
+        !f->diag.alloc((size_t)(std::max<int64_t>(f->n, 1) * nslots))) {
+        peec_set_error("out of device memory for %d factor value sets", nslots);
+        return PEEC_ERR_NOMEM;
+    }
+    f->nslots = nslots;
     return PEEC_OK;
 }
 
@@ -365,7 +395,7 @@ int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &bl
     return PEEC_OK;
 }
 
-int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs)
+int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs, int rps)
 {
     const int64_t pstride = std::max<int64_t>(f.nseg, 1);
     if ((size_t)(pstride * nrhs) > ctx->trsv_part.n) {
@@ -374,8 +404,14 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
             return PEEC_ERR_NOMEM;
         }
     }
+    const int nsl = (nrhs + rps - 1) / rps;
+    if (nsl > f.nslots) {
+        peec_set_error("triangular solve: %d right-hand sides at %d per slot need %d factor slots, %d set", nrhs, rps,
+                       nsl, f.nslots);
+        return PEEC_ERR_STATE;
+    }
     cd *part = ctx->trsv_part.p;
-    const int ry = std::min((nrhs + 3) / 4, 64);
+    const int gps = std::min((rps + 3) / 4, 16);
     const int nlev = (int)f.level_ptr.size() - 1;
     for (int l = 0; l < nlev; l++) {
         const int64_t s0 = f.seg_level_ptr[(size_t)l], ns = f.seg_level_ptr[(size_t)l + 1] - s0;
@@ -383,22 +419,25 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
         // partial sums written; of a block solve: the in-block entries, the
         // diagonal, b and x, and the partial sums read
         if (ns > 0) {
-            int pi = ctx->prof_begin(PK_TRSV_GATHER, f.lev_ob[(size_t)l] * 20.0 + (double)ns * 16.0 * nrhs);
+            int pi = ctx->prof_begin(PK_TRSV_GATHER, f.lev_ob[(size_t)l] * (4.0 + 16.0 * nsl) + (double)ns * 16.0 * nrhs);
             hipLaunchKernelGGL(k_trsv_gather, dim3((unsigned)((ns + 3) / 4)), dim3(256), 0, ctx->stream, f.seg_k0.p,
-                               f.seg_k1.p, s0, ns, f.ob_idx.p, f.ob_val.p, (const cd *)x, part, pstride, f.n, nrhs);
+                               f.seg_k1.p, s0, ns, f.ob_idx.p, f.ob_val.p, f.ob_n, (const cd *)x, part, pstride, f.n,
+                               nrhs, rps);
             ctx->prof_end(pi);
         }
-        int pi = ctx->prof_begin(lower ? PK_TRSV_BLOCK_L : PK_TRSV_BLOCK_U, f.lev_ib[(size_t)l] * 16.0 + f.lev_rows[(size_t)l] * (16.0 + 32.0 * nrhs) +
-                                                    (double)ns * 16.0 * nrhs);
+        int pi = ctx->prof_begin(lower ? PK_TRSV_BLOCK_L : PK_TRSV_BLOCK_U,
+                                 (f.lev_ib[(size_t)l] + f.lev_rows[(size_t)l]) * 16.0 * nsl +
+                                     f.lev_rows[(size_t)l] * 32.0 * nrhs + (double)ns * 16.0 * nrhs);
         const int nbl = f.level_ptr[(size_t)l + 1] - f.level_ptr[(size_t)l];
         const int4 *bi = reinterpret_cast<const int4 *>(f.binfo.p) + f.level_ptr[(size_t)l];
         const int2 *rsg = reinterpret_cast<const int2 *>(f.row_seg.p);
+        const dim3 grid(nbl, nsl * gps);
         if (lower)
-            hipLaunchKernelGGL(k_trsv_block<true>, dim3(nbl, ry), dim3(256), 0, ctx->stream, bi, f.dense.p, f.diag.p,
-                               rsg, (const cd *)part, pstride, b, x, f.n, nrhs);
+            hipLaunchKernelGGL(k_trsv_block<true>, grid, dim3(256), 0, ctx->stream, bi, f.dense.p, f.dense_n, f.diag.p,
+                               rsg, (const cd *)part, pstride, b, x, f.n, nrhs, rps, gps);
         else
-            hipLaunchKernelGGL(k_trsv_block<false>, dim3(nbl, ry), dim3(256), 0, ctx->stream, bi, f.dense.p, f.diag.p,
-                               rsg, (const cd *)part, pstride, b, x, f.n, nrhs);
+            hipLaunchKernelGGL(k_trsv_block<false>, grid, dim3(256), 0, ctx->stream, bi, f.dense.p, f.dense_n,
+                               f.diag.p, rsg, (const cd *)part, pstride, b, x, f.n, nrhs, rps, gps);
         ctx->prof_end(pi);
     }
     PEEC_HIP_TRY(hipGetLastError());

@@ -180,6 +180,58 @@ def frequency_set(w, DZ, Yle, ctx=None):
     L.check(L.lib().peec_frequency_set(ctx.handle, float(w), L.as_c128p(dz), ctypes.byref(Y)), "peec_frequency_set")
 
 
+def frequency_slots(nslots, ctx=None):
+    """Allocate nslots frequency slots for batched solves (clears frequency
+    and factor state)."""
+    ctx = ctx or default_context()
+    L.check(L.lib().peec_frequency_slots(ctx.handle, int(nslots)), "peec_frequency_slots")
+
+
+def frequency_set_slot(slot, w, DZ, Yle, ctx=None):
+    ctx = ctx or default_context()
+    dz = np.ascontiguousarray(DZ, dtype=np.complex128)
+    Y, keep = _csc_c128(Yle)
+    L.check(L.lib().peec_frequency_set_slot(ctx.handle, int(slot), float(w), L.as_c128p(dz), ctypes.byref(Y)),
+            "peec_frequency_set_slot")
+
+
+def lu_factor_slot(slot, ctx=None):
+    ctx = ctx or default_context()
+    L.check(L.lib().peec_lu_factor_slot(ctx.handle, int(slot)), "peec_lu_factor_slot")
+
+
+def precond_rhs_batched(IS, rhs_per_slot, ctx=None):
+    """M^-1 [0; 0; is_r] for the columns of IS (n x nrhs)."""
+    ctx = ctx or default_context()
+    ISt = np.ascontiguousarray(np.asarray(IS, dtype=np.float64).T)  # column-major n x nrhs
+    nrhs = ISt.shape[0]
+    out = np.zeros((nrhs, ctx_size(ctx)), dtype=np.complex128)
+    L.check(L.lib().peec_precond_rhs_batched(ctx.handle, nrhs, int(rhs_per_slot), L.as_dp(ISt), L.as_c128p(out)),
+            "peec_precond_rhs_batched")
+    return out.T
+
+
+def gmres_batched(B, rhs_per_slot, tol, X0=None, restart=60, max_cycles=20, mixed=False, inner_tol=0.0, ctx=None):
+    """Solve (M^-1 K) x_r = b_r for the columns of B (N x nrhs), column r
+    against frequency slot r / rhs_per_slot.  Returns (X, flag, relres,
+    iters, cycles)."""
+    ctx = ctx or default_context()
+    Bt = np.ascontiguousarray(np.asarray(B, dtype=np.complex128).T)  # column-major N x nrhs
+    nrhs, N = Bt.shape
+    Xt = np.zeros((nrhs, N), dtype=np.complex128) if X0 is None else \
+        np.ascontiguousarray(np.asarray(X0, dtype=np.complex128).T).copy()
+    flag = np.zeros(nrhs, dtype=np.int32)
+    relres = np.zeros(nrhs)
+    iters = np.zeros(nrhs, dtype=np.int32)
+    cycles = np.zeros(nrhs, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.check(L.lib().peec_gmres_batched(ctx.handle, nrhs, int(rhs_per_slot), L.as_c128p(Bt), float(tol),
+                                       int(restart), int(max_cycles), int(bool(mixed)), float(inner_tol),
+                                       L.as_c128p(Xt), flag.ctypes.data_as(ip), L.as_dp(relres),
+                                       iters.ctypes.data_as(ip), cycles.ctypes.data_as(ip)), "peec_gmres_batched")
+    return Xt.T, flag, relres, iters, cycles
+
+
 def lu_set_factors(Lf, Uf, p, q, Rs=None, ctx=None, blocks=None):
     """F = lu(SS) in Julia's convention L*U = (Rs .* SS)[p, q] (0-based p, q).
     blocks: optional block_ptr cutting the factor order into row blocks

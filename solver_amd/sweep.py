@@ -213,3 +213,62 @@ def s2y_fft(S, Zo):
     P = S.shape[0]
     I = np.eye(P)
     return np.stack([Zo * np.linalg.solve(I + S[:, :, k], I - S[:, :, k]) for k in range(S.shape[2])], axis=2)
+
+
+def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_cycles=20, inner_tol=0.0,
+                          ports_scatter_value=50.0, solver=None, record=None):
+    """The QS sweep of FFT_solver_QS_S_type.jl:60-232 with its frequency x
+    port loop batched: `batch` frequencies at a time, every port of each,
+    go through one peec_gmres_batched call (one batched matvec per
+    iteration; factors per frequency slot on the device).  Warm starts
+    chain across batches per port, like Vrest (:140, :196).  mixed = True
+    runs the inner cycles with FP32 FFT convolutions.  Returns dict(S, Z, Y,
+    f) as FFT_solver_QS_S_type."""
+    solver = solver or Solver(mesh, device_lu=True)
+    if not solver.device_lu:
+        raise ValueError("the batched sweep needs a Solver with device_lu=True")
+    esc = mesh.escalings
+    R0 = ports_scatter_value
+    freq = np.asarray(freq, dtype=np.float64) * esc["freq"]
+    F = freq.size
+    m, n, ns = mesh.m, mesh.n, mesh.ns
+    N = m + ns + n
+    P = mesh.port_nodes.shape[0]
+    w = 2 * np.pi * freq
+    S = np.zeros((P, P, F), dtype=np.complex128)
+    Xprev = np.zeros((N, P), dtype=np.complex128)
+    nslots = None
+    for g0 in range(0, F, batch):
+        ks = list(range(g0, min(F, g0 + batch)))
+        nb = len(ks)
+        if nslots != nb:
+            peec.frequency_slots(nb, ctx=solver.ctx)
+            nslots = nb
+        for slot, k in enumerate(ks):
+            Yle = build_Yle_FFT(mesh.lumped_elements, mesh.port_nodes, esc, n, w[k] / esc["freq"], R0)
+            peec.frequency_set_slot(slot, w[k], DZ_of(mesh, w[k]), Yle, ctx=solver.ctx)
+            peec.lu_factor_slot(slot, ctx=solver.ctx)
+        IS = np.zeros((n, nb * P))
+        for slot in range(nb):
+            for c1 in range(P):
+                n1, n2 = mesh.port_nodes[c1]
+                IS[n1, slot * P + c1] = esc["Is"]
+                IS[n2, slot * P + c1] = -esc["Is"]
+        TN = peec.precond_rhs_batched(IS, P, ctx=solver.ctx)
+        X0 = np.tile(Xprev, (1, nb))
+        X, flag, relres, iters, cycles = peec.gmres_batched(TN, P, tol, X0, restart, max_cycles, mixed, inner_tol,
+                                                            ctx=solver.ctx)
+        Xprev = X[:, (nb - 1) * P:nb * P]
+        for slot, k in enumerate(ks):
+            V = X[:, slot * P:(slot + 1) * P]
+            for c1 in range(P):
+                if record is not None:
+                    r = slot * P + c1
+                    record.append(dict(k=k, port=c1, flag=int(flag[r]), relres=float(relres[r]),
+                                       iters=int(iters[r]), cycles=int(cycles[r])))
+                for c2 in range(c1, P):
+                    n3, n4 = mesh.port_nodes[c2]
+                    dv = V[m + ns + n3, c1] - V[m + ns + n4, c1]
+                    S[c1, c2, k] = (2 * dv - R0) / R0 if c1 == c2 else 2 * dv / R0
+                    S[c2, c1, k] = S[c1, c2, k]
+    return dict(S=S, Z=s2z_fft(S, R0), Y=s2y_fft(S, R0), f=freq / esc["freq"])

@@ -1,0 +1,66 @@
+"""Batched frequency x port solves (peec_gmres_batched) against the oracle.
+
+The batched path runs restarted CGS2 GMRES inside FP64 iterative
+refinement, so it does not follow the reference's iteration sequence; its
+impedances are checked against the oracle's sweep at tight tolerances:
+plain FP64 to 1e-8 relative (both solved to a 1e-12 residual), mixed
+precision (FP32 FFT convolutions inside the cycles) to the 1e-6 relative
+impedance error the mixed path is held to."""
+import numpy as np
+import pytest
+
+from oracle import fft_solver as OF
+from solver_amd import peec, sweep
+
+from test_gpu_operator import crossbar_mesh, rel, small_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batched_matvec_matches_single():
+    """M^-1 K applied to 3 slots x 2 ports at once equals the one-slot
+    matvecs (same factors: a slot factored on the device at the same w)."""
+    M = small_mesh()
+    S = sweep.Solver(M, device_lu=True)
+    fs = [1e8, 1e9, 3e9]
+    ws = [2 * np.pi * f * M.escalings["freq"] for f in fs]
+    N = M.m + M.ns + M.n
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(N, 6)) + 1j * rng.normal(size=(N, 6))
+    single = []
+    for w in ws:
+        S.set_frequency(w)
+        single.append(np.stack([peec.ComputeMatrixVectorFTT(X[:, len(single) * 2 + c], ctx=S.ctx)
+                                for c in range(2)], axis=1))
+    peec.frequency_slots(3, ctx=S.ctx)
+    for slot, w in enumerate(ws):
+        Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+        peec.frequency_set_slot(slot, w, sweep.DZ_of(M, w), Yle, ctx=S.ctx)
+        peec.lu_factor_slot(slot, ctx=S.ctx)
+    # the batched path applies A only inside GMRES: solve A x = b with
+    # b = A x from the one-slot matvecs and check that it returns x (every
+    # column against its own slot's operator)
+    B = np.concatenate(single, axis=1)
+    Xs, flag, relres, iters, cycles = peec.gmres_batched(B, 2, 1e-11, None, restart=80, max_cycles=10, ctx=S.ctx)
+    print("batched solve of A x: flags", flag, "relres", relres, "iters", iters, flush=True)
+    assert np.all(relres < 1e-10)
+    for c in range(6):
+        print("column", c, "rel err", rel(Xs[:, c], X[:, c]), flush=True)
+        assert rel(Xs[:, c], X[:, c]) < 1e-6
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batched_crossbar_sweep(mixed):
+    M, cfg = crossbar_mesh()
+    rows = OF.kernel_rows(M.origin, M.Nx, M.Ny, M.Nz, M.sx, M.sy, M.sz)
+    spec = OF.spectra(rows, M.Nx, M.Ny, M.Nz)
+    out_o = OF.fft_solver_qs(cfg["freqs"], M, spec, 400, 1e-12)
+    rec = []
+    out_b = sweep.FFT_solver_QS_batched(cfg["freqs"], M, 1e-12 if not mixed else 1e-10, batch=4, mixed=mixed,
+                                        record=rec)
+    for r in rec:
+        print(r, flush=True)
+    assert all(r["flag"] == 0 for r in rec)
+    err = np.max(np.abs(out_b["Z"] - out_o["Z"]) / np.abs(out_o["Z"]))
+    print("batched (mixed=%s) crossbar Z max rel err %.3e" % (mixed, err), flush=True)
+    assert err < (1e-6 if mixed else 1e-8)
