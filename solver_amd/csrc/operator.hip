@@ -32,6 +32,15 @@ void peec_ctx::release_operator()
     op_ready = false;
     freq_ready = false;
     lu_ready = false;
+    // frequency slots and factor layouts are sized for the old geometry:
+    // the next peec_frequency_set / peec_frequency_slots re-creates them
+    nslots = 1;
+    slot_set.clear();
+    slot_lu.clear();
+    hYle.clear();
+    hw.clear();
+    lu_layout_from_plan = false;
+    lu.ready = false;
 }
 
 int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *vec_in, double2_t *vec_out, int nrhs,
