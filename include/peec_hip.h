@@ -213,6 +213,37 @@ int peec_gmres_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const peec_c12
  * (FFT_solver_QS_S_type.jl:237-262) batched. */
 int peec_precond_rhs_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const double *is, peec_c128 *tn);
 
+/* FFT-pass autotuner (no counterpart in the reference, whose FFTW plans are
+ * made with FFTW's defaults, compute_Matrix_vector_fft.jl:20-24).  For every
+ * convolution kernel of the current operator and each precision in
+ * `precisions` (bit 0: complex128, bit 1: complex64), times the five passes
+ * of one convolution over nrhs right-hand sides under each launch shape of
+ * the library's table (lines per tile, lanes per line), then the
+ * right-hand sides per block of the fused z pass, then (complex64 only) the
+ * radix schedule, and keeps the fastest for batched calls with exactly
+ * nrhs right-hand sides.  Launch shapes do not change any result; a
+ * complex64 radix schedule changes rounding only.  *report_ms (optional)
+ * receives the summed best pass times.  The choices last until the next
+ * peec_kernels_build or peec_operator_set. */
+int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *report_ms);
+/* The choice peec_fft_tune made for pass `pass` (0..4) of kernel `kernel`
+ * (0..8): out5 = {variant (-1 untuned), lines per tile, lanes per line,
+ * right-hand sides per block, radix schedule}. */
+int peec_fft_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int *out5);
+
+/* Sets the choice of one pass by hand (a tuning table kept from an earlier
+ * run, or a test of every launch shape): variant -1 restores the default.
+ * Non-zero radix schedules are accepted for complex64 only. */
+int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int variant, int rpb, int style);
+
+/* K x (precond = 0) or M^-1 K x (precond = 1) for nrhs vectors (N x nrhs,
+ * column-major), vector r against frequency slot r / rhs_per_slot, with
+ * the FFT convolutions in complex128 (fp32 = 0) or complex64 (fp32 = 1):
+ * the batched form of ComputeMatrixVectorFTT (compute_Matrix_vector_fft.jl
+ * :1-146) that peec_gmres_batched applies, exposed for checks. */
+int peec_apply_K_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, int precond, const peec_c128 *x,
+                         peec_c128 *y);
+
 /* Kernel timing (no counterpart in the reference, which only prints @time,
  * do_solving_fft.jl:133).  Reads and clears the per-kind totals recorded
  * since the previous call -- kernel milliseconds from HIP events on the

@@ -155,6 +155,23 @@ struct TriFactor {
     DevBuf<double2_t> dense, diag;           // nslots x dense_n, nslots x n; diag holds 1 / T[i, i]
 };
 
+// Launch shape of one FFT pass chosen by peec_fft_tune: variant of the
+// launch-shape table, right-hand sides per block (fused z pass) and radix
+// schedule.  Keyed by (kernel, pass, fp32, nrhs).
+struct FftChoice {
+    int variant = -1, rpb = 1, style = 0;
+};
+struct FftTuneKey {
+    int kernel, pass, fp32, nrhs;
+    bool operator<(const FftTuneKey &o) const
+    {
+        if (kernel != o.kernel) return kernel < o.kernel;
+        if (pass != o.pass) return pass < o.pass;
+        if (fp32 != o.fp32) return fp32 < o.fp32;
+        return nrhs < o.nrhs;
+    }
+};
+
 // Per-kernel timing with HIP events on the context's stream.
 struct ProfRec {
     int kind;
@@ -187,6 +204,7 @@ struct peec_ctx {
     DevBuf<double2_t> twiddles;             // concatenated exp(-2 pi i t / L) tables
     std::map<int, int64_t> twiddle_off;     // L -> offset into twiddles
     int64_t twiddle_offset(int L);          // builds the table on first use
+    std::map<FftTuneKey, FftChoice> fft_tune;  // peec_fft_tune results for the current geometry
 
     // ---- operator
     bool op_ready = false;

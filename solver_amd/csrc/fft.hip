@@ -94,6 +94,7 @@ void peec_ctx::release_kernels()
         k.spec32.reset();
         k.n[0] = k.n[1] = k.n[2] = 0;
     }
+    fft_tune.clear();
     kernels_ready = false;
 }
 
@@ -197,41 +198,52 @@ __device__ __forceinline__ void dftR(C2<T> *v)
     else dft8<T, DIR>(v);
 }
 
-// One Stockham stage on a line of length L held in LDS (buf).  Each of the
-// TPL lanes of the line owns butterflies k = lane + q*TPL.
-template <typename T, int DIR, int R, int TPL, int LMAX>
-__device__ __forceinline__ void stockham_stage(C2<T> *buf, int L, int Ns, int lane, const double2_t *__restrict__ tw)
+// One Stockham stage over a tile of W = (256 / TPL) * LPT lines held in LDS
+// (line stride LS).  Lane `lane` of line group `lg` owns butterflies
+// k = lane + q * TPL of lines lg + lp * (256 / TPL).
+template <typename T, int DIR, int R, int TPL, int LPT, int LMAX>
+__device__ __forceinline__ void stockham_stage(C2<T> *tile, int LS, int L, int Ns, int lg, int lane,
+                                               const double2_t *__restrict__ tw)
 {
+    constexpr int G = 256 / TPL;
     constexpr int MAXB = (LMAX + R * TPL - 1) / (R * TPL);
     const int nb = L / R;
     const int stride = L / (Ns * R);
-    C2<T> v[MAXB][R];
+    C2<T> v[LPT][MAXB][R];
 #pragma unroll
-    for (int q = 0; q < MAXB; q++) {
-        int k = lane + q * TPL;
-        if (k < nb) {
+    for (int lp = 0; lp < LPT; lp++) {
+        const C2<T> *buf = tile + (lg + lp * G) * LS;
 #pragma unroll
-            for (int r = 0; r < R; r++) v[q][r] = buf[k + r * nb];
+        for (int q = 0; q < MAXB; q++) {
+            const int k = lane + q * TPL;
+            if (k < nb) {
+#pragma unroll
+                for (int r = 0; r < R; r++) v[lp][q][r] = buf[k + r * nb];
+            }
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < MAXB; q++) {
-        int k = lane + q * TPL;
-        if (k < nb) {
-            int kk = k % Ns;
-            if (Ns > 1) {
+    for (int lp = 0; lp < LPT; lp++) {
+        C2<T> *buf = tile + (lg + lp * G) * LS;
 #pragma unroll
-                for (int r = 1; r < R; r++) {
-                    double2_t w = tw[kk * r * stride];
-                    C2<T> wt = {(T)w.x, (T)(DIR < 0 ? w.y : -w.y)};
-                    v[q][r] = cmul(v[q][r], wt);
+        for (int q = 0; q < MAXB; q++) {
+            const int k = lane + q * TPL;
+            if (k < nb) {
+                const int kk = k % Ns;
+                if (Ns > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; r++) {
+                        const double2_t w = tw[kk * r * stride];
+                        const C2<T> wt = {(T)w.x, (T)(DIR < 0 ? w.y : -w.y)};
+                        v[lp][q][r] = cmul(v[lp][q][r], wt);
+                    }
                 }
-            }
-            dftR<T, DIR, R>(v[q]);
-            int base = (k - kk) * R + kk;
+                dftR<T, DIR, R>(v[lp][q]);
+                const int base = (k - kk) * R + kk;
 #pragma unroll
-            for (int r = 0; r < R; r++) buf[base + r * Ns] = v[q][r];
+                for (int r = 0; r < R; r++) buf[base + r * Ns] = v[lp][q][r];
+            }
         }
     }
     __syncthreads();
@@ -242,11 +254,12 @@ struct PassDesc {
     int nstage;
     int radix[16];
     int64_t tw_off;
-    int nb, n2;                          // lines: b (consecutive, tiled) x l2
+    int nb, n2;                          // lines: nb consecutive positions (W per block) x n2
     int64_t in_sb, in_s2, in_sj;
     int64_t out_sb, out_s2, out_sj;
     int64_t in_rhs, out_rhs;             // stride between right-hand sides
     int nrhs;
+    int rpb;                             // right-hand sides per block (blockIdx.z covers rpb of them)
     const int32_t *map_in;               // optional 0/1 scatter (compact position -> unknown)
     const double2_t *vec_in;
     int64_t vec_in_ld;                   // stride between right-hand sides in vec_in
@@ -257,103 +270,126 @@ struct PassDesc {
     int64_t spec_sb, spec_s2, spec_sj;
 };
 
-template <typename T, int DIR, int TPL, int LMAX>
-__device__ void run_fft(C2<T> *line, int L, const PassDesc &d, int lane, const double2_t *tw)
+template <typename T, int DIR, int TPL, int LPT, int LMAX>
+__device__ void run_fft(C2<T> *tile, int LS, int L, const PassDesc &d, int lg, int lane, const double2_t *tw)
 {
     int Ns = 1;
     for (int s = 0; s < d.nstage; s++) {
-        int R = d.radix[s];
+        const int R = d.radix[s];
         switch (R) {
-        case 8: stockham_stage<T, DIR, 8, TPL, LMAX>(line, L, Ns, lane, tw); break;
-        case 4: stockham_stage<T, DIR, 4, TPL, LMAX>(line, L, Ns, lane, tw); break;
-        case 2: stockham_stage<T, DIR, 2, TPL, LMAX>(line, L, Ns, lane, tw); break;
-        case 3: stockham_stage<T, DIR, 3, TPL, LMAX>(line, L, Ns, lane, tw); break;
-        default: stockham_stage<T, DIR, 5, TPL, LMAX>(line, L, Ns, lane, tw); break;
+        case 8: stockham_stage<T, DIR, 8, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, tw); break;
+        case 4: stockham_stage<T, DIR, 4, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, tw); break;
+        case 2: stockham_stage<T, DIR, 2, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, tw); break;
+        case 3: stockham_stage<T, DIR, 3, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, tw); break;
+        default: stockham_stage<T, DIR, 5, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, tw); break;
         }
         Ns *= R;
     }
 }
 
+// e / n for 0 <= e < 2^17 and 1 <= n <= 1024: the float quotient of e + 1/2
+// is at least 1/(2n) away from an integer, far beyond the rounding error.
+__device__ __forceinline__ int qdiv(int e, float inv) { return (int)(((float)e + 0.5f) * inv); }
+
 // MODE: 0 forward, 1 inverse, 2 forward * spectrum * inverse.
-// src/dst are grid buffers of T2 (ignored when map_in / map_out are used).
-template <typename T, int MODE, int TPL, int LMAX>
+// A block transforms a tile of W consecutive lines (positions b0 .. b0+W-1
+// along the tiled axis) at line index l2, for right-hand sides
+// blockIdx.z * rpb .. + rpb - 1.  Lines contiguous in memory (sj == 1) are
+// read and written line after line; strided lines are read and written
+// row by row across the W lines (coalesced along the tiled axis).  The LDS
+// line stride is L + 1 so that both access orders avoid bank conflicts.
+// For MODE 2 the block's spectrum tile is read once into registers and
+// reused for all its right-hand sides.
+template <typename T, int MODE, int TPL, int LPT, int LMAX>
 __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, C2<T> *dst,
                                                   const double2_t *__restrict__ twall)
 {
-    constexpr int TB = 256 / TPL;
+    constexpr int G = 256 / TPL;
+    constexpr int W = G * LPT;
+    constexpr int NSPEC = MODE == 2 ? (W * LMAX + 255) / 256 : 1;
     extern __shared__ char smem_raw[];
     C2<T> *tile = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = d.L;
+    const int L = d.L, LS = L + 1;
     const int tid = threadIdx.x;
-    const int b0 = blockIdx.x * TB;
+    const int b0 = blockIdx.x * W;
     const int l2 = blockIdx.y;
-    const int rhs = blockIdx.z;
-    const int nbt = min(TB, d.nb - b0);
+    const int nbt = min(W, d.nb - b0);
     const double2_t *tw = twall + d.tw_off;
-    const bool contig = (d.in_sj == 1);
+    const int lg = tid / TPL, lane = tid % TPL;
+    const bool in_contig = (d.in_sj == 1), out_contig = (d.out_sj == 1);
+    const float invL = 1.0f / (float)L, invNout = 1.0f / (float)d.nout;
 
-    // ---- load (zero padding beyond nin)
-    const int64_t ibase = (int64_t)b0 * d.in_sb + (int64_t)l2 * d.in_s2 + (int64_t)rhs * d.in_rhs;
-    if (contig) {
-        for (int b = 0; b < TB; b++) {
-            for (int j = tid; j < L; j += 256) {
+    T sv[NSPEC];
+    if (MODE == 2) {
+        const T *sp = reinterpret_cast<const T *>(d.spec);
+        const int64_t sbase = (int64_t)b0 * d.spec_sb + (int64_t)l2 * d.spec_s2;
+#pragma unroll
+        for (int i = 0; i < NSPEC; i++) {
+            const int e = tid + 256 * i;
+            const int b = e % W, j = e / W;
+            sv[i] = (b < nbt && j < L) ? sp[sbase + (int64_t)b * d.spec_sb + (int64_t)j * d.spec_sj] : (T)0;
+        }
+    }
+
+    const int r_end = min(d.nrhs, ((int)blockIdx.z + 1) * d.rpb);
+    for (int rhs = (int)blockIdx.z * d.rpb; rhs < r_end; rhs++) {
+        // ---- load (zero padding beyond nin and beyond the last line)
+        const int64_t ibase = (int64_t)b0 * d.in_sb + (int64_t)l2 * d.in_s2 + (int64_t)rhs * d.in_rhs;
+        if (in_contig) {
+            for (int e = tid; e < W * L; e += 256) {
+                const int b = qdiv(e, invL), j = e - b * L;
                 C2<T> v = {(T)0, (T)0};
                 if (b < nbt && j < d.nin) {
-                    int64_t p = ibase + (int64_t)b * d.in_sb + j;
+                    const int64_t p = ibase + (int64_t)b * d.in_sb + j;
                     if (d.map_in) {
-                        int u = d.map_in[p - (int64_t)rhs * d.in_rhs];
+                        const int u = d.map_in[p - (int64_t)rhs * d.in_rhs];
                         if (u >= 0) {
-                            double2_t x = d.vec_in[(int64_t)u + (int64_t)rhs * d.vec_in_ld];
+                            const double2_t x = d.vec_in[(int64_t)u + (int64_t)rhs * d.vec_in_ld];
                             v = {(T)x.x, (T)x.y};
                         }
                     } else {
                         v = src[p];
                     }
                 }
-                tile[b * L + j] = v;
+                tile[b * LS + j] = v;
+            }
+        } else {
+            for (int e = tid; e < W * L; e += 256) {
+                const int b = e % W, j = e / W;
+                C2<T> v = {(T)0, (T)0};
+                if (b < nbt && j < d.nin) v = src[ibase + (int64_t)b * d.in_sb + (int64_t)j * d.in_sj];
+                tile[b * LS + j] = v;
             }
         }
-    } else {
-        for (int e = tid; e < TB * L; e += 256) {
-            int b = e % TB, j = e / TB;
-            C2<T> v = {(T)0, (T)0};
-            if (b < nbt && j < d.nin) v = src[ibase + (int64_t)b * d.in_sb + (int64_t)j * d.in_sj];
-            tile[b * L + j] = v;
-        }
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // ---- transform
-    const int lb = tid / TPL, lane = tid % TPL;
-    C2<T> *line = tile + lb * L;
-    if (MODE == 1) {
-        run_fft<T, 1, TPL, LMAX>(line, L, d, lane, tw);
-    } else {
-        run_fft<T, -1, TPL, LMAX>(line, L, d, lane, tw);
-        if (MODE == 2) {
-            const T *sp = reinterpret_cast<const T *>(d.spec);
-            const int64_t sbase = (int64_t)b0 * d.spec_sb + (int64_t)l2 * d.spec_s2;
-            for (int e = tid; e < TB * L; e += 256) {
-                int b = e % TB, j = e / TB;
-                if (b < nbt) {
-                    T s = sp[sbase + (int64_t)b * d.spec_sb + (int64_t)j * d.spec_sj];
-                    tile[b * L + j] = cscale(tile[b * L + j], s);
+        // ---- transform
+        if (MODE == 1) {
+            run_fft<T, 1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, tw);
+        } else {
+            run_fft<T, -1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, tw);
+            if (MODE == 2) {
+#pragma unroll
+                for (int i = 0; i < NSPEC; i++) {
+                    const int e = tid + 256 * i;
+                    const int b = e % W, j = e / W;
+                    if (j < L) tile[b * LS + j] = cscale(tile[b * LS + j], sv[i]);
                 }
+                __syncthreads();
+                run_fft<T, 1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, tw);
             }
-            __syncthreads();
-            run_fft<T, 1, TPL, LMAX>(line, L, d, lane, tw);
         }
-    }
 
-    // ---- store (first nout elements)
-    const int64_t obase = (int64_t)b0 * d.out_sb + (int64_t)l2 * d.out_s2 + (int64_t)rhs * d.out_rhs;
-    if (d.out_sj == 1) {
-        for (int b = 0; b < nbt; b++) {
-            for (int j = tid; j < d.nout; j += 256) {
-                C2<T> v = tile[b * L + j];
-                int64_t p = obase + (int64_t)b * d.out_sb + j;
+        // ---- store (first nout elements of each line)
+        const int64_t obase = (int64_t)b0 * d.out_sb + (int64_t)l2 * d.out_s2 + (int64_t)rhs * d.out_rhs;
+        if (out_contig) {
+            for (int e = tid; e < W * d.nout; e += 256) {
+                const int b = qdiv(e, invNout), j = e - b * d.nout;
+                if (b >= nbt) continue;
+                const C2<T> v = tile[b * LS + j];
+                const int64_t p = obase + (int64_t)b * d.out_sb + j;
                 if (d.map_out) {
-                    int u = d.map_out[p - (int64_t)rhs * d.out_rhs];
+                    const int u = d.map_out[p - (int64_t)rhs * d.out_rhs];
                     if (u >= 0) {
                         double2_t *o = d.vec_out + (int64_t)u + (int64_t)rhs * d.vec_out_ld;
                         double2_t a = *o;
@@ -365,12 +401,13 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, 
                     dst[p] = v;
                 }
             }
+        } else {
+            for (int e = tid; e < W * d.nout; e += 256) {
+                const int b = e % W, j = e / W;
+                if (b < nbt) dst[obase + (int64_t)b * d.out_sb + (int64_t)j * d.out_sj] = tile[b * LS + j];
+            }
         }
-    } else {
-        for (int e = tid; e < TB * d.nout; e += 256) {
-            int b = e % TB, j = e / TB;
-            if (b < nbt) dst[obase + (int64_t)b * d.out_sb + (int64_t)j * d.out_sj] = tile[b * L + j];
-        }
+        __syncthreads();
     }
 }
 
@@ -413,35 +450,123 @@ struct PassGeom {
     int64_t in_sb, in_s2, in_sj, out_sb, out_s2, out_sj;
 };
 
-template <typename T, int MODE>
-static int launch_pass_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst)
+// Launch shapes of k_fft_pass: TPL lanes per line, LPT lines per lane group,
+// LMAX the longest line; a tile holds W = 256 / TPL * LPT lines.
+struct FftVariant {
+    int tpl, lpt, lmax;
+};
+static constexpr FftVariant kFftVariants[] = {
+    {4, 1, 64},  {8, 1, 64},   {8, 2, 128}, {8, 1, 128}, {16, 1, 128}, {16, 2, 256},
+    {16, 1, 256}, {32, 1, 256}, {32, 2, 512}, {32, 1, 512}, {64, 1, 1024},
+};
+static constexpr int kNumFftVariants = (int)(sizeof(kFftVariants) / sizeof(kFftVariants[0]));
+static constexpr size_t kFftLdsMax = 64 * 1024;
+
+static int variant_lines(int v) { return 256 / kFftVariants[v].tpl * kFftVariants[v].lpt; }
+static size_t variant_lds(int v, int L, bool fp32)
 {
-    const int L = d.L;
-    int TPL = L <= 128 ? 16 : (L <= 512 ? 32 : 64);
-    int TB = 256 / TPL;
-    size_t lds = (size_t)TB * L * sizeof(C2<T>);
-    dim3 grid((unsigned)((d.nb + TB - 1) / TB), (unsigned)d.n2, (unsigned)d.nrhs);
+    return (size_t)variant_lines(v) * (size_t)(L + 1) * (fp32 ? 8 : 16);
+}
+static bool variant_fits(int v, int L, bool fp32)
+{
+    return kFftVariants[v].lmax >= L && variant_lds(v, L, fp32) <= kFftLdsMax;
+}
+
+// Default shape: the shortest LMAX that holds L, then the widest tile
+// whose LDS stays within 40 KiB (four blocks per CU), at least 4 lines.
+static int default_variant(int L, bool fp32)
+{
+    int best = -1;
+    for (int v = 0; v < kNumFftVariants; v++) {
+        if (!variant_fits(v, L, fp32)) continue;
+        if (best < 0) {
+            best = v;
+            continue;
+        }
+        const int lm = kFftVariants[v].lmax, lb = kFftVariants[best].lmax;
+        const bool ok_v = variant_lds(v, L, fp32) <= 40 * 1024, ok_b = variant_lds(best, L, fp32) <= 40 * 1024;
+        if (lm < lb && ok_v) best = v;
+        else if (lm == lb && ok_v && (!ok_b || variant_lines(v) > variant_lines(best))) best = v;
+        else if (!ok_b && ok_v) best = v;
+    }
+    return best;
+}
+
+#define PEEC_FFT_CASE(V, TPL_, LPT_, LMAX_)                                                                       \
+    case V:                                                                                                       \
+        hipLaunchKernelGGL((k_fft_pass<T, MODE, TPL_, LPT_, LMAX_>), grid, dim3(256), lds, ctx->stream, d, s, o, tw); \
+        break;
+
+template <typename T, int MODE>
+static int launch_pass_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst, int v)
+{
+    const int W = variant_lines(v);
+    const size_t lds = variant_lds(v, d.L, sizeof(T) == 4);
+    const int rpb = std::max(1, d.rpb);
+    dim3 grid((unsigned)((d.nb + W - 1) / W), (unsigned)d.n2, (unsigned)((d.nrhs + rpb - 1) / rpb));
     const double2_t *tw = ctx->twiddles.p;
     const C2<T> *s = (const C2<T> *)src;
     C2<T> *o = (C2<T> *)dst;
-    if (TPL == 16)
-        hipLaunchKernelGGL((k_fft_pass<T, MODE, 16, 128>), grid, dim3(256), lds, ctx->stream, d, s, o, tw);
-    else if (TPL == 32)
-        hipLaunchKernelGGL((k_fft_pass<T, MODE, 32, 512>), grid, dim3(256), lds, ctx->stream, d, s, o, tw);
-    else
-        hipLaunchKernelGGL((k_fft_pass<T, MODE, 64, 1024>), grid, dim3(256), lds, ctx->stream, d, s, o, tw);
+    switch (v) {
+        PEEC_FFT_CASE(0, 4, 1, 64)
+        PEEC_FFT_CASE(1, 8, 1, 64)
+        PEEC_FFT_CASE(2, 8, 2, 128)
+        PEEC_FFT_CASE(3, 8, 1, 128)
+        PEEC_FFT_CASE(4, 16, 1, 128)
+        PEEC_FFT_CASE(5, 16, 2, 256)
+        PEEC_FFT_CASE(6, 16, 1, 256)
+        PEEC_FFT_CASE(7, 32, 1, 256)
+        PEEC_FFT_CASE(8, 32, 2, 512)
+        PEEC_FFT_CASE(9, 32, 1, 512)
+        PEEC_FFT_CASE(10, 64, 1, 1024)
+    default:
+        peec_set_error("FFT launch variant %d out of range", v);
+        return PEEC_ERR_ARG;
+    }
     PEEC_HIP_TRY(hipGetLastError());
     return PEEC_OK;
 }
+#undef PEEC_FFT_CASE
 
-static int make_desc(peec_ctx *ctx, const PassGeom &g, PassDesc *d)
+// Radix schedules: 0 = radix 8 first (fewest stages), 1 = radix 4 first,
+// 2 = radix 3 and 5 first.
+static bool make_plan_style(int L, int style, FftPlan1D *p)
+{
+    if (style == 0) return fft_make_plan(L, p);
+    p->L = L;
+    p->nstages = 0;
+    int r = L;
+    int odd[8], nodd = 0, pow2 = 1;
+    while (r % 2 == 0) {
+        r /= 2;
+        pow2 *= 2;
+    }
+    while (r % 3 == 0) { r /= 3; odd[nodd++] = 3; if (nodd >= 8) return false; }
+    while (r % 5 == 0) { r /= 5; odd[nodd++] = 5; if (nodd >= 8) return false; }
+    if (r != 1) return false;
+    int p2[16], np2 = 0;
+    while (pow2 > 1) {
+        const int R = style == 1 ? (pow2 % 4 == 0 ? 4 : 2) : (pow2 % 8 == 0 ? 8 : (pow2 % 4 == 0 ? 4 : 2));
+        p2[np2++] = R;
+        pow2 /= R;
+    }
+    if (np2 + nodd > 16) return false;
+    if (style == 2)
+        for (int i = 0; i < nodd; i++) p->radix[p->nstages++] = odd[i];
+    for (int i = 0; i < np2; i++) p->radix[p->nstages++] = p2[i];
+    if (style != 2)
+        for (int i = 0; i < nodd; i++) p->radix[p->nstages++] = odd[i];
+    return true;
+}
+
+static int make_desc(peec_ctx *ctx, const PassGeom &g, PassDesc *d, int style = 0)
 {
     if (g.L > 1024) {
         peec_set_error("FFT length %d exceeds the 1024 supported per axis", g.L);
         return PEEC_ERR_ARG;
     }
     FftPlan1D p;
-    if (!fft_make_plan(g.L, &p)) {
+    if (!make_plan_style(g.L, style, &p)) {
         peec_set_error("FFT length %d is not 2^a 3^b 5^c", g.L);
         return PEEC_ERR_ARG;
     }
@@ -462,20 +587,26 @@ static int make_desc(peec_ctx *ctx, const PassGeom &g, PassDesc *d)
     d->in_sb = g.in_sb; d->in_s2 = g.in_s2; d->in_sj = g.in_sj;
     d->out_sb = g.out_sb; d->out_s2 = g.out_s2; d->out_sj = g.out_sj;
     d->nrhs = 1;
+    d->rpb = 1;
     return PEEC_OK;
 }
 
-static int launch_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const void *src, void *dst)
+static int launch_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const void *src, void *dst, int v = -1)
 {
     if (d.nb <= 0 || d.n2 <= 0 || d.nrhs <= 0) return PEEC_OK;
-    if (fp32) {
-        if (mode == 0) return launch_pass_t<float, 0>(ctx, d, src, dst);
-        if (mode == 1) return launch_pass_t<float, 1>(ctx, d, src, dst);
-        return launch_pass_t<float, 2>(ctx, d, src, dst);
+    if (v < 0) v = default_variant(d.L, fp32);
+    if (v < 0 || !variant_fits(v, d.L, fp32)) {
+        peec_set_error("no FFT launch shape for length %d", d.L);
+        return PEEC_ERR_ARG;
     }
-    if (mode == 0) return launch_pass_t<double, 0>(ctx, d, src, dst);
-    if (mode == 1) return launch_pass_t<double, 1>(ctx, d, src, dst);
-    return launch_pass_t<double, 2>(ctx, d, src, dst);
+    if (fp32) {
+        if (mode == 0) return launch_pass_t<float, 0>(ctx, d, src, dst, v);
+        if (mode == 1) return launch_pass_t<float, 1>(ctx, d, src, dst, v);
+        return launch_pass_t<float, 2>(ctx, d, src, dst, v);
+    }
+    if (mode == 0) return launch_pass_t<double, 0>(ctx, d, src, dst, v);
+    if (mode == 1) return launch_pass_t<double, 1>(ctx, d, src, dst, v);
+    return launch_pass_t<double, 2>(ctx, d, src, dst, v);
 }
 
 // compact dims of each circulant kernel
@@ -565,79 +696,306 @@ int peec_kernels_build_impl(peec_ctx *ctx, const double origin[3], int Nx, int N
     return PEEC_OK;
 }
 
-// One convolution application: vec_out[section] += gather(conv(scatter(vec_in)))
-// for nrhs right-hand sides (vectors at stride ld).  w1/w2 are grid work
-// buffers sized by conv_work_elems.
-int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *vec_in, double2_t *vec_out,
-               int nrhs, int64_t ld, void *w1, void *w2)
+int64_t conv_work_elems(const peec_ctx *ctx, int kernel);
+
+// The five pass descriptors of one convolution application for nrhs
+// right-hand sides (vectors at stride ld), with their algorithmic bytes.
+struct ConvPasses {
+    PassDesc d[5];
+    const void *src[5];
+    void *dst[5];
+    double bytes[5];
+};
+
+static int conv_passes(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *vec_in, double2_t *vec_out,
+                       int nrhs, int64_t ld, void *w1, void *w2, const int style[5], ConvPasses *cp)
 {
     const Kernel &K = ctx->kern[app.kernel];
     const int nx = K.n[0], ny = K.n[1], nz = K.n[2];
     const int64_t Lx = K.L[0], Ly = K.L[1], Lz = K.L[2];
-    if ((int64_t)nx * ny * nz <= 0 || app.nnz_in == 0 || app.nnz_out == 0) return PEEC_OK;
     const int64_t s1 = Lx * ny * nz, s2 = Lx * Ly * nz;  // per-rhs grid sizes
-    PassDesc d;
+    const double e = fp32 ? 8.0 : 16.0, es = fp32 ? 4.0 : 8.0;
+    const double g = (double)nx * ny * nz, G1 = (double)s1 * nrhs, G2 = (double)s2 * nrhs;
     int rc;
     // 1: x forward with scatter; lines (y, z)
     PassGeom g1 = {(int)Lx, nx, (int)Lx, ny, nz, nx, (int64_t)nx * ny, 1, Lx, Lx * ny, 1};
-    if ((rc = make_desc(ctx, g1, &d))) return rc;
-    d.map_in = app.map_in.p;
-    d.vec_in = vec_in + app.in_off;
-    d.vec_in_ld = ld;
-    d.nrhs = nrhs;
-    d.in_rhs = (int64_t)nx * ny * nz;  // only used to strip the rhs offset from map positions
-    d.out_rhs = s1;
-    const double e = fp32 ? 8.0 : 16.0, es = fp32 ? 4.0 : 8.0;
-    const double g = (double)nx * ny * nz, G1 = (double)s1 * nrhs, G2 = (double)s2 * nrhs;
+    PassDesc &d1 = cp->d[0];
+    if ((rc = make_desc(ctx, g1, &d1, style[0]))) return rc;
+    d1.map_in = app.map_in.p;
+    d1.vec_in = vec_in + app.in_off;
+    d1.vec_in_ld = ld;
+    d1.nrhs = nrhs;
+    d1.in_rhs = (int64_t)nx * ny * nz;  // only used to strip the rhs offset from map positions
+    d1.out_rhs = s1;
+    cp->src[0] = nullptr;
+    cp->dst[0] = w1;
     // algorithmic bytes: every pass reads its input once and writes its
     // output once; the scatter/gather passes also read the 0/1 position map
     // and the unknowns it selects (the gather read-modify-writes them)
-    int pi = ctx->prof_begin(PK_FFT_X_FWD, 4.0 * g + 16.0 * app.nnz_in * nrhs + e * G1);
-    if ((rc = launch_pass(ctx, fp32, 0, d, nullptr, w1))) return rc;
-    ctx->prof_end(pi);
+    cp->bytes[0] = 4.0 * g + 16.0 * app.nnz_in * nrhs + e * G1;
     // 2: y forward; lines (x', z)
     PassGeom g2 = {(int)Ly, ny, (int)Ly, (int)Lx, nz, 1, Lx * ny, Lx, 1, Lx * Ly, Lx};
-    if ((rc = make_desc(ctx, g2, &d))) return rc;
-    d.nrhs = nrhs;
-    d.in_rhs = s1;
-    d.out_rhs = s2;
-    pi = ctx->prof_begin(PK_FFT_Y_FWD, e * (G1 + G2));
-    if ((rc = launch_pass(ctx, fp32, 0, d, w1, w2))) return rc;
-    ctx->prof_end(pi);
+    PassDesc &d2 = cp->d[1];
+    if ((rc = make_desc(ctx, g2, &d2, style[1]))) return rc;
+    d2.nrhs = nrhs;
+    d2.in_rhs = s1;
+    d2.out_rhs = s2;
+    cp->src[1] = w1;
+    cp->dst[1] = w2;
+    cp->bytes[1] = e * (G1 + G2);
     // 3: z forward * spectrum * z inverse, in place; lines (x', y')
     PassGeom g3 = {(int)Lz, nz, nz, (int)Lx, (int)Ly, 1, Lx, Lx * Ly, 1, Lx, Lx * Ly};
-    if ((rc = make_desc(ctx, g3, &d))) return rc;
-    d.nrhs = nrhs;
-    d.in_rhs = s2;
-    d.out_rhs = s2;
-    d.spec = fp32 ? (const void *)K.spec32.p : (const void *)K.spec64.p;
-    d.spec_sb = 1;
-    d.spec_s2 = Lx;
-    d.spec_sj = Lx * Ly;
-    pi = ctx->prof_begin(PK_FFT_Z_CONV, 2.0 * e * G2 + es * (double)K.embedded());
-    if ((rc = launch_pass(ctx, fp32, 2, d, w2, w2))) return rc;
-    ctx->prof_end(pi);
+    PassDesc &d3 = cp->d[2];
+    if ((rc = make_desc(ctx, g3, &d3, style[2]))) return rc;
+    d3.nrhs = nrhs;
+    d3.in_rhs = s2;
+    d3.out_rhs = s2;
+    d3.spec = fp32 ? (const void *)K.spec32.p : (const void *)K.spec64.p;
+    d3.spec_sb = 1;
+    d3.spec_s2 = Lx;
+    d3.spec_sj = Lx * Ly;
+    cp->src[2] = w2;
+    cp->dst[2] = w2;
+    cp->bytes[2] = 2.0 * e * G2 + es * (double)K.embedded();
     // 4: y inverse, keep ny rows; lines (x', z)
     PassGeom g4 = {(int)Ly, (int)Ly, ny, (int)Lx, nz, 1, Lx * Ly, Lx, 1, Lx * ny, Lx};
-    if ((rc = make_desc(ctx, g4, &d))) return rc;
-    d.nrhs = nrhs;
-    d.in_rhs = s2;
-    d.out_rhs = s1;
-    pi = ctx->prof_begin(PK_FFT_Y_INV, e * (G1 + G2));
-    if ((rc = launch_pass(ctx, fp32, 1, d, w2, w1))) return rc;
-    ctx->prof_end(pi);
+    PassDesc &d4 = cp->d[3];
+    if ((rc = make_desc(ctx, g4, &d4, style[3]))) return rc;
+    d4.nrhs = nrhs;
+    d4.in_rhs = s2;
+    d4.out_rhs = s1;
+    cp->src[3] = w2;
+    cp->dst[3] = w1;
+    cp->bytes[3] = e * (G1 + G2);
     // 5: x inverse with gather; lines (y, z)
     PassGeom g5 = {(int)Lx, (int)Lx, nx, ny, nz, Lx, Lx * ny, 1, nx, (int64_t)nx * ny, 1};
-    if ((rc = make_desc(ctx, g5, &d))) return rc;
-    d.nrhs = nrhs;
-    d.in_rhs = s1;
-    d.out_rhs = (int64_t)nx * ny * nz;
-    d.map_out = app.map_out.p;
-    d.vec_out = vec_out + app.out_off;
-    d.vec_out_ld = ld;
-    pi = ctx->prof_begin(PK_FFT_X_INV, e * G1 + 4.0 * g + 32.0 * app.nnz_out * nrhs);
-    if ((rc = launch_pass(ctx, fp32, 1, d, w1, nullptr))) return rc;
-    ctx->prof_end(pi);
+    PassDesc &d5 = cp->d[4];
+    if ((rc = make_desc(ctx, g5, &d5, style[4]))) return rc;
+    d5.nrhs = nrhs;
+    d5.in_rhs = s1;
+    d5.out_rhs = (int64_t)nx * ny * nz;
+    d5.map_out = app.map_out.p;
+    d5.vec_out = vec_out + app.out_off;
+    d5.vec_out_ld = ld;
+    cp->src[4] = w1;
+    cp->dst[4] = nullptr;
+    cp->bytes[4] = e * G1 + 4.0 * g + 32.0 * app.nnz_out * nrhs;
+    return PEEC_OK;
+}
+
+// Right-hand sides per block of the fused z pass when untuned: enough
+// blocks (about 2048) to fill the chip, the rest share the spectrum tile.
+static int default_rpb(const PassDesc &d, int v)
+{
+    const int64_t tiles = (int64_t)((d.nb + variant_lines(v) - 1) / variant_lines(v)) * d.n2;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(16, tiles * d.nrhs / 2048));
+}
+
+static FftChoice tuned_choice(const peec_ctx *ctx, int kernel, int pass, bool fp32, int nrhs)
+{
+    auto it = ctx->fft_tune.find(FftTuneKey{kernel, pass, fp32 ? 1 : 0, nrhs});
+    return it == ctx->fft_tune.end() ? FftChoice() : it->second;
+}
+
+static const int kProfKind[5] = {PK_FFT_X_FWD, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV};
+static const int kPassMode[5] = {0, 0, 2, 1, 1};
+
+// One convolution application: vec_out[section] += gather(conv(scatter(vec_in)))
+// for nrhs right-hand sides (vectors at stride ld).  w1/w2 are grid work
+// buffers sized by conv_work_elems.  Launch shapes and radix schedules come
+// from peec_fft_tune when it has run for this nrhs.
+int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *vec_in, double2_t *vec_out,
+               int nrhs, int64_t ld, void *w1, void *w2)
+{
+    const Kernel &K = ctx->kern[app.kernel];
+    if (K.compact() <= 0 || app.nnz_in == 0 || app.nnz_out == 0) return PEEC_OK;
+    FftChoice ch[5];
+    int style[5];
+    for (int p = 0; p < 5; p++) {
+        ch[p] = tuned_choice(ctx, app.kernel, p, fp32, nrhs);
+        style[p] = ch[p].style;
+    }
+    ConvPasses cp;
+    int rc = conv_passes(ctx, app, fp32, vec_in, vec_out, nrhs, ld, w1, w2, style, &cp);
+    if (rc) return rc;
+    for (int p = 0; p < 5; p++) {
+        PassDesc &d = cp.d[p];
+        const int v = ch[p].variant >= 0 ? ch[p].variant : default_variant(d.L, fp32);
+        d.rpb = kPassMode[p] == 2 ? (ch[p].variant >= 0 ? ch[p].rpb : default_rpb(d, std::max(v, 0))) : 1;
+        const int pi = ctx->prof_begin(kProfKind[p], cp.bytes[p]);
+        if ((rc = launch_pass(ctx, fp32, kPassMode[p], d, cp.src[p], cp.dst[p], v))) return rc;
+        ctx->prof_end(pi);
+    }
+    return PEEC_OK;
+}
+
+// Time `reps` launches of one pass (after one warm-up launch), in ms each.
+static int time_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc d, const void *src, void *dst, int v, int reps,
+                     hipEvent_t a, hipEvent_t b, float *ms)
+{
+    int rc;
+    if ((rc = launch_pass(ctx, fp32, mode, d, src, dst, v))) return rc;
+    PEEC_HIP_TRY(hipEventRecord(a, ctx->stream));
+    for (int i = 0; i < reps; i++)
+        if ((rc = launch_pass(ctx, fp32, mode, d, src, dst, v))) return rc;
+    PEEC_HIP_TRY(hipEventRecord(b, ctx->stream));
+    PEEC_HIP_TRY(hipEventSynchronize(b));
+    float t = 0.f;
+    PEEC_HIP_TRY(hipEventElapsedTime(&t, a, b));
+    *ms = t / (float)reps;
+    return PEEC_OK;
+}
+
+extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *report_ms)
+{
+    if (!ctx || nrhs < 1 || (precisions & 3) == 0) {
+        peec_set_error("peec_fft_tune: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->op_ready) {
+        peec_set_error("peec_fft_tune: call peec_operator_set first");
+        return PEEC_ERR_STATE;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t N = ctx->m + ctx->ns + ctx->n;
+    size_t need = 0;
+    for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx, ctx->apps[a].kernel));
+    need *= (size_t)nrhs;
+    if (need > ctx->w1.n && (!ctx->w1.alloc(need) || !ctx->w2.alloc(need))) {
+        peec_set_error("peec_fft_tune: out of device memory for %d-rhs FFT work grids", nrhs);
+        return PEEC_ERR_NOMEM;
+    }
+    DevBuf<double2_t> vin, vout;
+    if (!vin.alloc((size_t)(N * nrhs)) || !vout.alloc((size_t)(N * nrhs))) {
+        peec_set_error("peec_fft_tune: out of device memory for %d right-hand sides", nrhs);
+        return PEEC_ERR_NOMEM;
+    }
+    PEEC_HIP_TRY(hipMemsetAsync(vin.p, 0, vin.n * sizeof(double2_t), ctx->stream));
+    PEEC_HIP_TRY(hipMemsetAsync(vout.p, 0, vout.n * sizeof(double2_t), ctx->stream));
+    PEEC_HIP_TRY(hipMemsetAsync(ctx->w1.p, 0, ctx->w1.n * sizeof(double2_t), ctx->stream));
+    PEEC_HIP_TRY(hipMemsetAsync(ctx->w2.p, 0, ctx->w2.n * sizeof(double2_t), ctx->stream));
+    hipEvent_t ea, eb;
+    PEEC_HIP_TRY(hipEventCreate(&ea));
+    PEEC_HIP_TRY(hipEventCreate(&eb));
+    double total_best = 0.0;
+    int rc = PEEC_OK;
+    bool seen[9] = {};
+    for (int a = 0; a < ctx->napps && rc == PEEC_OK; a++) {
+        const ConvApp &app = ctx->apps[a];
+        if (seen[app.kernel] || ctx->kern[app.kernel].compact() <= 0 || app.nnz_in == 0 || app.nnz_out == 0) continue;
+        seen[app.kernel] = true;
+        for (int prec = 0; prec < 2 && rc == PEEC_OK; prec++) {
+            if (!(precisions & (1 << prec))) continue;
+            const bool fp32 = prec == 1;
+            const int style0[5] = {0, 0, 0, 0, 0};
+            ConvPasses cp;
+            if ((rc = conv_passes(ctx, app, fp32, vin.p, vout.p, nrhs, N, ctx->w1.p, ctx->w2.p, style0, &cp))) break;
+            for (int p = 0; p < 5 && rc == PEEC_OK; p++) {
+                const int mode = kPassMode[p];
+                PassDesc d = cp.d[p];
+                FftChoice best;
+                float tbest = 1e30f;
+                // 1) launch shape at the default radix schedule and rhs grouping
+                for (int v = 0; v < kNumFftVariants && rc == PEEC_OK; v++) {
+                    if (!variant_fits(v, d.L, fp32)) continue;
+                    d.rpb = mode == 2 ? default_rpb(d, v) : 1;
+                    float t;
+                    if ((rc = time_pass(ctx, fp32, mode, d, cp.src[p], cp.dst[p], v, 3, ea, eb, &t))) break;
+                    if (t < tbest) {
+                        tbest = t;
+                        best.variant = v;
+                        best.rpb = d.rpb;
+                        best.style = 0;
+                    }
+                }
+                if (rc || best.variant < 0) break;
+                // 2) right-hand sides per block of the fused z pass
+                if (mode == 2) {
+                    for (int r = 1; r <= 32 && r <= nrhs && rc == PEEC_OK; r *= 2) {
+                        d.rpb = r;
+                        float t;
+                        if ((rc = time_pass(ctx, fp32, mode, d, cp.src[p], cp.dst[p], best.variant, 3, ea, eb, &t)))
+                            break;
+                        if (t < tbest) {
+                            tbest = t;
+                            best.rpb = r;
+                        }
+                    }
+                }
+                // 3) radix schedule (FP32 passes only: the FP64 path keeps
+                // one fixed operation order)
+                if (fp32) {
+                    for (int st = 1; st <= 2 && rc == PEEC_OK; st++) {
+                        PassDesc ds;
+                        PassGeom gg = {d.L, d.nin, d.nout, d.nb, d.n2, d.in_sb, d.in_s2, d.in_sj,
+                                       d.out_sb, d.out_s2, d.out_sj};
+                        if ((rc = make_desc(ctx, gg, &ds, st))) break;
+                        if (ds.nstage == d.nstage && !memcmp(ds.radix, d.radix, sizeof(int) * (size_t)d.nstage))
+                            continue;
+                        PassDesc dd = d;
+                        dd.nstage = ds.nstage;
+                        memcpy(dd.radix, ds.radix, sizeof(dd.radix));
+                        dd.rpb = best.rpb;
+                        float t;
+                        if ((rc = time_pass(ctx, fp32, mode, dd, cp.src[p], cp.dst[p], best.variant, 3, ea, eb, &t)))
+                            break;
+                        if (t < tbest) {
+                            tbest = t;
+                            best.style = st;
+                        }
+                    }
+                }
+                if (rc) break;
+                ctx->fft_tune[FftTuneKey{app.kernel, p, fp32 ? 1 : 0, nrhs}] = best;
+                total_best += tbest;
+            }
+        }
+    }
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    if (rc) return rc;
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (report_ms) *report_ms = total_best;
+    return PEEC_OK;
+}
+
+extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int variant, int rpb,
+                                   int style)
+{
+    if (!ctx || kernel < 0 || kernel > 8 || pass < 0 || pass > 4 || nrhs < 1 || variant < -1 ||
+        variant >= kNumFftVariants || rpb < 1 || style < 0 || style > 2 || (style != 0 && !fp32)) {
+        peec_set_error("peec_fft_set_choice: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    const FftTuneKey key{kernel, pass, fp32 ? 1 : 0, nrhs};
+    if (variant < 0) {
+        ctx->fft_tune.erase(key);
+        return PEEC_OK;
+    }
+    const Kernel &K = ctx->kern[kernel];
+    const int L = K.L[pass == 0 || pass == 4 ? 0 : (pass == 2 ? 2 : 1)];
+    if (!ctx->kernels_ready || !variant_fits(variant, L, fp32 != 0)) {
+        peec_set_error("peec_fft_set_choice: launch variant %d does not fit length %d", variant, L);
+        return PEEC_ERR_ARG;
+    }
+    ctx->fft_tune[key] = FftChoice{variant, rpb, style};
+    return PEEC_OK;
+}
+
+// Launch shape chosen for one pass (for reports): variant lines, LMAX, TPL,
+// rhs per block and radix schedule; -1 in variant when untuned.
+extern "C" int peec_fft_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int *out5)
+{
+    if (!ctx || kernel < 0 || kernel > 8 || pass < 0 || pass > 4 || !out5) {
+        peec_set_error("peec_fft_choice: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    const FftChoice c = tuned_choice(ctx, kernel, pass, fp32 != 0, nrhs);
+    out5[0] = c.variant;
+    out5[1] = c.variant >= 0 ? variant_lines(c.variant) : 0;
+    out5[2] = c.variant >= 0 ? kFftVariants[c.variant].tpl : 0;
+    out5[3] = c.rpb;
+    out5[4] = c.style;
     return PEEC_OK;
 }
 

@@ -41,6 +41,7 @@ void peec_ctx::release_operator()
     hw.clear();
     lu_layout_from_plan = false;
     lu.ready = false;
+    fft_tune.clear();
 }
 
 int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *vec_in, double2_t *vec_out, int nrhs,
@@ -728,6 +729,40 @@ extern "C" int peec_apply_K(peec_ctx *ctx, const peec_c128 *x, peec_c128 *y)
     int rc = apply_K(ctx, ctx->vx.p, ctx->vy.p, 1, N, false, 1);
     if (rc) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(y, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PEEC_OK;
+}
+
+extern "C" int peec_apply_K_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, int precond,
+                                    const peec_c128 *x, peec_c128 *y)
+{
+    if (!ctx || nrhs < 1 || rhs_per_slot < 1 || !x || !y) {
+        peec_set_error("peec_apply_K_batched: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->op_ready || !ctx->freq_ready || (precond && !ctx->lu_ready)) {
+        peec_set_error("peec_apply_K_batched: operator, frequency slots%s must be set first",
+                       precond ? " and factors" : "");
+        return PEEC_ERR_STATE;
+    }
+    if ((nrhs + rhs_per_slot - 1) / rhs_per_slot > ctx->nslots) {
+        peec_set_error("peec_apply_K_batched: %d right-hand sides at %d per slot need more than the %d slots", nrhs,
+                       rhs_per_slot, ctx->nslots);
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t N = ctx->m + ctx->ns + ctx->n;
+    const size_t bytes = (size_t)N * (size_t)nrhs * 16;
+    DevBuf<double2_t> X, Y;
+    if (!X.alloc((size_t)(N * nrhs)) || !Y.alloc((size_t)(N * nrhs))) {
+        peec_set_error("peec_apply_K_batched: out of device memory");
+        return PEEC_ERR_NOMEM;
+    }
+    PEEC_HIP_TRY(hipMemcpyAsync(X.p, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+    int rc = precond ? matvec_device(ctx, X.p, Y.p, nrhs, N, fp32 != 0, rhs_per_slot)
+                     : apply_K(ctx, X.p, Y.p, nrhs, N, fp32 != 0, rhs_per_slot);
+    if (rc) return rc;
+    PEEC_HIP_TRY(hipMemcpyAsync(y, Y.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return PEEC_OK;
 }

@@ -232,6 +232,43 @@ def gmres_batched(B, rhs_per_slot, tol, X0=None, restart=60, max_cycles=20, mixe
     return Xt.T, flag, relres, iters, cycles
 
 
+def apply_K_batched(X, rhs_per_slot=1, fp32=False, precond=False, ctx=None):
+    """K x (or M^-1 K x with precond) for the columns of X (N x nrhs), column
+    r against frequency slot r / rhs_per_slot, with complex128 or (fp32)
+    complex64 FFT convolutions."""
+    ctx = ctx or default_context()
+    Xt = np.ascontiguousarray(np.asarray(X, dtype=np.complex128).T)
+    Yt = np.zeros_like(Xt)
+    L.check(L.lib().peec_apply_K_batched(ctx.handle, Xt.shape[0], int(rhs_per_slot), int(bool(fp32)),
+                                         int(bool(precond)), L.as_c128p(Xt), L.as_c128p(Yt)), "peec_apply_K_batched")
+    return Yt.T
+
+
+def fft_tune(nrhs, fp64=True, fp32=True, ctx=None):
+    """Autotune the FFT-convolution passes for batched calls with nrhs
+    right-hand sides; returns the summed best pass time in ms."""
+    ctx = ctx or default_context()
+    ms = ctypes.c_double(0.0)
+    L.check(L.lib().peec_fft_tune(ctx.handle, int(nrhs), int(fp64) | (int(fp32) << 1), ctypes.byref(ms)),
+            "peec_fft_tune")
+    return ms.value
+
+
+def fft_set_choice(kernel, pass_, fp32, nrhs, variant, rpb=1, style=0, ctx=None):
+    ctx = ctx or default_context()
+    L.check(L.lib().peec_fft_set_choice(ctx.handle, int(kernel), int(pass_), int(bool(fp32)), int(nrhs),
+                                        int(variant), int(rpb), int(style)), "peec_fft_set_choice")
+
+
+def fft_choice(kernel, pass_, fp32, nrhs, ctx=None):
+    """(variant, lines per tile, lanes per line, rhs per block, radix schedule)."""
+    ctx = ctx or default_context()
+    out = (ctypes.c_int * 5)()
+    L.check(L.lib().peec_fft_choice(ctx.handle, int(kernel), int(pass_), int(bool(fp32)), int(nrhs), out),
+            "peec_fft_choice")
+    return tuple(out)
+
+
 def lu_set_factors(Lf, Uf, p, q, Rs=None, ctx=None, blocks=None):
     """F = lu(SS) in Julia's convention L*U = (Rs .* SS)[p, q] (0-based p, q).
     blocks: optional block_ptr cutting the factor order into row blocks

@@ -7,6 +7,8 @@ SuperLU (the reference uses UMFPACK on the host too, :118-119).  The kernel
 spectra, the FFT matvec, the preconditioner's triangular solves and GMRES
 run on the GPU through libpeec_hip.
 """
+import time
+
 import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
@@ -137,6 +139,15 @@ class Solver:
         peec.set_ctx_size(self.ctx, mesh.m + mesh.ns + mesh.n)
         if device_lu:
             peec.lu_analyze(self.dissection, ctx=self.ctx)
+        self.tuned = {}
+
+    def tune(self, nrhs, fp64=True, fp32=True):
+        """Autotune the FFT passes once for batched calls with nrhs
+        right-hand sides (peec_fft_tune); returns the summed best pass ms."""
+        key = (nrhs, fp64, fp32)
+        if key not in self.tuned:
+            self.tuned[key] = peec.fft_tune(nrhs, fp64=fp64, fp32=fp32, ctx=self.ctx)
+        return self.tuned[key]
 
     def set_frequency(self, w, R0=50.0):
         m = self.mesh
@@ -216,15 +227,22 @@ def s2y_fft(S, Zo):
 
 
 def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_cycles=20, inner_tol=0.0,
-                          ports_scatter_value=50.0, solver=None, record=None):
+                          ports_scatter_value=50.0, solver=None, record=None, timings=None, tune=False):
     """The QS sweep of FFT_solver_QS_S_type.jl:60-232 with its frequency x
     port loop batched: `batch` frequencies at a time, every port of each,
     go through one peec_gmres_batched call (one batched matvec per
     iteration; factors per frequency slot on the device).  Warm starts
     chain across batches per port, like Vrest (:140, :196).  mixed = True
     runs the inner cycles with FP32 FFT convolutions.  Returns dict(S, Z, Y,
-    f) as FFT_solver_QS_S_type."""
+    f) as FFT_solver_QS_S_type.  `timings`, a dict, accumulates the seconds
+    spent per phase (host frequency terms, device LU, preconditioned
+    right-hand sides, GMRES); every library call returns synchronised.
+    tune = True autotunes the FFT passes for each batch width first
+    (timed as "tune")."""
     solver = solver or Solver(mesh, device_lu=True)
+    tm = timings if timings is not None else {}
+    for key in ("tune", "freq_terms", "lu", "precond_rhs", "gmres"):
+        tm.setdefault(key, 0.0)
     if not solver.device_lu:
         raise ValueError("the batched sweep needs a Solver with device_lu=True")
     esc = mesh.escalings
@@ -244,10 +262,19 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_
         if nslots != nb:
             peec.frequency_slots(nb, ctx=solver.ctx)
             nslots = nb
+        if tune:
+            t0 = time.perf_counter()
+            solver.tune(nb * P, fp64=True, fp32=mixed)
+            tm["tune"] += time.perf_counter() - t0
         for slot, k in enumerate(ks):
+            t0 = time.perf_counter()
             Yle = build_Yle_FFT(mesh.lumped_elements, mesh.port_nodes, esc, n, w[k] / esc["freq"], R0)
             peec.frequency_set_slot(slot, w[k], DZ_of(mesh, w[k]), Yle, ctx=solver.ctx)
+            t1 = time.perf_counter()
             peec.lu_factor_slot(slot, ctx=solver.ctx)
+            tm["freq_terms"] += t1 - t0
+            tm["lu"] += time.perf_counter() - t1
+        t0 = time.perf_counter()
         IS = np.zeros((n, nb * P))
         for slot in range(nb):
             for c1 in range(P):
@@ -256,8 +283,11 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_
                 IS[n2, slot * P + c1] = -esc["Is"]
         TN = peec.precond_rhs_batched(IS, P, ctx=solver.ctx)
         X0 = np.tile(Xprev, (1, nb))
+        t1 = time.perf_counter()
         X, flag, relres, iters, cycles = peec.gmres_batched(TN, P, tol, X0, restart, max_cycles, mixed, inner_tol,
                                                             ctx=solver.ctx)
+        tm["precond_rhs"] += t1 - t0
+        tm["gmres"] += time.perf_counter() - t1
         Xprev = X[:, (nb - 1) * P:nb * P]
         for slot, k in enumerate(ks):
             V = X[:, slot * P:(slot + 1) * P]

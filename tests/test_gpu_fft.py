@@ -1,0 +1,121 @@
+"""FFT-convolution launch shapes and the autotuner.
+
+Every launch shape of the pass kernel (lines per tile, lanes per line,
+right-hand sides per block) runs the same butterflies in the same order,
+so K x must come out bit for bit the same under each of them, in complex128
+and in complex64.  The complex64 radix schedules change rounding only:
+they stay within complex64 accuracy of the complex128 result.  The
+complex128 result itself is held to the oracle's K x (1e-12, relative).
+"""
+import numpy as np
+import pytest
+
+from oracle import fft_solver as OF
+from solver_amd import _lib, configs, mesher, peec, sweep
+
+from test_gpu_operator import pself, rel
+
+pytestmark = pytest.mark.gpu
+
+NRHS = 3
+N_VARIANTS = 11
+
+
+@pytest.fixture(scope="module")
+def batched():
+    cell = (0.25e-3, 0.25e-3, 0.05e-3)
+    g, mats, _ = configs._layered(20, 14, 6, cell, 4, 0.45, 4.4, 0.02, seed=3)
+    ports = configs._x_face_ports(g, cell, 4)
+    M = mesher.mesh(g, mats, cell, (0.0, 0.0, 0.0), ports, p_self_fn=pself)
+    S = sweep.Solver(M, device_lu=True)
+    ws = [2 * np.pi * f * M.escalings["freq"] for f in (3e7, 1e9, 8e9)]
+    peec.frequency_slots(NRHS, ctx=S.ctx)
+    for slot, w in enumerate(ws):
+        Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+        peec.frequency_set_slot(slot, w, sweep.DZ_of(M, w), Yle, ctx=S.ctx)
+    N = M.m + M.ns + M.n
+    rng = np.random.default_rng(11)
+    X = rng.normal(size=(N, NRHS)) + 1j * rng.normal(size=(N, NRHS))
+    dims = sorted({n for k in range(9) for n in OF.kernel_dims(k, M.Nx, M.Ny, M.Nz)})
+    print("mesh n=%d m=%d ns=%d; compact kernel dims %s" % (M.n, M.m, M.ns, dims), flush=True)
+    return M, S, ws, X
+
+
+def bitwise_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.float64), np.ascontiguousarray(b).view(np.float64))
+
+
+def set_all(S, fp32, variant, rpb, style):
+    """Force one launch shape on every pass where it fits; returns how many
+    passes took it."""
+    took = 0
+    for k in range(9):
+        for p in range(5):
+            try:
+                peec.fft_set_choice(k, p, fp32, NRHS, variant, rpb if p == 2 else 1, style, ctx=S.ctx)
+                took += 1
+            except _lib.PeecError:
+                peec.fft_set_choice(k, p, fp32, NRHS, -1, ctx=S.ctx)
+    return took
+
+
+def clear_all(S):
+    for fp32 in (False, True):
+        for k in range(9):
+            for p in range(5):
+                peec.fft_set_choice(k, p, fp32, NRHS, -1, ctx=S.ctx)
+
+
+def test_default_shapes_match_oracle(batched):
+    M, S, ws, X = batched
+    clear_all(S)
+    rows = OF.kernel_rows(M.origin, M.Nx, M.Ny, M.Nz, M.sx, M.sy, M.sz)
+    spec = OF.spectra(rows, M.Nx, M.Ny, M.Nz)
+    Y = peec.apply_K_batched(X, 1, ctx=S.ctx)
+    for r, w in enumerate(ws):
+        op = OF.Operator(M, spec)
+        Yle = OF.build_Yle(M.lumped_elements, M.port_nodes, M.escalings["Yle"], M.n, w / M.escalings["freq"], 50.0)
+        op.set_frequency(w, OF.frequency_terms(M, w), Yle)
+        e = rel(Y[:, r], np.concatenate(op.K(X[:, r])))
+        print("slot %d: K x vs oracle %.2e" % (r, e), flush=True)
+        assert e < 1e-12
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_every_launch_shape_is_bitwise_identical(batched, fp32):
+    M, S, ws, X = batched
+    clear_all(S)
+    ref = peec.apply_K_batched(X, 1, fp32=fp32, ctx=S.ctx)
+    assert np.all(np.isfinite(ref))
+    for v in range(N_VARIANTS):
+        for rpb in (1, 2, 3):
+            took = set_all(S, fp32, v, rpb, 0)
+            if took == 0:
+                continue
+            Y = peec.apply_K_batched(X, 1, fp32=fp32, ctx=S.ctx)
+            same = bitwise_equal(Y, ref)
+            print("fp32=%s variant %d rpb %d on %d passes: bitwise %s" % (fp32, v, rpb, took, same), flush=True)
+            assert same
+    clear_all(S)
+
+
+def test_fp32_radix_schedules_and_tuner(batched):
+    M, S, ws, X = batched
+    clear_all(S)
+    y64 = peec.apply_K_batched(X, 1, fp32=False, ctx=S.ctx)
+    for style in (0, 1, 2):
+        assert set_all(S, True, 8, 1, style) > 0
+        y32 = peec.apply_K_batched(X, 1, fp32=True, ctx=S.ctx)
+        e = rel(y32, y64)
+        print("complex64 radix schedule %d: rel err vs complex128 %.2e" % (style, e), flush=True)
+        assert e < 1e-5
+    clear_all(S)
+    ms = peec.fft_tune(NRHS, ctx=S.ctx)
+    print("tuned: summed best pass time %.3f ms" % ms, flush=True)
+    for k in range(9):
+        print("kernel %d fp64" % k, [peec.fft_choice(k, p, False, NRHS, ctx=S.ctx) for p in range(5)],
+              "fp32", [peec.fft_choice(k, p, True, NRHS, ctx=S.ctx) for p in range(5)], flush=True)
+    y64t = peec.apply_K_batched(X, 1, fp32=False, ctx=S.ctx)
+    assert bitwise_equal(y64t, y64)
+    y32t = peec.apply_K_batched(X, 1, fp32=True, ctx=S.ctx)
+    assert rel(y32t, y64) < 1e-5
