@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
 
 // ------------------------------------------------------------------ plans
 int fft_smooth_length(int nmin)
@@ -200,10 +201,11 @@ __device__ __forceinline__ void dftR(C2<T> *v)
 
 // One Stockham stage over a tile of W = (256 / TPL) * LPT lines held in LDS
 // (line stride LS).  Lane `lane` of line group `lg` owns butterflies
-// k = lane + q * TPL of lines lg + lp * (256 / TPL).
+// k = lane + q * TPL of lines lg + lp * (256 / TPL).  twl: the L twiddles
+// exp(-2 pi i t / L) in LDS.
 template <typename T, int DIR, int R, int TPL, int LPT, int LMAX>
 __device__ __forceinline__ void stockham_stage(C2<T> *tile, int LS, int L, int Ns, int lg, int lane,
-                                               const double2_t *__restrict__ tw)
+                                               const C2<T> *twl)
 {
     constexpr int G = 256 / TPL;
     constexpr int MAXB = (LMAX + R * TPL - 1) / (R * TPL);
@@ -234,8 +236,8 @@ __device__ __forceinline__ void stockham_stage(C2<T> *tile, int LS, int L, int N
                 if (Ns > 1) {
 #pragma unroll
                     for (int r = 1; r < R; r++) {
-                        const double2_t w = tw[kk * r * stride];
-                        const C2<T> wt = {(T)w.x, (T)(DIR < 0 ? w.y : -w.y)};
+                        const C2<T> w = twl[kk * r * stride];
+                        const C2<T> wt = {w.x, DIR < 0 ? w.y : -w.y};
                         v[lp][q][r] = cmul(v[lp][q][r], wt);
                     }
                 }
@@ -271,17 +273,17 @@ struct PassDesc {
 };
 
 template <typename T, int DIR, int TPL, int LPT, int LMAX>
-__device__ void run_fft(C2<T> *tile, int LS, int L, const PassDesc &d, int lg, int lane, const double2_t *tw)
+__device__ void run_fft(C2<T> *tile, int LS, int L, const PassDesc &d, int lg, int lane, const C2<T> *twl)
 {
     int Ns = 1;
     for (int s = 0; s < d.nstage; s++) {
         const int R = d.radix[s];
         switch (R) {
-        case 8: stockham_stage<T, DIR, 8, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, tw); break;
-        case 4: stockham_stage<T, DIR, 4, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, tw); break;
-        case 2: stockham_stage<T, DIR, 2, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, tw); break;
-        case 3: stockham_stage<T, DIR, 3, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, tw); break;
-        default: stockham_stage<T, DIR, 5, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, tw); break;
+        case 8: stockham_stage<T, DIR, 8, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl); break;
+        case 4: stockham_stage<T, DIR, 4, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl); break;
+        case 2: stockham_stage<T, DIR, 2, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl); break;
+        case 3: stockham_stage<T, DIR, 3, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl); break;
+        default: stockham_stage<T, DIR, 5, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl); break;
         }
         Ns *= R;
     }
@@ -291,6 +293,15 @@ __device__ void run_fft(C2<T> *tile, int LS, int L, const PassDesc &d, int lg, i
 // is at least 1/(2n) away from an integer, far beyond the rounding error.
 __device__ __forceinline__ int qdiv(int e, float inv) { return (int)(((float)e + 0.5f) * inv); }
 
+// Input elements a thread carries from one right-hand side to the next:
+// forward passes read at most (L + 1) / 2 of L (the zero padding of the
+// circulant embedding), inverse passes the whole line.
+template <int MODE, int W, int LMAX>
+constexpr int fft_prefetch_slots()
+{
+    return MODE == 1 ? (W * LMAX + 255) / 256 : (W * (LMAX / 2 + 1) + 255) / 256;
+}
+
 // MODE: 0 forward, 1 inverse, 2 forward * spectrum * inverse.
 // A block transforms a tile of W consecutive lines (positions b0 .. b0+W-1
 // along the tiled axis) at line index l2, for right-hand sides
@@ -298,8 +309,13 @@ __device__ __forceinline__ int qdiv(int e, float inv) { return (int)(((float)e +
 // read and written line after line; strided lines are read and written
 // row by row across the W lines (coalesced along the tiled axis).  The LDS
 // line stride is L + 1 so that both access orders avoid bank conflicts.
-// For MODE 2 the block's spectrum tile is read once into registers and
-// reused for all its right-hand sides.
+//
+// Latency hiding: the input of the next right-hand side is loaded into
+// registers while the current one is transformed (the positions, and for
+// the scatter pass the unknown indices, are computed once per block); the
+// twiddles of the line length sit in LDS; for MODE 2 the block's spectrum
+// tile is read once into registers and reused for all its right-hand
+// sides.
 template <typename T, int MODE, int TPL, int LPT, int LMAX>
 __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, C2<T> *dst,
                                                   const double2_t *__restrict__ twall)
@@ -307,17 +323,25 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, 
     constexpr int G = 256 / TPL;
     constexpr int W = G * LPT;
     constexpr int NSPEC = MODE == 2 ? (W * LMAX + 255) / 256 : 1;
+    constexpr int NPF = fft_prefetch_slots<MODE, W, LMAX>();
     extern __shared__ char smem_raw[];
-    C2<T> *tile = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = d.L, LS = L + 1;
+    C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw);
+    C2<T> *tile = twl + L;
     const int tid = threadIdx.x;
     const int b0 = blockIdx.x * W;
     const int l2 = blockIdx.y;
     const int nbt = min(W, d.nb - b0);
-    const double2_t *tw = twall + d.tw_off;
     const int lg = tid / TPL, lane = tid % TPL;
     const bool in_contig = (d.in_sj == 1), out_contig = (d.out_sj == 1);
-    const float invL = 1.0f / (float)L, invNout = 1.0f / (float)d.nout;
+    const float invNin = 1.0f / (float)d.nin, invNout = 1.0f / (float)d.nout;
+    const int nin_tile = W * d.nin;
+    const bool pf = nin_tile <= 256 * NPF;  // else: load straight into LDS
+
+    for (int t = tid; t < L; t += 256) {
+        const double2_t w = twall[d.tw_off + t];
+        twl[t] = {(T)w.x, (T)w.y};
+    }
 
     T sv[NSPEC];
     if (MODE == 2) {
@@ -331,43 +355,96 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, 
         }
     }
 
-    const int r_end = min(d.nrhs, ((int)blockIdx.z + 1) * d.rpb);
-    for (int rhs = (int)blockIdx.z * d.rpb; rhs < r_end; rhs++) {
-        // ---- load (zero padding beyond nin and beyond the last line)
-        const int64_t ibase = (int64_t)b0 * d.in_sb + (int64_t)l2 * d.in_s2 + (int64_t)rhs * d.in_rhs;
+    // input positions of this thread's slots: offset inside one right-hand
+    // side's tile, or the unknown index for the scatter pass; -1 = zero
+    const int64_t ibase = (int64_t)b0 * d.in_sb + (int64_t)l2 * d.in_s2;
+    int32_t poff[NPF];
+    C2<T> pv[NPF];
+    auto slot_bj = [&](int e, int &b, int &j) {
         if (in_contig) {
-            for (int e = tid; e < W * L; e += 256) {
-                const int b = qdiv(e, invL), j = e - b * L;
+            b = qdiv(e, invNin);
+            j = e - b * d.nin;
+        } else {
+            b = e % W;
+            j = e / W;
+        }
+    };
+    if (pf) {
+#pragma unroll
+        for (int i = 0; i < NPF; i++) {
+            const int e = tid + 256 * i;
+            int o = -1;
+            if (e < nin_tile) {
+                int b, j;
+                slot_bj(e, b, j);
+                if (b < nbt) {
+                    const int64_t p = (int64_t)b * d.in_sb + (int64_t)j * d.in_sj;
+                    o = d.map_in ? d.map_in[ibase + p] : (int32_t)p;
+                }
+            }
+            poff[i] = o;
+        }
+    }
+    auto fetch = [&](int rhs) {
+#pragma unroll
+        for (int i = 0; i < NPF; i++) {
+            C2<T> v = {(T)0, (T)0};
+            if (poff[i] >= 0) {
+                if (d.map_in) {
+                    const double2_t x = d.vec_in[(int64_t)poff[i] + (int64_t)rhs * d.vec_in_ld];
+                    v = {(T)x.x, (T)x.y};
+                } else {
+                    v = src[ibase + (int64_t)rhs * d.in_rhs + poff[i]];
+                }
+            }
+            pv[i] = v;
+        }
+    };
+
+    const int r0 = (int)blockIdx.z * d.rpb;
+    const int r_end = min(d.nrhs, r0 + d.rpb);
+    if (pf && r0 < r_end) fetch(r0);
+    for (int rhs = r0; rhs < r_end; rhs++) {
+        // ---- fill the tile: the input rows, zeros beyond nin
+        if (pf) {
+#pragma unroll
+            for (int i = 0; i < NPF; i++) {
+                const int e = tid + 256 * i;
+                if (e < nin_tile) {
+                    int b, j;
+                    slot_bj(e, b, j);
+                    tile[b * LS + j] = pv[i];
+                }
+            }
+            if (rhs + 1 < r_end) fetch(rhs + 1);
+        } else {
+            for (int e = tid; e < nin_tile; e += 256) {
+                int b, j;
+                slot_bj(e, b, j);
                 C2<T> v = {(T)0, (T)0};
-                if (b < nbt && j < d.nin) {
-                    const int64_t p = ibase + (int64_t)b * d.in_sb + j;
+                if (b < nbt) {
+                    const int64_t p = ibase + (int64_t)b * d.in_sb + (int64_t)j * d.in_sj;
                     if (d.map_in) {
-                        const int u = d.map_in[p - (int64_t)rhs * d.in_rhs];
+                        const int u = d.map_in[p];
                         if (u >= 0) {
                             const double2_t x = d.vec_in[(int64_t)u + (int64_t)rhs * d.vec_in_ld];
                             v = {(T)x.x, (T)x.y};
                         }
                     } else {
-                        v = src[p];
+                        v = src[p + (int64_t)rhs * d.in_rhs];
                     }
                 }
                 tile[b * LS + j] = v;
             }
-        } else {
-            for (int e = tid; e < W * L; e += 256) {
-                const int b = e % W, j = e / W;
-                C2<T> v = {(T)0, (T)0};
-                if (b < nbt && j < d.nin) v = src[ibase + (int64_t)b * d.in_sb + (int64_t)j * d.in_sj];
-                tile[b * LS + j] = v;
-            }
         }
+        for (int e = tid; e < W * (L - d.nin); e += 256) tile[(e % W) * LS + d.nin + e / W] = {(T)0, (T)0};
         __syncthreads();
 
         // ---- transform
         if (MODE == 1) {
-            run_fft<T, 1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, tw);
+            run_fft<T, 1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, twl);
         } else {
-            run_fft<T, -1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, tw);
+            run_fft<T, -1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, twl);
             if (MODE == 2) {
 #pragma unroll
                 for (int i = 0; i < NSPEC; i++) {
@@ -376,7 +453,7 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, 
                     if (j < L) tile[b * LS + j] = cscale(tile[b * LS + j], sv[i]);
                 }
                 __syncthreads();
-                run_fft<T, 1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, tw);
+                run_fft<T, 1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, twl);
             }
         }
 
@@ -465,7 +542,7 @@ static constexpr size_t kFftLdsMax = 64 * 1024;
 static int variant_lines(int v) { return 256 / kFftVariants[v].tpl * kFftVariants[v].lpt; }
 static size_t variant_lds(int v, int L, bool fp32)
 {
-    return (size_t)variant_lines(v) * (size_t)(L + 1) * (fp32 ? 8 : 16);
+    return ((size_t)variant_lines(v) * (size_t)(L + 1) + (size_t)L) * (fp32 ? 8 : 16);
 }
 static bool variant_fits(int v, int L, bool fp32)
 {
@@ -597,6 +674,11 @@ static int launch_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const vo
     if (v < 0) v = default_variant(d.L, fp32);
     if (v < 0 || !variant_fits(v, d.L, fp32)) {
         peec_set_error("no FFT launch shape for length %d", d.L);
+        return PEEC_ERR_ARG;
+    }
+    // the kernel keeps element offsets inside one tile in 32 bits
+    if ((int64_t)variant_lines(v) * d.in_sb + (int64_t)d.L * d.in_sj >= (int64_t)INT32_MAX) {
+        peec_set_error("FFT tile offsets exceed 32 bits (line stride %lld)", (long long)d.in_sj);
         return PEEC_ERR_ARG;
     }
     if (fp32) {
@@ -783,8 +865,9 @@ static int conv_passes(peec_ctx *ctx, const ConvApp &app, bool fp32, const doubl
     return PEEC_OK;
 }
 
-// Right-hand sides per block of the fused z pass when untuned: enough
-// blocks (about 2048) to fill the chip, the rest share the spectrum tile.
+// Right-hand sides per block when untuned: enough blocks (about 2048) to
+// fill the chip; the rest are pipelined through each block (and share its
+// spectrum tile in the fused z pass).
 static int default_rpb(const PassDesc &d, int v)
 {
     const int64_t tiles = (int64_t)((d.nb + variant_lines(v) - 1) / variant_lines(v)) * d.n2;
@@ -821,7 +904,7 @@ int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *ve
     for (int p = 0; p < 5; p++) {
         PassDesc &d = cp.d[p];
         const int v = ch[p].variant >= 0 ? ch[p].variant : default_variant(d.L, fp32);
-        d.rpb = kPassMode[p] == 2 ? (ch[p].variant >= 0 ? ch[p].rpb : default_rpb(d, std::max(v, 0))) : 1;
+        d.rpb = ch[p].variant >= 0 ? ch[p].rpb : default_rpb(d, std::max(v, 0));
         const int pi = ctx->prof_begin(kProfKind[p], cp.bytes[p]);
         if ((rc = launch_pass(ctx, fp32, kPassMode[p], d, cp.src[p], cp.dst[p], v))) return rc;
         ctx->prof_end(pi);
@@ -852,6 +935,7 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
         peec_set_error("peec_fft_tune: invalid argument");
         return PEEC_ERR_ARG;
     }
+    const bool verbose = (precisions & 4) != 0;
     if (!ctx->op_ready) {
         peec_set_error("peec_fft_tune: call peec_operator_set first");
         return PEEC_ERR_STATE;
@@ -898,9 +982,14 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
                 // 1) launch shape at the default radix schedule and rhs grouping
                 for (int v = 0; v < kNumFftVariants && rc == PEEC_OK; v++) {
                     if (!variant_fits(v, d.L, fp32)) continue;
-                    d.rpb = mode == 2 ? default_rpb(d, v) : 1;
+                    d.rpb = default_rpb(d, v);
                     float t;
                     if ((rc = time_pass(ctx, fp32, mode, d, cp.src[p], cp.dst[p], v, 3, ea, eb, &t))) break;
+                    if (verbose)
+                        fprintf(stderr, "peec_fft_tune: kernel %d pass %d %s L %d nin %d nout %d: shape %d (W %d TPL %d) "
+                                        "rpb %d: %.1f us, %.0f GB/s\n", app.kernel, p, fp32 ? "c64" : "c128", d.L, d.nin,
+                                d.nout, v, variant_lines(v), kFftVariants[v].tpl, d.rpb, 1e3 * t,
+                                cp.bytes[p] / (1e6 * t));
                     if (t < tbest) {
                         tbest = t;
                         best.variant = v;
@@ -909,13 +998,16 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
                     }
                 }
                 if (rc || best.variant < 0) break;
-                // 2) right-hand sides per block of the fused z pass
-                if (mode == 2) {
+                // 2) right-hand sides per block
+                {
                     for (int r = 1; r <= 32 && r <= nrhs && rc == PEEC_OK; r *= 2) {
                         d.rpb = r;
                         float t;
                         if ((rc = time_pass(ctx, fp32, mode, d, cp.src[p], cp.dst[p], best.variant, 3, ea, eb, &t)))
                             break;
+                        if (verbose)
+                            fprintf(stderr, "peec_fft_tune:   rpb %d: %.1f us, %.0f GB/s\n", r, 1e3 * t,
+                                    cp.bytes[p] / (1e6 * t));
                         if (t < tbest) {
                             tbest = t;
                             best.rpb = r;

@@ -244,12 +244,13 @@ def apply_K_batched(X, rhs_per_slot=1, fp32=False, precond=False, ctx=None):
     return Yt.T
 
 
-def fft_tune(nrhs, fp64=True, fp32=True, ctx=None):
+def fft_tune(nrhs, fp64=True, fp32=True, verbose=False, ctx=None):
     """Autotune the FFT-convolution passes for batched calls with nrhs
     right-hand sides; returns the summed best pass time in ms."""
     ctx = ctx or default_context()
     ms = ctypes.c_double(0.0)
-    L.check(L.lib().peec_fft_tune(ctx.handle, int(nrhs), int(fp64) | (int(fp32) << 1), ctypes.byref(ms)),
+    flags = int(fp64) | (int(fp32) << 1) | (int(verbose) << 2)
+    L.check(L.lib().peec_fft_tune(ctx.handle, int(nrhs), flags, ctypes.byref(ms)),
             "peec_fft_tune")
     return ms.value
 

@@ -141,12 +141,12 @@ class Solver:
             peec.lu_analyze(self.dissection, ctx=self.ctx)
         self.tuned = {}
 
-    def tune(self, nrhs, fp64=True, fp32=True):
+    def tune(self, nrhs, fp64=True, fp32=True, verbose=False):
         """Autotune the FFT passes once for batched calls with nrhs
         right-hand sides (peec_fft_tune); returns the summed best pass ms."""
         key = (nrhs, fp64, fp32)
         if key not in self.tuned:
-            self.tuned[key] = peec.fft_tune(nrhs, fp64=fp64, fp32=fp32, ctx=self.ctx)
+            self.tuned[key] = peec.fft_tune(nrhs, fp64=fp64, fp32=fp32, verbose=verbose, ctx=self.ctx)
         return self.tuned[key]
 
     def set_frequency(self, w, R0=50.0):
@@ -226,8 +226,25 @@ def s2y_fft(S, Zo):
     return np.stack([Zo * np.linalg.solve(I + S[:, :, k], I - S[:, :, k]) for k in range(S.shape[2])], axis=2)
 
 
+def set_slots(solver, ws, R0=50.0, timings=None):
+    """Fill frequency slots 0 .. len(ws)-1 (already allocated) with the
+    per-frequency terms at scaled angular frequencies ws and factor each
+    slot's SS on the device (FFT_solver_QS_S_type.jl:105-119 per slot)."""
+    mesh = solver.mesh
+    esc = mesh.escalings
+    tm = timings if timings is not None else {}
+    for slot, w in enumerate(ws):
+        t0 = time.perf_counter()
+        Yle = build_Yle_FFT(mesh.lumped_elements, mesh.port_nodes, esc, mesh.n, w / esc["freq"], R0)
+        peec.frequency_set_slot(slot, w, DZ_of(mesh, w), Yle, ctx=solver.ctx)
+        t1 = time.perf_counter()
+        peec.lu_factor_slot(slot, ctx=solver.ctx)
+        tm["freq_terms"] = tm.get("freq_terms", 0.0) + t1 - t0
+        tm["lu"] = tm.get("lu", 0.0) + time.perf_counter() - t1
+
+
 def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_cycles=20, inner_tol=0.0,
-                          ports_scatter_value=50.0, solver=None, record=None, timings=None, tune=False):
+                          ports_scatter_value=50.0, solver=None, record=None, timings=None, tune=False, Vrest=None):
     """The QS sweep of FFT_solver_QS_S_type.jl:60-232 with its frequency x
     port loop batched: `batch` frequencies at a time, every port of each,
     go through one peec_gmres_batched call (one batched matvec per
@@ -238,7 +255,9 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_
     spent per phase (host frequency terms, device LU, preconditioned
     right-hand sides, GMRES); every library call returns synchronised.
     tune = True autotunes the FFT passes for each batch width first
-    (timed as "tune")."""
+    (timed as "tune").  Vrest (N x P) is the warm start of the first batch
+    (zeros by default); the result's "Vrest" is the last frequency's
+    solution, to chain a sweep split over several calls."""
     solver = solver or Solver(mesh, device_lu=True)
     tm = timings if timings is not None else {}
     for key in ("tune", "freq_terms", "lu", "precond_rhs", "gmres"):
@@ -254,7 +273,7 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_
     P = mesh.port_nodes.shape[0]
     w = 2 * np.pi * freq
     S = np.zeros((P, P, F), dtype=np.complex128)
-    Xprev = np.zeros((N, P), dtype=np.complex128)
+    Xprev = np.zeros((N, P), dtype=np.complex128) if Vrest is None else np.asarray(Vrest, dtype=np.complex128)
     nslots = None
     for g0 in range(0, F, batch):
         ks = list(range(g0, min(F, g0 + batch)))
@@ -266,14 +285,7 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_
             t0 = time.perf_counter()
             solver.tune(nb * P, fp64=True, fp32=mixed)
             tm["tune"] += time.perf_counter() - t0
-        for slot, k in enumerate(ks):
-            t0 = time.perf_counter()
-            Yle = build_Yle_FFT(mesh.lumped_elements, mesh.port_nodes, esc, n, w[k] / esc["freq"], R0)
-            peec.frequency_set_slot(slot, w[k], DZ_of(mesh, w[k]), Yle, ctx=solver.ctx)
-            t1 = time.perf_counter()
-            peec.lu_factor_slot(slot, ctx=solver.ctx)
-            tm["freq_terms"] += t1 - t0
-            tm["lu"] += time.perf_counter() - t1
+        set_slots(solver, w[ks], R0, tm)
         t0 = time.perf_counter()
         IS = np.zeros((n, nb * P))
         for slot in range(nb):
@@ -301,4 +313,4 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_
                     dv = V[m + ns + n3, c1] - V[m + ns + n4, c1]
                     S[c1, c2, k] = (2 * dv - R0) / R0 if c1 == c2 else 2 * dv / R0
                     S[c2, c1, k] = S[c1, c2, k]
-    return dict(S=S, Z=s2z_fft(S, R0), Y=s2y_fft(S, R0), f=freq / esc["freq"])
+    return dict(S=S, Z=s2z_fft(S, R0), Y=s2y_fft(S, R0), f=freq / esc["freq"], Vrest=Xprev)

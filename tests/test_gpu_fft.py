@@ -1,7 +1,8 @@
 """FFT-convolution launch shapes and the autotuner.
 
 Every launch shape of the pass kernel (lines per tile, lanes per line,
-right-hand sides per block) runs the same butterflies in the same order,
+right-hand sides per block, hence also the prefetching of the next
+right-hand side) runs the same butterflies in the same order,
 so K x must come out bit for bit the same under each of them, in complex128
 and in complex64.  The complex64 radix schedules change rounding only:
 they stay within complex64 accuracy of the complex128 result.  The
@@ -52,7 +53,7 @@ def set_all(S, fp32, variant, rpb, style):
     for k in range(9):
         for p in range(5):
             try:
-                peec.fft_set_choice(k, p, fp32, NRHS, variant, rpb if p == 2 else 1, style, ctx=S.ctx)
+                peec.fft_set_choice(k, p, fp32, NRHS, variant, rpb, style, ctx=S.ctx)
                 took += 1
             except _lib.PeecError:
                 peec.fft_set_choice(k, p, fp32, NRHS, -1, ctx=S.ctx)
