@@ -140,8 +140,8 @@ def main():
                          "reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X, and the sweep shards "
-              "by frequency across GPUs (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: every box here has a single MI355X (the frequency sweep would shard "
+              "across GPUs by frequency, one rank per GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from solver_amd import configs, mesher, peec, sweep

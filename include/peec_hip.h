@@ -237,13 +237,43 @@ int peec_fft_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int
  * Non-zero radix schedules are accepted for complex64 only. */
 int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int variant, int rpb, int style);
 
-/* K x (precond = 0) or M^-1 K x (precond = 1) for nrhs vectors (N x nrhs,
+/* K x (precond = 0), M^-1 K x (precond = 1) or M^-1 x (precond = 2, the
+ * preconditioner alone) for nrhs vectors (N x nrhs,
  * column-major), vector r against frequency slot r / rhs_per_slot, with
  * the FFT convolutions in complex128 (fp32 = 0) or complex64 (fp32 = 1):
  * the batched form of ComputeMatrixVectorFTT (compute_Matrix_vector_fft.jl
  * :1-146) that peec_gmres_batched applies, exposed for checks. */
 int peec_apply_K_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, int precond, const peec_c128 *x,
                          peec_c128 *y);
+
+/* ---- adjoint (conductivity gradients) ------------------------------------
+ * No counterpart in the reference, which has no derivatives.  K^H is
+ * applied explicitly: the 12 convolutions of K (Lp and P are real and
+ * symmetric) with the conjugate-transposed sparse blocks and frequency
+ * factors (-j w Lp I + conj(DZ) I - A Phi; P Q - j w Gamma' Phi;
+ * A' I - Gamma Q + conj(Yle) Phi).  The adjoint preconditioner M^-H reuses
+ * the forward factors of SS through the reciprocity similarity
+ * M^T = D1 M D2, D1 = diag(I, j w, -I), D2 = diag(I, 1/(j w), -I):
+ * M^-H y = conj(D2^-1 M^-1 D1^-1 conj(y)).
+ *
+ * peec_apply_KH_batched: K^H x (precond = 0), M^-H K^H x (precond = 1) or
+ * M^-H x (precond = 2), as peec_apply_K_batched.
+ * peec_precond_rhs_batched_adjoint: tn_r = M^-H [0; 0; g_r], g complex node
+ * excitations (n x nrhs, column-major).
+ * peec_gmres_batched_adjoint: peec_gmres_batched on M^-H K^H lambda = tn. */
+int peec_apply_KH_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, int precond, const peec_c128 *x,
+                          peec_c128 *y);
+int peec_precond_rhs_batched_adjoint(peec_ctx *ctx, int nrhs, int rhs_per_slot, const peec_c128 *g, peec_c128 *tn);
+int peec_gmres_batched_adjoint(peec_ctx *ctx, int nrhs, int rhs_per_slot, const peec_c128 *b, double tol,
+                               int restart, int max_cycles, int mixed, double inner_tol, peec_c128 *x, int *flag,
+                               double *relres, int *iters, int *cycles);
+
+/* HIP graphs (no counterpart in the reference).  With on = 1 (the default)
+ * peec_gmres and peec_gmres_batched capture their matvec (FFT passes,
+ * sparse terms, triangular solves: a few hundred launches) into a HIP graph
+ * at its first use in a call and replay it for every iteration; on = 0
+ * launches the kernels one by one.  Results are identical either way. */
+int peec_set_graphs(peec_ctx *ctx, int on);
 
 /* Kernel timing (no counterpart in the reference, which only prints @time,
  * do_solving_fft.jl:133).  Reads and clears the per-kind totals recorded

@@ -82,8 +82,9 @@ end
 
 # gmres_custom.jl:1-331.  The operator, the frequency data and the LU factors
 # are loaded into the context by FFT_solver_QS_S_type's replacement through
-# peec_operator_set / peec_frequency_set / peec_lu_set_factors (see
-# INTEGRATION.md); the FFT plans and spectra arguments are then unused.
+# peec_kernels_build / peec_operator_set once per geometry and
+# peec_frequency_set / peec_lu_set_factors per frequency; the FFT plans and
+# spectra arguments are then unused.
 function gmres_custom(b, restarted, tol, maxit, x, wk, incidence_selection, FFTCP, FFTCLp, DZ, Yle, expansions,
                       invZ, invP, lu, PLIVector, PVector, PLI2Vector, P2Vector, chi2Vector, id, chan, portIndex)
     restarted && error("restarted GMRES is dead code in the reference")
@@ -97,4 +98,125 @@ function gmres_custom(b, restarted, tol, maxit, x, wk, incidence_selection, FFTC
         ctx, bb, tol, maxit, xx, flag, relres, it, resvec, nres)
     peec_check(rc, "peec_gmres") == PEEC_STOPPED && return xx, 99, 0, it, 0
     return xx, Int(flag[]), relres[], it, resvec[1:nres[]]
+end
+
+# ---------------------------------------------------------------------------
+# Batched sweep and adjoint (no counterpart in the reference).  Frequency
+# slots hold per-frequency DZ, Yle and device LU factors; right-hand side r of
+# a batched call runs against slot r ÷ rhs_per_slot (0-based).
+
+peec_frequency_slots(ctx, nslots) =
+    peec_check(ccall((:peec_frequency_slots, LIBPEEC), Cint, (Ptr{Cvoid}, Cint), ctx, nslots), "peec_frequency_slots")
+
+function peec_frequency_set_slot(ctx, slot, w, DZ::Vector{ComplexF64}, Yle::SparseMatrixCSC{ComplexF64,Int64})
+    Y = CscC128(Yle)
+    rc = GC.@preserve DZ Yle ccall((:peec_frequency_set_slot, LIBPEEC), Cint,
+        (Ptr{Cvoid}, Cint, Float64, Ptr{ComplexF64}, Ref{CscC128}), ctx, slot, w, DZ, Y)
+    peec_check(rc, "peec_frequency_set_slot")
+    peec_check(ccall((:peec_lu_factor_slot, LIBPEEC), Cint, (Ptr{Cvoid}, Cint), ctx, slot), "peec_lu_factor_slot")
+end
+
+# tn = M^-1 [0; 0; is] (adjoint = false) or M^-H [0; 0; g] (adjoint = true)
+function peec_precond_rhs_batched(ctx, IS::AbstractMatrix, rhs_per_slot, N; adjoint = false)
+    n, nrhs = size(IS)
+    TN = zeros(ComplexF64, N, nrhs)
+    if adjoint
+        G = Matrix{ComplexF64}(IS)
+        rc = GC.@preserve G TN ccall((:peec_precond_rhs_batched_adjoint, LIBPEEC), Cint,
+            (Ptr{Cvoid}, Cint, Cint, Ptr{ComplexF64}, Ptr{ComplexF64}), ctx, nrhs, rhs_per_slot, G, TN)
+    else
+        R = Matrix{Float64}(IS)
+        rc = GC.@preserve R TN ccall((:peec_precond_rhs_batched, LIBPEEC), Cint,
+            (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Ptr{ComplexF64}), ctx, nrhs, rhs_per_slot, R, TN)
+    end
+    peec_check(rc, "peec_precond_rhs_batched")
+    TN
+end
+
+# Restarted GMRES in FP64 iterative refinement for all columns of B at once
+# (mixed = true: complex64 FFT convolutions inside the cycles).
+function peec_gmres_batched(ctx, B::Matrix{ComplexF64}, rhs_per_slot, tol; X0 = zeros(ComplexF64, size(B)),
+                            restart = 60, max_cycles = 20, mixed = false, inner_tol = 0.0, adjoint = false)
+    N, nrhs = size(B)
+    X = copy(X0)
+    flag = zeros(Cint, nrhs); relres = zeros(Float64, nrhs); iters = zeros(Cint, nrhs); cycles = zeros(Cint, nrhs)
+    rc = GC.@preserve B X flag relres iters cycles begin
+        if adjoint
+            ccall((:peec_gmres_batched_adjoint, LIBPEEC), Cint,
+                (Ptr{Cvoid}, Cint, Cint, Ptr{ComplexF64}, Float64, Cint, Cint, Cint, Float64, Ptr{ComplexF64},
+                 Ptr{Cint}, Ptr{Float64}, Ptr{Cint}, Ptr{Cint}),
+                ctx, nrhs, rhs_per_slot, B, tol, restart, max_cycles, mixed, inner_tol, X, flag, relres, iters, cycles)
+        else
+            ccall((:peec_gmres_batched, LIBPEEC), Cint,
+                (Ptr{Cvoid}, Cint, Cint, Ptr{ComplexF64}, Float64, Cint, Cint, Cint, Float64, Ptr{ComplexF64},
+                 Ptr{Cint}, Ptr{Float64}, Ptr{Cint}, Ptr{Cint}),
+                ctx, nrhs, rhs_per_slot, B, tol, restart, max_cycles, mixed, inner_tol, X, flag, relres, iters, cycles)
+        end
+    end
+    peec_check(rc, "peec_gmres_batched")
+    X, flag, relres, iters, cycles
+end
+
+peec_fft_tune(ctx, nrhs; fp64 = true, fp32 = true) =
+    peec_check(ccall((:peec_fft_tune, LIBPEEC), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}), ctx, nrhs,
+                     Cint(fp64) | (Cint(fp32) << 1), C_NULL), "peec_fft_tune")
+
+# ---------------------------------------------------------------------------
+# ChainRulesCore rule for J(sigma) = sum_f |Z_pp(f)|^2.
+#
+#   solve_ports(sigma, f)  -> (S::Matrix{ComplexF64} P x P, V::Matrix{ComplexF64} N x P)
+#       the forward sweep at one frequency (batched or plain path above),
+#   adjoint_solve(sigma, f, g) -> lambda::Vector{ComplexF64} (N)
+#       K^T lambda = g through peec_gmres_batched(...; adjoint = true) on
+#       conj(g), conjugated back,
+#   dDZ(sigma, f) -> SparseMatrixCSC{ComplexF64} m x nvox,  d DZ_e / d sigma_v
+#       (half-voxel resistances R = l/2 / (sigma W T), compute_diagonals.jl:10-16,
+#       and the skin term where it wins the switch, FFT_solver_QS_S_type.jl:107-114).
+#
+# With A = (I - S)^-1: dZ_pp = -4 lambda_I' (dDZ .* (V_I * A[p, :])), where
+# K^T lambda = sum_j A[j, p] g_j and g_j selects the node-voltage difference
+# of port j (S_ij = 2 g_j' V_i / R0 - delta_ij, dV_i = -K^-1 dK V_i, dK =
+# diag(dDZ) on the edge block; the Python mirror is
+# solver_amd/sweep.py: impedance_objective_gradient).
+using ChainRulesCore
+using LinearAlgebra
+
+struct PortObjective{F1,F2,F3}
+    freqs::Vector{Float64}
+    port::Int
+    m::Int
+    port_vector::Function      # weights -> node excitation vector g (length N, zero off the node block)
+    solve_ports::F1
+    adjoint_solve::F2
+    dDZ::F3
+    R0::Float64
+end
+
+function (J::PortObjective)(sigma)
+    s = 0.0
+    for f in J.freqs
+        S, _ = J.solve_ports(sigma, f)
+        Z = J.R0 * ((I - S) \ (I + S))
+        s += abs2(Z[J.port, J.port])
+    end
+    s
+end
+
+function ChainRulesCore.rrule(J::PortObjective, sigma)
+    val = 0.0
+    grad = zeros(length(sigma))
+    for f in J.freqs
+        S, V = J.solve_ports(sigma, f)
+        P = size(S, 1)
+        A = inv(Matrix{ComplexF64}(I, P, P) - S)
+        Z = J.R0 * A * (I + S)
+        z = Z[J.port, J.port]
+        lam = J.adjoint_solve(sigma, f, J.port_vector(A[:, J.port]))
+        U = V[1:J.m, :] * A[J.port, :]
+        dZ = -4 .* (transpose(J.dDZ(sigma, f)) * (lam[1:J.m] .* U))
+        val += abs2(z)
+        grad .+= 2 .* real.(conj(z) .* dZ)
+    end
+    pullback(dJ) = (NoTangent(), dJ .* grad)
+    return val, pullback
 end

@@ -71,7 +71,13 @@ def lib():
                                          ctypes.c_int, ctypes.c_int, ctypes.c_double, c128p, ip, dp, ip, ip]
         L.peec_precond_rhs_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, c128p]
         L.peec_apply_K_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c128p, c128p]
+        L.peec_apply_KH_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c128p, c128p]
+        L.peec_precond_rhs_batched_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int, c128p, c128p]
+        L.peec_gmres_batched_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int, c128p, ctypes.c_double,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, c128p,
+                                                 ip, dp, ip, ip]
         L.peec_fft_tune.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.peec_set_graphs.argtypes = [vp, ctypes.c_int]
         L.peec_fft_set_choice.argtypes = [vp] + [ctypes.c_int] * 7
         L.peec_fft_choice.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
         L.peec_matvec.argtypes = [vp, c128p, c128p]

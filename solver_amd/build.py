@@ -2,7 +2,8 @@
 
 Each translation unit is compiled to an object under solver_amd/_build/ and
 linked into solver_amd/libpeec_hip.so.  integrals.hip is compiled with
--ffp-contract=off so the FP64 closed forms are evaluated exactly as written.
+-ffp-contract=off so the FP64 closed forms are evaluated exactly as written,
+fft.hip so that every launch shape of a pass rounds the same way.
 """
 import os
 import subprocess
@@ -18,7 +19,9 @@ SOURCES = ["api.cpp", "integrals.hip", "fft.hip", "operator.hip", "trsv.hip", "l
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-result", "-I", CSRC]
-PER_FILE = {"integrals.hip": ["-ffp-contract=off"]}
+# integrals.hip: the closed forms evaluate exactly as written; fft.hip:
+# every launch shape of the FFT passes rounds identically
+PER_FILE = {"integrals.hip": ["-ffp-contract=off"], "fft.hip": ["-ffp-contract=off"]}
 
 
 def build(verbose=False):

@@ -29,7 +29,9 @@
 typedef std::complex<double> zc;
 
 int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps);
+int matvec_device_adj(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps);
 int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, int rps);
+int apply_precond_adj(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, int rps);
 
 namespace {
 
@@ -201,15 +203,48 @@ struct BWork {
         hipLaunchKernelGGL(k_bgemvn, dim3(nb256(N), nrhs), dim3(256), 0, ctx->stream, w, wld, V.p,
                            (int64_t)(m + 1) * N, ncol, h, hs, N, sign);
     }
-    // W = A x (x with stride N), in FP64 or with FP32 convolutions
-    int matvec(const cd *x, bool fp32) { return matvec_device(ctx, x, W.p, nrhs, N, fp32, rps); }
+    // W = A x (x with stride N), in FP64 or with FP32 convolutions; A =
+    // M^-1 K, or M^-H K^H for the adjoint solve; replayed from HIP graphs
+    bool adjoint = false;
+    MatvecGraphs graphs;
+    int matvec(const cd *x, bool fp32)
+    {
+        if (!graphs.ctx) {
+            graphs.ctx = ctx;
+            graphs.nrhs = nrhs;
+            graphs.rps = rps;
+            graphs.ld = N;
+            graphs.adjoint = adjoint;
+        }
+        return graphs.run(x, W.p, fp32);
+    }
 };
 
 }  // namespace
 
+static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per_slot, const peec_c128 *b,
+                              double tol, int restart, int max_cycles, int mixed, double inner_tol, peec_c128 *x,
+                              int *flag, double *relres, int *iters, int *cycles);
+
 extern "C" int peec_gmres_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const peec_c128 *b, double tol,
                                   int restart, int max_cycles, int mixed, double inner_tol, peec_c128 *x, int *flag,
                                   double *relres, int *iters, int *cycles)
+{
+    return gmres_batched_impl(ctx, false, nrhs, rhs_per_slot, b, tol, restart, max_cycles, mixed, inner_tol, x, flag,
+                              relres, iters, cycles);
+}
+
+extern "C" int peec_gmres_batched_adjoint(peec_ctx *ctx, int nrhs, int rhs_per_slot, const peec_c128 *b, double tol,
+                                          int restart, int max_cycles, int mixed, double inner_tol, peec_c128 *x,
+                                          int *flag, double *relres, int *iters, int *cycles)
+{
+    return gmres_batched_impl(ctx, true, nrhs, rhs_per_slot, b, tol, restart, max_cycles, mixed, inner_tol, x, flag,
+                              relres, iters, cycles);
+}
+
+static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per_slot, const peec_c128 *b,
+                              double tol, int restart, int max_cycles, int mixed, double inner_tol, peec_c128 *x,
+                              int *flag, double *relres, int *iters, int *cycles)
 {
     if (!ctx || nrhs < 1 || rhs_per_slot < 1 || !b || !x || restart < 1 || max_cycles < 1) {
         peec_set_error("peec_gmres_batched: invalid argument");
@@ -227,6 +262,7 @@ extern "C" int peec_gmres_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, con
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
     BWork W;
     W.ctx = ctx;
+    W.adjoint = adjoint;
     W.nrhs = nrhs;
     W.rps = rhs_per_slot;
     W.m = restart;
@@ -415,6 +451,39 @@ extern "C" int peec_precond_rhs_batched(peec_ctx *ctx, int nrhs, int rhs_per_slo
     if (!Y.alloc(h.size())) return PEEC_ERR_NOMEM;
     PEEC_HIP_TRY(hipMemcpyAsync(Y.p, h.data(), h.size() * 16, hipMemcpyHostToDevice, ctx->stream));
     int rc = apply_precond(ctx, Y.p, nrhs, N, rhs_per_slot);
+    if (rc) return rc;
+    PEEC_HIP_TRY(hipMemcpyAsync(tn, Y.p, h.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PEEC_OK;
+}
+
+// tn_r = M^-H [0; 0; g_r] for complex node excitations g (n x nrhs,
+// column-major): the right-hand side of the preconditioned adjoint system
+// M^-H K^H lambda = M^-H [0; 0; g] (peec_gmres_batched_adjoint).
+extern "C" int peec_precond_rhs_batched_adjoint(peec_ctx *ctx, int nrhs, int rhs_per_slot, const peec_c128 *g,
+                                                peec_c128 *tn)
+{
+    if (!ctx || nrhs < 1 || rhs_per_slot < 1 || !g || !tn) {
+        peec_set_error("peec_precond_rhs_batched_adjoint: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->op_ready || !ctx->freq_ready || !ctx->lu_ready) {
+        peec_set_error("peec_precond_rhs_batched_adjoint: operator, frequency slots and factors must be set first");
+        return PEEC_ERR_STATE;
+    }
+    if ((nrhs + rhs_per_slot - 1) / rhs_per_slot > ctx->nslots) {
+        peec_set_error("peec_precond_rhs_batched_adjoint: not enough frequency slots");
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n, N = m + ns + n;
+    std::vector<cd> h((size_t)(N * nrhs), cd{0.0, 0.0});
+    for (int r = 0; r < nrhs; r++)
+        for (int64_t v = 0; v < n; v++) h[(size_t)(r * N + m + ns + v)] = {g[r * n + v].re, g[r * n + v].im};
+    DevBuf<cd> Y;
+    if (!Y.alloc(h.size())) return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemcpyAsync(Y.p, h.data(), h.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+    int rc = apply_precond_adj(ctx, Y.p, nrhs, N, rhs_per_slot);
     if (rc) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(tn, Y.p, h.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -51,6 +51,15 @@ struct Kernel {
     DevBuf<float> spec32;
     int64_t compact() const { return (int64_t)n[0] * n[1] * n[2]; }
     int64_t embedded() const { return (int64_t)L[0] * L[1] * L[2]; }
+    // Half-cell reduction of an off-diagonal P kernel (built by
+    // peec_operator_set when every scattered and gathered position keeps
+    // one parity per half-cell axis): coarse box pn, embedding pL, parity
+    // offset delta = out - in per axis, complex spectrum (FP64 pairs and
+    // interleaved FP32) with 1/(pL0 pL1 pL2) folded in.
+    bool poly = false;
+    int pn[3] = {0, 0, 0}, pL[3] = {0, 0, 0}, delta[3] = {0, 0, 0};
+    DevBuf<double2_t> pspec64;
+    DevBuf<float> pspec32;
 };
 
 // One FFT-convolution application of the matvec: scatter unknowns of one
@@ -61,6 +70,14 @@ struct ConvApp {
     int64_t in_off = 0, out_off = 0;  // section offsets inside x / y
     DevBuf<int32_t> map_in, map_out;
     int64_t nnz_in = 0, nnz_out = 0;
+    // geometry of this application: the kernel's compact grid and
+    // embedding, or the coarse box of its half-cell reduction (poly), whose
+    // complex spectrum is applied as is or conjugated (conj: the reverse
+    // direction, parity offset -delta)
+    int n[3] = {0, 0, 0}, L[3] = {0, 0, 0};
+    bool poly = false, conj = false;
+    int64_t compact() const { return (int64_t)n[0] * n[1] * n[2]; }
+    int64_t embedded() const { return (int64_t)L[0] * L[1] * L[2]; }
 };
 
 // One front of the multifrontal factorization (lu.hip): an f x f
@@ -248,6 +265,9 @@ struct peec_ctx {
     DevBuf<double2_t> vx, vy, vt1, vt2, vt3;  // N-sized work vectors
     DevBuf<double2_t> trsv_part;              // segment partial sums of the triangular solves
 
+    // ---- HIP graphs: GMRES replays its captured matvec (peec_set_graphs)
+    bool use_graphs = true;
+
     // ---- profiling
     bool prof_on = false;
     std::vector<ProfRec> prof;
@@ -273,3 +293,24 @@ int tri_set_slots(TriFactor *f, int nslots);  // reallocates the value sets (con
 std::vector<int32_t> tri_chain_blocks(const peec_csc_c128 *L, int cap);
 int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f, const char *what);
 int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs, int rps);
+
+// The matvec of one GMRES call (M^-1 K x, or M^-H K^H x) captured once per
+// (x, y, precision) into a HIP graph and replayed: nothing the matvec reads
+// or writes moves while a GMRES call runs, so a cache scoped to the call
+// never goes stale.  Falls back to plain launches while profiling (events
+// are not recorded into graphs) or when graphs are switched off.
+struct MatvecGraphs {
+    struct Entry {
+        const void *x;
+        void *y;
+        int fp32;
+        hipGraphExec_t exec;
+    };
+    peec_ctx *ctx = nullptr;
+    int nrhs = 1, rps = 1;
+    int64_t ld = 0;
+    bool adjoint = false;
+    std::vector<Entry> entries;
+    ~MatvecGraphs();
+    int run(const double2_t *x, double2_t *y, bool fp32);
+};

@@ -94,6 +94,9 @@ void peec_ctx::release_kernels()
         k.spec64.reset();
         k.spec32.reset();
         k.n[0] = k.n[1] = k.n[2] = 0;
+        k.poly = false;
+        k.pspec64.reset();
+        k.pspec32.reset();
     }
     fft_tune.clear();
     kernels_ready = false;
@@ -268,8 +271,9 @@ struct PassDesc {
     const int32_t *map_out;              // optional 0/1 gather
     double2_t *vec_out;
     int64_t vec_out_ld;
-    const void *spec;                    // optional real spectrum (T) for the fused z pass
+    const void *spec;                    // spectrum for the fused z pass: real T (MODE 2), C2<T> (MODE 3)
     int64_t spec_sb, spec_s2, spec_sj;
+    int spec_conj;                       // MODE 3: multiply by the conjugate spectrum
 };
 
 template <typename T, int DIR, int TPL, int LPT, int LMAX>
@@ -299,10 +303,11 @@ __device__ __forceinline__ int qdiv(int e, float inv) { return (int)(((float)e +
 template <int MODE, int W, int LMAX>
 constexpr int fft_prefetch_slots()
 {
-    return MODE == 1 ? (W * LMAX + 255) / 256 : (W * (LMAX / 2 + 1) + 255) / 256;
+    return MODE == 1 ? (W * LMAX + 255) / 256 : (W * (LMAX / 2 + 1) + 255) / 256;  // MODE 0, 2, 3
 }
 
-// MODE: 0 forward, 1 inverse, 2 forward * spectrum * inverse.
+// MODE: 0 forward, 1 inverse, 2 forward * real spectrum * inverse, 3
+// forward * complex spectrum (or its conjugate) * inverse.
 // A block transforms a tile of W consecutive lines (positions b0 .. b0+W-1
 // along the tiled axis) at line index l2, for right-hand sides
 // blockIdx.z * rpb .. + rpb - 1.  Lines contiguous in memory (sj == 1) are
@@ -322,7 +327,8 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, 
 {
     constexpr int G = 256 / TPL;
     constexpr int W = G * LPT;
-    constexpr int NSPEC = MODE == 2 ? (W * LMAX + 255) / 256 : 1;
+    constexpr bool CONV = MODE >= 2;
+    constexpr int NSPEC = CONV ? (W * LMAX + 255) / 256 : 1;
     constexpr int NPF = fft_prefetch_slots<MODE, W, LMAX>();
     extern __shared__ char smem_raw[];
     const int L = d.L, LS = L + 1;
@@ -343,15 +349,23 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, 
         twl[t] = {(T)w.x, (T)w.y};
     }
 
-    T sv[NSPEC];
-    if (MODE == 2) {
-        const T *sp = reinterpret_cast<const T *>(d.spec);
+    // spectrum tile: real (MODE 2) or complex (MODE 3, imaginary parts in svi)
+    T sv[NSPEC], svi[MODE == 3 ? NSPEC : 1];
+    if (CONV) {
         const int64_t sbase = (int64_t)b0 * d.spec_sb + (int64_t)l2 * d.spec_s2;
 #pragma unroll
         for (int i = 0; i < NSPEC; i++) {
             const int e = tid + 256 * i;
             const int b = e % W, j = e / W;
-            sv[i] = (b < nbt && j < L) ? sp[sbase + (int64_t)b * d.spec_sb + (int64_t)j * d.spec_sj] : (T)0;
+            const bool in = b < nbt && j < L;
+            const int64_t q = sbase + (int64_t)b * d.spec_sb + (int64_t)j * d.spec_sj;
+            if (MODE == 2) {
+                sv[i] = in ? reinterpret_cast<const T *>(d.spec)[q] : (T)0;
+            } else {
+                const C2<T> c = in ? reinterpret_cast<const C2<T> *>(d.spec)[q] : C2<T>{(T)0, (T)0};
+                sv[i] = c.x;
+                svi[MODE == 3 ? i : 0] = d.spec_conj ? -c.y : c.y;
+            }
         }
     }
 
@@ -445,12 +459,17 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, 
             run_fft<T, 1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, twl);
         } else {
             run_fft<T, -1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, twl);
-            if (MODE == 2) {
+            if (CONV) {
 #pragma unroll
                 for (int i = 0; i < NSPEC; i++) {
                     const int e = tid + 256 * i;
                     const int b = e % W, j = e / W;
-                    if (j < L) tile[b * LS + j] = cscale(tile[b * LS + j], sv[i]);
+                    if (j < L) {
+                        if (MODE == 2)
+                            tile[b * LS + j] = cscale(tile[b * LS + j], sv[i]);
+                        else
+                            tile[b * LS + j] = cmul(tile[b * LS + j], C2<T>{sv[i], svi[MODE == 3 ? i : 0]});
+                    }
                 }
                 __syncthreads();
                 run_fft<T, 1, TPL, LPT, LMAX>(tile, LS, L, d, lg, lane, twl);
@@ -507,6 +526,55 @@ __global__ void k_embed(const double *__restrict__ row, int nx, int ny, int nz, 
     double v = 0.0;
     if (sx >= 0 && sy >= 0 && sz >= 0) v = row[sx + (int64_t)nx * (sy + (int64_t)ny * sz)];
     out[i] = {v, 0.0};
+}
+
+// Half-cell reduction of an off-diagonal P kernel.  Along a half-cell axis
+// the fine row holds the kernel at every half-cell offset f = 0 .. nf-1; a
+// block that scatters from positions of one parity and gathers at another
+// only ever uses the offsets f = |2 m + delta| (m the coarse offset, delta
+// = out parity - in parity), so the convolution runs on the coarse box
+// nb = (nf + 1) / 2 with the kernel Kc[m] = row(|2 m + delta|).  Kc is not
+// even, so it is embedded as a general circulant (offset m at m mod L,
+// m = -(nb-1) .. nb-1) and its spectrum is complex.  Axes without half
+// cells keep the even embedding (delta = 0, f = |m|).
+__device__ __forceinline__ int poly_offset(int i, int L, int nb, int half, int delta, int nf)
+{
+    int m;
+    if (i <= nb - 1) m = i;
+    else if (L - i <= nb - 1) m = i - L;
+    else return -1;
+    const int f = half ? abs(2 * m + delta) : abs(m);
+    return f < nf ? f : -1;
+}
+
+__global__ void k_embed_poly(const double *__restrict__ row, int nfx, int nfy, int nfz, int hx, int hy, int hz,
+                             int dx, int dy, int dz, int nbx, int nby, int nbz, int Lx, int Ly, int Lz,
+                             double2_t *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t G = (int64_t)Lx * Ly * Lz;
+    if (i >= G) return;
+    const int x = (int)(i % Lx);
+    const int64_t r = i / Lx;
+    const int y = (int)(r % Ly);
+    const int z = (int)(r / Ly);
+    const int fx = poly_offset(x, Lx, nbx, hx, dx, nfx);
+    const int fy = poly_offset(y, Ly, nby, hy, dy, nfy);
+    const int fz = poly_offset(z, Lz, nbz, hz, dz, nfz);
+    double v = 0.0;
+    if (fx >= 0 && fy >= 0 && fz >= 0) v = row[fx + (int64_t)nfx * (fy + (int64_t)nfy * fz)];
+    out[i] = {v, 0.0};
+}
+
+__global__ void k_spec_extract_c(const double2_t *__restrict__ g, int64_t G, double scale,
+                                 double2_t *__restrict__ s64, float *__restrict__ s32)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G) return;
+    const double2_t v = {g[i].x * scale, g[i].y * scale};
+    s64[i] = v;
+    s32[2 * i] = (float)v.x;
+    s32[2 * i + 1] = (float)v.y;
 }
 
 __global__ void k_spec_extract(const double2_t *__restrict__ g, int64_t G, double scale, double *__restrict__ s64,
@@ -684,11 +752,13 @@ static int launch_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const vo
     if (fp32) {
         if (mode == 0) return launch_pass_t<float, 0>(ctx, d, src, dst, v);
         if (mode == 1) return launch_pass_t<float, 1>(ctx, d, src, dst, v);
-        return launch_pass_t<float, 2>(ctx, d, src, dst, v);
+        if (mode == 2) return launch_pass_t<float, 2>(ctx, d, src, dst, v);
+        return launch_pass_t<float, 3>(ctx, d, src, dst, v);
     }
     if (mode == 0) return launch_pass_t<double, 0>(ctx, d, src, dst, v);
     if (mode == 1) return launch_pass_t<double, 1>(ctx, d, src, dst, v);
-    return launch_pass_t<double, 2>(ctx, d, src, dst, v);
+    if (mode == 2) return launch_pass_t<double, 2>(ctx, d, src, dst, v);
+    return launch_pass_t<double, 3>(ctx, d, src, dst, v);
 }
 
 // compact dims of each circulant kernel
@@ -723,6 +793,47 @@ static int fft3_forward_inplace(peec_ctx *ctx, double2_t *g, const int L[3])
     PassGeom gz = {L[2], L[2], L[2], L[0], L[1], 1, Lx, Lx * Ly, 1, Lx, Lx * Ly};
     if ((rc = make_desc(ctx, gz, &d))) return rc;
     if (L[2] > 1 && (rc = launch_pass(ctx, false, 0, d, g, g))) return rc;
+    return PEEC_OK;
+}
+
+// Axes along which a kernel's circulant grid has half-cell spacing: the
+// merged face grids of the off-diagonal P blocks (p1234, p1256, p3456,
+// build_center_P_Voxels.jl:53-111).
+extern const int kHalfAxes[9][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0},
+                             {0, 0, 0}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}};
+
+// The complex spectrum of kernel k's half-cell reduction for parity offset
+// delta (out - in, per half-cell axis).
+int poly_spectrum_build(peec_ctx *ctx, int k, const int delta[3])
+{
+    Kernel &K = ctx->kern[k];
+    for (int a = 0; a < 3; a++) {
+        K.pn[a] = kHalfAxes[k][a] ? (K.n[a] + 1) / 2 : K.n[a];
+        K.pL[a] = fft_smooth_length(2 * K.pn[a] - 1);
+        if (ctx->twiddle_offset(K.pL[a]) < 0) {
+            peec_set_error("operator_set: twiddle table allocation failed");
+            return PEEC_ERR_NOMEM;
+        }
+    }
+    const int64_t G = (int64_t)K.pL[0] * K.pL[1] * K.pL[2];
+    DevBuf<double2_t> grid;
+    if (!grid.alloc((size_t)G) || !K.pspec64.alloc((size_t)G) || !K.pspec32.alloc((size_t)(2 * G))) {
+        peec_set_error("operator_set: out of device memory for a reduced P spectrum");
+        return PEEC_ERR_NOMEM;
+    }
+    const unsigned nbk = (unsigned)((G + 255) / 256);
+    hipLaunchKernelGGL(k_embed_poly, dim3(nbk), dim3(256), 0, ctx->stream, K.row.p, K.n[0], K.n[1], K.n[2],
+                       kHalfAxes[k][0], kHalfAxes[k][1], kHalfAxes[k][2], delta[0], delta[1], delta[2], K.pn[0],
+                       K.pn[1], K.pn[2], K.pL[0], K.pL[1], K.pL[2], grid.p);
+    PEEC_HIP_TRY(hipGetLastError());
+    int rc = fft3_forward_inplace(ctx, grid.p, K.pL);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_spec_extract_c, dim3(nbk), dim3(256), 0, ctx->stream, grid.p, G, 1.0 / (double)G,
+                       K.pspec64.p, K.pspec32.p);
+    PEEC_HIP_TRY(hipGetLastError());
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int a = 0; a < 3; a++) K.delta[a] = delta[a];
+    K.poly = true;
     return PEEC_OK;
 }
 
@@ -778,7 +889,7 @@ int peec_kernels_build_impl(peec_ctx *ctx, const double origin[3], int Nx, int N
     return PEEC_OK;
 }
 
-int64_t conv_work_elems(const peec_ctx *ctx, int kernel);
+int64_t conv_work_elems(const ConvApp &app);
 
 // The five pass descriptors of one convolution application for nrhs
 // right-hand sides (vectors at stride ld), with their algorithmic bytes.
@@ -793,10 +904,10 @@ static int conv_passes(peec_ctx *ctx, const ConvApp &app, bool fp32, const doubl
                        int nrhs, int64_t ld, void *w1, void *w2, const int style[5], ConvPasses *cp)
 {
     const Kernel &K = ctx->kern[app.kernel];
-    const int nx = K.n[0], ny = K.n[1], nz = K.n[2];
-    const int64_t Lx = K.L[0], Ly = K.L[1], Lz = K.L[2];
+    const int nx = app.n[0], ny = app.n[1], nz = app.n[2];
+    const int64_t Lx = app.L[0], Ly = app.L[1], Lz = app.L[2];
     const int64_t s1 = Lx * ny * nz, s2 = Lx * Ly * nz;  // per-rhs grid sizes
-    const double e = fp32 ? 8.0 : 16.0, es = fp32 ? 4.0 : 8.0;
+    const double e = fp32 ? 8.0 : 16.0, es = (fp32 ? 4.0 : 8.0) * (app.poly ? 2.0 : 1.0);
     const double g = (double)nx * ny * nz, G1 = (double)s1 * nrhs, G2 = (double)s2 * nrhs;
     int rc;
     // 1: x forward with scatter; lines (y, z)
@@ -832,13 +943,17 @@ static int conv_passes(peec_ctx *ctx, const ConvApp &app, bool fp32, const doubl
     d3.nrhs = nrhs;
     d3.in_rhs = s2;
     d3.out_rhs = s2;
-    d3.spec = fp32 ? (const void *)K.spec32.p : (const void *)K.spec64.p;
+    if (app.poly)
+        d3.spec = fp32 ? (const void *)K.pspec32.p : (const void *)K.pspec64.p;
+    else
+        d3.spec = fp32 ? (const void *)K.spec32.p : (const void *)K.spec64.p;
+    d3.spec_conj = app.conj ? 1 : 0;
     d3.spec_sb = 1;
     d3.spec_s2 = Lx;
     d3.spec_sj = Lx * Ly;
     cp->src[2] = w2;
     cp->dst[2] = w2;
-    cp->bytes[2] = 2.0 * e * G2 + es * (double)K.embedded();
+    cp->bytes[2] = 2.0 * e * G2 + es * (double)app.embedded();
     // 4: y inverse, keep ny rows; lines (x', z)
     PassGeom g4 = {(int)Ly, (int)Ly, ny, (int)Lx, nz, 1, Lx * Ly, Lx, 1, Lx * ny, Lx};
     PassDesc &d4 = cp->d[3];
@@ -882,6 +997,7 @@ static FftChoice tuned_choice(const peec_ctx *ctx, int kernel, int pass, bool fp
 
 static const int kProfKind[5] = {PK_FFT_X_FWD, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV};
 static const int kPassMode[5] = {0, 0, 2, 1, 1};
+static int pass_mode(const ConvApp &app, int p) { return p == 2 && app.poly ? 3 : kPassMode[p]; }
 
 // One convolution application: vec_out[section] += gather(conv(scatter(vec_in)))
 // for nrhs right-hand sides (vectors at stride ld).  w1/w2 are grid work
@@ -890,8 +1006,7 @@ static const int kPassMode[5] = {0, 0, 2, 1, 1};
 int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *vec_in, double2_t *vec_out,
                int nrhs, int64_t ld, void *w1, void *w2)
 {
-    const Kernel &K = ctx->kern[app.kernel];
-    if (K.compact() <= 0 || app.nnz_in == 0 || app.nnz_out == 0) return PEEC_OK;
+    if (app.compact() <= 0 || app.nnz_in == 0 || app.nnz_out == 0) return PEEC_OK;
     FftChoice ch[5];
     int style[5];
     for (int p = 0; p < 5; p++) {
@@ -906,7 +1021,7 @@ int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *ve
         const int v = ch[p].variant >= 0 ? ch[p].variant : default_variant(d.L, fp32);
         d.rpb = ch[p].variant >= 0 ? ch[p].rpb : default_rpb(d, std::max(v, 0));
         const int pi = ctx->prof_begin(kProfKind[p], cp.bytes[p]);
-        if ((rc = launch_pass(ctx, fp32, kPassMode[p], d, cp.src[p], cp.dst[p], v))) return rc;
+        if ((rc = launch_pass(ctx, fp32, pass_mode(app, p), d, cp.src[p], cp.dst[p], v))) return rc;
         ctx->prof_end(pi);
     }
     return PEEC_OK;
@@ -943,7 +1058,7 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
     const int64_t N = ctx->m + ctx->ns + ctx->n;
     size_t need = 0;
-    for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx, ctx->apps[a].kernel));
+    for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx->apps[a]));
     need *= (size_t)nrhs;
     if (need > ctx->w1.n && (!ctx->w1.alloc(need) || !ctx->w2.alloc(need))) {
         peec_set_error("peec_fft_tune: out of device memory for %d-rhs FFT work grids", nrhs);
@@ -966,7 +1081,7 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
     bool seen[9] = {};
     for (int a = 0; a < ctx->napps && rc == PEEC_OK; a++) {
         const ConvApp &app = ctx->apps[a];
-        if (seen[app.kernel] || ctx->kern[app.kernel].compact() <= 0 || app.nnz_in == 0 || app.nnz_out == 0) continue;
+        if (seen[app.kernel] || app.compact() <= 0 || app.nnz_in == 0 || app.nnz_out == 0) continue;
         seen[app.kernel] = true;
         for (int prec = 0; prec < 2 && rc == PEEC_OK; prec++) {
             if (!(precisions & (1 << prec))) continue;
@@ -975,7 +1090,7 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
             ConvPasses cp;
             if ((rc = conv_passes(ctx, app, fp32, vin.p, vout.p, nrhs, N, ctx->w1.p, ctx->w2.p, style0, &cp))) break;
             for (int p = 0; p < 5 && rc == PEEC_OK; p++) {
-                const int mode = kPassMode[p];
+                const int mode = pass_mode(app, p);
                 PassDesc d = cp.d[p];
                 FftChoice best;
                 float tbest = 1e30f;
@@ -1064,9 +1179,11 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
         ctx->fft_tune.erase(key);
         return PEEC_OK;
     }
-    const Kernel &K = ctx->kern[kernel];
-    const int L = K.L[pass == 0 || pass == 4 ? 0 : (pass == 2 ? 2 : 1)];
-    if (!ctx->kernels_ready || !variant_fits(variant, L, fp32 != 0)) {
+    const ConvApp *app = nullptr;
+    for (int a = 0; a < ctx->napps && !app; a++)
+        if (ctx->apps[a].kernel == kernel) app = &ctx->apps[a];
+    const int L = app ? app->L[pass == 0 || pass == 4 ? 0 : (pass == 2 ? 2 : 1)] : 0;
+    if (!app || !variant_fits(variant, L, fp32 != 0)) {
         peec_set_error("peec_fft_set_choice: launch variant %d does not fit length %d", variant, L);
         return PEEC_ERR_ARG;
     }
@@ -1091,11 +1208,7 @@ extern "C" int peec_fft_choice(peec_ctx *ctx, int kernel, int pass, int fp32, in
     return PEEC_OK;
 }
 
-int64_t conv_work_elems(const peec_ctx *ctx, int kernel)
-{
-    const Kernel &K = ctx->kern[kernel];
-    return (int64_t)K.L[0] * K.L[1] * K.n[2];
-}
+int64_t conv_work_elems(const ConvApp &app) { return (int64_t)app.L[0] * app.L[1] * app.n[2]; }
 
 extern "C" int peec_kernels_build(peec_ctx *ctx, const double origin[3], int Nx, int Ny, int Nz, double sx, double sy,
                                   double sz, double esc_Lp, double esc_P)

@@ -250,7 +250,12 @@ extern "C" int peec_gmres(peec_ctx *ctx, const peec_c128 *b_h, double tol, int m
     GmresWork W{ctx, n, inner};
     int rc = W.alloc();
     if (rc) return rc;
-    auto matvec = [&](const cd *xin, cd *yout) { return matvec_device(ctx, xin, yout, 1, n, false, 1); };
+    MatvecGraphs graphs;
+    graphs.ctx = ctx;
+    graphs.nrhs = 1;
+    graphs.rps = 1;
+    graphs.ld = n;
+    auto matvec = [&](const cd *xin, cd *yout) { return graphs.run(xin, yout, false); };
 
     PEEC_HIP_TRY(hipMemcpyAsync(W.b.p, b_h, sizeof(cd) * n, hipMemcpyHostToDevice, ctx->stream));
     PEEC_HIP_TRY(hipMemcpyAsync(W.x.p, x_h, sizeof(cd) * n, hipMemcpyHostToDevice, ctx->stream));

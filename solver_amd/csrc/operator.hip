@@ -42,11 +42,18 @@ void peec_ctx::release_operator()
     lu_layout_from_plan = false;
     lu.ready = false;
     fft_tune.clear();
+    for (auto &k : kern) {
+        k.poly = false;
+        k.pspec64.reset();
+        k.pspec32.reset();
+    }
 }
 
 int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *vec_in, double2_t *vec_out, int nrhs,
                int64_t ld, void *w1, void *w2);
-int64_t conv_work_elems(const peec_ctx *ctx, int kernel);
+int64_t conv_work_elems(const ConvApp &app);
+int poly_spectrum_build(peec_ctx *ctx, int k, const int delta[3]);
+extern const int kHalfAxes[9][3];
 
 namespace {
 
@@ -183,6 +190,94 @@ __global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ0,
 
 }  // namespace
 
+// ------------------------------------------------------------------ adjoint
+// The conjugate transpose of K (the operator above):
+//   Y1 = -j w (Lp I) + conj(DZ) .* I - A Phi
+//   Y2 = P Q - j w Gamma' Phi
+//   Y3 = A' I - Gamma Q + conj(Yle) Phi
+// Lp and P are real and symmetric (each off-diagonal P block is applied in
+// both directions), so the 12 convolutions are those of K; only the sparse
+// terms and the frequency factors change.  Yle is symmetric.
+
+// Y1 = -j w Y1 + conj(DZ) .* I - A Phi
+__global__ void k_y1h(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const int32_t *ai, const double *av,
+                      const double *wv, int64_t m, int64_t phi0, int nrhs, int64_t ld, int rps)
+{
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    for (int r = 0; r < nrhs; r++) {
+        const int sl = r / rps;
+        const cd *xr = x + r * ld;
+        cd s = {0.0, 0.0};
+        for (int k = ap[e]; k < ap[e + 1]; k++) s = cadd(s, cscal(xr[phi0 + ai[k]], av[k]));
+        const cd dz = DZ[e + sl * m];
+        cd v = jw(y[e + r * ld], -wv[sl]);
+        v = cadd(v, cmul(cd{dz.x, -dz.y}, xr[e]));
+        y[e + r * ld] = csub(v, s);
+    }
+}
+
+// Y2 = Y2 - j w Gamma' Phi
+__global__ void k_y2h(cd *y, const cd *x, const int32_t *gp, const int32_t *gi, const double *gv, const double *wv,
+                      int64_t m, int64_t ns, int64_t phi0, int nrhs, int64_t ld, int rps)
+{
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    for (int r = 0; r < nrhs; r++) {
+        const cd *xr = x + r * ld;
+        cd a = {0.0, 0.0};
+        for (int k = gp[s]; k < gp[s + 1]; k++) a = cadd(a, cscal(xr[phi0 + gi[k]], gv[k]));
+        y[m + s + r * ld] = csub(y[m + s + r * ld], jw(a, wv[r / rps]));
+    }
+}
+
+// Y3 = A' I - Gamma Q + conj(Yle) Phi
+__global__ void k_y3h(cd *y, const cd *x, const int32_t *atp, const int32_t *ati, const double *atv,
+                      const int32_t *yp0, const int32_t *yi, const cd *yv, const int32_t *gp, const int32_t *gi,
+                      const double *gv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, int rps)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int64_t phi0 = m + ns;
+    for (int r = 0; r < nrhs; r++) {
+        const int sl = r / rps;
+        const int32_t *yp = yp0 + sl * (n + 1);
+        const cd *xr = x + r * ld;
+        cd a = {0.0, 0.0};
+        for (int k = atp[v]; k < atp[v + 1]; k++) a = cadd(a, cscal(xr[ati[k]], atv[k]));
+        cd b = {0.0, 0.0};
+        for (int k = yp[v]; k < yp[v + 1]; k++) b = cadd(b, cmul(cd{yv[k].x, -yv[k].y}, xr[phi0 + yi[k]]));
+        cd c = {0.0, 0.0};
+        for (int k = gp[v]; k < gp[v + 1]; k++) c = cadd(c, cscal(xr[m + gi[k]], gv[k]));
+        y[phi0 + v + r * ld] = cadd(csub(a, c), b);
+    }
+}
+
+// The diagonal similarity behind reciprocity: K^T = D1 K D2 and M^T = D1 M
+// D2 with D1 = diag(I, j w, -I), D2 = diag(I, 1/(j w), -I) on the (I, Q, Phi)
+// blocks, so M^-H y = conj(D2^-1 M^-1 D1^-1 conj(y)).  stage 0 applies
+// y <- D1^-1 conj(y), stage 1 y <- conj(D2^-1 y).
+__global__ void k_adj_xform(cd *y, const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, int rps,
+                            int stage)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m + ns + n) return;
+    for (int r = 0; r < nrhs; r++) {
+        const double w = wv[r / rps];
+        cd a = y[i + r * ld];
+        if (stage == 0) {
+            a.y = -a.y;
+            if (i >= m + ns) a = {-a.x, -a.y};
+            else if (i >= m) a = {a.y / w, -a.x / w};  // a / (j w)
+        } else {
+            if (i >= m + ns) a = {-a.x, -a.y};
+            else if (i >= m) a = jw(a, w);
+            a.y = -a.y;
+        }
+        y[i + r * ld] = a;
+    }
+}
+
 // ------------------------------------------------------------------ host helpers
 static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
@@ -229,17 +324,43 @@ static int csc_to_csr(const peec_csc_f64 *a, bool transpose, CsrR *out, const ch
     return PEEC_OK;
 }
 
+// Parity of the grid positions a 0/1 expansion matrix selects along each
+// half-cell axis of a kernel grid with dims nf: 0 or 1, -1 when mixed, -2
+// when it selects nothing.
+static void map_parity(const peec_csc_f64 *a, const int nf[3], const int half[3], int par[3])
+{
+    for (int d = 0; d < 3; d++) par[d] = -2;
+    const int64_t nnz = a->colptr[a->ncol] - 1;
+    for (int64_t k = 0; k < nnz; k++) {
+        const int64_t r = a->rowval[k] - 1;
+        const int64_t f[3] = {r % nf[0], (r / nf[0]) % nf[1], r / ((int64_t)nf[0] * nf[1])};
+        for (int d = 0; d < 3; d++) {
+            if (!half[d]) continue;
+            const int p = (int)(f[d] & 1);
+            if (par[d] == -2) par[d] = p;
+            else if (par[d] != p) par[d] = -1;
+        }
+    }
+}
+
 // 0/1 expansion matrix (rows = compact kernel grid, one entry per column) ->
 // position map; columns are local unknown indices [col0, col0 + ncols).
+// With coarse (dims nf of the fine grid, half-cell axes, coarse box nb)
+// every fine position is mapped to its coarse box position (f / 2 along the
+// half-cell axes).
+struct CoarseMap {
+    int nf[3], half[3], nb[3];
+};
 static int map_from_csc(const peec_csc_f64 *a, int64_t grid, int64_t col0, int64_t ncols, DevBuf<int32_t> *out,
-                        int64_t *nnz, const char *what)
+                        int64_t *nnz, const char *what, const CoarseMap *coarse = nullptr)
 {
     if (a->nrow != grid) {
         peec_set_error("operator_set: %s has %lld rows, kernel grid has %lld", what, (long long)a->nrow,
                        (long long)grid);
         return PEEC_ERR_ARG;
     }
-    std::vector<int32_t> map((size_t)std::max<int64_t>(grid, 1), -1);
+    const int64_t cells = coarse ? (int64_t)coarse->nb[0] * coarse->nb[1] * coarse->nb[2] : grid;
+    std::vector<int32_t> map((size_t)std::max<int64_t>(cells, 1), -1);
     int64_t cnt = 0;
     for (int64_t c = 0; c < a->ncol; c++) {
         int64_t k0 = a->colptr[c] - 1, k1 = a->colptr[c + 1] - 1;
@@ -250,6 +371,13 @@ static int map_from_csc(const peec_csc_f64 *a, int64_t grid, int64_t col0, int64
         }
         for (int64_t k = k0; k < k1; k++) {
             int64_t r = a->rowval[k] - 1;
+            if (coarse) {
+                const CoarseMap &C = *coarse;
+                int64_t f[3] = {r % C.nf[0], (r / C.nf[0]) % C.nf[1], r / ((int64_t)C.nf[0] * C.nf[1])};
+                for (int d = 0; d < 3; d++)
+                    if (C.half[d]) f[d] >>= 1;
+                r = f[0] + (int64_t)C.nb[0] * (f[1] + (int64_t)C.nb[1] * f[2]);
+            }
             if (a->nzval[k] != 1.0) {
                 peec_set_error("operator_set: %s is not a 0/1 selection", what);
                 return PEEC_ERR_ARG;
@@ -361,7 +489,12 @@ extern "C" int peec_operator_set(peec_ctx *ctx, int64_t mx, int64_t my, int64_t 
         }
         if ((rc = map_from_csc(&lp_map[d], ctx->kern[d].compact(), 0, msz[d], &a.map_in, &a.nnz_in, nm[d]))) return rc;
         if ((rc = map_from_csc(&lp_map[d], ctx->kern[d].compact(), 0, msz[d], &a.map_out, &a.nnz_out, nm[d]))) return rc;
-        wmax = std::max(wmax, conv_work_elems(ctx, d));
+        for (int q = 0; q < 3; q++) {
+            a.n[q] = ctx->kern[d].n[q];
+            a.L[q] = ctx->kern[d].L[q];
+        }
+        a.poly = a.conj = false;
+        wmax = std::max(wmax, conv_work_elems(a));
         na++;
     }
     // P applications (:35-65): pairs (c1, c2), c2 >= c1; off-diagonal pairs twice
@@ -374,14 +507,55 @@ extern "C" int peec_operator_set(peec_ctx *ctx, int64_t mx, int64_t my, int64_t 
                 const peec_csc_f64 *min_ = &p_map[3 * j_in + i_in];  // exp_P[i_in, j_in]
                 const peec_csc_f64 *mout = &p_map[3 * i_in + j_in];  // exp_P[j_in, i_in]
                 ConvApp &a = ctx->apps[na];
+                Kernel &K = ctx->kern[k];
                 a.kernel = k;
                 a.in_off = m;
                 a.out_off = m;
-                if ((rc = map_from_csc(min_, ctx->kern[k].compact(), 0, ns, &a.map_in, &a.nnz_in, "exp_P (scatter)")))
+                for (int q = 0; q < 3; q++) {
+                    a.n[q] = K.n[q];
+                    a.L[q] = K.L[q];
+                }
+                a.poly = a.conj = false;
+                // Half-cell reduction of an off-diagonal block: when every
+                // scattered and every gathered position keeps one parity
+                // per half-cell axis, convolve on the coarse box
+                if (c1 != c2 && min_->nrow == K.compact() && mout->nrow == K.compact()) {
+                    int pin[3], pout[3], delta[3] = {0, 0, 0};
+                    map_parity(min_, K.n, kHalfAxes[k], pin);
+                    map_parity(mout, K.n, kHalfAxes[k], pout);
+                    bool ok = true;
+                    for (int q = 0; q < 3; q++)
+                        if (kHalfAxes[k][q]) {
+                            if (pin[q] < 0 || pout[q] < 0) ok = false;
+                            else delta[q] = pout[q] - pin[q];
+                        }
+                    if (ok && !K.poly && (rc = poly_spectrum_build(ctx, k, delta))) return rc;
+                    bool same = ok, flip = ok;
+                    for (int q = 0; q < 3 && ok; q++) {
+                        same = same && delta[q] == K.delta[q];
+                        flip = flip && delta[q] == -K.delta[q];
+                    }
+                    if (ok && (same || flip)) {
+                        a.poly = true;
+                        a.conj = !same;
+                        for (int q = 0; q < 3; q++) {
+                            a.n[q] = K.pn[q];
+                            a.L[q] = K.pL[q];
+                        }
+                    }
+                }
+                CoarseMap cm;
+                for (int q = 0; q < 3; q++) {
+                    cm.nf[q] = K.n[q];
+                    cm.half[q] = kHalfAxes[k][q];
+                    cm.nb[q] = K.pn[q];
+                }
+                const CoarseMap *cmp = a.poly ? &cm : nullptr;
+                if ((rc = map_from_csc(min_, K.compact(), 0, ns, &a.map_in, &a.nnz_in, "exp_P (scatter)", cmp)))
                     return rc;
-                if ((rc = map_from_csc(mout, ctx->kern[k].compact(), 0, ns, &a.map_out, &a.nnz_out, "exp_P (gather)")))
+                if ((rc = map_from_csc(mout, K.compact(), 0, ns, &a.map_out, &a.nnz_out, "exp_P (gather)", cmp)))
                     return rc;
-                wmax = std::max(wmax, conv_work_elems(ctx, k));
+                wmax = std::max(wmax, conv_work_elems(a));
                 na++;
             }
         }
@@ -652,7 +826,7 @@ int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t l
     hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns)), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
     PEEC_HIP_TRY(hipGetLastError());
     size_t need = 0;
-    for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx, ctx->apps[a].kernel));
+    for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx->apps[a]));
     need *= (size_t)nrhs;
     if (need > ctx->w1.n) {
         if (!ctx->w1.alloc(need) || !ctx->w2.alloc(need)) {
@@ -689,6 +863,134 @@ int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int
     int rc = apply_K(ctx, x, y, nrhs, ld, fp32, rps);
     if (rc) return rc;
     return apply_precond(ctx, y, nrhs, ld, rps);
+}
+
+// y = K^H x (device vectors)
+int apply_KH(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps)
+{
+    const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n;
+    hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns)), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
+    PEEC_HIP_TRY(hipGetLastError());
+    size_t need = 0;
+    for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx->apps[a]));
+    need *= (size_t)nrhs;
+    if (need > ctx->w1.n && (!ctx->w1.alloc(need) || !ctx->w2.alloc(need))) {
+        peec_set_error("out of device memory for %d-rhs FFT work grids", nrhs);
+        return PEEC_ERR_NOMEM;
+    }
+    for (int a = 0; a < ctx->napps; a++) {
+        int rc = conv_apply(ctx, ctx->apps[a], fp32, x, y, nrhs, ld, ctx->w1.p, ctx->w2.p);
+        if (rc) return rc;
+    }
+    const int64_t phi0 = m + ns;
+    hipLaunchKernelGGL(k_y1h, dim3(nblk(m)), dim3(256), 0, ctx->stream, y, x, ctx->DZ.p, ctx->A.ptr.p, ctx->A.idx.p,
+                       ctx->A.val.p, ctx->wv.p, m, phi0, nrhs, ld, rps);
+    hipLaunchKernelGGL(k_y2h, dim3(nblk(ns)), dim3(256), 0, ctx->stream, y, x, ctx->Gt.ptr.p, ctx->Gt.idx.p,
+                       ctx->Gt.val.p, ctx->wv.p, m, ns, phi0, nrhs, ld, rps);
+    hipLaunchKernelGGL(k_y3h, dim3(nblk(n)), dim3(256), 0, ctx->stream, y, x, ctx->At.ptr.p, ctx->At.idx.p,
+                       ctx->At.val.p, ctx->Yle.ptr.p, ctx->Yle.idx.p, ctx->Yle.val.p, ctx->G.ptr.p, ctx->G.idx.p,
+                       ctx->G.val.p, m, ns, n, nrhs, ld, rps);
+    PEEC_HIP_TRY(hipGetLastError());
+    return PEEC_OK;
+}
+
+// In place: y <- M^-H y, through the forward factors of SS
+int apply_precond_adj(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, int rps)
+{
+    const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n;
+    hipLaunchKernelGGL(k_adj_xform, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, ctx->wv.p, m, ns, n, nrhs,
+                       ld, rps, 0);
+    PEEC_HIP_TRY(hipGetLastError());
+    int rc = apply_precond(ctx, y, nrhs, ld, rps);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_adj_xform, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, ctx->wv.p, m, ns, n, nrhs,
+                       ld, rps, 1);
+    PEEC_HIP_TRY(hipGetLastError());
+    return PEEC_OK;
+}
+
+// adjoint GMRES operator: y = M^-H K^H x
+int matvec_device_adj(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps)
+{
+    int rc = apply_KH(ctx, x, y, nrhs, ld, fp32, rps);
+    if (rc) return rc;
+    return apply_precond_adj(ctx, y, nrhs, ld, rps);
+}
+
+// Allocate every work buffer a matvec of nrhs right-hand sides uses, so
+// that a captured matvec allocates nothing.
+static int matvec_prepare(peec_ctx *ctx, int nrhs)
+{
+    size_t need = 0;
+    for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx->apps[a]));
+    need *= (size_t)nrhs;
+    if (need > ctx->w1.n && (!ctx->w1.alloc(need) || !ctx->w2.alloc(need))) {
+        peec_set_error("out of device memory for %d-rhs FFT work grids", nrhs);
+        return PEEC_ERR_NOMEM;
+    }
+    if ((size_t)(nrhs * ctx->n) > ctx->vt1.n &&
+        (!ctx->vt1.alloc((size_t)(nrhs * ctx->n)) || !ctx->vt2.alloc((size_t)(nrhs * ctx->n)))) {
+        peec_set_error("out of device memory for %d-rhs triangular-solve vectors", nrhs);
+        return PEEC_ERR_NOMEM;
+    }
+    const size_t part = (size_t)std::max<int64_t>(std::max<int64_t>(ctx->Lf.nseg, ctx->Uf.nseg), 1) * (size_t)nrhs;
+    if (part > ctx->trsv_part.n && !ctx->trsv_part.alloc(part)) {
+        peec_set_error("out of device memory for triangular-solve partial sums");
+        return PEEC_ERR_NOMEM;
+    }
+    return PEEC_OK;
+}
+
+MatvecGraphs::~MatvecGraphs()
+{
+    for (auto &e : entries) (void)hipGraphExecDestroy(e.exec);
+}
+
+int MatvecGraphs::run(const double2_t *x, double2_t *y, bool fp32)
+{
+    auto direct = [&]() {
+        return adjoint ? matvec_device_adj(ctx, x, y, nrhs, ld, fp32, rps) : matvec_device(ctx, x, y, nrhs, ld, fp32, rps);
+    };
+    if (!ctx->use_graphs || ctx->prof_on) return direct();
+    for (auto &e : entries)
+        if (e.x == x && e.y == y && e.fp32 == (int)fp32) {
+            PEEC_HIP_TRY(hipGraphLaunch(e.exec, ctx->stream));
+            return PEEC_OK;
+        }
+    int rc = matvec_prepare(ctx, nrhs);
+    if (rc) return rc;
+    PEEC_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    rc = direct();
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) {
+        peec_set_error("matvec graph capture failed: %s", hipGetErrorString(ec));
+        return PEEC_ERR_HIP;
+    }
+    Entry e{x, y, (int)fp32, nullptr};
+    const hipError_t ei = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        peec_set_error("matvec graph instantiation failed: %s", hipGetErrorString(ei));
+        return PEEC_ERR_HIP;
+    }
+    entries.push_back(e);
+    PEEC_HIP_TRY(hipGraphLaunch(e.exec, ctx->stream));
+    return PEEC_OK;
+}
+
+extern "C" int peec_set_graphs(peec_ctx *ctx, int on)
+{
+    if (!ctx) {
+        peec_set_error("peec_set_graphs: null context");
+        return PEEC_ERR_ARG;
+    }
+    ctx->use_graphs = on != 0;
+    return PEEC_OK;
 }
 
 static int ready(peec_ctx *ctx, const char *what)
@@ -733,8 +1035,23 @@ extern "C" int peec_apply_K(peec_ctx *ctx, const peec_c128 *x, peec_c128 *y)
     return PEEC_OK;
 }
 
+static int apply_batched_impl(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, int precond, bool adjoint,
+                              const peec_c128 *x, peec_c128 *y);
+
 extern "C" int peec_apply_K_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, int precond,
                                     const peec_c128 *x, peec_c128 *y)
+{
+    return apply_batched_impl(ctx, nrhs, rhs_per_slot, fp32, precond, false, x, y);
+}
+
+extern "C" int peec_apply_KH_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, int precond,
+                                     const peec_c128 *x, peec_c128 *y)
+{
+    return apply_batched_impl(ctx, nrhs, rhs_per_slot, fp32, precond, true, x, y);
+}
+
+static int apply_batched_impl(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, int precond, bool adjoint,
+                              const peec_c128 *x, peec_c128 *y)
 {
     if (!ctx || nrhs < 1 || rhs_per_slot < 1 || !x || !y) {
         peec_set_error("peec_apply_K_batched: invalid argument");
@@ -759,8 +1076,18 @@ extern "C" int peec_apply_K_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, i
         return PEEC_ERR_NOMEM;
     }
     PEEC_HIP_TRY(hipMemcpyAsync(X.p, x, bytes, hipMemcpyHostToDevice, ctx->stream));
-    int rc = precond ? matvec_device(ctx, X.p, Y.p, nrhs, N, fp32 != 0, rhs_per_slot)
+    int rc;
+    if (precond == 2) {
+        // the preconditioner alone: M^-1 x or M^-H x
+        PEEC_HIP_TRY(hipMemcpyAsync(Y.p, X.p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        rc = adjoint ? apply_precond_adj(ctx, Y.p, nrhs, N, rhs_per_slot) : apply_precond(ctx, Y.p, nrhs, N, rhs_per_slot);
+    } else if (adjoint) {
+        rc = precond ? matvec_device_adj(ctx, X.p, Y.p, nrhs, N, fp32 != 0, rhs_per_slot)
+                     : apply_KH(ctx, X.p, Y.p, nrhs, N, fp32 != 0, rhs_per_slot);
+    } else {
+        rc = precond ? matvec_device(ctx, X.p, Y.p, nrhs, N, fp32 != 0, rhs_per_slot)
                      : apply_K(ctx, X.p, Y.p, nrhs, N, fp32 != 0, rhs_per_slot);
+    }
     if (rc) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(y, Y.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -59,6 +59,24 @@ class Mesh:
     """Output of mesh(): every object the FFT solver hot path consumes."""
 
 
+def conductor_terms(M, sigma_vox):
+    """(diag_R scaled m x 4, Zs m) of mesh M with voxel conductivities
+    sigma_vox in place of M.sigma_vox (compute_diagonals.jl:10-16,
+    compute_Zs_with_indices.jl); every other slot keeps its value."""
+    c = M.cond
+    sig = np.asarray(sigma_vox, dtype=np.float64)
+    R = M.diag_R / M.escalings["R"]
+    on = c["slot_vox"] >= 0
+    R = R.copy()
+    R[on] = c["slot_lh"][on] / (sig[c["slot_vox"][on]] * c["slot_wt"][on])
+    Zs = np.zeros(M.m, dtype=np.complex128)
+    for h in range(2):
+        v = c["zs_vox"][:, h]
+        ok = v >= 0
+        Zs[ok] += c["zs_geo"][ok, h] * np.sqrt(2 * np.pi * 1e-7 / sig[v[ok]])
+    return M.escalings["R"] * R, Zs
+
+
 def _lp_self(l, W, T):
     # compute_diagonals.jl:274-293, host copy for the diagonal terms
     w = W / l
@@ -84,11 +102,16 @@ def _sel(rows, cols, nrow, ncol):
                          shape=(nrow, ncol))
 
 
-def mesh(mat_grid, materials, cell, origin=(0.0, 0.0, 0.0), ports=(), lumped_elements=(), p_self_fn=None):
+def mesh(mat_grid, materials, cell, origin=(0.0, 0.0, 0.0), ports=(), lumped_elements=(), p_self_fn=None,
+         sigma_grid=None):
     """mat_grid: int array (Nx, Ny, Nz), 0 = empty, k >= 1 = materials[k-1].
     ports: list of (start_xyz, end_xyz) in metres; lumped_elements: list of
     dicts with start, end, type, R, L, C.  p_self_fn(sx, sy, sz) -> (self P of
-    y-, x-, z-faces) unscaled (compute_diagonal_P_v2, mesher_FFT.jl:114-138)."""
+    y-, x-, z-faces) unscaled (compute_diagonal_P_v2, mesher_FFT.jl:114-138).
+    sigma_grid (Nx, Ny, Nz), optional: the conductivity of each conductor
+    voxel in place of its material's (the reference assigns sigma per
+    material, compute_diagonals.jl:5-33; per-voxel values are what the
+    conductivity gradients are taken against)."""
     g = np.asarray(mat_grid, dtype=np.int64)
     Nx, Ny, Nz = g.shape
     sx, sy, sz = (float(c) for c in cell)
@@ -116,6 +139,15 @@ def mesh(mat_grid, materials, cell, origin=(0.0, 0.0, 0.0), ports=(), lumped_ele
     M.vox_centers[vox_id[sel], 2] = origin[2] + sz * zs[sel] + sz / 2
     M.vox_material = np.zeros(n, dtype=np.int64)
     M.vox_material[vox_id[sel]] = g[sel]
+    # conductivity per voxel (0 for non-conductors)
+    mat_sigma = np.array([0.0] + [mt.sigmar for mt in materials])
+    M.sigma_vox = mat_sigma[M.vox_material]
+    if sigma_grid is not None:
+        sg = np.asarray(sigma_grid, dtype=np.float64)[sel]
+        cond = M.sigma_vox[vox_id[sel]] != 0
+        M.sigma_vox[vox_id[sel][cond]] = sg[cond]
+        if np.any(M.sigma_vox[vox_id[sel][cond]] <= 0):
+            raise ValueError("sigma_grid must be positive on conductor voxels")
 
     # edges
     def edges(axis):
@@ -281,19 +313,28 @@ def mesh(mat_grid, materials, cell, origin=(0.0, 0.0, 0.0), ports=(), lumped_ele
     R = np.zeros((m, 4))
     Cd = np.zeros((m, 4), dtype=np.complex128)
     lens = [(sx, sy, sz), (sy, sx, sz), (sz, sy, sx)]
+    # conductor slots: voxel, half length and cross-section (R = lh / (sigma wt))
+    slot_vox = -np.ones((m, 4), dtype=np.int64)
+    slot_lh = np.zeros((m, 4))
+    slot_wt = np.zeros((m, 4))
     start = 0
     dl = []
     for d, (E_d, cnt) in enumerate(zip(E, (M.mx, M.my, M.mz))):
         sl = slice(start, start + cnt)
         l, W, T = lens[d]
         for slot, mats in ((0, li_mat[sl, 0]), (1, li_mat[sl, 1]), (2, border[sl, 0]), (3, border[sl, 1])):
+            vox = (v1 if slot in (0, 2) else v2)[sl]
             for kmat in np.unique(mats):
                 if kmat == 0:
                     continue
                 mat = materials[kmat - 1]
-                idx = np.nonzero(mats == kmat)[0] + start
+                loc = np.nonzero(mats == kmat)[0]
+                idx = loc + start
                 if mat.sigmar != 0:
-                    R[idx, slot] = 0.5 * l / (mat.sigmar * W * T)
+                    R[idx, slot] = 0.5 * l / (M.sigma_vox[vox[loc]] * W * T)
+                    slot_vox[idx, slot] = vox[loc]
+                    slot_lh[idx, slot] = 0.5 * l
+                    slot_wt[idx, slot] = W * T
                     if mat.epsr != 1:
                         Cd[idx, slot] = EPS0 * (mat.epsr - 1) * W * T / (0.5 * l)
                 else:
@@ -311,8 +352,11 @@ def mesh(mat_grid, materials, cell, origin=(0.0, 0.0, 0.0), ports=(), lumped_ele
     else:
         M.diag_P = None
 
-    # skin-effect surface impedance per edge (compute_Zs_with_indices.jl)
+    # skin-effect surface impedance per edge (compute_Zs_with_indices.jl):
+    # Zs = sum over the two halves of zs_geo * sqrt(2 pi 1e-7 / sigma)
     Zs = np.zeros(m, dtype=np.complex128)
+    zs_vox = -np.ones((m, 2), dtype=np.int64)
+    zs_geo = np.zeros((m, 2), dtype=np.complex128)
     start = 0
     for d, e in enumerate(E):
         cnt = e["v1"].size
@@ -335,9 +379,14 @@ def mesh(mat_grid, materials, cell, origin=(0.0, 0.0, 0.0), ports=(), lumped_ele
                     if mat.sigmar == 0:
                         continue
                     sel_ = mat_ids == kmat
-                    Zs[start + np.nonzero(sel_)[0]] += cntf[sel_] * 0.5 * l / width * (1 + 1j) * np.sqrt(2 * np.pi * 1e-7 / mat.sigmar)
+                    rows_ = start + np.nonzero(sel_)[0]
+                    geo = cntf[sel_] * 0.5 * l / width * (1 + 1j)
+                    Zs[rows_] += geo * np.sqrt(2 * np.pi * 1e-7 / M.sigma_vox[vk[sel_]])
+                    zs_geo[rows_, half] += geo
+                    zs_vox[rows_, half] = vk[sel_]
         start += cnt
     M.Zs = Zs
+    M.cond = dict(slot_vox=slot_vox, slot_lh=slot_lh, slot_wt=slot_wt, zs_vox=zs_vox, zs_geo=zs_geo)
     M.surface_edges = np.nonzero(np.abs(Zs) != 0)[0]
 
     # ports / lumped elements: nearest voxel centre, the end skipping the

@@ -211,10 +211,11 @@ def precond_rhs_batched(IS, rhs_per_slot, ctx=None):
     return out.T
 
 
-def gmres_batched(B, rhs_per_slot, tol, X0=None, restart=60, max_cycles=20, mixed=False, inner_tol=0.0, ctx=None):
-    """Solve (M^-1 K) x_r = b_r for the columns of B (N x nrhs), column r
-    against frequency slot r / rhs_per_slot.  Returns (X, flag, relres,
-    iters, cycles)."""
+def gmres_batched(B, rhs_per_slot, tol, X0=None, restart=60, max_cycles=20, mixed=False, inner_tol=0.0, ctx=None,
+                  adjoint=False):
+    """Solve (M^-1 K) x_r = b_r (adjoint: (M^-H K^H) x_r = b_r) for the
+    columns of B (N x nrhs), column r against frequency slot
+    r / rhs_per_slot.  Returns (X, flag, relres, iters, cycles)."""
     ctx = ctx or default_context()
     Bt = np.ascontiguousarray(np.asarray(B, dtype=np.complex128).T)  # column-major N x nrhs
     nrhs, N = Bt.shape
@@ -225,23 +226,44 @@ def gmres_batched(B, rhs_per_slot, tol, X0=None, restart=60, max_cycles=20, mixe
     iters = np.zeros(nrhs, dtype=np.int32)
     cycles = np.zeros(nrhs, dtype=np.int32)
     ip = ctypes.POINTER(ctypes.c_int)
-    L.check(L.lib().peec_gmres_batched(ctx.handle, nrhs, int(rhs_per_slot), L.as_c128p(Bt), float(tol),
-                                       int(restart), int(max_cycles), int(bool(mixed)), float(inner_tol),
-                                       L.as_c128p(Xt), flag.ctypes.data_as(ip), L.as_dp(relres),
-                                       iters.ctypes.data_as(ip), cycles.ctypes.data_as(ip)), "peec_gmres_batched")
+    fn = L.lib().peec_gmres_batched_adjoint if adjoint else L.lib().peec_gmres_batched
+    L.check(fn(ctx.handle, nrhs, int(rhs_per_slot), L.as_c128p(Bt), float(tol), int(restart), int(max_cycles),
+               int(bool(mixed)), float(inner_tol), L.as_c128p(Xt), flag.ctypes.data_as(ip), L.as_dp(relres),
+               iters.ctypes.data_as(ip), cycles.ctypes.data_as(ip)), "peec_gmres_batched")
     return Xt.T, flag, relres, iters, cycles
 
 
-def apply_K_batched(X, rhs_per_slot=1, fp32=False, precond=False, ctx=None):
-    """K x (or M^-1 K x with precond) for the columns of X (N x nrhs), column
-    r against frequency slot r / rhs_per_slot, with complex128 or (fp32)
-    complex64 FFT convolutions."""
+def apply_K_batched(X, rhs_per_slot=1, fp32=False, precond=False, adjoint=False, ctx=None):
+    """K x (precond False / 0), M^-1 K x (precond True / 1) or M^-1 x
+    (precond 2) for the columns of X (N x nrhs), column r against frequency
+    slot r / rhs_per_slot, with complex128 or (fp32) complex64 FFT
+    convolutions; adjoint: K^H x, M^-H K^H x or M^-H x."""
     ctx = ctx or default_context()
     Xt = np.ascontiguousarray(np.asarray(X, dtype=np.complex128).T)
     Yt = np.zeros_like(Xt)
-    L.check(L.lib().peec_apply_K_batched(ctx.handle, Xt.shape[0], int(rhs_per_slot), int(bool(fp32)),
-                                         int(bool(precond)), L.as_c128p(Xt), L.as_c128p(Yt)), "peec_apply_K_batched")
+    fn = L.lib().peec_apply_KH_batched if adjoint else L.lib().peec_apply_K_batched
+    L.check(fn(ctx.handle, Xt.shape[0], int(rhs_per_slot), int(bool(fp32)), int(precond), L.as_c128p(Xt),
+               L.as_c128p(Yt)), "peec_apply_K%s_batched" % ("H" if adjoint else ""))
     return Yt.T
+
+
+def precond_rhs_batched_adjoint(G, rhs_per_slot, ctx=None):
+    """M^-H [0; 0; g_r] for the columns of the complex node excitations G
+    (n x nrhs)."""
+    ctx = ctx or default_context()
+    Gt = np.ascontiguousarray(np.asarray(G, dtype=np.complex128).T)
+    nrhs, n = Gt.shape
+    N = ctx_size(ctx)
+    T = np.zeros((nrhs, N), dtype=np.complex128)
+    L.check(L.lib().peec_precond_rhs_batched_adjoint(ctx.handle, nrhs, int(rhs_per_slot), L.as_c128p(Gt),
+                                                     L.as_c128p(T)), "peec_precond_rhs_batched_adjoint")
+    return T.T
+
+
+def set_graphs(on, ctx=None):
+    """Replay GMRES matvecs from HIP graphs (default) or launch kernels one by one."""
+    ctx = ctx or default_context()
+    L.check(L.lib().peec_set_graphs(ctx.handle, int(bool(on))), "peec_set_graphs")
 
 
 def fft_tune(nrhs, fp64=True, fp32=True, verbose=False, ctx=None):

@@ -13,7 +13,7 @@ import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from . import ordering, peec
+from . import mesher, ordering, peec
 
 
 def lumped_admittance(typ, R, L, C, w):
@@ -83,17 +83,74 @@ def compute_Z_self(R, Cd, w):
     return Z
 
 
-def DZ_of(mesh, w):
-    """FFT_solver_QS_S_type.jl:106-114 with the scaled angular frequency w."""
+def DZ_of(mesh, w, sigma=None):
+    """FFT_solver_QS_S_type.jl:106-114 with the scaled angular frequency w
+    (sigma: per-voxel conductivities in place of the mesh's)."""
     esc = mesh.escalings
-    Z_self = compute_Z_self(mesh.diag_R, mesh.diag_Cd, w)
-    Zs = esc["R"] * (mesh.Zs * np.sqrt(w / esc["freq"]))
+    diag_R, Zs0 = (mesh.diag_R, mesh.Zs) if sigma is None else mesher.conductor_terms(mesh, sigma)
+    Z_self = compute_Z_self(diag_R, mesh.diag_Cd, w)
+    Zs = esc["R"] * (Zs0 * np.sqrt(w / esc["freq"]))
     se = mesh.surface_edges
     d = np.real(Zs[se]) - np.real(Z_self[se])
     a, b = se[d > 0], se[d < 0]
     Z_self[a] = 1j * np.imag(Z_self[a])
     Zs[b] = 1j * np.imag(Zs[b])
     return Z_self + np.real(Zs) + 1j * w * mesh.diag_Lp
+
+
+def dDZ_dsigma(mesh, w, sigma=None):
+    """d DZ_e / d sigma_v (m x n sparse, complex) at the scaled angular
+    frequency w, on the branch of the skin-effect switch DZ_of takes
+    (FFT_solver_QS_S_type.jl:107-114): the half-voxel resistances
+    R = l/2 / (sigma W T) (compute_diagonals.jl:10-16) enter Z_self, the
+    surface impedance Zs ~ sigma^-1/2 (compute_Zs_with_indices.jl) enters
+    Re DZ where it wins the switch.  The switch itself is not differentiable
+    at its ties (Re Zs = Re Z_self)."""
+    esc = mesh.escalings
+    c = mesh.cond
+    sig = mesh.sigma_vox if sigma is None else np.asarray(sigma, dtype=np.float64)
+    diag_R, Zs0 = mesher.conductor_terms(mesh, sig)
+    Cd = mesh.diag_Cd
+    m = mesh.m
+    Z_self = compute_Z_self(diag_R, Cd, w)
+    Zs = esc["R"] * (Zs0 * np.sqrt(w / esc["freq"]))
+    se = mesh.surface_edges
+    d = np.real(Zs[se]) - np.real(Z_self[se])
+    skin = np.zeros(m, dtype=bool)      # Re Zs replaces Re Z_self
+    own = np.ones(m, dtype=bool)        # Z_self keeps its real part
+    zs_on = np.zeros(m, dtype=bool)     # Re Zs is added
+    skin[se[d > 0]] = True
+    own[se[d > 0]] = False
+    zs_on[se[d >= 0]] = True
+    rows, cols, vals = [], [], []
+    for slot in range(4):
+        v = c["slot_vox"][:, slot]
+        e = np.nonzero(v >= 0)[0]
+        if e.size == 0:
+            continue
+        R = diag_R[e, slot]
+        C = Cd[e, slot]
+        # d f / d R with f = R (no Cd) or 1 / (1/R + j w Cd) (compute_Z_self.jl:4-12)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            f = np.where(C != 0, 1 / (1 / R + 1j * w * C), R)
+            dfdR = np.where(C != 0, (f / R) ** 2, 1.0)
+        dval = dfdR * (-R / sig[v[e]])
+        dval = np.where(own[e], dval, 1j * np.imag(dval))
+        rows.append(e)
+        cols.append(v[e])
+        vals.append(dval)
+    for h in range(2):
+        v = c["zs_vox"][:, h]
+        e = np.nonzero((v >= 0) & zs_on)[0]
+        if e.size == 0:
+            continue
+        zh = esc["R"] * c["zs_geo"][e, h] * np.sqrt(2 * np.pi * 1e-7 / sig[v[e]]) * np.sqrt(w / esc["freq"])
+        rows.append(e)
+        cols.append(v[e])
+        vals.append(np.real(-zh / (2 * sig[v[e]])) + 0j)
+    if not rows:
+        return sp.csr_matrix((m, mesh.n), dtype=np.complex128)
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(m, mesh.n))
 
 
 def SS_of(mesh, w, DZ, Yle):
@@ -149,11 +206,11 @@ class Solver:
             self.tuned[key] = peec.fft_tune(nrhs, fp64=fp64, fp32=fp32, verbose=verbose, ctx=self.ctx)
         return self.tuned[key]
 
-    def set_frequency(self, w, R0=50.0):
+    def set_frequency(self, w, R0=50.0, sigma=None):
         m = self.mesh
         esc = m.escalings
         Yle = build_Yle_FFT(m.lumped_elements, m.port_nodes, esc, m.n, w / esc["freq"], R0)
-        DZ = DZ_of(m, w)
+        DZ = DZ_of(m, w, sigma)
         peec.frequency_set(w, DZ, Yle, ctx=self.ctx)
         if self.device_lu:
             peec.lu_factor(ctx=self.ctx)
@@ -169,7 +226,8 @@ class Solver:
         return peec.ComputeMatrixVectorFTT(x, ctx=self.ctx)
 
 
-def FFT_solver_QS_S_type(freq, mesh, inner_iter, tol, ports_scatter_value=50.0, solver=None, record=None):
+def FFT_solver_QS_S_type(freq, mesh, inner_iter, tol, ports_scatter_value=50.0, solver=None, record=None,
+                         sigma=None, keep=False):
     """FFT_solver_QS_S_type.jl:1-234 for QS_Rcc_FW = 1.  Returns
     dict(S, Z, Y, f), or None when a stop was requested."""
     solver = solver or Solver(mesh)
@@ -183,10 +241,11 @@ def FFT_solver_QS_S_type(freq, mesh, inner_iter, tol, ports_scatter_value=50.0, 
     S = np.zeros((P, P, len(freq)), dtype=np.complex128)
     Vrest = np.zeros((m + n + ns, P), dtype=np.complex128)
     is_ = np.zeros(n)
+    Vkeep = []
     for k in range(len(freq)):
         if solver.ctx._stop.value:
             return None
-        solver.set_frequency(w[k], R0)
+        solver.set_frequency(w[k], R0, sigma)
         for c1 in range(P):
             n1, n2 = mesh.port_nodes[c1]
             is_[n1] = esc["Is"]
@@ -208,7 +267,12 @@ def FFT_solver_QS_S_type(freq, mesh, inner_iter, tol, ports_scatter_value=50.0, 
                 else:
                     S[c1, c2, k] = (2 * (V[m + ns + n3] - V[m + ns + n4])) / R0
                     S[c2, c1, k] = S[c1, c2, k]
-    return dict(S=S, Z=s2z_fft(S, R0), Y=s2y_fft(S, R0), f=freq / esc["freq"])
+        if keep:
+            Vkeep.append(Vrest.copy())
+    out = dict(S=S, Z=s2z_fft(S, R0), Y=s2y_fft(S, R0), f=freq / esc["freq"])
+    if keep:
+        out["V"] = Vkeep
+    return out
 
 
 def s2z_fft(S, Zo):
@@ -226,7 +290,7 @@ def s2y_fft(S, Zo):
     return np.stack([Zo * np.linalg.solve(I + S[:, :, k], I - S[:, :, k]) for k in range(S.shape[2])], axis=2)
 
 
-def set_slots(solver, ws, R0=50.0, timings=None):
+def set_slots(solver, ws, R0=50.0, timings=None, sigma=None):
     """Fill frequency slots 0 .. len(ws)-1 (already allocated) with the
     per-frequency terms at scaled angular frequencies ws and factor each
     slot's SS on the device (FFT_solver_QS_S_type.jl:105-119 per slot)."""
@@ -236,7 +300,7 @@ def set_slots(solver, ws, R0=50.0, timings=None):
     for slot, w in enumerate(ws):
         t0 = time.perf_counter()
         Yle = build_Yle_FFT(mesh.lumped_elements, mesh.port_nodes, esc, mesh.n, w / esc["freq"], R0)
-        peec.frequency_set_slot(slot, w, DZ_of(mesh, w), Yle, ctx=solver.ctx)
+        peec.frequency_set_slot(slot, w, DZ_of(mesh, w, sigma), Yle, ctx=solver.ctx)
         t1 = time.perf_counter()
         peec.lu_factor_slot(slot, ctx=solver.ctx)
         tm["freq_terms"] = tm.get("freq_terms", 0.0) + t1 - t0
@@ -244,7 +308,8 @@ def set_slots(solver, ws, R0=50.0, timings=None):
 
 
 def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_cycles=20, inner_tol=0.0,
-                          ports_scatter_value=50.0, solver=None, record=None, timings=None, tune=False, Vrest=None):
+                          ports_scatter_value=50.0, solver=None, record=None, timings=None, tune=False, Vrest=None,
+                          sigma=None, keep=False):
     """The QS sweep of FFT_solver_QS_S_type.jl:60-232 with its frequency x
     port loop batched: `batch` frequencies at a time, every port of each,
     go through one peec_gmres_batched call (one batched matvec per
@@ -257,7 +322,8 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_
     tune = True autotunes the FFT passes for each batch width first
     (timed as "tune").  Vrest (N x P) is the warm start of the first batch
     (zeros by default); the result's "Vrest" is the last frequency's
-    solution, to chain a sweep split over several calls."""
+    solution, to chain a sweep split over several calls.  sigma and keep as
+    in FFT_solver_QS_S_type."""
     solver = solver or Solver(mesh, device_lu=True)
     tm = timings if timings is not None else {}
     for key in ("tune", "freq_terms", "lu", "precond_rhs", "gmres"):
@@ -275,6 +341,7 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_
     S = np.zeros((P, P, F), dtype=np.complex128)
     Xprev = np.zeros((N, P), dtype=np.complex128) if Vrest is None else np.asarray(Vrest, dtype=np.complex128)
     nslots = None
+    Vkeep = []
     for g0 in range(0, F, batch):
         ks = list(range(g0, min(F, g0 + batch)))
         nb = len(ks)
@@ -285,7 +352,7 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_
             t0 = time.perf_counter()
             solver.tune(nb * P, fp64=True, fp32=mixed)
             tm["tune"] += time.perf_counter() - t0
-        set_slots(solver, w[ks], R0, tm)
+        set_slots(solver, w[ks], R0, tm, sigma)
         t0 = time.perf_counter()
         IS = np.zeros((n, nb * P))
         for slot in range(nb):
@@ -303,6 +370,8 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_
         Xprev = X[:, (nb - 1) * P:nb * P]
         for slot, k in enumerate(ks):
             V = X[:, slot * P:(slot + 1) * P]
+            if keep:
+                Vkeep.append(np.array(V))
             for c1 in range(P):
                 if record is not None:
                     r = slot * P + c1
@@ -313,4 +382,106 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_
                     dv = V[m + ns + n3, c1] - V[m + ns + n4, c1]
                     S[c1, c2, k] = (2 * dv - R0) / R0 if c1 == c2 else 2 * dv / R0
                     S[c2, c1, k] = S[c1, c2, k]
-    return dict(S=S, Z=s2z_fft(S, R0), Y=s2y_fft(S, R0), f=freq / esc["freq"], Vrest=Xprev)
+    out = dict(S=S, Z=s2z_fft(S, R0), Y=s2y_fft(S, R0), f=freq / esc["freq"], Vrest=Xprev)
+    if keep:
+        out["V"] = Vkeep
+    return out
+
+
+# ---------------------------------------------------------------- gradients
+def _port_vector(mesh, weights):
+    """Node excitation sum_j weights[j] (e_{n1_j} - e_{n2_j}) (n, complex)."""
+    g = np.zeros(mesh.n, dtype=np.complex128)
+    for j, (n1, n2) in enumerate(mesh.port_nodes):
+        g[n1] += weights[j]
+        g[n2] -= weights[j]
+    return g
+
+
+def impedance_objective_gradient(freq, mesh, port=0, sigma=None, tol=1e-10, batch=8, restart=60, max_cycles=20,
+                                 mixed=False, ports_scatter_value=50.0, solver=None, method="adjoint", record=None):
+    """J = sum_f |Z_pp(f)|^2 and dJ/dsigma_v for every voxel (zero off the
+    conductors), by the adjoint method.
+
+    With Z = R0 (I - S)^-1 (I + S) and A = (I - S)^-1 per frequency,
+    dZ_pp = 2 R0 sum_ij A_pi dS_ij A_jp; S_ij = 2 g_j' V_i / R0 - delta_ij
+    (FFT_solver_QS_S_type.jl:203-205, V_i the solution for port i, g_j the
+    node-voltage difference of port j) and dV_i = -K^-1 dK V_i with dK =
+    diag(dDZ) on the edge block, so
+        dZ_pp = -4 lambda_I' (dDZ .* U),  U = sum_i A_pi V_i (edge block),
+    where K^T lambda = sum_j A_jp g_j.  method = "adjoint" solves
+    K^H conj(lambda) = conj(sum_j A_jp g_j) on the device
+    (peec_gmres_batched_adjoint: explicit K^H, M^-H from the forward
+    factors), one right-hand side per frequency; method = "reciprocity"
+    uses lambda_I = -sum_j A_jp V_j / Is (K^T = D1 K D2, no extra solve);
+    method = "plain" takes the forward solutions from the plain FP64 path
+    (gmres_custom per port, the reference's solve) with the reciprocity
+    adjoint.
+
+    Forward solves: the batched path (FP64, or mixed), all ports, keeping
+    the solutions.  Returns dict(J, grad (n), Z, f, adjoint_flags)."""
+    solver = solver or Solver(mesh, device_lu=True)
+    esc = mesh.escalings
+    R0 = ports_scatter_value
+    if method == "plain":
+        # the plain FP64 path's own solutions (gmres_custom per port) with
+        # the reciprocity adjoint
+        fw = FFT_solver_QS_S_type(freq, mesh, 400, tol, ports_scatter_value=R0, solver=solver, record=record,
+                                  sigma=sigma, keep=True)
+        method = "reciprocity"
+    else:
+        fw = FFT_solver_QS_batched(freq, mesh, tol, batch=batch, mixed=mixed, restart=restart, max_cycles=max_cycles,
+                                   ports_scatter_value=R0, solver=solver, record=record, sigma=sigma, keep=True)
+    S, Z = fw["S"], fw["Z"]
+    F = len(freq)
+    P = mesh.port_nodes.shape[0]
+    m = mesh.m
+    ws = 2 * np.pi * np.asarray(freq, dtype=np.float64) * esc["freq"]
+    I = np.eye(P)
+    Am = [np.linalg.inv(I - S[:, :, k]) for k in range(F)]
+    lam = [None] * F
+    flags = []
+    if method == "adjoint":
+        for g0 in range(0, F, batch):
+            ks = list(range(g0, min(F, g0 + batch)))
+            peec.frequency_slots(len(ks), ctx=solver.ctx)
+            set_slots(solver, ws[ks], R0, sigma=sigma)
+            G = np.stack([np.conj(_port_vector(mesh, Am[k][:, port])) for k in ks], axis=1)
+            TN = peec.precond_rhs_batched_adjoint(G, 1, ctx=solver.ctx)
+            eta, fl, relres, it, cyc = peec.gmres_batched(TN, 1, tol, None, restart, max_cycles, mixed,
+                                                          ctx=solver.ctx, adjoint=True)
+            flags += [int(x) for x in fl]
+            for slot, k in enumerate(ks):
+                lam[k] = np.conj(eta[:m, slot])
+    elif method == "reciprocity":
+        for k in range(F):
+            lam[k] = -(fw["V"][k][:m, :] @ Am[k][:, port]) / esc["Is"]
+    else:
+        raise ValueError("method must be 'adjoint', 'reciprocity' or 'plain'")
+    J = 0.0
+    grad = np.zeros(mesh.n)
+    for k in range(F):
+        z, dZ = impedance_sensitivity(mesh, ws[k], S[:, :, k], fw["V"][k], lam[k], port, sigma)
+        J += abs(z) ** 2
+        grad += 2.0 * np.real(np.conj(z) * dZ)
+    return dict(J=J, grad=grad, Z=Z, f=np.asarray(freq), adjoint_flags=flags)
+
+
+def impedance_sensitivity(mesh, w, S, V, lam_I, port, sigma=None, R0=50.0):
+    """(Z_pp, dZ_pp/dsigma (n)) at one frequency from the port scattering
+    matrix S (P x P), the forward solutions V (N x P) and the edge block of
+    the adjoint solution lam_I (K^T lambda = sum_j A_jp g_j):
+    dZ_pp = -4 lambda_I' (dDZ .* sum_i A_pi V_i)."""
+    P = S.shape[0]
+    A = np.linalg.inv(np.eye(P) - S)
+    Zm = R0 * A @ (np.eye(P) + S)
+    U = V[:mesh.m, :] @ A[port, :]
+    dZ = -4.0 * (dDZ_dsigma(mesh, w, sigma).T @ (lam_I * U))
+    return Zm[port, port], dZ
+
+
+def impedance_objective(freq, mesh, port=0, sigma=None, tol=1e-10, solver=None, inner_iter=400):
+    """J = sum_f |Z_pp(f)|^2 with the plain FP64 path (one port at a time,
+    gmres_custom; the reference's own solve), for finite differences."""
+    out = FFT_solver_QS_S_type(freq, mesh, inner_iter, tol, solver=solver, sigma=sigma)
+    return float(np.sum(np.abs(out["Z"][port, port, :]) ** 2)), out["Z"]

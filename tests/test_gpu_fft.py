@@ -28,7 +28,7 @@ def batched():
     g, mats, _ = configs._layered(20, 14, 6, cell, 4, 0.45, 4.4, 0.02, seed=3)
     ports = configs._x_face_ports(g, cell, 4)
     M = mesher.mesh(g, mats, cell, (0.0, 0.0, 0.0), ports, p_self_fn=pself)
-    S = sweep.Solver(M, device_lu=True)
+    S = sweep.Solver(M, ctx=peec.Context(0), device_lu=True)
     ws = [2 * np.pi * f * M.escalings["freq"] for f in (3e7, 1e9, 8e9)]
     peec.frequency_slots(NRHS, ctx=S.ctx)
     for slot, w in enumerate(ws):
@@ -95,7 +95,8 @@ def test_every_launch_shape_is_bitwise_identical(batched, fp32):
                 continue
             Y = peec.apply_K_batched(X, 1, fp32=fp32, ctx=S.ctx)
             same = bitwise_equal(Y, ref)
-            print("fp32=%s variant %d rpb %d on %d passes: bitwise %s" % (fp32, v, rpb, took, same), flush=True)
+            print("fp32=%s variant %d rpb %d on %d passes: bitwise %s (rel diff %.2e)" % (fp32, v, rpb, took, same,
+                                                                                      rel(Y, ref)), flush=True)
             assert same
     clear_all(S)
 
