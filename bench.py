@@ -133,6 +133,9 @@ def main():
     ap.add_argument("--plain-freqs", type=int, default=2)
     ap.add_argument("--inner", type=int, default=400, help="maxit of the plain path's gmres_custom")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the GMRES matvec kernels one by one instead of replaying a HIP graph "
+                         "(for kernel-tracing profilers)")
     ap.add_argument("--cpu-matvecs", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -155,11 +158,12 @@ def main():
     t0 = time.perf_counter()
     solver = sweep.Solver(mesh, device_lu=True)
     t_setup = time.perf_counter() - t0
+    peec.set_graphs(not args.no_graphs, ctx=solver.ctx)
     log("setup (integrals, spectra, operator upload, LU analysis): %.2f s" % t_setup)
     mixed = args.path == "mixed"
     t0 = time.perf_counter()
     if not args.no_tune:
-        solver.tune(args.batch * P, fp64=True, fp32=mixed)
+        solver.tune(args.batch * P, fp64=True, fp32=mixed, progress=True)
     t_tune = time.perf_counter() - t0
     log("FFT autotune for %d right-hand sides: %.2f s" % (args.batch * P, t_tune))
 
@@ -271,7 +275,7 @@ def main():
                    "z_err_vs_fp64": err_nw, "z_err_port_self": err_diag,
                    "plain_fp64_solves_per_s": plain_value, "speedup_vs_plain_fp64": value / plain_value,
                    "plain_matvecs_per_port": mv_per_port,
-                   "setup_s": round(t_setup, 3), "tune_s": round(t_tune, 3),
+                   "setup_s": round(t_setup, 3), "tune_s": round(t_tune, 3), "hip_graphs": not args.no_graphs,
                    "batched_matvec_kernel_ms": round(mv_ms, 3), "fft_matvec_ms": round(fft_ms, 3),
                    "fft_matvec_GBs": round(fft_bytes / fft_ms / 1e6, 1)},
         "roofline": {"bound": "hbm", "kernel": dom["name"], "symbol": KERNEL_SYMBOLS[dom["name"]],

@@ -217,7 +217,7 @@ int peec_precond_rhs_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const do
  * made with FFTW's defaults, compute_Matrix_vector_fft.jl:20-24).  For every
  * convolution kernel of the current operator and each precision in
  * `precisions` (bit 0: complex128, bit 1: complex64; bit 2 prints every
- * timing to stderr), times the five passes
+ * timing to stderr, bit 3 one line per tuned kernel), times the five passes
  * of one convolution over nrhs right-hand sides under each launch shape of
  * the library's table (lines per tile, lanes per line), then the
  * right-hand sides per block of the fused z pass, then (complex64 only) the

@@ -1050,7 +1050,7 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
         peec_set_error("peec_fft_tune: invalid argument");
         return PEEC_ERR_ARG;
     }
-    const bool verbose = (precisions & 4) != 0;
+    const bool verbose = (precisions & 4) != 0, progress = (precisions & 12) != 0;
     if (!ctx->op_ready) {
         peec_set_error("peec_fft_tune: call peec_operator_set first");
         return PEEC_ERR_STATE;
@@ -1156,6 +1156,8 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
                 ctx->fft_tune[FftTuneKey{app.kernel, p, fp32 ? 1 : 0, nrhs}] = best;
                 total_best += tbest;
             }
+            if (progress && rc == PEEC_OK)
+                fprintf(stderr, "peec_fft_tune: kernel %d %s tuned\n", app.kernel, fp32 ? "c64" : "c128");
         }
     }
     (void)hipEventDestroy(ea);

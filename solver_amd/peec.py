@@ -266,12 +266,12 @@ def set_graphs(on, ctx=None):
     L.check(L.lib().peec_set_graphs(ctx.handle, int(bool(on))), "peec_set_graphs")
 
 
-def fft_tune(nrhs, fp64=True, fp32=True, verbose=False, ctx=None):
+def fft_tune(nrhs, fp64=True, fp32=True, verbose=False, progress=False, ctx=None):
     """Autotune the FFT-convolution passes for batched calls with nrhs
     right-hand sides; returns the summed best pass time in ms."""
     ctx = ctx or default_context()
     ms = ctypes.c_double(0.0)
-    flags = int(fp64) | (int(fp32) << 1) | (int(verbose) << 2)
+    flags = int(fp64) | (int(fp32) << 1) | (int(verbose) << 2) | (int(progress) << 3)
     L.check(L.lib().peec_fft_tune(ctx.handle, int(nrhs), flags, ctypes.byref(ms)),
             "peec_fft_tune")
     return ms.value

@@ -198,12 +198,13 @@ class Solver:
             peec.lu_analyze(self.dissection, ctx=self.ctx)
         self.tuned = {}
 
-    def tune(self, nrhs, fp64=True, fp32=True, verbose=False):
+    def tune(self, nrhs, fp64=True, fp32=True, verbose=False, progress=False):
         """Autotune the FFT passes once for batched calls with nrhs
         right-hand sides (peec_fft_tune); returns the summed best pass ms."""
         key = (nrhs, fp64, fp32)
         if key not in self.tuned:
-            self.tuned[key] = peec.fft_tune(nrhs, fp64=fp64, fp32=fp32, verbose=verbose, ctx=self.ctx)
+            self.tuned[key] = peec.fft_tune(nrhs, fp64=fp64, fp32=fp32, verbose=verbose, progress=progress,
+                                            ctx=self.ctx)
         return self.tuned[key]
 
     def set_frequency(self, w, R0=50.0, sigma=None):
