@@ -219,7 +219,8 @@ int peec_precond_rhs_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const do
  * `precisions` (bit 0: complex128, bit 1: complex64; bit 2 prints every
  * timing to stderr, bit 3 one line per tuned kernel), times the five passes
  * of one convolution over nrhs right-hand sides under each launch shape of
- * the library's table (lines per tile, lanes per line), then the
+ * the library's table (lines per tile, lanes per line; for the fused z pass
+ * also variant 11, the kernel that holds a whole z line in registers), then the
  * right-hand sides per block of the fused z pass, then (complex64 only) the
  * radix schedule, and keeps the fastest for batched calls with exactly
  * nrhs right-hand sides.  Launch shapes do not change any result; a

@@ -25,6 +25,7 @@
 #include <cstdio>
 
 // ------------------------------------------------------------------ plans
+int fft_z_length(int n);
 int fft_smooth_length(int nmin)
 {
     if (nmin <= 1) return 1;
@@ -507,6 +508,107 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, 
     }
 }
 
+// ---- z convolution with the whole line in registers
+//
+// For short z lines one thread owns one line: it loads the nin compact
+// rows (strided by Lx Ly, coalesced across the threads' consecutive x),
+// runs the forward FFT, the spectrum product and the inverse FFT entirely
+// in registers (compile-time length: every index below is a constant after
+// unrolling, so the arrays live in VGPRs), and stores the nout rows.  No
+// LDS and no barriers; the block's spectrum stays in registers for its rpb
+// right-hand sides.  Same Stockham stages and twiddles as the LDS passes.
+template <int L, int S>
+struct RegPlan {
+    // radix of stage S (8 first, then 4, 2, 3, 5), and the stages' product
+    static constexpr int rem(int l, int s)
+    {
+        return s == 0 ? l : rem(l / (l % 8 == 0 ? 8 : l % 4 == 0 ? 4 : l % 2 == 0 ? 2 : l % 3 == 0 ? 3 : 5), s - 1);
+    }
+    static constexpr int r = rem(L, S) == 1 ? 1
+                             : rem(L, S) % 8 == 0 ? 8
+                             : rem(L, S) % 4 == 0 ? 4
+                             : rem(L, S) % 2 == 0 ? 2
+                             : rem(L, S) % 3 == 0 ? 3 : 5;
+    static constexpr int ns = L / rem(L, S);  // product of the radices of stages 0 .. S-1
+};
+
+template <typename T, int DIR, int L, int S>
+__device__ __forceinline__ void reg_stages(C2<T> (&a)[L], const double2_t *__restrict__ tw)
+{
+    constexpr int R = RegPlan<L, S>::r;
+    if constexpr (R > 1) {
+        constexpr int Ns = RegPlan<L, S>::ns;
+        constexpr int nb = L / R;
+        constexpr int stride = L / (Ns * R);
+        C2<T> b[L];
+#pragma unroll
+        for (int k = 0; k < nb; k++) {
+            const int kk = k % Ns;
+            C2<T> v[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = a[k + r * nb];
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    const double2_t w = tw[kk * r * stride];
+                    v[r] = cmul(v[r], C2<T>{(T)w.x, (T)(DIR < 0 ? w.y : -w.y)});
+                }
+            }
+            dftR<T, DIR, R>(v);
+#pragma unroll
+            for (int r = 0; r < R; r++) b[(k - kk) * R + kk + r * Ns] = v[r];
+        }
+#pragma unroll
+        for (int j = 0; j < L; j++) a[j] = b[j];
+        reg_stages<T, DIR, L, S + 1>(a, tw);
+    }
+}
+
+// MODE 2: real spectrum, MODE 3: complex spectrum (conjugated with spec_conj).
+// Lines: x = blockIdx.x * 256 + threadIdx.x < nb, y = blockIdx.y; right-hand
+// sides blockIdx.z * rpb ...; src and dst may alias (in place).
+template <typename T, int MODE, int L>
+__global__ void __launch_bounds__(256) k_zconv_reg(PassDesc d, const C2<T> *src, C2<T> *dst,
+                                                   const double2_t *__restrict__ twall)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= d.nb) return;  // no barrier in this kernel
+    const int y = blockIdx.y;
+    const double2_t *tw = twall + d.tw_off;
+    const int64_t sbase = (int64_t)x * d.spec_sb + (int64_t)y * d.spec_s2;
+    T sr[L], si[MODE == 3 ? L : 1];
+#pragma unroll
+    for (int j = 0; j < L; j++) {
+        if (MODE == 2) {
+            sr[j] = reinterpret_cast<const T *>(d.spec)[sbase + (int64_t)j * d.spec_sj];
+        } else {
+            const C2<T> c = reinterpret_cast<const C2<T> *>(d.spec)[sbase + (int64_t)j * d.spec_sj];
+            sr[j] = c.x;
+            si[MODE == 3 ? j : 0] = d.spec_conj ? -c.y : c.y;
+        }
+    }
+    const int64_t ib = (int64_t)x * d.in_sb + (int64_t)y * d.in_s2;
+    const int64_t ob = (int64_t)x * d.out_sb + (int64_t)y * d.out_s2;
+    const int r_end = min(d.nrhs, ((int)blockIdx.z + 1) * d.rpb);
+    for (int rhs = (int)blockIdx.z * d.rpb; rhs < r_end; rhs++) {
+        C2<T> a[L];
+        const C2<T> *in = src + ib + (int64_t)rhs * d.in_rhs;
+#pragma unroll
+        for (int j = 0; j < L; j++) a[j] = j < d.nin ? in[(int64_t)j * d.in_sj] : C2<T>{(T)0, (T)0};
+        reg_stages<T, -1, L, 0>(a, tw);
+#pragma unroll
+        for (int j = 0; j < L; j++) {
+            if (MODE == 2) a[j] = cscale(a[j], sr[j]);
+            else a[j] = cmul(a[j], C2<T>{sr[j], si[MODE == 3 ? j : 0]});
+        }
+        reg_stages<T, 1, L, 0>(a, tw);
+        C2<T> *out = dst + ob + (int64_t)rhs * d.out_rhs;
+#pragma unroll
+        for (int j = 0; j < L; j++)
+            if (j < d.nout) out[(int64_t)j * d.out_sj] = a[j];
+    }
+}
+
 // Even-symmetric embedding of a compact row into an L grid
 // (store_circulant_fft, compute_FFT_mutual_coupling_mats.jl:130-155, at
 // lengths L >= 2n - 1 instead of 2n).
@@ -673,6 +775,50 @@ static int launch_pass_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst,
 }
 #undef PEEC_FFT_CASE
 
+// z lengths the register kernel is built for (complex64; complex128 up to
+// 32), and the z embedding length of a compact extent n: the smallest of
+// them >= 2n - 1 (any length >= 2n - 1 embeds the convolution), else the
+// smallest 2^a 3^b 5^c.
+static const int kRegLengths[] = {8, 16, 32, 48, 64};
+static bool reg_length(int L, bool fp32)
+{
+    for (int l : kRegLengths)
+        if (l == L) return fp32 || L <= 32;
+    return false;
+}
+int fft_z_length(int n)
+{
+    for (int l : kRegLengths)
+        if (l >= 2 * n - 1) return l;
+    return fft_smooth_length(2 * n - 1);
+}
+
+template <typename T, int MODE>
+static int launch_zreg_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst)
+{
+    const int rpb = std::max(1, d.rpb);
+    dim3 grid((unsigned)((d.nb + 255) / 256), (unsigned)d.n2, (unsigned)((d.nrhs + rpb - 1) / rpb));
+    const C2<T> *s = (const C2<T> *)src;
+    C2<T> *o = (C2<T> *)dst;
+    const double2_t *tw = ctx->twiddles.p;
+    switch (d.L) {
+    case 8: hipLaunchKernelGGL((k_zconv_reg<T, MODE, 8>), grid, dim3(256), 0, ctx->stream, d, s, o, tw); break;
+    case 16: hipLaunchKernelGGL((k_zconv_reg<T, MODE, 16>), grid, dim3(256), 0, ctx->stream, d, s, o, tw); break;
+    case 32: hipLaunchKernelGGL((k_zconv_reg<T, MODE, 32>), grid, dim3(256), 0, ctx->stream, d, s, o, tw); break;
+    case 48:
+        if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_zconv_reg<T, MODE, 48>), grid, dim3(256), 0, ctx->stream, d, s, o, tw);
+        break;
+    case 64:
+        if constexpr (sizeof(T) == 4) hipLaunchKernelGGL((k_zconv_reg<T, MODE, 64>), grid, dim3(256), 0, ctx->stream, d, s, o, tw);
+        break;
+    default:
+        peec_set_error("register z pass: no kernel for length %d", d.L);
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipGetLastError());
+    return PEEC_OK;
+}
+
 // Radix schedules: 0 = radix 8 first (fewest stages), 1 = radix 4 first,
 // 2 = radix 3 and 5 first.
 static bool make_plan_style(int L, int style, FftPlan1D *p)
@@ -736,9 +882,23 @@ static int make_desc(peec_ctx *ctx, const PassGeom &g, PassDesc *d, int style = 
     return PEEC_OK;
 }
 
+static int RegFirstRadix(int L) { return L % 8 == 0 ? 8 : L % 4 == 0 ? 4 : L % 2 == 0 ? 2 : L % 3 == 0 ? 3 : 5; }
+
+// Launch shape kRegVariant: the register z kernel (fused z passes only).
+static constexpr int kRegVariant = kNumFftVariants;
+
 static int launch_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const void *src, void *dst, int v = -1)
 {
     if (d.nb <= 0 || d.n2 <= 0 || d.nrhs <= 0) return PEEC_OK;
+    if (v == kRegVariant) {
+        // radix-8-first schedule only (the plan the kernel unrolls)
+        if (mode < 2 || !reg_length(d.L, fp32) || d.nstage < 1 || d.radix[0] != RegFirstRadix(d.L)) {
+            peec_set_error("register z pass does not apply (mode %d, length %d)", mode, d.L);
+            return PEEC_ERR_ARG;
+        }
+        if (fp32) return mode == 2 ? launch_zreg_t<float, 2>(ctx, d, src, dst) : launch_zreg_t<float, 3>(ctx, d, src, dst);
+        return mode == 2 ? launch_zreg_t<double, 2>(ctx, d, src, dst) : launch_zreg_t<double, 3>(ctx, d, src, dst);
+    }
     if (v < 0) v = default_variant(d.L, fp32);
     if (v < 0 || !variant_fits(v, d.L, fp32)) {
         peec_set_error("no FFT launch shape for length %d", d.L);
@@ -809,7 +969,7 @@ int poly_spectrum_build(peec_ctx *ctx, int k, const int delta[3])
     Kernel &K = ctx->kern[k];
     for (int a = 0; a < 3; a++) {
         K.pn[a] = kHalfAxes[k][a] ? (K.n[a] + 1) / 2 : K.n[a];
-        K.pL[a] = fft_smooth_length(2 * K.pn[a] - 1);
+        K.pL[a] = a == 2 ? fft_z_length(K.pn[a]) : fft_smooth_length(2 * K.pn[a] - 1);
         if (ctx->twiddle_offset(K.pL[a]) < 0) {
             peec_set_error("operator_set: twiddle table allocation failed");
             return PEEC_ERR_NOMEM;
@@ -845,7 +1005,8 @@ int peec_kernels_build_impl(peec_ctx *ctx, const double origin[3], int Nx, int N
     for (int k = 0; k < 9; k++) {
         Kernel &K = ctx->kern[k];
         kernel_dims(k, Nx, Ny, Nz, K.n);
-        for (int a = 0; a < 3; a++) K.L[a] = K.n[a] > 0 ? fft_smooth_length(2 * K.n[a] - 1) : 0;
+        for (int a = 0; a < 3; a++)
+            K.L[a] = K.n[a] > 0 ? (a == 2 ? fft_z_length(K.n[a]) : fft_smooth_length(2 * K.n[a] - 1)) : 0;
         if (K.compact() > 0) Gmax = std::max(Gmax, K.embedded());
     }
     // register every FFT length before the first launch: the twiddle table
@@ -1018,8 +1179,9 @@ int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *ve
     if (rc) return rc;
     for (int p = 0; p < 5; p++) {
         PassDesc &d = cp.d[p];
-        const int v = ch[p].variant >= 0 ? ch[p].variant : default_variant(d.L, fp32);
-        d.rpb = ch[p].variant >= 0 ? ch[p].rpb : default_rpb(d, std::max(v, 0));
+        const bool reg_ok = p == 2 && reg_length(d.L, fp32) && d.radix[0] == RegFirstRadix(d.L);
+        const int v = ch[p].variant >= 0 ? ch[p].variant : (reg_ok ? kRegVariant : default_variant(d.L, fp32));
+        d.rpb = ch[p].variant >= 0 ? ch[p].rpb : default_rpb(d, v == kRegVariant ? 0 : std::max(v, 0));
         const int pi = ctx->prof_begin(kProfKind[p], cp.bytes[p]);
         if ((rc = launch_pass(ctx, fp32, pass_mode(app, p), d, cp.src[p], cp.dst[p], v))) return rc;
         ctx->prof_end(pi);
@@ -1112,6 +1274,21 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
                         best.style = 0;
                     }
                 }
+                if (rc == PEEC_OK && mode >= 2 && reg_length(d.L, fp32) && d.radix[0] == RegFirstRadix(d.L)) {
+                    d.rpb = default_rpb(d, 0);
+                    float t;
+                    if ((rc = time_pass(ctx, fp32, mode, d, cp.src[p], cp.dst[p], kRegVariant, 3, ea, eb, &t))) break;
+                    if (verbose)
+                        fprintf(stderr, "peec_fft_tune: kernel %d pass %d %s L %d: register z kernel rpb %d: %.1f us, "
+                                        "%.0f GB/s\n", app.kernel, p, fp32 ? "c64" : "c128", d.L, d.rpb, 1e3 * t,
+                                cp.bytes[p] / (1e6 * t));
+                    if (t < tbest) {
+                        tbest = t;
+                        best.variant = kRegVariant;
+                        best.rpb = d.rpb;
+                        best.style = 0;
+                    }
+                }
                 if (rc || best.variant < 0) break;
                 // 2) right-hand sides per block
                 {
@@ -1130,8 +1307,9 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
                     }
                 }
                 // 3) radix schedule (FP32 passes only: the FP64 path keeps
-                // one fixed operation order)
-                if (fp32) {
+                // one fixed operation order; the register kernel unrolls
+                // the radix-8-first schedule)
+                if (fp32 && best.variant != kRegVariant) {
                     for (int st = 1; st <= 2 && rc == PEEC_OK; st++) {
                         PassDesc ds;
                         PassGeom gg = {d.L, d.nin, d.nout, d.nb, d.n2, d.in_sb, d.in_s2, d.in_sj,
@@ -1172,7 +1350,8 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
                                    int style)
 {
     if (!ctx || kernel < 0 || kernel > 8 || pass < 0 || pass > 4 || nrhs < 1 || variant < -1 ||
-        variant >= kNumFftVariants || rpb < 1 || style < 0 || style > 2 || (style != 0 && !fp32)) {
+        variant > kRegVariant || rpb < 1 || style < 0 || style > 2 || (style != 0 && !fp32) ||
+        (variant == kRegVariant && (pass != 2 || style != 0))) {
         peec_set_error("peec_fft_set_choice: invalid argument");
         return PEEC_ERR_ARG;
     }
@@ -1185,7 +1364,8 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
     for (int a = 0; a < ctx->napps && !app; a++)
         if (ctx->apps[a].kernel == kernel) app = &ctx->apps[a];
     const int L = app ? app->L[pass == 0 || pass == 4 ? 0 : (pass == 2 ? 2 : 1)] : 0;
-    if (!app || !variant_fits(variant, L, fp32 != 0)) {
+    if (!app || (variant == kRegVariant ? !(reg_length(L, fp32 != 0) && L % RegFirstRadix(L) == 0)
+                                        : !variant_fits(variant, L, fp32 != 0))) {
         peec_set_error("peec_fft_set_choice: launch variant %d does not fit length %d", variant, L);
         return PEEC_ERR_ARG;
     }
@@ -1202,9 +1382,10 @@ extern "C" int peec_fft_choice(peec_ctx *ctx, int kernel, int pass, int fp32, in
         return PEEC_ERR_ARG;
     }
     const FftChoice c = tuned_choice(ctx, kernel, pass, fp32 != 0, nrhs);
+    const bool table = c.variant >= 0 && c.variant < kNumFftVariants;
     out5[0] = c.variant;
-    out5[1] = c.variant >= 0 ? variant_lines(c.variant) : 0;
-    out5[2] = c.variant >= 0 ? kFftVariants[c.variant].tpl : 0;
+    out5[1] = table ? variant_lines(c.variant) : (c.variant == kRegVariant ? 256 : 0);
+    out5[2] = table ? kFftVariants[c.variant].tpl : (c.variant == kRegVariant ? 1 : 0);
     out5[3] = c.rpb;
     out5[4] = c.style;
     return PEEC_OK;

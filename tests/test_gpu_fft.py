@@ -19,7 +19,7 @@ from test_gpu_operator import pself, rel
 pytestmark = pytest.mark.gpu
 
 NRHS = 3
-N_VARIANTS = 11
+N_VARIANTS = 12  # 11 LDS tile shapes + the register z kernel (fused z pass only)
 
 
 @pytest.fixture(scope="module")
