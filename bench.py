@@ -119,6 +119,69 @@ def cpu_baseline(mesh, cfg, freq, matvecs_per_port, sample_matvecs):
                        % (cfg["name"], freq, t_lu, sample_matvecs, t_mv, P, matvecs_per_port, t_setup))
 
 
+def matvec_only(args):
+    """FFT-matvec roofline on one workload: kernel spectra and operator on
+    the device (no LU), then `--matvec-reps` batched K x with --nrhs
+    right-hand sides in complex64 and complex128 under the tuned launch
+    shapes, every launch timed with HIP events.  Prints one JSON line."""
+    from solver_amd import configs, mesher, peec, sweep
+    cfg = configs.CONFIGS[args.workload]()
+    t0 = time.perf_counter()
+    mesh = mesher.mesh(cfg["grid"], cfg["materials"], cfg["cell"], cfg["origin"], cfg["ports"], p_self_fn=gpu_p_self)
+    N = mesh.m + mesh.ns + mesh.n
+    log("mesh %s: n=%d m=%d ns=%d N=%d (%.1f s)" % (cfg["name"], mesh.n, mesh.m, mesh.ns, N, time.perf_counter() - t0))
+    ctx = peec.Context(0)
+    t0 = time.perf_counter()
+    peec.compute_FFT_mutual_coupling_mats(mesh.origin, mesh.Nx, mesh.Ny, mesh.Nz, mesh.sx, mesh.sy, mesh.sz,
+                                          mesh.escalings, ctx=ctx)
+    ctx_sync = time.perf_counter()
+    t_kern = ctx_sync - t0
+    peec.operator_set(mesh, ctx=ctx)
+    w = 2 * np.pi * cfg["freqs"][0] * mesh.escalings["freq"]
+    Yle = sweep.build_Yle_FFT(mesh.lumped_elements, mesh.port_nodes, mesh.escalings, mesh.n,
+                              w / mesh.escalings["freq"], 50.0)
+    peec.frequency_set(w, sweep.DZ_of(mesh, w), Yle, ctx=ctx)
+    t_setup = time.perf_counter() - t0
+    log("integrals + spectra %.2f s, setup %.2f s" % (t_kern, t_setup))
+    nrhs = args.nrhs
+    t0 = time.perf_counter()
+    peec.fft_tune(nrhs, progress=True, ctx=ctx)
+    t_tune = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(N, nrhs)) + 1j * rng.normal(size=(N, nrhs))
+    res = {}
+    for fp32 in (True, False):
+        peec.apply_K_batched(X, nrhs, fp32=fp32, ctx=ctx)
+        profile_read(ctx, True)
+        for _ in range(args.matvec_reps):
+            peec.apply_K_batched(X, nrhs, fp32=fp32, ctx=ctx)
+        prof = [r for r in profile_read(ctx, False) if r["name"].startswith("fft")]
+        tot_ms = sum(r["ms"] for r in prof) / args.matvec_reps
+        tot_b = sum(r["bytes"] for r in prof) / args.matvec_reps
+        kinds = {}
+        for r in prof:
+            kinds[r["name"]] = dict(avg_us=round(1e3 * r["ms"] / max(r["count"], 1), 2),
+                                    GBs=round(r["bytes"] / max(r["ms"], 1e-12) / 1e6, 1),
+                                    bytes_per_launch=r["bytes"] / max(r["count"], 1))
+        key = "c64" if fp32 else "c128"
+        res[key] = dict(fft_ms_per_matvec=round(tot_ms, 3), fft_GB_per_matvec=round(tot_b / 1e9, 3),
+                        fft_GBs=round(tot_b / tot_ms / 1e6, 1), kinds=kinds)
+        log("%s: FFT part of one %d-rhs matvec %.2f ms, %.2f GB algorithmic, %.0f GB/s" % (
+            key, nrhs, tot_ms, tot_b / 1e9, tot_b / tot_ms / 1e6))
+    dom = max(res["c64"]["kinds"].items(), key=lambda kv: kv[1]["avg_us"])
+    out = {"metric": "FFT-matvec HBM GB/s (complex64, %d right-hand sides)" % nrhs,
+           "value": res["c64"]["fft_GBs"], "unit": "GB/s", "n_gpus": 1, "higher_is_better": True,
+           "data": "synthetic (seeded geometry, %s)" % cfg["name"],
+           "config": {"workload": cfg["name"], "grid": list(cfg["grid"].shape), "unknowns": int(N), "nrhs": nrhs,
+                      "setup_s": round(t_setup, 2), "integrals_spectra_s": round(t_kern, 2),
+                      "tune_s": round(t_tune, 2)},
+           "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": dom[1]["GBs"], "peak": PEAK_HBM_GBS,
+                        "unit": "GB/s", "frac": dom[1]["GBs"] / PEAK_HBM_GBS, "avg_launch_us": dom[1]["avg_us"],
+                        "bytes_per_launch": dom[1]["bytes_per_launch"]},
+           "fft": res}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,8 +192,8 @@ def main():
                     help="batched sweep with complex64 (mixed) or complex128 FFT convolutions")
     ap.add_argument("--batch", type=int, default=8, help="frequency points per batch (step)")
     ap.add_argument("--tol", type=float, default=1e-9, help="GMRES relative residual (both paths)")
-    ap.add_argument("--restart", type=int, default=60)
-    ap.add_argument("--plain-freqs", type=int, default=2)
+    ap.add_argument("--restart", type=int, default=100)
+    ap.add_argument("--plain-freqs", type=int, default=2, help="0 skips the plain path (profiling runs)")
     ap.add_argument("--inner", type=int, default=400, help="maxit of the plain path's gmres_custom")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
@@ -141,11 +204,20 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB device to device (a known-size "
                          "reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--matvec-only", action="store_true",
+                    help="measure the FFT matvec alone on --workload (e.g. large_192x192x48) with --nrhs "
+                         "right-hand sides")
+    ap.add_argument("--nrhs", type=int, default=4)
+    ap.add_argument("--matvec-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X (the frequency sweep would shard "
               "across GPUs by frequency, one rank per GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.matvec_only:
+        matvec_only(args)
+        copy_reference(args.copy_mib)
+        return
 
     from solver_amd import configs, mesher, peec, sweep
     cfg = configs.CONFIGS[args.workload]()
@@ -214,22 +286,25 @@ def main():
                       max(iters), flags))
 
     # plain FP64 drop-in path at a few of the timed frequencies
-    pidx = np.unique(np.linspace(0, nfreq - 1, args.plain_freqs).round().astype(int))
-    fpl = ftimed[pidx]
-    prec = []
-    t0 = time.perf_counter()
-    outp = sweep.FFT_solver_QS_S_type(fpl, mesh, args.inner, args.tol, solver=solver, record=prec)
-    dt_plain = time.perf_counter() - t0
-    plain_value = len(fpl) / dt_plain
-    mv_per_port = float(np.mean([r["nres"] + 1 for r in prec]))
-    log("plain FP64: %d frequency points in %.2f s -> %.3f solves/s (%.1f matvecs per port)"
-        % (len(fpl), dt_plain, plain_value, mv_per_port))
-    # reference impedances: the plain FP64 path to a 1e-12 residual
-    outr = sweep.FFT_solver_QS_S_type(fpl, mesh, args.inner, 1e-12, solver=solver)
-    err_nw, err_diag = z_error(Zt[:, :, pidx], outr["Z"])
-    err_plain_nw, _ = z_error(outp["Z"], outr["Z"])
-    log("impedance error vs FP64 at 1e-12: mixed batched %.2e (norm-wise), %.2e (port self-impedances); "
-        "plain at tol %.0e: %.2e" % (err_nw, err_diag, args.tol, err_plain_nw))
+    plain_value = mv_per_port = err_nw = err_diag = None
+    fpl = ftimed[:1]
+    if args.plain_freqs > 0:
+        pidx = np.unique(np.linspace(0, nfreq - 1, args.plain_freqs).round().astype(int))
+        fpl = ftimed[pidx]
+        prec = []
+        t0 = time.perf_counter()
+        outp = sweep.FFT_solver_QS_S_type(fpl, mesh, args.inner, args.tol, solver=solver, record=prec)
+        dt_plain = time.perf_counter() - t0
+        plain_value = len(fpl) / dt_plain
+        mv_per_port = float(np.mean([r["nres"] + 1 for r in prec]))
+        log("plain FP64: %d frequency points in %.2f s -> %.3f solves/s (%.1f matvecs per port)"
+            % (len(fpl), dt_plain, plain_value, mv_per_port))
+        # reference impedances: the plain FP64 path to a 1e-12 residual
+        outr = sweep.FFT_solver_QS_S_type(fpl, mesh, args.inner, 1e-12, solver=solver)
+        err_nw, err_diag = z_error(Zt[:, :, pidx], outr["Z"])
+        err_plain_nw, _ = z_error(outp["Z"], outr["Z"])
+        log("impedance error vs FP64 at 1e-12: mixed batched %.2e (norm-wise), %.2e (port self-impedances); "
+            "plain at tol %.0e: %.2e" % (err_nw, err_diag, args.tol, err_plain_nw))
 
     # profiled batched matvec at the last batch's frequency slots
     last = batches[-1]
@@ -253,7 +328,7 @@ def main():
     fft_bytes = sum(r["bytes"] for r in prof if r["name"].startswith("fft")) / nprof
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and mv_per_port is not None:
         cpu = cpu_baseline(mesh, cfg, float(fpl[0]), mv_per_port, args.cpu_matvecs)
         log("cpu baseline: %s" % cpu)
 
@@ -280,7 +355,8 @@ def main():
                    "inner_iters_mean_per_batch": needed / len(per_batch),
                    "phase_s": {k: round(v, 3) for k, v in tm.items()},
                    "z_err_vs_fp64": err_nw, "z_err_port_self": err_diag,
-                   "plain_fp64_solves_per_s": plain_value, "speedup_vs_plain_fp64": value / plain_value,
+                   "plain_fp64_solves_per_s": plain_value,
+                   "speedup_vs_plain_fp64": value / plain_value if plain_value else None,
                    "plain_matvecs_per_port": mv_per_port,
                    "setup_s": round(t_setup, 3), "tune_s": round(t_tune, 3), "hip_graphs": not args.no_graphs,
                    "batched_matvec_kernel_ms": round(mv_ms, 3), "fft_matvec_ms": round(fft_ms, 3),
@@ -294,16 +370,22 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    if args.copy_mib > 0:
-        # an elementwise kernel (not a DMA copy) that reads and writes
-        # exactly copy_mib MiB each, to scale the HBM traffic counters
-        import torch
-        a = torch.ones((args.copy_mib << 20) // 4, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            torch.mul(a, 2.0, out=b)
-        torch.cuda.synchronize()
-        log("scaled %d MiB device to device 3 times (elementwise kernel)" % args.copy_mib)
+    copy_reference(args.copy_mib)
+
+
+def copy_reference(mib):
+    """An elementwise kernel (not a DMA copy) that reads and writes exactly
+    mib MiB, three times: the known-size dispatch that scales the HBM
+    traffic counters."""
+    if mib <= 0:
+        return
+    import torch
+    a = torch.ones((mib << 20) // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    log("scaled %d MiB device to device 3 times (elementwise kernel)" % mib)
 
 
 if __name__ == "__main__":

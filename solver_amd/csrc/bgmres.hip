@@ -28,10 +28,6 @@
 
 typedef std::complex<double> zc;
 
-int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps);
-int matvec_device_adj(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps);
-int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, int rps);
-int apply_precond_adj(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, int rps);
 
 namespace {
 
@@ -148,6 +144,22 @@ __global__ void k_bcopy(cd *dst, int64_t dld, const cd *src, int64_t sld, int64_
     if (i < n) dst[(int64_t)r * dld + i] = src[(int64_t)r * sld + i];
 }
 
+// dst[:, i] = src[:, col[i]] (stride sld) for the listed columns
+__global__ void k_bgather(cd *dst, int64_t dld, const cd *src, int64_t sld, const int32_t *col, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i < n) dst[(int64_t)r * dld + i] = src[(int64_t)col[r] * sld + i];
+}
+
+// dst[:, col[i]] = src[:, i]
+__global__ void k_bscatter(cd *dst, int64_t dld, const cd *src, int64_t sld, const int32_t *col, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i < n) dst[(int64_t)col[r] * dld + i] = src[(int64_t)r * sld + i];
+}
+
 // r = b - y
 __global__ void k_bsub(cd *r, const cd *b, const cd *y, int64_t total)
 {
@@ -165,13 +177,15 @@ struct BWork {
     peec_ctx *ctx;
     int nrhs, rps, m;
     int64_t N;
-    DevBuf<cd> V, B, X, R, W, T, hbuf, part;
+    DevBuf<cd> V, B, X, R, W, T, Wc, hbuf, part;
     DevBuf<double> nrm, npart;
+    DevBuf<int32_t> cmap;  // compacted columns: col[nrhs], rslot[nrhs], slot_ptr[nslots + 1]
     int alloc()
     {
         const size_t nr = (size_t)nrhs, n = (size_t)N;
         if (!V.alloc(nr * (size_t)(m + 1) * n) || !B.alloc(nr * n) || !X.alloc(nr * n) || !R.alloc(nr * n) ||
-            !W.alloc(nr * n) || !T.alloc(nr * n) || !hbuf.alloc(nr * 2 * (size_t)(m + 1)) ||
+            !W.alloc(nr * n) || !T.alloc(nr * n) || !Wc.alloc(nr * n) ||
+            !cmap.alloc(2 * nr + (size_t)ctx->nslots + 1) || !hbuf.alloc(nr * 2 * (size_t)(m + 1)) ||
             !part.alloc(nr * kGx * 4 * (size_t)(m + 1)) || !nrm.alloc(nr) || !npart.alloc(nr * kGx * 4)) {
             peec_set_error("peec_gmres_batched: out of device memory for %d right-hand sides x %d basis vectors", nrhs,
                            m + 1);
@@ -207,7 +221,8 @@ struct BWork {
     // M^-1 K, or M^-H K^H for the adjoint solve; replayed from HIP graphs
     bool adjoint = false;
     MatvecGraphs graphs;
-    int matvec(const cd *x, bool fp32)
+    std::vector<int32_t> cols;  // the compacted columns now uploaded
+    void graphs_init()
     {
         if (!graphs.ctx) {
             graphs.ctx = ctx;
@@ -216,7 +231,52 @@ struct BWork {
             graphs.ld = N;
             graphs.adjoint = adjoint;
         }
+    }
+    int matvec(const cd *x, bool fp32)
+    {
+        graphs_init();
         return graphs.run(x, W.p, fp32);
+    }
+    // W[:, c] = A x[:, c] for the listed columns c only (x with stride
+    // xld): the columns are gathered into T, multiplied as one smaller
+    // batch with an explicit column -> slot map, and scattered back; W of
+    // the other columns is left as it was.  Uses every column when most are
+    // listed.
+    int matvec_cols(const cd *x, int64_t xld, const std::vector<int32_t> &c, bool fp32)
+    {
+        const int nc = (int)c.size();
+        if (nc == 0) return PEEC_OK;
+        if (nc * 8 > nrhs * 7) {  // most columns: no compaction
+            if (x != T.p || xld != N)
+                hipLaunchKernelGGL(k_bcopy, dim3(nb256(N), nrhs), dim3(256), 0, ctx->stream, T.p, N, x, xld, N);
+            return matvec(T.p, fp32);
+        }
+        graphs_init();
+        const int nsl = ctx->nslots;
+        if (c != cols) {
+            std::vector<int32_t> h((size_t)(2 * nrhs + nsl + 1), 0);
+            for (int i = 0; i < nc; i++) {
+                h[(size_t)i] = c[(size_t)i];
+                h[(size_t)(nrhs + i)] = c[(size_t)i] / rps;
+            }
+            // columns grouped by slot (c is increasing): slot_ptr by counting
+            for (int i = 0; i < nc; i++) h[(size_t)(2 * nrhs + c[(size_t)i] / rps + 1)]++;
+            for (int sl = 0; sl < nsl; sl++) h[(size_t)(2 * nrhs + sl + 1)] += h[(size_t)(2 * nrhs + sl)];
+            PEEC_HIP_TRY(hipMemcpyAsync(cmap.p, h.data(), h.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+            cols = c;
+        }
+        ColMap cm;
+        cm.rslot = cmap.p + nrhs;
+        cm.slot_ptr = cmap.p + 2 * nrhs;
+        cm.rps = rps;
+        cm.nsl = nsl;
+        hipLaunchKernelGGL(k_bgather, dim3(nb256(N), nc), dim3(256), 0, ctx->stream, T.p, N, x, xld, cmap.p, N);
+        PEEC_HIP_TRY(hipGetLastError());
+        int rc = graphs.run_cols(T.p, Wc.p, fp32, nc, cm);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_bscatter, dim3(nb256(N), nc), dim3(256), 0, ctx->stream, W.p, N, Wc.p, N, cmap.p, N);
+        PEEC_HIP_TRY(hipGetLastError());
+        return PEEC_OK;
     }
 };
 
@@ -275,15 +335,25 @@ static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per
     PEEC_HIP_TRY(hipMemcpyAsync(W.X.p, x, bytes, hipMemcpyHostToDevice, ctx->stream));
     std::vector<double> nb, beta;
     if ((rc = W.norms(W.B.p, N, nb))) return rc;
-    // r = b - A x
-    auto residual = [&]() -> int {
-        int e = W.matvec(W.X.p, false);
+    // r = b - A x for the listed columns (the others keep r and beta)
+    auto residual = [&](const std::vector<int32_t> &cl) -> int {
+        int e = W.matvec_cols(W.X.p, N, cl, false);
         if (e) return e;
         hipLaunchKernelGGL(k_bsub, dim3(nb256((int64_t)nrhs * N)), dim3(256), 0, ctx->stream, W.R.p, W.B.p, W.W.p,
                            (int64_t)nrhs * N);
-        return W.norms(W.R.p, N, beta);
+        const std::vector<double> old = beta;
+        if ((e = W.norms(W.R.p, N, beta))) return e;
+        if (!old.empty()) {
+            std::vector<char> in((size_t)nrhs, 0);
+            for (int32_t c : cl) in[(size_t)c] = 1;
+            for (int r = 0; r < nrhs; r++)
+                if (!in[(size_t)r]) beta[(size_t)r] = old[(size_t)r];
+        }
+        return PEEC_OK;
     };
-    if ((rc = residual())) return rc;
+    std::vector<int32_t> allc((size_t)nrhs);
+    for (int r = 0; r < nrhs; r++) allc[(size_t)r] = r;
+    if ((rc = residual(allc))) return rc;
     std::vector<char> active((size_t)nrhs);
     std::vector<int> it((size_t)nrhs, 0), cyc((size_t)nrhs, 0), fl((size_t)nrhs, 1), stall((size_t)nrhs, 0);
     for (int r = 0; r < nrhs; r++) {
@@ -318,11 +388,14 @@ static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per
             const double target = tol * nb[(size_t)r] / std::max(beta[(size_t)r], 1e-300);
             eta[(size_t)r] = mixed ? std::max(target, inner_tol > 0 ? inner_tol : 1e-5) : std::max(target, 1e-15);
         }
+        std::vector<int32_t> live;
         for (int j = 0; j < m; j++) {
-            // w = A_in v_j
-            hipLaunchKernelGGL(k_bcopy, dim3(nb256(N), nrhs), dim3(256), 0, ctx->stream, W.T.p, N,
-                               W.V.p + (int64_t)j * N, (int64_t)(m + 1) * N, N);
-            if ((rc = W.matvec(W.T.p, mixed != 0))) return rc;
+            // w = A_in v_j for the right-hand sides still iterating (the
+            // batch shrinks as they converge)
+            live.clear();
+            for (int r = 0; r < nrhs; r++)
+                if (inner[(size_t)r]) live.push_back(r);
+            if ((rc = W.matvec_cols(W.V.p + (int64_t)j * N, (int64_t)(m + 1) * N, live, mixed != 0))) return rc;
             // CGS2: two passes of h = V' w, w -= V h
             W.dot(j + 1, W.W.p, N, 0);
             W.gemv(W.W.p, N, j + 1, W.hbuf.p, hs, -1.0);
@@ -397,7 +470,10 @@ static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per
         if (kmax > 0) W.gemv(W.X.p, N, kmax, W.hbuf.p, hs, 1.0);
         PEEC_HIP_TRY(hipGetLastError());
         const std::vector<double> beta_old = beta;
-        if ((rc = residual())) return rc;
+        std::vector<int32_t> act;
+        for (int r = 0; r < nrhs; r++)
+            if (active[(size_t)r]) act.push_back(r);
+        if ((rc = residual(act))) return rc;
         for (int r = 0; r < nrhs; r++) {
             if (!active[(size_t)r]) continue;
             cyc[(size_t)r]++;
@@ -450,7 +526,7 @@ extern "C" int peec_precond_rhs_batched(peec_ctx *ctx, int nrhs, int rhs_per_slo
     DevBuf<cd> Y;
     if (!Y.alloc(h.size())) return PEEC_ERR_NOMEM;
     PEEC_HIP_TRY(hipMemcpyAsync(Y.p, h.data(), h.size() * 16, hipMemcpyHostToDevice, ctx->stream));
-    int rc = apply_precond(ctx, Y.p, nrhs, N, rhs_per_slot);
+    int rc = apply_precond(ctx, Y.p, nrhs, N, uniform_cols(nrhs, rhs_per_slot));
     if (rc) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(tn, Y.p, h.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -483,7 +559,7 @@ extern "C" int peec_precond_rhs_batched_adjoint(peec_ctx *ctx, int nrhs, int rhs
     DevBuf<cd> Y;
     if (!Y.alloc(h.size())) return PEEC_ERR_NOMEM;
     PEEC_HIP_TRY(hipMemcpyAsync(Y.p, h.data(), h.size() * 16, hipMemcpyHostToDevice, ctx->stream));
-    int rc = apply_precond_adj(ctx, Y.p, nrhs, N, rhs_per_slot);
+    int rc = apply_precond_adj(ctx, Y.p, nrhs, N, uniform_cols(nrhs, rhs_per_slot));
     if (rc) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(tn, Y.p, h.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));

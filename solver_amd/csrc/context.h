@@ -41,6 +41,31 @@ struct double2_t {
     double x, y;
 };
 
+// Which frequency slot each right-hand side (column) of a batched call
+// uses.  Uniform: column r -> slot r / rps.  Explicit (rslot non-null): the
+// columns are grouped by slot in increasing order, rslot[r] is the slot of
+// column r and the columns of slot s are slot_ptr[s] .. slot_ptr[s+1]-1
+// (device arrays) -- the compacted batches of GMRES.
+struct ColMap {
+    const int32_t *rslot = nullptr;
+    const int32_t *slot_ptr = nullptr;
+    int rps = 1;
+    int nsl = 1;  // slots spanned
+    __host__ __device__ int slot(int r) const { return rslot ? rslot[r] : r / rps; }
+    __host__ __device__ int first(int s) const { return slot_ptr ? slot_ptr[s] : s * rps; }
+    __host__ __device__ int end(int s, int nrhs) const
+    {
+        return slot_ptr ? slot_ptr[s + 1] : (s + 1) * rps < nrhs ? (s + 1) * rps : nrhs;
+    }
+};
+inline ColMap uniform_cols(int nrhs, int rps)
+{
+    ColMap c;
+    c.rps = rps;
+    c.nsl = (nrhs + rps - 1) / rps;
+    return c;
+}
+
 // One circulant kernel (3 Lp + 6 P): compact dims, embedded dims and its
 // real spectrum (already divided by Lx*Ly*Lz) in FP64 and FP32.
 struct Kernel {
@@ -292,7 +317,8 @@ int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std
 int tri_set_slots(TriFactor *f, int nslots);  // reallocates the value sets (contents undefined)
 std::vector<int32_t> tri_chain_blocks(const peec_csc_c128 *L, int cap);
 int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f, const char *what);
-int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs, int rps);
+int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs,
+               const ColMap &cm);
 
 // The matvec of one GMRES call (M^-1 K x, or M^-H K^H x) captured once per
 // (x, y, precision) into a HIP graph and replayed: nothing the matvec reads
@@ -303,7 +329,8 @@ struct MatvecGraphs {
     struct Entry {
         const void *x;
         void *y;
-        int fp32;
+        int fp32, nrhs;
+        const int32_t *rslot;
         hipGraphExec_t exec;
     };
     peec_ctx *ctx = nullptr;
@@ -313,4 +340,15 @@ struct MatvecGraphs {
     std::vector<Entry> entries;
     ~MatvecGraphs();
     int run(const double2_t *x, double2_t *y, bool fp32);
+    // the same with nrhs_c columns grouped by an explicit column map (the
+    // map's device arrays may change between calls; their pointers may not)
+    int run_cols(const double2_t *x, double2_t *y, bool fp32, int nrhs_c, const ColMap &cm);
 };
+
+int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, const ColMap &cm);
+int apply_KH(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, const ColMap &cm);
+int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, const ColMap &cm);
+int apply_precond_adj(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, const ColMap &cm);
+int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, const ColMap &cm);
+int matvec_device_adj(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32,
+                      const ColMap &cm);

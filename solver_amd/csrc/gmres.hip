@@ -19,7 +19,6 @@
 
 typedef std::complex<double> zc;
 
-int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps);
 int sync_stream(peec_ctx *ctx);
 
 namespace {

@@ -78,14 +78,14 @@ __global__ void k_zero(cd *y, int64_t n, int nrhs, int64_t ld)
 }
 
 // Y1 = j w Y1 + DZ .* I + A Phi           (compute_Matrix_vector_fft.jl:29-31)
-// Right-hand side r takes w and DZ of frequency slot r / rps.
+// Right-hand side r takes w and DZ of its frequency slot (cm).
 __global__ void k_y1(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const int32_t *ai, const double *av,
-                     const double *wv, int64_t m, int64_t phi0, int nrhs, int64_t ld, int rps)
+                     const double *wv, int64_t m, int64_t phi0, int nrhs, int64_t ld, ColMap cm)
 {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m) return;
     for (int r = 0; r < nrhs; r++) {
-        const int sl = r / rps;
+        const int sl = cm.slot(r);
         const cd *xr = x + r * ld;
         cd s = {0.0, 0.0};
         for (int k = ap[e]; k < ap[e + 1]; k++) s = cadd(s, cscal(xr[phi0 + ai[k]], av[k]));
@@ -112,13 +112,13 @@ __global__ void k_y2(cd *y, const cd *x, const int32_t *gp, const int32_t *gi, c
 // Y3 = -A' I + Yle Phi + j w Gamma Q      (:69-77)
 __global__ void k_y3(cd *y, const cd *x, const int32_t *atp, const int32_t *ati, const double *atv, const int32_t *yp0,
                      const int32_t *yi, const cd *yv, const int32_t *gp, const int32_t *gi, const double *gv,
-                     const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, int rps)
+                     const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, ColMap cm)
 {
     int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
     for (int r = 0; r < nrhs; r++) {
-        const int sl = r / rps;
+        const int sl = cm.slot(r);
         const int32_t *yp = yp0 + sl * (n + 1);
         const double w = wv[sl];
         const cd *xr = x + r * ld;
@@ -139,13 +139,13 @@ __global__ void k_y3(cd *y, const cd *x, const int32_t *atp, const int32_t *ati,
 __global__ void k_pre(const cd *y, const cd *invZ0, const double *invP, const int32_t *atp, const int32_t *ati,
                       const double *atv, const int32_t *gp, const int32_t *gi, const double *gv, const int32_t *pinv,
                       const double *Rs, cd *b, const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs,
-                      int64_t ld, int rps)
+                      int64_t ld, ColMap cm)
 {
     int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
     for (int r = 0; r < nrhs; r++) {
-        const int sl = r / rps;
+        const int sl = cm.slot(r);
         const cd *invZ = invZ0 + sl * m;
         const double w = wv[sl];
         const cd *yr = y + r * ld;
@@ -164,13 +164,13 @@ __global__ void k_pre(const cd *y, const cd *invZ0, const double *invP, const in
 // M[q[j]] = z[j]; read through the inverse column permutation)
 __global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ0, const double *invP, const int32_t *ap,
                        const int32_t *ai, const double *av, const int32_t *gtp, const int32_t *gti, const double *gtv,
-                       int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, int rps)
+                       int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, ColMap cm)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t N = m + ns + n;
     if (i >= N) return;
     for (int r = 0; r < nrhs; r++) {
-        const cd *invZ = invZ0 + (r / rps) * m;
+        const cd *invZ = invZ0 + cm.slot(r) * m;
         const cd *zr = z + r * n;
         cd *yr = y + r * ld;
         if (i < m) {
@@ -201,12 +201,12 @@ __global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ0,
 
 // Y1 = -j w Y1 + conj(DZ) .* I - A Phi
 __global__ void k_y1h(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const int32_t *ai, const double *av,
-                      const double *wv, int64_t m, int64_t phi0, int nrhs, int64_t ld, int rps)
+                      const double *wv, int64_t m, int64_t phi0, int nrhs, int64_t ld, ColMap cm)
 {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m) return;
     for (int r = 0; r < nrhs; r++) {
-        const int sl = r / rps;
+        const int sl = cm.slot(r);
         const cd *xr = x + r * ld;
         cd s = {0.0, 0.0};
         for (int k = ap[e]; k < ap[e + 1]; k++) s = cadd(s, cscal(xr[phi0 + ai[k]], av[k]));
@@ -219,7 +219,7 @@ __global__ void k_y1h(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const
 
 // Y2 = Y2 - j w Gamma' Phi
 __global__ void k_y2h(cd *y, const cd *x, const int32_t *gp, const int32_t *gi, const double *gv, const double *wv,
-                      int64_t m, int64_t ns, int64_t phi0, int nrhs, int64_t ld, int rps)
+                      int64_t m, int64_t ns, int64_t phi0, int nrhs, int64_t ld, ColMap cm)
 {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
@@ -227,20 +227,20 @@ __global__ void k_y2h(cd *y, const cd *x, const int32_t *gp, const int32_t *gi, 
         const cd *xr = x + r * ld;
         cd a = {0.0, 0.0};
         for (int k = gp[s]; k < gp[s + 1]; k++) a = cadd(a, cscal(xr[phi0 + gi[k]], gv[k]));
-        y[m + s + r * ld] = csub(y[m + s + r * ld], jw(a, wv[r / rps]));
+        y[m + s + r * ld] = csub(y[m + s + r * ld], jw(a, wv[cm.slot(r)]));
     }
 }
 
 // Y3 = A' I - Gamma Q + conj(Yle) Phi
 __global__ void k_y3h(cd *y, const cd *x, const int32_t *atp, const int32_t *ati, const double *atv,
                       const int32_t *yp0, const int32_t *yi, const cd *yv, const int32_t *gp, const int32_t *gi,
-                      const double *gv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, int rps)
+                      const double *gv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, ColMap cm)
 {
     int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
     for (int r = 0; r < nrhs; r++) {
-        const int sl = r / rps;
+        const int sl = cm.slot(r);
         const int32_t *yp = yp0 + sl * (n + 1);
         const cd *xr = x + r * ld;
         cd a = {0.0, 0.0};
@@ -257,13 +257,13 @@ __global__ void k_y3h(cd *y, const cd *x, const int32_t *atp, const int32_t *ati
 // D2 with D1 = diag(I, j w, -I), D2 = diag(I, 1/(j w), -I) on the (I, Q, Phi)
 // blocks, so M^-H y = conj(D2^-1 M^-1 D1^-1 conj(y)).  stage 0 applies
 // y <- D1^-1 conj(y), stage 1 y <- conj(D2^-1 y).
-__global__ void k_adj_xform(cd *y, const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, int rps,
+__global__ void k_adj_xform(cd *y, const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, ColMap cm,
                             int stage)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m + ns + n) return;
     for (int r = 0; r < nrhs; r++) {
-        const double w = wv[r / rps];
+        const double w = wv[cm.slot(r)];
         cd a = y[i + r * ld];
         if (stage == 0) {
             a.y = -a.y;
@@ -798,7 +798,7 @@ extern "C" int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_
 
 // ------------------------------------------------------------------ apply
 // In place on y (length N per rhs, stride ld): y <- M^-1 y
-int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, int rps)
+int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, const ColMap &cm)
 {
     const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n;
     if (nrhs * n > (int64_t)ctx->vt1.n) {
@@ -807,20 +807,20 @@ int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, int rps)
     double2_t *b = ctx->vt1.p, *z = ctx->vt2.p;
     hipLaunchKernelGGL(k_pre, dim3(nblk(n)), dim3(256), 0, ctx->stream, y, ctx->invZ.p, ctx->invP.p, ctx->At.ptr.p,
                        ctx->At.idx.p, ctx->At.val.p, ctx->G.ptr.p, ctx->G.idx.p, ctx->G.val.p, ctx->perm_r.p,
-                       ctx->has_Rs ? ctx->Rs.p : nullptr, b, ctx->wv.p, m, ns, n, nrhs, ld, rps);
+                       ctx->has_Rs ? ctx->Rs.p : nullptr, b, ctx->wv.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
     int rc;
-    if ((rc = trsv_solve(ctx, ctx->Lf, true, b, z, nrhs, rps))) return rc;   // L c = b   (c in z)
-    if ((rc = trsv_solve(ctx, ctx->Uf, false, z, b, nrhs, rps))) return rc;  // U z' = c  (z' in b)
+    if ((rc = trsv_solve(ctx, ctx->Lf, true, b, z, nrhs, cm))) return rc;   // L c = b   (c in z)
+    if ((rc = trsv_solve(ctx, ctx->Uf, false, z, b, nrhs, cm))) return rc;  // U z' = c  (z' in b)
     hipLaunchKernelGGL(k_post, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, b, ctx->perm_c.p, ctx->invZ.p,
                        ctx->invP.p, ctx->A.ptr.p, ctx->A.idx.p, ctx->A.val.p, ctx->Gt.ptr.p, ctx->Gt.idx.p,
-                       ctx->Gt.val.p, m, ns, n, nrhs, ld, rps);
+                       ctx->Gt.val.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
     return PEEC_OK;
 }
 
 // y = K x (before the preconditioner), device vectors
-int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps)
+int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, const ColMap &cm)
 {
     const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n, N = m + ns + n;
     hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns)), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
@@ -840,12 +840,12 @@ int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t l
     }
     const int64_t phi0 = m + ns;
     hipLaunchKernelGGL(k_y1, dim3(nblk(m)), dim3(256), 0, ctx->stream, y, x, ctx->DZ.p, ctx->A.ptr.p, ctx->A.idx.p,
-                       ctx->A.val.p, ctx->wv.p, m, phi0, nrhs, ld, rps);
+                       ctx->A.val.p, ctx->wv.p, m, phi0, nrhs, ld, cm);
     hipLaunchKernelGGL(k_y2, dim3(nblk(ns)), dim3(256), 0, ctx->stream, y, x, ctx->Gt.ptr.p, ctx->Gt.idx.p,
                        ctx->Gt.val.p, m, ns, phi0, nrhs, ld);
     hipLaunchKernelGGL(k_y3, dim3(nblk(n)), dim3(256), 0, ctx->stream, y, x, ctx->At.ptr.p, ctx->At.idx.p,
                        ctx->At.val.p, ctx->Yle.ptr.p, ctx->Yle.idx.p, ctx->Yle.val.p, ctx->G.ptr.p, ctx->G.idx.p,
-                       ctx->G.val.p, ctx->wv.p, m, ns, n, nrhs, ld, rps);
+                       ctx->G.val.p, ctx->wv.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
     (void)N;
     return PEEC_OK;
@@ -858,15 +858,15 @@ int sync_stream(peec_ctx *ctx)
 }
 
 // device-vector matvec used by GMRES: y = M^-1 K x
-int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps)
+int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, const ColMap &cm)
 {
-    int rc = apply_K(ctx, x, y, nrhs, ld, fp32, rps);
+    int rc = apply_K(ctx, x, y, nrhs, ld, fp32, cm);
     if (rc) return rc;
-    return apply_precond(ctx, y, nrhs, ld, rps);
+    return apply_precond(ctx, y, nrhs, ld, cm);
 }
 
 // y = K^H x (device vectors)
-int apply_KH(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps)
+int apply_KH(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, const ColMap &cm)
 {
     const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n;
     hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns)), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
@@ -884,37 +884,38 @@ int apply_KH(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t 
     }
     const int64_t phi0 = m + ns;
     hipLaunchKernelGGL(k_y1h, dim3(nblk(m)), dim3(256), 0, ctx->stream, y, x, ctx->DZ.p, ctx->A.ptr.p, ctx->A.idx.p,
-                       ctx->A.val.p, ctx->wv.p, m, phi0, nrhs, ld, rps);
+                       ctx->A.val.p, ctx->wv.p, m, phi0, nrhs, ld, cm);
     hipLaunchKernelGGL(k_y2h, dim3(nblk(ns)), dim3(256), 0, ctx->stream, y, x, ctx->Gt.ptr.p, ctx->Gt.idx.p,
-                       ctx->Gt.val.p, ctx->wv.p, m, ns, phi0, nrhs, ld, rps);
+                       ctx->Gt.val.p, ctx->wv.p, m, ns, phi0, nrhs, ld, cm);
     hipLaunchKernelGGL(k_y3h, dim3(nblk(n)), dim3(256), 0, ctx->stream, y, x, ctx->At.ptr.p, ctx->At.idx.p,
                        ctx->At.val.p, ctx->Yle.ptr.p, ctx->Yle.idx.p, ctx->Yle.val.p, ctx->G.ptr.p, ctx->G.idx.p,
-                       ctx->G.val.p, m, ns, n, nrhs, ld, rps);
+                       ctx->G.val.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
     return PEEC_OK;
 }
 
 // In place: y <- M^-H y, through the forward factors of SS
-int apply_precond_adj(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, int rps)
+int apply_precond_adj(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, const ColMap &cm)
 {
     const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n;
     hipLaunchKernelGGL(k_adj_xform, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, ctx->wv.p, m, ns, n, nrhs,
-                       ld, rps, 0);
+                       ld, cm, 0);
     PEEC_HIP_TRY(hipGetLastError());
-    int rc = apply_precond(ctx, y, nrhs, ld, rps);
+    int rc = apply_precond(ctx, y, nrhs, ld, cm);
     if (rc) return rc;
     hipLaunchKernelGGL(k_adj_xform, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, ctx->wv.p, m, ns, n, nrhs,
-                       ld, rps, 1);
+                       ld, cm, 1);
     PEEC_HIP_TRY(hipGetLastError());
     return PEEC_OK;
 }
 
 // adjoint GMRES operator: y = M^-H K^H x
-int matvec_device_adj(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, int rps)
+int matvec_device_adj(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32,
+                      const ColMap &cm)
 {
-    int rc = apply_KH(ctx, x, y, nrhs, ld, fp32, rps);
+    int rc = apply_KH(ctx, x, y, nrhs, ld, fp32, cm);
     if (rc) return rc;
-    return apply_precond_adj(ctx, y, nrhs, ld, rps);
+    return apply_precond_adj(ctx, y, nrhs, ld, cm);
 }
 
 // Allocate every work buffer a matvec of nrhs right-hand sides uses, so
@@ -948,16 +949,22 @@ MatvecGraphs::~MatvecGraphs()
 
 int MatvecGraphs::run(const double2_t *x, double2_t *y, bool fp32)
 {
+    return run_cols(x, y, fp32, nrhs, uniform_cols(nrhs, rps));
+}
+
+int MatvecGraphs::run_cols(const double2_t *x, double2_t *y, bool fp32, int nrhs_c, const ColMap &cm)
+{
     auto direct = [&]() {
-        return adjoint ? matvec_device_adj(ctx, x, y, nrhs, ld, fp32, rps) : matvec_device(ctx, x, y, nrhs, ld, fp32, rps);
+        return adjoint ? matvec_device_adj(ctx, x, y, nrhs_c, ld, fp32, cm)
+                       : matvec_device(ctx, x, y, nrhs_c, ld, fp32, cm);
     };
     if (!ctx->use_graphs || ctx->prof_on) return direct();
     for (auto &e : entries)
-        if (e.x == x && e.y == y && e.fp32 == (int)fp32) {
+        if (e.x == x && e.y == y && e.fp32 == (int)fp32 && e.nrhs == nrhs_c && e.rslot == cm.rslot) {
             PEEC_HIP_TRY(hipGraphLaunch(e.exec, ctx->stream));
             return PEEC_OK;
         }
-    int rc = matvec_prepare(ctx, nrhs);
+    int rc = matvec_prepare(ctx, nrhs_c);
     if (rc) return rc;
     PEEC_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     rc = direct();
@@ -971,7 +978,7 @@ int MatvecGraphs::run(const double2_t *x, double2_t *y, bool fp32)
         peec_set_error("matvec graph capture failed: %s", hipGetErrorString(ec));
         return PEEC_ERR_HIP;
     }
-    Entry e{x, y, (int)fp32, nullptr};
+    Entry e{x, y, (int)fp32, nrhs_c, cm.rslot, nullptr};
     const hipError_t ei = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ei != hipSuccess) {
@@ -1013,7 +1020,7 @@ extern "C" int peec_matvec(peec_ctx *ctx, const peec_c128 *x, peec_c128 *y)
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
     const int64_t N = ctx->m + ctx->ns + ctx->n;
     PEEC_HIP_TRY(hipMemcpyAsync(ctx->vx.p, x, (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
-    if ((rc = matvec_device(ctx, ctx->vx.p, ctx->vy.p, 1, N, false, 1))) return rc;
+    if ((rc = matvec_device(ctx, ctx->vx.p, ctx->vy.p, 1, N, false, uniform_cols(1, 1)))) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(y, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
     return sync_stream(ctx);
 }
@@ -1028,7 +1035,7 @@ extern "C" int peec_apply_K(peec_ctx *ctx, const peec_c128 *x, peec_c128 *y)
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
     const int64_t N = ctx->m + ctx->ns + ctx->n;
     PEEC_HIP_TRY(hipMemcpyAsync(ctx->vx.p, x, (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
-    int rc = apply_K(ctx, ctx->vx.p, ctx->vy.p, 1, N, false, 1);
+    int rc = apply_K(ctx, ctx->vx.p, ctx->vy.p, 1, N, false, uniform_cols(1, 1));
     if (rc) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(y, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1076,17 +1083,18 @@ static int apply_batched_impl(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp3
         return PEEC_ERR_NOMEM;
     }
     PEEC_HIP_TRY(hipMemcpyAsync(X.p, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+    const ColMap cm = uniform_cols(nrhs, rhs_per_slot);
     int rc;
     if (precond == 2) {
         // the preconditioner alone: M^-1 x or M^-H x
         PEEC_HIP_TRY(hipMemcpyAsync(Y.p, X.p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-        rc = adjoint ? apply_precond_adj(ctx, Y.p, nrhs, N, rhs_per_slot) : apply_precond(ctx, Y.p, nrhs, N, rhs_per_slot);
+        rc = adjoint ? apply_precond_adj(ctx, Y.p, nrhs, N, cm) : apply_precond(ctx, Y.p, nrhs, N, cm);
     } else if (adjoint) {
-        rc = precond ? matvec_device_adj(ctx, X.p, Y.p, nrhs, N, fp32 != 0, rhs_per_slot)
-                     : apply_KH(ctx, X.p, Y.p, nrhs, N, fp32 != 0, rhs_per_slot);
+        rc = precond ? matvec_device_adj(ctx, X.p, Y.p, nrhs, N, fp32 != 0, cm)
+                     : apply_KH(ctx, X.p, Y.p, nrhs, N, fp32 != 0, cm);
     } else {
-        rc = precond ? matvec_device(ctx, X.p, Y.p, nrhs, N, fp32 != 0, rhs_per_slot)
-                     : apply_K(ctx, X.p, Y.p, nrhs, N, fp32 != 0, rhs_per_slot);
+        rc = precond ? matvec_device(ctx, X.p, Y.p, nrhs, N, fp32 != 0, cm)
+                     : apply_K(ctx, X.p, Y.p, nrhs, N, fp32 != 0, cm);
     }
     if (rc) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(y, Y.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -1108,7 +1116,7 @@ extern "C" int peec_precond_rhs(peec_ctx *ctx, const double *is, peec_c128 *tn)
     std::vector<double2_t> h((size_t)N, {0.0, 0.0});
     for (int64_t v = 0; v < n; v++) h[(size_t)(m + ns + v)] = {is[v], 0.0};
     PEEC_HIP_TRY(hipMemcpyAsync(ctx->vy.p, h.data(), (size_t)N * 16, hipMemcpyHostToDevice, ctx->stream));
-    if ((rc = apply_precond(ctx, ctx->vy.p, 1, N, 1))) return rc;
+    if ((rc = apply_precond(ctx, ctx->vy.p, 1, N, uniform_cols(1, 1)))) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(tn, ctx->vy.p, (size_t)N * 16, hipMemcpyDeviceToHost, ctx->stream));
     return sync_stream(ctx);
 }

@@ -38,11 +38,11 @@ __device__ __forceinline__ cd tcdiv(cd a, cd b)
 }
 
 // part[s + r * pstride] = sum over the entries of segment s of T[row, c] * x[c]
-// (right-hand side r uses the value set of slot r / rps, ob_n values each)
+// (right-hand side r uses the value set of its slot, ob_n values each)
 __global__ void __launch_bounds__(256) k_trsv_gather(const int64_t *seg_k0, const int64_t *seg_k1, int64_t s0,
                                                      int64_t nseg, const int32_t *idx, const cd *val, int64_t ob_n,
                                                      const cd *x, cd *part, int64_t pstride, int64_t n, int nrhs,
-                                                     int rps)
+                                                     ColMap cm)
 {
     const int64_t sg = s0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (sg >= s0 + nseg) return;  // whole wavefront: the kernel has no barrier
@@ -58,8 +58,8 @@ __global__ void __launch_bounds__(256) k_trsv_gather(const int64_t *seg_k0, cons
     cd v[4];
     int slot = -1;
     for (int r = 0; r < nrhs; r++) {
-        if (r / rps != slot) {
-            slot = r / rps;
+        if (cm.slot(r) != slot) {
+            slot = cm.slot(r);
             const cd *vs = val + (int64_t)slot * ob_n;
 #pragma unroll
             for (int i = 0; i < 4; i++) v[i] = c[i] >= 0 ? vs[k0 + lane + 64 * i] : cd{0.0, 0.0};
@@ -93,7 +93,7 @@ __device__ __forceinline__ double xsh(double v, int d) { return __shfl_xor(v, d,
 __global__ void __launch_bounds__(256) k_trsv_gather4(const int64_t *seg_k0, const int64_t *seg_k1, int64_t s0,
                                                       int64_t nseg, const int32_t *idx, const cd *val, int64_t ob_n,
                                                       const cd *x, cd *part, int64_t pstride, int64_t n, int nrhs,
-                                                      int rps)
+                                                      ColMap cm)
 {
     const int64_t sg = s0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (sg >= s0 + nseg) return;  // whole wavefront: the kernel has no barrier
@@ -107,14 +107,14 @@ __global__ void __launch_bounds__(256) k_trsv_gather4(const int64_t *seg_k0, con
     }
     const bool h32 = lane & 32, h16 = lane & 16;
     const int gsel = (h32 ? 2 : 0) + (h16 ? 1 : 0);
-    const int nsl = (nrhs + rps - 1) / rps;
-    for (int slot = 0; slot < nsl; slot++) {
+    for (int slot = 0; slot < cm.nsl; slot++) {
+        const int rbeg = cm.first(slot), rend = cm.end(slot, nrhs);
+        if (rbeg >= rend) continue;
         cd v[4];
         const cd *vs = val + (int64_t)slot * ob_n;
 #pragma unroll
         for (int i = 0; i < 4; i++) v[i] = c[i] >= 0 ? vs[k0 + lane + 64 * i] : cd{0.0, 0.0};
-        const int rend = min(nrhs, (slot + 1) * rps);
-        for (int r0 = slot * rps; r0 < rend; r0 += 4) {
+        for (int r0 = rbeg; r0 < rend; r0 += 4) {
             cd xv[4][4];
 #pragma unroll
             for (int g = 0; g < 4; g++) {
@@ -169,12 +169,12 @@ __device__ __forceinline__ double lane_value(double v, int l)
 // One workgroup per (block of the level, value set, group of right-hand
 // sides): binfo lists the level's blocks in launch order; blockIdx.y =
 // slot * gps + g, and the workgroup solves the right-hand sides
-// slot * rps + g * 4 + wave (stride 4 gps) of that slot.  inv_diag holds
+// first + g * 4 + wave (stride 4 gps) of that slot.  inv_diag holds
 // 1 / T[i, i].
 template <bool LOWER>
 __global__ void __launch_bounds__(256) k_trsv_block(const int4 *binfo, const cd *dense, int64_t dense_n,
                                                     const cd *inv_diag, const int2 *row_seg, const cd *part,
-                                                    int64_t pstride, const cd *b, cd *x, int64_t n, int nrhs, int rps,
+                                                    int64_t pstride, const cd *b, cd *x, int64_t n, int nrhs, ColMap cm,
                                                     int gps)
 {
     __shared__ cd tile[64 * 64];
@@ -205,8 +205,8 @@ __global__ void __launch_bounds__(256) k_trsv_block(const int4 *binfo, const cd 
     const int2 gs = mine ? row_seg[row] : int2{0, 0};
     const int g0 = gs.x, g1 = gs.y;
     __syncthreads();
-    const int rend = min(nrhs, (slot + 1) * rps);
-    for (int r = slot * rps + g * 4 + wave; r < rend; r += gps * 4) {
+    const int rend = cm.end(slot, nrhs);
+    for (int r = cm.first(slot) + g * 4 + wave; r < rend; r += gps * 4) {
         cd acc = {0.0, 0.0};
         if (mine) {
             acc = b[row + (int64_t)r * n];
@@ -470,7 +470,8 @@ int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &bl
     return PEEC_OK;
 }
 
-int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs, int rps)
+int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs,
+               const ColMap &cm)
 {
     const int64_t pstride = std::max<int64_t>(f.nseg, 1);
     if ((size_t)(pstride * nrhs) > ctx->trsv_part.n) {
@@ -479,14 +480,15 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
             return PEEC_ERR_NOMEM;
         }
     }
-    const int nsl = (nrhs + rps - 1) / rps;
+    const int nsl = cm.nsl;
     if (nsl > f.nslots) {
-        peec_set_error("triangular solve: %d right-hand sides at %d per slot need %d factor slots, %d set", nrhs, rps,
-                       nsl, f.nslots);
+        peec_set_error("triangular solve: %d right-hand sides need %d factor slots, %d set", nrhs, nsl, f.nslots);
         return PEEC_ERR_STATE;
     }
     cd *part = ctx->trsv_part.p;
-    const int gps = std::min((rps + 3) / 4, 16);
+    // workgroups per slot: enough for the slot's columns, 4 per workgroup
+    const int cols = cm.rslot ? (nrhs + nsl - 1) / nsl : cm.rps;
+    const int gps = std::min((cols + 3) / 4, 16);
     const int nlev = (int)f.level_ptr.size() - 1;
     for (int l = 0; l < nlev; l++) {
         const int64_t s0 = f.seg_level_ptr[(size_t)l], ns = f.seg_level_ptr[(size_t)l + 1] - s0;
@@ -498,11 +500,11 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
             if (nrhs >= 4)
                 hipLaunchKernelGGL(k_trsv_gather4, dim3((unsigned)((ns + 3) / 4)), dim3(256), 0, ctx->stream,
                                    f.seg_k0.p, f.seg_k1.p, s0, ns, f.ob_idx.p, f.ob_val.p, f.ob_n, (const cd *)x, part,
-                                   pstride, f.n, nrhs, rps);
+                                   pstride, f.n, nrhs, cm);
             else
                 hipLaunchKernelGGL(k_trsv_gather, dim3((unsigned)((ns + 3) / 4)), dim3(256), 0, ctx->stream,
                                    f.seg_k0.p, f.seg_k1.p, s0, ns, f.ob_idx.p, f.ob_val.p, f.ob_n, (const cd *)x,
-                                   part, pstride, f.n, nrhs, rps);
+                                   part, pstride, f.n, nrhs, cm);
             ctx->prof_end(pi);
         }
         int pi = ctx->prof_begin(lower ? PK_TRSV_BLOCK_L : PK_TRSV_BLOCK_U,
@@ -514,10 +516,10 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
         const dim3 grid(nbl, nsl * gps);
         if (lower)
             hipLaunchKernelGGL(k_trsv_block<true>, grid, dim3(256), 0, ctx->stream, bi, f.dense.p, f.dense_n, f.diag.p,
-                               rsg, (const cd *)part, pstride, b, x, f.n, nrhs, rps, gps);
+                               rsg, (const cd *)part, pstride, b, x, f.n, nrhs, cm, gps);
         else
             hipLaunchKernelGGL(k_trsv_block<false>, grid, dim3(256), 0, ctx->stream, bi, f.dense.p, f.dense_n,
-                               f.diag.p, rsg, (const cd *)part, pstride, b, x, f.n, nrhs, rps, gps);
+                               f.diag.p, rsg, (const cd *)part, pstride, b, x, f.n, nrhs, cm, gps);
         ctx->prof_end(pi);
     }
     PEEC_HIP_TRY(hipGetLastError());

@@ -308,7 +308,7 @@ def set_slots(solver, ws, R0=50.0, timings=None, sigma=None):
         tm["lu"] = tm.get("lu", 0.0) + time.perf_counter() - t1
 
 
-def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=60, max_cycles=20, inner_tol=0.0,
+def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=100, max_cycles=20, inner_tol=0.0,
                           ports_scatter_value=50.0, solver=None, record=None, timings=None, tune=False, Vrest=None,
                           sigma=None, keep=False):
     """The QS sweep of FFT_solver_QS_S_type.jl:60-232 with its frequency x
@@ -399,7 +399,7 @@ def _port_vector(mesh, weights):
     return g
 
 
-def impedance_objective_gradient(freq, mesh, port=0, sigma=None, tol=1e-10, batch=8, restart=60, max_cycles=20,
+def impedance_objective_gradient(freq, mesh, port=0, sigma=None, tol=1e-10, batch=8, restart=100, max_cycles=20,
                                  mixed=False, ports_scatter_value=50.0, solver=None, method="adjoint", record=None):
     """J = sum_f |Z_pp(f)|^2 and dJ/dsigma_v for every voxel (zero off the
     conductors), by the adjoint method.
