@@ -374,18 +374,15 @@ def main():
 
 
 def copy_reference(mib):
-    """An elementwise kernel (not a DMA copy) that reads and writes exactly
-    mib MiB, three times: the known-size dispatch that scales the HBM
-    traffic counters."""
+    """A plain copy kernel (not a DMA copy) that reads and writes exactly
+    mib MiB, three times (peec_copy_reference): the known-size dispatch that
+    scales the HBM traffic counters."""
     if mib <= 0:
         return
-    import torch
-    a = torch.ones((mib << 20) // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        torch.mul(a, 2.0, out=b)
-    torch.cuda.synchronize()
-    log("scaled %d MiB device to device 3 times (elementwise kernel)" % mib)
+    from solver_amd import _lib as L
+    from solver_amd import peec
+    L.check(L.lib().peec_copy_reference(peec.default_context().handle, int(mib) << 20), "peec_copy_reference")
+    log("copied %d MiB device to device 3 times (k_copy_reference)" % mib)
 
 
 if __name__ == "__main__":

@@ -276,6 +276,12 @@ int peec_gmres_batched_adjoint(peec_ctx *ctx, int nrhs, int rhs_per_slot, const 
  * launches the kernels one by one.  Results are identical either way. */
 int peec_set_graphs(peec_ctx *ctx, int on);
 
+/* Copies `bytes` bytes device to device three times with a plain kernel
+ * (k_copy_reference, 16 B per thread): a dispatch of known size for scaling
+ * hardware traffic counters (FETCH_SIZE / WRITE_SIZE).  No counterpart in
+ * the reference. */
+int peec_copy_reference(peec_ctx *ctx, int64_t bytes);
+
 /* Kernel timing (no counterpart in the reference, which only prints @time,
  * do_solving_fft.jl:133).  Reads and clears the per-kind totals recorded
  * since the previous call -- kernel milliseconds from HIP events on the

@@ -5,9 +5,9 @@
 // The reference solves one port at one frequency at a time with
 // unrestarted Householder GMRES (gmres_custom.jl, called from
 // FFT_solver_QS_S_type.jl:140).  Here every right-hand side runs restarted
-// GMRES with classical Gram-Schmidt applied twice (CGS2: two passes of
-// V' w and w - V h over the basis, so the orthogonalisation is two
-// streaming passes instead of j dependent ones), wrapped in iterative
+// GMRES with classical Gram-Schmidt (streaming passes of V' w and
+// w - V h over the basis instead of j dependent ones), repeated where a pass
+// cancels more than 1 - 1/sqrt(2) of ||w|| (the DGKS criterion), wrapped in iterative
 // refinement: each cycle solves M^-1 K d = r to a relative tolerance, adds
 // d to x, and recomputes the true residual r = M^-1 (b - K x) with the FP64
 // operator.  With mixed = 1 the inner cycles apply K with the FFT
@@ -61,33 +61,56 @@ __global__ void __launch_bounds__(256) k_bnorm_final(const double *part, int np,
 }
 
 // part[((r * gx + bx) * 4 + wave) * ncol + k] = partial of conj(V_r[:, k]) . w_r
+// part[((r * gx + bx) * 4 + wave) * ncol + k] = partial of conj(V_r[:, k]) . w_r
+// Columns k go 8 at a time, so w is read once per 8 basis vectors.  Right-
+// hand sides with mask[r] == 0 are skipped (mask may be null).
 __global__ void __launch_bounds__(256) k_bdot_part(const cd *V, int64_t vstride, int ncol, const cd *w, int64_t wld,
-                                                   int64_t n, cd *part)
+                                                   int64_t n, cd *part, const int32_t *mask)
 {
     const int r = blockIdx.y, lane = threadIdx.x & 63;
+    if (mask && !mask[r]) return;
     const int64_t slot = ((int64_t)r * gridDim.x + blockIdx.x) * 4 + (threadIdx.x >> 6);
     const cd *wr = w + (int64_t)r * wld;
-    for (int k = 0; k < ncol; k++) {
-        const cd *u = V + (int64_t)r * vstride + (int64_t)k * n;
-        double sx = 0.0, sy = 0.0;
+    const cd *Vr = V + (int64_t)r * vstride;
+    for (int k0 = 0; k0 < ncol; k0 += 8) {
+        double sx[8], sy[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) sx[q] = sy[q] = 0.0;
         for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-            const cd a = u[i], b = wr[i];
-            sx += a.x * b.x + a.y * b.y;
-            sy += a.x * b.y - a.y * b.x;
+            const cd b = wr[i];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if (k0 + q < ncol) {
+                    const cd a = Vr[(int64_t)(k0 + q) * n + i];
+                    sx[q] += a.x * b.x + a.y * b.y;
+                    sy[q] += a.x * b.y - a.y * b.x;
+                }
+            }
         }
-        for (int o = 32; o > 0; o >>= 1) {
-            sx += __shfl_xor(sx, o, 64);
-            sy += __shfl_xor(sy, o, 64);
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (k0 + q >= ncol) break;
+            double x = sx[q], y = sy[q];
+            for (int o = 32; o > 0; o >>= 1) {
+                x += __shfl_xor(x, o, 64);
+                y += __shfl_xor(y, o, 64);
+            }
+            if (lane == 0) part[slot * ncol + k0 + q] = {x, y};
         }
-        if (lane == 0) part[slot * ncol + k] = {sx, sy};
     }
 }
 
-// h[r * hs + k] = sum of the np partials of (r, k); grid (ncol, nrhs)
-__global__ void __launch_bounds__(256) k_bdot_final(const cd *part, int np, int ncol, cd *h, int hs)
+// h[r * hs + k] = sum of the np partials of (r, k); grid (ncol, nrhs); zero
+// for right-hand sides the mask skips
+__global__ void __launch_bounds__(256) k_bdot_final(const cd *part, int np, int ncol, cd *h, int hs,
+                                                    const int32_t *mask)
 {
     __shared__ double shx[4], shy[4];
     const int k = blockIdx.x, r = blockIdx.y;
+    if (mask && !mask[r]) {
+        if (threadIdx.x == 0) h[(int64_t)r * hs + k] = {0.0, 0.0};
+        return;
+    }
     double sx = 0.0, sy = 0.0;
     for (int i = threadIdx.x; i < np; i += 256) {
         const cd p = part[((int64_t)r * np + i) * ncol + k];
@@ -106,13 +129,13 @@ __global__ void __launch_bounds__(256) k_bdot_final(const cd *part, int np, int 
     if (threadIdx.x == 0) h[(int64_t)r * hs + k] = {(shx[0] + shx[1]) + (shx[2] + shx[3]), (shy[0] + shy[1]) + (shy[2] + shy[3])};
 }
 
-// w_r += sign * V_r[:, 0:ncol) h_r
+// w_r += sign * V_r[:, 0:ncol) h_r (right-hand sides the mask skips untouched)
 __global__ void k_bgemvn(cd *w, int64_t wld, const cd *V, int64_t vstride, int ncol, const cd *h, int hs, int64_t n,
-                         double sign)
+                         double sign, const int32_t *mask)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
-    if (i >= n) return;
+    if (i >= n || (mask && !mask[r])) return;
     const cd *Vr = V + (int64_t)r * vstride;
     const cd *hr = h + (int64_t)r * hs;
     double ax = 0.0, ay = 0.0;
@@ -123,6 +146,15 @@ __global__ void k_bgemvn(cd *w, int64_t wld, const cd *V, int64_t vstride, int n
     }
     cd *p = w + (int64_t)r * wld + i;
     *p = {p->x + sign * ax, p->y + sign * ay};
+}
+
+// Selective reorthogonalisation (Daniel, Gragg, Kaufman, Stewart 1976): a
+// second Gram-Schmidt pass only where the first one cancelled more than
+// 1 - 1/sqrt(2) of ||w||.  mask2[r] = live[r] && ||w_after|| < ||w_before|| / sqrt(2)
+__global__ void k_dgks_mask(const int32_t *live, const double *before, const double *after, int32_t *mask2, int nrhs)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrhs) mask2[r] = (live[r] && after[r] < 0.70710678118654752 * before[r]) ? 1 : 0;
 }
 
 // dst_r = src_r / nrm_r (0 where nrm_r = 0)
@@ -180,11 +212,13 @@ struct BWork {
     DevBuf<cd> V, B, X, R, W, T, Wc, hbuf, part;
     DevBuf<double> nrm, npart;
     DevBuf<int32_t> cmap;  // compacted columns: col[nrhs], rslot[nrhs], slot_ptr[nslots + 1]
+    DevBuf<int32_t> masks; // live[nrhs] (right-hand sides still iterating), second-pass mask[nrhs]
+    DevBuf<double> nrm2;   // ||w|| before and after the first Gram-Schmidt pass
     int alloc()
     {
         const size_t nr = (size_t)nrhs, n = (size_t)N;
         if (!V.alloc(nr * (size_t)(m + 1) * n) || !B.alloc(nr * n) || !X.alloc(nr * n) || !R.alloc(nr * n) ||
-            !W.alloc(nr * n) || !T.alloc(nr * n) || !Wc.alloc(nr * n) ||
+            !W.alloc(nr * n) || !T.alloc(nr * n) || !Wc.alloc(nr * n) || !masks.alloc(2 * nr) || !nrm2.alloc(2 * nr) ||
             !cmap.alloc(2 * nr + (size_t)ctx->nslots + 1) || !hbuf.alloc(nr * 2 * (size_t)(m + 1)) ||
             !part.alloc(nr * kGx * 4 * (size_t)(m + 1)) || !nrm.alloc(nr) || !npart.alloc(nr * kGx * 4)) {
             peec_set_error("peec_gmres_batched: out of device memory for %d right-hand sides x %d basis vectors", nrhs,
@@ -205,17 +239,23 @@ struct BWork {
         return PEEC_OK;
     }
     // h = V[:, 0:ncol)' w into hbuf at column offset hoff (stride hs = 2(m+1))
-    void dot(int ncol, const cd *w, int64_t wld, int hoff)
+    void dot(int ncol, const cd *w, int64_t wld, int hoff, const int32_t *mask = nullptr)
     {
         hipLaunchKernelGGL(k_bdot_part, dim3(kGx, nrhs), dim3(256), 0, ctx->stream, V.p, (int64_t)(m + 1) * N, ncol, w,
-                           wld, N, part.p);
+                           wld, N, part.p, mask);
         hipLaunchKernelGGL(k_bdot_final, dim3(ncol, nrhs), dim3(256), 0, ctx->stream, part.p, kGx * 4, ncol,
-                           hbuf.p + hoff, 2 * (m + 1));
+                           hbuf.p + hoff, 2 * (m + 1), mask);
     }
-    void gemv(cd *w, int64_t wld, int ncol, const cd *h, int hs, double sign)
+    void gemv(cd *w, int64_t wld, int ncol, const cd *h, int hs, double sign, const int32_t *mask = nullptr)
     {
         hipLaunchKernelGGL(k_bgemvn, dim3(nb256(N), nrhs), dim3(256), 0, ctx->stream, w, wld, V.p,
-                           (int64_t)(m + 1) * N, ncol, h, hs, N, sign);
+                           (int64_t)(m + 1) * N, ncol, h, hs, N, sign, mask);
+    }
+    // nrm_out[r] = ||v_r|| on the device (no copy to the host)
+    void norms_device(const cd *v, int64_t ld, double *nrm_out)
+    {
+        hipLaunchKernelGGL(k_bnorm_part, dim3(kGx, nrhs), dim3(256), 0, ctx->stream, v, ld, N, npart.p);
+        hipLaunchKernelGGL(k_bnorm_final, dim3(nrhs), dim3(256), 0, ctx->stream, npart.p, kGx * 4, nrm_out);
     }
     // W = A x (x with stride N), in FP64 or with FP32 convolutions; A =
     // M^-1 K, or M^-H K^H for the adjoint solve; replayed from HIP graphs
@@ -388,7 +428,7 @@ static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per
             const double target = tol * nb[(size_t)r] / std::max(beta[(size_t)r], 1e-300);
             eta[(size_t)r] = mixed ? std::max(target, inner_tol > 0 ? inner_tol : 1e-5) : std::max(target, 1e-15);
         }
-        std::vector<int32_t> live;
+        std::vector<int32_t> live, live_up;
         for (int j = 0; j < m; j++) {
             // w = A_in v_j for the right-hand sides still iterating (the
             // batch shrinks as they converge)
@@ -396,11 +436,23 @@ static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per
             for (int r = 0; r < nrhs; r++)
                 if (inner[(size_t)r]) live.push_back(r);
             if ((rc = W.matvec_cols(W.V.p + (int64_t)j * N, (int64_t)(m + 1) * N, live, mixed != 0))) return rc;
-            // CGS2: two passes of h = V' w, w -= V h
-            W.dot(j + 1, W.W.p, N, 0);
-            W.gemv(W.W.p, N, j + 1, W.hbuf.p, hs, -1.0);
-            W.dot(j + 1, W.W.p, N, m + 1);
-            W.gemv(W.W.p, N, j + 1, W.hbuf.p + (m + 1), hs, -1.0);
+            // classical Gram-Schmidt h = V' w, w -= V h on the live right-
+            // hand sides, and a second pass where it cancelled too much
+            if (live != live_up) {
+                std::vector<int32_t> lm((size_t)nrhs, 0);
+                for (int32_t r : live) lm[(size_t)r] = 1;
+                PEEC_HIP_TRY(hipMemcpyAsync(W.masks.p, lm.data(), (size_t)nrhs * 4, hipMemcpyHostToDevice,
+                                            ctx->stream));
+                live_up = live;
+            }
+            W.norms_device(W.W.p, N, W.nrm2.p);
+            W.dot(j + 1, W.W.p, N, 0, W.masks.p);
+            W.gemv(W.W.p, N, j + 1, W.hbuf.p, hs, -1.0, W.masks.p);
+            W.norms_device(W.W.p, N, W.nrm2.p + nrhs);
+            hipLaunchKernelGGL(k_dgks_mask, dim3((nrhs + 255) / 256), dim3(256), 0, ctx->stream, W.masks.p, W.nrm2.p,
+                               W.nrm2.p + nrhs, W.masks.p + nrhs, nrhs);
+            W.dot(j + 1, W.W.p, N, m + 1, W.masks.p + nrhs);
+            W.gemv(W.W.p, N, j + 1, W.hbuf.p + (m + 1), hs, -1.0, W.masks.p + nrhs);
             PEEC_HIP_TRY(hipGetLastError());
             PEEC_HIP_TRY(hipMemcpyAsync(hh.data(), W.hbuf.p, hh.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
             if ((rc = W.norms(W.W.p, N, hn))) return rc;

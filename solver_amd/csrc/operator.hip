@@ -990,6 +990,34 @@ int MatvecGraphs::run_cols(const double2_t *x, double2_t *y, bool fp32, int nrhs
     return PEEC_OK;
 }
 
+// dst = src, one 16-byte element per thread (a known-size dispatch)
+__global__ void k_copy_reference(double2_t *__restrict__ dst, const double2_t *__restrict__ src, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+extern "C" int peec_copy_reference(peec_ctx *ctx, int64_t bytes)
+{
+    if (!ctx || bytes < 16) {
+        peec_set_error("peec_copy_reference: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t n = bytes / 16;
+    DevBuf<double2_t> a, b;
+    if (!a.alloc((size_t)n) || !b.alloc((size_t)n)) {
+        peec_set_error("peec_copy_reference: out of device memory");
+        return PEEC_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_zero, dim3(nblk(n)), dim3(256), 0, ctx->stream, a.p, n, 1, n);
+    for (int i = 0; i < 3; i++)
+        hipLaunchKernelGGL(k_copy_reference, dim3(nblk(n)), dim3(256), 0, ctx->stream, b.p, a.p, n);
+    PEEC_HIP_TRY(hipGetLastError());
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PEEC_OK;
+}
+
 extern "C" int peec_set_graphs(peec_ctx *ctx, int on)
 {
     if (!ctx) {
