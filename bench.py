@@ -219,6 +219,7 @@ def main():
         copy_reference(args.copy_mib)
         return
 
+    from solver_amd import _lib as L
     from solver_amd import configs, mesher, peec, sweep
     cfg = configs.CONFIGS[args.workload]()
     t0 = time.perf_counter()
@@ -247,11 +248,10 @@ def main():
     batches = [fsel[i * args.batch:(i + 1) * args.batch] for i in range(nb)]
     batches = [b for b in batches if len(b)]
 
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None  # noqa: E731
+    # every library call returns with its stream drained; this also waits
+    # for the whole device (hipDeviceSynchronize)
+    def sync():
+        L.check(L.lib().peec_synchronize(solver.ctx.handle), "peec_synchronize")
 
     Vrest = None
     rec, tm = [], {}

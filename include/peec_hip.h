@@ -53,6 +53,8 @@ const char *peec_last_error(void);
 /* Cooperative stop (is_stop_requested, solver_start.jl:370-374): the
  * library polls *flag between GMRES iterations and integral tiles. */
 int peec_set_stop_flag(peec_ctx *ctx, volatile const int32_t *flag);
+/* Waits for the context's stream and the whole device (for timing). */
+int peec_synchronize(peec_ctx *ctx);
 
 /* ---- partial-element rows (FP64) --------------------------------------- */
 /* compute_Lp_Voxels(centri_m, centri_n, sx, sy, sz, sx2, sy2, sz2, dc,

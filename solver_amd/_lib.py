@@ -78,6 +78,7 @@ def lib():
                                                  ip, dp, ip, ip]
         L.peec_fft_tune.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.peec_set_graphs.argtypes = [vp, ctypes.c_int]
+        L.peec_synchronize.argtypes = [vp]
         L.peec_copy_reference.argtypes = [vp, i64]
         L.peec_fft_set_choice.argtypes = [vp] + [ctypes.c_int] * 7
         L.peec_fft_choice.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]

@@ -1018,6 +1018,18 @@ extern "C" int peec_copy_reference(peec_ctx *ctx, int64_t bytes)
     return PEEC_OK;
 }
 
+extern "C" int peec_synchronize(peec_ctx *ctx)
+{
+    if (!ctx) {
+        peec_set_error("peec_synchronize: null context");
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    PEEC_HIP_TRY(hipDeviceSynchronize());
+    return PEEC_OK;
+}
+
 extern "C" int peec_set_graphs(peec_ctx *ctx, int on)
 {
     if (!ctx) {
