@@ -294,6 +294,24 @@ __device__ void run_fft(C2<T> *tile, int LS, int L, const PassDesc &d, int lg, i
     }
 }
 
+// XCD-aware block order.  The dispatcher hands workgroups to the 8 XCDs
+// round-robin in linear block order, and each XCD has its own L2, so tiles
+// b and b + 1 (which share the 128 B lines of a strided row when a tile row
+// is 64 B, e.g. 8 complex64 lines) would be fetched once per XCD.  The
+// linear id is remapped so that each XCD walks one contiguous range of the
+// (x fastest, then y, then z) tile order: a bijection on [0, n) for any n.
+__device__ __forceinline__ void xcd_block(unsigned &bx, unsigned &by, unsigned &bz)
+{
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned n = gx * gy * gridDim.z;
+    const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned q = n / 8, r = n % 8, xcd = lin % 8, loc = lin / 8;
+    const unsigned t = xcd < r ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+    bx = t % gx;
+    by = (t / gx) % gy;
+    bz = t / (gx * gy);
+}
+
 // e / n for 0 <= e < 2^17 and 1 <= n <= 1024: the float quotient of e + 1/2
 // is at least 1/(2n) away from an integer, far beyond the rounding error.
 __device__ __forceinline__ int qdiv(int e, float inv) { return (int)(((float)e + 0.5f) * inv); }
@@ -309,9 +327,9 @@ constexpr int fft_prefetch_slots()
 
 // MODE: 0 forward, 1 inverse, 2 forward * real spectrum * inverse, 3
 // forward * complex spectrum (or its conjugate) * inverse.
-// A block transforms a tile of W consecutive lines (positions b0 .. b0+W-1
-// along the tiled axis) at line index l2, for right-hand sides
-// blockIdx.z * rpb .. + rpb - 1.  Lines contiguous in memory (sj == 1) are
+// A block (tile (bx, by, bz) after the XCD remap) transforms W consecutive
+// lines (positions b0 = bx W .. b0+W-1 along the tiled axis) at line index
+// l2 = by, for right-hand sides bz * rpb .. + rpb - 1.  Lines contiguous in memory (sj == 1) are
 // read and written line after line; strided lines are read and written
 // row by row across the W lines (coalesced along the tiled axis).  The LDS
 // line stride is L + 1 so that both access orders avoid bank conflicts.
@@ -336,8 +354,10 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, 
     C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw);
     C2<T> *tile = twl + L;
     const int tid = threadIdx.x;
-    const int b0 = blockIdx.x * W;
-    const int l2 = blockIdx.y;
+    unsigned bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int b0 = (int)bx * W;
+    const int l2 = (int)by;
     const int nbt = min(W, d.nb - b0);
     const int lg = tid / TPL, lane = tid % TPL;
     const bool in_contig = (d.in_sj == 1), out_contig = (d.out_sj == 1);
@@ -416,7 +436,7 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, 
         }
     };
 
-    const int r0 = (int)blockIdx.z * d.rpb;
+    const int r0 = (int)bz * d.rpb;
     const int r_end = min(d.nrhs, r0 + d.rpb);
     if (pf && r0 < r_end) fetch(r0);
     for (int rhs = r0; rhs < r_end; rhs++) {
