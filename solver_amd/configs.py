@@ -61,11 +61,20 @@ def _port_to_slab(g, cell, vox, z_diel):
     return (a, b)
 
 
-def _x_face_ports(g, cell, z_diel):
+def _x_face_voxels(g):
     cu = np.argwhere(g == 1)
     lo = cu[np.lexsort((cu[:, 2], cu[:, 1], cu[:, 0]))][0]
     hi = cu[np.lexsort((cu[:, 2], cu[:, 1], -cu[:, 0]))][0]
+    return lo, hi
+
+
+def _x_face_ports(g, cell, z_diel):
+    lo, hi = _x_face_voxels(g)
     return [_port_to_slab(g, cell, lo, z_diel), _port_to_slab(g, cell, hi, z_diel)]
+
+
+def _centre(cell, vox):
+    return tuple((v + 0.5) * c for v, c in zip(vox, cell))
 
 
 def two_material_64x64x16(seed=0):
@@ -97,9 +106,19 @@ def batched_96x96x24(seed=0, nports=8):
 
 
 def adjoint_64x64x16(seed=0):
+    """The two-material 64x64x16 geometry with per-voxel copper conductivity
+    uniform in [3e7, 6e7] S/m from the seed.  Port 1 spans the copper
+    component from its -x face voxel to its +x face voxel (a conduction
+    path, so Z_11 = R + j w L depends on the conductivity at first order);
+    port 2 runs from the +x face voxel to the slab below it.  A trace-to-slab
+    port 1 would make Z_11 a series capacitance whose relative sensitivity to
+    one voxel's conductivity is ~1e-17, below what central differences can
+    resolve in FP64."""
     d = two_material_64x64x16(seed)
     g, mats, sigma = _layered(64, 64, 16, d["cell"], 12, 0.35, 4.4, 0.02, seed, sigma_range=(3e7, 6e7))
-    d.update(name="adjoint_64x64x16", sigma=sigma)
+    lo, hi = _x_face_voxels(g)
+    ports = [(_centre(d["cell"], lo), _centre(d["cell"], hi)), _port_to_slab(g, d["cell"], hi, 12)]
+    d.update(name="adjoint_64x64x16", sigma=sigma, ports=ports)
     return d
 
 
