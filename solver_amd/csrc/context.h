@@ -41,6 +41,18 @@ struct double2_t {
     double x, y;
 };
 
+// Workgroup order that keeps runs of consecutive tiles on one XCD.  The
+// dispatcher deals workgroups to the 8 XCDs round-robin by linear id and
+// each XCD has its own L2, so neighbouring tiles that read the same lines
+// would fetch them once per XCD.  xcd_tile maps linear id lin (of n) to a
+// tile so that XCD k walks the k-th contiguous range of tiles; it is a
+// bijection on [0, n) for every n.
+__device__ __forceinline__ unsigned xcd_tile(unsigned lin, unsigned n)
+{
+    const unsigned q = n / 8, r = n % 8, xcd = lin % 8, loc = lin / 8;
+    return xcd < r ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+}
+
 // Which frequency slot each right-hand side (column) of a batched call
 // uses.  Uniform: column r -> slot r / rps.  Explicit (rslot non-null): the
 // columns are grouped by slot in increasing order, rslot[r] is the slot of

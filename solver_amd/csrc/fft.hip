@@ -294,19 +294,12 @@ __device__ void run_fft(C2<T> *tile, int LS, int L, const PassDesc &d, int lg, i
     }
 }
 
-// XCD-aware block order.  The dispatcher hands workgroups to the 8 XCDs
-// round-robin in linear block order, and each XCD has its own L2, so tiles
-// b and b + 1 (which share the 128 B lines of a strided row when a tile row
-// is 64 B, e.g. 8 complex64 lines) would be fetched once per XCD.  The
-// linear id is remapped so that each XCD walks one contiguous range of the
-// (x fastest, then y, then z) tile order: a bijection on [0, n) for any n.
+// Tile (bx, by, bz) of this workgroup in XCD order (xcd_tile over the x
+// fastest, then y, then z tile order).
 __device__ __forceinline__ void xcd_block(unsigned &bx, unsigned &by, unsigned &bz)
 {
     const unsigned gx = gridDim.x, gy = gridDim.y;
-    const unsigned n = gx * gy * gridDim.z;
-    const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-    const unsigned q = n / 8, r = n % 8, xcd = lin % 8, loc = lin / 8;
-    const unsigned t = xcd < r ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+    const unsigned t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
     bx = t % gx;
     by = (t / gx) % gy;
     bz = t / (gx * gy);

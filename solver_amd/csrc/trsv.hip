@@ -44,7 +44,9 @@ __global__ void __launch_bounds__(256) k_trsv_gather(const int64_t *seg_k0, cons
                                                      const cd *x, cd *part, int64_t pstride, int64_t n, int nrhs,
                                                      ColMap cm)
 {
-    const int64_t sg = s0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // segments of one block's rows read the same columns of x: keep them on
+    // one XCD (one L2)
+    const int64_t sg = s0 + (int64_t)xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (sg >= s0 + nseg) return;  // whole wavefront: the kernel has no barrier
     const int lane = threadIdx.x & 63;
     const int64_t k0 = seg_k0[sg], k1 = seg_k1[sg];
@@ -86,24 +88,55 @@ __global__ void __launch_bounds__(256) k_trsv_gather(const int64_t *seg_k0, cons
 // once, then its right-hand sides go 4 per pass, with the 16 x loads of a
 // pass (4 right-hand sides x 4 entries per lane) all in flight, and the 4
 // lane sums reduced together by recursive halving -- exchanges at distance
-// 32 and 16 keep 2, then 1 of the sums per lane, then 8, 4, 2, 1 finish
-// it: 7 exchanges for 4 complex sums instead of 24.
+// 32 and 16 keep 2, then 1 of the sums per lane, then a 16-lane all-reduce
+// finishes it: 7 exchanges for 4 complex sums instead of 24.
+//
+// The loads carry no branches: a lane past the segment's end reads column
+// 0 against a zero value, and right-hand sides past the slot's end re-read
+// the last one (their sums are not written).  The complex products are
+// FMA chains.  The 16-lane all-reduce uses DPP lane moves (row mirror,
+// half-row mirror, then quad swaps 1 and 2: after the four steps every lane
+// of the row holds the sum of all 16), the two cross-row exchanges
+// ds_bpermute.
 __device__ __forceinline__ double xsh(double v, int d) { return __shfl_xor(v, d, 64); }
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppQuadSwap1 = 0xB1, kDppQuadSwap2 = 0x4E;
+
+__device__ __forceinline__ double row16_sum(double v)
+{
+    v += dpp_move<kDppRowMirror>(v);
+    v += dpp_move<kDppRowHalfMirror>(v);
+    v += dpp_move<kDppQuadSwap1>(v);
+    v += dpp_move<kDppQuadSwap2>(v);
+    return v;
+}
 
 __global__ void __launch_bounds__(256) k_trsv_gather4(const int64_t *seg_k0, const int64_t *seg_k1, int64_t s0,
                                                       int64_t nseg, const int32_t *idx, const cd *val, int64_t ob_n,
                                                       const cd *x, cd *part, int64_t pstride, int64_t n, int nrhs,
                                                       ColMap cm)
 {
-    const int64_t sg = s0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // segments of one block's rows read the same columns of x: keep them on
+    // one XCD (one L2)
+    const int64_t sg = s0 + (int64_t)xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (sg >= s0 + nseg) return;  // whole wavefront: the kernel has no barrier
     const int lane = threadIdx.x & 63;
     const int64_t k0 = seg_k0[sg], k1 = seg_k1[sg];
     int c[4];
+    bool ok[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int64_t k = k0 + lane + 64 * i;
-        c[i] = k < k1 ? idx[k] : -1;
+        ok[i] = k < k1;
+        c[i] = ok[i] ? idx[k] : 0;
     }
     const bool h32 = lane & 32, h16 = lane & 16;
     const int gsel = (h32 ? 2 : 0) + (h16 ? 1 : 0);
@@ -113,15 +146,14 @@ __global__ void __launch_bounds__(256) k_trsv_gather4(const int64_t *seg_k0, con
         cd v[4];
         const cd *vs = val + (int64_t)slot * ob_n;
 #pragma unroll
-        for (int i = 0; i < 4; i++) v[i] = c[i] >= 0 ? vs[k0 + lane + 64 * i] : cd{0.0, 0.0};
+        for (int i = 0; i < 4; i++) v[i] = ok[i] ? vs[k0 + lane + 64 * i] : cd{0.0, 0.0};
         for (int r0 = rbeg; r0 < rend; r0 += 4) {
             cd xv[4][4];
 #pragma unroll
             for (int g = 0; g < 4; g++) {
-                const int r = min(r0 + g, rend - 1);
-                const cd *xr = x + (int64_t)r * n;
+                const cd *xr = x + (int64_t)min(r0 + g, rend - 1) * n;
 #pragma unroll
-                for (int i = 0; i < 4; i++) xv[g][i] = (c[i] >= 0 && r0 + g < rend) ? xr[c[i]] : cd{0.0, 0.0};
+                for (int i = 0; i < 4; i++) xv[g][i] = xr[c[i]];
             }
             double sx[4], sy[4];
 #pragma unroll
@@ -129,8 +161,10 @@ __global__ void __launch_bounds__(256) k_trsv_gather4(const int64_t *seg_k0, con
                 double ax = 0.0, ay = 0.0;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    ax += v[i].x * xv[g][i].x - v[i].y * xv[g][i].y;
-                    ay += v[i].x * xv[g][i].y + v[i].y * xv[g][i].x;
+                    ax = __builtin_fma(v[i].x, xv[g][i].x, ax);
+                    ax = __builtin_fma(-v[i].y, xv[g][i].y, ax);
+                    ay = __builtin_fma(v[i].x, xv[g][i].y, ay);
+                    ay = __builtin_fma(v[i].y, xv[g][i].x, ay);
                 }
                 sx[g] = ax;
                 sy[g] = ay;
@@ -145,13 +179,8 @@ __global__ void __launch_bounds__(256) k_trsv_gather4(const int64_t *seg_k0, con
                 tx[g] = kx + xsh(ox, 32);
                 ty[g] = ky + xsh(oy, 32);
             }
-            double wx = (h16 ? tx[1] : tx[0]) + xsh(h16 ? tx[0] : tx[1], 16);
-            double wy = (h16 ? ty[1] : ty[0]) + xsh(h16 ? ty[0] : ty[1], 16);
-#pragma unroll
-            for (int d = 8; d > 0; d >>= 1) {
-                wx += xsh(wx, d);
-                wy += xsh(wy, d);
-            }
+            const double wx = row16_sum((h16 ? tx[1] : tx[0]) + xsh(h16 ? tx[0] : tx[1], 16));
+            const double wy = row16_sum((h16 ? ty[1] : ty[0]) + xsh(h16 ? ty[0] : ty[1], 16));
             if ((lane & 15) == 0 && r0 + gsel < rend) part[sg + (int64_t)(r0 + gsel) * pstride] = {wx, wy};
         }
     }
