@@ -42,15 +42,16 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv_gather",
-              "trsv_block_L", "trsv_block_U", "lu_factor"]
+              "trsv_block_L", "trsv_block_U", "lu_factor", "fft_yz_conv"]
 # kernel symbol behind each profile kind (the FFT pass kernel is
 # instantiated per precision T, mode -- 0 forward, 1 inverse, 2 forward *
 # spectrum * inverse -- and launch shape TPL, LPT, LMAX chosen by the tuner)
 KERNEL_SYMBOLS = {"fft_x_fwd_scatter": "k_fft_pass<T, 0, TPL, LPT, LMAX>", "fft_y_fwd": "k_fft_pass<T, 0, TPL, LPT, LMAX>",
                   "fft_z_conv": "k_fft_pass<T, 2, TPL, LPT, LMAX>", "fft_y_inv": "k_fft_pass<T, 1, TPL, LPT, LMAX>",
-                  "fft_x_inv_gather": "k_fft_pass<T, 1, TPL, LPT, LMAX>", "trsv_gather": "k_trsv_gather",
+                  "fft_x_inv_gather": "k_fft_pass<T, 1, TPL, LPT, LMAX>", "trsv_gather": "k_trsv_gather4",
                   "trsv_block_L": "k_trsv_block<true>", "trsv_block_U": "k_trsv_block<false>",
-                  "lu_factor": "peec_lu_factor (all its kernels)"}
+                  "lu_factor": "peec_lu_factor (all its kernels)",
+                  "fft_yz_conv": "k_yz_conv<float, MODE, 32, LPT, 256, LZ>"}
 
 
 def log(*a):

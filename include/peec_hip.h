@@ -230,14 +230,18 @@ int peec_precond_rhs_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const do
  * receives the summed best pass times.  The choices last until the next
  * peec_kernels_build or peec_operator_set. */
 int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *report_ms);
-/* The choice peec_fft_tune made for pass `pass` (0..4) of kernel `kernel`
- * (0..8): out5 = {variant (-1 untuned), lines per tile, lanes per line,
- * right-hand sides per block, radix schedule}. */
+/* The choice peec_fft_tune made for pass `pass` of kernel `kernel` (0..8):
+ * out5 = {variant (-1 untuned), lines per tile, lanes per line, right-hand
+ * sides per block, radix schedule}.  Passes 0..4 are the five passes of a
+ * convolution; 5..7 (complex64) the fused pipeline: 5 x forward into
+ * planes, 6 the fused y-z pass (variant 0: the pipeline is used, 1: the
+ * five passes are), 7 x inverse from planes. */
 int peec_fft_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int *out5);
 
 /* Sets the choice of one pass by hand (a tuning table kept from an earlier
- * run, or a test of every launch shape): variant -1 restores the default.
- * Non-zero radix schedules are accepted for complex64 only. */
+ * run, or a test of every launch shape): variant -1 restores the default
+ * (for pass 6: the five passes).  Non-zero radix schedules, and passes
+ * 5..7, are accepted for complex64 only. */
 int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int variant, int rpb, int style);
 
 /* K x (precond = 0), M^-1 K x (precond = 1) or M^-1 x (precond = 2, the
@@ -292,7 +296,9 @@ int peec_copy_reference(peec_ctx *ctx, int64_t bytes);
  * passes (x fwd + scatter, y fwd, z fwd * spectrum * z inv, y inv, x inv +
  * gather), 5 triangular-solve gathers, 6 and 7 triangular-solve block
  * substitutions with L and with U (one record per launch), 8 device
- * factorizations (peec_lu_factor, one record per call). */
+ * factorizations (peec_lu_factor, one record per call), 9 the fused
+ * y forward / z convolution / y inverse pass (complex64 applications that
+ * take it; their x passes count under 0 and 4). */
 int peec_profile(peec_ctx *ctx, int enable, double *ms, double *bytes, int64_t *count, int nkinds);
 
 #ifdef __cplusplus

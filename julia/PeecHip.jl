@@ -136,7 +136,7 @@ end
 # Restarted GMRES in FP64 iterative refinement for all columns of B at once
 # (mixed = true: complex64 FFT convolutions inside the cycles).
 function peec_gmres_batched(ctx, B::Matrix{ComplexF64}, rhs_per_slot, tol; X0 = zeros(ComplexF64, size(B)),
-                            restart = 60, max_cycles = 20, mixed = false, inner_tol = 0.0, adjoint = false)
+                            restart = 100, max_cycles = 20, mixed = false, inner_tol = 0.0, adjoint = false)
     N, nrhs = size(B)
     X = copy(X0)
     flag = zeros(Cint, nrhs); relres = zeros(Float64, nrhs); iters = zeros(Cint, nrhs); cycles = zeros(Cint, nrhs)

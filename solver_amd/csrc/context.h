@@ -97,6 +97,10 @@ struct Kernel {
     int pn[3] = {0, 0, 0}, pL[3] = {0, 0, 0}, delta[3] = {0, 0, 0};
     DevBuf<double2_t> pspec64;
     DevBuf<float> pspec32;
+    // FP32 spectra in plane order [x][z][y] for the fused y-z pass (built
+    // for the applications that use it): even kernel (real) and half-cell
+    // reduction (complex)
+    DevBuf<float> fspec32, pfspec32;
 };
 
 // One FFT-convolution application of the matvec: scatter unknowns of one
@@ -233,7 +237,8 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 enum ProfKind {
-    PK_FFT_X_FWD = 0, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV, PK_TRSV_GATHER, PK_TRSV_BLOCK_L, PK_TRSV_BLOCK_U, PK_LU, PK_COUNT
+    PK_FFT_X_FWD = 0, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV, PK_TRSV_GATHER, PK_TRSV_BLOCK_L, PK_TRSV_BLOCK_U, PK_LU,
+    PK_FFT_YZ_CONV, PK_COUNT
 };
 
 struct peec_ctx {

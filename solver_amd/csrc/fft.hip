@@ -204,12 +204,12 @@ __device__ __forceinline__ void dftR(C2<T> *v)
 }
 
 // One Stockham stage over a tile of W = (256 / TPL) * LPT lines held in LDS
-// (line stride LS).  Lane `lane` of line group `lg` owns butterflies
-// k = lane + q * TPL of lines lg + lp * (256 / TPL).  twl: the L twiddles
-// exp(-2 pi i t / L) in LDS.
+// (line stride LS), of which the first nl are transformed.  Lane `lane` of
+// line group `lg` owns butterflies k = lane + q * TPL of lines
+// lg + lp * (256 / TPL).  twl: the L twiddles exp(-2 pi i t / L) in LDS.
 template <typename T, int DIR, int R, int TPL, int LPT, int LMAX>
 __device__ __forceinline__ void stockham_stage(C2<T> *tile, int LS, int L, int Ns, int lg, int lane,
-                                               const C2<T> *twl)
+                                               const C2<T> *twl, int nl)
 {
     constexpr int G = 256 / TPL;
     constexpr int MAXB = (LMAX + R * TPL - 1) / (R * TPL);
@@ -222,7 +222,7 @@ __device__ __forceinline__ void stockham_stage(C2<T> *tile, int LS, int L, int N
 #pragma unroll
         for (int q = 0; q < MAXB; q++) {
             const int k = lane + q * TPL;
-            if (k < nb) {
+            if (k < nb && lg + lp * G < nl) {
 #pragma unroll
                 for (int r = 0; r < R; r++) v[lp][q][r] = buf[k + r * nb];
             }
@@ -235,7 +235,7 @@ __device__ __forceinline__ void stockham_stage(C2<T> *tile, int LS, int L, int N
 #pragma unroll
         for (int q = 0; q < MAXB; q++) {
             const int k = lane + q * TPL;
-            if (k < nb) {
+            if (k < nb && lg + lp * G < nl) {
                 const int kk = k % Ns;
                 if (Ns > 1) {
 #pragma unroll
@@ -275,20 +275,22 @@ struct PassDesc {
     const void *spec;                    // spectrum for the fused z pass: real T (MODE 2), C2<T> (MODE 3)
     int64_t spec_sb, spec_s2, spec_sj;
     int spec_conj;                       // MODE 3: multiply by the conjugate spectrum
+    int64_t tw_off_z;                    // fused y-z pass: z twiddles (z length is a template argument)
 };
 
 template <typename T, int DIR, int TPL, int LPT, int LMAX>
-__device__ void run_fft(C2<T> *tile, int LS, int L, const PassDesc &d, int lg, int lane, const C2<T> *twl)
+__device__ void run_fft(C2<T> *tile, int LS, int L, const PassDesc &d, int lg, int lane, const C2<T> *twl,
+                        int nl = 256 / TPL * LPT)
 {
     int Ns = 1;
     for (int s = 0; s < d.nstage; s++) {
         const int R = d.radix[s];
         switch (R) {
-        case 8: stockham_stage<T, DIR, 8, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl); break;
-        case 4: stockham_stage<T, DIR, 4, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl); break;
-        case 2: stockham_stage<T, DIR, 2, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl); break;
-        case 3: stockham_stage<T, DIR, 3, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl); break;
-        default: stockham_stage<T, DIR, 5, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl); break;
+        case 8: stockham_stage<T, DIR, 8, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl, nl); break;
+        case 4: stockham_stage<T, DIR, 4, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl, nl); break;
+        case 2: stockham_stage<T, DIR, 2, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl, nl); break;
+        case 3: stockham_stage<T, DIR, 3, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl, nl); break;
+        default: stockham_stage<T, DIR, 5, TPL, LPT, LMAX>(tile, LS, L, Ns, lg, lane, twl, nl); break;
         }
         Ns *= R;
     }
@@ -622,6 +624,142 @@ __global__ void __launch_bounds__(256) k_zconv_reg(PassDesc d, const C2<T> *src,
     }
 }
 
+// ---- fused y forward, z convolution, y inverse over one x' plane
+//
+// The x pass writes its output plane by plane (x' slowest, then z, then y;
+// ny * nz elements per plane per right-hand side).  One block owns one x'
+// plane for rpb right-hand sides and keeps it in LDS as nz lines of length
+// Ly (line stride Ly + 1):
+//   1. read the plane's ny x nz compact values, zero-pad each line to Ly;
+//   2. forward y FFT of the nz lines (Stockham stages, as k_fft_pass);
+//   3. z convolution: thread t owns column y' = t, reads its nz values out
+//      of the tile, zero-pads to LZ, and runs forward z FFT * spectrum *
+//      inverse z FFT in registers (as k_zconv_reg), then writes back the
+//      nz kept values;
+//   4. inverse y FFT of the nz lines, keep ny values per line;
+//   5. write the plane back (in place).
+// Three HBM passes (y forward, z, y inverse) become one, over the compact
+// plane only.  The spectrum is stored plane by plane too, [x'][kz][y']
+// (real for MODE 2, complex for MODE 3), so step 3 reads it coalesced; it
+// stays in registers for the block's right-hand sides.  The same stages and
+// twiddles as the unfused passes give bitwise-identical results.
+// d: L = Ly, nin = nout = ny, nb = number of planes, n2 = nz, in_rhs =
+// out_rhs = plane stride * planes, in_s2 = plane size, tw_off (y),
+// tw_off_z (z).
+template <typename T, int MODE, int TPL, int LPT, int LMAXY, int LZ>
+__global__ void __launch_bounds__(256) k_yz_conv(PassDesc d, const C2<T> *src, C2<T> *dst,
+                                                 const double2_t *__restrict__ twall)
+{
+    constexpr int G = 256 / TPL;
+    extern __shared__ char smem_raw[];
+    const int Ly = d.L, LS = Ly + 1, ny = d.nin, nz = d.n2;
+    C2<T> *twl = reinterpret_cast<C2<T> *>(smem_raw);
+    C2<T> *tile = twl + Ly;
+    const int tid = threadIdx.x;
+    unsigned bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int xp = (int)bx;
+    const int lg = tid / TPL, lane = tid % TPL;
+    for (int t = tid; t < Ly; t += 256) {
+        const double2_t w = twall[d.tw_off + t];
+        twl[t] = {(T)w.x, (T)w.y};
+    }
+    const double2_t *twz = twall + d.tw_off_z;
+    const bool col = tid < Ly;
+    T sr[LZ], si[MODE == 3 ? LZ : 1];
+    {
+        const int64_t sb = (int64_t)xp * LZ * Ly + (col ? tid : 0);
+#pragma unroll
+        for (int j = 0; j < LZ; j++) {
+            if (MODE == 2) {
+                sr[j] = col ? reinterpret_cast<const T *>(d.spec)[sb + (int64_t)j * Ly] : (T)0;
+            } else {
+                const C2<T> c = col ? reinterpret_cast<const C2<T> *>(d.spec)[sb + (int64_t)j * Ly] : C2<T>{(T)0, (T)0};
+                sr[j] = c.x;
+                si[MODE == 3 ? j : 0] = d.spec_conj ? -c.y : c.y;
+            }
+        }
+    }
+    const float invNy = 1.0f / (float)ny, invPad = Ly > ny ? 1.0f / (float)(Ly - ny) : 0.0f;
+    const int r0 = (int)by * d.rpb, r_end = min(d.nrhs, r0 + d.rpb);
+    // the plane of the next right-hand side is loaded into registers while
+    // the current one is transformed (nz * ny <= 8 LPT * 128 elements)
+    constexpr int NPF = 4 * LPT;
+    const int npl = nz * ny;
+    C2<T> pv[NPF];
+    auto fetch = [&](int rhs) {
+        const int64_t pb = (int64_t)rhs * d.in_rhs + (int64_t)xp * d.in_s2;
+#pragma unroll
+        for (int i = 0; i < NPF; i++) {
+            const int e = tid + 256 * i;
+            pv[i] = e < npl ? src[pb + e] : C2<T>{(T)0, (T)0};
+        }
+    };
+    if (r0 < r_end) fetch(r0);
+    for (int rhs = r0; rhs < r_end; rhs++) {
+        const int64_t pbase = (int64_t)rhs * d.in_rhs + (int64_t)xp * d.in_s2;
+        // 1. the plane, zero-padded lines
+#pragma unroll
+        for (int i = 0; i < NPF; i++) {
+            const int e = tid + 256 * i;
+            if (e < npl) {
+                const int z = qdiv(e, invNy);
+                tile[z * LS + e - z * ny] = pv[i];
+            }
+        }
+        for (int e = tid; e < nz * (Ly - ny); e += 256) {
+            const int z = qdiv(e, invPad);
+            tile[z * LS + ny + e - z * (Ly - ny)] = C2<T>{(T)0, (T)0};
+        }
+        if (rhs + 1 < r_end) fetch(rhs + 1);
+        __syncthreads();
+        // 2. forward y FFT
+        run_fft<T, -1, TPL, LPT, LMAXY>(tile, LS, Ly, d, lg, lane, twl, nz);
+        // 3. z convolution of column y' = tid
+        if (col) {
+            C2<T> a[LZ];
+#pragma unroll
+            for (int j = 0; j < LZ; j++) a[j] = j < nz ? tile[j * LS + tid] : C2<T>{(T)0, (T)0};
+            reg_stages<T, -1, LZ, 0>(a, twz);
+#pragma unroll
+            for (int j = 0; j < LZ; j++) {
+                if (MODE == 2) a[j] = cscale(a[j], sr[j]);
+                else a[j] = cmul(a[j], C2<T>{sr[j], si[MODE == 3 ? j : 0]});
+            }
+            reg_stages<T, 1, LZ, 0>(a, twz);
+#pragma unroll
+            for (int j = 0; j < LZ; j++)
+                if (j < nz) tile[j * LS + tid] = a[j];
+        }
+        __syncthreads();
+        // 4. inverse y FFT
+        run_fft<T, 1, TPL, LPT, LMAXY>(tile, LS, Ly, d, lg, lane, twl, nz);
+        // 5. the plane's ny kept values per line
+        for (int e = tid; e < nz * ny; e += 256) {
+            const int z = qdiv(e, invNy), y = e - z * ny;
+            dst[pbase + (int64_t)z * ny + y] = tile[z * LS + y];
+        }
+        __syncthreads();
+    }
+    (void)G;
+}
+
+// Spectrum in plane order for k_yz_conv: out[(x * Lz + z) * Ly + y] =
+// in[(z * Ly + y) * Lx + x], NC reals per element (1: real, 2: complex).
+template <typename T, int NC>
+__global__ void k_spec_planes(const T *__restrict__ in, int Lx, int Ly, int Lz, T *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t G = (int64_t)Lx * Ly * Lz;
+    if (i >= G) return;
+    const int y = (int)(i % Ly);
+    const int64_t r = i / Ly;
+    const int z = (int)(r % Lz), x = (int)(r / Lz);
+    const int64_t j = ((int64_t)z * Ly + y) * Lx + x;
+#pragma unroll
+    for (int c = 0; c < NC; c++) out[i * NC + c] = in[j * NC + c];
+}
+
 // Even-symmetric embedding of a compact row into an L grid
 // (store_circulant_fft, compute_FFT_mutual_coupling_mats.jl:130-155, at
 // lengths L >= 2n - 1 instead of 2n).
@@ -934,6 +1072,107 @@ static int launch_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const vo
     return launch_pass_t<double, 3>(ctx, d, src, dst, v);
 }
 
+// Fused y-z pass (k_yz_conv): complex64; y lines up to 256, z lengths of
+// the register kernel, at most 32 z lines, the plane within 64 KiB of LDS.
+static constexpr int kYzMaxLy = 256, kYzMaxLines = 32;
+static size_t yz_lds(int Ly, int nz) { return ((size_t)Ly + (size_t)nz * (size_t)(Ly + 1)) * sizeof(C2<float>); }
+static bool yz_fits(const ConvApp &app, bool fp32)
+{
+    return fp32 && app.L[1] <= kYzMaxLy && app.n[2] >= 1 && app.n[2] <= kYzMaxLines && app.n[2] <= app.L[2] &&
+           reg_length(app.L[2], true) && yz_lds(app.L[1], app.n[2]) <= kFftLdsMax;
+}
+
+#define PEEC_YZ_CASE(LZ_)                                                                                           \
+    case LZ_:                                                                                                       \
+        hipLaunchKernelGGL((k_yz_conv<float, MODE, 32, LPT, 256, LZ_>), grid, dim3(256), lds, ctx->stream, d, s, o, \
+                           tw);                                                                                     \
+        break;
+
+template <int MODE, int LPT>
+static int launch_yz_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst, int Lz)
+{
+    const int rpb = std::max(1, d.rpb);
+    dim3 grid((unsigned)d.nb, (unsigned)((d.nrhs + rpb - 1) / rpb));
+    const size_t lds = yz_lds(d.L, d.n2);
+    const C2<float> *s = (const C2<float> *)src;
+    C2<float> *o = (C2<float> *)dst;
+    const double2_t *tw = ctx->twiddles.p;
+    switch (Lz) {
+        PEEC_YZ_CASE(8)
+        PEEC_YZ_CASE(16)
+        PEEC_YZ_CASE(32)
+        PEEC_YZ_CASE(48)
+        PEEC_YZ_CASE(64)
+    default:
+        peec_set_error("fused y-z pass: no kernel for z length %d", Lz);
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipGetLastError());
+    return PEEC_OK;
+}
+#undef PEEC_YZ_CASE
+
+// d: the fused pass descriptor (conv_passes_fused); mode 2 (real
+// spectrum) or 3 (complex); Lz the z embedding length.
+static int launch_yz(peec_ctx *ctx, int mode, PassDesc &d, const void *src, void *dst, int Lz)
+{
+    if (d.nb <= 0 || d.n2 <= 0 || d.nrhs <= 0) return PEEC_OK;
+    // (the kernel's plane prefetch holds nz * ny <= 8 LPT * 128 elements)
+    if (d.L > kYzMaxLy || d.n2 > kYzMaxLines || d.nin > kYzMaxLy / 2 || yz_lds(d.L, d.n2) > kFftLdsMax ||
+        !reg_length(Lz, true) || d.n2 > Lz || !d.spec) {
+        peec_set_error("fused y-z pass does not apply (Ly %d, %d z lines, z length %d)", d.L, d.n2, Lz);
+        return PEEC_ERR_ARG;
+    }
+    const bool sixteen = d.n2 <= 16;  // tile of 16 lines (LPT 2) or 32 (LPT 4), 32 lanes per line
+    if (mode == 2) return sixteen ? launch_yz_t<2, 2>(ctx, d, src, dst, Lz) : launch_yz_t<2, 4>(ctx, d, src, dst, Lz);
+    return sixteen ? launch_yz_t<3, 2>(ctx, d, src, dst, Lz) : launch_yz_t<3, 4>(ctx, d, src, dst, Lz);
+}
+
+// Plane-order FP32 spectrum of an application that the fused pass can
+// take (the even kernel's real spectrum or the half-cell reduction's
+// complex one).
+static int yz_spectrum_build(peec_ctx *ctx, const ConvApp &app)
+{
+    Kernel &K = ctx->kern[app.kernel];
+    const int64_t G = (int64_t)app.L[0] * app.L[1] * app.L[2];
+    const unsigned nb = (unsigned)((G + 255) / 256);
+    if (app.poly) {
+        if (!K.pfspec32.alloc((size_t)(2 * G))) {
+            peec_set_error("operator_set: out of device memory for a plane-order spectrum");
+            return PEEC_ERR_NOMEM;
+        }
+        hipLaunchKernelGGL((k_spec_planes<float, 2>), dim3(nb), dim3(256), 0, ctx->stream, K.pspec32.p, app.L[0],
+                           app.L[1], app.L[2], K.pfspec32.p);
+    } else {
+        if (!K.fspec32.alloc((size_t)G)) {
+            peec_set_error("operator_set: out of device memory for a plane-order spectrum");
+            return PEEC_ERR_NOMEM;
+        }
+        hipLaunchKernelGGL((k_spec_planes<float, 1>), dim3(nb), dim3(256), 0, ctx->stream, K.spec32.p, app.L[0],
+                           app.L[1], app.L[2], K.fspec32.p);
+    }
+    PEEC_HIP_TRY(hipGetLastError());
+    return PEEC_OK;
+}
+
+// After peec_operator_set has laid out the applications: plane-order
+// spectra (and the z twiddles) for every one the fused pass can take.
+int yz_prepare(peec_ctx *ctx)
+{
+    for (int a = 0; a < ctx->napps; a++) {
+        const ConvApp &app = ctx->apps[a];
+        if (!yz_fits(app, true) || app.compact() <= 0) continue;
+        if (ctx->twiddle_offset(app.L[2]) < 0) {
+            peec_set_error("operator_set: twiddle table allocation failed");
+            return PEEC_ERR_NOMEM;
+        }
+        int rc = yz_spectrum_build(ctx, app);
+        if (rc) return rc;
+    }
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PEEC_OK;
+}
+
 // compact dims of each circulant kernel
 static void kernel_dims(int kind, int Nx, int Ny, int Nz, int d[3])
 {
@@ -1038,6 +1277,7 @@ int peec_kernels_build_impl(peec_ctx *ctx, const double origin[3], int Nx, int N
     for (int k = 0; k < 9; k++) {
         if (ctx->stop_requested()) return PEEC_STOPPED;
         Kernel &K = ctx->kern[k];
+        K.fspec32.reset();  // plane-order copy: rebuilt by the next peec_operator_set
         int64_t g = K.compact();
         if (g <= 0) continue;
         if (!K.row.alloc((size_t)g) || !K.spec64.alloc((size_t)K.embedded()) || !K.spec32.alloc((size_t)K.embedded())) {
@@ -1154,6 +1394,62 @@ static int conv_passes(peec_ctx *ctx, const ConvApp &app, bool fp32, const doubl
     return PEEC_OK;
 }
 
+// The three pass descriptors of one application through the fused y-z
+// pass (complex64): x forward with scatter writing planes [x'][z][y], the
+// fused pass in place on the planes, x inverse with gather reading them.
+static int conv_passes_fused(peec_ctx *ctx, const ConvApp &app, const double2_t *vec_in, double2_t *vec_out,
+                             int nrhs, int64_t ld, void *w1, const int style[3], ConvPasses *cp)
+{
+    const Kernel &K = ctx->kern[app.kernel];
+    const int nx = app.n[0], ny = app.n[1], nz = app.n[2];
+    const int64_t Lx = app.L[0], Ly = app.L[1];
+    const int64_t pl = (int64_t)ny * nz, s1 = Lx * pl;  // plane size, per-rhs grid size
+    const double e = 8.0, es = 4.0 * (app.poly ? 2.0 : 1.0);
+    const double g = (double)nx * ny * nz, G1 = (double)s1 * nrhs;
+    int rc;
+    PassGeom g1 = {(int)Lx, nx, (int)Lx, ny, nz, nx, (int64_t)nx * ny, 1, 1, ny, pl};
+    PassDesc &d1 = cp->d[0];
+    if ((rc = make_desc(ctx, g1, &d1, style[0]))) return rc;
+    d1.map_in = app.map_in.p;
+    d1.vec_in = vec_in + app.in_off;
+    d1.vec_in_ld = ld;
+    d1.nrhs = nrhs;
+    d1.in_rhs = (int64_t)nx * ny * nz;
+    d1.out_rhs = s1;
+    cp->src[0] = nullptr;
+    cp->dst[0] = w1;
+    cp->bytes[0] = 4.0 * g + 16.0 * app.nnz_in * nrhs + e * G1;
+    PassGeom g2 = {(int)Ly, ny, ny, (int)Lx, nz, 0, pl, 0, 0, pl, 0};
+    PassDesc &d2 = cp->d[1];
+    if ((rc = make_desc(ctx, g2, &d2, style[1]))) return rc;
+    d2.nrhs = nrhs;
+    d2.in_rhs = s1;
+    d2.out_rhs = s1;
+    d2.spec = app.poly ? (const void *)K.pfspec32.p : (const void *)K.fspec32.p;
+    d2.spec_conj = app.conj ? 1 : 0;
+    d2.tw_off_z = ctx->twiddle_offset(app.L[2]);
+    if (d2.tw_off_z < 0) {
+        peec_set_error("twiddle table allocation failed");
+        return PEEC_ERR_NOMEM;
+    }
+    cp->src[1] = w1;
+    cp->dst[1] = w1;
+    cp->bytes[1] = 2.0 * e * G1 + es * (double)app.embedded();
+    PassGeom g3 = {(int)Lx, (int)Lx, nx, ny, nz, 1, ny, pl, nx, (int64_t)nx * ny, 1};
+    PassDesc &d3 = cp->d[2];
+    if ((rc = make_desc(ctx, g3, &d3, style[2]))) return rc;
+    d3.nrhs = nrhs;
+    d3.in_rhs = s1;
+    d3.out_rhs = (int64_t)nx * ny * nz;
+    d3.map_out = app.map_out.p;
+    d3.vec_out = vec_out + app.out_off;
+    d3.vec_out_ld = ld;
+    cp->src[2] = w1;
+    cp->dst[2] = nullptr;
+    cp->bytes[2] = e * G1 + 4.0 * g + 32.0 * app.nnz_out * nrhs;
+    return PEEC_OK;
+}
+
 // Right-hand sides per block when untuned: enough blocks (about 2048) to
 // fill the chip; the rest are pipelined through each block (and share its
 // spectrum tile in the fused z pass).
@@ -1173,6 +1469,60 @@ static const int kProfKind[5] = {PK_FFT_X_FWD, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_F
 static const int kPassMode[5] = {0, 0, 2, 1, 1};
 static int pass_mode(const ConvApp &app, int p) { return p == 2 && app.poly ? 3 : kPassMode[p]; }
 
+// Tuning-table passes of the fused pipeline: 5 x forward (plane output),
+// 6 the fused y-z pass (variant 0: use the fused pipeline, 1: the five
+// passes), 7 x inverse (plane input).
+static constexpr int kPassXFwdPlanes = 5, kPassYz = 6, kPassXInvPlanes = 7;
+
+static int default_yz_rpb(const PassDesc &d)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>(16, (int64_t)d.nb * d.nrhs / 2048));
+}
+
+// The fused pipeline runs when it fits, its plane-order spectrum exists,
+// and the tuner (or peec_fft_set_choice) chose it for this nrhs; untuned,
+// the five passes run.  On the 96 x 96 x 24 workload (z lengths 48 and 64,
+// 64 right-hand sides) it measured 1.3-2.1x slower than the five passes:
+// its z registers (spectrum + line) hold it to one wave per SIMD, and a
+// block's LDS stages and barriers then leave HBM idle.
+static bool use_fused(const peec_ctx *ctx, const ConvApp &app, bool fp32, int nrhs)
+{
+    if (!yz_fits(app, fp32)) return false;
+    const Kernel &K = ctx->kern[app.kernel];
+    if (app.poly ? !K.pfspec32.p : !K.fspec32.p) return false;
+    return tuned_choice(ctx, app.kernel, kPassYz, fp32, nrhs).variant == 0;
+}
+
+static int conv_apply_fused(peec_ctx *ctx, const ConvApp &app, const double2_t *vec_in, double2_t *vec_out, int nrhs,
+                            int64_t ld, void *w1)
+{
+    FftChoice ch[3];
+    int style[3];
+    for (int p = 0; p < 3; p++) {
+        ch[p] = tuned_choice(ctx, app.kernel, kPassXFwdPlanes + p, true, nrhs);
+        style[p] = ch[p].style;
+    }
+    ConvPasses cp;
+    int rc = conv_passes_fused(ctx, app, vec_in, vec_out, nrhs, ld, w1, style, &cp);
+    if (rc) return rc;
+    for (int p = 0; p < 3; p += 2) {
+        PassDesc &d = cp.d[p];
+        const int v = ch[p].variant >= 0 ? ch[p].variant : default_variant(d.L, true);
+        d.rpb = ch[p].variant >= 0 ? ch[p].rpb : default_rpb(d, std::max(v, 0));
+        const int pi = ctx->prof_begin(p == 0 ? PK_FFT_X_FWD : PK_FFT_X_INV, cp.bytes[p]);
+        if ((rc = launch_pass(ctx, true, p == 0 ? 0 : 1, d, cp.src[p], cp.dst[p], v))) return rc;
+        ctx->prof_end(pi);
+        if (p == 0) {
+            PassDesc &y = cp.d[1];
+            y.rpb = ch[1].variant >= 0 ? ch[1].rpb : default_yz_rpb(y);
+            const int pj = ctx->prof_begin(PK_FFT_YZ_CONV, cp.bytes[1]);
+            if ((rc = launch_yz(ctx, pass_mode(app, 2), y, cp.src[1], cp.dst[1], app.L[2]))) return rc;
+            ctx->prof_end(pj);
+        }
+    }
+    return PEEC_OK;
+}
+
 // One convolution application: vec_out[section] += gather(conv(scatter(vec_in)))
 // for nrhs right-hand sides (vectors at stride ld).  w1/w2 are grid work
 // buffers sized by conv_work_elems.  Launch shapes and radix schedules come
@@ -1181,6 +1531,7 @@ int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *ve
                int nrhs, int64_t ld, void *w1, void *w2)
 {
     if (app.compact() <= 0 || app.nnz_in == 0 || app.nnz_out == 0) return PEEC_OK;
+    if (use_fused(ctx, app, fp32, nrhs)) return conv_apply_fused(ctx, app, vec_in, vec_out, nrhs, ld, w1);
     FftChoice ch[5];
     int style[5];
     for (int p = 0; p < 5; p++) {
@@ -1216,6 +1567,102 @@ static int time_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc d, const void 
     float t = 0.f;
     PEEC_HIP_TRY(hipEventElapsedTime(&t, a, b));
     *ms = t / (float)reps;
+    return PEEC_OK;
+}
+
+static int time_yz(peec_ctx *ctx, int mode, PassDesc d, const void *src, void *dst, int Lz, int reps,
+                   hipEvent_t a, hipEvent_t b, float *ms)
+{
+    int rc;
+    if ((rc = launch_yz(ctx, mode, d, src, dst, Lz))) return rc;
+    PEEC_HIP_TRY(hipEventRecord(a, ctx->stream));
+    for (int i = 0; i < reps; i++)
+        if ((rc = launch_yz(ctx, mode, d, src, dst, Lz))) return rc;
+    PEEC_HIP_TRY(hipEventRecord(b, ctx->stream));
+    PEEC_HIP_TRY(hipEventSynchronize(b));
+    float t = 0.f;
+    PEEC_HIP_TRY(hipEventElapsedTime(&t, a, b));
+    *ms = t / (float)reps;
+    return PEEC_OK;
+}
+
+// Tune one pass: 1) launch shape at the default radix schedule and rhs
+// grouping (and the register z kernel where allowed), 2) right-hand sides
+// per block, 3) radix schedule (FP32 passes only: the FP64 path keeps one
+// fixed operation order; the register kernel unrolls the radix-8-first
+// schedule).
+static int tune_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc d, const void *src, void *dst, double bytes,
+                     bool allow_reg, bool verbose, int kernel, int pass, hipEvent_t ea, hipEvent_t eb, FftChoice *out,
+                     float *tout)
+{
+    FftChoice best;
+    float tbest = 1e30f;
+    int rc = PEEC_OK;
+    for (int v = 0; v < kNumFftVariants && rc == PEEC_OK; v++) {
+        if (!variant_fits(v, d.L, fp32)) continue;
+        d.rpb = default_rpb(d, v);
+        float t;
+        if ((rc = time_pass(ctx, fp32, mode, d, src, dst, v, 3, ea, eb, &t))) break;
+        if (verbose)
+            fprintf(stderr, "peec_fft_tune: kernel %d pass %d %s L %d nin %d nout %d: shape %d (W %d TPL %d) "
+                            "rpb %d: %.1f us, %.0f GB/s\n", kernel, pass, fp32 ? "c64" : "c128", d.L, d.nin, d.nout, v,
+                    variant_lines(v), kFftVariants[v].tpl, d.rpb, 1e3 * t, bytes / (1e6 * t));
+        if (t < tbest) {
+            tbest = t;
+            best.variant = v;
+            best.rpb = d.rpb;
+            best.style = 0;
+        }
+    }
+    if (rc) return rc;
+    if (allow_reg && mode >= 2 && reg_length(d.L, fp32) && d.radix[0] == RegFirstRadix(d.L)) {
+        d.rpb = default_rpb(d, 0);
+        float t;
+        if ((rc = time_pass(ctx, fp32, mode, d, src, dst, kRegVariant, 3, ea, eb, &t))) return rc;
+        if (verbose)
+            fprintf(stderr, "peec_fft_tune: kernel %d pass %d %s L %d: register z kernel rpb %d: %.1f us, %.0f GB/s\n",
+                    kernel, pass, fp32 ? "c64" : "c128", d.L, d.rpb, 1e3 * t, bytes / (1e6 * t));
+        if (t < tbest) {
+            tbest = t;
+            best.variant = kRegVariant;
+            best.rpb = d.rpb;
+            best.style = 0;
+        }
+    }
+    if (best.variant < 0) {
+        peec_set_error("peec_fft_tune: no launch shape for length %d", d.L);
+        return PEEC_ERR_ARG;
+    }
+    for (int r = 1; r <= 32 && r <= d.nrhs; r *= 2) {
+        d.rpb = r;
+        float t;
+        if ((rc = time_pass(ctx, fp32, mode, d, src, dst, best.variant, 3, ea, eb, &t))) return rc;
+        if (verbose) fprintf(stderr, "peec_fft_tune:   rpb %d: %.1f us, %.0f GB/s\n", r, 1e3 * t, bytes / (1e6 * t));
+        if (t < tbest) {
+            tbest = t;
+            best.rpb = r;
+        }
+    }
+    if (fp32 && best.variant != kRegVariant) {
+        for (int st = 1; st <= 2; st++) {
+            PassDesc ds;
+            PassGeom gg = {d.L, d.nin, d.nout, d.nb, d.n2, d.in_sb, d.in_s2, d.in_sj, d.out_sb, d.out_s2, d.out_sj};
+            if ((rc = make_desc(ctx, gg, &ds, st))) return rc;
+            if (ds.nstage == d.nstage && !memcmp(ds.radix, d.radix, sizeof(int) * (size_t)d.nstage)) continue;
+            PassDesc dd = d;
+            dd.nstage = ds.nstage;
+            memcpy(dd.radix, ds.radix, sizeof(dd.radix));
+            dd.rpb = best.rpb;
+            float t;
+            if ((rc = time_pass(ctx, fp32, mode, dd, src, dst, best.variant, 3, ea, eb, &t))) return rc;
+            if (t < tbest) {
+                tbest = t;
+                best.style = st;
+            }
+        }
+    }
+    *out = best;
+    *tout = tbest;
     return PEEC_OK;
 }
 
@@ -1264,89 +1711,56 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
             const int style0[5] = {0, 0, 0, 0, 0};
             ConvPasses cp;
             if ((rc = conv_passes(ctx, app, fp32, vin.p, vout.p, nrhs, N, ctx->w1.p, ctx->w2.p, style0, &cp))) break;
+            float t5 = 0.f;
             for (int p = 0; p < 5 && rc == PEEC_OK; p++) {
-                const int mode = pass_mode(app, p);
-                PassDesc d = cp.d[p];
                 FftChoice best;
                 float tbest = 1e30f;
-                // 1) launch shape at the default radix schedule and rhs grouping
-                for (int v = 0; v < kNumFftVariants && rc == PEEC_OK; v++) {
-                    if (!variant_fits(v, d.L, fp32)) continue;
-                    d.rpb = default_rpb(d, v);
+                if ((rc = tune_pass(ctx, fp32, pass_mode(app, p), cp.d[p], cp.src[p], cp.dst[p], cp.bytes[p], p == 2,
+                                    verbose, app.kernel, p, ea, eb, &best, &tbest)))
+                    break;
+                ctx->fft_tune[FftTuneKey{app.kernel, p, fp32 ? 1 : 0, nrhs}] = best;
+                t5 += tbest;
+            }
+            if (rc) break;
+            float tsum = t5;
+            // the fused pipeline (x forward into planes, fused y-z, x inverse)
+            const Kernel &K = ctx->kern[app.kernel];
+            if (yz_fits(app, fp32) && (app.poly ? K.pfspec32.p : K.fspec32.p)) {
+                const int st3[3] = {0, 0, 0};
+                ConvPasses cf;
+                if ((rc = conv_passes_fused(ctx, app, vin.p, vout.p, nrhs, N, ctx->w1.p, st3, &cf))) break;
+                FftChoice bx[2], by;
+                float tx[2] = {1e30f, 1e30f}, ty = 1e30f;
+                for (int q = 0; q < 2 && rc == PEEC_OK; q++)
+                    rc = tune_pass(ctx, true, q == 0 ? 0 : 1, cf.d[2 * q], cf.src[2 * q], cf.dst[2 * q],
+                                   cf.bytes[2 * q], false, verbose, app.kernel, q == 0 ? kPassXFwdPlanes : kPassXInvPlanes,
+                                   ea, eb, &bx[q], &tx[q]);
+                if (rc) break;
+                for (int r = 1; r <= 32 && r <= nrhs && rc == PEEC_OK; r *= 2) {
+                    PassDesc d = cf.d[1];
+                    d.rpb = r;
                     float t;
-                    if ((rc = time_pass(ctx, fp32, mode, d, cp.src[p], cp.dst[p], v, 3, ea, eb, &t))) break;
+                    if ((rc = time_yz(ctx, pass_mode(app, 2), d, cf.src[1], cf.dst[1], app.L[2], 3, ea, eb, &t))) break;
                     if (verbose)
-                        fprintf(stderr, "peec_fft_tune: kernel %d pass %d %s L %d nin %d nout %d: shape %d (W %d TPL %d) "
-                                        "rpb %d: %.1f us, %.0f GB/s\n", app.kernel, p, fp32 ? "c64" : "c128", d.L, d.nin,
-                                d.nout, v, variant_lines(v), kFftVariants[v].tpl, d.rpb, 1e3 * t,
-                                cp.bytes[p] / (1e6 * t));
-                    if (t < tbest) {
-                        tbest = t;
-                        best.variant = v;
-                        best.rpb = d.rpb;
-                        best.style = 0;
-                    }
-                }
-                if (rc == PEEC_OK && mode >= 2 && reg_length(d.L, fp32) && d.radix[0] == RegFirstRadix(d.L)) {
-                    d.rpb = default_rpb(d, 0);
-                    float t;
-                    if ((rc = time_pass(ctx, fp32, mode, d, cp.src[p], cp.dst[p], kRegVariant, 3, ea, eb, &t))) break;
-                    if (verbose)
-                        fprintf(stderr, "peec_fft_tune: kernel %d pass %d %s L %d: register z kernel rpb %d: %.1f us, "
-                                        "%.0f GB/s\n", app.kernel, p, fp32 ? "c64" : "c128", d.L, d.rpb, 1e3 * t,
-                                cp.bytes[p] / (1e6 * t));
-                    if (t < tbest) {
-                        tbest = t;
-                        best.variant = kRegVariant;
-                        best.rpb = d.rpb;
-                        best.style = 0;
-                    }
-                }
-                if (rc || best.variant < 0) break;
-                // 2) right-hand sides per block
-                {
-                    for (int r = 1; r <= 32 && r <= nrhs && rc == PEEC_OK; r *= 2) {
-                        d.rpb = r;
-                        float t;
-                        if ((rc = time_pass(ctx, fp32, mode, d, cp.src[p], cp.dst[p], best.variant, 3, ea, eb, &t)))
-                            break;
-                        if (verbose)
-                            fprintf(stderr, "peec_fft_tune:   rpb %d: %.1f us, %.0f GB/s\n", r, 1e3 * t,
-                                    cp.bytes[p] / (1e6 * t));
-                        if (t < tbest) {
-                            tbest = t;
-                            best.rpb = r;
-                        }
-                    }
-                }
-                // 3) radix schedule (FP32 passes only: the FP64 path keeps
-                // one fixed operation order; the register kernel unrolls
-                // the radix-8-first schedule)
-                if (fp32 && best.variant != kRegVariant) {
-                    for (int st = 1; st <= 2 && rc == PEEC_OK; st++) {
-                        PassDesc ds;
-                        PassGeom gg = {d.L, d.nin, d.nout, d.nb, d.n2, d.in_sb, d.in_s2, d.in_sj,
-                                       d.out_sb, d.out_s2, d.out_sj};
-                        if ((rc = make_desc(ctx, gg, &ds, st))) break;
-                        if (ds.nstage == d.nstage && !memcmp(ds.radix, d.radix, sizeof(int) * (size_t)d.nstage))
-                            continue;
-                        PassDesc dd = d;
-                        dd.nstage = ds.nstage;
-                        memcpy(dd.radix, ds.radix, sizeof(dd.radix));
-                        dd.rpb = best.rpb;
-                        float t;
-                        if ((rc = time_pass(ctx, fp32, mode, dd, cp.src[p], cp.dst[p], best.variant, 3, ea, eb, &t)))
-                            break;
-                        if (t < tbest) {
-                            tbest = t;
-                            best.style = st;
-                        }
+                        fprintf(stderr, "peec_fft_tune: kernel %d fused y-z Ly %d nz %d Lz %d rpb %d: %.1f us, %.0f GB/s\n",
+                                app.kernel, d.L, d.n2, app.L[2], r, 1e3 * t, cf.bytes[1] / (1e6 * t));
+                    if (t < ty) {
+                        ty = t;
+                        by.rpb = r;
                     }
                 }
                 if (rc) break;
-                ctx->fft_tune[FftTuneKey{app.kernel, p, fp32 ? 1 : 0, nrhs}] = best;
-                total_best += tbest;
+                const float t3 = tx[0] + ty + tx[1];
+                by.variant = t3 < t5 ? 0 : 1;
+                ctx->fft_tune[FftTuneKey{app.kernel, kPassXFwdPlanes, 1, nrhs}] = bx[0];
+                ctx->fft_tune[FftTuneKey{app.kernel, kPassYz, 1, nrhs}] = by;
+                ctx->fft_tune[FftTuneKey{app.kernel, kPassXInvPlanes, 1, nrhs}] = bx[1];
+                if (verbose)
+                    fprintf(stderr, "peec_fft_tune: kernel %d c64: five passes %.1f us, fused pipeline %.1f us -> %s\n",
+                            app.kernel, 1e3 * t5, 1e3 * t3, t3 < t5 ? "fused" : "five passes");
+                tsum = std::min(t3, t5);
             }
+            total_best += tsum;
             if (progress && rc == PEEC_OK)
                 fprintf(stderr, "peec_fft_tune: kernel %d %s tuned\n", app.kernel, fp32 ? "c64" : "c128");
         }
@@ -1362,9 +1776,10 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
 extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int variant, int rpb,
                                    int style)
 {
-    if (!ctx || kernel < 0 || kernel > 8 || pass < 0 || pass > 4 || nrhs < 1 || variant < -1 ||
-        variant > kRegVariant || rpb < 1 || style < 0 || style > 2 || (style != 0 && !fp32) ||
-        (variant == kRegVariant && (pass != 2 || style != 0))) {
+    const bool fused = pass >= kPassXFwdPlanes;
+    if (!ctx || kernel < 0 || kernel > 8 || pass < 0 || pass > kPassXInvPlanes || nrhs < 1 || variant < -1 ||
+        variant > kRegVariant || rpb < 1 || style < 0 || style > 2 || (style != 0 && !fp32) || (fused && !fp32) ||
+        (variant == kRegVariant && (pass != 2 || style != 0)) || (pass == kPassYz && variant > 1)) {
         peec_set_error("peec_fft_set_choice: invalid argument");
         return PEEC_ERR_ARG;
     }
@@ -1376,7 +1791,16 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
     const ConvApp *app = nullptr;
     for (int a = 0; a < ctx->napps && !app; a++)
         if (ctx->apps[a].kernel == kernel) app = &ctx->apps[a];
-    const int L = app ? app->L[pass == 0 || pass == 4 ? 0 : (pass == 2 ? 2 : 1)] : 0;
+    if (pass == kPassYz) {
+        if (!app || (variant == 0 && !yz_fits(*app, true))) {
+            peec_set_error("peec_fft_set_choice: the fused y-z pass does not apply to kernel %d", kernel);
+            return PEEC_ERR_ARG;
+        }
+        ctx->fft_tune[key] = FftChoice{variant, rpb, style};
+        return PEEC_OK;
+    }
+    const int axis = (pass == 0 || pass == 4 || fused) ? 0 : (pass == 2 ? 2 : 1);
+    const int L = app ? app->L[axis] : 0;
     if (!app || (variant == kRegVariant ? !(reg_length(L, fp32 != 0) && L % RegFirstRadix(L) == 0)
                                         : !variant_fits(variant, L, fp32 != 0))) {
         peec_set_error("peec_fft_set_choice: launch variant %d does not fit length %d", variant, L);
@@ -1390,12 +1814,12 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
 // rhs per block and radix schedule; -1 in variant when untuned.
 extern "C" int peec_fft_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int *out5)
 {
-    if (!ctx || kernel < 0 || kernel > 8 || pass < 0 || pass > 4 || !out5) {
+    if (!ctx || kernel < 0 || kernel > 8 || pass < 0 || pass > kPassXInvPlanes || !out5) {
         peec_set_error("peec_fft_choice: invalid argument");
         return PEEC_ERR_ARG;
     }
     const FftChoice c = tuned_choice(ctx, kernel, pass, fp32 != 0, nrhs);
-    const bool table = c.variant >= 0 && c.variant < kNumFftVariants;
+    const bool table = c.variant >= 0 && c.variant < kNumFftVariants && pass != kPassYz;
     out5[0] = c.variant;
     out5[1] = table ? variant_lines(c.variant) : (c.variant == kRegVariant ? 256 : 0);
     out5[2] = table ? kFftVariants[c.variant].tpl : (c.variant == kRegVariant ? 1 : 0);

@@ -46,6 +46,8 @@ void peec_ctx::release_operator()
         k.poly = false;
         k.pspec64.reset();
         k.pspec32.reset();
+        k.fspec32.reset();
+        k.pfspec32.reset();
     }
 }
 
@@ -53,6 +55,7 @@ int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *ve
                int64_t ld, void *w1, void *w2);
 int64_t conv_work_elems(const ConvApp &app);
 int poly_spectrum_build(peec_ctx *ctx, int k, const int delta[3]);
+int yz_prepare(peec_ctx *ctx);
 extern const int kHalfAxes[9][3];
 
 namespace {
@@ -571,6 +574,7 @@ extern "C" int peec_operator_set(peec_ctx *ctx, int64_t mx, int64_t my, int64_t 
         peec_set_error("peec_operator_set: out of device memory");
         return PEEC_ERR_NOMEM;
     }
+    if ((rc = yz_prepare(ctx))) return rc;
     ctx->op_ready = true;
     return PEEC_OK;
 }

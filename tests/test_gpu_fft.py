@@ -1,12 +1,16 @@
-"""FFT-convolution launch shapes and the autotuner.
+"""FFT-convolution launch shapes, the fused y-z pass and the autotuner.
 
 Every launch shape of the pass kernel (lines per tile, lanes per line,
 right-hand sides per block, hence also the prefetching of the next
 right-hand side) runs the same butterflies in the same order,
 so K x must come out bit for bit the same under each of them, in complex128
-and in complex64.  The complex64 radix schedules change rounding only:
-they stay within complex64 accuracy of the complex128 result.  The
-complex128 result itself is held to the oracle's K x (1e-12, relative).
+and in complex64.  The complex64 fused pipeline (x pass into planes, one y
+forward / z convolution / y inverse pass per plane, x pass back), which the
+tuner may choose, runs the same stages again: it must equal the five-pass
+path bit for bit, also with the 48- and 64-point z kernels.  The complex64
+radix schedules change rounding only: they stay within complex64 accuracy
+of the complex128 result.  The complex128 result itself is held to the
+oracle's K x (1e-12, relative).
 """
 import numpy as np
 import pytest
@@ -20,6 +24,7 @@ pytestmark = pytest.mark.gpu
 
 NRHS = 3
 N_VARIANTS = 12  # 11 LDS tile shapes + the register z kernel (fused z pass only)
+PASS_YZ = 6      # tuning-table pass of the fused y-z pass: variant 0 fused, 1 five passes
 
 
 @pytest.fixture(scope="module")
@@ -46,25 +51,38 @@ def bitwise_equal(a, b):
     return np.array_equal(np.ascontiguousarray(a).view(np.float64), np.ascontiguousarray(b).view(np.float64))
 
 
-def set_all(S, fp32, variant, rpb, style):
-    """Force one launch shape on every pass where it fits; returns how many
-    passes took it."""
+def set_all(S, fp32, variant, rpb, style, nrhs=NRHS):
+    """Force one launch shape on every one of the five passes where it fits
+    (and, in complex64, the five-pass path instead of the fused one);
+    returns how many passes took it."""
     took = 0
     for k in range(9):
         for p in range(5):
             try:
-                peec.fft_set_choice(k, p, fp32, NRHS, variant, rpb, style, ctx=S.ctx)
+                peec.fft_set_choice(k, p, fp32, nrhs, variant, rpb, style, ctx=S.ctx)
                 took += 1
             except _lib.PeecError:
-                peec.fft_set_choice(k, p, fp32, NRHS, -1, ctx=S.ctx)
+                peec.fft_set_choice(k, p, fp32, nrhs, -1, ctx=S.ctx)
+        if fp32:
+            set_fused(S, k, False, nrhs=nrhs)
     return took
 
 
-def clear_all(S):
+def set_fused(S, k, on, rpb=1, nrhs=NRHS):
+    """Fused y-z pipeline on or off for kernel k (complex64); False when it
+    does not apply."""
+    try:
+        peec.fft_set_choice(k, PASS_YZ, True, nrhs, 0 if on else 1, rpb, 0, ctx=S.ctx)
+        return True
+    except _lib.PeecError:
+        return False
+
+
+def clear_all(S, nrhs=NRHS):
     for fp32 in (False, True):
         for k in range(9):
-            for p in range(5):
-                peec.fft_set_choice(k, p, fp32, NRHS, -1, ctx=S.ctx)
+            for p in range(8 if fp32 else 5):
+                peec.fft_set_choice(k, p, fp32, nrhs, -1, ctx=S.ctx)
 
 
 def test_default_shapes_match_oracle(batched):
@@ -121,3 +139,68 @@ def test_fp32_radix_schedules_and_tuner(batched):
     assert bitwise_equal(y64t, y64)
     y32t = peec.apply_K_batched(X, 1, fp32=True, ctx=S.ctx)
     assert rel(y32t, y64) < 1e-5
+
+
+def fused_profile(S, X, rps):
+    """K x in complex64 with the per-kind launch counts of that one call."""
+    nk = 10
+    import ctypes
+    ms, by, cnt = (ctypes.c_double * nk)(), (ctypes.c_double * nk)(), (ctypes.c_int64 * nk)()
+    _lib.check(_lib.lib().peec_profile(S.ctx.handle, 1, ms, by, cnt, nk), "peec_profile")
+    Y = peec.apply_K_batched(X, rps, fp32=True, ctx=S.ctx)
+    _lib.check(_lib.lib().peec_profile(S.ctx.handle, 0, ms, by, cnt, nk), "peec_profile")
+    return Y, list(cnt)
+
+
+def test_fused_yz_matches_five_passes(batched):
+    M, S, ws, X = batched
+    clear_all(S)
+    five, cnt5 = fused_profile(S, X, 1)
+    assert cnt5[9] == 0 and cnt5[1] > 0  # untuned: the five passes
+    for k in range(9):
+        assert set_fused(S, k, True)
+    fused, cnt = fused_profile(S, X, 1)
+    print("fused complex64 launches per kind", cnt, flush=True)
+    assert cnt[9] > 0 and cnt[1] == 0  # fused y-z passes ran, no separate y pass
+    print("fused vs five passes: bitwise %s (rel diff %.2e)" % (bitwise_equal(fused, five), rel(fused, five)),
+          flush=True)
+    assert bitwise_equal(fused, five)
+    for rpb in (1, 2, 3):
+        for k in range(9):
+            set_fused(S, k, True, rpb)
+        assert bitwise_equal(peec.apply_K_batched(X, 1, fp32=True, ctx=S.ctx), five)
+    clear_all(S)
+
+
+def test_fused_yz_long_z_lines():
+    """24 z cells: the 48- and 64-point register z kernels inside the fused
+    pass (real and complex spectra), 16 right-hand sides over 2 slots."""
+    cell = (0.25e-3, 0.25e-3, 0.05e-3)
+    g, mats, _ = configs._layered(12, 10, 24, cell, 18, 0.5, 4.4, 0.02, seed=5)
+    ports = configs._x_face_ports(g, cell, 18)
+    M = mesher.mesh(g, mats, cell, (0.0, 0.0, 0.0), ports, p_self_fn=pself)
+    S = sweep.Solver(M, ctx=peec.Context(0), device_lu=True)
+    nrhs, rps = 16, 8
+    peec.frequency_slots(2, ctx=S.ctx)
+    for slot, f in enumerate((1e8, 5e9)):
+        w = 2 * np.pi * f * M.escalings["freq"]
+        Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+        peec.frequency_set_slot(slot, w, sweep.DZ_of(M, w), Yle, ctx=S.ctx)
+    N = M.m + M.ns + M.n
+    rng = np.random.default_rng(23)
+    X = rng.normal(size=(N, nrhs)) + 1j * rng.normal(size=(N, nrhs))
+    y64 = peec.apply_K_batched(X, rps, fp32=False, ctx=S.ctx)
+    five = peec.apply_K_batched(X, rps, fp32=True, ctx=S.ctx)
+    for k in range(9):
+        assert set_fused(S, k, True, nrhs=nrhs)
+    fused, cnt = fused_profile(S, X, rps)
+    assert cnt[9] > 0
+    print("nz 24: fused vs five passes bitwise %s (rel diff %.2e); vs complex128 %.2e"
+          % (bitwise_equal(fused, five), rel(fused, five), rel(fused, y64)), flush=True)
+    assert bitwise_equal(fused, five)
+    assert rel(fused, y64) < 1e-5
+    ms = peec.fft_tune(nrhs, ctx=S.ctx)
+    fused_on = [peec.fft_choice(k, PASS_YZ, True, nrhs, ctx=S.ctx)[0] for k in range(9)]
+    print("tuned %.3f ms; fused choice per kernel %s" % (ms, fused_on), flush=True)
+    # the tuner may pick other complex64 radix schedules: rounding changes
+    assert rel(peec.apply_K_batched(X, rps, fp32=True, ctx=S.ctx), y64) < 1e-5
