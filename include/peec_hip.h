@@ -215,6 +215,27 @@ int peec_gmres_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const peec_c12
  * (FFT_solver_QS_S_type.jl:237-262) batched. */
 int peec_precond_rhs_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const double *is, peec_c128 *tn);
 
+/* One batch of the quasi-static sweep kept on the device (no counterpart in
+ * the reference; the body of its frequency x port loop,
+ * FFT_solver_QS_S_type.jl:120-206, for every frequency slot at once).  For
+ * slot s and port c (right-hand side r = s * nports + c), the excitation
+ * is = is_value (e_n1 - e_n2) with port_nodes[2c], port_nodes[2c + 1]
+ * (0-based node indices, :124-130), its preconditioned right-hand side
+ * M^-1 [0; 0; is] (peec_precond_rhs_batched) and the solve of
+ * peec_gmres_batched, all without leaving the device.  The warm start of
+ * every slot is x0 (N x nports, host) when given, else the previous call's
+ * last-slot solutions, which the context keeps (:140, :196), else zero.
+ * Returns vports[r * nports + c2] = x_r[n1(c2)] - x_r[n2(c2)] (the node
+ * voltage differences s2z needs, :200-206), optionally all solutions
+ * (x_out: N x nrhs, or NULL), and per right-hand side flag, relres, iters
+ * and cycles as peec_gmres_batched. */
+int peec_sweep_batch(peec_ctx *ctx, int nports, const int64_t *port_nodes, double is_value, const peec_c128 *x0,
+                     double tol, int restart, int max_cycles, int mixed, double inner_tol, peec_c128 *vports,
+                     peec_c128 *x_out, int *flag, double *relres, int *iters, int *cycles);
+/* The last-slot solutions peec_sweep_batch keeps (N x nports) copied to x:
+ * the warm start to hand to a later call or another context. */
+int peec_sweep_state(peec_ctx *ctx, int nports, peec_c128 *x);
+
 /* FFT-pass autotuner (no counterpart in the reference, whose FFTW plans are
  * made with FFTW's defaults, compute_Matrix_vector_fft.jl:20-24).  For every
  * convolution kernel of the current operator and each precision in
@@ -222,10 +243,11 @@ int peec_precond_rhs_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const do
  * timing to stderr, bit 3 one line per tuned kernel), times the five passes
  * of one convolution over nrhs right-hand sides under each launch shape of
  * the library's table (lines per tile, lanes per line; for the fused z pass
- * also variant 11, the kernel that holds a whole z line in registers), then the
- * right-hand sides per block of the fused z pass, then (complex64 only) the
- * radix schedule, and keeps the fastest for batched calls with exactly
- * nrhs right-hand sides.  Launch shapes do not change any result; a
+ * also variant 11, the kernel that holds a whole z line in registers), then
+ * the right-hand sides per block, then (complex64 only) the radix schedule;
+ * in complex64 it also times the fused pipeline (x pass into planes, one
+ * y-z pass per plane, x pass back) where it applies, and keeps the faster
+ * of the two for batched calls with exactly nrhs right-hand sides.  Launch shapes do not change any result; a
  * complex64 radix schedule changes rounding only.  *report_ms (optional)
  * receives the summed best pass times.  The choices last until the next
  * peec_kernels_build or peec_operator_set. */

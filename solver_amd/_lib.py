@@ -70,6 +70,9 @@ def lib():
         L.peec_gmres_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, c128p, ctypes.c_double, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_double, c128p, ip, dp, ip, ip]
         L.peec_precond_rhs_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, c128p]
+        L.peec_sweep_batch.argtypes = [vp, ctypes.c_int, i64p, ctypes.c_double, c128p, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_double, c128p, c128p, ip, dp, ip, ip]
+        L.peec_sweep_state.argtypes = [vp, ctypes.c_int, c128p]
         L.peec_apply_K_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c128p, c128p]
         L.peec_apply_KH_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c128p, c128p]
         L.peec_precond_rhs_batched_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int, c128p, c128p]

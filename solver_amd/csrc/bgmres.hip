@@ -193,6 +193,29 @@ __global__ void k_bscatter(cd *dst, int64_t dld, const cd *src, int64_t sld, con
 }
 
 // r = b - y
+// Port excitations of a sweep batch: column r holds Is at node n1 and -Is at
+// node n2 of port r % nports (node block at offset off); the rest is zero.
+__global__ void k_port_rhs(cd *B, int64_t N, int64_t off, const int64_t *pn, int nports, int nrhs, double Is)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrhs) return;
+    const int c = r % nports;
+    cd *b = B + (int64_t)r * N + off;
+    b[pn[2 * c]] = {Is, 0.0};
+    b[pn[2 * c + 1]] = {-Is, 0.0};
+}
+
+// v[r * nports + c2] = x_r[n3] - x_r[n4] for the nodes (n3, n4) of port c2.
+__global__ void k_port_volts(const cd *X, int64_t N, int64_t off, const int64_t *pn, int nports, int nrhs, cd *v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrhs * nports) return;
+    const int r = i / nports, c2 = i % nports;
+    const cd *x = X + (int64_t)r * N + off;
+    const cd a = x[pn[2 * c2]], b = x[pn[2 * c2 + 1]];
+    v[i] = {a.x - b.x, a.y - b.y};
+}
+
 __global__ void k_bsub(cd *r, const cd *b, const cd *y, int64_t total)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -342,37 +365,28 @@ extern "C" int peec_gmres_batched_adjoint(peec_ctx *ctx, int nrhs, int rhs_per_s
                               relres, iters, cycles);
 }
 
-static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per_slot, const peec_c128 *b,
-                              double tol, int restart, int max_cycles, int mixed, double inner_tol, peec_c128 *x,
-                              int *flag, double *relres, int *iters, int *cycles)
+// Per right-hand side results of a batched solve.
+struct BOut {
+    std::vector<int> fl, it, cyc;
+    std::vector<double> nb, beta;
+    void report(int nrhs, int *flag, double *relres, int *iters, int *cycles) const
+    {
+        for (int r = 0; r < nrhs; r++) {
+            if (flag) flag[r] = fl[(size_t)r];
+            if (relres) relres[r] = nb[(size_t)r] > 0 ? beta[(size_t)r] / nb[(size_t)r] : 0.0;
+            if (iters) iters[r] = it[(size_t)r];
+            if (cycles) cycles[r] = cyc[(size_t)r];
+        }
+    }
+};
+
+// The batched solve on W.B and W.X (device, N x nrhs at stride N): W.X holds
+// the start and receives the solution.
+static int bgmres_solve(peec_ctx *ctx, BWork &W, double tol, int max_cycles, int mixed, double inner_tol, BOut &o)
 {
-    if (!ctx || nrhs < 1 || rhs_per_slot < 1 || !b || !x || restart < 1 || max_cycles < 1) {
-        peec_set_error("peec_gmres_batched: invalid argument");
-        return PEEC_ERR_ARG;
-    }
-    if (!ctx->op_ready || !ctx->freq_ready || !ctx->lu_ready) {
-        peec_set_error("peec_gmres_batched: operator, frequency slots and factors must be set first");
-        return PEEC_ERR_STATE;
-    }
-    if ((nrhs + rhs_per_slot - 1) / rhs_per_slot > ctx->nslots) {
-        peec_set_error("peec_gmres_batched: %d right-hand sides at %d per slot need more than the %d slots", nrhs,
-                       rhs_per_slot, ctx->nslots);
-        return PEEC_ERR_ARG;
-    }
-    PEEC_HIP_TRY(hipSetDevice(ctx->device));
-    BWork W;
-    W.ctx = ctx;
-    W.adjoint = adjoint;
-    W.nrhs = nrhs;
-    W.rps = rhs_per_slot;
-    W.m = restart;
-    W.N = ctx->m + ctx->ns + ctx->n;
+    const int nrhs = W.nrhs, restart = W.m;
     const int64_t N = W.N;
-    int rc = W.alloc();
-    if (rc) return rc;
-    const size_t bytes = (size_t)nrhs * (size_t)N * 16;
-    PEEC_HIP_TRY(hipMemcpyAsync(W.B.p, b, bytes, hipMemcpyHostToDevice, ctx->stream));
-    PEEC_HIP_TRY(hipMemcpyAsync(W.X.p, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+    int rc = PEEC_OK;
     std::vector<double> nb, beta;
     if ((rc = W.norms(W.B.p, N, nb))) return rc;
     // r = b - A x for the listed columns (the others keep r and beta)
@@ -543,14 +557,143 @@ static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per
             }
         }
     }
+    o.fl = fl;
+    o.it = it;
+    o.cyc = cyc;
+    o.nb = nb;
+    o.beta = beta;
+    return PEEC_OK;
+}
+
+static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per_slot, const peec_c128 *b,
+                              double tol, int restart, int max_cycles, int mixed, double inner_tol, peec_c128 *x,
+                              int *flag, double *relres, int *iters, int *cycles)
+{
+    if (!ctx || nrhs < 1 || rhs_per_slot < 1 || !b || !x || restart < 1 || max_cycles < 1) {
+        peec_set_error("peec_gmres_batched: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->op_ready || !ctx->freq_ready || !ctx->lu_ready) {
+        peec_set_error("peec_gmres_batched: operator, frequency slots and factors must be set first");
+        return PEEC_ERR_STATE;
+    }
+    if ((nrhs + rhs_per_slot - 1) / rhs_per_slot > ctx->nslots) {
+        peec_set_error("peec_gmres_batched: %d right-hand sides at %d per slot need more than the %d slots", nrhs,
+                       rhs_per_slot, ctx->nslots);
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    BWork W;
+    W.ctx = ctx;
+    W.adjoint = adjoint;
+    W.nrhs = nrhs;
+    W.rps = rhs_per_slot;
+    W.m = restart;
+    W.N = ctx->m + ctx->ns + ctx->n;
+    const int64_t N = W.N;
+    int rc = W.alloc();
+    if (rc) return rc;
+    const size_t bytes = (size_t)nrhs * (size_t)N * 16;
+    PEEC_HIP_TRY(hipMemcpyAsync(W.B.p, b, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PEEC_HIP_TRY(hipMemcpyAsync(W.X.p, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+    BOut o;
+    if ((rc = bgmres_solve(ctx, W, tol, max_cycles, mixed, inner_tol, o))) return rc;
     PEEC_HIP_TRY(hipMemcpyAsync(x, W.X.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    for (int r = 0; r < nrhs; r++) {
-        if (flag) flag[r] = fl[(size_t)r];
-        if (relres) relres[r] = nb[(size_t)r] > 0 ? beta[(size_t)r] / nb[(size_t)r] : 0.0;
-        if (iters) iters[r] = it[(size_t)r];
-        if (cycles) cycles[r] = cyc[(size_t)r];
+    o.report(nrhs, flag, relres, iters, cycles);
+    return PEEC_OK;
+}
+
+// One batch of the quasi-static sweep kept on the device: for every
+// frequency slot and port, the port excitation, its preconditioned
+// right-hand side and the GMRES solve (warm-started from x0, or from the
+// previous call's last slot), returning only the port voltages.
+extern "C" int peec_sweep_batch(peec_ctx *ctx, int nports, const int64_t *port_nodes, double is_value,
+                                const peec_c128 *x0, double tol, int restart, int max_cycles, int mixed,
+                                double inner_tol, peec_c128 *vports, peec_c128 *x_out, int *flag, double *relres,
+                                int *iters, int *cycles)
+{
+    if (!ctx || nports < 1 || !port_nodes || !vports || restart < 1 || max_cycles < 1) {
+        peec_set_error("peec_sweep_batch: invalid argument");
+        return PEEC_ERR_ARG;
     }
+    if (!ctx->op_ready || !ctx->freq_ready || !ctx->lu_ready) {
+        peec_set_error("peec_sweep_batch: operator, frequency slots and factors must be set first");
+        return PEEC_ERR_STATE;
+    }
+    const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n, N = m + ns + n, off = m + ns;
+    for (int i = 0; i < 2 * nports; i++)
+        if (port_nodes[i] < 0 || port_nodes[i] >= n) {
+            peec_set_error("peec_sweep_batch: port node %lld outside 0 .. %lld", (long long)port_nodes[i],
+                           (long long)(n - 1));
+            return PEEC_ERR_ARG;
+        }
+    const int nsl = ctx->nslots, nrhs = nsl * nports;
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    BWork W;
+    W.ctx = ctx;
+    W.adjoint = false;
+    W.nrhs = nrhs;
+    W.rps = nports;
+    W.m = restart;
+    W.N = N;
+    int rc = W.alloc();
+    if (rc) return rc;
+    DevBuf<int64_t> pn;
+    DevBuf<cd> vd;
+    if (!pn.alloc((size_t)(2 * nports)) || !vd.alloc((size_t)nrhs * nports)) return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemcpyAsync(pn.p, port_nodes, (size_t)(2 * nports) * 8, hipMemcpyHostToDevice, ctx->stream));
+    // b = M^-1 [0; 0; is] for every column, in place
+    PEEC_HIP_TRY(hipMemsetAsync(W.B.p, 0, (size_t)nrhs * N * 16, ctx->stream));
+    hipLaunchKernelGGL(k_port_rhs, dim3((unsigned)((nrhs + 63) / 64)), dim3(64), 0, ctx->stream, W.B.p, N, off, pn.p,
+                       nports, nrhs, is_value);
+    PEEC_HIP_TRY(hipGetLastError());
+    if ((rc = apply_precond(ctx, W.B.p, nrhs, N, uniform_cols(nrhs, nports)))) return rc;
+    // warm start: x0, else the previous batch's last slot, else zero
+    const size_t blk = (size_t)N * nports * 16;
+    if (x0) {
+        if (!ctx->sweep_x0.alloc((size_t)N * nports)) return PEEC_ERR_NOMEM;
+        PEEC_HIP_TRY(hipMemcpyAsync(ctx->sweep_x0.p, x0, blk, hipMemcpyHostToDevice, ctx->stream));
+        ctx->sweep_nports = nports;
+    }
+    if (ctx->sweep_x0.p && ctx->sweep_nports == nports) {
+        for (int s = 0; s < nsl; s++)
+            PEEC_HIP_TRY(hipMemcpyAsync(W.X.p + (size_t)s * N * nports, ctx->sweep_x0.p, blk,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        PEEC_HIP_TRY(hipMemsetAsync(W.X.p, 0, (size_t)nrhs * N * 16, ctx->stream));
+    }
+    BOut o;
+    if ((rc = bgmres_solve(ctx, W, tol, max_cycles, mixed, inner_tol, o))) return rc;
+    if (!ctx->sweep_x0.alloc((size_t)N * nports)) return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemcpyAsync(ctx->sweep_x0.p, W.X.p + (size_t)(nsl - 1) * N * nports, blk,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->sweep_nports = nports;
+    hipLaunchKernelGGL(k_port_volts, dim3((unsigned)((nrhs * nports + 255) / 256)), dim3(256), 0, ctx->stream, W.X.p,
+                       N, off, pn.p, nports, nrhs, vd.p);
+    PEEC_HIP_TRY(hipGetLastError());
+    PEEC_HIP_TRY(hipMemcpyAsync(vports, vd.p, (size_t)nrhs * nports * 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (x_out)
+        PEEC_HIP_TRY(hipMemcpyAsync(x_out, W.X.p, (size_t)nrhs * N * 16, hipMemcpyDeviceToHost, ctx->stream));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    o.report(nrhs, flag, relres, iters, cycles);
+    return PEEC_OK;
+}
+
+// The warm start peec_sweep_batch keeps (N x nports): copied out to x.
+extern "C" int peec_sweep_state(peec_ctx *ctx, int nports, peec_c128 *x)
+{
+    if (!ctx || nports < 1 || !x) {
+        peec_set_error("peec_sweep_state: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->sweep_x0.p || ctx->sweep_nports != nports) {
+        peec_set_error("peec_sweep_state: no kept solution for %d ports", nports);
+        return PEEC_ERR_STATE;
+    }
+    const size_t bytes = (size_t)(ctx->m + ctx->ns + ctx->n) * nports * 16;
+    PEEC_HIP_TRY(hipMemcpyAsync(x, ctx->sweep_x0.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return PEEC_OK;
 }
 

@@ -305,6 +305,10 @@ struct peec_ctx {
 
     // ---- vectors
     DevBuf<double2_t> vx, vy, vt1, vt2, vt3;  // N-sized work vectors
+    // peec_sweep_batch: the last slot's port solutions (N x sweep_nports),
+    // the warm start of the next batch
+    DevBuf<double2_t> sweep_x0;
+    int sweep_nports = 0;
     DevBuf<double2_t> trsv_part;              // segment partial sums of the triangular solves
 
     // ---- HIP graphs: GMRES replays its captured matvec (peec_set_graphs)

@@ -42,6 +42,8 @@ void peec_ctx::release_operator()
     lu_layout_from_plan = false;
     lu.ready = false;
     fft_tune.clear();
+    sweep_x0.reset();
+    sweep_nports = 0;
     for (auto &k : kern) {
         k.poly = false;
         k.pspec64.reset();

@@ -233,6 +233,43 @@ def gmres_batched(B, rhs_per_slot, tol, X0=None, restart=60, max_cycles=20, mixe
     return Xt.T, flag, relres, iters, cycles
 
 
+def sweep_batch(port_nodes, is_value, nslots, x0=None, tol=1e-9, restart=100, max_cycles=20, mixed=True,
+                inner_tol=0.0, want_x=False, ctx=None):
+    """One batch of the quasi-static sweep on the device (peec_sweep_batch):
+    for every frequency slot s < nslots and port c, the port excitation, its
+    preconditioned right-hand side and the GMRES solve, warm-started from x0
+    (N x P) or, when x0 is None, from the last slot of the previous call.
+    Returns (Vp, X, flag, relres, iters, cycles): Vp[s, c1, c2] is the node
+    voltage difference of port c2 in the solution for (slot s, port c1); X
+    (N x nslots P) only when want_x."""
+    ctx = ctx or default_context()
+    pn = np.ascontiguousarray(np.asarray(port_nodes, dtype=np.int64).reshape(-1, 2))
+    P = pn.shape[0]
+    nrhs, N = int(nslots) * P, ctx_size(ctx)
+    x0t = None if x0 is None else np.ascontiguousarray(np.asarray(x0, dtype=np.complex128).T)
+    vp = np.zeros((nrhs, P), dtype=np.complex128)
+    Xt = np.zeros((nrhs, N), dtype=np.complex128) if want_x else None
+    flag = np.zeros(nrhs, dtype=np.int32)
+    relres = np.zeros(nrhs)
+    iters = np.zeros(nrhs, dtype=np.int32)
+    cycles = np.zeros(nrhs, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.check(L.lib().peec_sweep_batch(ctx.handle, P, pn.ctypes.data_as(L.i64p), float(is_value),
+                                     None if x0t is None else L.as_c128p(x0t), float(tol), int(restart),
+                                     int(max_cycles), int(bool(mixed)), float(inner_tol), L.as_c128p(vp),
+                                     None if Xt is None else L.as_c128p(Xt), flag.ctypes.data_as(ip), L.as_dp(relres),
+                                     iters.ctypes.data_as(ip), cycles.ctypes.data_as(ip)), "peec_sweep_batch")
+    return (vp.reshape(int(nslots), P, P), None if Xt is None else Xt.T, flag, relres, iters, cycles)
+
+
+def sweep_state(nports, ctx=None):
+    """The last-slot solutions peec_sweep_batch keeps (N x nports)."""
+    ctx = ctx or default_context()
+    out = np.zeros((int(nports), ctx_size(ctx)), dtype=np.complex128)
+    L.check(L.lib().peec_sweep_state(ctx.handle, int(nports), L.as_c128p(out)), "peec_sweep_state")
+    return out.T
+
+
 def apply_K_batched(X, rhs_per_slot=1, fp32=False, precond=False, adjoint=False, ctx=None):
     """K x (precond False / 0), M^-1 K x (precond True / 1) or M^-1 x
     (precond 2) for the columns of X (N x nrhs), column r against frequency

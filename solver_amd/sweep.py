@@ -320,6 +320,10 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=100, max
     f) as FFT_solver_QS_S_type.  `timings`, a dict, accumulates the seconds
     spent per phase (host frequency terms, device LU, preconditioned
     right-hand sides, GMRES); every library call returns synchronised.
+    Without keep, each batch is one peec_sweep_batch call: excitations,
+    preconditioned right-hand sides, solves and the warm start chained
+    from batch to batch stay on the device (timed as "gmres"), and only
+    the port voltages come back.
     tune = True autotunes the FFT passes for each batch width first
     (timed as "tune").  Vrest (N x P) is the warm start of the first batch
     (zeros by default); the result's "Vrest" is the last frequency's
@@ -354,6 +358,24 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=100, max
             solver.tune(nb * P, fp64=True, fp32=mixed)
             tm["tune"] += time.perf_counter() - t0
         set_slots(solver, w[ks], R0, tm, sigma)
+        if not keep:
+            # right-hand sides, solves and warm starts stay on the device
+            t0 = time.perf_counter()
+            Vp, _, flag, relres, iters, cycles = peec.sweep_batch(mesh.port_nodes, esc["Is"], nb,
+                                                                  Xprev if g0 == 0 else None, tol, restart,
+                                                                  max_cycles, mixed, inner_tol, ctx=solver.ctx)
+            tm["gmres"] += time.perf_counter() - t0
+            for slot, k in enumerate(ks):
+                for c1 in range(P):
+                    if record is not None:
+                        r = slot * P + c1
+                        record.append(dict(k=k, port=c1, flag=int(flag[r]), relres=float(relres[r]),
+                                           iters=int(iters[r]), cycles=int(cycles[r])))
+                    for c2 in range(c1, P):
+                        dv = Vp[slot, c1, c2]
+                        S[c1, c2, k] = (2 * dv - R0) / R0 if c1 == c2 else 2 * dv / R0
+                        S[c2, c1, k] = S[c1, c2, k]
+            continue
         t0 = time.perf_counter()
         IS = np.zeros((n, nb * P))
         for slot in range(nb):
@@ -383,6 +405,8 @@ def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=100, max
                     dv = V[m + ns + n3, c1] - V[m + ns + n4, c1]
                     S[c1, c2, k] = (2 * dv - R0) / R0 if c1 == c2 else 2 * dv / R0
                     S[c2, c1, k] = S[c1, c2, k]
+    if not keep and F > 0:
+        Xprev = peec.sweep_state(P, ctx=solver.ctx)
     out = dict(S=S, Z=s2z_fft(S, R0), Y=s2y_fft(S, R0), f=freq / esc["freq"], Vrest=Xprev)
     if keep:
         out["V"] = Vkeep

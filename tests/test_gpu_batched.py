@@ -64,3 +64,30 @@ def test_batched_crossbar_sweep(mixed):
     err = np.max(np.abs(out_b["Z"] - out_o["Z"]) / np.abs(out_o["Z"]))
     print("batched (mixed=%s) crossbar Z max rel err %.3e" % (mixed, err), flush=True)
     assert err < (1e-6 if mixed else 1e-8)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_device_sweep_matches_host_driven(mixed):
+    """peec_sweep_batch (excitations, preconditioned right-hand sides,
+    solves and the warm start chained across batches, all on the device)
+    against the same sweep driven from the host (keep=True: right-hand
+    sides and solutions cross PCIe): identical inputs to the same GMRES, so
+    identical impedances and final warm start."""
+    M = small_mesh()
+    S = sweep.Solver(M, device_lu=True)
+    fs = np.logspace(7, 10, 6)
+    kw = dict(batch=3, mixed=mixed, restart=40, solver=S)
+    rec_h, rec_d = [], []
+    host = sweep.FFT_solver_QS_batched(fs, M, 1e-10, keep=True, record=rec_h, **kw)
+    dev = sweep.FFT_solver_QS_batched(fs, M, 1e-10, record=rec_d, **kw)
+    e_z = float(np.max(np.abs(dev["Z"] - host["Z"]) / np.abs(host["Z"])))
+    e_v = rel(dev["Vrest"], host["Vrest"])
+    print("mixed=%s: Z device vs host %.2e (bitwise %s), Vrest %.2e; iterations %s vs %s"
+          % (mixed, e_z, np.array_equal(dev["Z"], host["Z"]), e_v, [r["iters"] for r in rec_d],
+             [r["iters"] for r in rec_h]), flush=True)
+    assert [r["iters"] for r in rec_d] == [r["iters"] for r in rec_h]
+    assert all(r["flag"] == 0 for r in rec_d)
+    assert e_z < 1e-13 and e_v < 1e-13
+    # a chained second call continues from the kept warm start
+    dev2 = sweep.FFT_solver_QS_batched(fs[:3], M, 1e-10, Vrest=dev["Vrest"], **kw)
+    assert np.all(np.isfinite(dev2["Z"]))
