@@ -41,6 +41,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
+# known size) measured with rocprofv3 --pmc on the bench command, per
+# (workload, profile kind); the summaries and commands are in
+# profiles/traffic_trsv_gather_batched_96x96x24.md and
+# profiles/traffic_fft_passes_large_192x192x48.md.
+MEASURED_TRAFFIC = {("batched_96x96x24", "trsv_gather"): 145.78e6,
+                    ("batched_96x96x24", "trsv_block_L"): 26.35e6,
+                    ("batched_96x96x24", "trsv_block_U"): 27.34e6,
+                    ("large_192x192x48", "fft_z_conv"): 577.2e6}
 KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv_gather",
               "trsv_block_L", "trsv_block_U", "lu_factor", "fft_yz_conv"]
 # kernel symbol behind each profile kind (the FFT pass kernel is
@@ -177,7 +186,8 @@ def matvec_only(args):
                       "setup_s": round(t_setup, 2), "integrals_spectra_s": round(t_kern, 2),
                       "tune_s": round(t_tune, 2)},
            "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": dom[1]["GBs"], "peak": PEAK_HBM_GBS,
-                        "unit": "GB/s", "frac": dom[1]["GBs"] / PEAK_HBM_GBS, "avg_launch_us": dom[1]["avg_us"],
+                        "unit": "GB/s", "frac": dom[1]["GBs"] / PEAK_HBM_GBS,
+                        "traffic": MEASURED_TRAFFIC.get((cfg["name"], dom[0])), "avg_launch_us": dom[1]["avg_us"],
                         "bytes_per_launch": dom[1]["bytes_per_launch"]},
            "fft": res}
     print(json.dumps(out), flush=True)
@@ -364,7 +374,7 @@ def main():
                    "fft_matvec_GBs": round(fft_bytes / fft_ms / 1e6, 1)},
         "roofline": {"bound": "hbm", "kernel": dom["name"], "symbol": KERNEL_SYMBOLS[dom["name"]],
                      "achieved": dom["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": dom["gbs"] / PEAK_HBM_GBS,
-                     "traffic": None, "avg_launch_us": dom["avg_us"],
+                     "traffic": MEASURED_TRAFFIC.get((cfg["name"], dom["name"])), "avg_launch_us": dom["avg_us"],
                      "bytes_per_launch": dom["bytes"] / max(dom["count"], 1)},
         "kernels": {r["name"]: {"launches_per_matvec": r["count"] / nprof, "avg_us": round(r["avg_us"], 3),
                                 "GBs": round(r["gbs"], 1)} for r in prof},

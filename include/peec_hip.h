@@ -128,6 +128,23 @@ int peec_frequency_set(peec_ctx *ctx, double w, const peec_c128 *DZ, const peec_
  * peec_frequency_set and peec_lu_factor are the one-slot forms (slot 0). */
 int peec_frequency_slots(peec_ctx *ctx, int nslots);
 int peec_frequency_set_slot(peec_ctx *ctx, int slot, double w, const peec_c128 *DZ, const peec_csc_c128 *Yle);
+
+/* DZ on the device (no counterpart in the reference, which evaluates the
+ * edge impedances on the host per frequency, FFT_solver_QS_S_type.jl:
+ * 106-115 with compute_Z_self.jl:1-15).  peec_edge_terms_set uploads the
+ * frequency-independent edge data once per geometry (and conductivity):
+ * R (m x 4, row-major: the four half-edge resistances, scaled), Cd (m x 4
+ * complex, the half-edge capacitances), Zs0 (m, the surface impedance at
+ * unit scaled frequency), Lp (m, the self partial inductances), surface
+ * (m flags: edges whose Zs is not zero) and the R and freq escalings.
+ * peec_frequency_set_slot_edges then fills a slot as peec_frequency_set_slot
+ * does, with DZ = Z_self + Re(Zs) + j w Lp (after the skin-effect switch)
+ * and 1 ./ DZ computed on the device.  peec_frequency_dz reads a slot's DZ
+ * back (m values). */
+int peec_edge_terms_set(peec_ctx *ctx, int64_t m, const double *R, const peec_c128 *Cd, const peec_c128 *Zs0,
+                        const double *Lp, const int8_t *surface, double esc_R, double esc_freq);
+int peec_frequency_set_slot_edges(peec_ctx *ctx, int slot, double w, const peec_csc_c128 *Yle);
+int peec_frequency_dz(peec_ctx *ctx, int slot, peec_c128 *out);
 int peec_lu_factor_slot(peec_ctx *ctx, int slot);
 
 /* LU factors of SS = Yle + A' Z^-1 A + j w Gamma P^-1 Gamma'

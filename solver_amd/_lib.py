@@ -73,6 +73,10 @@ def lib():
         L.peec_sweep_batch.argtypes = [vp, ctypes.c_int, i64p, ctypes.c_double, c128p, ctypes.c_double, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, c128p, c128p, ip, dp, ip, ip]
         L.peec_sweep_state.argtypes = [vp, ctypes.c_int, c128p]
+        L.peec_edge_terms_set.argtypes = [vp, i64, dp, c128p, c128p, dp, ctypes.c_void_p, ctypes.c_double,
+                                          ctypes.c_double]
+        L.peec_frequency_set_slot_edges.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(CscC128)]
+        L.peec_frequency_dz.argtypes = [vp, ctypes.c_int, c128p]
         L.peec_apply_K_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c128p, c128p]
         L.peec_apply_KH_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c128p, c128p]
         L.peec_precond_rhs_batched_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int, c128p, c128p]

@@ -286,6 +286,14 @@ struct peec_ctx {
     std::vector<double> hw;         // per slot
     DevBuf<double> wv;              // per slot
     DevBuf<double2_t> DZ, invZ;     // nslots x m
+    // per-edge data for DZ on the device (peec_edge_terms_set): half-edge
+    // R and Cd (4 per edge), unit-frequency surface impedance, Lp, surface
+    // flags, and the escalings
+    DevBuf<double> eR, eLp;
+    DevBuf<double2_t> eCd, eZs;
+    DevBuf<int8_t> eSurf;
+    double esc_R = 0.0, esc_freq = 0.0;
+    bool edges_ready = false;
     CsrC Yle;                       // nslots consecutive CSRs: ptr nslots x (n + 1), offsets into idx / val
     struct HostCsr {
         std::vector<int32_t> ptr, idx;

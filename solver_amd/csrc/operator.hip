@@ -44,6 +44,7 @@ void peec_ctx::release_operator()
     fft_tune.clear();
     sweep_x0.reset();
     sweep_nports = 0;
+    edges_ready = false;
     for (auto &k : kern) {
         k.poly = false;
         k.pspec64.reset();
@@ -646,6 +647,8 @@ extern "C" int peec_frequency_slots(peec_ctx *ctx, int nslots)
     return PEEC_OK;
 }
 
+static int slot_set_rest(peec_ctx *ctx, int slot, double w, const peec_csc_c128 *Yle);
+
 extern "C" int peec_frequency_set_slot(peec_ctx *ctx, int slot, double w, const peec_c128 *DZ,
                                        const peec_csc_c128 *Yle)
 {
@@ -667,7 +670,7 @@ extern "C" int peec_frequency_set_slot(peec_ctx *ctx, int slot, double w, const 
     }
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    const int64_t m = ctx->m, n = ctx->n;
+    const int64_t m = ctx->m;
     std::vector<double2_t> dz((size_t)m), iz((size_t)m);
     for (int64_t e = 0; e < m; e++) {
         dz[(size_t)e] = {DZ[e].re, DZ[e].im};
@@ -681,6 +684,14 @@ extern "C" int peec_frequency_set_slot(peec_ctx *ctx, int slot, double w, const 
     }
     PEEC_HIP_TRY(hipMemcpy(ctx->DZ.p + (int64_t)slot * m, dz.data(), (size_t)m * 16, hipMemcpyHostToDevice));
     PEEC_HIP_TRY(hipMemcpy(ctx->invZ.p + (int64_t)slot * m, iz.data(), (size_t)m * 16, hipMemcpyHostToDevice));
+    return slot_set_rest(ctx, slot, w, Yle);
+}
+
+// The rest of setting a slot once its DZ and 1/DZ are on the device: w and
+// the admittance matrix Yle.
+static int slot_set_rest(peec_ctx *ctx, int slot, double w, const peec_csc_c128 *Yle)
+{
+    const int64_t n = ctx->n;
     ctx->hw[(size_t)slot] = w;
     if (slot == 0) ctx->w = w;
     PEEC_HIP_TRY(hipMemcpy(ctx->wv.p, ctx->hw.data(), ctx->hw.size() * 8, hipMemcpyHostToDevice));
@@ -709,6 +720,138 @@ extern "C" int peec_frequency_set_slot(peec_ctx *ctx, int slot, double w, const 
     ctx->slot_lu[(size_t)slot] = 0;
     ctx->freq_ready = std::all_of(ctx->slot_set.begin(), ctx->slot_set.end(), [](char c) { return c != 0; });
     ctx->lu_ready = false;
+    return PEEC_OK;
+}
+
+// Edge impedances at one frequency on the device (FFT_solver_QS_S_type.jl
+// :106-115): Z_self from the four half-edge R/Cd pairs (compute_Z_self.jl:
+// 1-15: R || Cd, R alone or Cd alone, summed in order; Cd is complex for
+// lossy dielectrics, j w Cd = (-w Im Cd, w Re Cd)), the surface
+// impedance Zs = esc_R * (Zs0 * sqrt(w / esc_freq)), the skin switch on
+// surface edges (the larger real part wins; the loser keeps only its
+// imaginary part), then DZ = Z_self + Re(Zs) + j w Lp and 1 / DZ.  The
+// complex reciprocals follow Smith's method, the operation order the host
+// restatement uses, so results match it to the last bit or so.
+__global__ void k_edge_dz(const double *__restrict__ R, const double2_t *__restrict__ Cd,
+                          const double2_t *__restrict__ Zs0, const double *__restrict__ Lp,
+                          const int8_t *__restrict__ surf, int64_t m, double w, double esc_R, double esc_freq,
+                          double2_t *__restrict__ dz, double2_t *__restrict__ iz)
+{
+#pragma clang fp contract(off)
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    // 1 / (a + j b) by Smith's method
+    auto recip = [](double a, double b, double &x, double &y) {
+        if (fabs(a) >= fabs(b)) {
+            const double rat = b / a, scl = 1.0 / (a + b * rat);
+            x = (1.0 + 0.0 * rat) * scl;
+            y = (0.0 - 1.0 * rat) * scl;
+        } else {
+            const double rat = a / b, scl = 1.0 / (b + a * rat);
+            x = (1.0 * rat + 0.0) * scl;
+            y = (0.0 * rat - 1.0) * scl;
+        }
+    };
+    double zr = 0.0, zi = 0.0;
+    for (int a = 0; a < 4; a++) {
+        const double r = R[4 * e + a];
+        const double2_t c = Cd[4 * e + a];
+        const bool hc = c.x != 0.0 || c.y != 0.0;
+        // j w c, as (0 + j w) * c
+        const double jr = 0.0 * c.x - w * c.y, ji = 0.0 * c.y + w * c.x;
+        double tr, ti;
+        if (r != 0.0 && hc) {
+            recip(1.0 / r + jr, 0.0 + ji, tr, ti);
+        } else if (r != 0.0) {
+            tr = r;
+            ti = 0.0;
+        } else if (hc) {
+            recip(jr, ji, tr, ti);
+        } else {
+            continue;
+        }
+        zr = zr + tr;
+        zi = zi + ti;
+    }
+    const double s = sqrt(w / esc_freq);
+    const double2_t z0 = Zs0[e];
+    double sr = esc_R * (z0.x * s), si = esc_R * (z0.y * s);
+    if (surf[e]) {
+        const double d = sr - zr;
+        if (d > 0.0) zr = 0.0;
+        else if (d < 0.0) sr = 0.0;
+    }
+    (void)si;
+    const double dr = zr + sr + 0.0 * Lp[e], di = zi + w * Lp[e];
+    dz[e] = {dr, di};
+    const double q = dr * dr + di * di;
+    iz[e] = {dr / q, -di / q};
+}
+
+extern "C" int peec_edge_terms_set(peec_ctx *ctx, int64_t m, const double *R, const peec_c128 *Cd, const peec_c128 *Zs0,
+                                   const double *Lp, const int8_t *surface, double esc_R, double esc_freq)
+{
+    if (!ctx || !R || !Cd || !Zs0 || !Lp || !surface || !(esc_freq > 0)) {
+        peec_set_error("peec_edge_terms_set: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->op_ready || m != ctx->m) {
+        peec_set_error("peec_edge_terms_set: call peec_operator_set first; m must be %lld", (long long)ctx->m);
+        return PEEC_ERR_STATE;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    if (!ctx->eR.alloc((size_t)(4 * m)) || !ctx->eCd.alloc((size_t)(4 * m)) || !ctx->eLp.alloc((size_t)m) ||
+        !ctx->eZs.alloc((size_t)m) || !ctx->eSurf.alloc((size_t)m))
+        return PEEC_ERR_NOMEM;
+    PEEC_HIP_TRY(hipMemcpy(ctx->eR.p, R, (size_t)(4 * m) * 8, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(ctx->eCd.p, Cd, (size_t)(4 * m) * 16, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(ctx->eLp.p, Lp, (size_t)m * 8, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(ctx->eZs.p, Zs0, (size_t)m * 16, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(ctx->eSurf.p, surface, (size_t)m, hipMemcpyHostToDevice));
+    ctx->esc_R = esc_R;
+    ctx->esc_freq = esc_freq;
+    ctx->edges_ready = true;
+    return PEEC_OK;
+}
+
+extern "C" int peec_frequency_set_slot_edges(peec_ctx *ctx, int slot, double w, const peec_csc_c128 *Yle)
+{
+    if (!ctx || !Yle || !(w > 0)) {
+        peec_set_error("peec_frequency_set_slot_edges: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->op_ready || !ctx->edges_ready) {
+        peec_set_error("peec_frequency_set_slot_edges: call peec_operator_set and peec_edge_terms_set first");
+        return PEEC_ERR_STATE;
+    }
+    if (slot < 0 || slot >= ctx->nslots || (int)ctx->slot_set.size() != ctx->nslots) {
+        peec_set_error("peec_frequency_set_slot_edges: slot %d outside the %d slots", slot, ctx->nslots);
+        return PEEC_ERR_ARG;
+    }
+    if (Yle->nrow != ctx->n || Yle->ncol != ctx->n) {
+        peec_set_error("peec_frequency_set_slot_edges: Yle must be n x n");
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    const int64_t m = ctx->m;
+    hipLaunchKernelGGL(k_edge_dz, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, ctx->eR.p, ctx->eCd.p,
+                       ctx->eZs.p, ctx->eLp.p, ctx->eSurf.p, m, w, ctx->esc_R, ctx->esc_freq,
+                       ctx->DZ.p + (int64_t)slot * m, ctx->invZ.p + (int64_t)slot * m);
+    PEEC_HIP_TRY(hipGetLastError());
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return slot_set_rest(ctx, slot, w, Yle);
+}
+
+extern "C" int peec_frequency_dz(peec_ctx *ctx, int slot, peec_c128 *out)
+{
+    if (!ctx || !out || slot < 0 || slot >= ctx->nslots || (int)ctx->slot_set.size() != ctx->nslots ||
+        !ctx->slot_set[(size_t)slot]) {
+        peec_set_error("peec_frequency_dz: slot %d is not set", slot);
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    PEEC_HIP_TRY(hipMemcpy(out, ctx->DZ.p + (int64_t)slot * ctx->m, (size_t)ctx->m * 16, hipMemcpyDeviceToHost));
     return PEEC_OK;
 }
 

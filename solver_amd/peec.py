@@ -195,6 +195,42 @@ def frequency_set_slot(slot, w, DZ, Yle, ctx=None):
             "peec_frequency_set_slot")
 
 
+def edge_terms_set(R, Cd, Zs0, Lp, surface, esc_R, esc_freq, ctx=None):
+    """Per-edge data from which frequency_set_slot_edges computes DZ on the
+    device: R (m x 4), Cd (m x 4, complex for lossy dielectrics), Zs0 (m,
+    surface impedance at unit scaled frequency), Lp (m), surface (m flags),
+    and the R / freq escalings."""
+    ctx = ctx or default_context()
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    Cd = np.ascontiguousarray(Cd, dtype=np.complex128)
+    zs = np.ascontiguousarray(Zs0, dtype=np.complex128)
+    lp = np.ascontiguousarray(Lp, dtype=np.float64)
+    sf = np.ascontiguousarray(np.asarray(surface) != 0, dtype=np.int8)
+    m = lp.shape[0]
+    if R.shape != (m, 4) or Cd.shape != (m, 4) or zs.shape != (m,) or sf.shape != (m,):
+        raise ValueError("edge_terms_set: R, Cd must be m x 4 and Zs0, surface length m")
+    L.check(L.lib().peec_edge_terms_set(ctx.handle, m, L.as_dp(R), L.as_c128p(Cd), L.as_c128p(zs), L.as_dp(lp),
+                                        sf.ctypes.data_as(ctypes.c_void_p), float(esc_R), float(esc_freq)),
+            "peec_edge_terms_set")
+
+
+def frequency_set_slot_edges(slot, w, Yle, ctx=None):
+    """frequency_set_slot with DZ (and 1/DZ) computed on the device from
+    the edge data of edge_terms_set."""
+    ctx = ctx or default_context()
+    Y, keep = _csc_c128(Yle)
+    L.check(L.lib().peec_frequency_set_slot_edges(ctx.handle, int(slot), float(w), ctypes.byref(Y)),
+            "peec_frequency_set_slot_edges")
+
+
+def frequency_dz(slot, m, ctx=None):
+    """DZ of a frequency slot (m edges), read back from the device."""
+    ctx = ctx or default_context()
+    out = np.zeros(int(m), dtype=np.complex128)
+    L.check(L.lib().peec_frequency_dz(ctx.handle, int(slot), L.as_c128p(out)), "peec_frequency_dz")
+    return out
+
+
 def lu_factor_slot(slot, ctx=None):
     ctx = ctx or default_context()
     L.check(L.lib().peec_lu_factor_slot(ctx.handle, int(slot)), "peec_lu_factor_slot")

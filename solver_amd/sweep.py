@@ -7,6 +7,7 @@ SuperLU (the reference uses UMFPACK on the host too, :118-119).  The kernel
 spectra, the FFT matvec, the preconditioner's triangular solves and GMRES
 run on the GPU through libpeec_hip.
 """
+import hashlib
 import time
 
 import numpy as np
@@ -291,17 +292,34 @@ def s2y_fft(S, Zo):
     return np.stack([Zo * np.linalg.solve(I + S[:, :, k], I - S[:, :, k]) for k in range(S.shape[2])], axis=2)
 
 
+def _device_edges(solver, sigma):
+    """Upload the per-edge data DZ_of works from (for this sigma) once, so
+    that every frequency's DZ is computed on the device."""
+    mesh = solver.mesh
+    key = None if sigma is None else hashlib.sha1(np.ascontiguousarray(sigma, dtype=np.float64)).hexdigest()
+    if getattr(solver, "_edge_key", "unset") == key:
+        return
+    diag_R, Zs0 = (mesh.diag_R, mesh.Zs) if sigma is None else mesher.conductor_terms(mesh, sigma)
+    surf = np.zeros(mesh.m, dtype=np.int8)
+    surf[mesh.surface_edges] = 1
+    peec.edge_terms_set(diag_R, mesh.diag_Cd, Zs0, mesh.diag_Lp, surf, mesh.escalings["R"], mesh.escalings["freq"],
+                        ctx=solver.ctx)
+    solver._edge_key = key
+
+
 def set_slots(solver, ws, R0=50.0, timings=None, sigma=None):
     """Fill frequency slots 0 .. len(ws)-1 (already allocated) with the
     per-frequency terms at scaled angular frequencies ws and factor each
-    slot's SS on the device (FFT_solver_QS_S_type.jl:105-119 per slot)."""
+    slot's SS on the device (FFT_solver_QS_S_type.jl:105-119 per slot).
+    DZ (DZ_of) is computed on the device from per-edge data uploaded once."""
     mesh = solver.mesh
     esc = mesh.escalings
     tm = timings if timings is not None else {}
+    _device_edges(solver, sigma)
     for slot, w in enumerate(ws):
         t0 = time.perf_counter()
         Yle = build_Yle_FFT(mesh.lumped_elements, mesh.port_nodes, esc, mesh.n, w / esc["freq"], R0)
-        peec.frequency_set_slot(slot, w, DZ_of(mesh, w, sigma), Yle, ctx=solver.ctx)
+        peec.frequency_set_slot_edges(slot, w, Yle, ctx=solver.ctx)
         t1 = time.perf_counter()
         peec.lu_factor_slot(slot, ctx=solver.ctx)
         tm["freq_terms"] = tm.get("freq_terms", 0.0) + t1 - t0

@@ -91,3 +91,36 @@ def test_device_sweep_matches_host_driven(mixed):
     # a chained second call continues from the kept warm start
     dev2 = sweep.FFT_solver_QS_batched(fs[:3], M, 1e-10, Vrest=dev["Vrest"], **kw)
     assert np.all(np.isfinite(dev2["Z"]))
+
+
+def test_device_edge_impedances_match_host():
+    """DZ computed on the device (peec_frequency_set_slot_edges) against
+    DZ_of on the host, over frequencies on both sides of the skin-effect
+    switch and with per-voxel conductivities."""
+    from solver_amd import configs, mesher
+    M = small_mesh()
+    S = sweep.Solver(M, ctx=peec.Context(0), device_lu=True)
+    cell = (0.25e-3, 0.25e-3, 0.05e-3)
+    g, mats, sg = configs._layered(12, 10, 6, cell, 4, 0.5, 4.4, 0.02, seed=2, sigma_range=(3e7, 6e7))
+    M2 = mesher.mesh(g, mats, cell, (0.0, 0.0, 0.0), configs._x_face_ports(g, cell, 4), p_self_fn=pself_fn(),
+                     sigma_grid=sg)
+    S2 = sweep.Solver(M2, ctx=peec.Context(0), device_lu=True)
+    for mesh, solver, sigma in ((M, S, None), (M2, S2, M2.sigma_vox * 1.3)):
+        fs = np.logspace(5, 10.5, 8)
+        peec.frequency_slots(len(fs), ctx=solver.ctx)
+        sweep.set_slots(solver, [2 * np.pi * f * mesh.escalings["freq"] for f in fs], sigma=sigma)
+        worst, exact = 0.0, 0
+        for slot, f in enumerate(fs):
+            w = 2 * np.pi * f * mesh.escalings["freq"]
+            ref = sweep.DZ_of(mesh, w, sigma)
+            dz = peec.frequency_dz(slot, mesh.m, ctx=solver.ctx)
+            worst = max(worst, float(np.max(np.abs(dz - ref) / np.abs(ref))))
+            exact += int(np.sum(dz == ref))
+        print("m=%d: device DZ vs DZ_of max rel %.2e, %d of %d bitwise" % (mesh.m, worst, exact, mesh.m * len(fs)),
+              flush=True)
+        assert worst < 1e-15
+
+
+def pself_fn():
+    from test_gpu_operator import pself
+    return pself
