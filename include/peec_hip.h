@@ -146,6 +146,13 @@ int peec_edge_terms_set(peec_ctx *ctx, int64_t m, const double *R, const peec_c1
 int peec_frequency_set_slot_edges(peec_ctx *ctx, int slot, double w, const peec_csc_c128 *Yle);
 int peec_frequency_dz(peec_ctx *ctx, int slot, peec_c128 *out);
 int peec_lu_factor_slot(peec_ctx *ctx, int slot);
+/* Factors the SS of slots slot0 .. slot0 + count - 1 (no counterpart in the
+ * reference): the slots go through each launch of the multifrontal schedule
+ * together (one copy of the front pool per slot, in groups of at most
+ * 48 GiB of pools), so the small fronts near the leaves run as one wide
+ * launch instead of one per slot.  Same arithmetic as
+ * peec_lu_factor_slot on each slot. */
+int peec_lu_factor_slots(peec_ctx *ctx, int slot0, int count);
 
 /* LU factors of SS = Yle + A' Z^-1 A + j w Gamma P^-1 Gamma'
  * (F = lu(SS), FFT_solver_QS_S_type.jl:118-119) in UMFPACK's convention,

@@ -43,18 +43,26 @@ __device__ __forceinline__ cd lcinv(cd a)
     return {a.x / d, -a.y / d};
 }
 
-__global__ void k_lu_zero(cd *p, int64_t n)
+// The factorization kernels run for several frequency slots at once:
+// blockIdx.y selects the slot, whose fronts live at pool + y * stride.
+__global__ void k_lu_zero(cd *p, int64_t n, int64_t stride)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = {0.0, 0.0};
+    if (i < n) p[(int64_t)blockIdx.y * stride + i] = {0.0, 0.0};
 }
 
 // SS entry k = sum_t coef_t / DZ[edge_t] + j w gsum_k, written to its front
+// (slot slot0 + y: its 1/DZ at invZ + (slot0 + y) m, its w at wv[slot0 + y])
 __global__ void k_lu_ss(const int64_t *tgt, const int32_t *cptr, const int32_t *cedge, const double *ccoef,
-                        const double *gsum, const cd *invZ, double w, cd *pool, int64_t nss)
+                        const double *gsum, const cd *invZ0, const double *wv, int slot0, int64_t m, cd *pool0,
+                        int64_t nss, int64_t stride)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nss) return;
+    const int slot = slot0 + (int)blockIdx.y;
+    const cd *invZ = invZ0 + (int64_t)slot * m;
+    const double w = wv[slot];
+    cd *pool = pool0 + (int64_t)blockIdx.y * stride;
     double vx = 0.0, vy = 0.0;
     for (int32_t t = cptr[k]; t < cptr[k + 1]; t++) {
         const cd z = invZ[cedge[t]];
@@ -65,7 +73,8 @@ __global__ void k_lu_ss(const int64_t *tgt, const int32_t *cptr, const int32_t *
     pool[tgt[k]] = {vx, vy};
 }
 
-// pool[tgt[i]] += val[i] (Yle entries; each target once)
+// pool[tgt[i]] += val[i] (Yle entries; each target once per slot; tgt
+// includes the slot's pool offset)
 __global__ void k_lu_add(const int64_t *tgt, const cd *val, int64_t cnt, cd *pool)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -76,8 +85,9 @@ __global__ void k_lu_add(const int64_t *tgt, const cd *val, int64_t cnt, cd *poo
 
 // Column j of child c's update matrix added into its parent.
 __global__ void __launch_bounds__(256) k_lu_extend_add(const int2 *items, const LuFrontInfo *fr, const int64_t *rel_off,
-                                                       const int32_t *rel, cd *pool)
+                                                       const int32_t *rel, cd *pool0, int64_t stride)
 {
+    cd *pool = pool0 + (int64_t)blockIdx.y * stride;
     const int2 it = items[blockIdx.x];
     const LuFrontInfo C = fr[it.x];
     const LuFrontInfo P = fr[C.parent];
@@ -94,10 +104,12 @@ __global__ void __launch_bounds__(256) k_lu_extend_add(const int2 *items, const 
 // LU (no pivoting) of the w x w diagonal block at pivot k0 of a front, in
 // LDS; the factors go back to the front, the inverses of U and of the unit
 // lower L to the item's scratch slot (64 x 64 each, column-major).
-__global__ void __launch_bounds__(256) k_lu_diag(const int4 *items, const LuFrontInfo *fr, cd *pool, int64_t scratch0)
+__global__ void __launch_bounds__(256) k_lu_diag(const int4 *items, const LuFrontInfo *fr, cd *pool0, int64_t scratch0,
+                                                 int64_t stride)
 {
     __shared__ cd A[64 * 64];
     __shared__ cd X[64 * 64];
+    cd *pool = pool0 + (int64_t)blockIdx.y * stride;
     const int4 it = items[blockIdx.x];
     const LuFrontInfo F = fr[it.x];
     const int k0 = it.z, w = min(64, F.s - k0), ld = F.f;
@@ -186,8 +198,9 @@ __global__ void __launch_bounds__(256) k_lu_diag(const int4 *items, const LuFron
 // GEMM tiles (see LuGemmItem).  256 threads, 4 x 4 outputs each; k in
 // chunks of 16 staged through LDS.  A tile may alias C (rows of C read as
 // A) or B may alias C: every read of the tile happens before its write.
-__global__ void __launch_bounds__(256) k_lu_gemm(const LuGemmItem *items, cd *pool)
+__global__ void __launch_bounds__(256) k_lu_gemm(const LuGemmItem *items, cd *pool0, int64_t stride)
 {
+    cd *pool = pool0 + (int64_t)blockIdx.y * stride;
     __shared__ cd As[16][65];
     __shared__ cd Bs[16][65];
     const LuGemmItem it = items[blockIdx.x];
@@ -677,48 +690,46 @@ extern "C" int peec_lu_analyze(peec_ctx *ctx, int64_t n, const int64_t *order, i
     return PEEC_OK;
 }
 
-extern "C" int peec_lu_factor_slot(peec_ctx *ctx, int slot)
+// Factor slots slot0 .. slot0 + ns - 1 together: every launch of the
+// schedule covers all of them (blockIdx.y = slot), each slot's fronts in
+// its own copy of the pool.
+static int lu_factor_range(peec_ctx *ctx, int slot0, int ns)
 {
-    if (!ctx) {
-        peec_set_error("peec_lu_factor: null context");
-        return PEEC_ERR_ARG;
-    }
     LuPlan &P = ctx->lu;
-    if (!P.ready || !ctx->lu_layout_from_plan || !ctx->op_ready) {
-        peec_set_error("peec_lu_factor: call peec_lu_analyze first (after peec_operator_set, and again after "
-                       "peec_lu_set_factors)");
-        return PEEC_ERR_STATE;
+    const int64_t stride = P.pool_total;
+    if (P.pool.n < (size_t)(stride * ns) && !P.pool.alloc((size_t)(stride * ns))) {
+        peec_set_error("peec_lu_factor: out of device memory for %d x %lld front entries", ns, (long long)stride);
+        return PEEC_ERR_NOMEM;
     }
-    if (slot < 0 || slot >= ctx->nslots || (int)ctx->slot_set.size() != ctx->nslots || !ctx->slot_set[(size_t)slot]) {
-        peec_set_error("peec_lu_factor: frequency slot %d is not set (peec_frequency_set / _slot)", slot);
-        return PEEC_ERR_STATE;
-    }
-    const peec_ctx::HostCsr &Y = ctx->hYle[(size_t)slot];
-    PEEC_HIP_TRY(hipSetDevice(ctx->device));
-    // Yle entries -> pool targets (host; a handful of port / lumped entries)
+    // Yle entries -> pool targets (host; a handful of port / lumped entries per slot)
     std::vector<int64_t> ytgt;
     std::vector<double2_t> yval;
-    for (int64_t i = 0; i < P.n; i++)
-        for (int32_t k = Y.ptr[(size_t)i]; k < Y.ptr[(size_t)i + 1]; k++) {
-            const int32_t pi = P.pos[(size_t)i], pj = P.pos[(size_t)Y.idx[(size_t)k]];
-            const int K = P.front_of[(size_t)std::min(pi, pj)];
-            const int li = lu_local(P, K, pi), lj = lu_local(P, K, pj);
-            if (li < 0 || lj < 0) {
-                peec_set_error("peec_lu_factor: Yle couples nodes %lld and %d that share no front; put port and "
-                               "lumped-element nodes in the root front",
-                               (long long)i + 1, Y.idx[(size_t)k] + 1);
-                return PEEC_ERR_ARG;
+    for (int y = 0; y < ns; y++) {
+        const peec_ctx::HostCsr &Y = ctx->hYle[(size_t)(slot0 + y)];
+        for (int64_t i = 0; i < P.n; i++)
+            for (int32_t k = Y.ptr[(size_t)i]; k < Y.ptr[(size_t)i + 1]; k++) {
+                const int32_t pi = P.pos[(size_t)i], pj = P.pos[(size_t)Y.idx[(size_t)k]];
+                const int K = P.front_of[(size_t)std::min(pi, pj)];
+                const int li = lu_local(P, K, pi), lj = lu_local(P, K, pj);
+                if (li < 0 || lj < 0) {
+                    peec_set_error("peec_lu_factor: Yle couples nodes %lld and %d that share no front; put port and "
+                                   "lumped-element nodes in the root front",
+                                   (long long)i + 1, Y.idx[(size_t)k] + 1);
+                    return PEEC_ERR_ARG;
+                }
+                const LuFrontInfo &F = P.fronts[(size_t)K];
+                ytgt.push_back((int64_t)y * stride + F.off + (int64_t)lj * F.f + li);
+                yval.push_back(Y.val[(size_t)k]);
             }
-            const LuFrontInfo &F = P.fronts[(size_t)K];
-            ytgt.push_back(F.off + (int64_t)lj * F.f + li);
-            yval.push_back(Y.val[(size_t)k]);
-        }
+    }
     int pi = ctx->prof_begin(PK_LU, 0.0);
     cd *pool = P.pool.p;
-    hipLaunchKernelGGL(k_lu_zero, dim3(nblk256(P.pool_fronts)), dim3(256), 0, ctx->stream, pool, P.pool_fronts);
-    hipLaunchKernelGGL(k_lu_ss, dim3(nblk256(P.nss)), dim3(256), 0, ctx->stream, P.ss_tgt.p, P.ss_cptr.p, P.ss_cedge.p,
-                       P.ss_ccoef.p, P.ss_gsum.p, ctx->invZ.p + (int64_t)slot * ctx->m, ctx->hw[(size_t)slot], pool,
-                       P.nss);
+    const unsigned ny = (unsigned)ns;
+    hipLaunchKernelGGL(k_lu_zero, dim3(nblk256(P.pool_fronts), ny), dim3(256), 0, ctx->stream, pool, P.pool_fronts,
+                       stride);
+    hipLaunchKernelGGL(k_lu_ss, dim3(nblk256(P.nss), ny), dim3(256), 0, ctx->stream, P.ss_tgt.p, P.ss_cptr.p,
+                       P.ss_cedge.p, P.ss_ccoef.p, P.ss_gsum.p, ctx->invZ.p, ctx->wv.p, slot0, ctx->m, pool, P.nss,
+                       stride);
     DevBuf<int64_t> dyt;
     DevBuf<double2_t> dyv;
     if (!ytgt.empty()) {
@@ -730,39 +741,89 @@ extern "C" int peec_lu_factor_slot(peec_ctx *ctx, int slot)
     }
     for (const auto &L : P.sched) {
         if (L.count <= 0) continue;
+        const dim3 grid((unsigned)L.count, ny);
         if (L.kind == 0)
-            hipLaunchKernelGGL(k_lu_extend_add, dim3((unsigned)L.count), dim3(256), 0, ctx->stream,
-                               P.ea_items.p + L.first, P.d_fronts.p, P.rel_off.p, P.rel.p, pool);
+            hipLaunchKernelGGL(k_lu_extend_add, grid, dim3(256), 0, ctx->stream, P.ea_items.p + L.first, P.d_fronts.p,
+                               P.rel_off.p, P.rel.p, pool, stride);
         else if (L.kind == 1)
-            hipLaunchKernelGGL(k_lu_diag, dim3((unsigned)L.count), dim3(256), 0, ctx->stream, P.diag_items.p + L.first,
-                               P.d_fronts.p, pool, P.scratch0);
+            hipLaunchKernelGGL(k_lu_diag, grid, dim3(256), 0, ctx->stream, P.diag_items.p + L.first, P.d_fronts.p, pool,
+                               P.scratch0, stride);
         else
-            hipLaunchKernelGGL(k_lu_gemm, dim3((unsigned)L.count), dim3(256), 0, ctx->stream, P.gemm_items.p + L.first,
-                               pool);
+            hipLaunchKernelGGL(k_lu_gemm, grid, dim3(256), 0, ctx->stream, P.gemm_items.p + L.first, pool, stride);
     }
     PEEC_HIP_TRY(hipGetLastError());
-    // gather into the slot's value sets of the triangular-solve layouts
+    // gather into the slots' value sets of the triangular-solve layouts
     TriFactor &Lf = ctx->Lf, &Uf = ctx->Uf;
-    if (Lf.nslots <= slot || Uf.nslots <= slot) {
-        peec_set_error("peec_lu_factor: factor layouts hold %d value sets, slot %d requested", Lf.nslots, slot);
+    if (Lf.nslots < slot0 + ns || Uf.nslots < slot0 + ns) {
+        peec_set_error("peec_lu_factor: factor layouts hold %d value sets, slots up to %d requested", Lf.nslots,
+                       slot0 + ns - 1);
         return PEEC_ERR_STATE;
     }
-    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Lf.ob_n)), dim3(256), 0, ctx->stream, P.l_ob_src.p,
-                       Lf.ob_val.p + slot * Lf.ob_n, Lf.ob_n, pool, 0);
-    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Lf.dense_n)), dim3(256), 0, ctx->stream, P.l_dense_src.p,
-                       Lf.dense.p + slot * Lf.dense_n, Lf.dense_n, pool, 0);
-    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Uf.ob_n)), dim3(256), 0, ctx->stream, P.u_ob_src.p,
-                       Uf.ob_val.p + slot * Uf.ob_n, Uf.ob_n, pool, 0);
-    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Uf.dense_n)), dim3(256), 0, ctx->stream, P.u_dense_src.p,
-                       Uf.dense.p + slot * Uf.dense_n, Uf.dense_n, pool, 0);
-    hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(P.n)), dim3(256), 0, ctx->stream, P.u_diag_src.p,
-                       Uf.diag.p + slot * P.n, P.n, pool, 1);
+    for (int y = 0; y < ns; y++) {
+        const int slot = slot0 + y;
+        const cd *pl = pool + (int64_t)y * stride;
+        hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Lf.ob_n)), dim3(256), 0, ctx->stream, P.l_ob_src.p,
+                           Lf.ob_val.p + slot * Lf.ob_n, Lf.ob_n, pl, 0);
+        hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Lf.dense_n)), dim3(256), 0, ctx->stream, P.l_dense_src.p,
+                           Lf.dense.p + slot * Lf.dense_n, Lf.dense_n, pl, 0);
+        hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Uf.ob_n)), dim3(256), 0, ctx->stream, P.u_ob_src.p,
+                           Uf.ob_val.p + slot * Uf.ob_n, Uf.ob_n, pl, 0);
+        hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Uf.dense_n)), dim3(256), 0, ctx->stream, P.u_dense_src.p,
+                           Uf.dense.p + slot * Uf.dense_n, Uf.dense_n, pl, 0);
+        hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(P.n)), dim3(256), 0, ctx->stream, P.u_diag_src.p,
+                           Uf.diag.p + slot * P.n, P.n, pl, 1);
+    }
     PEEC_HIP_TRY(hipGetLastError());
     ctx->prof_end(pi);
     // the Yle staging buffers are freed on return: wait for their use
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->slot_lu[(size_t)slot] = 1;
+    for (int y = 0; y < ns; y++) ctx->slot_lu[(size_t)(slot0 + y)] = 1;
     ctx->lu_ready = std::all_of(ctx->slot_lu.begin(), ctx->slot_lu.end(), [](char c) { return c != 0; });
+    return PEEC_OK;
+}
+
+static int lu_factor_check(peec_ctx *ctx, int slot0, int ns)
+{
+    if (!ctx) {
+        peec_set_error("peec_lu_factor: null context");
+        return PEEC_ERR_ARG;
+    }
+    LuPlan &P = ctx->lu;
+    if (!P.ready || !ctx->lu_layout_from_plan || !ctx->op_ready) {
+        peec_set_error("peec_lu_factor: call peec_lu_analyze first (after peec_operator_set, and again after "
+                       "peec_lu_set_factors)");
+        return PEEC_ERR_STATE;
+    }
+    for (int slot = slot0; slot < slot0 + ns; slot++)
+        if (slot < 0 || slot >= ctx->nslots || (int)ctx->slot_set.size() != ctx->nslots ||
+            !ctx->slot_set[(size_t)slot]) {
+            peec_set_error("peec_lu_factor: frequency slot %d is not set (peec_frequency_set / _slot)", slot);
+            return PEEC_ERR_STATE;
+        }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    return PEEC_OK;
+}
+
+extern "C" int peec_lu_factor_slot(peec_ctx *ctx, int slot)
+{
+    int rc = lu_factor_check(ctx, slot, 1);
+    return rc ? rc : lu_factor_range(ctx, slot, 1);
+}
+
+// Slots slot0 .. slot0 + count - 1, in groups whose front pools fit kLuBatchBytes.
+static constexpr int64_t kLuBatchBytes = (int64_t)48 << 30;
+extern "C" int peec_lu_factor_slots(peec_ctx *ctx, int slot0, int count)
+{
+    if (count < 1) {
+        peec_set_error("peec_lu_factor_slots: count must be positive");
+        return PEEC_ERR_ARG;
+    }
+    int rc = lu_factor_check(ctx, slot0, count);
+    if (rc) return rc;
+    const int64_t per = std::max<int64_t>(ctx->lu.pool_total, 1) * 16;
+    const int group = (int)std::max<int64_t>(1, std::min<int64_t>(count, kLuBatchBytes / per));
+    for (int s0 = slot0; s0 < slot0 + count; s0 += group)
+        if ((rc = lu_factor_range(ctx, s0, std::min(group, slot0 + count - s0)))) return rc;
     return PEEC_OK;
 }
 

@@ -236,6 +236,13 @@ def lu_factor_slot(slot, ctx=None):
     L.check(L.lib().peec_lu_factor_slot(ctx.handle, int(slot)), "peec_lu_factor_slot")
 
 
+def lu_factor_slots(slot0, count, ctx=None):
+    """Factor the SS of frequency slots slot0 .. slot0 + count - 1 on the
+    device, all of them through each launch together."""
+    ctx = ctx or default_context()
+    L.check(L.lib().peec_lu_factor_slots(ctx.handle, int(slot0), int(count)), "peec_lu_factor_slots")
+
+
 def precond_rhs_batched(IS, rhs_per_slot, ctx=None):
     """M^-1 [0; 0; is_r] for the columns of IS (n x nrhs)."""
     ctx = ctx or default_context()

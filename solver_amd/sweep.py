@@ -316,14 +316,14 @@ def set_slots(solver, ws, R0=50.0, timings=None, sigma=None):
     esc = mesh.escalings
     tm = timings if timings is not None else {}
     _device_edges(solver, sigma)
+    t0 = time.perf_counter()
     for slot, w in enumerate(ws):
-        t0 = time.perf_counter()
         Yle = build_Yle_FFT(mesh.lumped_elements, mesh.port_nodes, esc, mesh.n, w / esc["freq"], R0)
         peec.frequency_set_slot_edges(slot, w, Yle, ctx=solver.ctx)
-        t1 = time.perf_counter()
-        peec.lu_factor_slot(slot, ctx=solver.ctx)
-        tm["freq_terms"] = tm.get("freq_terms", 0.0) + t1 - t0
-        tm["lu"] = tm.get("lu", 0.0) + time.perf_counter() - t1
+    t1 = time.perf_counter()
+    peec.lu_factor_slots(0, len(ws), ctx=solver.ctx)
+    tm["freq_terms"] = tm.get("freq_terms", 0.0) + t1 - t0
+    tm["lu"] = tm.get("lu", 0.0) + time.perf_counter() - t1
 
 
 def FFT_solver_QS_batched(freq, mesh, tol, batch=8, mixed=True, restart=100, max_cycles=20, inner_tol=0.0,

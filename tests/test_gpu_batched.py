@@ -124,3 +124,28 @@ def test_device_edge_impedances_match_host():
 def pself_fn():
     from test_gpu_operator import pself
     return pself
+
+
+def test_batched_lu_equals_per_slot():
+    """peec_lu_factor_slots (all slots through each launch of the
+    multifrontal schedule together) gives the same factors, bit for bit,
+    as factoring the slots one at a time: M^-1 x on every slot agrees."""
+    M = small_mesh()
+    S = sweep.Solver(M, ctx=peec.Context(0), device_lu=True)
+    fs = [1e7, 3e8, 2e9, 9e9]
+    ws = [2 * np.pi * f * M.escalings["freq"] for f in fs]
+    peec.frequency_slots(len(ws), ctx=S.ctx)
+    for slot, w in enumerate(ws):
+        Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+        peec.frequency_set_slot(slot, w, sweep.DZ_of(M, w), Yle, ctx=S.ctx)
+    N = M.m + M.ns + M.n
+    rng = np.random.default_rng(17)
+    X = rng.normal(size=(N, 8)) + 1j * rng.normal(size=(N, 8))
+    for slot in range(len(ws)):
+        peec.lu_factor_slot(slot, ctx=S.ctx)
+    one = peec.apply_K_batched(X, 2, precond=2, ctx=S.ctx)
+    peec.lu_factor_slots(0, len(ws), ctx=S.ctx)
+    together = peec.apply_K_batched(X, 2, precond=2, ctx=S.ctx)
+    print("batched LU vs per-slot: bitwise %s (rel %.2e)" % (np.array_equal(one, together), rel(together, one)),
+          flush=True)
+    assert np.array_equal(one, together)
