@@ -80,7 +80,7 @@ __global__ void k_zero(cd *y, int64_t n, int nrhs, int64_t ld)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    for (int r = 0; r < nrhs; r++) y[i + r * ld] = {0.0, 0.0};
+    y[i + (int64_t)blockIdx.y * ld] = {0.0, 0.0};  // one right-hand side per grid row
 }
 
 // Y1 = j w Y1 + DZ .* I + A Phi           (compute_Matrix_vector_fft.jl:29-31)
@@ -90,7 +90,8 @@ __global__ void k_y1(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const 
 {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m) return;
-    for (int r = 0; r < nrhs; r++) {
+    {
+        const int r = (int)blockIdx.y;  // one right-hand side per grid row
         const int sl = cm.slot(r);
         const cd *xr = x + r * ld;
         cd s = {0.0, 0.0};
@@ -107,7 +108,8 @@ __global__ void k_y2(cd *y, const cd *x, const int32_t *gp, const int32_t *gi, c
 {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    for (int r = 0; r < nrhs; r++) {
+    {
+        const int r = (int)blockIdx.y;  // one right-hand side per grid row
         const cd *xr = x + r * ld;
         cd a = {0.0, 0.0};
         for (int k = gp[s]; k < gp[s + 1]; k++) a = cadd(a, cscal(xr[phi0 + gi[k]], gv[k]));
@@ -123,7 +125,8 @@ __global__ void k_y3(cd *y, const cd *x, const int32_t *atp, const int32_t *ati,
     int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
-    for (int r = 0; r < nrhs; r++) {
+    {
+        const int r = (int)blockIdx.y;  // one right-hand side per grid row
         const int sl = cm.slot(r);
         const int32_t *yp = yp0 + sl * (n + 1);
         const double w = wv[sl];
@@ -150,7 +153,8 @@ __global__ void k_pre(const cd *y, const cd *invZ0, const double *invP, const in
     int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
-    for (int r = 0; r < nrhs; r++) {
+    {
+        const int r = (int)blockIdx.y;  // one right-hand side per grid row
         const int sl = cm.slot(r);
         const cd *invZ = invZ0 + sl * m;
         const double w = wv[sl];
@@ -175,7 +179,8 @@ __global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ0,
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t N = m + ns + n;
     if (i >= N) return;
-    for (int r = 0; r < nrhs; r++) {
+    {
+        const int r = (int)blockIdx.y;  // one right-hand side per grid row
         const cd *invZ = invZ0 + cm.slot(r) * m;
         const cd *zr = z + r * n;
         cd *yr = y + r * ld;
@@ -211,7 +216,8 @@ __global__ void k_y1h(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const
 {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m) return;
-    for (int r = 0; r < nrhs; r++) {
+    {
+        const int r = (int)blockIdx.y;  // one right-hand side per grid row
         const int sl = cm.slot(r);
         const cd *xr = x + r * ld;
         cd s = {0.0, 0.0};
@@ -229,7 +235,8 @@ __global__ void k_y2h(cd *y, const cd *x, const int32_t *gp, const int32_t *gi, 
 {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    for (int r = 0; r < nrhs; r++) {
+    {
+        const int r = (int)blockIdx.y;  // one right-hand side per grid row
         const cd *xr = x + r * ld;
         cd a = {0.0, 0.0};
         for (int k = gp[s]; k < gp[s + 1]; k++) a = cadd(a, cscal(xr[phi0 + gi[k]], gv[k]));
@@ -245,7 +252,8 @@ __global__ void k_y3h(cd *y, const cd *x, const int32_t *atp, const int32_t *ati
     int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
-    for (int r = 0; r < nrhs; r++) {
+    {
+        const int r = (int)blockIdx.y;  // one right-hand side per grid row
         const int sl = cm.slot(r);
         const int32_t *yp = yp0 + sl * (n + 1);
         const cd *xr = x + r * ld;
@@ -268,7 +276,8 @@ __global__ void k_adj_xform(cd *y, const double *wv, int64_t m, int64_t ns, int6
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m + ns + n) return;
-    for (int r = 0; r < nrhs; r++) {
+    {
+        const int r = (int)blockIdx.y;  // one right-hand side per grid row
         const double w = wv[cm.slot(r)];
         cd a = y[i + r * ld];
         if (stage == 0) {
@@ -949,19 +958,20 @@ extern "C" int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_
 // In place on y (length N per rhs, stride ld): y <- M^-1 y
 int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, const ColMap &cm)
 {
+    if (nrhs <= 0) return PEEC_OK;
     const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n;
     if (nrhs * n > (int64_t)ctx->vt1.n) {
         if (!ctx->vt1.alloc((size_t)(nrhs * n)) || !ctx->vt2.alloc((size_t)(nrhs * n))) return PEEC_ERR_NOMEM;
     }
     double2_t *b = ctx->vt1.p, *z = ctx->vt2.p;
-    hipLaunchKernelGGL(k_pre, dim3(nblk(n)), dim3(256), 0, ctx->stream, y, ctx->invZ.p, ctx->invP.p, ctx->At.ptr.p,
+    hipLaunchKernelGGL(k_pre, dim3(nblk(n), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, ctx->invZ.p, ctx->invP.p, ctx->At.ptr.p,
                        ctx->At.idx.p, ctx->At.val.p, ctx->G.ptr.p, ctx->G.idx.p, ctx->G.val.p, ctx->perm_r.p,
                        ctx->has_Rs ? ctx->Rs.p : nullptr, b, ctx->wv.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
     int rc;
     if ((rc = trsv_solve(ctx, ctx->Lf, true, b, z, nrhs, cm))) return rc;   // L c = b   (c in z)
     if ((rc = trsv_solve(ctx, ctx->Uf, false, z, b, nrhs, cm))) return rc;  // U z' = c  (z' in b)
-    hipLaunchKernelGGL(k_post, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, b, ctx->perm_c.p, ctx->invZ.p,
+    hipLaunchKernelGGL(k_post, dim3(nblk(m + ns + n), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, b, ctx->perm_c.p, ctx->invZ.p,
                        ctx->invP.p, ctx->A.ptr.p, ctx->A.idx.p, ctx->A.val.p, ctx->Gt.ptr.p, ctx->Gt.idx.p,
                        ctx->Gt.val.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
@@ -971,8 +981,9 @@ int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, const ColMa
 // y = K x (before the preconditioner), device vectors
 int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, const ColMap &cm)
 {
+    if (nrhs <= 0) return PEEC_OK;
     const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n, N = m + ns + n;
-    hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns)), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
+    hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
     PEEC_HIP_TRY(hipGetLastError());
     size_t need = 0;
     for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx->apps[a]));
@@ -988,11 +999,11 @@ int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t l
         if (rc) return rc;
     }
     const int64_t phi0 = m + ns;
-    hipLaunchKernelGGL(k_y1, dim3(nblk(m)), dim3(256), 0, ctx->stream, y, x, ctx->DZ.p, ctx->A.ptr.p, ctx->A.idx.p,
+    hipLaunchKernelGGL(k_y1, dim3(nblk(m), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, x, ctx->DZ.p, ctx->A.ptr.p, ctx->A.idx.p,
                        ctx->A.val.p, ctx->wv.p, m, phi0, nrhs, ld, cm);
-    hipLaunchKernelGGL(k_y2, dim3(nblk(ns)), dim3(256), 0, ctx->stream, y, x, ctx->Gt.ptr.p, ctx->Gt.idx.p,
+    hipLaunchKernelGGL(k_y2, dim3(nblk(ns), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, x, ctx->Gt.ptr.p, ctx->Gt.idx.p,
                        ctx->Gt.val.p, m, ns, phi0, nrhs, ld);
-    hipLaunchKernelGGL(k_y3, dim3(nblk(n)), dim3(256), 0, ctx->stream, y, x, ctx->At.ptr.p, ctx->At.idx.p,
+    hipLaunchKernelGGL(k_y3, dim3(nblk(n), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, x, ctx->At.ptr.p, ctx->At.idx.p,
                        ctx->At.val.p, ctx->Yle.ptr.p, ctx->Yle.idx.p, ctx->Yle.val.p, ctx->G.ptr.p, ctx->G.idx.p,
                        ctx->G.val.p, ctx->wv.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
@@ -1017,8 +1028,9 @@ int matvec_device(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int
 // y = K^H x (device vectors)
 int apply_KH(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, const ColMap &cm)
 {
+    if (nrhs <= 0) return PEEC_OK;
     const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n;
-    hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns)), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
+    hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
     PEEC_HIP_TRY(hipGetLastError());
     size_t need = 0;
     for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx->apps[a]));
@@ -1032,11 +1044,11 @@ int apply_KH(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t 
         if (rc) return rc;
     }
     const int64_t phi0 = m + ns;
-    hipLaunchKernelGGL(k_y1h, dim3(nblk(m)), dim3(256), 0, ctx->stream, y, x, ctx->DZ.p, ctx->A.ptr.p, ctx->A.idx.p,
+    hipLaunchKernelGGL(k_y1h, dim3(nblk(m), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, x, ctx->DZ.p, ctx->A.ptr.p, ctx->A.idx.p,
                        ctx->A.val.p, ctx->wv.p, m, phi0, nrhs, ld, cm);
-    hipLaunchKernelGGL(k_y2h, dim3(nblk(ns)), dim3(256), 0, ctx->stream, y, x, ctx->Gt.ptr.p, ctx->Gt.idx.p,
+    hipLaunchKernelGGL(k_y2h, dim3(nblk(ns), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, x, ctx->Gt.ptr.p, ctx->Gt.idx.p,
                        ctx->Gt.val.p, ctx->wv.p, m, ns, phi0, nrhs, ld, cm);
-    hipLaunchKernelGGL(k_y3h, dim3(nblk(n)), dim3(256), 0, ctx->stream, y, x, ctx->At.ptr.p, ctx->At.idx.p,
+    hipLaunchKernelGGL(k_y3h, dim3(nblk(n), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, x, ctx->At.ptr.p, ctx->At.idx.p,
                        ctx->At.val.p, ctx->Yle.ptr.p, ctx->Yle.idx.p, ctx->Yle.val.p, ctx->G.ptr.p, ctx->G.idx.p,
                        ctx->G.val.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
@@ -1046,13 +1058,14 @@ int apply_KH(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t 
 // In place: y <- M^-H y, through the forward factors of SS
 int apply_precond_adj(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, const ColMap &cm)
 {
+    if (nrhs <= 0) return PEEC_OK;
     const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n;
-    hipLaunchKernelGGL(k_adj_xform, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, ctx->wv.p, m, ns, n, nrhs,
+    hipLaunchKernelGGL(k_adj_xform, dim3(nblk(m + ns + n), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, ctx->wv.p, m, ns, n, nrhs,
                        ld, cm, 0);
     PEEC_HIP_TRY(hipGetLastError());
     int rc = apply_precond(ctx, y, nrhs, ld, cm);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_adj_xform, dim3(nblk(m + ns + n)), dim3(256), 0, ctx->stream, y, ctx->wv.p, m, ns, n, nrhs,
+    hipLaunchKernelGGL(k_adj_xform, dim3(nblk(m + ns + n), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, ctx->wv.p, m, ns, n, nrhs,
                        ld, cm, 1);
     PEEC_HIP_TRY(hipGetLastError());
     return PEEC_OK;
@@ -1159,7 +1172,7 @@ extern "C" int peec_copy_reference(peec_ctx *ctx, int64_t bytes)
         peec_set_error("peec_copy_reference: out of device memory");
         return PEEC_ERR_NOMEM;
     }
-    hipLaunchKernelGGL(k_zero, dim3(nblk(n)), dim3(256), 0, ctx->stream, a.p, n, 1, n);
+    hipLaunchKernelGGL(k_zero, dim3(nblk(n), 1), dim3(256), 0, ctx->stream, a.p, n, 1, n);
     for (int i = 0; i < 3; i++)
         hipLaunchKernelGGL(k_copy_reference, dim3(nblk(n)), dim3(256), 0, ctx->stream, b.p, a.p, n);
     PEEC_HIP_TRY(hipGetLastError());
