@@ -94,6 +94,21 @@ def z_error(Z, Zref):
     return float(nw), float(np.max(d))
 
 
+def host_cpu():
+    """Model name and logical CPU count of the host (the baseline uses one
+    thread: numpy's FFT and SuperLU run single-threaded)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return "%s, %d logical CPUs, 1 thread used" % (model, os.cpu_count() or 0)
+
+
 def cpu_baseline(mesh, cfg, freq, matvecs_per_port, sample_matvecs):
     """The oracle on the host, one core: one SS assembly + LU and
     `sample_matvecs` preconditioned FFT matvecs, extrapolated to one
@@ -122,7 +137,7 @@ def cpu_baseline(mesh, cfg, freq, matvecs_per_port, sample_matvecs):
     t_mv = (time.perf_counter() - t0) / sample_matvecs
     P = mesh.port_nodes.shape[0]
     per_freq = t_lu + P * matvecs_per_port * t_mv
-    return dict(value=1.0 / per_freq, unit="freq-point solves/s", cores=1, kind="port",
+    return dict(value=1.0 / per_freq, unit="freq-point solves/s", cores=1, kind="port", host=host_cpu(),
                 sample="oracle on %s at %.3g Hz: one SS assembly + LU (%.2f s) and %d preconditioned FFT matvec(s) "
                        "(%.2f s each), extrapolated to %d ports x %.1f matvecs per port (the plain path's average); "
                        "one-time integrals + spectra %.1f s not included"
@@ -233,6 +248,7 @@ def main():
     from solver_amd import _lib as L
     from solver_amd import configs, mesher, peec, sweep
     cfg = configs.CONFIGS[args.workload]()
+    t_start = time.perf_counter()
     t0 = time.perf_counter()
     mesh = mesher.mesh(cfg["grid"], cfg["materials"], cfg["cell"], cfg["origin"], cfg["ports"], p_self_fn=gpu_p_self)
     t_mesh = time.perf_counter() - t0
@@ -281,6 +297,7 @@ def main():
         log("step %d: %d frequencies %.3g..%.3g Hz done" % (i, len(b), b[0], b[-1]))
     sync()
     dt = time.perf_counter() - t0
+    t_e2e = time.perf_counter() - t_start
     ftimed = np.concatenate(batches[args.warmup:])
     Zt = np.concatenate(Zs, axis=2)
     nfreq = len(ftimed)
@@ -369,7 +386,8 @@ def main():
                    "plain_fp64_solves_per_s": plain_value,
                    "speedup_vs_plain_fp64": value / plain_value if plain_value else None,
                    "plain_matvecs_per_port": mv_per_port,
-                   "setup_s": round(t_setup, 3), "tune_s": round(t_tune, 3), "hip_graphs": not args.no_graphs,
+                   "setup_s": round(t_setup, 3), "tune_s": round(t_tune, 3),
+                   "end_to_end_s": round(t_e2e, 3), "hip_graphs": not args.no_graphs,
                    "batched_matvec_kernel_ms": round(mv_ms, 3), "fft_matvec_ms": round(fft_ms, 3),
                    "fft_matvec_GBs": round(fft_bytes / fft_ms / 1e6, 1)},
         "roofline": {"bound": "hbm", "kernel": dom["name"], "symbol": KERNEL_SYMBOLS[dom["name"]],

@@ -1459,10 +1459,29 @@ static int default_rpb(const PassDesc &d, int v)
     return (int)std::max<int64_t>(1, std::min<int64_t>(16, tiles * d.nrhs / 2048));
 }
 
+// The tuned choice for exactly nrhs right-hand sides, else the one tuned for
+// the nearest width within a factor 4 (GMRES applies the matvec to the
+// columns still iterating, whose count falls below the tuned width): its launch
+// shape and radix schedule, with right-hand sides per block scaled to keep
+// the number of blocks.  Launch shapes do not change results.
 static FftChoice tuned_choice(const peec_ctx *ctx, int kernel, int pass, bool fp32, int nrhs)
 {
     auto it = ctx->fft_tune.find(FftTuneKey{kernel, pass, fp32 ? 1 : 0, nrhs});
-    return it == ctx->fft_tune.end() ? FftChoice() : it->second;
+    if (it != ctx->fft_tune.end()) return it->second;
+    const FftChoice *best = nullptr;
+    int bw = 0;
+    for (const auto &kv : ctx->fft_tune) {
+        const FftTuneKey &k = kv.first;
+        if (k.kernel != kernel || k.pass != pass || k.fp32 != (fp32 ? 1 : 0)) continue;
+        if (!best || std::abs(k.nrhs - nrhs) < std::abs(bw - nrhs) || (std::abs(k.nrhs - nrhs) == std::abs(bw - nrhs) && k.nrhs > bw)) {
+            best = &kv.second;
+            bw = k.nrhs;
+        }
+    }
+    if (!best || nrhs * 4 < bw || nrhs > 4 * bw) return FftChoice();
+    FftChoice c = *best;
+    c.rpb = std::max(1, (int)(((int64_t)c.rpb * nrhs + bw / 2) / std::max(bw, 1)));
+    return c;
 }
 
 static const int kProfKind[5] = {PK_FFT_X_FWD, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV};
