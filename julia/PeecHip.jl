@@ -157,6 +157,50 @@ function peec_gmres_batched(ctx, B::Matrix{ComplexF64}, rhs_per_slot, tol; X0 = 
     X, flag, relres, iters, cycles
 end
 
+# Per-edge data for DZ on the device (FFT_solver_QS_S_type.jl:106-115):
+# R, Cd m x 4 (R real, Cd complex), Zs0 (m), Lp (m), surface flags (m).
+function peec_edge_terms_set(ctx, R::Matrix{Float64}, Cd::Matrix{ComplexF64}, Zs0::Vector{ComplexF64},
+                             Lp::Vector{Float64}, surface::AbstractVector{Bool}, esc_R, esc_freq)
+    m = length(Lp)
+    Rt = Matrix(transpose(R)); Ct = Matrix(transpose(Cd))   # 4 per edge, edge after edge
+    sf = Int8.(surface)
+    rc = GC.@preserve Rt Ct Zs0 Lp sf ccall((:peec_edge_terms_set, LIBPEEC), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{Float64}, Ptr{Int8}, Float64, Float64),
+        ctx, m, Rt, Ct, Zs0, Lp, sf, esc_R, esc_freq)
+    peec_check(rc, "peec_edge_terms_set")
+end
+
+function peec_frequency_set_slot_edges(ctx, slot, w, Yle::SparseMatrixCSC{ComplexF64,Int64})
+    Y = CscC128(Yle)
+    rc = GC.@preserve Yle ccall((:peec_frequency_set_slot_edges, LIBPEEC), Cint,
+        (Ptr{Cvoid}, Cint, Float64, Ref{CscC128}), ctx, slot, w, Y)
+    peec_check(rc, "peec_frequency_set_slot_edges")
+end
+
+peec_lu_factor_slots(ctx, slot0, count) =
+    peec_check(ccall((:peec_lu_factor_slots, LIBPEEC), Cint, (Ptr{Cvoid}, Cint, Cint), ctx, slot0, count),
+               "peec_lu_factor_slots")
+
+# One batch of the sweep on the device: port_nodes is P x 2 (1-based node
+# indices, as the reference's port_nodes); returns V[s, c1, c2], the node
+# voltage difference of port c2 in the solution of (slot s, port c1).
+function peec_sweep_batch(ctx, port_nodes::AbstractMatrix, Is, nslots; x0 = nothing, tol = 1e-9, restart = 100,
+                          max_cycles = 20, mixed = true, inner_tol = 0.0)
+    P = size(port_nodes, 1)
+    nrhs = nslots * P
+    pn = Int64[port_nodes[c, k] - 1 for k in 1:2, c in 1:P]   # (n1, n2) per port, 0-based
+    vp = zeros(ComplexF64, P, nrhs)                           # vp[c2, r]
+    flag = zeros(Cint, nrhs); relres = zeros(Float64, nrhs); iters = zeros(Cint, nrhs); cycles = zeros(Cint, nrhs)
+    X0 = x0 === nothing ? C_NULL : Matrix{ComplexF64}(x0)
+    rc = GC.@preserve pn vp flag relres iters cycles X0 ccall((:peec_sweep_batch, LIBPEEC), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Int64}, Float64, Ptr{ComplexF64}, Float64, Cint, Cint, Cint, Float64,
+         Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{Cint}, Ptr{Float64}, Ptr{Cint}, Ptr{Cint}),
+        ctx, P, pn, Is, X0, tol, restart, max_cycles, mixed, inner_tol, vp, C_NULL, flag, relres, iters, cycles)
+    peec_check(rc, "peec_sweep_batch")
+    V = permutedims(reshape(vp, P, P, nslots), (3, 2, 1))     # V[s, c1, c2]
+    V, flag, relres, iters, cycles
+end
+
 peec_fft_tune(ctx, nrhs; fp64 = true, fp32 = true) =
     peec_check(ccall((:peec_fft_tune, LIBPEEC), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}), ctx, nrhs,
                      Cint(fp64) | (Cint(fp32) << 1), C_NULL), "peec_fft_tune")
