@@ -114,7 +114,6 @@ def cpu_baseline(mesh, cfg, freq, matvecs_per_port, sample_matvecs):
     `sample_matvecs` preconditioned FFT matvecs, extrapolated to one
     frequency point of the reference algorithm (per port, the plain path's
     average matvec count)."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
     from oracle import fft_solver as OF
     t0 = time.perf_counter()
     rows = OF.kernel_rows(mesh.origin, mesh.Nx, mesh.Ny, mesh.Nz, mesh.sx, mesh.sy, mesh.sz)
