@@ -45,11 +45,13 @@ PEAK_HBM_GBS = 8000.0
 # known size) measured with rocprofv3 --pmc on the bench command, per
 # (workload, profile kind); the summaries and commands are in
 # profiles/traffic_trsv_gather_batched_96x96x24.md and
-# profiles/traffic_fft_passes_large_192x192x48.md.
+# profiles/kernels_fft_large_192x192x48.md (complex64 passes).
 MEASURED_TRAFFIC = {("batched_96x96x24", "trsv_gather"): 145.78e6,
                     ("batched_96x96x24", "trsv_block_L"): 26.35e6,
                     ("batched_96x96x24", "trsv_block_U"): 27.34e6,
-                    ("large_192x192x48", "fft_z_conv"): 577.2e6}
+                    ("large_192x192x48", "fft_z_conv"): 562.8e6,
+                    ("large_192x192x48", "fft_y_fwd"): 353.0e6,
+                    ("large_192x192x48", "fft_y_inv"): 353.1e6}
 KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv_gather",
               "trsv_block_L", "trsv_block_U", "lu_factor", "fft_yz_conv"]
 # kernel symbol behind each profile kind (the FFT pass kernel is
