@@ -24,18 +24,30 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
 PER_FILE = {"integrals.hip": ["-ffp-contract=off"], "fft.hip": ["-ffp-contract=off"]}
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
+    """Compile every translation unit (in parallel, `jobs` at a time; at
+    most 8 and the host's CPU count) and link the library."""
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(BUILD, src + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [HIPCC, *COMMON, *PER_FILE.get(src, []), *lang, "-c", path, "-o", obj]
+        cmds.append([HIPCC, *COMMON, *PER_FILE.get(src, []), *lang, "-c", path, "-o", obj])
+        objs.append(obj)
+    jobs = jobs or max(1, min(8, os.cpu_count() or 1))
+    running, failed = [], None
+    for cmd in cmds:
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
-        objs.append(obj)
+        running.append((cmd, subprocess.Popen(cmd)))
+        while len([p for _, p in running if p.poll() is None]) >= jobs:
+            running[[p.poll() is None for _, p in running].index(True)][1].wait()
+    for cmd, p in running:
+        if p.wait() != 0 and failed is None:
+            failed = cmd
+    if failed is not None:
+        raise subprocess.CalledProcessError(1, failed)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     subprocess.check_call(cmd)
