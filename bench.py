@@ -46,14 +46,11 @@ PEAK_HBM_GBS = 8000.0
 # (workload, profile kind); the summaries and commands are in
 # profiles/traffic_trsv_gather_batched_96x96x24.md and
 # profiles/kernels_fft_large_192x192x48.md (complex64 passes).
-MEASURED_TRAFFIC = {("batched_96x96x24", "trsv_gather"): 145.78e6,
-                    ("batched_96x96x24", "trsv_block_L"): 26.35e6,
-                    ("batched_96x96x24", "trsv_block_U"): 27.34e6,
-                    ("large_192x192x48", "fft_z_conv"): 562.8e6,
+MEASURED_TRAFFIC = {("large_192x192x48", "fft_z_conv"): 562.8e6,
                     ("large_192x192x48", "fft_y_fwd"): 353.0e6,
                     ("large_192x192x48", "fft_y_inv"): 353.1e6}
 KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv_gather",
-              "trsv_block_L", "trsv_block_U", "lu_factor", "fft_yz_conv"]
+              "trsv_block_L", "trsv_block_U", "lu_factor", "fft_yz_conv", "trsv_panel"]
 # kernel symbol behind each profile kind (the FFT pass kernel is
 # instantiated per precision T, mode -- 0 forward, 1 inverse, 2 forward *
 # spectrum * inverse -- and launch shape TPL, LPT, LMAX chosen by the tuner)
@@ -62,7 +59,7 @@ KERNEL_SYMBOLS = {"fft_x_fwd_scatter": "k_fft_pass<T, 0, TPL, LPT, LMAX>", "fft_
                   "fft_x_inv_gather": "k_fft_pass<T, 1, TPL, LPT, LMAX>", "trsv_gather": "k_trsv_gather4",
                   "trsv_block_L": "k_trsv_block<true>", "trsv_block_U": "k_trsv_block<false>",
                   "lu_factor": "peec_lu_factor (all its kernels)",
-                  "fft_yz_conv": "k_yz_conv<float, MODE, 32, LPT, 256, LZ>"}
+                  "fft_yz_conv": "k_yz_conv<float, MODE, 32, LPT, 256, LZ>", "trsv_panel": "k_trsv_panel"}
 
 
 def log(*a):

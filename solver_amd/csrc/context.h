@@ -187,6 +187,8 @@ struct CsrC {
 
 constexpr int kTrsvMaxBlock = 64;   // rows of one block (one per lane)
 constexpr int kTrsvSeg = 256;       // entries summed by one wavefront (4 per lane)
+constexpr int kTrsvPanelCols = 128; // panel blocks: columns of one segment (2 per lane)
+constexpr int kTrsvPanelRows = 16;  // panel blocks: rows one wavefront sums against one x load
 
 // One triangular factor laid out for the blocked solve (trsv.hip).  The
 // factor order is cut into row blocks of at most 64 rows; blocks are
@@ -194,7 +196,11 @@ constexpr int kTrsvSeg = 256;       // entries summed by one wavefront (4 per la
 // U).  Entries that reach into other blocks are kept per row (CSR, ob_*)
 // and summed in segments of at most kTrsvSeg entries, one wavefront per
 // segment; the entries inside a block are kept as a dense s x s
-// column-major tile.
+// column-major tile.  A block whose rows all reach the same columns (the
+// rows of one front's panel) is a panel block: its segments are chunks of
+// kTrsvPanelCols columns, and one wavefront sums a chunk for up to
+// kTrsvPanelRows rows against one load of x.  In each level the segments
+// of the other rows come first, then those of the panel blocks.
 struct TriFactor {
     int64_t n = 0;
     int nblocks = 0, max_block = 0;
@@ -204,7 +210,12 @@ struct TriFactor {
     double nnz = 0.0;                        // stored off-diagonal entries + diagonal
     std::vector<int32_t> level_ptr;          // host: blocks of level l are binfo entries level_ptr[l] ..
     std::vector<int64_t> seg_level_ptr;      // host: segments of level l
-    std::vector<double> lev_ob, lev_ib, lev_rows;  // host, per level: off-block and in-block entries, rows
+    std::vector<int64_t> seg_level_np;       // host: end of level l's segments outside panel blocks
+    std::vector<int64_t> item_level_ptr;     // host: panel items of level l
+    std::vector<double> lev_ob, lev_pob, lev_ib, lev_rows;  // host, per level: off-block entries (outside
+                                                            // and inside panel blocks), in-block entries, rows
+    int64_t nitems = 0;
+    DevBuf<int32_t> pitem;                   // per panel item: first segment, rows, segment stride, columns
     DevBuf<int32_t> binfo;                   // per block, in level order: r0, rows, dense offset (2 x 32 bit)
     DevBuf<int64_t> seg_k0, seg_k1;          // entry range of each segment
     DevBuf<int32_t> row_seg;                 // per row: first and end segment
@@ -238,7 +249,7 @@ struct ProfRec {
 };
 enum ProfKind {
     PK_FFT_X_FWD = 0, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV, PK_TRSV_GATHER, PK_TRSV_BLOCK_L, PK_TRSV_BLOCK_U, PK_LU,
-    PK_FFT_YZ_CONV, PK_COUNT
+    PK_FFT_YZ_CONV, PK_TRSV_PANEL, PK_COUNT
 };
 
 struct peec_ctx {

@@ -186,6 +186,146 @@ __global__ void __launch_bounds__(256) k_trsv_gather4(const int64_t *seg_k0, con
     }
 }
 
+// a + b summed over the lane pairs (l, l + 32): lanes 0..31 return the sum
+// of a, lanes 32..63 that of b.  v_permlane32_swap exchanges the upper half
+// of its first operand with the lower half of its second, so after the swap
+// each lane holds both terms of its own sum and no lane selects.
+__device__ __forceinline__ double swap32_add(double a, double b)
+{
+    const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+    const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)ua, (uint32_t)ub, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), false, false);
+    const double na = __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi[0] << 32) | (uint32_t)lo[0]);
+    const double nb = __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi[1] << 32) | (uint32_t)lo[1]);
+    return na + nb;
+}
+
+// The same over the lane pairs (l, l + 16) of each 32-lane half:
+// v_permlane16_swap exchanges the odd 16-lane rows of its first operand
+// with the even rows of its second.  Even rows return the sum of a, odd
+// rows that of b.
+__device__ __forceinline__ double swap16_add(double a, double b)
+{
+    const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+    const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)ua, (uint32_t)ub, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), false, false);
+    const double na = __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi[0] << 32) | (uint32_t)lo[0]);
+    const double nb = __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi[1] << 32) | (uint32_t)lo[1]);
+    return na + nb;
+}
+
+// Sixteen lane sums reduced together by recursive halving: after the steps
+// at lane distance 32, 16, 8 (row mirror) and 4 (half-row mirror) each lane
+// keeps one of them, two quad swaps finish it, and lane l holds the total of
+// s[(l >> 2) & 15] (so do lanes l ^ 1, l ^ 2, l ^ 3).  24 lane exchanges
+// for 16 sums instead of 96.
+__device__ __forceinline__ double reduce16(const double (&s)[16], int lane)
+{
+    double t[8], u[4], w[2];
+#pragma unroll
+    for (int j = 0; j < 8; j++) t[j] = swap32_add(s[j], s[j + 8]);
+#pragma unroll
+    for (int j = 0; j < 4; j++) u[j] = swap16_add(t[j], t[j + 4]);
+    const bool h8 = lane & 8, h4 = lane & 4;
+#pragma unroll
+    for (int j = 0; j < 2; j++) w[j] = (h8 ? u[j + 2] : u[j]) + dpp_move<kDppRowMirror>(h8 ? u[j] : u[j + 2]);
+    double y = (h4 ? w[1] : w[0]) + dpp_move<kDppRowHalfMirror>(h4 ? w[0] : w[1]);
+    y += dpp_move<kDppQuadSwap1>(y);
+    y += dpp_move<kDppQuadSwap2>(y);
+    return y;
+}
+
+// One right-hand side: the complex sum (s[0], s[1]) over the wavefront;
+// lanes 0..31 return the real part, lanes 32..63 the imaginary part.
+__device__ __forceinline__ double reduce2(const double (&s)[2])
+{
+    double t = swap32_add(s[0], s[1]);
+    t = swap16_add(t, t);
+    return row16_sum(t);
+}
+
+// Panel blocks: the rows of a block reach the same columns, so a wavefront
+// loads x at one chunk of at most kTrsvPanelCols columns (2 per lane) for R
+// right-hand sides of a slot (8, or 1 when slots have few) once and sums
+// the chunk for every row of its item against it; only the values are read
+// per row, and the next row's are in flight while a row is reduced.
+// items[w] = (segment of the first row's chunk, rows, segment stride between
+// rows, entries per row); row rho's chunk is segment item.x + rho * item.z
+// and its values start item.w entries after the previous row's.  Wavefront
+// w of the launch takes item w % nitems of slot w / nitems.
+template <int R>
+__global__ void __launch_bounds__(256) k_trsv_panel(const int4 *items, int64_t nitems, const int64_t *seg_k0,
+                                                    const int64_t *seg_k1, const int32_t *idx, const cd *val,
+                                                    int64_t ob_n, const cd *x, cd *part, int64_t pstride, int64_t n,
+                                                    int nrhs, ColMap cm)
+{
+    static_assert(R == 8 || R == 1, "8 or 1 right-hand sides per pass");
+    // items of one chunk (same columns of x) are adjacent: keep them on one XCD
+    const int64_t w = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    if (w >= nitems * cm.nsl) return;  // whole wavefront: the kernel has no barrier
+    const int slot = (int)(w / nitems);
+    const int4 it = items[w % nitems];
+    const int rbeg = cm.first(slot), rend = cm.end(slot, nrhs);
+    if (rbeg >= rend) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t c0 = seg_k0[it.x];
+    const int cnt = (int)(seg_k1[it.x] - c0);
+    int c[2];
+    bool ok[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        ok[i] = lane + 64 * i < cnt;
+        c[i] = ok[i] ? idx[c0 + lane + 64 * i] : 0;
+    }
+    const cd *vs = val + (int64_t)slot * ob_n + c0;
+    double *pd = reinterpret_cast<double *>(part);
+    // the lanes that hold a total after the reduction, and which one
+    constexpr int kShift = R == 8 ? 2 : 5;
+    const bool writer = (lane & ((1 << kShift) - 1)) == 0;
+    const int vi = (lane >> kShift) & (2 * R - 1), gi = vi >> 1;
+    for (int r0 = rbeg; r0 < rend; r0 += R) {
+        cd xv[R][2];
+#pragma unroll
+        for (int g = 0; g < R; g++) {
+            const cd *xr = x + (int64_t)min(r0 + g, rend - 1) * n;
+#pragma unroll
+            for (int i = 0; i < 2; i++) xv[g][i] = xr[c[i]];
+        }
+        cd v[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) v[i] = ok[i] ? vs[lane + 64 * i] : cd{0.0, 0.0};
+        for (int rho = 0; rho < it.y; rho++) {
+            cd vn[2];
+            const int64_t kn = (int64_t)min(rho + 1, it.y - 1) * it.w;
+#pragma unroll
+            for (int i = 0; i < 2; i++) vn[i] = ok[i] ? vs[kn + lane + 64 * i] : cd{0.0, 0.0};
+            double s[2 * R];
+#pragma unroll
+            for (int g = 0; g < R; g++) {
+                double ax = 0.0, ay = 0.0;
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    ax = __builtin_fma(v[i].x, xv[g][i].x, ax);
+                    ax = __builtin_fma(-v[i].y, xv[g][i].y, ax);
+                    ay = __builtin_fma(v[i].x, xv[g][i].y, ay);
+                    ay = __builtin_fma(v[i].y, xv[g][i].x, ay);
+                }
+                s[2 * g] = ax;
+                s[2 * g + 1] = ay;
+            }
+            double tot;
+            if constexpr (R == 8)
+                tot = reduce16(s, lane);
+            else
+                tot = reduce2(s);
+            const int64_t sg = it.x + (int64_t)rho * it.z;
+            if (writer && r0 + gi < rend) pd[2 * (sg + (int64_t)(r0 + gi) * pstride) + (vi & 1)] = tot;
+#pragma unroll
+            for (int i = 0; i < 2; i++) v[i] = vn[i];
+        }
+    }
+}
+
 // Value of v in lane l (l uniform across the wavefront).
 __device__ __forceinline__ double lane_value(double v, int l)
 {
@@ -368,12 +508,26 @@ int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std
         std::vector<int32_t> lpos(lptr.begin(), lptr.end() - 1);
         for (int B = 0; B < nb; B++) lblk[(size_t)lpos[(size_t)level[(size_t)B]]++] = B;
     }
+    // panel blocks: two or more rows, all with the same off-block columns
+    std::vector<char> panel((size_t)std::max(nb, 1), 0);
+    for (int B = 0; B < nb; B++) {
+        const int r0 = blk[(size_t)B], r1 = blk[(size_t)B + 1];
+        const int64_t K = optr[(size_t)r0 + 1] - optr[(size_t)r0];
+        bool same = r1 - r0 >= 2 && K > 0;
+        for (int i = r0 + 1; same && i < r1; i++)
+            same = optr[(size_t)i + 1] - optr[(size_t)i] == K &&
+                   std::equal(oidx.begin() + optr[(size_t)i], oidx.begin() + optr[(size_t)i] + K,
+                              oidx.begin() + optr[(size_t)r0]);
+        panel[(size_t)B] = same;
+    }
     f->lev_ob.assign((size_t)nlev, 0.0);
+    f->lev_pob.assign((size_t)nlev, 0.0);
     f->lev_ib.assign((size_t)nlev, 0.0);
     f->lev_rows.assign((size_t)nlev, 0.0);
-    // segments, numbered level by level
-    std::vector<int64_t> sk0, sk1, slev((size_t)nlev + 1, 0);
-    std::vector<int32_t> rs((size_t)n * 2, 0);
+    // segments, numbered level by level: in each level first the rows
+    // outside panel blocks, then the panel blocks row by row
+    std::vector<int64_t> sk0, sk1, slev((size_t)nlev + 1, 0), snp((size_t)nlev, 0), ilev((size_t)nlev + 1, 0);
+    std::vector<int32_t> rs((size_t)n * 2, 0), items;
     std::vector<int32_t> binfo((size_t)std::max(nb, 1) * 4, 0);
     for (int l = 0; l < nlev; l++) {
         for (int e = lptr[(size_t)l]; e < lptr[(size_t)l + 1]; e++) {
@@ -384,6 +538,7 @@ int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std
             binfo[(size_t)e * 4 + 3] = (int32_t)(uint32_t)((uint64_t)doff[(size_t)B] >> 32);
             f->lev_ib[(size_t)l] += ibn[(size_t)B];
             f->lev_rows[(size_t)l] += blk[(size_t)B + 1] - blk[(size_t)B];
+            if (panel[(size_t)B]) continue;
             for (int i = blk[(size_t)B]; i < blk[(size_t)B + 1]; i++) {
                 f->lev_ob[(size_t)l] += (double)(optr[(size_t)i + 1] - optr[(size_t)i]);
                 rs[(size_t)i * 2] = (int32_t)sk0.size();
@@ -394,7 +549,33 @@ int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std
                 rs[(size_t)i * 2 + 1] = (int32_t)sk0.size();
             }
         }
+        snp[(size_t)l] = (int64_t)sk0.size();
+        for (int e = lptr[(size_t)l]; e < lptr[(size_t)l + 1]; e++) {
+            const int B = lblk[(size_t)e];
+            if (!panel[(size_t)B]) continue;
+            const int r0 = blk[(size_t)B], s = blk[(size_t)B + 1] - r0;
+            const int64_t K = optr[(size_t)r0 + 1] - optr[(size_t)r0];
+            const int64_t nch = (K + kTrsvPanelCols - 1) / kTrsvPanelCols;
+            const int64_t base = (int64_t)sk0.size();
+            f->lev_pob[(size_t)l] += (double)K * s;
+            for (int i = r0; i < r0 + s; i++) {
+                rs[(size_t)i * 2] = (int32_t)sk0.size();
+                for (int64_t k = optr[(size_t)i]; k < optr[(size_t)i + 1]; k += kTrsvPanelCols) {
+                    sk0.push_back(k);
+                    sk1.push_back(std::min<int64_t>(k + kTrsvPanelCols, optr[(size_t)i + 1]));
+                }
+                rs[(size_t)i * 2 + 1] = (int32_t)sk0.size();
+            }
+            for (int64_t c = 0; c < nch; c++)
+                for (int g = 0; g < s; g += kTrsvPanelRows) {
+                    items.push_back((int32_t)(base + g * nch + c));
+                    items.push_back(std::min(kTrsvPanelRows, s - g));
+                    items.push_back((int32_t)nch);
+                    items.push_back((int32_t)K);
+                }
+        }
         slev[(size_t)l + 1] = (int64_t)sk0.size();
+        ilev[(size_t)l + 1] = (int64_t)(items.size() / 4);
     }
     if (sk0.size() > (size_t)INT32_MAX) {
         peec_set_error("%s: too many segments", what);
@@ -404,6 +585,9 @@ int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std
     f->nblocks = nb;
     f->level_ptr = lptr;
     f->seg_level_ptr = slev;
+    f->seg_level_np = snp;
+    f->item_level_ptr = ilev;
+    f->nitems = (int64_t)(items.size() / 4);
     f->nseg = (int64_t)sk0.size();
     f->max_block = 0;
     for (int B = 0; B < nb; B++) f->max_block = std::max(f->max_block, blk[(size_t)B + 1] - blk[(size_t)B]);
@@ -412,16 +596,19 @@ int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std
         sk0.push_back(0);
         sk1.push_back(0);
     }
+    if (items.empty()) items.assign(4, 0);
     f->nslots = 1;
     f->ob_n = (int64_t)oidx.size();
     f->dense_n = (int64_t)std::max<size_t>(src->dense.size(), 1);
     if (!f->binfo.alloc(binfo.size()) || !f->seg_k0.alloc(sk0.size()) || !f->seg_k1.alloc(sk1.size()) ||
         !f->row_seg.alloc(rs.size()) || !f->ob_idx.alloc(oidx.size()) || !f->ob_val.alloc((size_t)f->ob_n) ||
-        !f->dense.alloc((size_t)f->dense_n) || !f->diag.alloc((size_t)std::max<int64_t>(n, 1))) {
+        !f->dense.alloc((size_t)f->dense_n) || !f->diag.alloc((size_t)std::max<int64_t>(n, 1)) ||
+        !f->pitem.alloc(items.size())) {
         peec_set_error("%s: out of device memory", what);
         return PEEC_ERR_NOMEM;
     }
     PEEC_HIP_TRY(hipMemcpy(f->binfo.p, binfo.data(), binfo.size() * 4, hipMemcpyHostToDevice));
+    PEEC_HIP_TRY(hipMemcpy(f->pitem.p, items.data(), items.size() * 4, hipMemcpyHostToDevice));
     PEEC_HIP_TRY(hipMemcpy(f->seg_k0.p, sk0.data(), sk0.size() * 8, hipMemcpyHostToDevice));
     PEEC_HIP_TRY(hipMemcpy(f->seg_k1.p, sk1.data(), sk1.size() * 8, hipMemcpyHostToDevice));
     PEEC_HIP_TRY(hipMemcpy(f->row_seg.p, rs.data(), rs.size() * 4, hipMemcpyHostToDevice));
@@ -520,10 +707,26 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
     const int gps = std::min((cols + 3) / 4, 16);
     const int nlev = (int)f.level_ptr.size() - 1;
     for (int l = 0; l < nlev; l++) {
-        const int64_t s0 = f.seg_level_ptr[(size_t)l], ns = f.seg_level_ptr[(size_t)l + 1] - s0;
+        const int64_t s0 = f.seg_level_ptr[(size_t)l], ns = f.seg_level_np[(size_t)l] - s0;
+        const int64_t nps = f.seg_level_ptr[(size_t)l + 1] - f.seg_level_np[(size_t)l];
+        const int64_t i0 = f.item_level_ptr[(size_t)l], ni = f.item_level_ptr[(size_t)l + 1] - i0;
         // algorithmic bytes of a gather: its entries (value + index) and the
         // partial sums written; of a block solve: the in-block entries, the
         // diagonal, b and x, and the partial sums read
+        if (ni > 0) {
+            int pi = ctx->prof_begin(PK_TRSV_PANEL, f.lev_pob[(size_t)l] * (4.0 + 16.0 * nsl) + (double)nps * 16.0 * nrhs);
+            const int64_t waves = ni * nsl;
+            const int4 *it = reinterpret_cast<const int4 *>(f.pitem.p) + i0;
+            if (cols >= 4)
+                hipLaunchKernelGGL(k_trsv_panel<8>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, it,
+                                   ni, f.seg_k0.p, f.seg_k1.p, f.ob_idx.p, f.ob_val.p, f.ob_n, (const cd *)x, part,
+                                   pstride, f.n, nrhs, cm);
+            else
+                hipLaunchKernelGGL(k_trsv_panel<1>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, it,
+                                   ni, f.seg_k0.p, f.seg_k1.p, f.ob_idx.p, f.ob_val.p, f.ob_n, (const cd *)x, part,
+                                   pstride, f.n, nrhs, cm);
+            ctx->prof_end(pi);
+        }
         if (ns > 0) {
             int pi = ctx->prof_begin(PK_TRSV_GATHER, f.lev_ob[(size_t)l] * (4.0 + 16.0 * nsl) + (double)ns * 16.0 * nrhs);
             if (nrhs >= 4)
@@ -538,7 +741,7 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
         }
         int pi = ctx->prof_begin(lower ? PK_TRSV_BLOCK_L : PK_TRSV_BLOCK_U,
                                  (f.lev_ib[(size_t)l] + f.lev_rows[(size_t)l]) * 16.0 * nsl +
-                                     f.lev_rows[(size_t)l] * 32.0 * nrhs + (double)ns * 16.0 * nrhs);
+                                     f.lev_rows[(size_t)l] * 32.0 * nrhs + (double)(ns + nps) * 16.0 * nrhs);
         const int nbl = f.level_ptr[(size_t)l + 1] - f.level_ptr[(size_t)l];
         const int4 *bi = reinterpret_cast<const int4 *>(f.binfo.p) + f.level_ptr[(size_t)l];
         const int2 *rsg = reinterpret_cast<const int2 *>(f.row_seg.p);

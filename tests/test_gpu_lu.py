@@ -99,3 +99,50 @@ def test_device_lu_crossbar_sweep_matches_oracle():
         err = np.max(np.abs(out_g["Z"][..., k] - out_o["Z"][..., k]) / np.abs(out_o["Z"][..., k]))
         print("device LU crossbar f %.3e Z rel err %.2e tol %.1e" % (f, err, tol), flush=True)
         assert err < tol
+
+
+def test_panel_sums_match_oracle_preconditioner():
+    """With the device factors every U block is a panel block (its rows are
+    one front's pivot rows and reach the same columns), so the backward
+    sweep sums them as panels (profile kind 10) and the forward sweep's
+    rows through the per-row gathers (kind 5).  M^-1 x over 2 slots x 9
+    right-hand sides (one group of 8 and a remainder of 1 per slot) matches
+    the oracle's precond_3_3_vector column by column."""
+    import ctypes
+    from solver_amd import _lib, configs, mesher, peec
+    from test_gpu_operator import pself
+    cell = (0.25e-3, 0.25e-3, 0.05e-3)
+    g, mats, _ = configs._layered(12, 10, 6, cell, 4, 0.45, 4.4, 0.02, seed=3)
+    M = mesher.mesh(g, mats, cell, (0.0, 0.0, 0.0), configs._x_face_ports(g, cell, 4), p_self_fn=pself)
+    S = sweep.Solver(M, device_lu=True)  # 526 nodes in 16 fronts
+    rows = OF.kernel_rows(M.origin, M.Nx, M.Ny, M.Nz, M.sx, M.sy, M.sz)
+    spec = OF.spectra(rows, M.Nx, M.Ny, M.Nz)
+    fs, rps = [2e8, 4e9], 9
+    ws = [2 * np.pi * f * M.escalings["freq"] for f in fs]
+    peec.frequency_slots(len(ws), ctx=S.ctx)
+    for slot, w in enumerate(ws):
+        Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+        peec.frequency_set_slot(slot, w, sweep.DZ_of(M, w), Yle, ctx=S.ctx)
+        peec.lu_factor_slot(slot, ctx=S.ctx)
+    N = M.m + M.ns + M.n
+    rng = np.random.default_rng(17)
+    X = rng.normal(size=(N, rps * len(ws))) + 1j * rng.normal(size=(N, rps * len(ws)))
+    nk = 11
+    ms, by, cnt = (ctypes.c_double * nk)(), (ctypes.c_double * nk)(), (ctypes.c_int64 * nk)()
+    _lib.check(_lib.lib().peec_profile(S.ctx.handle, 1, ms, by, cnt, nk), "peec_profile")
+    Y = peec.apply_K_batched(X, rps, precond=2, ctx=S.ctx)
+    _lib.check(_lib.lib().peec_profile(S.ctx.handle, 0, ms, by, cnt, nk), "peec_profile")
+    print("launches per kind", list(cnt), flush=True)
+    assert cnt[10] > 0 and cnt[5] > 0
+    m, ns = M.m, M.ns
+    for slot, w in enumerate(ws):
+        op = OF.Operator(M, spec)
+        Yle = OF.build_Yle(M.lumped_elements, M.port_nodes, M.escalings["Yle"], M.n, w / M.escalings["freq"], 50.0)
+        op.set_frequency(w, OF.frequency_terms(M, w), Yle, order=S.order)
+        cond = np.linalg.cond(op.SS.toarray())
+        tol = max(1e-12, cond * np.finfo(float).eps)
+        for c in range(slot * rps, (slot + 1) * rps):
+            x = X[:, c]
+            e = rel(Y[:, c], op.precond(x[:m], x[m:m + ns], x[m + ns:]))
+            print("slot %d column %d: M^-1 x vs oracle %.2e (tol %.1e)" % (slot, c, e, tol), flush=True)
+            assert e < tol
