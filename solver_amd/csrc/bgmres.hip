@@ -285,6 +285,7 @@ struct BWork {
     bool adjoint = false;
     MatvecGraphs graphs;
     std::vector<int32_t> cols;  // the compacted columns now uploaded
+    int wmax = 0;               // most of them in one slot
     void graphs_init()
     {
         if (!graphs.ctx) {
@@ -327,12 +328,16 @@ struct BWork {
             for (int sl = 0; sl < nsl; sl++) h[(size_t)(2 * nrhs + sl + 1)] += h[(size_t)(2 * nrhs + sl)];
             PEEC_HIP_TRY(hipMemcpyAsync(cmap.p, h.data(), h.size() * 4, hipMemcpyHostToDevice, ctx->stream));
             cols = c;
+            wmax = 0;
+            for (int sl = 0; sl < nsl; sl++)
+                wmax = std::max(wmax, h[(size_t)(2 * nrhs + sl + 1)] - h[(size_t)(2 * nrhs + sl)]);
         }
         ColMap cm;
         cm.rslot = cmap.p + nrhs;
         cm.slot_ptr = cmap.p + 2 * nrhs;
         cm.rps = rps;
         cm.nsl = nsl;
+        cm.wmax = wmax;
         hipLaunchKernelGGL(k_bgather, dim3(nb256(N), nc), dim3(256), 0, ctx->stream, T.p, N, x, xld, cmap.p, N);
         PEEC_HIP_TRY(hipGetLastError());
         int rc = graphs.run_cols(T.p, Wc.p, fp32, nc, cm);

@@ -62,7 +62,8 @@ struct ColMap {
     const int32_t *rslot = nullptr;
     const int32_t *slot_ptr = nullptr;
     int rps = 1;
-    int nsl = 1;  // slots spanned
+    int nsl = 1;   // slots spanned
+    int wmax = 0;  // host: most columns of one slot when slot_ptr is set (0: not known)
     __host__ __device__ int slot(int r) const { return rslot ? rslot[r] : r / rps; }
     __host__ __device__ int first(int s) const { return slot_ptr ? slot_ptr[s] : s * rps; }
     __host__ __device__ int end(int s, int nrhs) const

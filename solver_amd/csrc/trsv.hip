@@ -244,9 +244,27 @@ __device__ __forceinline__ double reduce2(const double (&s)[2])
     return row16_sum(t);
 }
 
+// Four right-hand sides: the 8 sums s[2g + part] reduced together (halving
+// at lane distance 32, 16 and 8, then a row all-reduce); lane l holds the
+// total of s[(l >> 3) & 7].
+__device__ __forceinline__ double reduce8(const double (&s)[8], int lane)
+{
+    double t[4], u[2];
+#pragma unroll
+    for (int j = 0; j < 4; j++) t[j] = swap32_add(s[j], s[j + 4]);
+#pragma unroll
+    for (int j = 0; j < 2; j++) u[j] = swap16_add(t[j], t[j + 2]);
+    const bool h8 = lane & 8;
+    double y = (h8 ? u[1] : u[0]) + dpp_move<kDppRowMirror>(h8 ? u[0] : u[1]);
+    y += dpp_move<kDppRowHalfMirror>(y);
+    y += dpp_move<kDppQuadSwap1>(y);
+    y += dpp_move<kDppQuadSwap2>(y);
+    return y;
+}
+
 // Panel blocks: the rows of a block reach the same columns, so a wavefront
 // loads x at one chunk of at most kTrsvPanelCols columns (2 per lane) for R
-// right-hand sides of a slot (8, or 1 when slots have few) once and sums
+// right-hand sides of a slot (8, 4 or 1, by the slots' widths) once and sums
 // the chunk for every row of its item against it; only the values are read
 // per row, and the next row's are in flight while a row is reduced.
 // items[w] = (segment of the first row's chunk, rows, segment stride between
@@ -259,7 +277,7 @@ __global__ void __launch_bounds__(256) k_trsv_panel(const int4 *items, int64_t n
                                                     int64_t ob_n, const cd *x, cd *part, int64_t pstride, int64_t n,
                                                     int nrhs, ColMap cm)
 {
-    static_assert(R == 8 || R == 1, "8 or 1 right-hand sides per pass");
+    static_assert(R == 8 || R == 4 || R == 1, "8, 4 or 1 right-hand sides per pass");
     // items of one chunk (same columns of x) are adjacent: keep them on one XCD
     const int64_t w = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (w >= nitems * cm.nsl) return;  // whole wavefront: the kernel has no barrier
@@ -280,7 +298,7 @@ __global__ void __launch_bounds__(256) k_trsv_panel(const int4 *items, int64_t n
     const cd *vs = val + (int64_t)slot * ob_n + c0;
     double *pd = reinterpret_cast<double *>(part);
     // the lanes that hold a total after the reduction, and which one
-    constexpr int kShift = R == 8 ? 2 : 5;
+    constexpr int kShift = R == 8 ? 2 : R == 4 ? 3 : 5;
     const bool writer = (lane & ((1 << kShift) - 1)) == 0;
     const int vi = (lane >> kShift) & (2 * R - 1), gi = vi >> 1;
     for (int r0 = rbeg; r0 < rend; r0 += R) {
@@ -316,6 +334,8 @@ __global__ void __launch_bounds__(256) k_trsv_panel(const int4 *items, int64_t n
             double tot;
             if constexpr (R == 8)
                 tot = reduce16(s, lane);
+            else if constexpr (R == 4)
+                tot = reduce8(s, lane);
             else
                 tot = reduce2(s);
             const int64_t sg = it.x + (int64_t)rho * it.z;
@@ -702,8 +722,9 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
         return PEEC_ERR_STATE;
     }
     cd *part = ctx->trsv_part.p;
-    // workgroups per slot: enough for the slot's columns, 4 per workgroup
-    const int cols = cm.rslot ? (nrhs + nsl - 1) / nsl : cm.rps;
+    // columns of the widest slot; workgroups per slot: enough for them, 4
+    // per workgroup
+    const int cols = cm.rslot ? (cm.wmax > 0 ? cm.wmax : (nrhs + nsl - 1) / nsl) : cm.rps;
     const int gps = std::min((cols + 3) / 4, 16);
     const int nlev = (int)f.level_ptr.size() - 1;
     for (int l = 0; l < nlev; l++) {
@@ -717,8 +738,13 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
             int pi = ctx->prof_begin(PK_TRSV_PANEL, f.lev_pob[(size_t)l] * (4.0 + 16.0 * nsl) + (double)nps * 16.0 * nrhs);
             const int64_t waves = ni * nsl;
             const int4 *it = reinterpret_cast<const int4 *>(f.pitem.p) + i0;
-            if (cols >= 4)
+            // right-hand sides per pass: a slot's columns in one or two passes
+            if (cols >= 5)
                 hipLaunchKernelGGL(k_trsv_panel<8>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, it,
+                                   ni, f.seg_k0.p, f.seg_k1.p, f.ob_idx.p, f.ob_val.p, f.ob_n, (const cd *)x, part,
+                                   pstride, f.n, nrhs, cm);
+            else if (cols >= 2)
+                hipLaunchKernelGGL(k_trsv_panel<4>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, it,
                                    ni, f.seg_k0.p, f.seg_k1.p, f.ob_idx.p, f.ob_val.p, f.ob_n, (const cd *)x, part,
                                    pstride, f.n, nrhs, cm);
             else
