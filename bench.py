@@ -46,7 +46,11 @@ PEAK_HBM_GBS = 8000.0
 # (workload, profile kind); the summaries and commands are in
 # profiles/traffic_trsv_gather_batched_96x96x24.md and
 # profiles/kernels_fft_large_192x192x48.md (complex64 passes).
-MEASURED_TRAFFIC = {("large_192x192x48", "fft_z_conv"): 562.8e6,
+MEASURED_TRAFFIC = {("batched_96x96x24", "trsv_gather"): 160.95e6,
+                    ("batched_96x96x24", "trsv_panel"): 114.52e6,
+                    ("batched_96x96x24", "trsv_block_L"): 26.37e6,
+                    ("batched_96x96x24", "trsv_block_U"): 30.02e6,
+                    ("large_192x192x48", "fft_z_conv"): 562.8e6,
                     ("large_192x192x48", "fft_y_fwd"): 353.0e6,
                     ("large_192x192x48", "fft_y_inv"): 353.1e6}
 KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv_gather",

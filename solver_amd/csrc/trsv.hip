@@ -727,10 +727,15 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
     const int cols = cm.rslot ? (cm.wmax > 0 ? cm.wmax : (nrhs + nsl - 1) / nsl) : cm.rps;
     const int gps = std::min((cols + 3) / 4, 16);
     const int nlev = (int)f.level_ptr.size() - 1;
+    // with fewer than 4 right-hand sides a panel wavefront has too little
+    // work per row to cover the loads: every segment of the level (the
+    // panel segments follow the others) goes through the per-row gather
+    const bool per_row = nrhs < 4;
     for (int l = 0; l < nlev; l++) {
-        const int64_t s0 = f.seg_level_ptr[(size_t)l], ns = f.seg_level_np[(size_t)l] - s0;
+        const int64_t s0 = f.seg_level_ptr[(size_t)l];
+        const int64_t ns = (per_row ? f.seg_level_ptr[(size_t)l + 1] : f.seg_level_np[(size_t)l]) - s0;
         const int64_t nps = f.seg_level_ptr[(size_t)l + 1] - f.seg_level_np[(size_t)l];
-        const int64_t i0 = f.item_level_ptr[(size_t)l], ni = f.item_level_ptr[(size_t)l + 1] - i0;
+        const int64_t i0 = f.item_level_ptr[(size_t)l], ni = per_row ? 0 : f.item_level_ptr[(size_t)l + 1] - i0;
         // algorithmic bytes of a gather: its entries (value + index) and the
         // partial sums written; of a block solve: the in-block entries, the
         // diagonal, b and x, and the partial sums read
@@ -754,7 +759,8 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
             ctx->prof_end(pi);
         }
         if (ns > 0) {
-            int pi = ctx->prof_begin(PK_TRSV_GATHER, f.lev_ob[(size_t)l] * (4.0 + 16.0 * nsl) + (double)ns * 16.0 * nrhs);
+            const double ent = f.lev_ob[(size_t)l] + (per_row ? f.lev_pob[(size_t)l] : 0.0);
+            int pi = ctx->prof_begin(PK_TRSV_GATHER, ent * (4.0 + 16.0 * nsl) + (double)ns * 16.0 * nrhs);
             if (nrhs >= 4)
                 hipLaunchKernelGGL(k_trsv_gather4, dim3((unsigned)((ns + 3) / 4)), dim3(256), 0, ctx->stream,
                                    f.seg_k0.p, f.seg_k1.p, s0, ns, f.ob_idx.p, f.ob_val.p, f.ob_n, (const cd *)x, part,
@@ -767,7 +773,8 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
         }
         int pi = ctx->prof_begin(lower ? PK_TRSV_BLOCK_L : PK_TRSV_BLOCK_U,
                                  (f.lev_ib[(size_t)l] + f.lev_rows[(size_t)l]) * 16.0 * nsl +
-                                     f.lev_rows[(size_t)l] * 32.0 * nrhs + (double)(ns + nps) * 16.0 * nrhs);
+                                     f.lev_rows[(size_t)l] * 32.0 * nrhs +
+                                     (double)(f.seg_level_ptr[(size_t)l + 1] - s0) * 16.0 * nrhs);
         const int nbl = f.level_ptr[(size_t)l + 1] - f.level_ptr[(size_t)l];
         const int4 *bi = reinterpret_cast<const int4 *>(f.binfo.p) + f.level_ptr[(size_t)l];
         const int2 *rsg = reinterpret_cast<const int2 *>(f.row_seg.p);

@@ -101,15 +101,15 @@ def test_device_lu_crossbar_sweep_matches_oracle():
         assert err < tol
 
 
-@pytest.mark.parametrize("rps", [9, 3])
-def test_panel_sums_match_oracle_preconditioner(rps):
+@pytest.mark.parametrize("rps,fs", [(9, [2e8, 4e9]), (3, [2e8, 4e9]), (1, [1e8, 6e8, 2e9, 7e9])])
+def test_panel_sums_match_oracle_preconditioner(rps, fs):
     """With the device factors every U block is a panel block (its rows are
     one front's pivot rows and reach the same columns), so the backward
     sweep sums them as panels (profile kind 10) and the forward sweep's
-    rows through the per-row gathers (kind 5).  M^-1 x over 2 slots x rps
-    right-hand sides -- 9: a pass of 8 and a remainder of 1 per slot; 3:
-    one pass of the 4-wide variant -- matches the oracle's
-    precond_3_3_vector column by column."""
+    rows through the per-row gathers (kind 5).  M^-1 x over len(fs) slots x
+    rps right-hand sides -- 9: a pass of 8 and a remainder of 1 per slot;
+    3: one pass of the 4-wide variant; 1 (4 slots): the 1-wide variant --
+    matches the oracle's precond_3_3_vector column by column."""
     import ctypes
     from solver_amd import _lib, configs, mesher, peec
     from test_gpu_operator import pself
@@ -119,7 +119,6 @@ def test_panel_sums_match_oracle_preconditioner(rps):
     S = sweep.Solver(M, device_lu=True)  # 526 nodes in 16 fronts
     rows = OF.kernel_rows(M.origin, M.Nx, M.Ny, M.Nz, M.sx, M.sy, M.sz)
     spec = OF.spectra(rows, M.Nx, M.Ny, M.Nz)
-    fs = [2e8, 4e9]
     ws = [2 * np.pi * f * M.escalings["freq"] for f in fs]
     peec.frequency_slots(len(ws), ctx=S.ctx)
     for slot, w in enumerate(ws):
