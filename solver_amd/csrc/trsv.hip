@@ -84,8 +84,9 @@ __global__ void __launch_bounds__(256) k_trsv_gather(const int64_t *seg_k0, cons
     }
 }
 
-// The same sums for many right-hand sides: the values of a slot are loaded
-// once, then its right-hand sides go 4 per pass, with the 16 x loads of a
+// The same sums for many right-hand sides, one frequency slot per
+// blockIdx.y: the values of the slot are loaded once, then its right-hand
+// sides go 4 per pass, with the 16 x loads of a
 // pass (4 right-hand sides x 4 entries per lane) all in flight, and the 4
 // lane sums reduced together by recursive halving -- exchanges at distance
 // 32 and 16 keep 2, then 1 of the sums per lane, then a 16-lane all-reduce
@@ -140,9 +141,13 @@ __global__ void __launch_bounds__(256) k_trsv_gather4(const int64_t *seg_k0, con
     }
     const bool h32 = lane & 32, h16 = lane & 16;
     const int gsel = (h32 ? 2 : 0) + (h16 ? 1 : 0);
-    for (int slot = 0; slot < cm.nsl; slot++) {
+    {
+        // one slot per blockIdx.y: the slots' sums run side by side, so a
+        // level with few segments (the separator chain) is not a chain of
+        // dependent loads slot after slot
+        const int slot = (int)blockIdx.y;
         const int rbeg = cm.first(slot), rend = cm.end(slot, nrhs);
-        if (rbeg >= rend) continue;
+        if (rbeg >= rend) return;
         cd v[4];
         const cd *vs = val + (int64_t)slot * ob_n;
 #pragma unroll
@@ -762,7 +767,7 @@ int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b
             const double ent = f.lev_ob[(size_t)l] + (per_row ? f.lev_pob[(size_t)l] : 0.0);
             int pi = ctx->prof_begin(PK_TRSV_GATHER, ent * (4.0 + 16.0 * nsl) + (double)ns * 16.0 * nrhs);
             if (nrhs >= 4)
-                hipLaunchKernelGGL(k_trsv_gather4, dim3((unsigned)((ns + 3) / 4)), dim3(256), 0, ctx->stream,
+                hipLaunchKernelGGL(k_trsv_gather4, dim3((unsigned)((ns + 3) / 4), (unsigned)nsl), dim3(256), 0, ctx->stream,
                                    f.seg_k0.p, f.seg_k1.p, s0, ns, f.ob_idx.p, f.ob_val.p, f.ob_n, (const cd *)x, part,
                                    pstride, f.n, nrhs, cm);
             else
