@@ -44,9 +44,11 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
 # known size) measured with rocprofv3 --pmc on the bench command, per
 # (workload, profile kind); the summaries and commands are in
-# profiles/traffic_trsv_gather_batched_96x96x24.md and
+# profiles/traffic_trsv_gather_batched_96x96x24.md,
+# profiles/traffic_fft_z_conv_batched_96x96x24.md and
 # profiles/kernels_fft_large_192x192x48.md (complex64 passes).
-MEASURED_TRAFFIC = {("batched_96x96x24", "trsv_gather"): 160.95e6,
+MEASURED_TRAFFIC = {("batched_96x96x24", "fft_z_conv"): 1023.63e6,
+                    ("batched_96x96x24", "trsv_gather"): 176.94e6,
                     ("batched_96x96x24", "trsv_panel"): 114.52e6,
                     ("batched_96x96x24", "trsv_block_L"): 26.37e6,
                     ("batched_96x96x24", "trsv_block_U"): 30.02e6,
@@ -59,7 +61,7 @@ KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_
 # instantiated per precision T, mode -- 0 forward, 1 inverse, 2 forward *
 # spectrum * inverse -- and launch shape TPL, LPT, LMAX chosen by the tuner)
 KERNEL_SYMBOLS = {"fft_x_fwd_scatter": "k_fft_pass<T, 0, TPL, LPT, LMAX>", "fft_y_fwd": "k_fft_pass<T, 0, TPL, LPT, LMAX>",
-                  "fft_z_conv": "k_fft_pass<T, 2, TPL, LPT, LMAX>", "fft_y_inv": "k_fft_pass<T, 1, TPL, LPT, LMAX>",
+                  "fft_z_conv": "k_zconv_reg<T, MODE, LZ> or k_fft_pass<T, 2, TPL, LPT, LMAX> (tuned)", "fft_y_inv": "k_fft_pass<T, 1, TPL, LPT, LMAX>",
                   "fft_x_inv_gather": "k_fft_pass<T, 1, TPL, LPT, LMAX>", "trsv_gather": "k_trsv_gather4",
                   "trsv_block_L": "k_trsv_block<true>", "trsv_block_U": "k_trsv_block<false>",
                   "lu_factor": "peec_lu_factor (all its kernels)",
