@@ -344,7 +344,9 @@ int peec_copy_reference(peec_ctx *ctx, int64_t bytes);
  * substitutions with L and with U (one record per launch), 8 device
  * factorizations (peec_lu_factor, one record per call), 9 the fused
  * y forward / z convolution / y inverse pass (complex64 applications that
- * take it; their x passes count under 0 and 4). */
+ * take it; their x passes count under 0 and 4), 10 triangular-solve panel
+ * sums (blocks whose rows share their columns; solves with 4 or more
+ * right-hand sides). */
 int peec_profile(peec_ctx *ctx, int enable, double *ms, double *bytes, int64_t *count, int nkinds);
 
 #ifdef __cplusplus
