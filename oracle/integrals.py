@@ -35,6 +35,12 @@ def lib():
         L.oracle_lp_self.restype = ctypes.c_double
         L.oracle_closed_form.argtypes = [ctypes.c_int, dp]
         L.oracle_closed_form.restype = ctypes.c_double
+        ip = ctypes.POINTER(ctypes.c_int32)
+        L.oracle_lp_row_forms.argtypes = [dp, dp, ctypes.c_int64] + [ctypes.c_double] * 6 + [ctypes.c_int, dp, ip]
+        L.oracle_lp_row_forms.restype = None
+        L.oracle_p_row_forms.argtypes = [dp, dp, ctypes.c_int64] + [ctypes.c_double] * 3 + \
+            [ctypes.c_int, ctypes.c_int, dp, ip]
+        L.oracle_p_row_forms.restype = None
         _lib = L
     return _lib
 
@@ -63,6 +69,44 @@ def p_row(src, centers, sx, sy, sz, caser, caser2):
     cf = np.ascontiguousarray(c.T).reshape(-1)
     lib().oracle_p_row(_dp(src), _dp(cf), N, sx, sy, sz, int(caser), int(caser2), _dp(out))
     return out
+
+
+# closed-form ids of the routing trace (oracle_closed_form's ids, plus the
+# far-field point-point term and the two collinear-segment forms)
+FORM_NAMES = {0: "vol_vol", 1: "vol_surf", 2: "lp_surf_surf_para", 3: "lp_surf_surf_ortho", 4: "line_vol",
+              5: "line_surf_para", 6: "lp_line_surf_ortho", 7: "point_vol", 8: "line_line_ortho_xy",
+              9: "lp_line_line_parall", 10: "point_sup", 11: "line_point", 12: "qs_surf_surf_para",
+              13: "p_surf_surf_ortho", 14: "p_line_surf_ortho", 15: "p_line_line_parall", 16: "far_field",
+              17: "lp_line_line_sp", 18: "p_line_line_sp"}
+# the forms that do not evaluate the integral their call sites use them for
+DEVIANT_FORMS = (1, 3, 6, 17, 18)
+
+
+def lp_row_forms(src, centers, sx, sy, sz, sx2, sy2, sz2, dc):
+    """(values, forms) of one Lp row: forms[n] is the closed-form id entry n
+    routes to, + 100 where a suppression test sat at its threshold."""
+    src = np.ascontiguousarray(src, dtype=np.float64).reshape(3)
+    c = np.asarray(centers, dtype=np.float64).reshape(-1, 3)
+    N = c.shape[0]
+    out = np.empty(N, dtype=np.float64)
+    form = np.empty(N, dtype=np.int32)
+    cf = np.ascontiguousarray(c.T).reshape(-1)
+    lib().oracle_lp_row_forms(_dp(src), _dp(cf), N, sx, sy, sz, sx2, sy2, sz2, int(dc), _dp(out),
+                              form.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return out, form
+
+
+def p_row_forms(src, centers, sx, sy, sz, caser, caser2):
+    """(values, forms) of one P row, as lp_row_forms."""
+    src = np.ascontiguousarray(src, dtype=np.float64).reshape(3)
+    c = np.asarray(centers, dtype=np.float64).reshape(-1, 3)
+    N = c.shape[0]
+    out = np.empty(N, dtype=np.float64)
+    form = np.empty(N, dtype=np.int32)
+    cf = np.ascontiguousarray(c.T).reshape(-1)
+    lib().oracle_p_row_forms(_dp(src), _dp(cf), N, sx, sy, sz, int(caser), int(caser2), _dp(out),
+                             form.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return out, form
 
 
 def lp_self(l, W, T):

@@ -29,6 +29,18 @@
 
 #define ISBAD(t) (isnan(t) || isinf(t))
 
+/* ---- routing trace (fixtures and branch statistics only) ---------------
+ * g_form: id of the last closed form entered (the ids of
+ * oracle_closed_form below; 16 = the point-point far-field term of
+ * sum_supp == 6, 17 / 18 = the Lp / P collinear-segment forms).
+ * g_tie: set when a suppression test's quantity lies within a relative
+ * 1e-9 of its threshold, i.e. where the branch taken depends on the last
+ * bits of the geometry arithmetic. */
+static int g_form = -1;
+static int g_tie = 0;
+
+static int near_thr(double v, double thr) { return fabs(v - thr) <= 1e-9 * thr; }
+
 /* ---- Julia scalar helpers ------------------------------------------- */
 
 /* Base.Math.pow_body(x, 4): two_mul based compensated squaring. */
@@ -75,6 +87,7 @@ static double sgnpow(int k) { return (k & 1) ? -1.0 : 1.0; }
 static double lp_line_point(const double *x1v, double y3, double z3,
                             double x1, double y1, double z1)
 {
+    g_form = 11;
     double x3 = x1v[0], x4 = x1v[1];
     double check = 1.0 / (x1 - x3);
     if (ISBAD(check))
@@ -94,6 +107,7 @@ static double lp_line_point(const double *x1v, double y3, double z3,
 static double point_sup(double x1, double y1, double z1,
                         const double *x2v, const double *y2v, double z2)
 {
+    g_form = 10;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x2 = x2v[c1 - 1];
@@ -114,6 +128,7 @@ static double point_sup(double x1, double y1, double z1,
 /* Lp copy :514-529 */
 static double lp_line_line_sp(const double *x1v, const double *x2v)
 {
+    g_form = 17;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -131,6 +146,7 @@ static double lp_line_line_sp(const double *x1v, const double *x2v)
 /* P copy compute_row_P_sup.jl:461-476 (adds x1*log|x1-x2|) */
 static double p_line_line_sp(const double *x1v, const double *x2v)
 {
+    g_form = 18;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -149,6 +165,7 @@ static double p_line_line_sp(const double *x1v, const double *x2v)
 static double line_line_parall(const double *x1v, double y1, double z1,
                                const double *x2v, double y2, double z2, int is_p)
 {
+    g_form = is_p ? 15 : 9;
     double dy = y1 - y2, dz = z1 - z2;
     if (fabs(dy) < 1e-10 && fabs(dz) < 1e-10)
         return is_p ? p_line_line_sp(x1v, x2v) : lp_line_line_sp(x1v, x2v);
@@ -170,6 +187,7 @@ static double line_line_parall(const double *x1v, double y1, double z1,
 static double line_line_ortho_xy(const double *x1v, double y1, double z1,
                                  double x2, const double *y2v, double z2)
 {
+    g_form = 8;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -191,6 +209,7 @@ static double line_line_ortho_xy(const double *x1v, double y1, double z1,
 static double lp_point_vol(double x1, double y1, double z1,
                            const double *x2v, const double *y2v, const double *z2v)
 {
+    g_form = 7;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x2 = x2v[c1 - 1];
@@ -220,6 +239,7 @@ static double lp_point_vol(double x1, double y1, double z1,
 static double line_surf_ortho(const double *x1v, double y1, double z1,
                               double x2, const double *y2v, const double *z2v)
 {
+    g_form = 6;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -245,6 +265,7 @@ static double line_surf_ortho(const double *x1v, double y1, double z1,
 static double p_line_surf_ortho(const double *x1v, double y1, double z1,
                                 double x2, const double *y2v, const double *z2v)
 {
+    g_form = 14;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -273,6 +294,7 @@ static double p_line_surf_ortho(const double *x1v, double y1, double z1,
 static double line_surf_para(const double *x1v, const double *y1v, double z1,
                              const double *x2v, double y2, double z2)
 {
+    g_form = 5;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -298,6 +320,7 @@ static double line_surf_para(const double *x1v, const double *y1v, double z1,
 static double lp_line_vol(const double *x1v, const double *y1v, const double *z1v,
                           const double *x2v, double y2, double z2)
 {
+    g_form = 4;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -331,6 +354,7 @@ static double lp_line_vol(const double *x1v, const double *y1v, const double *z1
 static double lp_surf_surf_para(const double *x1v, const double *y1v, double z1,
                                 const double *x2v, const double *y2v, double z2)
 {
+    g_form = 2;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -360,6 +384,7 @@ static double lp_surf_surf_para(const double *x1v, const double *y1v, double z1,
 static double qs_surf_surf_para(const double *x1v, const double *y1v, double z1,
                                 const double *x2v, const double *y2v, double z2)
 {
+    g_form = 12;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -389,6 +414,7 @@ static double qs_surf_surf_para(const double *x1v, const double *y1v, double z1,
 static double lp_surf_surf_ortho(const double *x1v, const double *y1v, double z1,
                                  const double *x2v, double y2, const double *z2v)
 {
+    g_form = 3;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -418,6 +444,7 @@ static double lp_surf_surf_ortho(const double *x1v, const double *y1v, double z1
 static double p_surf_surf_ortho(const double *x1v, const double *y1v, double z1,
                                 const double *x2v, double y2, const double *z2v)
 {
+    g_form = 13;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -451,6 +478,7 @@ static double p_surf_surf_ortho(const double *x1v, const double *y1v, double z1,
 static double lp_vol_surf(const double *x1v, const double *y1v, const double *z1v,
                           const double *x2v, const double *y2v, double z2)
 {
+    g_form = 1;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -487,6 +515,7 @@ static double lp_vol_surf(const double *x1v, const double *y1v, const double *z1
 static double lp_vol_vol(const double *x1v, const double *y1v, const double *z1v,
                          const double *x2v, const double *y2v, const double *z2v)
 {
+    g_form = 0;
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -533,7 +562,16 @@ static int lp_check(double e1, double e2, double e3, double V1, double V2, doubl
     double condX1a = V1 * V2 * max_d / (m * m * m);
     double condX1f = sd / m;
     double condX1b = sd / max_oth;
+    if (near_thr(condX1b, e3) || near_thr(condX1f, e1) || near_thr(condX1a, e2)) g_tie = 1;
     return ((condX1b <= e3 || condX1f < e1) && condX1a < e2) ? 1 : 0;
+}
+
+/* the second-stage test of lp_supp_box (:102, :115, :128) */
+static int lp_e4(double other, double min_R, double e4)
+{
+    double v = other / (min_R + 1e-15);
+    if (near_thr(v, e4)) g_tie = 1;
+    return v < e4;
 }
 
 static double max4(const double *a) { double m = a[0]; for (int i = 1; i < 4; i++) if (a[i] > m) m = a[i]; return m; }
@@ -547,19 +585,19 @@ static void lp_supp_box(double a, double b, double c, const double *ax, const do
     const double e1 = 5e-3, e2 = 1e-3, e3 = 1e-3, e4 = 3e-1;
     if (a <= b && a <= c) {
         *sx = lp_check(e1, e2, e3, V1, V2, max4(ax), min_R, a, b, c);
-        if (*sx == 1 && fmax(b, c) / (min_R + 1e-15) < e4) {
+        if (*sx == 1 && lp_e4(fmax(b, c), min_R, e4)) {
             *sy = lp_check(e1, e2, e3, V1, V2, max4(ay), min_R, b, a, c);
             *sz = lp_check(e1, e2, e3, V1, V2, max4(az), min_R, c, a, b);
         }
     } else if (b <= a && b <= c) {
         *sy = lp_check(e1, e2, e3, V1, V2, max4(ay), min_R, b, a, c);
-        if (*sy == 1 && fmax(a, c) / (min_R + 1e-15) < e4) {
+        if (*sy == 1 && lp_e4(fmax(a, c), min_R, e4)) {
             *sx = lp_check(e1, e2, e3, V1, V2, max4(ax), min_R, a, b, c);
             *sz = lp_check(e1, e2, e3, V1, V2, max4(az), min_R, c, a, b);
         }
     } else {
         *sz = lp_check(e1, e2, e3, V1, V2, max4(az), min_R, c, a, b);
-        if (*sz == 1 && fmax(a, b) / (min_R + 1e-15) < e4) {
+        if (*sz == 1 && lp_e4(fmax(a, b), min_R, e4)) {
             *sx = lp_check(e1, e2, e3, V1, V2, max4(ax), min_R, a, b, c);
             *sy = lp_check(e1, e2, e3, V1, V2, max4(ay), min_R, b, a, c);
         }
@@ -588,6 +626,7 @@ double oracle_integ_vol_vol(const double *x1v, const double *y1v, const double *
     double X2[2] = {x2v[0], x2v[1]}, Y2[2] = {y2v[0], y2v[1]}, Z2[2] = {z2v[0], z2v[1]};
 
     if (sum_supp == 6) {
+        g_form = 16;
         integ = V1 * V2 / sqrt((xc1 - xc2) * (xc1 - xc2) + (yc1 - yc2) * (yc1 - yc2) + (zc1 - zc2) * (zc1 - zc2));
     } else if (sum_supp == 5) {
         int is_point_v1 = (sx1 + sy1 + sz1 == 3);
@@ -735,6 +774,7 @@ static int p_check(double e1, double e2, double e3, double e4, double S1, double
     double condX1a = S1 * S2 * max_d / (m * m * m);
     double condX1f = sd / m;
     double condX1b = sd / max_oth;
+    if (near_thr(condS, e4) || near_thr(condX1b, e3) || near_thr(condX1f, e1) || near_thr(condX1a, e2)) g_tie = 1;
     return (condS < e4 && (condX1b <= e3 || condX1f < e1) && condX1a < e2) ? 1 : 0;
 }
 
@@ -772,6 +812,7 @@ double oracle_integ_sup_sup_P(double xc1, double yc1, double zc1, double xc2, do
     double integ = 0.0;
 
     if (sum_supp == 6) {
+        g_form = 16;
         integ = sup1 * sup2 / sqrt((xc1 - xc2) * (xc1 - xc2) + (yc1 - yc2) * (yc1 - yc2) + (zc1 - zc2) * (zc1 - zc2));
     } else if (sum_supp == 5) {
         int is_point_v1 = (sx1 + sy1 + sz1 == 3);
@@ -894,6 +935,35 @@ void oracle_p_row(const double *src, const double *centers, int64_t N,
         default: v = 1 / (4 * pi * eps0 * A1 * sx * sy) * oracle_integ_sup_sup_P(xc1, yc1, zc1, cx, cy, cz + sz / 2, a1, b1, c1, sx, sy, 0); break;
         }
         out[k] = v;
+    }
+}
+
+/* Routing of every entry of a row: form[n] = id of the closed form the
+ * entry evaluates (g_form ids), + 100 when one of its suppression tests sat
+ * within a relative 1e-9 of its threshold.  Same arguments as the row
+ * functions; values are written to out as well. */
+void oracle_lp_row_forms(const double *src, const double *centers, int64_t N,
+                         double sx, double sy, double sz, double sx2, double sy2, double sz2,
+                         int dc, double *out, int32_t *form)
+{
+    for (int64_t n = 0; n < N; n++) {
+        double c[3] = {centers[n], centers[n + N], centers[n + 2 * N]};
+        g_form = -1;
+        g_tie = 0;
+        oracle_lp_row(src, c, 1, sx, sy, sz, sx2, sy2, sz2, dc, out + n);
+        form[n] = g_form + (g_tie ? 100 : 0);
+    }
+}
+
+void oracle_p_row_forms(const double *src, const double *centers, int64_t N,
+                        double sx, double sy, double sz, int caser, int caser2, double *out, int32_t *form)
+{
+    for (int64_t n = 0; n < N; n++) {
+        double c[3] = {centers[n], centers[n + N], centers[n + 2 * N]};
+        g_form = -1;
+        g_tie = 0;
+        oracle_p_row(src, c, 1, sx, sy, sz, caser, caser2, out + n);
+        form[n] = g_form + (g_tie ? 100 : 0);
     }
 }
 

@@ -97,6 +97,8 @@ def lib():
                                  ctypes.POINTER(d), i64p, dp, i64p]
         L.peec_test_lp_row_host.argtypes = [dp, dp, i64] + [d] * 6 + [ctypes.c_int, dp]
         L.peec_test_p_row_host.argtypes = [dp, dp, i64] + [d] * 3 + [ctypes.c_int, ctypes.c_int, dp]
+        L.peec_test_grid_row_host.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + \
+            [d] * 4 + [i64p, i64, dp, i64p]
         L.peec_profile.argtypes = [vp, ctypes.c_int, dp, dp, i64p, ctypes.c_int]
         L.peec_test_probe.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, i64, dp]
         _lib = L

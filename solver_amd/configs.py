@@ -90,16 +90,49 @@ def two_material_64x64x16(seed=0):
                 ports=_x_face_ports(g, cell, 12), freqs=np.logspace(7, 10, 32), tol=1e-10)
 
 
+def boundary_faces(g, mat=1):
+    """Exposed faces of the voxels of material `mat`: (voxel, axis, side)
+    for every face whose neighbour across it is not `mat` (outside the grid
+    counts as not `mat`), in voxel order (x fastest), then axis, then side."""
+    cu = g == mat
+    out = []
+    for axis in range(3):
+        for side in (-1, 1):
+            nb = np.zeros_like(cu)
+            sl_dst = [slice(None)] * 3
+            sl_src = [slice(None)] * 3
+            if side < 0:
+                sl_dst[axis], sl_src[axis] = slice(1, None), slice(None, -1)
+            else:
+                sl_dst[axis], sl_src[axis] = slice(None, -1), slice(1, None)
+            nb[tuple(sl_dst)] = cu[tuple(sl_src)]
+            for v in np.argwhere(cu & ~nb):
+                out.append((tuple(int(t) for t in v), axis, side))
+    out.sort(key=lambda f: (f[0][2], f[0][1], f[0][0], f[1], f[2]))
+    return out
+
+
 def batched_96x96x24(seed=0, nports=8):
     """96x96x24, same recipe with copper Bernoulli(0.3); 8 ports on boundary
-    voxels of the largest component chosen by the seed; 128 frequencies
-    1e6..2e10 Hz."""
+    faces of the largest component, chosen uniformly by the seed among the
+    component's exposed faces (distinct voxels); each port runs from the
+    voxel owning the chosen face to the slab voxel under it; 128
+    frequencies 1e6..2e10 Hz."""
     Nx, Ny, Nz = 96, 96, 24
     cell = (0.25e-3, 0.25e-3, 0.05e-3)
     g, mats, _ = _layered(Nx, Ny, Nz, cell, 18, 0.3, 4.4, 0.02, seed)
     rng = np.random.default_rng(seed + 1000)
-    cu = np.argwhere(g == 1)
-    pick = cu[np.sort(rng.choice(len(cu), size=min(nports, len(cu)), replace=False))]
+    faces = boundary_faces(g)
+    pick, seen = [], set()
+    for f in rng.permutation(len(faces)):
+        v = faces[int(f)][0]
+        if v in seen:
+            continue
+        seen.add(v)
+        pick.append(v)
+        if len(pick) == nports:
+            break
+    pick.sort(key=lambda v: (v[2], v[1], v[0]))
     ports = [_port_to_slab(g, cell, v, 18) for v in pick]
     return dict(name="batched_96x96x24", grid=g, materials=mats, cell=cell, origin=(0.0, 0.0, 0.0), ports=ports,
                 freqs=np.logspace(6, np.log10(2e10), 128), tol=1e-6)

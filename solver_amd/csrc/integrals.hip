@@ -776,10 +776,10 @@ __global__ void __launch_bounds__(256) k_p_row(Box src, const double *__restrict
     out[n] = p_entry(src, A1, cen[n], cen[n + N], cen[n + 2 * N], sx, sy, sz, caser2, scale);
 }
 
-__global__ void __launch_bounds__(256) k_grid_row(GridRow g, int64_t N, double *__restrict__ out)
+// Entry n of circulant row g.kind, centres generated from the grid (the
+// device kernel below and the host test hook share this code).
+HD double grid_row_entry(const GridRow &g, int64_t n)
 {
-    int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
     double c0[3], c[3];
     grid_center(g, 0, c0);
     grid_center(g, n, c);
@@ -801,7 +801,14 @@ __global__ void __launch_bounds__(256) k_grid_row(GridRow g, int64_t N, double *
         const double pi = 3.141592653589793, eps0 = 8.854187816997944e-12;
         v = 1 / (4 * pi * eps0 * (a1 * a2) * u1 * u2) * integ_sup_sup(b1, b2);
     }
-    out[n] = g.scale * v;
+    return g.scale * v;
+}
+
+__global__ void __launch_bounds__(256) k_grid_row(GridRow g, int64_t N, double *__restrict__ out)
+{
+    int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    out[n] = grid_row_entry(g, n);
 }
 
 // Test probe: scalar building blocks evaluated on the device.
@@ -934,8 +941,8 @@ static void grid_dims(int kind, int Nx, int Ny, int Nz, int d[3])
     }
 }
 
-int peec_grid_row_device(hipStream_t s, int kind, const double origin[3], int Nx, int Ny, int Nz, double sx,
-                         double sy, double sz, double scale, double *d_out, int64_t *n_out, int dims_out[3])
+static GridRow make_grid_row(int kind, const double origin[3], int Nx, int Ny, int Nz, double sx, double sy,
+                             double sz, double scale)
 {
     GridRow g;
     g.kind = kind;
@@ -943,6 +950,13 @@ int peec_grid_row_device(hipStream_t s, int kind, const double origin[3], int Nx
     for (int a = 0; a < 3; a++) g.origin[a] = origin[a];
     g.s[0] = sx; g.s[1] = sy; g.s[2] = sz;
     g.scale = scale;
+    return g;
+}
+
+int peec_grid_row_device(hipStream_t s, int kind, const double origin[3], int Nx, int Ny, int Nz, double sx,
+                         double sy, double sz, double scale, double *d_out, int64_t *n_out, int dims_out[3])
+{
+    GridRow g = make_grid_row(kind, origin, Nx, Ny, Nz, sx, sy, sz, scale);
     int64_t N = (int64_t)g.dims[0] * g.dims[1] * g.dims[2];
     if (n_out) *n_out = N;
     if (dims_out) for (int a = 0; a < 3; a++) dims_out[a] = g.dims[a];
@@ -964,6 +978,24 @@ extern "C" int peec_test_lp_row_host(const double *src, const double *cen, int64
     else if (dc == 3) S = sx * sy * sx2 * sy2;
     Box b1 = host_lp_box(src, sx, sy, sz);
     for (int64_t n = 0; n < N; n++) out[n] = lp_entry(b1, cen[n], cen[n + N], cen[n + 2 * N], sx2, sy2, sz2, 1e-7 / S, 1.0);
+    return PEEC_OK;
+}
+
+// Entries idx[0..count) of circulant row `kind` (0..8) on an Nx x Ny x Nz
+// grid, evaluated by k_grid_row's per-entry code (device centre arithmetic
+// included) on the host; returns the row length in *n_out.
+extern "C" int peec_test_grid_row_host(int kind, const double *origin, int Nx, int Ny, int Nz, double sx,
+                                       double sy, double sz, double scale, const int64_t *idx, int64_t count,
+                                       double *out, int64_t *n_out)
+{
+    if (kind < 0 || kind > 8 || !origin || Nx < 1 || Ny < 1 || Nz < 1) return PEEC_ERR_ARG;
+    GridRow g = make_grid_row(kind, origin, Nx, Ny, Nz, sx, sy, sz, scale);
+    const int64_t N = (int64_t)g.dims[0] * g.dims[1] * g.dims[2];
+    if (n_out) *n_out = N;
+    for (int64_t i = 0; i < count; i++) {
+        if (idx[i] < 0 || idx[i] >= N) return PEEC_ERR_ARG;
+        out[i] = grid_row_entry(g, idx[i]);
+    }
     return PEEC_OK;
 }
 

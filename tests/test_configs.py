@@ -46,3 +46,29 @@ def test_batched_ports_are_seeded_and_on_copper():
     g, cell = d1["grid"], d1["cell"]
     for a, b in d1["ports"]:
         assert g[_vox(cell, a)] == 1 and g[_vox(cell, b)] == 2
+
+
+def test_batched_ports_sit_on_exposed_faces_of_the_component():
+    """Every port's copper voxel owns at least one exposed face of the
+    largest component (a face whose neighbour is not copper), the voxels
+    are distinct, and the slab end is the top dielectric layer under it."""
+    d = configs.batched_96x96x24(0)
+    g, cell = d["grid"], d["cell"]
+    faces = configs.boundary_faces(g)
+    owners = {f[0] for f in faces}
+    vox = [_vox(cell, a) for a, _ in d["ports"]]
+    assert len(set(vox)) == 8
+    for (a, b), v in zip(d["ports"], vox):
+        assert v in owners
+        w = _vox(cell, b)
+        assert w[:2] == v[:2] and w[2] == 17 and g[w] == 2
+
+
+def test_boundary_faces_of_a_single_voxel_and_a_bar():
+    g = np.zeros((3, 3, 3), dtype=np.int64)
+    g[1, 1, 1] = 1
+    assert len(configs.boundary_faces(g)) == 6
+    g[:, 1, 1] = 1          # a 3-voxel bar along x: 3 * 4 side faces + 2 end faces
+    f = configs.boundary_faces(g)
+    assert len(f) == 14
+    assert sum(1 for v, ax, s in f if ax == 0) == 2
