@@ -191,6 +191,14 @@ int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_csc_c128 *L
 int peec_lu_analyze(peec_ctx *ctx, int64_t n, const int64_t *order, int64_t nfront, const int64_t *front_ptr,
                     const int64_t *front_parent, int64_t nblocks, const int64_t *block_ptr);
 int peec_lu_factor(peec_ctx *ctx);
+/* The device factorization has no pivoting: peec_lu_factor, _slot and
+ * _slots return PEEC_ERR_NUMERIC (and leave the slots unfactored) when a
+ * pivot is zero or non-finite, where UMFPACK would pivot or fail.
+ * peec_lu_info (no counterpart in the reference) reports its sizes after
+ * peec_lu_analyze: out8 = {n, fronts, largest front order, front-pool
+ * entries per slot, pool entries per slot with scratch, stored SS entries,
+ * L entries, U entries}. */
+int peec_lu_info(peec_ctx *ctx, int64_t *out8);
 /* Front K of the last device factorization, f x f column-major (tests),
  * and the factor position of each of its rows/columns (positions, f).
  * out and positions may be NULL to query f, s (pivots) and first (first

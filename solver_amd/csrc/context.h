@@ -171,6 +171,7 @@ struct LuPlan {
     int64_t scratch0 = 0;       // pool offset of the per-slot 64 x 64 inverse pairs
     // values of the triangular-solve layouts, gathered from the pool
     DevBuf<int64_t> l_ob_src, l_dense_src, u_ob_src, u_dense_src, u_diag_src;
+    DevBuf<int32_t> bad_pivot;  // per slot of a factorization group: 1 = a zero or non-finite pivot
     double flops = 0.0;
 };
 

@@ -43,12 +43,19 @@ __device__ __forceinline__ cd lcinv(cd a)
     return {a.x / d, -a.y / d};
 }
 
+// The element-wise kernels below loop with a grid stride: their counts (the
+// front pool holds more than 2^32 entries at 192 x 192 x 48) can exceed
+// the 2^32 work-items one dispatch dimension addresses, so they launch at
+// most kMaxBlocks1d blocks (grid1d) and each thread walks its share.
+__device__ __forceinline__ int64_t gtid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int64_t gstride() { return (int64_t)gridDim.x * blockDim.x; }
+
 // The factorization kernels run for several frequency slots at once:
 // blockIdx.y selects the slot, whose fronts live at pool + y * stride.
 __global__ void k_lu_zero(cd *p, int64_t n, int64_t stride)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[(int64_t)blockIdx.y * stride + i] = {0.0, 0.0};
+    cd *q = p + (int64_t)blockIdx.y * stride;
+    for (int64_t i = gtid(); i < n; i += gstride()) q[i] = {0.0, 0.0};
 }
 
 // SS entry k = sum_t coef_t / DZ[edge_t] + j w gsum_k, written to its front
@@ -57,30 +64,30 @@ __global__ void k_lu_ss(const int64_t *tgt, const int32_t *cptr, const int32_t *
                         const double *gsum, const cd *invZ0, const double *wv, int slot0, int64_t m, cd *pool0,
                         int64_t nss, int64_t stride)
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nss) return;
     const int slot = slot0 + (int)blockIdx.y;
     const cd *invZ = invZ0 + (int64_t)slot * m;
     const double w = wv[slot];
     cd *pool = pool0 + (int64_t)blockIdx.y * stride;
-    double vx = 0.0, vy = 0.0;
-    for (int32_t t = cptr[k]; t < cptr[k + 1]; t++) {
-        const cd z = invZ[cedge[t]];
-        vx += ccoef[t] * z.x;
-        vy += ccoef[t] * z.y;
+    for (int64_t k = gtid(); k < nss; k += gstride()) {
+        double vx = 0.0, vy = 0.0;
+        for (int32_t t = cptr[k]; t < cptr[k + 1]; t++) {
+            const cd z = invZ[cedge[t]];
+            vx += ccoef[t] * z.x;
+            vy += ccoef[t] * z.y;
+        }
+        vy += w * gsum[k];
+        pool[tgt[k]] = {vx, vy};
     }
-    vy += w * gsum[k];
-    pool[tgt[k]] = {vx, vy};
 }
 
 // pool[tgt[i]] += val[i] (Yle entries; each target once per slot; tgt
 // includes the slot's pool offset)
 __global__ void k_lu_add(const int64_t *tgt, const cd *val, int64_t cnt, cd *pool)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnt) return;
-    const cd a = pool[tgt[i]], b = val[i];
-    pool[tgt[i]] = {a.x + b.x, a.y + b.y};
+    for (int64_t i = gtid(); i < cnt; i += gstride()) {
+        const cd a = pool[tgt[i]], b = val[i];
+        pool[tgt[i]] = {a.x + b.x, a.y + b.y};
+    }
 }
 
 // Column j of child c's update matrix added into its parent.
@@ -259,15 +266,38 @@ __global__ void __launch_bounds__(256) k_lu_gemm(const LuGemmItem *items, cd *po
 // dst[i] = pool[src[i]] (or 1 / pool[src[i]] when invert), 0 where src < 0
 __global__ void k_lu_gather(const int64_t *src, cd *dst, int64_t cnt, const cd *pool, int invert)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnt) return;
-    const int64_t s = src[i];
-    cd v = s >= 0 ? pool[s] : cd{0.0, 0.0};
-    if (invert) v = lcinv(v);
-    dst[i] = v;
+    for (int64_t i = gtid(); i < cnt; i += gstride()) {
+        const int64_t s = src[i];
+        cd v = s >= 0 ? pool[s] : cd{0.0, 0.0};
+        if (invert) v = lcinv(v);
+        dst[i] = v;
+    }
 }
 
-inline unsigned nblk256(int64_t n) { return (unsigned)((n + 255) / 256); }
+// bad[y] = 1 when slot y's inverted pivots 1/U[i,i] hold a non-finite
+// value (a zero pivot inverts to Inf / NaN; a non-finite SS propagates);
+// bad[] is cleared by k_lu_clear_flags before.  Benign races: every writer
+// stores the same value.
+__global__ void k_lu_pivot_check(const cd *diag0, int64_t n, int32_t *bad)
+{
+    const cd *diag = diag0 + (int64_t)blockIdx.y * n;
+    bool ok = true;
+    for (int64_t i = gtid(); i < n; i += gstride()) {
+        const cd v = diag[i];
+        ok = ok && isfinite(v.x) && isfinite(v.y) && (v.x != 0.0 || v.y != 0.0);
+    }
+    if (!ok) bad[blockIdx.y] = 1;
+}
+
+__global__ void k_lu_clear_flags(int32_t *bad, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) bad[i] = 0;
+}
+
+// Blocks of 256 for a grid-stride kernel over n elements: enough to fill
+// the chip, never more than kMaxBlocks1d (n > 2^32 included).
+constexpr int64_t kMaxBlocks1d = 1 << 16;
+inline unsigned grid1d(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, kMaxBlocks1d)); }
 
 template <typename T>
 int upload(DevBuf<T> &d, const std::vector<T> &h)
@@ -725,9 +755,9 @@ static int lu_factor_range(peec_ctx *ctx, int slot0, int ns)
     int pi = ctx->prof_begin(PK_LU, 0.0);
     cd *pool = P.pool.p;
     const unsigned ny = (unsigned)ns;
-    hipLaunchKernelGGL(k_lu_zero, dim3(nblk256(P.pool_fronts), ny), dim3(256), 0, ctx->stream, pool, P.pool_fronts,
+    hipLaunchKernelGGL(k_lu_zero, dim3(grid1d(P.pool_fronts), ny), dim3(256), 0, ctx->stream, pool, P.pool_fronts,
                        stride);
-    hipLaunchKernelGGL(k_lu_ss, dim3(nblk256(P.nss), ny), dim3(256), 0, ctx->stream, P.ss_tgt.p, P.ss_cptr.p,
+    hipLaunchKernelGGL(k_lu_ss, dim3(grid1d(P.nss), ny), dim3(256), 0, ctx->stream, P.ss_tgt.p, P.ss_cptr.p,
                        P.ss_cedge.p, P.ss_ccoef.p, P.ss_gsum.p, ctx->invZ.p, ctx->wv.p, slot0, ctx->m, pool, P.nss,
                        stride);
     DevBuf<int64_t> dyt;
@@ -736,7 +766,7 @@ static int lu_factor_range(peec_ctx *ctx, int slot0, int ns)
         if (!dyt.alloc(ytgt.size()) || !dyv.alloc(yval.size())) return PEEC_ERR_NOMEM;
         PEEC_HIP_TRY(hipMemcpyAsync(dyt.p, ytgt.data(), ytgt.size() * 8, hipMemcpyHostToDevice, ctx->stream));
         PEEC_HIP_TRY(hipMemcpyAsync(dyv.p, yval.data(), yval.size() * 16, hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_lu_add, dim3(nblk256((int64_t)ytgt.size())), dim3(256), 0, ctx->stream, dyt.p, dyv.p,
+        hipLaunchKernelGGL(k_lu_add, dim3(grid1d((int64_t)ytgt.size())), dim3(256), 0, ctx->stream, dyt.p, dyv.p,
                            (int64_t)ytgt.size(), pool);
     }
     for (const auto &L : P.sched) {
@@ -762,21 +792,40 @@ static int lu_factor_range(peec_ctx *ctx, int slot0, int ns)
     for (int y = 0; y < ns; y++) {
         const int slot = slot0 + y;
         const cd *pl = pool + (int64_t)y * stride;
-        hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Lf.ob_n)), dim3(256), 0, ctx->stream, P.l_ob_src.p,
+        hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Lf.ob_n)), dim3(256), 0, ctx->stream, P.l_ob_src.p,
                            Lf.ob_val.p + slot * Lf.ob_n, Lf.ob_n, pl, 0);
-        hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Lf.dense_n)), dim3(256), 0, ctx->stream, P.l_dense_src.p,
+        hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Lf.dense_n)), dim3(256), 0, ctx->stream, P.l_dense_src.p,
                            Lf.dense.p + slot * Lf.dense_n, Lf.dense_n, pl, 0);
-        hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Uf.ob_n)), dim3(256), 0, ctx->stream, P.u_ob_src.p,
+        hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Uf.ob_n)), dim3(256), 0, ctx->stream, P.u_ob_src.p,
                            Uf.ob_val.p + slot * Uf.ob_n, Uf.ob_n, pl, 0);
-        hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(Uf.dense_n)), dim3(256), 0, ctx->stream, P.u_dense_src.p,
+        hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Uf.dense_n)), dim3(256), 0, ctx->stream, P.u_dense_src.p,
                            Uf.dense.p + slot * Uf.dense_n, Uf.dense_n, pl, 0);
-        hipLaunchKernelGGL(k_lu_gather, dim3(nblk256(P.n)), dim3(256), 0, ctx->stream, P.u_diag_src.p,
+        hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(P.n)), dim3(256), 0, ctx->stream, P.u_diag_src.p,
                            Uf.diag.p + slot * P.n, P.n, pl, 1);
     }
+    PEEC_HIP_TRY(hipGetLastError());
+    // no pivoting: a zero or non-finite pivot makes the factors unusable
+    if (P.bad_pivot.n < (size_t)ns && !P.bad_pivot.alloc((size_t)std::max(ns, 64))) {
+        peec_set_error("peec_lu_factor: out of device memory for the pivot flags");
+        return PEEC_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_lu_clear_flags, dim3(1), dim3(256), 0, ctx->stream, P.bad_pivot.p, ns);
+    hipLaunchKernelGGL(k_lu_pivot_check, dim3(grid1d(P.n), ny), dim3(256), 0, ctx->stream,
+                       Uf.diag.p + (int64_t)slot0 * P.n, P.n, P.bad_pivot.p);
     PEEC_HIP_TRY(hipGetLastError());
     ctx->prof_end(pi);
     // the Yle staging buffers are freed on return: wait for their use
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<int32_t> bad((size_t)ns, 0);
+    PEEC_HIP_TRY(hipMemcpy(bad.data(), P.bad_pivot.p, (size_t)ns * 4, hipMemcpyDeviceToHost));
+    for (int y = 0; y < ns; y++)
+        if (bad[(size_t)y]) {
+            for (int z = 0; z < ns; z++) ctx->slot_lu[(size_t)(slot0 + z)] = 0;
+            ctx->lu_ready = false;
+            peec_set_error("peec_lu_factor: zero or non-finite pivot in the factorization of frequency slot %d "
+                           "(the device LU does not pivot)", slot0 + y);
+            return PEEC_ERR_NUMERIC;
+        }
     for (int y = 0; y < ns; y++) ctx->slot_lu[(size_t)(slot0 + y)] = 1;
     ctx->lu_ready = std::all_of(ctx->slot_lu.begin(), ctx->slot_lu.end(), [](char c) { return c != 0; });
     return PEEC_OK;
@@ -828,6 +877,30 @@ extern "C" int peec_lu_factor_slots(peec_ctx *ctx, int slot0, int count)
 }
 
 extern "C" int peec_lu_factor(peec_ctx *ctx) { return peec_lu_factor_slot(ctx, 0); }
+
+// Sizes of the device factorization (after peec_lu_analyze): out[0] n,
+// [1] fronts, [2] largest front order f, [3] front-pool entries per slot,
+// [4] pool entries per slot with the inverse scratch, [5] stored SS
+// entries, [6] L entries (off-block + dense tiles), [7] U entries.
+extern "C" int peec_lu_info(peec_ctx *ctx, int64_t *out8)
+{
+    if (!ctx || !out8 || !ctx->lu.ready) {
+        peec_set_error("peec_lu_info: call peec_lu_analyze first");
+        return PEEC_ERR_STATE;
+    }
+    const LuPlan &P = ctx->lu;
+    int64_t fmax = 0;
+    for (const auto &F : P.fronts) fmax = std::max<int64_t>(fmax, F.f);
+    out8[0] = P.n;
+    out8[1] = (int64_t)P.fronts.size();
+    out8[2] = fmax;
+    out8[3] = P.pool_fronts;
+    out8[4] = P.pool_total;
+    out8[5] = P.nss;
+    out8[6] = ctx->Lf.ob_n + ctx->Lf.dense_n;
+    out8[7] = ctx->Uf.ob_n + ctx->Uf.dense_n;
+    return PEEC_OK;
+}
 
 // Read back one front of the last factorization (tests): f x f column-major,
 // and the factor position of each of its f rows/columns.

@@ -416,6 +416,15 @@ def lu_factor(ctx=None):
     L.check(L.lib().peec_lu_factor(ctx.handle), "peec_lu_factor")
 
 
+def lu_info(ctx=None):
+    """Sizes of the device factorization (peec_lu_info)."""
+    ctx = ctx or default_context()
+    out = (ctypes.c_int64 * 8)()
+    L.check(L.lib().peec_lu_info(ctx.handle, out), "peec_lu_info")
+    keys = ("n", "fronts", "max_front", "pool_per_slot", "pool_with_scratch", "ss_entries", "l_entries", "u_entries")
+    return dict(zip(keys, (int(v) for v in out)))
+
+
 def lu_front(K, ctx=None):
     """(s, positions, F): front K of the last device factorization -- its
     pivot count, the factor position of each row/column, and the f x f

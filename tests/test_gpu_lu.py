@@ -147,3 +147,22 @@ def test_panel_sums_match_oracle_preconditioner(rps, fs):
             e = rel(Y[:, c], op.precond(x[:m], x[m:m + ns], x[m + ns:]))
             print("slot %d column %d: M^-1 x vs oracle %.2e (tol %.1e)" % (slot, c, e, tol), flush=True)
             assert e < tol
+
+
+def test_zero_or_nonfinite_pivot_is_an_error():
+    """The device LU does not pivot: a non-finite SS entry (here a NaN
+    edge impedance) must come back as PEEC_ERR_NUMERIC, not as factors."""
+    from solver_amd import _lib, peec
+    M = small_mesh()
+    S = sweep.Solver(M, ctx=peec.Context(0), device_lu=True)
+    w = 2 * np.pi * 3e8 * M.escalings["freq"]
+    Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+    DZ = sweep.DZ_of(M, w)
+    DZ[0] = np.nan
+    peec.frequency_slots(2, ctx=S.ctx)
+    peec.frequency_set_slot(0, w, sweep.DZ_of(M, w), Yle, ctx=S.ctx)
+    peec.frequency_set_slot(1, w, DZ, Yle, ctx=S.ctx)
+    with pytest.raises(_lib.PeecError, match="pivot"):
+        peec.lu_factor_slots(0, 2, ctx=S.ctx)
+    peec.lu_factor_slot(0, ctx=S.ctx)  # the good slot alone still factors
+    S.ctx.close()

@@ -1,0 +1,164 @@
+"""Sweeps of the benchmark geometries against the oracle's sweep fixtures
+(tests/golden/make_sweeps.py, CPU), and the 192x192x48 solve.
+
+64x64x16 two-material geometry, plain FP64 path: the reference's sweep
+(warm start, gmres_custom, tol 1e-10) at 1e7, 3.0e8 and 1e10 Hz.  The GPU
+runs the same Krylov iteration as the oracle, so flags and iteration counts
+must be equal, and Z agrees with the oracle's Z to rounding amplified by
+the conditioning of SS (the bar is stated in the test).
+
+96x96x24 batched geometry (8 ports): the oracle solved to a 1e-12 residual
+at 1e6, 1.0e8 and 2e10 Hz.  The batched mixed-precision path (complex64
+FFT convolutions inside FP64 refinement) is held to 1e-6 relative error of
+every port's self impedance Z_ii and of every Z entry relative to the
+largest |Z| of its row; the plain FP64 path at tol 1e-12 to 1e-9.
+
+192x192x48 (one port): no oracle sweep fits in the CPU budget, so the
+mixed path is held to the plain FP64 path run to 1e-12 at two of its
+frequencies (1e-6 relative on Z_11).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from solver_amd import configs, mesher, peec, sweep
+
+from test_gpu_operator import pself
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def fixture(cfg_name):
+    return np.load(os.path.join(GOLDEN, "sweep_%s.npz" % cfg_name))
+
+
+def z_errors(Z, Zref):
+    """(largest relative error of a self impedance Z_ii, largest error of
+    an entry relative to the largest |Zref| of its row), over frequencies."""
+    P = Z.shape[0]
+    d = np.max([np.abs(Z[i, i] - Zref[i, i]) / np.abs(Zref[i, i]) for i in range(P)])
+    rows = np.max(np.abs(Z - Zref) / np.max(np.abs(Zref), axis=1, keepdims=True))
+    return float(d), float(rows)
+
+
+def geometry(name):
+    cfg = configs.CONFIGS[name]()
+    M = mesher.mesh(cfg["grid"], cfg["materials"], cfg["cell"], cfg["origin"], cfg["ports"], p_self_fn=pself)
+    return cfg, M
+
+
+def test_two_material_plain_sweep_matches_oracle():
+    fx = fixture("two_material_64x64x16")
+    cfg, M = geometry("two_material_64x64x16")
+    assert np.array_equal(M.port_nodes, fx["port_nodes"])
+    print("two-material: N=%d, frequencies %s" % (M.m + M.ns + M.n, fx["freqs"]), flush=True)
+    with peec.Context(0) as ctx:
+        S = sweep.Solver(M, ctx=ctx, device_lu=True)
+        rec = []
+        out = sweep.FFT_solver_QS_S_type(fx["freqs"], M, int(fx["inner"]), float(fx["tol"]), solver=S, record=rec)
+    for r in rec:
+        k, c = r["k"], r["port"]
+        print("  f=%.4g port %d: flag %d / %d, iterations %d / %d (GPU / oracle)"
+              % (fx["freqs"][k], c, r["flag"], fx["flag"][c, k], r["iter"][1], fx["iters"][c, k]), flush=True)
+        assert r["flag"] == fx["flag"][c, k]
+        assert r["iter"][1] == fx["iters"][c, k]
+    d, rows = z_errors(out["Z"], fx["Z"])
+    print("  Z vs oracle: self %.2e, entries %.2e" % (d, rows), flush=True)
+    # same iteration; the trajectories differ by FFT and LU rounding,
+    # amplified by cond(SS) ~ 1e9 at 10 MHz
+    assert d <= 1e-10 and rows <= 1e-10
+
+
+@pytest.fixture(scope="module")
+def batched():
+    fx = fixture("batched_96x96x24")
+    cfg, M = geometry("batched_96x96x24")
+    assert np.array_equal(M.port_nodes, fx["port_nodes"])
+    ctx = peec.Context(0)
+    S = sweep.Solver(M, ctx=ctx, device_lu=True)
+    print("batched: N=%d, %d ports, frequencies %s" % (M.m + M.ns + M.n, M.port_nodes.shape[0], fx["freqs"]),
+          flush=True)
+    yield fx, M, S
+    ctx.close()
+
+
+def test_batched_mixed_per_port_impedance(batched):
+    fx, M, S = batched
+    F = fx["freqs"].size
+    P = M.port_nodes.shape[0]
+    S.tune(F * P, fp64=True, fp32=True)
+    rec = []
+    out = sweep.FFT_solver_QS_batched(fx["freqs"], M, 1e-9, batch=F, mixed=True, inner_tol=1e-7, solver=S,
+                                      record=rec)
+    assert sorted({r["flag"] for r in rec}) == [0]
+    for k in range(F):
+        d, rows = z_errors(out["Z"][:, :, k:k + 1], fx["Z"][:, :, k:k + 1])
+        zd = [abs(out["Z"][i, i, k] - fx["Z"][i, i, k]) / abs(fx["Z"][i, i, k]) for i in range(P)]
+        print("  f=%.4g: per-port self-impedance errors %s; entries %.2e"
+              % (fx["freqs"][k], " ".join("%.1e" % e for e in zd), rows), flush=True)
+    d, rows = z_errors(out["Z"], fx["Z"])
+    assert d <= 1e-6 and rows <= 1e-6
+
+
+def test_batched_plain_fp64_matches_oracle(batched):
+    fx, M, S = batched
+    rec = []
+    out = sweep.FFT_solver_QS_S_type(fx["freqs"], M, 400, 1e-12, solver=S, record=rec)
+    d, rows = z_errors(out["Z"], fx["Z"])
+    print("  plain FP64 at 1e-12 vs oracle: self %.2e, entries %.2e; iterations %s"
+          % (d, rows, [r["iter"][1] for r in rec]), flush=True)
+    assert d <= 1e-9 and rows <= 1e-9
+
+
+@pytest.fixture(scope="module")
+def large():
+    cfg, M = geometry("large_192x192x48")
+    ctx = peec.Context(0)
+    S = sweep.Solver(M, ctx=ctx, device_lu=True)
+    info = peec.lu_info(ctx=ctx)
+    print("large: N=%d, device LU %s" % (M.m + M.ns + M.n, info), flush=True)
+    yield cfg, M, S, info
+    ctx.close()
+
+
+def test_large_factorization_is_reproducible(large):
+    """Factoring the same SS again -- as the one-slot path, and through the
+    frequency slots -- gives bitwise the same M^-1 x.  The front pool here
+    exceeds 2^32 complex entries (one dispatch dimension's work-item
+    range), so every element-wise LU kernel must cover it by a grid-stride
+    loop; before, the pool was only cleared in part and every
+    factorization after the first one read stale update matrices."""
+    cfg, M, S, info = large
+    assert info["pool_per_slot"] > 2 ** 32
+    N = M.m + M.ns + M.n
+    w = 2 * np.pi * cfg["freqs"][0] * M.escalings["freq"]
+    x = (np.random.default_rng(0).normal(size=N) + 0j)[:, None]
+    ys = []
+    for how in ("plain", "plain again", "slots"):
+        if how == "slots":
+            peec.frequency_slots(1, ctx=S.ctx)
+            sweep.set_slots(S, np.array([w]))
+        else:
+            S.set_frequency(w)
+        ys.append(peec.apply_K_batched(x, 1, precond=2, ctx=S.ctx)[:, 0])
+        print("  %s: |M^-1 x| = %.6e" % (how, np.linalg.norm(ys[-1])), flush=True)
+    assert np.all(np.isfinite(ys[0]))
+    assert np.array_equal(ys[1], ys[0]) and np.array_equal(ys[2], ys[0])
+
+
+def test_large_mixed_vs_plain_fp64(large):
+    cfg, M, S, info = large
+    fs = np.asarray(cfg["freqs"])[[0, -1]]
+    print("large: frequencies %s" % fs, flush=True)
+    ref = sweep.FFT_solver_QS_S_type(fs, M, 400, 1e-12, solver=S)
+    print("  plain FP64 done: Z11 %s" % ref["Z"][0, 0], flush=True)
+    rec = []
+    out = sweep.FFT_solver_QS_batched(fs, M, 1e-9, batch=2, mixed=True, inner_tol=1e-7, solver=S, record=rec)
+    e = np.abs(out["Z"][0, 0] - ref["Z"][0, 0]) / np.abs(ref["Z"][0, 0])
+    print("  mixed vs plain FP64: Z11 relative errors %s, flags %s, iterations %s"
+          % (e, [r["flag"] for r in rec], [r["iters"] for r in rec]), flush=True)
+    assert all(r["flag"] == 0 for r in rec)
+    assert np.all(e <= 1e-6)
