@@ -1,33 +1,39 @@
 """Benchmark: frequency-point solves per second of the voxel-PEEC FFT solver.
 
-Default workload: the batched mixed-precision sweep on the 96 x 96 x 24
-two-material geometry (221,184 voxels, copper Bernoulli(0.3) largest
-component over a dielectric slab, 8 ports, 128 frequencies 1e6..2e10 Hz).
+Workload: the batched mixed-precision sweep on the 96 x 96 x 24
+two-material geometry (221,184 voxels; copper Bernoulli(0.3), largest
+component, over a dielectric slab; 8 ports on exposed faces of the copper;
+128 frequencies 1e6..2e10 Hz).
 
 A step is one batch of the sweep: `--batch` frequency points (default 8),
 every port of each, solved together -- per frequency slot the frequency
-terms and the device LU of SS, then one peec_gmres_batched call (restarted
+terms and the device LU of SS, then one peec_sweep_batch call (restarted
 GMRES in FP64 iterative refinement, complex64 FFT convolutions inside the
 cycles, FP64 triangular solves, orthogonalisation and residuals).  The
-batches walk a frequency subset spread over the whole band in increasing
-order, warm-started from the previous batch like Vrest in
-FFT_solver_QS_S_type.jl:140,196.  Inputs are resident on the device before
-the timed region; geometry setup (integrals, spectra, operator upload, LU
-analysis) and FFT autotuning are reported separately.
+timed steps walk the whole band in increasing frequency, batch after batch
+(16 batches of 8 cover the 128 points; more steps start the band again),
+warm-started from the previous batch like Vrest in
+FFT_solver_QS_S_type.jl:140,196.  The warm-up steps repeat the first batch
+and are discarded.  Inputs are resident on the device before the timed
+region; geometry setup (integrals, spectra, operator upload, LU analysis)
+and FFT autotuning are reported separately.
 
 Beside the timed steps, on the same GPU:
-  - the plain FP64 drop-in path (one port at a time, gmres_custom with the
-    reference's warm start) on `--plain-freqs` of the timed frequencies at
-    the same GMRES tolerance: its solves/s is the baseline for the
-    mixed / plain speed-up;
-  - the impedance error of the timed mixed-precision solves at those
-    frequencies against the plain FP64 path run to a 1e-12 residual;
-  - a profiled batched matvec (HIP events around every launch) for the
-    roofline of the dominant kernel;
+  - the impedance error of every port at 8 frequencies spread over the band
+    (1e6 .. 2e10 Hz) against the plain FP64 drop-in path (gmres_custom per
+    port, the reference's warm start) run to a 1e-12 residual;
+  - the plain FP64 path's solves/s at the bench tolerance at those 8
+    frequencies: the baseline of the mixed / plain speed-up;
+  - a profiled batched matvec (HIP events around every launch) per kernel;
+  - the roofline leg: the FFT matvec on the 192 x 192 x 48 workload (one
+    launch per pass and convolution, HIP events on the context's stream),
+    whose dominant pass gives `roofline`;
   - the CPU baseline: the oracle (numpy/scipy restatement of the reference
-    algorithm) on a bounded sample, on one host core.
+    algorithm) on a bounded sample, FFTs on all host cores of the box.
 
-Prints one JSON line (rank 0, N = 1).
+Prints one JSON line (rank 0, N = 1).  `--roofline-only` runs only the
+roofline leg (the command the rocprofv3 summaries in profiles/ are taken
+from).
 """
 import argparse
 import json
@@ -41,27 +47,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
-# known size) measured with rocprofv3 --pmc on the bench command, per
-# (workload, profile kind); the summaries and commands are in
-# profiles/traffic_trsv_gather_batched_96x96x24.md,
-# profiles/traffic_fft_z_conv_batched_96x96x24.md and
-# profiles/kernels_fft_large_192x192x48.md (complex64 passes).
-MEASURED_TRAFFIC = {("batched_96x96x24", "fft_z_conv"): 1023.63e6,
-                    ("batched_96x96x24", "trsv_gather"): 176.94e6,
-                    ("batched_96x96x24", "trsv_panel"): 114.52e6,
-                    ("batched_96x96x24", "trsv_block_L"): 26.37e6,
-                    ("batched_96x96x24", "trsv_block_U"): 30.02e6,
-                    ("large_192x192x48", "fft_z_conv"): 562.8e6,
-                    ("large_192x192x48", "fft_y_fwd"): 353.0e6,
-                    ("large_192x192x48", "fft_y_inv"): 353.1e6}
+# HBM bytes per launch of a roofline-leg pass (FETCH_SIZE + WRITE_SIZE,
+# scaled by a copy kernel of known size), measured with rocprofv3 --pmc on
+# `bench.py --roofline-only`; the summaries and commands are in profiles/.
+MEASURED_TRAFFIC = {}
 KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv_gather",
               "trsv_block_L", "trsv_block_U", "lu_factor", "fft_yz_conv", "trsv_panel"]
-# kernel symbol behind each profile kind (the FFT pass kernel is
-# instantiated per precision T, mode -- 0 forward, 1 inverse, 2 forward *
-# spectrum * inverse -- and launch shape TPL, LPT, LMAX chosen by the tuner)
-KERNEL_SYMBOLS = {"fft_x_fwd_scatter": "k_fft_pass<T, 0, TPL, LPT, LMAX>", "fft_y_fwd": "k_fft_pass<T, 0, TPL, LPT, LMAX>",
-                  "fft_z_conv": "k_zconv_reg<T, MODE, LZ> or k_fft_pass<T, 2, TPL, LPT, LMAX> (tuned)", "fft_y_inv": "k_fft_pass<T, 1, TPL, LPT, LMAX>",
+# kernel symbols behind each profile kind: the LDS pass k_fft_pass<T, MODE,
+# TPL, LPT, LMAX> (MODE 0 forward, 1 inverse, 2/3 forward * spectrum *
+# inverse), the register z kernel k_zconv_reg<T, MODE, L> and the
+# lane-split pass k_lines<T, MODE, L, Q>, as the tuner chooses per pass
+KERNEL_SYMBOLS = {"fft_x_fwd_scatter": "k_fft_pass<T, 0, TPL, LPT, LMAX>",
+                  "fft_y_fwd": "k_lines<T, 0, L, Q> or k_fft_pass<T, 0, TPL, LPT, LMAX> (tuned)",
+                  "fft_z_conv": "k_lines<T, 2|3, L, Q>, k_zconv_reg<T, 2|3, L> or k_fft_pass<T, 2|3, ...> (tuned)",
+                  "fft_y_inv": "k_lines<T, 1, L, Q> or k_fft_pass<T, 1, TPL, LPT, LMAX> (tuned)",
                   "fft_x_inv_gather": "k_fft_pass<T, 1, TPL, LPT, LMAX>", "trsv_gather": "k_trsv_gather4",
                   "trsv_block_L": "k_trsv_block<true>", "trsv_block_U": "k_trsv_block<false>",
                   "lu_factor": "peec_lu_factor (all its kernels)",
@@ -89,19 +88,9 @@ def profile_read(ctx, enable):
     return [dict(name=KIND_NAMES[k], ms=ms[k], bytes=by[k], count=cnt[k]) for k in range(nk)]
 
 
-def z_error(Z, Zref):
-    """Per frequency: max |Z - Zref| / max |Zref| over the port matrix
-    (norm-wise), and the largest relative error of a port's own impedance
-    Z_ii; each maximised over the frequencies."""
-    nw = max(np.max(np.abs(Z[:, :, k] - Zref[:, :, k])) / np.max(np.abs(Zref[:, :, k])) for k in range(Z.shape[2]))
-    d = np.abs(np.diagonal(Z, axis1=0, axis2=1) - np.diagonal(Zref, axis1=0, axis2=1)) / \
-        np.abs(np.diagonal(Zref, axis1=0, axis2=1))
-    return float(nw), float(np.max(d))
-
-
 def host_cpu():
-    """Model name and logical CPU count of the host (the baseline uses one
-    thread: numpy's FFT and SuperLU run single-threaded)."""
+    """(model name, logical CPUs, threads the baseline may use): the box's
+    CPU share is 16 threads, whatever the host reports."""
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -111,15 +100,20 @@ def host_cpu():
                     break
     except OSError:
         pass
-    return "%s, %d logical CPUs, 1 thread used" % (model, os.cpu_count() or 0)
+    n = os.cpu_count() or 1
+    return model, n, max(1, min(16, n))
 
 
 def cpu_baseline(mesh, cfg, freq, matvecs_per_port, sample_matvecs):
-    """The oracle on the host, one core: one SS assembly + LU and
-    `sample_matvecs` preconditioned FFT matvecs, extrapolated to one
-    frequency point of the reference algorithm (per port, the plain path's
-    average matvec count)."""
+    """The oracle on the host: one SS assembly + LU (SuperLU, one thread --
+    the reference's UMFPACK factorization is single-threaded too) and
+    `sample_matvecs` preconditioned FFT matvecs with the FFTs on all the
+    threads the box gives (the reference's FFTW plans run 2 * nthreads
+    threads, compute_FFT_mutual_coupling_mats.jl:4), extrapolated to one
+    frequency point (per port, the plain path's average matvec count)."""
     from oracle import fft_solver as OF
+    model, ncpu, threads = host_cpu()
+    OF.set_fft_workers(threads)
     t0 = time.perf_counter()
     rows = OF.kernel_rows(mesh.origin, mesh.Nx, mesh.Ny, mesh.Nz, mesh.sx, mesh.sy, mesh.sz)
     spec = OF.spectra(rows, mesh.Nx, mesh.Ny, mesh.Nz)
@@ -139,84 +133,84 @@ def cpu_baseline(mesh, cfg, freq, matvecs_per_port, sample_matvecs):
         x = op.matvec(x)
         x /= np.linalg.norm(x)
     t_mv = (time.perf_counter() - t0) / sample_matvecs
+    OF.set_fft_workers(None)
     P = mesh.port_nodes.shape[0]
     per_freq = t_lu + P * matvecs_per_port * t_mv
-    return dict(value=1.0 / per_freq, unit="freq-point solves/s", cores=1, kind="port", host=host_cpu(),
+    return dict(value=1.0 / per_freq, unit="freq-point solves/s", cores=threads, kind="port",
+                host="%s, %d logical CPUs; %d threads for the FFTs, LU factorization and triangular solves on one"
+                     % (model, ncpu, threads),
                 sample="oracle on %s at %.3g Hz: one SS assembly + LU (%.2f s) and %d preconditioned FFT matvec(s) "
                        "(%.2f s each), extrapolated to %d ports x %.1f matvecs per port (the plain path's average); "
                        "one-time integrals + spectra %.1f s not included"
                        % (cfg["name"], freq, t_lu, sample_matvecs, t_mv, P, matvecs_per_port, t_setup))
 
 
-def matvec_only(args):
-    """FFT-matvec roofline on one workload: kernel spectra and operator on
-    the device (no LU), then `--matvec-reps` batched K x with --nrhs
-    right-hand sides in complex64 and complex128 under the tuned launch
-    shapes, every launch timed with HIP events.  Prints one JSON line."""
+def roofline_leg(workload, nrhs, reps, tune=True):
+    """The FFT matvec on `workload`: kernel spectra and operator on the
+    device (no LU), the passes tuned for `nrhs` right-hand sides in
+    complex64, then `reps` batched K x with every launch timed by HIP events
+    on the context's stream.  Returns the per-kind table and the roofline
+    of the pass kind with the largest time."""
     from solver_amd import configs, mesher, peec, sweep
-    cfg = configs.CONFIGS[args.workload]()
+    cfg = configs.CONFIGS[workload]()
     t0 = time.perf_counter()
     mesh = mesher.mesh(cfg["grid"], cfg["materials"], cfg["cell"], cfg["origin"], cfg["ports"], p_self_fn=gpu_p_self)
     N = mesh.m + mesh.ns + mesh.n
-    log("mesh %s: n=%d m=%d ns=%d N=%d (%.1f s)" % (cfg["name"], mesh.n, mesh.m, mesh.ns, N, time.perf_counter() - t0))
+    log("roofline leg %s: N=%d (mesh %.1f s)" % (workload, N, time.perf_counter() - t0))
     ctx = peec.Context(0)
     t0 = time.perf_counter()
-    peec.compute_FFT_mutual_coupling_mats(mesh.origin, mesh.Nx, mesh.Ny, mesh.Nz, mesh.sx, mesh.sy, mesh.sz,
-                                          mesh.escalings, ctx=ctx)
-    ctx_sync = time.perf_counter()
-    t_kern = ctx_sync - t0
-    peec.operator_set(mesh, ctx=ctx)
+    sweep.Solver(mesh, ctx=ctx, device_lu=False)
     w = 2 * np.pi * cfg["freqs"][0] * mesh.escalings["freq"]
     Yle = sweep.build_Yle_FFT(mesh.lumped_elements, mesh.port_nodes, mesh.escalings, mesh.n,
                               w / mesh.escalings["freq"], 50.0)
     peec.frequency_set(w, sweep.DZ_of(mesh, w), Yle, ctx=ctx)
     t_setup = time.perf_counter() - t0
-    log("integrals + spectra %.2f s, setup %.2f s" % (t_kern, t_setup))
-    nrhs = args.nrhs
     t0 = time.perf_counter()
-    peec.fft_tune(nrhs, progress=True, ctx=ctx)
+    if tune:
+        peec.fft_tune(nrhs, fp64=False, fp32=True, ctx=ctx)
     t_tune = time.perf_counter() - t0
-    rng = np.random.default_rng(0)
-    X = rng.normal(size=(N, nrhs)) + 1j * rng.normal(size=(N, nrhs))
-    res = {}
-    for fp32 in (True, False):
-        peec.apply_K_batched(X, nrhs, fp32=fp32, ctx=ctx)
-        profile_read(ctx, True)
-        for _ in range(args.matvec_reps):
-            peec.apply_K_batched(X, nrhs, fp32=fp32, ctx=ctx)
-        prof = [r for r in profile_read(ctx, False) if r["name"].startswith("fft")]
-        tot_ms = sum(r["ms"] for r in prof) / args.matvec_reps
-        tot_b = sum(r["bytes"] for r in prof) / args.matvec_reps
-        kinds = {}
-        for r in prof:
-            kinds[r["name"]] = dict(avg_us=round(1e3 * r["ms"] / max(r["count"], 1), 2),
-                                    GBs=round(r["bytes"] / max(r["ms"], 1e-12) / 1e6, 1),
-                                    bytes_per_launch=r["bytes"] / max(r["count"], 1))
-        key = "c64" if fp32 else "c128"
-        res[key] = dict(fft_ms_per_matvec=round(tot_ms, 3), fft_GB_per_matvec=round(tot_b / 1e9, 3),
-                        fft_GBs=round(tot_b / tot_ms / 1e6, 1), kinds=kinds)
-        log("%s: FFT part of one %d-rhs matvec %.2f ms, %.2f GB algorithmic, %.0f GB/s" % (
-            key, nrhs, tot_ms, tot_b / 1e9, tot_b / tot_ms / 1e6))
-    dom = max(res["c64"]["kinds"].items(), key=lambda kv: kv[1]["avg_us"])
-    out = {"metric": "FFT-matvec HBM GB/s (complex64, %d right-hand sides)" % nrhs,
-           "value": res["c64"]["fft_GBs"], "unit": "GB/s", "n_gpus": 1, "higher_is_better": True,
-           "data": "synthetic (seeded geometry, %s)" % cfg["name"],
-           "config": {"workload": cfg["name"], "grid": list(cfg["grid"].shape), "unknowns": int(N), "nrhs": nrhs,
-                      "setup_s": round(t_setup, 2), "integrals_spectra_s": round(t_kern, 2),
-                      "tune_s": round(t_tune, 2)},
-           "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": dom[1]["GBs"], "peak": PEAK_HBM_GBS,
-                        "unit": "GB/s", "frac": dom[1]["GBs"] / PEAK_HBM_GBS,
-                        "traffic": MEASURED_TRAFFIC.get((cfg["name"], dom[0])), "avg_launch_us": dom[1]["avg_us"],
-                        "bytes_per_launch": dom[1]["bytes_per_launch"]},
-           "fft": res}
-    print(json.dumps(out), flush=True)
+    X = np.random.default_rng(0).normal(size=(N, nrhs)) + 0j
+    peec.apply_K_batched(X, nrhs, fp32=True, ctx=ctx)
+    profile_read(ctx, True)
+    for _ in range(reps):
+        peec.apply_K_batched(X, nrhs, fp32=True, ctx=ctx)
+    prof = [r for r in profile_read(ctx, False) if r["name"].startswith("fft") and r["count"]]
+    ctx.close()
+    kinds = {}
+    for r in prof:
+        kinds[r["name"]] = dict(avg_us=1e3 * r["ms"] / r["count"], bytes_per_launch=r["bytes"] / r["count"],
+                                GBs=r["bytes"] / r["ms"] / 1e6, launches=r["count"])
+        log("  %-18s launches %4d  avg %8.1f us  %8.1f MB/launch  %7.1f GB/s  frac %.3f"
+            % (r["name"], r["count"], kinds[r["name"]]["avg_us"], kinds[r["name"]]["bytes_per_launch"] / 1e6,
+               kinds[r["name"]]["GBs"], kinds[r["name"]]["GBs"] / PEAK_HBM_GBS))
+    tot_ms = sum(r["ms"] for r in prof) / reps
+    tot_b = sum(r["bytes"] for r in prof) / reps
+    dom = max(prof, key=lambda r: r["ms"])["name"]
+    k = kinds[dom]
+    return dict(workload=workload, grid=list(cfg["grid"].shape), unknowns=int(N), nrhs=nrhs, setup_s=round(t_setup, 2),
+                tune_s=round(t_tune, 2), fft_matvec_ms=tot_ms, fft_GB_per_matvec=tot_b / 1e9,
+                fft_matvec_GBs=tot_b / tot_ms / 1e6, kinds=kinds,
+                roofline={"bound": "hbm", "kernel": dom, "symbol": KERNEL_SYMBOLS[dom], "achieved": k["GBs"],
+                          "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": k["GBs"] / PEAK_HBM_GBS,
+                          "traffic": MEASURED_TRAFFIC.get((workload, dom)), "avg_launch_us": k["avg_us"],
+                          "bytes_per_launch": k["bytes_per_launch"], "workload": workload, "nrhs": nrhs})
+
+
+def per_port_errors(Z, Zref):
+    """Relative error of every port's self impedance Z_ii (ports x
+    frequencies) and, per frequency, max |Z - Zref| / max |Zref| over the
+    port matrix."""
+    P, _, F = Z.shape
+    self_err = np.array([[abs(Z[i, i, k] - Zref[i, i, k]) / abs(Zref[i, i, k]) for k in range(F)] for i in range(P)])
+    nw = np.array([np.max(np.abs(Z[:, :, k] - Zref[:, :, k])) / np.max(np.abs(Zref[:, :, k])) for k in range(F)])
+    return self_err, nw
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4, help="timed batches")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed batches before the timed ones")
+    ap.add_argument("--steps", type=int, default=16, help="timed batches (16 x 8 frequencies = the whole band)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed repeats of the first batch")
     ap.add_argument("--workload", default="batched_96x96x24")
     ap.add_argument("--path", choices=["mixed", "fp64"], default="mixed",
                     help="batched sweep with complex64 (mixed) or complex128 FFT convolutions")
@@ -225,8 +219,10 @@ def main():
     ap.add_argument("--restart", type=int, default=100)
     ap.add_argument("--inner-tol", type=float, default=1e-7,
                     help="residual reduction each mixed-precision GMRES cycle aims for before the FP64 residual is "
-                         "recomputed (0: the library's 1e-5); 1e-7 measured fastest at the 1e-9 outer tolerance")
-    ap.add_argument("--plain-freqs", type=int, default=2, help="0 skips the plain path (profiling runs)")
+                         "recomputed (0: the library's 1e-5)")
+    ap.add_argument("--check-freqs", type=int, default=8,
+                    help="frequencies (spread over the band) of the per-port error check and the plain-path timing; "
+                         "0 skips both (profiling runs)")
     ap.add_argument("--inner", type=int, default=400, help="maxit of the plain path's gmres_custom")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
@@ -234,21 +230,22 @@ def main():
                          "(for kernel-tracing profilers)")
     ap.add_argument("--cpu-matvecs", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-workload", default="large_192x192x48")
+    ap.add_argument("--roofline-nrhs", type=int, default=8)
+    ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="run only the roofline leg and print its JSON (the profiled command)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB device to device (a known-size "
                          "reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--matvec-only", action="store_true",
-                    help="measure the FFT matvec alone on --workload (e.g. large_192x192x48) with --nrhs "
-                         "right-hand sides")
-    ap.add_argument("--nrhs", type=int, default=4)
-    ap.add_argument("--matvec-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X (the frequency sweep would shard "
               "across GPUs by frequency, one rank per GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.matvec_only:
-        matvec_only(args)
+    if args.roofline_only:
+        leg = roofline_leg(args.roofline_workload, args.roofline_nrhs, 3)
+        print(json.dumps(leg), flush=True)
         copy_reference(args.copy_mib)
         return
 
@@ -263,7 +260,7 @@ def main():
     N = mesh.m + mesh.ns + mesh.n
     log("mesh %s: n=%d m=%d ns=%d N=%d ports=%d (%.1f s)" % (cfg["name"], mesh.n, mesh.m, mesh.ns, N, P, t_mesh))
     t0 = time.perf_counter()
-    solver = sweep.Solver(mesh, device_lu=True)
+    solver = sweep.Solver(mesh, ctx=peec.Context(0), device_lu=True)
     t_setup = time.perf_counter() - t0
     peec.set_graphs(not args.no_graphs, ctx=solver.ctx)
     log("setup (integrals, spectra, operator upload, LU analysis): %.2f s" % t_setup)
@@ -274,77 +271,77 @@ def main():
     t_tune = time.perf_counter() - t0
     log("FFT autotune for %d right-hand sides: %.2f s" % (args.batch * P, t_tune))
 
-    # frequency subset: (warmup + steps) batches spread over the band
     fall = np.asarray(cfg["freqs"], dtype=np.float64)
-    nb = args.warmup + args.steps
-    idx = np.unique(np.linspace(0, len(fall) - 1, nb * args.batch).round().astype(int))
-    fsel = fall[idx]
-    batches = [fsel[i * args.batch:(i + 1) * args.batch] for i in range(nb)]
-    batches = [b for b in batches if len(b)]
+    band = [fall[i:i + args.batch] for i in range(0, len(fall), args.batch)]
+    steps = [band[i % len(band)] for i in range(args.steps)]
 
-    # every library call returns with its stream drained; this also waits
-    # for the whole device (hipDeviceSynchronize)
     def sync():
         L.check(L.lib().peec_synchronize(solver.ctx.handle), "peec_synchronize")
 
-    Vrest = None
-    rec, tm = [], {}
-    Zs = []
-    for b in batches[:args.warmup]:
-        out = sweep.FFT_solver_QS_batched(b, mesh, args.tol, batch=args.batch, mixed=mixed, restart=args.restart,
-                                          inner_tol=args.inner_tol,
-                                          solver=solver, Vrest=Vrest)
-        Vrest = out["Vrest"]
+    def run_batch(b, Vrest, rec=None, tm=None):
+        return sweep.FFT_solver_QS_batched(b, mesh, args.tol, batch=args.batch, mixed=mixed, restart=args.restart,
+                                           inner_tol=args.inner_tol, solver=solver, record=rec, timings=tm,
+                                           Vrest=Vrest)
+
+    for _ in range(args.warmup):
+        run_batch(band[0], None)
     sync()
+    rec, tm, Zs = [], {}, []
+    Vrest = None
     t0 = time.perf_counter()
-    for i, b in enumerate(batches[args.warmup:]):
-        out = sweep.FFT_solver_QS_batched(b, mesh, args.tol, batch=args.batch, mixed=mixed, restart=args.restart,
-                                          inner_tol=args.inner_tol,
-                                          solver=solver, record=rec, timings=tm, Vrest=Vrest)
+    for i, b in enumerate(steps):
+        if i % len(band) == 0:
+            Vrest = None  # each pass over the band starts cold, like the reference's sweep
+        out = run_batch(b, Vrest, rec, tm)
         Vrest = out["Vrest"]
         Zs.append(out["Z"])
         log("step %d: %d frequencies %.3g..%.3g Hz done" % (i, len(b), b[0], b[-1]))
     sync()
     dt = time.perf_counter() - t0
-    t_e2e = time.perf_counter() - t_start
-    ftimed = np.concatenate(batches[args.warmup:])
+    ftimed = np.concatenate(steps)
     Zt = np.concatenate(Zs, axis=2)
     nfreq = len(ftimed)
     value = nfreq / dt
     iters = [r["iters"] for r in rec]
     flags = sorted({r["flag"] for r in rec})
-    # GMRES runs a batch in lockstep: its matvec count is set by the slowest
-    # right-hand side, while each right-hand side needs only its own
     per_batch = [iters[i:i + args.batch * P] for i in range(0, len(iters), args.batch * P)]
-    lockstep = float(sum(max(b) for b in per_batch))
-    needed = float(sum(np.mean(b) for b in per_batch))
+    lockstep = float(np.mean([max(b) for b in per_batch]))
+    needed = float(np.mean([np.mean(b) for b in per_batch]))
     log("batched %s: %d frequency points in %.2f s -> %.3f solves/s; phases %s; inner iterations mean %.1f max %d, "
         "flags %s" % (args.path, nfreq, dt, value, {k: round(v, 2) for k, v in tm.items()}, np.mean(iters),
                       max(iters), flags))
 
-    # plain FP64 drop-in path at a few of the timed frequencies
-    plain_value = mv_per_port = err_nw = err_diag = None
-    fpl = ftimed[:1]
-    if args.plain_freqs > 0:
-        pidx = np.unique(np.linspace(0, nfreq - 1, args.plain_freqs).round().astype(int))
-        fpl = ftimed[pidx]
+    # per-port impedance error vs the plain FP64 path at 1e-12, and the
+    # plain path's own rate at the bench tolerance, at frequencies spread
+    # over the band (taken from the first pass over it)
+    plain_value = mv_per_port = None
+    err = None
+    if args.check_freqs > 0:
+        nb = min(len(fall), nfreq)
+        cidx = np.unique(np.linspace(0, nb - 1, args.check_freqs).round().astype(int))
+        fchk = ftimed[cidx]
         prec = []
         t0 = time.perf_counter()
-        outp = sweep.FFT_solver_QS_S_type(fpl, mesh, args.inner, args.tol, solver=solver, record=prec)
+        outp = sweep.FFT_solver_QS_S_type(fchk, mesh, args.inner, args.tol, solver=solver, record=prec)
         dt_plain = time.perf_counter() - t0
-        plain_value = len(fpl) / dt_plain
+        plain_value = len(fchk) / dt_plain
         mv_per_port = float(np.mean([r["nres"] + 1 for r in prec]))
         log("plain FP64: %d frequency points in %.2f s -> %.3f solves/s (%.1f matvecs per port)"
-            % (len(fpl), dt_plain, plain_value, mv_per_port))
-        # reference impedances: the plain FP64 path to a 1e-12 residual
-        outr = sweep.FFT_solver_QS_S_type(fpl, mesh, args.inner, 1e-12, solver=solver)
-        err_nw, err_diag = z_error(Zt[:, :, pidx], outr["Z"])
-        err_plain_nw, _ = z_error(outp["Z"], outr["Z"])
-        log("impedance error vs FP64 at 1e-12: mixed batched %.2e (norm-wise), %.2e (port self-impedances); "
-            "plain at tol %.0e: %.2e" % (err_nw, err_diag, args.tol, err_plain_nw))
+            % (len(fchk), dt_plain, plain_value, mv_per_port))
+        outr = sweep.FFT_solver_QS_S_type(fchk, mesh, args.inner, 1e-12, solver=solver)
+        self_err, nw = per_port_errors(Zt[:, :, cidx], outr["Z"])
+        self_plain, nw_plain = per_port_errors(outp["Z"], outr["Z"])
+        for k in range(len(fchk)):
+            log("  %.4g Hz: mixed per-port |dZ_ii|/|Z_ii| %s; matrix %.2e; plain at tol %.0e: %.2e"
+                % (fchk[k], " ".join("%.1e" % e for e in self_err[:, k]), nw[k], args.tol, nw_plain[k]))
+        err = dict(freqs_hz=[float(f) for f in fchk], reference="plain FP64 path (gmres_custom per port) at 1e-12",
+                   port_self_rel_err=[[float(e) for e in row] for row in self_err],
+                   matrix_rel_err=[float(e) for e in nw], max_port_self_rel_err=float(self_err.max()),
+                   max_matrix_rel_err=float(nw.max()), plain_at_tol_max_matrix_rel_err=float(nw_plain.max()),
+                   within_1e6=bool(self_err.max() <= 1e-6 and nw.max() <= 1e-6))
 
-    # profiled batched matvec at the last batch's frequency slots
-    last = batches[-1]
+    # one profiled batched matvec at the last batch's frequency slots
+    last = steps[-1]
     peec.frequency_slots(len(last), ctx=solver.ctx)
     sweep.set_slots(solver, 2 * np.pi * last * mesh.escalings["freq"])
     X = np.ascontiguousarray(np.tile(Vrest, (1, len(last))))
@@ -359,24 +356,29 @@ def main():
         r["gbs"] = r["bytes"] / max(r["ms"], 1e-12) / 1e6
         log("  kernel %-18s launches %6d  time %8.3f ms  avg %9.2f us  bytes %.3e  %.1f GB/s" % (
             r["name"], r["count"], r["ms"], r["avg_us"], r["bytes"], r["gbs"]))
-    dom = max((r for r in prof if r["name"] != "lu_factor"), key=lambda r: r["ms"])
     mv_ms = sum(r["ms"] for r in prof) / nprof
     fft_ms = sum(r["ms"] for r in prof if r["name"].startswith("fft")) / nprof
     fft_bytes = sum(r["bytes"] for r in prof if r["name"].startswith("fft")) / nprof
+    t_e2e = time.perf_counter() - t_start
 
     cpu = None
     if not args.no_cpu_baseline and mv_per_port is not None:
-        cpu = cpu_baseline(mesh, cfg, float(fpl[0]), mv_per_port, args.cpu_matvecs)
+        cpu = cpu_baseline(mesh, cfg, float(ftimed[0]), mv_per_port, args.cpu_matvecs)
         log("cpu baseline: %s" % cpu)
+    solver.ctx.close()
+
+    leg = None
+    if not args.no_roofline:
+        leg = roofline_leg(args.roofline_workload, args.roofline_nrhs, 3)
 
     out = {
         "metric": "freq-point solves/sec at 1e-6 rel. impedance err; FFT-matvec HBM GB/s",
         "value": value,
         "unit": "freq-point solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": len(steps),
         "warmup": args.warmup,
-        "ms_per_step": dt * 1e3 / max(len(batches) - args.warmup, 1),
+        "ms_per_step": dt * 1e3 / len(steps),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -386,26 +388,23 @@ def main():
                    "path": "batched %s" % ("mixed precision (complex64 FFT convolutions, FP64 refinement)" if mixed
                                            else "FP64"),
                    "freq_points_timed": int(nfreq), "freq_points_per_step": args.batch,
-                   "freq_band_hz": [float(ftimed[0]), float(ftimed[-1])], "gmres_tol": args.tol,
+                   "freq_band_hz": [float(ftimed.min()), float(ftimed.max())], "gmres_tol": args.tol,
                    "restart": args.restart, "inner_tol": args.inner_tol, "inner_iters_mean": float(np.mean(iters)),
-                   "flags": flags,
-                   "inner_iters_lockstep_per_batch": lockstep / len(per_batch),
-                   "inner_iters_mean_per_batch": needed / len(per_batch),
+                   "flags": flags, "inner_iters_lockstep_per_batch": lockstep,
+                   "inner_iters_mean_per_batch": needed,
                    "phase_s": {k: round(v, 3) for k, v in tm.items()},
-                   "z_err_vs_fp64": err_nw, "z_err_port_self": err_diag,
+                   "z_err": err,
                    "plain_fp64_solves_per_s": plain_value,
                    "speedup_vs_plain_fp64": value / plain_value if plain_value else None,
                    "plain_matvecs_per_port": mv_per_port,
                    "setup_s": round(t_setup, 3), "tune_s": round(t_tune, 3),
                    "end_to_end_s": round(t_e2e, 3), "hip_graphs": not args.no_graphs,
                    "batched_matvec_kernel_ms": round(mv_ms, 3), "fft_matvec_ms": round(fft_ms, 3),
-                   "fft_matvec_GBs": round(fft_bytes / fft_ms / 1e6, 1)},
-        "roofline": {"bound": "hbm", "kernel": dom["name"], "symbol": KERNEL_SYMBOLS[dom["name"]],
-                     "achieved": dom["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": dom["gbs"] / PEAK_HBM_GBS,
-                     "traffic": MEASURED_TRAFFIC.get((cfg["name"], dom["name"])), "avg_launch_us": dom["avg_us"],
-                     "bytes_per_launch": dom["bytes"] / max(dom["count"], 1)},
+                   "fft_matvec_GBs": round(fft_bytes / fft_ms / 1e6, 1),
+                   "roofline_leg": None if leg is None else {k: v for k, v in leg.items() if k != "roofline"}},
+        "roofline": None if leg is None else leg["roofline"],
         "kernels": {r["name"]: {"launches_per_matvec": r["count"] / nprof, "avg_us": round(r["avg_us"], 3),
-                                "GBs": round(r["gbs"], 1)} for r in prof},
+                                "GBs": round(r["gbs"], 1)} for r in prof if r["count"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

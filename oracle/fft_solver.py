@@ -76,12 +76,28 @@ def spectra(rows, Nx, Ny, Nz):
     return out
 
 
+# FFT worker threads of the convolutions (None: numpy's single-threaded
+# pocketfft; an integer: scipy.fft with that many threads, the reference's
+# multithreaded FFTW plan, compute_FFT_mutual_coupling_mats.jl:4).  Both
+# compute every 1-D transform the same way; the threads split the batch.
+FFT_WORKERS = None
+
+
+def set_fft_workers(n):
+    global FFT_WORKERS
+    FFT_WORKERS = n
+
+
 def conv(spec, vec, n):
     """ifft(spec .* fft(pad(vec))), first octant (compute_Matrix_vector_fft.jl:149-161)."""
     nx, ny, nz = n
     pad = np.zeros(spec.shape, dtype=np.complex128)
     pad[:nx, :ny, :nz] = vec.reshape(nz, ny, nx).transpose(2, 1, 0)
-    out = np.fft.ifftn(spec * np.fft.fftn(pad))
+    if FFT_WORKERS:
+        import scipy.fft as sfft
+        out = sfft.ifftn(spec * sfft.fftn(pad, workers=FFT_WORKERS), workers=FFT_WORKERS)
+    else:
+        out = np.fft.ifftn(spec * np.fft.fftn(pad))
     return out[:nx, :ny, :nz].transpose(2, 1, 0).reshape(-1)
 
 

@@ -18,6 +18,7 @@
 // (radix 8/4/2/3/5 butterflies held in registers between barriers), and
 // stores the tile back.
 #include "context.h"
+#include "fft_common.h"
 
 #include <algorithm>
 #include <cmath>
@@ -88,6 +89,11 @@ int64_t peec_ctx::twiddle_offset(int L)
     return twiddle_off[L];
 }
 
+// Register an FFT length's twiddles: the table is rebuilt (and its offsets
+// move) whenever a length appears, so every length is registered before the
+// first launch.
+static bool register_fft_length(peec_ctx *ctx, int L) { return L <= 0 || ctx->twiddle_offset(L) >= 0; }
+
 void peec_ctx::release_kernels()
 {
     for (auto &k : kern) {
@@ -106,102 +112,7 @@ void peec_ctx::release_kernels()
 // ------------------------------------------------------------------ device
 namespace {
 
-template <typename T>
-struct C2 {
-    T x, y;
-};
-
-template <typename T>
-__device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
-template <typename T>
-__device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
-template <typename T>
-__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-template <typename T>
-__device__ __forceinline__ C2<T> cscale(C2<T> a, T s) { return {a.x * s, a.y * s}; }
-// multiply by -i (forward) or +i (inverse)
-template <typename T, int DIR>
-__device__ __forceinline__ C2<T> rot(C2<T> a) { return DIR < 0 ? C2<T>{a.y, -a.x} : C2<T>{-a.y, a.x}; }
-
-template <typename T, int DIR>
-__device__ __forceinline__ void dft2(C2<T> *v)
-{
-    C2<T> a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-}
-
-template <typename T, int DIR>
-__device__ __forceinline__ void dft4(C2<T> *v)
-{
-    C2<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-    C2<T> s13 = cadd(v[1], v[3]), d13 = rot<T, DIR>(csub(v[1], v[3]));
-    v[0] = cadd(s02, s13);
-    v[2] = csub(s02, s13);
-    v[1] = cadd(d02, d13);
-    v[3] = csub(d02, d13);
-}
-
-template <typename T, int DIR>
-__device__ __forceinline__ void dft8(C2<T> *v)
-{
-    C2<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
-    dft4<T, DIR>(e);
-    dft4<T, DIR>(o);
-    const T h = (T)0.70710678118654752440084436210485;
-    // o[k] *= exp(DIR * 2 pi i k / 8)
-    C2<T> w1 = DIR < 0 ? C2<T>{h, -h} : C2<T>{h, h};
-    C2<T> w3 = DIR < 0 ? C2<T>{-h, -h} : C2<T>{-h, h};
-    o[1] = cmul(o[1], w1);
-    o[2] = rot<T, DIR>(o[2]);
-    o[3] = cmul(o[3], w3);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        v[k] = cadd(e[k], o[k]);
-        v[k + 4] = csub(e[k], o[k]);
-    }
-}
-
-template <typename T, int DIR>
-__device__ __forceinline__ void dft3(C2<T> *v)
-{
-    const T s = (T)0.86602540378443864676372317075294;
-    C2<T> b = cadd(v[1], v[2]);
-    C2<T> d = rot<T, DIR>(cscale(csub(v[1], v[2]), s));
-    C2<T> t = csub(v[0], cscale(b, (T)0.5));
-    v[0] = cadd(v[0], b);
-    v[1] = cadd(t, d);
-    v[2] = csub(t, d);
-}
-
-template <typename T, int DIR>
-__device__ __forceinline__ void dft5(C2<T> *v)
-{
-    const T c1 = (T)0.30901699437494742410229341718282, c2 = (T)-0.80901699437494742410229341718282;
-    const T s1 = (T)0.95105651629515357211643933337938, s2 = (T)0.58778525229247312916870595463907;
-    C2<T> b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
-    C2<T> d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
-    C2<T> a0 = v[0];
-    C2<T> r1 = cadd(a0, cadd(cscale(b1, c1), cscale(b2, c2)));
-    C2<T> r2 = cadd(a0, cadd(cscale(b1, c2), cscale(b2, c1)));
-    C2<T> i1 = rot<T, DIR>(cadd(cscale(d1, s1), cscale(d2, s2)));
-    C2<T> i2 = rot<T, DIR>(csub(cscale(d1, s2), cscale(d2, s1)));
-    v[0] = cadd(a0, cadd(b1, b2));
-    v[1] = cadd(r1, i1);
-    v[4] = csub(r1, i1);
-    v[2] = cadd(r2, i2);
-    v[3] = csub(r2, i2);
-}
-
-template <typename T, int DIR, int R>
-__device__ __forceinline__ void dftR(C2<T> *v)
-{
-    if (R == 2) dft2<T, DIR>(v);
-    else if (R == 3) dft3<T, DIR>(v);
-    else if (R == 4) dft4<T, DIR>(v);
-    else if (R == 5) dft5<T, DIR>(v);
-    else dft8<T, DIR>(v);
-}
+using namespace fftk;
 
 // One Stockham stage over a tile of W = (256 / TPL) * LPT lines held in LDS
 // (line stride LS), of which the first nl are transformed.  Lane `lane` of
@@ -255,29 +166,6 @@ __device__ __forceinline__ void stockham_stage(C2<T> *tile, int LS, int L, int N
     __syncthreads();
 }
 
-struct PassDesc {
-    int L, nin, nout;
-    int nstage;
-    int radix[16];
-    int64_t tw_off;
-    int nb, n2;                          // lines: nb consecutive positions (W per block) x n2
-    int64_t in_sb, in_s2, in_sj;
-    int64_t out_sb, out_s2, out_sj;
-    int64_t in_rhs, out_rhs;             // stride between right-hand sides
-    int nrhs;
-    int rpb;                             // right-hand sides per block (blockIdx.z covers rpb of them)
-    const int32_t *map_in;               // optional 0/1 scatter (compact position -> unknown)
-    const double2_t *vec_in;
-    int64_t vec_in_ld;                   // stride between right-hand sides in vec_in
-    const int32_t *map_out;              // optional 0/1 gather
-    double2_t *vec_out;
-    int64_t vec_out_ld;
-    const void *spec;                    // spectrum for the fused z pass: real T (MODE 2), C2<T> (MODE 3)
-    int64_t spec_sb, spec_s2, spec_sj;
-    int spec_conj;                       // MODE 3: multiply by the conjugate spectrum
-    int64_t tw_off_z;                    // fused y-z pass: z twiddles (z length is a template argument)
-};
-
 template <typename T, int DIR, int TPL, int LPT, int LMAX>
 __device__ void run_fft(C2<T> *tile, int LS, int L, const PassDesc &d, int lg, int lane, const C2<T> *twl,
                         int nl = 256 / TPL * LPT)
@@ -294,17 +182,6 @@ __device__ void run_fft(C2<T> *tile, int LS, int L, const PassDesc &d, int lg, i
         }
         Ns *= R;
     }
-}
-
-// Tile (bx, by, bz) of this workgroup in XCD order (xcd_tile over the x
-// fastest, then y, then z tile order).
-__device__ __forceinline__ void xcd_block(unsigned &bx, unsigned &by, unsigned &bz)
-{
-    const unsigned gx = gridDim.x, gy = gridDim.y;
-    const unsigned t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx * gy * gridDim.z);
-    bx = t % gx;
-    by = (t / gx) % gy;
-    bz = t / (gx * gy);
 }
 
 // e / n for 0 <= e < 2^17 and 1 <= n <= 1024: the float quotient of e + 1/2
@@ -532,53 +409,6 @@ __global__ void __launch_bounds__(256) k_fft_pass(PassDesc d, const C2<T> *src, 
 // unrolling, so the arrays live in VGPRs), and stores the nout rows.  No
 // LDS and no barriers; the block's spectrum stays in registers for its rpb
 // right-hand sides.  Same Stockham stages and twiddles as the LDS passes.
-template <int L, int S>
-struct RegPlan {
-    // radix of stage S (8 first, then 4, 2, 3, 5), and the stages' product
-    static constexpr int rem(int l, int s)
-    {
-        return s == 0 ? l : rem(l / (l % 8 == 0 ? 8 : l % 4 == 0 ? 4 : l % 2 == 0 ? 2 : l % 3 == 0 ? 3 : 5), s - 1);
-    }
-    static constexpr int r = rem(L, S) == 1 ? 1
-                             : rem(L, S) % 8 == 0 ? 8
-                             : rem(L, S) % 4 == 0 ? 4
-                             : rem(L, S) % 2 == 0 ? 2
-                             : rem(L, S) % 3 == 0 ? 3 : 5;
-    static constexpr int ns = L / rem(L, S);  // product of the radices of stages 0 .. S-1
-};
-
-template <typename T, int DIR, int L, int S>
-__device__ __forceinline__ void reg_stages(C2<T> (&a)[L], const double2_t *__restrict__ tw)
-{
-    constexpr int R = RegPlan<L, S>::r;
-    if constexpr (R > 1) {
-        constexpr int Ns = RegPlan<L, S>::ns;
-        constexpr int nb = L / R;
-        constexpr int stride = L / (Ns * R);
-        C2<T> b[L];
-#pragma unroll
-        for (int k = 0; k < nb; k++) {
-            const int kk = k % Ns;
-            C2<T> v[R];
-#pragma unroll
-            for (int r = 0; r < R; r++) v[r] = a[k + r * nb];
-            if (Ns > 1) {
-#pragma unroll
-                for (int r = 1; r < R; r++) {
-                    const double2_t w = tw[kk * r * stride];
-                    v[r] = cmul(v[r], C2<T>{(T)w.x, (T)(DIR < 0 ? w.y : -w.y)});
-                }
-            }
-            dftR<T, DIR, R>(v);
-#pragma unroll
-            for (int r = 0; r < R; r++) b[(k - kk) * R + kk + r * Ns] = v[r];
-        }
-#pragma unroll
-        for (int j = 0; j < L; j++) a[j] = b[j];
-        reg_stages<T, DIR, L, S + 1>(a, tw);
-    }
-}
-
 // MODE 2: real spectrum, MODE 3: complex spectrum (conjugated with spec_conj).
 // Lines: x = blockIdx.x * 256 + threadIdx.x < nb, y = blockIdx.y; right-hand
 // sides blockIdx.z * rpb ...; src and dst may alias (in place).
@@ -843,6 +673,11 @@ __global__ void k_spec_extract(const double2_t *__restrict__ g, int64_t G, doubl
 }  // namespace
 
 // ------------------------------------------------------------------ host side
+// lane-split passes (fft_lanes.hip)
+bool lane_fits(const fftk::PassDesc &d, int Q, int mode);
+bool lane_built(int L, int Q, int mode);
+int launch_lane_pass(peec_ctx *ctx, bool fp32, int mode, fftk::PassDesc &d, const void *src, void *dst, int Q);
+
 struct PassGeom {
     int L, nin, nout, nb, n2;
     int64_t in_sb, in_s2, in_sj, out_sb, out_s2, out_sj;
@@ -929,7 +764,7 @@ static int launch_pass_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst,
 // z lengths the register kernel is built for (complex64; complex128 up to
 // 32), and the z embedding length of a compact extent n: the smallest of
 // them >= 2n - 1 (any length >= 2n - 1 embeds the convolution), else the
-// smallest 2^a 3^b 5^c.
+// smallest multiple of 16 of the form 16 * 2^a 3^b 5^c.
 static const int kRegLengths[] = {8, 16, 32, 48, 64};
 static bool reg_length(int L, bool fp32)
 {
@@ -941,7 +776,10 @@ int fft_z_length(int n)
 {
     for (int l : kRegLengths)
         if (l >= 2 * n - 1) return l;
-    return fft_smooth_length(2 * n - 1);
+    // longer z lines: a multiple of 16 (16 * 2^a 3^b 5^c), so that the lane
+    // kernels can split them over 8 or 16 lanes with an even share per lane
+    for (int L = 2 * n - 1;; L++)
+        if (L % 16 == 0 && fft_smooth_length(L / 16) == L / 16) return L;
 }
 
 template <typename T, int MODE>
@@ -1018,12 +856,12 @@ static int make_desc(peec_ctx *ctx, const PassGeom &g, PassDesc *d, int style = 
     d->nout = g.nout;
     d->nstage = p.nstages;
     for (int i = 0; i < p.nstages; i++) d->radix[i] = p.radix[i];
-    int64_t off = ctx->twiddle_offset(g.L);
-    if (off < 0) {
+    // every offset is read after the last registration (a new length moves them)
+    if (!register_fft_length(ctx, g.L)) {
         peec_set_error("twiddle table allocation failed");
         return PEEC_ERR_NOMEM;
     }
-    d->tw_off = off;
+    d->tw_off = ctx->twiddle_offset(g.L);
     d->nb = g.nb;
     d->n2 = g.n2;
     d->in_sb = g.in_sb; d->in_s2 = g.in_s2; d->in_sj = g.in_sj;
@@ -1037,10 +875,23 @@ static int RegFirstRadix(int L) { return L % 8 == 0 ? 8 : L % 4 == 0 ? 4 : L % 2
 
 // Launch shape kRegVariant: the register z kernel (fused z passes only).
 static constexpr int kRegVariant = kNumFftVariants;
+// Launch shapes kLaneBase + k: the lane-split kernel with Q = 2 << k lanes
+// per line (strided passes: y and the fused z pass), in fft_lanes.hip.
+static constexpr int kLaneBase = kRegVariant + 1, kNumLaneQ = 5;
+static bool is_lane_variant(int v) { return v >= kLaneBase && v < kLaneBase + kNumLaneQ; }
+static int lane_q(int v) { return 2 << (v - kLaneBase); }
 
 static int launch_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const void *src, void *dst, int v = -1)
 {
     if (d.nb <= 0 || d.n2 <= 0 || d.nrhs <= 0) return PEEC_OK;
+    if (is_lane_variant(v)) {
+        const int Q = lane_q(v);
+        if (!lane_fits(d, Q, mode)) {
+            peec_set_error("lane-split FFT pass does not apply (mode %d, length %d, %d lanes)", mode, d.L, Q);
+            return PEEC_ERR_ARG;
+        }
+        return launch_lane_pass(ctx, fp32, mode, d, src, dst, Q);
+    }
     if (v == kRegVariant) {
         // radix-8-first schedule only (the plan the kernel unrolls)
         if (mode < 2 || !reg_length(d.L, fp32) || d.nstage < 1 || d.radix[0] != RegFirstRadix(d.L)) {
@@ -1222,7 +1073,7 @@ int poly_spectrum_build(peec_ctx *ctx, int k, const int delta[3])
     for (int a = 0; a < 3; a++) {
         K.pn[a] = kHalfAxes[k][a] ? (K.n[a] + 1) / 2 : K.n[a];
         K.pL[a] = a == 2 ? fft_z_length(K.pn[a]) : fft_smooth_length(2 * K.pn[a] - 1);
-        if (ctx->twiddle_offset(K.pL[a]) < 0) {
+        if (!register_fft_length(ctx, K.pL[a])) {
             peec_set_error("operator_set: twiddle table allocation failed");
             return PEEC_ERR_NOMEM;
         }
@@ -1265,7 +1116,7 @@ int peec_kernels_build_impl(peec_ctx *ctx, const double origin[3], int Nx, int N
     // is reallocated when a new length appears
     for (int k = 0; k < 9; k++)
         for (int a = 0; a < 3; a++)
-            if (ctx->kern[k].L[a] > 0 && ctx->twiddle_offset(ctx->kern[k].L[a]) < 0) {
+            if (!register_fft_length(ctx, ctx->kern[k].L[a])) {
                 peec_set_error("kernels_build: twiddle table allocation failed");
                 return PEEC_ERR_NOMEM;
             }
@@ -1459,6 +1310,13 @@ static int default_rpb(const PassDesc &d, int v)
     return (int)std::max<int64_t>(1, std::min<int64_t>(16, tiles * d.nrhs / 2048));
 }
 
+static int default_lane_rpb(const PassDesc &d, int Q)
+{
+    const int lpb = 256 / Q;
+    const int64_t tiles = (int64_t)((d.nb + lpb - 1) / lpb) * d.n2;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(16, tiles * d.nrhs / 4096));
+}
+
 // The tuned choice for exactly nrhs right-hand sides, else the one tuned for
 // the nearest width within a factor 4 (GMRES applies the matvec to the
 // columns still iterating, whose count falls below the tuned width): its launch
@@ -1648,6 +1506,22 @@ static int tune_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc d, const void 
             best.style = 0;
         }
     }
+    for (int k = 0; k < kNumLaneQ; k++) {
+        const int Q = 2 << k;
+        if (!lane_fits(d, Q, mode)) continue;
+        d.rpb = default_lane_rpb(d, Q);
+        float t;
+        if ((rc = time_pass(ctx, fp32, mode, d, src, dst, kLaneBase + k, 3, ea, eb, &t))) return rc;
+        if (verbose)
+            fprintf(stderr, "peec_fft_tune: kernel %d pass %d %s L %d: lane-split kernel Q %d rpb %d: %.1f us, %.0f GB/s\n",
+                    kernel, pass, fp32 ? "c64" : "c128", d.L, Q, d.rpb, 1e3 * t, bytes / (1e6 * t));
+        if (t < tbest) {
+            tbest = t;
+            best.variant = kLaneBase + k;
+            best.rpb = d.rpb;
+            best.style = 0;
+        }
+    }
     if (best.variant < 0) {
         peec_set_error("peec_fft_tune: no launch shape for length %d", d.L);
         return PEEC_ERR_ARG;
@@ -1662,7 +1536,7 @@ static int tune_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc d, const void 
             best.rpb = r;
         }
     }
-    if (fp32 && best.variant != kRegVariant) {
+    if (fp32 && best.variant != kRegVariant && !is_lane_variant(best.variant)) {
         for (int st = 1; st <= 2; st++) {
             PassDesc ds;
             PassGeom gg = {d.L, d.nin, d.nout, d.nb, d.n2, d.in_sb, d.in_s2, d.in_sj, d.out_sb, d.out_s2, d.out_sj};
@@ -1797,8 +1671,9 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
 {
     const bool fused = pass >= kPassXFwdPlanes;
     if (!ctx || kernel < 0 || kernel > 8 || pass < 0 || pass > kPassXInvPlanes || nrhs < 1 || variant < -1 ||
-        variant > kRegVariant || rpb < 1 || style < 0 || style > 2 || (style != 0 && !fp32) || (fused && !fp32) ||
-        (variant == kRegVariant && (pass != 2 || style != 0)) || (pass == kPassYz && variant > 1)) {
+        variant >= kLaneBase + kNumLaneQ || rpb < 1 || style < 0 || style > 2 || (style != 0 && !fp32) ||
+        (fused && !fp32) || (variant == kRegVariant && (pass != 2 || style != 0)) ||
+        (is_lane_variant(variant) && (pass < 1 || pass > 3 || style != 0)) || (pass == kPassYz && variant > 1)) {
         peec_set_error("peec_fft_set_choice: invalid argument");
         return PEEC_ERR_ARG;
     }
@@ -1820,8 +1695,11 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
     }
     const int axis = (pass == 0 || pass == 4 || fused) ? 0 : (pass == 2 ? 2 : 1);
     const int L = app ? app->L[axis] : 0;
-    if (!app || (variant == kRegVariant ? !(reg_length(L, fp32 != 0) && L % RegFirstRadix(L) == 0)
-                                        : !variant_fits(variant, L, fp32 != 0))) {
+    const bool fits = !app ? false
+                      : variant == kRegVariant ? reg_length(L, fp32 != 0) && L % RegFirstRadix(L) == 0
+                      : is_lane_variant(variant) ? L % lane_q(variant) == 0 && lane_built(L, lane_q(variant), pass == 2 ? 2 : 0)
+                      : variant_fits(variant, L, fp32 != 0);
+    if (!fits) {
         peec_set_error("peec_fft_set_choice: launch variant %d does not fit length %d", variant, L);
         return PEEC_ERR_ARG;
     }
@@ -1840,8 +1718,9 @@ extern "C" int peec_fft_choice(peec_ctx *ctx, int kernel, int pass, int fp32, in
     const FftChoice c = tuned_choice(ctx, kernel, pass, fp32 != 0, nrhs);
     const bool table = c.variant >= 0 && c.variant < kNumFftVariants && pass != kPassYz;
     out5[0] = c.variant;
-    out5[1] = table ? variant_lines(c.variant) : (c.variant == kRegVariant ? 256 : 0);
-    out5[2] = table ? kFftVariants[c.variant].tpl : (c.variant == kRegVariant ? 1 : 0);
+    const bool lane = is_lane_variant(c.variant) && pass != kPassYz;
+    out5[1] = table ? variant_lines(c.variant) : lane ? 256 / lane_q(c.variant) : (c.variant == kRegVariant ? 256 : 0);
+    out5[2] = table ? kFftVariants[c.variant].tpl : lane ? lane_q(c.variant) : (c.variant == kRegVariant ? 1 : 0);
     out5[3] = c.rpb;
     out5[4] = c.style;
     return PEEC_OK;

@@ -1,0 +1,462 @@
+// Lane-split FFT passes: strided lines (the y passes and the fused z
+// convolution) transformed in registers with Q lanes per line and
+// cross-lane butterflies.  Part of the circulant convolution pipeline of
+// fft.hip, which picks these kernels through its autotuner.
+#include "fft_common.h"
+
+#include <algorithm>
+
+namespace {
+
+using namespace fftk;
+
+// ---- strided-line passes with each line split over Q lanes
+//
+// Lines whose elements lie a plane or a row apart (the z and y passes; a
+// block's tile runs along the contiguous x axis) are transformed in
+// registers by Q lanes per line of length L = Q M: lane q holds the M
+// elements n = q + Q m.  Forward: an M-point DFT per lane (Stockham stages,
+// radix 8 first), the twiddle W_L^(q k2) of its output k2, then a Q-point
+// DFT across the Q lanes by radix-2 decimation-in-frequency exchanges (DPP
+// quad permutations for lane distances 1 and 2, ds_swizzle for 4 .. 16).
+// Lane p then holds the transform's entries k2 + M rev(p) (rev: bit
+// reversal over log2 Q bits), where MODE 2/3 take the spectrum product in
+// place.  The inverse runs the mirror image -- decimation-in-time exchanges
+// with conjugate twiddles, the conjugate W_L^(q k2), the inverse M-point
+// DFT -- and ends in natural order.  MODE 0 (forward only) stores from the
+// bit-reversed layout, MODE 1 (inverse only) loads into it.
+//
+// Registers: no LDS and no barriers, so occupancy is set by VGPRs alone.
+// The twiddles of the per-lane DFT are compile-time constants (cx_twiddle,
+// folded into the instructions); the lane-dependent ones are selected from
+// constants every right-hand side instead of being held across the loop;
+// addresses are a wave-uniform base plus one 32-bit lane offset.  What stays
+// in registers for the block's rpb right-hand sides is the spectrum.
+
+// exp(-2 pi i t / L) at compile time: octant reduction of t / L, Taylor
+// series of sin and cos on [0, pi/4] (within 2 ulp of a long-double
+// evaluation for every t, L <= 1024).
+struct CxTw {
+    double re, im;
+};
+constexpr double kCxPi = 3.141592653589793238462643383279502884;
+constexpr double cx_sin_q(double x)
+{
+    double x2 = x * x, t = x, s = x;
+    for (int n = 1; n < 12; n++) {
+        t *= -x2 / (double)((2 * n) * (2 * n + 1));
+        s += t;
+    }
+    return s;
+}
+constexpr double cx_cos_q(double x)
+{
+    double x2 = x * x, t = 1.0, s = 1.0;
+    for (int n = 1; n < 12; n++) {
+        t *= -x2 / (double)((2 * n - 1) * (2 * n));
+        s += t;
+    }
+    return s;
+}
+constexpr CxTw cx_twiddle(int t, int L)
+{
+    const long long r = ((long long)t % L + L) % L;
+    const long long a = 8 * r, o = a / L, f = a - o * L;  // angle 2 pi r / L = (pi / 4) (o + f / L)
+    double c = 0.0, s = 0.0;
+    if ((o & 1) == 0) {
+        const double ph = (kCxPi / 4) * ((double)f / (double)L), cp = cx_cos_q(ph), sp = cx_sin_q(ph);
+        switch ((o / 2) & 3) {
+        case 0: c = cp; s = sp; break;
+        case 1: c = -sp; s = cp; break;
+        case 2: c = -cp; s = -sp; break;
+        default: c = sp; s = -cp; break;
+        }
+    } else {
+        const double ph = (kCxPi / 4) * ((double)(L - f) / (double)L), cp = cx_cos_q(ph), sp = cx_sin_q(ph);
+        switch (((o + 1) / 2) & 3) {
+        case 0: c = cp; s = -sp; break;
+        case 1: c = sp; s = cp; break;
+        case 2: c = -cp; s = sp; break;
+        default: c = -sp; s = -cp; break;
+        }
+    }
+    return {c, -s};
+}
+
+// W_L^E (DIR < 0) or its conjugate as a constant of type T: the template
+// arguments force the front end to evaluate cx_twiddle.
+template <typename T, int DIR, int L, int E>
+__device__ __forceinline__ C2<T> cxc()
+{
+    constexpr CxTw w = cx_twiddle(E, L);
+    return {(T)w.re, (T)(DIR < 0 ? w.im : -w.im)};
+}
+
+// v[r] *= W_L^(r BASE), r = 1 .. R-1
+template <typename T, int DIR, int L, int R, int BASE, int r = 1>
+__device__ __forceinline__ void cx_tw_apply(C2<T> (&v)[R])
+{
+    if constexpr (BASE > 0 && r < R) {
+        v[r] = cmul(v[r], cxc<T, DIR, L, r * BASE>());
+        cx_tw_apply<T, DIR, L, R, BASE, r + 1>(v);
+    }
+}
+
+// DFT of radix R (2, 4 or 8) whose inputs v[R/2 .. R-1] are zero: the
+// operations of dft2 / dft4 / dft8 that do not involve those zeros, in the
+// same order (an x + 0 cannot be folded by the compiler under IEEE rules).
+template <typename T, int DIR, int R>
+__device__ __forceinline__ void dft_halfin(C2<T> *v)
+{
+    if constexpr (R == 2) {
+        v[1] = v[0];
+    } else if constexpr (R == 4) {
+        const C2<T> a = v[0], b = v[1], rb = rot<T, DIR>(b);
+        v[0] = cadd(a, b);
+        v[2] = csub(a, b);
+        v[1] = cadd(a, rb);
+        v[3] = csub(a, rb);
+    } else {
+        C2<T> e[4] = {v[0], v[2], v[0], v[0]}, o[4] = {v[1], v[3], v[1], v[1]};
+        dft_halfin<T, DIR, 4>(e);
+        dft_halfin<T, DIR, 4>(o);
+        const T h = (T)0.70710678118654752440084436210485;
+        C2<T> w1 = DIR < 0 ? C2<T>{h, -h} : C2<T>{h, h};
+        C2<T> w3 = DIR < 0 ? C2<T>{-h, -h} : C2<T>{-h, h};
+        o[1] = cmul(o[1], w1);
+        o[2] = rot<T, DIR>(o[2]);
+        o[3] = cmul(o[3], w3);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = cadd(e[k], o[k]);
+            v[k + 4] = csub(e[k], o[k]);
+        }
+    }
+}
+
+// butterfly K of Stockham stage S of a length-L register FFT, a -> b; HIN:
+// the inputs a[L/2 ..] are zero (stage 0 then runs half-input butterflies)
+template <typename T, int DIR, int L, int S, int K, bool HIN>
+__device__ __forceinline__ void cx_butterflies(const C2<T> (&a)[L], C2<T> (&b)[L])
+{
+    constexpr int R = RegPlan<L, S>::r, Ns = RegPlan<L, S>::ns, nb = L / R, stride = L / (Ns * R);
+    if constexpr (K < nb) {
+        constexpr int kk = K % Ns;
+        C2<T> v[R];
+        if constexpr (HIN && S == 0 && R % 2 == 0) {
+#pragma unroll
+            for (int r = 0; r < R / 2; r++) v[r] = a[K + r * nb];
+            dft_halfin<T, DIR, R>(v);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = a[K + r * nb];
+            cx_tw_apply<T, DIR, L, R, kk * stride>(v);
+            dftR<T, DIR, R>(v);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) b[(K - kk) * R + kk + r * Ns] = v[r];
+        cx_butterflies<T, DIR, L, S, K + 1, HIN>(a, b);
+    }
+}
+
+// Stockham stages of a length-L register FFT with compile-time twiddles
+// (the schedule of RegPlan: radix 8 first, then 4, 2, 3, 5); HIN: a[L/2 ..]
+// are zero on entry (L even, so the first radix is even)
+template <typename T, int DIR, int L, int S, bool HIN = false>
+__device__ __forceinline__ void cx_stages(C2<T> (&a)[L])
+{
+    if constexpr (RegPlan<L, S>::r > 1) {
+        C2<T> b[L];
+        cx_butterflies<T, DIR, L, S, 0, HIN>(a, b);
+#pragma unroll
+        for (int j = 0; j < L; j++) a[j] = b[j];
+        cx_stages<T, DIR, L, S + 1, false>(a);
+    }
+}
+
+// W_L^(sel E) (DIR < 0) or its conjugate for a lane value 0 <= sel < 2^B:
+// the product over the set bits of sel of the constants W_L^(2^i E), each
+// factor applied by a select between computed values (a select among
+// constants becomes a lookup table in scratch memory).  At most B - 1
+// roundings beyond the constants' own.
+template <typename T, int DIR, int L, int E, int B, int I = 0>
+__device__ __forceinline__ C2<T> tw_lane(int sel, C2<T> w = C2<T>{(T)1, (T)0})
+{
+    if constexpr (I < B) {
+        const C2<T> c = cxc<T, DIR, L, (1 << I) * E>();
+        const C2<T> f = I == 0 ? c : cmul(w, c);
+        w = ((sel >> I) & 1) ? f : w;
+        return tw_lane<T, DIR, L, E, B, I + 1>(sel, w);
+    } else {
+        return w;
+    }
+}
+
+// a[k2] *= W_L^(sel k2) (or the conjugate), k2 = 1 .. M-1
+template <typename T, int DIR, int L, int M, int B, int K2 = 1>
+__device__ __forceinline__ void lane_twiddle(C2<T> (&a)[M], int sel)
+{
+    if constexpr (K2 < M) {
+        a[K2] = cmul(a[K2], tw_lane<T, DIR, L, K2, B>(sel));
+        lane_twiddle<T, DIR, L, M, B, K2 + 1>(a, sel);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ int xlane_i(int v)
+{
+    if constexpr (D == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+    else if constexpr (D == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+    else return __builtin_amdgcn_ds_swizzle(v, (D << 10) | 0x1F);  // bit mode: lane ^ D inside 32 lanes
+}
+template <int D>
+__device__ __forceinline__ float xlane(float v) { return __int_as_float(xlane_i<D>(__float_as_int(v))); }
+template <int D>
+__device__ __forceinline__ double xlane(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)xlane_i<D>((int)(unsigned)(b & 0xffffffffull));
+    const unsigned hi = (unsigned)xlane_i<D>((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename T, int D>
+__device__ __forceinline__ C2<T> xlane2(C2<T> v) { return {xlane<D>(v.x), xlane<D>(v.y)}; }
+
+template <int H>
+constexpr int log2c() { return H <= 1 ? 0 : 1 + log2c<H / 2>(); }
+
+template <typename T>
+__device__ __forceinline__ T fma_t(T a, T b, T c) { return fma(a, b, c); }
+template <>
+__device__ __forceinline__ float fma_t<float>(float a, float b, float c) { return fmaf(a, b, c); }
+
+// p + sg * a with sg = +-1: one rounding, the same as p + a or p - a
+template <typename T>
+__device__ __forceinline__ C2<T> padd(C2<T> p, T sg, C2<T> a) { return {fma_t(sg, a.x, p.x), fma_t(sg, a.y, p.y)}; }
+
+// forward exchanges, lane distance H = Q/2, Q/4, .., 1: the lower lane of a
+// pair keeps the sum, the upper one the difference times W_(2H)^(q mod H)
+// (1 for H = 1, 1 or -i for H = 2)
+template <typename T, int M, int H>
+__device__ __forceinline__ void xl_fwd(C2<T> (&a)[M], int q)
+{
+    if constexpr (H >= 1) {
+        const bool up = (q & H) != 0;
+        const T sg = up ? (T)-1 : (T)1;
+        if constexpr (H == 1) {
+#pragma unroll
+            for (int m = 0; m < M; m++) a[m] = padd(xlane2<T, H>(a[m]), sg, a[m]);
+        } else if constexpr (H == 2) {
+            const bool mi = up && (q & 1);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const C2<T> t = padd(xlane2<T, H>(a[m]), sg, a[m]);
+                a[m] = mi ? C2<T>{t.y, -t.x} : t;
+            }
+        } else {
+            const C2<T> w0 = tw_lane<T, -1, 2 * H, 1, log2c<H>()>(q & (H - 1));
+            const C2<T> w = up ? w0 : C2<T>{(T)1, (T)0};
+#pragma unroll
+            for (int m = 0; m < M; m++) a[m] = cmul(padd(xlane2<T, H>(a[m]), sg, a[m]), w);
+        }
+        xl_fwd<T, M, H / 2>(a, q);
+    }
+}
+
+// inverse exchanges, lane distance H = 1, 2, .., Q/2: the upper lane first
+// takes its value times conj(W_(2H)^(q mod H)) (1 for H = 1, 1 or +i for
+// H = 2), then the lower lane keeps the sum, the upper one the difference
+template <typename T, int M, int Q, int H>
+__device__ __forceinline__ void xl_inv(C2<T> (&a)[M], int q)
+{
+    if constexpr (H < Q) {
+        const bool up = (q & H) != 0;
+        const T sg = up ? (T)-1 : (T)1;
+        if constexpr (H == 1) {
+#pragma unroll
+            for (int m = 0; m < M; m++) a[m] = padd(xlane2<T, H>(a[m]), sg, a[m]);
+        } else if constexpr (H == 2) {
+            const bool pi = up && (q & 1);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const C2<T> t = pi ? C2<T>{-a[m].y, a[m].x} : a[m];
+                a[m] = padd(xlane2<T, H>(t), sg, t);
+            }
+        } else {
+            const C2<T> w0 = tw_lane<T, 1, 2 * H, 1, log2c<H>()>(q & (H - 1));
+            const C2<T> w = up ? w0 : C2<T>{(T)1, (T)0};
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const C2<T> t = cmul(a[m], w);
+                a[m] = padd(xlane2<T, H>(t), sg, t);
+            }
+        }
+        xl_inv<T, M, Q, 2 * H>(a, q);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ C2<T> ld_at(const char *base, uint32_t off) { return *(const C2<T> *)(base + off); }
+template <typename T>
+__device__ __forceinline__ void st_at(char *base, uint32_t off, C2<T> v) { *(C2<T> *)(base + off) = v; }
+
+// Lines: b = tile * 256/Q + lane / Q along the tiled axis (< nb), l2 =
+// blockIdx.y, right-hand sides blockIdx.z * rpb ..; src and dst may alias
+// (every lane loads its line before any lane of it stores).  The host
+// checks that the line offsets fit the 32-bit lane offsets.
+template <typename T, int MODE, int L, int Q>
+__global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<T> *dst)
+{
+    constexpr int M = L / Q, LPB = 256 / Q, LQ = log2c<Q>();
+    // forward inputs and inverse outputs live in the lower half of the line
+    // (nin, nout <= L/2, checked on the host): with M even that is m < M/2
+    // in every lane, so the zero half is never loaded or transformed and
+    // the unused half of the inverse is never formed
+    constexpr bool HALF = M % 2 == 0;
+    constexpr bool CONV = MODE >= 2;
+    constexpr uint32_t ES = sizeof(C2<T>);
+    unsigned bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int q = (int)threadIdx.x % Q;
+    const int lb = (int)threadIdx.x / Q;
+    const int b0 = (int)bx * LPB;
+    const bool live = b0 + lb < d.nb;
+    const int l2 = (int)by;
+    const int rq = (int)(__brev((unsigned)q) >> (32 - LQ));
+    // 32-bit lane offsets (bytes) of element 0 of this lane's stride pattern
+    const uint32_t in_nat = (uint32_t)(lb * d.in_sb + (int64_t)q * d.in_sj) * ES;
+    const uint32_t in_rev = (uint32_t)(lb * d.in_sb + (int64_t)M * rq * d.in_sj) * ES;
+    const uint32_t out_nat = (uint32_t)(lb * d.out_sb + (int64_t)q * d.out_sj) * ES;
+    const uint32_t out_rev = (uint32_t)(lb * d.out_sb + (int64_t)M * rq * d.out_sj) * ES;
+    T sr[CONV ? M : 1], si[MODE == 3 ? M : 1];
+    if (CONV) {
+        const int64_t sbase = (int64_t)(b0 + lb) * d.spec_sb + (int64_t)l2 * d.spec_s2 + (int64_t)M * rq * d.spec_sj;
+#pragma unroll
+        for (int k2 = 0; k2 < M; k2++) {
+            const int64_t s = sbase + (int64_t)k2 * d.spec_sj;
+            if (MODE == 2) {
+                sr[k2] = live ? reinterpret_cast<const T *>(d.spec)[s] : (T)0;
+            } else {
+                const C2<T> c = live ? reinterpret_cast<const C2<T> *>(d.spec)[s] : C2<T>{(T)0, (T)0};
+                sr[k2] = c.x;
+                si[MODE == 3 ? k2 : 0] = d.spec_conj ? -c.y : c.y;
+            }
+        }
+    }
+    const int r_end = min(d.nrhs, ((int)bz + 1) * d.rpb);
+    for (int rhs = (int)bz * d.rpb; rhs < r_end; rhs++) {
+        // the lane's index is laundered so that its twiddles are formed here,
+        // not held in registers across the loop
+        int qv = q;
+        asm volatile("" : "+v"(qv));
+        const char *in = (const char *)(src + (int64_t)b0 * d.in_sb + (int64_t)l2 * d.in_s2 + (int64_t)rhs * d.in_rhs);
+        char *out = (char *)(dst + (int64_t)b0 * d.out_sb + (int64_t)l2 * d.out_s2 + (int64_t)rhs * d.out_rhs);
+        C2<T> a[M];
+        if (MODE == 1) {
+#pragma unroll
+            for (int k2 = 0; k2 < M; k2++) {
+                const int k = k2 + M * rq;
+                a[k2] = live && k < d.nin ? ld_at<T>(in + (int64_t)k2 * d.in_sj * ES, in_rev) : C2<T>{(T)0, (T)0};
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < (HALF ? M / 2 : M); m++) {
+                const int n = q + Q * m;
+                a[m] = live && n < d.nin ? ld_at<T>(in + (int64_t)Q * m * d.in_sj * ES, in_nat) : C2<T>{(T)0, (T)0};
+            }
+            cx_stages<T, -1, M, 0, HALF>(a);
+            lane_twiddle<T, -1, L, M, LQ>(a, qv);
+            xl_fwd<T, M, Q / 2>(a, qv);
+            if (MODE == 0) {
+                if (live) {
+#pragma unroll
+                    for (int k2 = 0; k2 < M; k2++)
+                        if (k2 + M * rq < d.nout) st_at<T>(out + (int64_t)k2 * d.out_sj * ES, out_rev, a[k2]);
+                }
+                continue;
+            }
+#pragma unroll
+            for (int k2 = 0; k2 < M; k2++) {
+                if (MODE == 2) a[k2] = cscale(a[k2], sr[k2]);
+                else a[k2] = cmul(a[k2], C2<T>{sr[k2], si[MODE == 3 ? k2 : 0]});
+            }
+        }
+        xl_inv<T, M, Q, 1>(a, qv);
+        lane_twiddle<T, 1, L, M, LQ>(a, qv);
+        cx_stages<T, 1, M, 0>(a);
+        if (live) {
+#pragma unroll
+            for (int m = 0; m < (HALF ? M / 2 : M); m++)
+                if (q + Q * m < d.nout) st_at<T>(out + (int64_t)Q * m * d.out_sj * ES, out_nat, a[m]);
+        }
+    }
+}
+
+}  // namespace
+
+// (L, Q) pairs k_lines is built for: z convolutions (MODE 2, 3) and y
+// passes (MODE 0, 1) of the workloads' lengths, at most 32 points per lane.
+#define PEEC_LANES_Z(X) X(16, 4) X(32, 4) X(32, 8) X(48, 4) X(48, 8) X(64, 4) X(64, 8) X(64, 16) X(96, 8) \
+    X(96, 16) X(128, 8) X(128, 16)
+#define PEEC_LANES_Y(X) X(128, 8) X(128, 16) X(192, 8) X(192, 16) X(200, 8) X(384, 16) X(384, 32) X(400, 16)
+
+bool lane_built(int L, int Q, int mode)
+{
+    bool ok = false;
+#define PEEC_LANE_HAS(L_, Q_) ok = ok || (L == L_ && Q == Q_);
+    if (mode >= 2) { PEEC_LANES_Z(PEEC_LANE_HAS) }
+    else { PEEC_LANES_Y(PEEC_LANE_HAS) }
+#undef PEEC_LANE_HAS
+    return ok;
+}
+
+// the lane kernel applies to a pass whose tile axis is contiguous and whose
+// lines are strided, with the forward input (modes 0, 2, 3) and the inverse
+// output (modes 1, 2, 3) in the lower half of the line
+bool lane_fits(const PassDesc &d, int Q, int mode)
+{
+    // the kernel's lane offsets are 32-bit byte offsets within L line elements
+    const int64_t span = ((int64_t)d.L * std::max(d.in_sj, d.out_sj) + 256) * 16;
+    return Q >= 2 && d.L % Q == 0 && lane_built(d.L, Q, mode) && d.in_sb == 1 && d.out_sb == 1 && d.in_sj > 1 &&
+           d.out_sj > 1 && d.nin <= d.L && d.nout <= d.L && span < ((int64_t)1 << 31) &&
+           (mode == 1 || 2 * d.nin <= d.L) && (mode == 0 || 2 * d.nout <= d.L);
+}
+
+template <typename T, int MODE>
+static int launch_lines_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst, int Q)
+{
+    const int rpb = std::max(1, d.rpb), lpb = 256 / Q;
+    dim3 grid((unsigned)((d.nb + lpb - 1) / lpb), (unsigned)d.n2, (unsigned)((d.nrhs + rpb - 1) / rpb));
+    const C2<T> *s = (const C2<T> *)src;
+    C2<T> *o = (C2<T> *)dst;
+    bool done = false;
+#define PEEC_LANE_LAUNCH(L_, Q_)                                                                                   \
+    if (!done && d.L == L_ && Q == Q_) {                                                                           \
+        hipLaunchKernelGGL((k_lines<T, MODE, L_, Q_>), grid, dim3(256), 0, ctx->stream, d, s, o);              \
+        done = true;                                                                                               \
+    }
+    if constexpr (MODE >= 2) { PEEC_LANES_Z(PEEC_LANE_LAUNCH) }
+    else { PEEC_LANES_Y(PEEC_LANE_LAUNCH) }
+#undef PEEC_LANE_LAUNCH
+    if (!done) {
+        peec_set_error("lane-split FFT pass: no kernel for length %d with %d lanes per line", d.L, Q);
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipGetLastError());
+    return PEEC_OK;
+}
+
+
+int launch_lane_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const void *src, void *dst, int Q)
+{
+    if (fp32) {
+        if (mode == 0) return launch_lines_t<float, 0>(ctx, d, src, dst, Q);
+        if (mode == 1) return launch_lines_t<float, 1>(ctx, d, src, dst, Q);
+        if (mode == 2) return launch_lines_t<float, 2>(ctx, d, src, dst, Q);
+        return launch_lines_t<float, 3>(ctx, d, src, dst, Q);
+    }
+    if (mode == 0) return launch_lines_t<double, 0>(ctx, d, src, dst, Q);
+    if (mode == 1) return launch_lines_t<double, 1>(ctx, d, src, dst, Q);
+    if (mode == 2) return launch_lines_t<double, 2>(ctx, d, src, dst, Q);
+    return launch_lines_t<double, 3>(ctx, d, src, dst, Q);
+}

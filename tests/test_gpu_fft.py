@@ -1,6 +1,6 @@
 """FFT-convolution launch shapes, the fused y-z pass and the autotuner.
 
-Every launch shape of the pass kernel (lines per tile, lanes per line,
+Every launch shape of the LDS pass kernel (lines per tile, lanes per line,
 right-hand sides per block, hence also the prefetching of the next
 right-hand side) runs the same butterflies in the same order,
 so K x must come out bit for bit the same under each of them, in complex128
@@ -10,7 +10,10 @@ tuner may choose, runs the same stages again: it must equal the five-pass
 path bit for bit, also with the 48- and 64-point z kernels.  The complex64
 radix schedules change rounding only: they stay within complex64 accuracy
 of the complex128 result.  The complex128 result itself is held to the
-oracle's K x (1e-12, relative).
+oracle's K x (1e-12, relative).  The lane-split passes (a line spread over
+Q lanes, cross-lane butterflies) compute the same transforms in another
+operation order: complex128 within 1e-13 of the LDS passes and 1e-12 of
+the oracle, complex64 within 1e-5 of complex128.
 """
 import numpy as np
 import pytest
@@ -24,6 +27,7 @@ pytestmark = pytest.mark.gpu
 
 NRHS = 3
 N_VARIANTS = 12  # 11 LDS tile shapes + the register z kernel (fused z pass only)
+LANE_BASE, N_LANE = 12, 5  # lane-split passes with 2, 4, .. 32 lanes per line (y and z passes)
 PASS_YZ = 6      # tuning-table pass of the fused y-z pass: variant 0 fused, 1 five passes
 
 
@@ -135,8 +139,9 @@ def test_fp32_radix_schedules_and_tuner(batched):
     for k in range(9):
         print("kernel %d fp64" % k, [peec.fft_choice(k, p, False, NRHS, ctx=S.ctx) for p in range(5)],
               "fp32", [peec.fft_choice(k, p, True, NRHS, ctx=S.ctx) for p in range(5)], flush=True)
+    # the tuner may pick lane-split passes, whose butterflies round differently
     y64t = peec.apply_K_batched(X, 1, fp32=False, ctx=S.ctx)
-    assert bitwise_equal(y64t, y64)
+    assert rel(y64t, y64) < 1e-13
     y32t = peec.apply_K_batched(X, 1, fp32=True, ctx=S.ctx)
     assert rel(y32t, y64) < 1e-5
 
@@ -204,3 +209,79 @@ def test_fused_yz_long_z_lines():
     print("tuned %.3f ms; fused choice per kernel %s" % (ms, fused_on), flush=True)
     # the tuner may pick other complex64 radix schedules: rounding changes
     assert rel(peec.apply_K_batched(X, rps, fp32=True, ctx=S.ctx), y64) < 1e-5
+
+
+@pytest.fixture(scope="module")
+def lanes_geometry():
+    """20 x 64 x 24: 128-point y lines and 48-/64-point z lines, the lengths
+    the lane-split kernels are built for."""
+    cell = (0.25e-3, 0.25e-3, 0.05e-3)
+    g, mats, _ = configs._layered(20, 64, 24, cell, 18, 0.45, 4.4, 0.02, seed=7)
+    ports = configs._x_face_ports(g, cell, 18)
+    M = mesher.mesh(g, mats, cell, (0.0, 0.0, 0.0), ports, p_self_fn=pself)
+    S = sweep.Solver(M, ctx=peec.Context(0), device_lu=True)
+    ws = [2 * np.pi * f * M.escalings["freq"] for f in (3e7, 8e9)]
+    peec.frequency_slots(2, ctx=S.ctx)
+    for slot, w in enumerate(ws):
+        Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+        peec.frequency_set_slot(slot, w, sweep.DZ_of(M, w), Yle, ctx=S.ctx)
+    N = M.m + M.ns + M.n
+    rng = np.random.default_rng(19)
+    X = rng.normal(size=(N, 4)) + 1j * rng.normal(size=(N, 4))
+    return M, S, ws, X
+
+
+def set_lanes(S, fp32, Q, rpb, nrhs):
+    """Force the lane-split kernel with Q lanes per line on the y and z
+    passes of every kernel where it is built; returns (passes taken,
+    z passes taken)."""
+    took = tz = 0
+    v = LANE_BASE + int(np.log2(Q)) - 1
+    for k in range(9):
+        if fp32:
+            set_fused(S, k, False, nrhs=nrhs)
+        for p in (1, 2, 3):
+            try:
+                peec.fft_set_choice(k, p, fp32, nrhs, v, rpb, 0, ctx=S.ctx)
+                took += 1
+                tz += p == 2
+            except _lib.PeecError:
+                peec.fft_set_choice(k, p, fp32, nrhs, -1, ctx=S.ctx)
+    return took, tz
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_lane_split_passes_match(lanes_geometry, fp32):
+    M, S, ws, X = lanes_geometry
+    nrhs, rps = X.shape[1], 2
+    clear_all(S, nrhs)
+    y64 = peec.apply_K_batched(X, rps, fp32=False, ctx=S.ctx)
+    ref = peec.apply_K_batched(X, rps, fp32=fp32, ctx=S.ctx)
+    total = 0
+    for Q in (4, 8, 16):
+        for rpb in (1, 3):
+            took, tz = set_lanes(S, fp32, Q, rpb, nrhs)
+            if took == 0:
+                continue
+            total += took
+            Y = peec.apply_K_batched(X, rps, fp32=fp32, ctx=S.ctx)
+            e, e64 = rel(Y, ref), rel(Y, y64)
+            print("fp32=%s Q %d rpb %d: lane passes on %d passes (%d z); vs LDS passes %.2e, vs complex128 %.2e"
+                  % (fp32, Q, rpb, took, tz, e, e64), flush=True)
+            assert e64 < (1e-5 if fp32 else 1e-13)
+        clear_all(S, nrhs)
+    assert total > 0
+    if not fp32:
+        rows = OF.kernel_rows(M.origin, M.Nx, M.Ny, M.Nz, M.sx, M.sy, M.sz)
+        spec = OF.spectra(rows, M.Nx, M.Ny, M.Nz)
+        set_lanes(S, False, 8, 1, nrhs)
+        Y = peec.apply_K_batched(X, rps, fp32=False, ctx=S.ctx)
+        for r in range(nrhs):
+            w = ws[r // rps]
+            op = OF.Operator(M, spec)
+            Yle = OF.build_Yle(M.lumped_elements, M.port_nodes, M.escalings["Yle"], M.n, w / M.escalings["freq"], 50.0)
+            op.set_frequency(w, OF.frequency_terms(M, w), Yle)
+            e = rel(Y[:, r], np.concatenate(op.K(X[:, r])))
+            print("lane passes Q 8, column %d: K x vs oracle %.2e" % (r, e), flush=True)
+            assert e < 1e-12
+        clear_all(S, nrhs)

@@ -3,9 +3,11 @@
 
 64x64x16 two-material geometry, plain FP64 path: the reference's sweep
 (warm start, gmres_custom, tol 1e-10) at 1e7, 3.0e8 and 1e10 Hz.  The GPU
-runs the same Krylov iteration as the oracle, so flags and iteration counts
-must be equal, and Z agrees with the oracle's Z to rounding amplified by
-the conditioning of SS (the bar is stated in the test).
+runs the same Krylov iteration as the oracle: the same flags everywhere and
+the same iteration counts where the solve converges; Z agrees with the
+oracle's Z to rounding amplified by the conditioning of SS (the bars are
+stated in the test).  At 1e7 Hz both stop on gmres_custom's stagnation test
+a little above the tolerance, at a step that depends on rounding.
 
 96x96x24 batched geometry (8 ports): the oracle solved to a 1e-12 residual
 at 1e6, 1.0e8 and 2e10 Hz.  The batched mixed-precision path (complex64
@@ -61,15 +63,29 @@ def test_two_material_plain_sweep_matches_oracle():
         out = sweep.FFT_solver_QS_S_type(fx["freqs"], M, int(fx["inner"]), float(fx["tol"]), solver=S, record=rec)
     for r in rec:
         k, c = r["k"], r["port"]
-        print("  f=%.4g port %d: flag %d / %d, iterations %d / %d (GPU / oracle)"
-              % (fx["freqs"][k], c, r["flag"], fx["flag"][c, k], r["iter"][1], fx["iters"][c, k]), flush=True)
+        print("  f=%.4g port %d: flag %d / %d, iterations %d / %d, relres %.2e / %.2e (GPU / oracle)"
+              % (fx["freqs"][k], c, r["flag"], fx["flag"][c, k], r["iter"][1], fx["iters"][c, k], r["relres"],
+                 fx["relres"][c, k]), flush=True)
+    for k in range(len(fx["freqs"])):
+        d, rows = z_errors(out["Z"][:, :, k:k + 1], fx["Z"][:, :, k:k + 1])
+        print("  f=%.4g: Z vs oracle: self %.2e, entries %.2e" % (fx["freqs"][k], d, rows), flush=True)
+    for r in rec:
+        k, c = r["k"], r["port"]
         assert r["flag"] == fx["flag"][c, k]
-        assert r["iter"][1] == fx["iters"][c, k]
-    d, rows = z_errors(out["Z"], fx["Z"])
-    print("  Z vs oracle: self %.2e, entries %.2e" % (d, rows), flush=True)
-    # same iteration; the trajectories differ by FFT and LU rounding,
-    # amplified by cond(SS) ~ 1e9 at 10 MHz
+        # a converged solve (flag 0) takes the oracle's iteration; where both
+        # stop on stagnation (flag 3, gmres_custom.jl:171-241, at 10 MHz) the
+        # step it is detected at depends on rounding, and is parity unpinned
+        if r["flag"] == 0:
+            assert r["iter"][1] == fx["iters"][c, k]
+    conv = [k for k in range(len(fx["freqs"])) if np.all(fx["flag"][:, k] == 0)]
+    stag = [k for k in range(len(fx["freqs"])) if k not in conv]
+    # converged frequencies: the same Krylov iteration, rounding amplified by
+    # cond(SS); stagnated ones: both stop near a 2e-10 residual
+    d, rows = z_errors(out["Z"][:, :, conv], fx["Z"][:, :, conv])
     assert d <= 1e-10 and rows <= 1e-10
+    if stag:
+        d, rows = z_errors(out["Z"][:, :, stag], fx["Z"][:, :, stag])
+        assert d <= 1e-8 and rows <= 1e-8
 
 
 @pytest.fixture(scope="module")
