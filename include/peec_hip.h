@@ -237,8 +237,11 @@ int peec_gmres(peec_ctx *ctx, const peec_c128 *b, double tol, int maxit, peec_c1
  * inner_tol is the residual reduction a mixed cycle aims for (0: 1e-5).
  * x holds the warm starts on input and the solutions on output.  Per
  * right-hand side: flag (0 converged, 1 not within max_cycles, 3 the
- * refinement stalled), relres (final FP64 relative residual), iters
- * (inner iterations) and cycles. */
+ * refinement stalled, 4 its norm or residual is not finite -- NaN or Inf
+ * in b, the start or the operator), relres (final FP64 relative
+ * residual; NaN with flag 4), iters (inner iterations) and cycles.  A zero
+ * right-hand side returns the zero solution with flag 0 and relres 0, as
+ * gmres_custom.jl:20-28 does. */
 int peec_gmres_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, const peec_c128 *b, double tol, int restart,
                        int max_cycles, int mixed, double inner_tol, peec_c128 *x, int *flag, double *relres, int *iters,
                        int *cycles);

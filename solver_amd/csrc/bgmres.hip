@@ -216,6 +216,13 @@ __global__ void k_port_volts(const cd *X, int64_t N, int64_t off, const int64_t 
     v[i] = {a.x - b.x, a.y - b.y};
 }
 
+// x[0 .. n) = 0 (the solution of a zero right-hand side)
+__global__ void k_bzero(cd *x, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = {0.0, 0.0};
+}
+
 __global__ void k_bsub(cd *r, const cd *b, const cd *y, int64_t total)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -370,6 +377,11 @@ extern "C" int peec_gmres_batched_adjoint(peec_ctx *ctx, int nrhs, int rhs_per_s
                               relres, iters, cycles);
 }
 
+// Flag of a right-hand side whose norm or residual is not finite (the
+// batched solve's analogue of MATLAB gmres flag 4: a scalar quantity became
+// too small or too large to continue).
+constexpr int kFlagNonFinite = 4;
+
 // Per right-hand side results of a batched solve.
 struct BOut {
     std::vector<int> fl, it, cyc;
@@ -378,7 +390,8 @@ struct BOut {
     {
         for (int r = 0; r < nrhs; r++) {
             if (flag) flag[r] = fl[(size_t)r];
-            if (relres) relres[r] = nb[(size_t)r] > 0 ? beta[(size_t)r] / nb[(size_t)r] : 0.0;
+            const double b = nb[(size_t)r];
+            if (relres) relres[r] = b > 0 && std::isfinite(b) ? beta[(size_t)r] / b : b == 0.0 ? 0.0 : NAN;
             if (iters) iters[r] = it[(size_t)r];
             if (cycles) cycles[r] = cyc[(size_t)r];
         }
@@ -416,9 +429,25 @@ static int bgmres_solve(peec_ctx *ctx, BWork &W, double tol, int max_cycles, int
     std::vector<char> active((size_t)nrhs);
     std::vector<int> it((size_t)nrhs, 0), cyc((size_t)nrhs, 0), fl((size_t)nrhs, 1), stall((size_t)nrhs, 0);
     for (int r = 0; r < nrhs; r++) {
-        active[(size_t)r] = nb[(size_t)r] > 0.0 && beta[(size_t)r] > tol * nb[(size_t)r];
-        if (!active[(size_t)r]) fl[(size_t)r] = 0;
+        const double b = nb[(size_t)r], be = beta[(size_t)r];
+        if (!std::isfinite(b) || !std::isfinite(be)) {
+            // a non-finite right-hand side or start residual: flag 4 (a
+            // quantity became too large or too small to continue), not a
+            // convergence
+            active[(size_t)r] = 0;
+            fl[(size_t)r] = kFlagNonFinite;
+        } else if (b == 0.0) {
+            // a zero right-hand side has the zero solution (gmres_custom.jl:20-28)
+            hipLaunchKernelGGL(k_bzero, dim3(nb256(N)), dim3(256), 0, ctx->stream, W.X.p + (int64_t)r * N, N);
+            active[(size_t)r] = 0;
+            fl[(size_t)r] = 0;
+            beta[(size_t)r] = 0.0;
+        } else {
+            active[(size_t)r] = be > tol * b;
+            if (!active[(size_t)r]) fl[(size_t)r] = 0;
+        }
     }
+    PEEC_HIP_TRY(hipGetLastError());
     const int m = restart, hs = 2 * (m + 1);
     std::vector<cd> hh((size_t)nrhs * hs);
     std::vector<double> hn;
@@ -548,7 +577,10 @@ static int bgmres_solve(peec_ctx *ctx, BWork &W, double tol, int max_cycles, int
         for (int r = 0; r < nrhs; r++) {
             if (!active[(size_t)r]) continue;
             cyc[(size_t)r]++;
-            if (beta[(size_t)r] <= tol * nb[(size_t)r]) {
+            if (!std::isfinite(beta[(size_t)r])) {
+                active[(size_t)r] = 0;
+                fl[(size_t)r] = kFlagNonFinite;
+            } else if (beta[(size_t)r] <= tol * nb[(size_t)r]) {
                 active[(size_t)r] = 0;
                 fl[(size_t)r] = 0;
             } else if (beta[(size_t)r] > 0.5 * beta_old[(size_t)r]) {

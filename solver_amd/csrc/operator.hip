@@ -979,23 +979,41 @@ int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, const ColMa
 }
 
 // y = K x (before the preconditioner), device vectors
+// complex128 elements of each FFT work grid that nrhs right-hand sides need
+static size_t work_grid_elems(const peec_ctx *ctx, int nrhs)
+{
+    size_t need = 0;
+    for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx->apps[a]));
+    return need * (size_t)nrhs;
+}
+
+// K x with the two FFT work grids w1, w2 given (work_grid_elems each);
+// launches only, on the context's stream
+static int apply_K_work(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32,
+                        const ColMap &cm, double2_t *w1, double2_t *w2);
+
 int apply_K(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32, const ColMap &cm)
 {
     if (nrhs <= 0) return PEEC_OK;
-    const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n, N = m + ns + n;
-    hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
-    PEEC_HIP_TRY(hipGetLastError());
-    size_t need = 0;
-    for (int a = 0; a < ctx->napps; a++) need = std::max<size_t>(need, (size_t)conv_work_elems(ctx->apps[a]));
-    need *= (size_t)nrhs;
+    const size_t need = work_grid_elems(ctx, nrhs);
     if (need > ctx->w1.n) {
         if (!ctx->w1.alloc(need) || !ctx->w2.alloc(need)) {
             peec_set_error("out of device memory for %d-rhs FFT work grids", nrhs);
             return PEEC_ERR_NOMEM;
         }
     }
+    return apply_K_work(ctx, x, y, nrhs, ld, fp32, cm, ctx->w1.p, ctx->w2.p);
+}
+
+static int apply_K_work(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrhs, int64_t ld, bool fp32,
+                        const ColMap &cm, double2_t *w1, double2_t *w2)
+{
+    if (nrhs <= 0) return PEEC_OK;
+    const int64_t m = ctx->m, ns = ctx->ns, n = ctx->n, N = m + ns + n;
+    hipLaunchKernelGGL(k_zero, dim3(nblk(m + ns), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, m + ns, nrhs, ld);
+    PEEC_HIP_TRY(hipGetLastError());
     for (int a = 0; a < ctx->napps; a++) {
-        int rc = conv_apply(ctx, ctx->apps[a], fp32, x, y, nrhs, ld, ctx->w1.p, ctx->w2.p);
+        int rc = conv_apply(ctx, ctx->apps[a], fp32, x, y, nrhs, ld, w1, w2);
         if (rc) return rc;
     }
     const int64_t phi0 = m + ns;
@@ -1258,6 +1276,7 @@ extern "C" int peec_apply_KH_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, 
 {
     return apply_batched_impl(ctx, nrhs, rhs_per_slot, fp32, precond, true, x, y);
 }
+
 
 static int apply_batched_impl(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, int precond, bool adjoint,
                               const peec_c128 *x, peec_c128 *y)

@@ -20,6 +20,7 @@ import argparse
 import json
 import sys
 import time
+from types import SimpleNamespace
 
 import numpy as np
 
@@ -35,6 +36,13 @@ def main(argv=None):
     ap.add_argument("--fd-path", choices=["plain", "batched"], default="batched")
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
+    print(json.dumps(run(a.nfreq, a.nvox, a.tol, a.step, a.fd_path, a.seed)), flush=True)
+
+
+def run(nfreq=32, nvox=16, tol=1e-12, step=1e-4, fd_path="batched", seed=0, ctx=None):
+    """The check above; returns its record (timings, agreements, and per
+    checked voxel the adjoint and central-difference derivatives)."""
+    a = SimpleNamespace(nfreq=nfreq, nvox=nvox, tol=tol, step=step, fd_path=fd_path, seed=seed)
 
     def log(*x):
         print(*x, file=sys.stderr, flush=True)
@@ -51,7 +59,7 @@ def main(argv=None):
     log("adjoint workload: N=%d, %d conductor voxels, %d frequencies" % (M.m + M.ns + M.n,
                                                                         int(np.sum(M.sigma_vox > 0)), len(fs)))
     t0 = time.perf_counter()
-    S = sweep.Solver(M, device_lu=True)
+    S = sweep.Solver(M, ctx=ctx, device_lu=True)
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     adj = sweep.impedance_objective_gradient(fs, M, port=0, tol=a.tol, solver=S, method="adjoint")
@@ -91,12 +99,12 @@ def main(argv=None):
         log("voxel %6d: adjoint %+.6e  FD %+.6e  rel %.2e  of max %.2e" % (v, g[v], fd, rows[-1]["err_rel"],
                                                                           rows[-1]["err_of_max"]))
     t_fd = time.perf_counter() - t0
-    print(json.dumps(dict(workload=cfg["name"], unknowns=int(M.m + M.ns + M.n), frequencies=len(fs),
-                          conductor_voxels=int(cond.size), J=adj["J"], setup_s=t_setup, adjoint_gradient_s=t_adj,
-                          reciprocity_gradient_s=t_rec, adjoint_vs_reciprocity=e_rec, plain_gradient_s=t_pla,
-                          adjoint_vs_plain=e_pla, J_plain=pla["J"], fd_path=a.fd_path,
-                          fd_step=a.step, fd_s=t_fd, fd=rows,
-                          fd_max_err_of_max=max(r["err_of_max"] for r in rows) if rows else None)), flush=True)
+    return dict(workload=cfg["name"], unknowns=int(M.m + M.ns + M.n), frequencies=len(fs),
+                conductor_voxels=int(cond.size), J=adj["J"], setup_s=t_setup, adjoint_gradient_s=t_adj,
+                reciprocity_gradient_s=t_rec, adjoint_vs_reciprocity=e_rec, plain_gradient_s=t_pla,
+                adjoint_vs_plain=e_pla, J_plain=pla["J"], fd_path=a.fd_path,
+                fd_step=a.step, fd_s=t_fd, fd=rows, adjoint_flags=sorted(set(adj["adjoint_flags"])),
+                fd_max_err_of_max=max(r["err_of_max"] for r in rows) if rows else None)
 
 
 if __name__ == "__main__":

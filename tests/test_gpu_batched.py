@@ -149,3 +149,31 @@ def test_batched_lu_equals_per_slot():
     print("batched LU vs per-slot: bitwise %s (rel %.2e)" % (np.array_equal(one, together), rel(together, one)),
           flush=True)
     assert np.array_equal(one, together)
+
+
+def test_batched_gmres_nonfinite_and_zero_rhs():
+    """A NaN right-hand side comes back flagged 4 (not converged, relres
+    NaN) without touching the other columns; a zero right-hand side returns
+    the zero solution with flag 0 even from a non-zero start, as
+    gmres_custom does (gmres_custom.jl:20-28)."""
+    M = small_mesh()
+    S = sweep.Solver(M, device_lu=True)
+    w = 2 * np.pi * 1e9 * M.escalings["freq"]
+    peec.frequency_slots(1, ctx=S.ctx)
+    Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+    peec.frequency_set_slot(0, w, sweep.DZ_of(M, w), Yle, ctx=S.ctx)
+    peec.lu_factor_slot(0, ctx=S.ctx)
+    N = M.m + M.ns + M.n
+    rng = np.random.default_rng(9)
+    B = rng.normal(size=(N, 3)) + 1j * rng.normal(size=(N, 3))
+    B[5, 1] = np.nan
+    B[:, 2] = 0.0
+    X0 = np.zeros((N, 3), dtype=np.complex128)
+    X0[:, 2] = 1.0
+    ref, fl_ref, _, _, _ = peec.gmres_batched(B[:, :1], 3, 1e-10, None, restart=60, ctx=S.ctx)
+    X, flag, relres, iters, cycles = peec.gmres_batched(B, 3, 1e-10, X0, restart=60, ctx=S.ctx)
+    print("flags", flag, "relres", relres, "iters", iters, flush=True)
+    assert list(flag) == [0, 4, 0]
+    assert np.isnan(relres[1]) and relres[2] == 0.0 and relres[0] < 1e-10
+    assert np.all(X[:, 2] == 0)
+    assert rel(X[:, 0], ref[:, 0]) < 1e-12

@@ -166,3 +166,44 @@ def test_zero_or_nonfinite_pivot_is_an_error():
         peec.lu_factor_slots(0, 2, ctx=S.ctx)
     peec.lu_factor_slot(0, ctx=S.ctx)  # the good slot alone still factors
     S.ctx.close()
+
+
+def test_umfpack_convention_factors_with_pivoting_and_row_scaling():
+    """peec_lu_set_factors takes F = lu(SS) in UMFPACK's convention, which
+    Julia exposes as F.L, F.U, F.p, F.q, F.Rs: L * U = (Rs .* SS)[p, q].
+    Here SuperLU factors Rs .* SS with partial pivoting (p != q) and a
+    row scaling Rs far from 1; M^-1 x from those factors must match the
+    oracle's precond_3_3_vector (its own unpivoted LU) to rounding times
+    the conditioning of SS."""
+    from solver_amd import peec
+    M = small_mesh()
+    S = sweep.Solver(M, ctx=peec.Context(0), nested_dissection=False, device_lu=False)
+    for f in (3e7, 2e9):
+        w = 2 * np.pi * f * M.escalings["freq"]
+        Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+        DZ = sweep.DZ_of(M, w)
+        peec.frequency_set(w, DZ, Yle, ctx=S.ctx)
+        SS = sweep.SS_of(M, w, DZ, Yle).tocsc()
+        rng = np.random.default_rng(4)
+        Rs = 10.0 ** rng.uniform(-3, 3, SS.shape[0])
+        A = SS.multiply(Rs[:, None]).tocsc()
+        F = spla.splu(A, permc_spec="COLAMD", diag_pivot_thresh=1.0)
+        p, q = np.argsort(F.perm_r), np.argsort(F.perm_c)
+        assert not np.array_equal(p, q)
+        LU = (F.L @ F.U).toarray()
+        assert np.abs(LU - A.toarray()[np.ix_(p, q)]).max() <= 1e-12 * np.abs(A).max()
+        peec.lu_set_factors(F.L, F.U, p, q, Rs=Rs, ctx=S.ctx)
+        N = M.m + M.ns + M.n
+        x = rng.normal(size=N) + 1j * rng.normal(size=N)
+        got = peec.apply_K_batched(x[:, None], 1, precond=2, ctx=S.ctx)[:, 0]
+        op = OF.Operator(M, None)
+        op.set_frequency(w, OF.frequency_terms(M, w), OF.build_Yle(M.lumped_elements, M.port_nodes,
+                                                                  M.escalings["Yle"], M.n, w / M.escalings["freq"],
+                                                                  50.0))
+        ref = op.precond(x[:M.m], x[M.m:M.m + M.ns], x[M.m + M.ns:])
+        cond = np.linalg.cond(SS.toarray())
+        e = rel(got, ref)
+        print("f %.3g: M^-1 x with pivoted, row-scaled factors vs oracle %.2e (cond(SS) %.1e)" % (f, e, cond),
+              flush=True)
+        assert e <= max(1e-12, 10 * cond * np.finfo(float).eps)
+    S.ctx.close()
