@@ -310,6 +310,24 @@ int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs,
 int peec_apply_K_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, int precond, const peec_c128 *x,
                          peec_c128 *y);
 
+/* K x for nrhs device vectors (N x nrhs column-major, vector r against
+ * frequency slot r / rhs_per_slot, complex128 or (fp32 = 1) complex64 FFT
+ * convolutions), allocating nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory -- code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc.  d_x and d_y
+ * are device pointers the caller owns; the FFT work grids come from
+ * one static device array of 2 x 2^21 complex128 entries (64 MiB) per GPU,
+ * held until the process ends.  Calls needing larger grids (work grid
+ * entries per right-hand side x nrhs above 2^21; e.g. more than one
+ * right-hand side at 200 x 200 x 24 grids) return PEEC_ERR_ARG.  The call
+ * is asynchronous on the context's stream (peec_synchronize waits for it);
+ * calls that overlap on one GPU share the array and race.  Operator and
+ * frequency slots must be set (peec_operator_set, peec_frequency_set*);
+ * profiling must be off.  Same kernels and results as
+ * peec_apply_K_batched(..., precond = 0, ...). */
+int peec_apply_K_noalloc(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, const peec_c128 *d_x,
+                         peec_c128 *d_y);
+
 /* ---- adjoint (conductivity gradients) ------------------------------------
  * No counterpart in the reference, which has no derivatives.  K^H is
  * applied explicitly: the 12 convolutions of K (Lp and P are real and

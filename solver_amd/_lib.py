@@ -80,6 +80,8 @@ def lib():
         L.peec_frequency_set_slot_edges.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(CscC128)]
         L.peec_frequency_dz.argtypes = [vp, ctypes.c_int, c128p]
         L.peec_apply_K_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c128p, c128p]
+        L.peec_apply_K_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                           ctypes.c_void_p]
         L.peec_apply_KH_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c128p, c128p]
         L.peec_precond_rhs_batched_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int, c128p, c128p]
         L.peec_gmres_batched_adjoint.argtypes = [vp, ctypes.c_int, ctypes.c_int, c128p, ctypes.c_double,

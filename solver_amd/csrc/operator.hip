@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 
 void peec_ctx::release_operator()
@@ -1277,6 +1278,86 @@ extern "C" int peec_apply_KH_batched(peec_ctx *ctx, int nrhs, int rhs_per_slot, 
     return apply_batched_impl(ctx, nrhs, rhs_per_slot, fp32, precond, true, x, y);
 }
 
+// ---- K x without allocating (peec_apply_K_noalloc)
+//
+// The only device memory K x needs beyond its operands and the operator
+// data set up earlier is the pair of FFT work grids.  Here they are one
+// file-scope device array of kNoAllocGridElems complex128 entries per grid
+// (64 MiB in all), which every GPU the library runs on holds until the
+// process ends.  In host code `scratch` is a stand-in for the symbol, not a
+// device address: the address is looked up once per device
+// (hipGetSymbolAddress, that device current) and only that pointer is ever
+// passed to kernels.  All calls on one GPU share the array; calls that
+// overlap on one GPU (two contexts, two streams) race.  Every grid entry a
+// convolution reads is written by its own earlier pass, so nothing needs
+// clearing between calls.
+constexpr size_t kNoAllocGridElems = (size_t)1 << 21;
+static __device__ __align__(256) double2_t scratch[2 * kNoAllocGridElems];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double2_t *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which the caller has made
+// current).
+static int scratch_address(int device, double2_t **out)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices) {
+        peec_set_error("peec_apply_K_noalloc: device %d outside the %d the library tracks", device,
+                       kNoAllocMaxDevices);
+        return PEEC_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            peec_set_error("peec_apply_K_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                           hipGetErrorString(e));
+            return PEEC_ERR_HIP;
+        }
+        g_scratch_ptr[device] = (double2_t *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return PEEC_OK;
+}
+
+extern "C" int peec_apply_K_noalloc(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, const peec_c128 *d_x,
+                                    peec_c128 *d_y)
+{
+    if (!ctx || nrhs < 1 || rhs_per_slot < 1 || !d_x || !d_y) {
+        peec_set_error("peec_apply_K_noalloc: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (!ctx->op_ready || !ctx->freq_ready) {
+        peec_set_error("peec_apply_K_noalloc: operator and frequency slots must be set first");
+        return PEEC_ERR_STATE;
+    }
+    if (ctx->prof_on) {
+        peec_set_error("peec_apply_K_noalloc: kernel profiling (peec_profile) creates events; switch it off first");
+        return PEEC_ERR_STATE;
+    }
+    if ((nrhs + rhs_per_slot - 1) / rhs_per_slot > ctx->nslots) {
+        peec_set_error("peec_apply_K_noalloc: %d right-hand sides at %d per slot need more than the %d slots", nrhs,
+                       rhs_per_slot, ctx->nslots);
+        return PEEC_ERR_ARG;
+    }
+    const size_t need = work_grid_elems(ctx, nrhs);
+    if (need > kNoAllocGridElems) {
+        peec_set_error("peec_apply_K_noalloc: %d right-hand sides need FFT work grids of %zu complex128 entries, "
+                       "above the %zu this entry point holds; use fewer right-hand sides per call or "
+                       "peec_apply_K_batched",
+                       nrhs, need, kNoAllocGridElems);
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    double2_t *w = nullptr;
+    int rc = scratch_address(ctx->device, &w);
+    if (rc) return rc;
+    const int64_t N = ctx->m + ctx->ns + ctx->n;
+    return apply_K_work(ctx, (const double2_t *)d_x, (double2_t *)d_y, nrhs, N, fp32 != 0,
+                        uniform_cols(nrhs, rhs_per_slot), w, w + kNoAllocGridElems);
+}
 
 static int apply_batched_impl(peec_ctx *ctx, int nrhs, int rhs_per_slot, int fp32, int precond, bool adjoint,
                               const peec_c128 *x, peec_c128 *y)
