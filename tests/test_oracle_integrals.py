@@ -216,6 +216,96 @@ def test_p_diag_rows_vs_quadrature():
         assert rel(r, q) < 1e-8
 
 
+def test_collinear_p_form_depends_on_segment_order():
+    """integ_line_line_sp_P (compute_row_P_sup.jl:461-476) evaluates the
+    collinear-segment integral exactly when the source segment lies after
+    the target one (x1 > x2 at every corner); when it lies before, its text
+    gives about minus the integral.  The rows of the benchmark grids reach it
+    only in the second order: in the P rows of the (1,1) and (3,3) face pairs
+    on the 0.25 x 0.25 x 0.05 mm grids, the faces 41 or more cells along x
+    (resp. y) from the source lose their z extent and become collinear
+    segments (23 entries per row at 64x64x16, 55 at 96x96x24).  The oracle
+    follows the text there, so those entries are pinned by the exact order
+    only; their measured value is -1.02 x the face integral."""
+    A = (iv(0, S), pt(0.0), pt(0.0))
+    for off in (1.5, 3.0, 41.0):
+        B_after = (iv(-off * S, S), pt(0.0), pt(0.0))     # source segment follows
+        q = box_integral(A, B_after, n=28)
+        assert rel(_cf(115, [*A[0], 0.0, 0.0, *B_after[0], 0.0, 0.0]), q) < 1e-10
+        B_before = (iv(off * S, S), pt(0.0), pt(0.0))     # source segment precedes
+        v = _cf(115, [*A[0], 0.0, 0.0, *B_before[0], 0.0, 0.0])
+        assert -1.25 < v / box_integral(A, B_before, n=28) < -0.99
+    # the benchmark-grid entries, through the oracle's P row
+    sx, sy, sz = 0.25e-3, 0.25e-3, 0.05e-3
+    src = np.array([0.0, 0.0, 0.0])
+    eps0 = 8.854187816997944e-12
+    for k in (41, 44, 63):
+        tgt = np.array([[k * sx, 0.0, 0.0]])
+        v, form = OI.p_row_forms(src, tgt, sx, sy, sz, 1, 1)
+        assert int(form[0]) % 100 == 18
+        f1, A1 = _p_face(1, src, sx, sy, sz)
+        f2, A2 = _p_face(1, tgt[0], sx, sy, sz)
+        q = box_integral(f1, f2, n=24) / (4 * np.pi * eps0 * A1 * A2)
+        assert -1.03 < v[0] / q < -1.0
+
+
+def _p_face(caser, c, sx, sy, sz):
+    """The face patch of voxel centre c for a caser code and its area
+    (compute_row_P_sup.jl:3-51): 1/2 the -y/+y face, 3/4 the -x/+x face,
+    5/6 the -z/+z face."""
+    x, y, z = c
+    if caser in (1, 2):
+        return (iv(x, sx), pt(y - sy / 2 if caser == 1 else y + sy / 2), iv(z, sz)), sx * sz
+    if caser in (3, 4):
+        return (pt(x - sx / 2 if caser == 3 else x + sx / 2), iv(y, sy), iv(z, sz)), sy * sz
+    return (iv(x, sx), iv(y, sy), pt(z - sz / 2 if caser == 5 else z + sz / 2)), sx * sy
+
+
+# target offsets in cells: touching and near neighbours (the full closed
+# forms), then 40..1500 cells away, where the suppression tests of
+# Integ_sup_sup_P (compute_row_P_sup.jl:106-394) drop extents and the entry
+# routes to a line/line, line/surface or point/point form
+P_OFFSETS = [(1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 1, 0), (1, 2, 1), (3, 0, 2), (0, 3, 1), (40, 3, 1), (5, 60, 2),
+             (3, 4, 150), (300, 20, 7), (20, 700, 3), (1500, 900, 40)]
+P_PAIRS = [(1, 1), (3, 3), (5, 5), (3, 2), (5, 2), (5, 4), (2, 1), (4, 3), (6, 5), (1, 3), (1, 5), (3, 5)]
+FULL_P_FORMS = (12, 13)  # qs_surf_surf_para, p_surf_surf_ortho
+
+
+@pytest.mark.parametrize("pair", P_PAIRS)
+def test_p_pair_rows_vs_quadrature(pair):
+    """Every face pair of the P rows, at every routing distance, against the
+    Gauss-Legendre integral over the two faces / (4 pi eps0 A1 A2).  Full
+    surface-surface forms hold 1e-7 (the closed forms lose digits to
+    cancellation at 40+ cells: measured <= 1.9e-8); coincident faces are the
+    singular self terms, held at 1e-7 with a 64-point rule; entries that a
+    suppression test routes to a line or point form hold the suppression
+    error, 5e-6 (measured <= 2.1e-6).  A source face on the wrong side of
+    its voxel (+s/2 for -s/2) moves the near entries of every pair by 26% or
+    more (measured with the oracle's caser 2/4/6 rows against the 1/3/5
+    faces), and a wrong area in the prefactor moves every entry by S/T."""
+    c1, c2 = pair
+    eps0 = 8.854187816997944e-12
+    src = np.zeros(3)
+    reached = set()
+    for o in [(0, 0, 0)] + P_OFFSETS:
+        cj = np.array([o[0] * S, o[1] * S, o[2] * T])
+        f1, A1 = _p_face(c1, src, S, S, T)
+        f2, A2 = _p_face(c2, cj, S, S, T)
+        coincident = f1 == f2
+        if o == (0, 0, 0) and not coincident:
+            continue  # two faces of one voxel that share an edge: covered by the neighbour offsets
+        v, form = OI.p_row_forms(src, cj[None, :], S, S, T, c1, c2)
+        fid = int(form[0]) % 100
+        assert int(form[0]) < 100, ("threshold tie", pair, o)
+        assert fid not in OI.DEVIANT_FORMS, (pair, o, OI.FORM_NAMES[fid])
+        reached.add(fid)
+        q = box_integral(f1, f2, n=64 if coincident else 24) / (4 * np.pi * eps0 * A1 * A2)
+        bar = 1e-7 if fid in FULL_P_FORMS else 5e-6
+        assert rel(v[0], q) < bar, (pair, o, OI.FORM_NAMES[fid], rel(v[0], q))
+    # the offsets reach both the full form of the pair and a suppressed one
+    assert reached & set(FULL_P_FORMS) and reached - set(FULL_P_FORMS), reached
+
+
 def test_far_field_branch_is_point_point():
     # very distant thin voxels take the sum_supp == 6 branch: V1 V2 / R
     src = np.array([0.0, 0.0, 0.0])
