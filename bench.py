@@ -50,7 +50,11 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of a roofline-leg pass (FETCH_SIZE + WRITE_SIZE,
 # scaled by a copy kernel of known size), measured with rocprofv3 --pmc on
 # `bench.py --roofline-only`; the summaries and commands are in profiles/.
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {("large_192x192x48", "fft_x_fwd_scatter"): 370.18e6,
+                    ("large_192x192x48", "fft_y_fwd"): 844.69e6,
+                    ("large_192x192x48", "fft_z_conv"): 1098.52e6,
+                    ("large_192x192x48", "fft_y_inv"): 714.03e6,
+                    ("large_192x192x48", "fft_x_inv_gather"): 411.64e6}
 KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv_gather",
               "trsv_block_L", "trsv_block_U", "lu_factor", "fft_yz_conv", "trsv_panel"]
 # kernel symbols behind each profile kind: the LDS pass k_fft_pass<T, MODE,
@@ -169,6 +173,16 @@ def roofline_leg(workload, nrhs, reps, tune=True):
     if tune:
         peec.fft_tune(nrhs, fp64=False, fp32=True, ctx=ctx)
     t_tune = time.perf_counter() - t0
+    # the launch shapes the tuner chose (variant, lines per tile, lanes per
+    # line, rhs per block, radix schedule) per kernel and pass, so that
+    # profiles of separate runs can be matched to the same kernels
+    choices = {}
+    for kern in range(9):
+        for p in range(5):
+            c = peec.fft_choice(kern, p, True, nrhs, ctx=ctx)
+            if c[0] >= 0:
+                choices["k%d_%s" % (kern, KIND_NAMES[p])] = list(c)
+    log("  tuned choices: %s" % json.dumps(choices))
     X = np.random.default_rng(0).normal(size=(N, nrhs)) + 0j
     peec.apply_K_batched(X, nrhs, fp32=True, ctx=ctx)
     profile_read(ctx, True)
@@ -189,7 +203,7 @@ def roofline_leg(workload, nrhs, reps, tune=True):
     k = kinds[dom]
     return dict(workload=workload, grid=list(cfg["grid"].shape), unknowns=int(N), nrhs=nrhs, setup_s=round(t_setup, 2),
                 tune_s=round(t_tune, 2), fft_matvec_ms=tot_ms, fft_GB_per_matvec=tot_b / 1e9,
-                fft_matvec_GBs=tot_b / tot_ms / 1e6, kinds=kinds,
+                fft_matvec_GBs=tot_b / tot_ms / 1e6, kinds=kinds, choices=choices,
                 roofline={"bound": "hbm", "kernel": dom, "symbol": KERNEL_SYMBOLS[dom], "achieved": k["GBs"],
                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": k["GBs"] / PEAK_HBM_GBS,
                           "traffic": MEASURED_TRAFFIC.get((workload, dom)), "avg_launch_us": k["avg_us"],
