@@ -101,6 +101,204 @@ function gmres_custom(b, restarted, tol, maxit, x, wk, incidence_selection, FFTC
 end
 
 # ---------------------------------------------------------------------------
+# Device state behind the reference's FFT-solver functions.  The spectra, the
+# operator, the frequency terms and the LU factors live in the context; the
+# Julia objects the reference passes around (FFTCP, FFTCLp, lu, the FFTW
+# plans) become handles or are only used to tell whether the device copy is
+# current.
+
+# What compute_FFT_mutual_coupling_mats returns in place of the spectra:
+# FFTCP and FFTCLp are both this handle.
+struct PeecSpectra
+    ctx::Ptr{Cvoid}
+    id::Any
+end
+
+mutable struct PeecDeviceState
+    operator::UInt64      # objectid of the incidence_selection uploaded
+    freq_key::Any         # (w, objectid(DZ), objectid(Yle)) of the frequency terms uploaded
+    factors::UInt64       # objectid of the LU object uploaded
+end
+const PEEC_STATE = Dict{Ptr{Cvoid},PeecDeviceState}()
+const PEEC_CURRENT = Ref{Ptr{Cvoid}}(C_NULL)   # context of the running sweep (precond_3_3_Kt_fft! has no id)
+peec_state(ctx) = get!(() -> PeecDeviceState(0, nothing, 0), PEEC_STATE, ctx)
+
+function peec_kernels_build(ctx, origin, Nx, Ny, Nz, sx, sy, sz, esc_Lp, esc_P)
+    o = Float64[origin[1], origin[2], origin[3]]
+    rc = GC.@preserve o ccall((:peec_kernels_build, LIBPEEC), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Cint, Cint, Cint, Float64, Float64, Float64, Float64, Float64),
+        ctx, o, Nx, Ny, Nz, sx, sy, sz, esc_Lp, esc_P)
+    peec_check(rc, "peec_kernels_build")
+end
+
+# A, Gamma, mat_map_Lp[:, 1], exp_P (3 x 3, column-major) and diag(P)
+function peec_operator_set(ctx, incidence_selection, expansions, diag_P)
+    st = peec_state(ctx)
+    st.operator == objectid(incidence_selection) && return
+    A = SparseMatrixCSC{Float64,Int64}(incidence_selection["A"])
+    G = SparseMatrixCSC{Float64,Int64}(incidence_selection["Gamma"])
+    lp = [SparseMatrixCSC{Float64,Int64}(expansions["mat_map_Lp"][d, 1]) for d in 1:3]
+    pm = [SparseMatrixCSC{Float64,Int64}(expansions["exp_P"][i, j]) for i in 1:3, j in 1:3]   # vec: i fastest
+    dP = Vector{Float64}(vec(diag_P))
+    lpc = [CscF64(x) for x in lp]
+    pmc = [CscF64(x) for x in vec(pm)]
+    rc = GC.@preserve A G lp pm dP lpc pmc ccall((:peec_operator_set, LIBPEEC), Cint,
+        (Ptr{Cvoid}, Int64, Int64, Int64, Ref{CscF64}, Ref{CscF64}, Ptr{CscF64}, Ptr{CscF64}, Ptr{Float64}),
+        ctx, size(lp[1], 2), size(lp[2], 2), size(lp[3], 2), CscF64(A), CscF64(G), lpc, pmc, dP)
+    peec_check(rc, "peec_operator_set")
+    st.operator = objectid(incidence_selection)
+    st.freq_key = nothing
+    st.factors = 0
+end
+
+function peec_frequency_set(ctx, w, DZ::AbstractVecOrMat, Yle::SparseMatrixCSC{ComplexF64,Int64})
+    st = peec_state(ctx)
+    key = (w, objectid(DZ), objectid(Yle))
+    st.freq_key == key && return
+    dz = Vector{ComplexF64}(vec(DZ))
+    rc = GC.@preserve dz Yle ccall((:peec_frequency_set, LIBPEEC), Cint,
+        (Ptr{Cvoid}, Float64, Ptr{ComplexF64}, Ref{CscC128}), ctx, w, dz, CscC128(Yle))
+    peec_check(rc, "peec_frequency_set")
+    st.freq_key = key
+end
+
+# UMFPACK's F.L * F.U == (F.Rs .* SS)[F.p, F.q] is the library's convention
+function peec_lu_set_factors(ctx, F)
+    st = peec_state(ctx)
+    st.factors == objectid(F) && return
+    L = SparseMatrixCSC{ComplexF64,Int64}(F.L); U = SparseMatrixCSC{ComplexF64,Int64}(F.U)
+    p = Int64.(F.p) .- 1; q = Int64.(F.q) .- 1; Rs = Vector{Float64}(F.Rs)
+    rc = GC.@preserve L U p q Rs ccall((:peec_lu_set_factors, LIBPEEC), Cint,
+        (Ptr{Cvoid}, Int64, Ref{CscC128}, Ref{CscC128}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+        ctx, size(L, 1), CscC128(L), CscC128(U), p, q, Rs)
+    peec_check(rc, "peec_lu_set_factors")
+    st.factors = objectid(F)
+end
+
+# compute_FFT_mutual_coupling_mats.jl:1-33.  The nine kernel rows and their
+# spectra are computed on the device from the grid description: the voxel
+# origin follows from the first Lp centre (build_centers_Lp_with_air.jl,
+# build_center_air_Voxels.jl: centres at origin + s .* (i - 1/2), Lp centres
+# half a cell further along their axis).
+function compute_FFT_mutual_coupling_mats(circulant_centers, escalings, Nx, Ny, Nz, QS_Rcc_FW, id, chan)
+    QS_Rcc_FW == 1 || error("libpeec_hip builds the quasi-static (QS_Rcc_FW = 1) spectra only")
+    ctx = peec_ctx(id); peec_sync_stop!(id)
+    sx, sy, sz = circulant_centers["sx"], circulant_centers["sy"], circulant_centers["sz"]
+    half = (sx / 2, sy / 2, sz / 2)
+    axis = findfirst(d -> size(circulant_centers[d], 1) > 0, ("Lpx", "Lpy", "Lpz"))
+    axis === nothing && error("compute_FFT_mutual_coupling_mats: a 1 x 1 x 1 grid has no edges")
+    c = circulant_centers[("Lpx", "Lpy", "Lpz")[axis]][1, :]
+    origin = [c[d] - half[d] - (d == axis ? half[d] : 0.0) for d in 1:3]
+    peec_kernels_build(ctx, origin, Nx, Ny, Nz, sx, sy, sz, escalings["Lp"], escalings["P"])
+    PEEC_CURRENT[] = ctx
+    h = PeecSpectra(ctx, id)
+    return h, h
+end
+
+# DZ of one frequency (FFT_solver_QS_S_type.jl:106-114): the self impedance
+# of every edge plus its skin-effect impedance, of which only the larger
+# real part is kept on surface edges, plus j w Lp.
+function peec_edge_impedance(diagonals, escalings, Zs_info, wk)
+    Zself = vec(compute_Z_self(diagonals["R"], diagonals["Cd"], wk))
+    Zs = vec(escalings["R"] * (Zs_info["Zs"] * sqrt(wk / escalings["freq"])))
+    se = vec(Zs_info["surface_edges"])
+    dre = real.(Zs[se]) .- real.(Zself[se])
+    keep_skin = se[dre .> 0]
+    keep_self = se[dre .< 0]
+    Zself[keep_skin] .= im .* imag.(Zself[keep_skin])
+    Zs[keep_self] .= im .* imag.(Zs[keep_self])
+    return (Zself .+ real.(Zs)) .+ im * wk .* vec(diagonals["Lp"])
+end
+
+# FFT_solver_QS_S_type.jl:1-234.  Same arguments and the same Dict result
+# ("S", "Z", "Y", "f").  Per frequency the host still assembles SS and
+# factors it with UMFPACK (the reference's pivoting and scaling); the
+# frequency terms and factors then go to the device, where every port is
+# preconditioned and solved by gmres_custom's algorithm, warm-started from
+# the previous frequency's solution as the reference does (Vrest).
+function FFT_solver_QS_S_type(freq, escalings, incidence_selection, FFTCP, FFTCLp, diagonals, ports,
+                              ports_scatter_value, lumped_elements, expansions, GMRES_settings, Zs_info, QS_Rcc_FW,
+                              id, chan, commentsEnabled::Bool)
+    QS_Rcc_FW == 1 || error("libpeec_hip runs the quasi-static (QS_Rcc_FW = 1) solver only")
+    FFTCP isa PeecSpectra || error("FFT_solver_QS_S_type: FFTCP must come from compute_FFT_mutual_coupling_mats")
+    ctx = FFTCP.ctx; PEEC_CURRENT[] = ctx
+    A = incidence_selection["A"]; Gamma = incidence_selection["Gamma"]
+    m, n = size(A); ns = size(Gamma, 2)
+    N = m + ns + n
+    peec_operator_set(ctx, incidence_selection, expansions, diagonals["P"])
+    invP = sparse(1:ns, 1:ns, 1 ./ vec(diagonals["P"]), ns, ns)
+    fs = freq .* escalings["freq"]
+    P = size(ports["port_nodes"], 1)
+    S = zeros(ComplexF64, P, P, length(fs))
+    Vrest = zeros(ComplexF64, N, P)
+    maxit = GMRES_settings["Inner_Iter"]
+    for k in eachindex(fs)
+        peec_sync_stop!(id)
+        is_stop_requested(id) && return nothing
+        wk = 2pi * fs[k]
+        Yle = build_Yle_FFT(lumped_elements, [], ports, escalings, n, wk / escalings["freq"], ports_scatter_value)
+        DZ = peec_edge_impedance(diagonals, escalings, Zs_info, wk)
+        peec_frequency_set(ctx, wk, DZ, Yle)
+        invZ = sparse(1:m, 1:m, 1 ./ DZ, m, m)
+        F = lu(Yle + transpose(A) * (invZ * A) + im * wk * (Gamma * invP) * transpose(Gamma))
+        peec_lu_set_factors(ctx, F)
+        for c1 in 1:P
+            n1, n2 = Int(ports["port_nodes"][c1, 1]), Int(ports["port_nodes"][c1, 2])
+            is = zeros(Float64, n); is[n1] = escalings["Is"]; is[n2] = -escalings["Is"]
+            tn = Vector{ComplexF64}(undef, N)
+            peec_check(GC.@preserve is tn ccall((:peec_precond_rhs, LIBPEEC), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{ComplexF64}), ctx, is, tn), "peec_precond_rhs")
+            V, flag, relres, it, resvec = gmres_custom(tn, false, GMRES_settings["tol"][k], maxit, Vrest[:, c1], wk,
+                incidence_selection, FFTCP, FFTCLp, DZ, Yle, expansions, invZ, invP, F, nothing, nothing, nothing,
+                nothing, nothing, id, chan, c1)
+            flag == 99 && return nothing
+            commentsEnabled && println("Flag $flag - Iteration = $k - port $c1, iterations $(it)")
+            Vrest[:, c1] = V
+            for c2 in c1:P
+                n3, n4 = Int(ports["port_nodes"][c2, 1]), Int(ports["port_nodes"][c2, 2])
+                dv = 2 * (V[m+ns+n3] - V[m+ns+n4])
+                S[c1, c2, k] = c1 == c2 ? (dv - ports_scatter_value) / ports_scatter_value : dv / ports_scatter_value
+                S[c2, c1, k] = S[c1, c2, k]
+            end
+        end
+    end
+    return Dict("S" => S, "Z" => s2z_fft(S, ports_scatter_value), "Y" => s2y_fft(S, ports_scatter_value),
+                "f" => fs ./ escalings["freq"])
+end
+
+# compute_Matrix_vector_fft.jl:1-81: y = M^-1 K x.  The spectra, plans and
+# factors the reference passes are handles here; the device copies of the
+# frequency terms (w, DZ, Yle) and of the factors (lu) are refreshed when
+# the caller passes different objects.
+function ComputeMatrixVectorFTT(x, w::Float64, incidence_selection::Dict, FFTCP, FFTCLp, DZ, Yle, expansions, invZ,
+                                invP, lu, PLIVector, PVector, PLI2Vector, P2Vector, chi2Vector)
+    FFTCP isa PeecSpectra || error("ComputeMatrixVectorFTT: FFTCP must come from compute_FFT_mutual_coupling_mats")
+    ctx = FFTCP.ctx; PEEC_CURRENT[] = ctx
+    peec_state(ctx).operator == objectid(incidence_selection) ||
+        error("ComputeMatrixVectorFTT: the operator of this geometry is not on the device (FFT_solver_QS_S_type sets it)")
+    peec_frequency_set(ctx, w, DZ, Yle)
+    peec_lu_set_factors(ctx, lu)
+    xx = Vector{ComplexF64}(vec(x)); y = similar(xx)
+    peec_check(GC.@preserve xx y ccall((:peec_matvec, LIBPEEC), Cint,
+        (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{ComplexF64}), ctx, xx, y), "peec_matvec")
+    return y
+end
+
+# FFT_solver_QS_S_type.jl:237-262: Y = M^-1 [0; 0; X3] with the factors F
+# (uploaded if F is not the object on the device).  The context is the one
+# of the running sweep.
+function precond_3_3_Kt_fft!(F, invZ, invP, A, Gamma, n1, n2, X3, Y, resProd)
+    ctx = PEEC_CURRENT[]
+    ctx == C_NULL && error("precond_3_3_Kt_fft!: no libpeec_hip context (compute_FFT_mutual_coupling_mats creates it)")
+    peec_lu_set_factors(ctx, F)
+    is = Vector{Float64}(vec(X3)); tn = Vector{ComplexF64}(undef, length(Y))
+    peec_check(GC.@preserve is tn ccall((:peec_precond_rhs, LIBPEEC), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{ComplexF64}), ctx, is, tn), "peec_precond_rhs")
+    Y[:] .= tn
+    return Y
+end
+
+# ---------------------------------------------------------------------------
 # Batched sweep and adjoint (no counterpart in the reference).  Frequency
 # slots hold per-frequency DZ, Yle and device LU factors; right-hand side r of
 # a batched call runs against slot r ÷ rhs_per_slot (0-based).

@@ -33,6 +33,35 @@ def test_every_declared_entry_point_is_exported(lib):
     assert not missing, missing
 
 
+# the reference functions julia/PeecHip.jl redefines, with their arities
+# (compute_Lp_Voxels.jl:1, compute_row_P_sup.jl:1, compute_diagonals.jl:136,
+# gmres_custom.jl:1, compute_FFT_mutual_coupling_mats.jl:1,
+# FFT_solver_QS_S_type.jl:1, compute_Matrix_vector_fft.jl:1,
+# FFT_solver_QS_S_type.jl:237)
+JULIA_SEAM = {"compute_Lp_Voxels": 11, "compute_row_P_sup": 8, "Lp_self": 3, "gmres_custom": 23,
+              "compute_FFT_mutual_coupling_mats": 8, "FFT_solver_QS_S_type": 16, "ComputeMatrixVectorFTT": 16,
+              "precond_3_3_Kt_fft!": 10}
+
+
+def _julia_functions(txt):
+    """name -> argument count of every top-level `function name(...)`."""
+    out = {}
+    for m in re.finditer(r"^function ([A-Za-z_][A-Za-z0-9_!]*)\((.*?)\)\s*$", txt, flags=re.M | re.S):
+        args = re.sub(r"\s+", " ", m.group(2)).split(";")[0]
+        out[m.group(1)] = len([a for a in args.split(",") if a.strip()])
+    return out
+
+
+def test_julia_shim_redefines_the_seam_and_calls_only_declared_symbols(lib):
+    txt = open(os.path.join(ROOT, "julia", "PeecHip.jl")).read()
+    funcs = _julia_functions(txt)
+    for name, arity in JULIA_SEAM.items():
+        assert funcs.get(name) == arity, (name, funcs.get(name), arity)
+    called = set(re.findall(r"ccall\(\(:(peec_[a-z0-9_]+), LIBPEEC\)", txt))
+    assert called and called <= set(declared_symbols()), called - set(declared_symbols())
+    assert all(hasattr(lib, n) for n in called)
+
+
 def _host_lp(lib, src, c, s, dc):
     from solver_amd import _lib as L
     cf = np.ascontiguousarray(c.T).reshape(-1)
