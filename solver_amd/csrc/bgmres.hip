@@ -233,8 +233,6 @@ inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
-namespace {
-
 struct BWork {
     peec_ctx *ctx;
     int nrhs, rps, m;
@@ -355,7 +353,35 @@ struct BWork {
     }
 };
 
-}  // namespace
+void bwork_free(BWork *w) { delete w; }
+
+// The context's batched workspace for (nrhs, restart, N), reallocated only
+// when the shape changes (the old one is freed first).  Matvec graphs are
+// captured per call: buffers of the context may move between calls.
+static BWork *bwork_acquire(peec_ctx *ctx, bool adjoint, int nrhs, int rps, int restart, int64_t N, int *rc)
+{
+    BWork *W = ctx->bwork.get();
+    if (!W || W->nrhs != nrhs || W->m != restart || W->N != N) {
+        ctx->bwork.reset();
+        ctx->bwork.reset(new BWork());
+        W = ctx->bwork.get();
+        W->nrhs = nrhs;
+        W->m = restart;
+        W->N = N;
+    }
+    W->ctx = ctx;
+    W->adjoint = adjoint;
+    W->rps = rps;
+    W->graphs.clear();
+    W->graphs.ctx = nullptr;
+    W->cols.clear();
+    W->wmax = 0;
+    if ((*rc = W->alloc())) {  // sizes unchanged: no reallocation
+        ctx->bwork.reset();
+        return nullptr;
+    }
+    return W;
+}
 
 static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per_slot, const peec_c128 *b,
                               double tol, int restart, int max_cycles, int mixed, double inner_tol, peec_c128 *x,
@@ -620,16 +646,11 @@ static int gmres_batched_impl(peec_ctx *ctx, bool adjoint, int nrhs, int rhs_per
         return PEEC_ERR_ARG;
     }
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
-    BWork W;
-    W.ctx = ctx;
-    W.adjoint = adjoint;
-    W.nrhs = nrhs;
-    W.rps = rhs_per_slot;
-    W.m = restart;
-    W.N = ctx->m + ctx->ns + ctx->n;
-    const int64_t N = W.N;
-    int rc = W.alloc();
-    if (rc) return rc;
+    const int64_t N = ctx->m + ctx->ns + ctx->n;
+    int rc = PEEC_OK;
+    BWork *Wp = bwork_acquire(ctx, adjoint, nrhs, rhs_per_slot, restart, N, &rc);
+    if (!Wp) return rc;
+    BWork &W = *Wp;
     const size_t bytes = (size_t)nrhs * (size_t)N * 16;
     PEEC_HIP_TRY(hipMemcpyAsync(W.B.p, b, bytes, hipMemcpyHostToDevice, ctx->stream));
     PEEC_HIP_TRY(hipMemcpyAsync(W.X.p, x, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -667,15 +688,10 @@ extern "C" int peec_sweep_batch(peec_ctx *ctx, int nports, const int64_t *port_n
         }
     const int nsl = ctx->nslots, nrhs = nsl * nports;
     PEEC_HIP_TRY(hipSetDevice(ctx->device));
-    BWork W;
-    W.ctx = ctx;
-    W.adjoint = false;
-    W.nrhs = nrhs;
-    W.rps = nports;
-    W.m = restart;
-    W.N = N;
-    int rc = W.alloc();
-    if (rc) return rc;
+    int rc = PEEC_OK;
+    BWork *Wp = bwork_acquire(ctx, false, nrhs, nports, restart, N, &rc);
+    if (!Wp) return rc;
+    BWork &W = *Wp;
     DevBuf<int64_t> pn;
     DevBuf<cd> vd;
     if (!pn.alloc((size_t)(2 * nports)) || !vd.alloc((size_t)nrhs * nports)) return PEEC_ERR_NOMEM;

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <vector>
 
 #include "peec_internal.h"
@@ -254,6 +255,13 @@ enum ProfKind {
     PK_FFT_YZ_CONV, PK_TRSV_PANEL, PK_COUNT
 };
 
+// The batched GMRES workspace, defined in bgmres.hip.
+struct BWork;
+void bwork_free(BWork *w);
+struct BWorkDeleter {
+    void operator()(BWork *w) const { bwork_free(w); }
+};
+
 struct peec_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -335,6 +343,11 @@ struct peec_ctx {
     // ---- HIP graphs: GMRES replays its captured matvec (peec_set_graphs)
     bool use_graphs = true;
 
+    // ---- batched GMRES workspace (bgmres.hip): the Krylov basis and the
+    // N x nrhs vectors, kept across peec_gmres_batched / peec_sweep_batch
+    // calls of the same shape (nrhs, restart, N) and freed with the operator
+    std::unique_ptr<BWork, BWorkDeleter> bwork;
+
     // ---- profiling
     bool prof_on = false;
     std::vector<ProfRec> prof;
@@ -381,6 +394,7 @@ struct MatvecGraphs {
     bool adjoint = false;
     std::vector<Entry> entries;
     ~MatvecGraphs();
+    void clear();  // destroys the captured graphs
     int run(const double2_t *x, double2_t *y, bool fp32);
     // the same with nrhs_c columns grouped by an explicit column map (the
     // map's device arrays may change between calls; their pointers may not)

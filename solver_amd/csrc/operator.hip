@@ -45,6 +45,7 @@ void peec_ctx::release_operator()
     fft_tune.clear();
     sweep_x0.reset();
     sweep_nports = 0;
+    bwork.reset();
     edges_ready = false;
     for (auto &k : kern) {
         k.poly = false;
@@ -1123,9 +1124,12 @@ static int matvec_prepare(peec_ctx *ctx, int nrhs)
     return PEEC_OK;
 }
 
-MatvecGraphs::~MatvecGraphs()
+MatvecGraphs::~MatvecGraphs() { clear(); }
+
+void MatvecGraphs::clear()
 {
     for (auto &e : entries) (void)hipGraphExecDestroy(e.exec);
+    entries.clear();
 }
 
 int MatvecGraphs::run(const double2_t *x, double2_t *y, bool fp32)
