@@ -2,6 +2,7 @@
 
 ctypes front end of the C restatement in peec_integrals_oracle.c:
   lp_row   <- compute_Lp_Voxels      (compute_Lp_Voxels.jl:1-63)
+  lp_row_qs <- compute_Lp_Voxels_QS  (compute_Lp_Voxels_QS.jl:9-42, the Rcc rows)
   p_row    <- compute_row_P_sup      (compute_row_P_sup.jl:1-104)
   lp_self  <- Lp_self                (compute_diagonals.jl:136-155)
 """
@@ -41,6 +42,10 @@ def lib():
         L.oracle_p_row_forms.argtypes = [dp, dp, ctypes.c_int64] + [ctypes.c_double] * 3 + \
             [ctypes.c_int, ctypes.c_int, dp, ip]
         L.oracle_p_row_forms.restype = None
+        L.oracle_lp_row_qs.argtypes = L.oracle_lp_row.argtypes
+        L.oracle_lp_row_qs.restype = None
+        L.oracle_lp_row_qs_forms.argtypes = L.oracle_lp_row_forms.argtypes
+        L.oracle_lp_row_qs_forms.restype = None
         _lib = L
     return _lib
 
@@ -77,9 +82,36 @@ FORM_NAMES = {0: "vol_vol", 1: "vol_surf", 2: "lp_surf_surf_para", 3: "lp_surf_s
               5: "line_surf_para", 6: "lp_line_surf_ortho", 7: "point_vol", 8: "line_line_ortho_xy",
               9: "lp_line_line_parall", 10: "point_sup", 11: "line_point", 12: "qs_surf_surf_para",
               13: "p_surf_surf_ortho", 14: "p_line_surf_ortho", 15: "p_line_line_parall", 16: "far_field",
-              17: "lp_line_line_sp", 18: "p_line_line_sp"}
+              17: "lp_line_line_sp", 18: "p_line_line_sp", 19: "qs_line_line_sp", 20: "qs_line_point",
+              21: "qs_line_surf_ortho", 22: "qs_line_surf_para", 23: "qs_line_vol", 24: "qs_vol_surf",
+              25: "qs_surf_surf_ortho_call_error"}
 # the forms that do not evaluate the integral their call sites use them for
 DEVIANT_FORMS = (1, 3, 6, 17, 18)
+
+
+def lp_row_qs(src, centers, sx, sy, sz, sx2, sy2, sz2, dc):
+    """One row of compute_Lp_Voxels_QS (compute_Lp_Voxels_QS.jl:9-42), the
+    closed-form copy of the retarded (Rcc) rows; without the Lp escaling."""
+    src = np.ascontiguousarray(src, dtype=np.float64).reshape(3)
+    c = np.asarray(centers, dtype=np.float64).reshape(-1, 3)
+    N = c.shape[0]
+    out = np.empty(N, dtype=np.float64)
+    cf = np.ascontiguousarray(c.T).reshape(-1)
+    lib().oracle_lp_row_qs(_dp(src), _dp(cf), N, sx, sy, sz, sx2, sy2, sz2, int(dc), _dp(out))
+    return out
+
+
+def lp_row_qs_forms(src, centers, sx, sy, sz, sx2, sy2, sz2, dc):
+    """(values, forms) of one compute_Lp_Voxels_QS row, as lp_row_forms."""
+    src = np.ascontiguousarray(src, dtype=np.float64).reshape(3)
+    c = np.asarray(centers, dtype=np.float64).reshape(-1, 3)
+    N = c.shape[0]
+    out = np.empty(N, dtype=np.float64)
+    form = np.empty(N, dtype=np.int32)
+    cf = np.ascontiguousarray(c.T).reshape(-1)
+    lib().oracle_lp_row_qs_forms(_dp(src), _dp(cf), N, sx, sy, sz, sx2, sy2, sz2, int(dc), _dp(out),
+                                 form.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return out, form
 
 
 def lp_row_forms(src, centers, sx, sy, sz, sx2, sy2, sz2, dc):

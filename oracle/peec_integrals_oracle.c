@@ -38,6 +38,10 @@
  * bits of the geometry arithmetic. */
 static int g_form = -1;
 static int g_tie = 0;
+/* 1 while a row of compute_Lp_Voxels_QS.jl (the closed-form copy the
+ * retarded Rcc rows use) is evaluated: the dispatcher then resolves to that
+ * file's forms (see the QS block below) */
+static int g_qs = 0;
 
 static int near_thr(double v, double thr) { return fabs(v - thr) <= 1e-9 * thr; }
 
@@ -161,6 +165,177 @@ static double p_line_line_sp(const double *x1v, const double *x2v)
     return sol;
 }
 
+/* ---- compute_Lp_Voxels_QS.jl: the closed forms that differ from
+ * compute_Lp_Voxels.jl.  Every real(log((d) + R)) of these forms is the real
+ * log of a non-negative real (R >= |d|), not the real part of a complex log;
+ * the forms not listed here are the same arithmetic in both files. ------ */
+
+/* QS :501-519: x1*log|x1-x2| inside the (x1-x2)/R factor */
+static double qs_line_line_sp(const double *x1v, const double *x2v)
+{
+    g_form = 19;
+    double sol = 0.0;
+    for (int c1 = 1; c1 <= 2; c1++) {
+        double x1 = x1v[c1 - 1];
+        for (int c2 = 1; c2 <= 2; c2++) {
+            double x2 = x2v[c2 - 1];
+            double R = sqrt((x1 - x2) * (x1 - x2));
+            double t1 = (x1 - x2) / R * (x1 - x2 * jl_clog_re(x2 - x1) + x1 * jl_clog_re(x1 - x2));
+            if (ISBAD(t1)) t1 = 0.0;
+            sol += sgnpow(c1 + c2 + 1) * t1;
+        }
+    }
+    return sol;
+}
+
+/* QS :418-442: `(isnan(check) || isinf(check)) && x1 > 0` */
+static double qs_line_point(const double *x1v, double y3, double z3,
+                            double x1, double y1, double z1)
+{
+    g_form = 20;
+    double x3 = x1v[0], x4 = x1v[1];
+    double check = 1.0 / (x1 - x3);
+    if (ISBAD(check))
+        x1 -= 1e-8;
+    check = 1.0 / (x1 - x4);
+    if (ISBAD(check) && x1 > 0)
+        x1 += 1e-8;
+    double R1 = sqrt((x1 - x3) * (x1 - x3) + (y1 - y3) * (y1 - y3) + (z1 - z3) * (z1 - z3));
+    double R2 = sqrt((x1 - x4) * (x1 - x4) + (y1 - y3) * (y1 - y3) + (z1 - z3) * (z1 - z3));
+    double Ip = (or_det_log((x1 - x3) + R1)) - (or_det_log((x1 - x4) + R2));
+    if (ISBAD(Ip))
+        Ip = (x1 - x3) / fabs(x1 - x3) * or_det_log(fabs(x1 - x3)) - (x1 - x4) / fabs(x1 - x4) * or_det_log(fabs(x1 - x4));
+    return Ip;
+}
+
+/* QS :595-625: three log and three atan terms */
+static double qs_line_surf_ortho(const double *x1v, double y1, double z1,
+                                 double x2, const double *y2v, const double *z2v)
+{
+    g_form = 21;
+    double sol = 0.0;
+    for (int c1 = 1; c1 <= 2; c1++) {
+        double x1 = x1v[c1 - 1];
+        for (int c2 = 1; c2 <= 2; c2++) {
+            double y2 = y2v[c2 - 1];
+            for (int c3 = 1; c3 <= 2; c3++) {
+                double z2 = z2v[c3 - 1];
+                double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+                double R = sqrt(dx * dx + dy * dy + dz * dz);
+                double t1 = dy * dz * or_det_log(dx + R);
+                if (ISBAD(t1)) t1 = 0.0;
+                double t2 = dx * dz * or_det_log(dy + R);
+                if (ISBAD(t2)) t2 = 0.0;
+                double t3 = dy * dx * or_det_log(dz + R);
+                if (ISBAD(t3)) t3 = 0.0;
+                double t4 = -0.5 * fabs(dz) * dz * or_det_atan2(dx * dy, fabs(dz) * R);
+                double t5 = -0.5 * fabs(dy) * dy * or_det_atan2(dx * dz, fabs(dy) * R);
+                double t6 = -0.5 * fabs(dx) * dx * or_det_atan2(dy * dz, fabs(dx) * R);
+                sol += sgnpow(c1 + c2 + c3) * (t1 + t2 + t3 + t4 + t5 + t6);
+            }
+        }
+    }
+    return sol;
+}
+
+/* QS :627-654 */
+static double qs_line_surf_para(const double *x1v, const double *y1v, double z1,
+                                const double *x2v, double y2, double z2)
+{
+    g_form = 22;
+    double sol = 0.0;
+    for (int c1 = 1; c1 <= 2; c1++) {
+        double x1 = x1v[c1 - 1];
+        for (int c2 = 1; c2 <= 2; c2++) {
+            double y1 = y1v[c2 - 1];
+            for (int c3 = 1; c3 <= 2; c3++) {
+                double x2 = x2v[c3 - 1];
+                double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+                double R = sqrt(dx * dx + dy * dy + dz * dz);
+                double t1 = dx * dy * or_det_log(dx + R);
+                if (ISBAD(t1)) t1 = 0.0;
+                double t2 = (dx * dx - dz * dz) / 2 * or_det_log(dy + R);
+                if (ISBAD(t2)) t2 = 0.0;
+                double t3 = -dx * fabs(dz) * or_det_atan2(dx * dy, fabs(dz) * R);
+                double t4 = -dy / 2 * R;
+                sol += sgnpow(c1 + c2 + c3 + 1) * (t1 + t2 + t3 + t4);
+            }
+        }
+    }
+    return sol;
+}
+
+/* QS :656-693 */
+static double qs_line_vol(const double *x1v, const double *y1v, const double *z1v,
+                          const double *x2v, double y2, double z2)
+{
+    g_form = 23;
+    double sol = 0.0;
+    for (int c1 = 1; c1 <= 2; c1++) {
+        double x1 = x1v[c1 - 1];
+        for (int c2 = 1; c2 <= 2; c2++) {
+            double y1 = y1v[c2 - 1];
+            for (int c3 = 1; c3 <= 2; c3++) {
+                double z1 = z1v[c3 - 1];
+                for (int c4 = 1; c4 <= 2; c4++) {
+                    double x2 = x2v[c4 - 1];
+                    double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+                    double R = sqrt(dx * dx + dy * dy + dz * dz);
+                    double t1 = -1.0 / 3 * dy * dz * R;
+                    double t2 = -1.0 / 2 * dx * fabs(dz) * dz * or_det_atan2(dx * dy, fabs(dz) * R);
+                    double t3 = -1.0 / 2 * dx * fabs(dy) * dy * or_det_atan2(dx * dz, fabs(dy) * R);
+                    double t4 = -1.0 / 6 * (fabs(dx) * fabs(dx) * fabs(dx)) * or_det_atan2(dy * dz, fabs(dx) * R);
+                    double t5 = dx * dy * dz * or_det_log(dx + R);
+                    if (ISBAD(t5)) t5 = 0.0;
+                    double t6 = (1.0 / 2 * (dx * dx) - 1.0 / 6 * (dz * dz)) * dz * or_det_log(dy + R);
+                    if (ISBAD(t6)) t6 = 0.0;
+                    double t7 = (1.0 / 2 * (dx * dx) - 1.0 / 6 * (dy * dy)) * dy * or_det_log(dz + R);
+                    if (ISBAD(t7)) t7 = 0.0;
+                    sol += sgnpow(c1 + c2 + c3 + c4 + 1) * (t1 + t2 + t3 + t4 + t5 + t6 + t7);
+                }
+            }
+        }
+    }
+    return sol;
+}
+
+/* QS :777-817: terms 2 and 3 carry a (z1 - z2) factor */
+static double qs_vol_surf(const double *x1v, const double *y1v, const double *z1v,
+                          const double *x2v, const double *y2v, double z2)
+{
+    g_form = 24;
+    double sol = 0.0;
+    for (int c1 = 1; c1 <= 2; c1++) {
+        double x1 = x1v[c1 - 1];
+        for (int c2 = 1; c2 <= 2; c2++) {
+            double y1 = y1v[c2 - 1];
+            for (int c3 = 1; c3 <= 2; c3++) {
+                double z1 = z1v[c3 - 1];
+                for (int c4 = 1; c4 <= 2; c4++) {
+                    double x2 = x2v[c4 - 1];
+                    for (int c5 = 1; c5 <= 2; c5++) {
+                        double y2 = y2v[c5 - 1];
+                        double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+                        double R = sqrt(dx * dx + dy * dy + dz * dz);
+                        double t1 = ((dz * dz) / 12 - ((dx * dx) + (dy * dy)) / 8) * dz * R;
+                        double t2 = ((dy * dy) / 2 - (dz * dz) / 6) * dx * dz * or_det_log(dx + R);
+                        if (ISBAD(t2)) t2 = 0.0;
+                        double t3 = ((dx * dx) / 2 - (dz * dz) / 6) * dy * dz * or_det_log(dy + R);
+                        if (ISBAD(t3)) t3 = 0.0;
+                        double t4 = (-jl_pow4(dx) / 24 - jl_pow4(dy) / 24 + ((dy * dy) * (dx * dx)) / 4) * or_det_log(dz + R);
+                        if (ISBAD(t4)) t4 = 0.0;
+                        double t5 = -(fabs(dx) * fabs(dx) * fabs(dx)) * dy / 6 * or_det_atan2(dy * dz, fabs(dx) * R);
+                        double t6 = -dx * (fabs(dy) * fabs(dy) * fabs(dy)) / 6 * or_det_atan2(dx * dz, fabs(dy) * R);
+                        double t7 = -dx * dy * fabs(dz) * dz / 2 * or_det_atan2(dx * dy, fabs(dz) * R);
+                        sol += sgnpow(c1 + c2 + c3 + c4 + c5) * (t1 + t2 + t3 + t4 + t5 + t6 + t7);
+                    }
+                }
+            }
+        }
+    }
+    return sol;
+}
+
 /* :491-512 / compute_row_P_sup.jl:438-459 */
 static double line_line_parall(const double *x1v, double y1, double z1,
                                const double *x2v, double y2, double z2, int is_p)
@@ -168,7 +343,7 @@ static double line_line_parall(const double *x1v, double y1, double z1,
     g_form = is_p ? 15 : 9;
     double dy = y1 - y2, dz = z1 - z2;
     if (fabs(dy) < 1e-10 && fabs(dz) < 1e-10)
-        return is_p ? p_line_line_sp(x1v, x2v) : lp_line_line_sp(x1v, x2v);
+        return is_p ? p_line_line_sp(x1v, x2v) : g_qs ? qs_line_line_sp(x1v, x2v) : lp_line_line_sp(x1v, x2v);
     double sol = 0.0;
     for (int c1 = 1; c1 <= 2; c1++) {
         double x1 = x1v[c1 - 1];
@@ -604,6 +779,49 @@ static void lp_supp_box(double a, double b, double c, const double *ax, const do
     }
 }
 
+/* The Lp dispatcher's forms: compute_Lp_Voxels.jl's, or while g_qs is set
+ * compute_Lp_Voxels_QS.jl's (whose Integ_vol_vol, :44-423, has the same
+ * branch structure). */
+static double d_line_point(const double *x1v, double y3, double z3, double x1, double y1, double z1)
+{
+    return g_qs ? qs_line_point(x1v, y3, z3, x1, y1, z1) : lp_line_point(x1v, y3, z3, x1, y1, z1);
+}
+static double d_line_surf_ortho(const double *x1v, double y1, double z1, double x2, const double *y2v, const double *z2v)
+{
+    return g_qs ? qs_line_surf_ortho(x1v, y1, z1, x2, y2v, z2v) : line_surf_ortho(x1v, y1, z1, x2, y2v, z2v);
+}
+static double d_line_surf_para(const double *x1v, const double *y1v, double z1, const double *x2v, double y2, double z2)
+{
+    return g_qs ? qs_line_surf_para(x1v, y1v, z1, x2v, y2, z2) : line_surf_para(x1v, y1v, z1, x2v, y2, z2);
+}
+static double d_line_vol(const double *x1v, const double *y1v, const double *z1v, const double *x2v, double y2,
+                         double z2)
+{
+    return g_qs ? qs_line_vol(x1v, y1v, z1v, x2v, y2, z2) : lp_line_vol(x1v, y1v, z1v, x2v, y2, z2);
+}
+static double d_surf_surf_para(const double *x1v, const double *y1v, double z1, const double *x2v,
+                               const double *y2v, double z2)
+{
+    return g_qs ? qs_surf_surf_para(x1v, y1v, z1, x2v, y2v, z2) : lp_surf_surf_para(x1v, y1v, z1, x2v, y2v, z2);
+}
+/* QS :345-361 pass an interval as the scalar argument y2 and a scalar as
+ * the interval z2v of integ_surf_surf_ortho (:732): Julia raises a
+ * MethodError there, restated as NaN (form 25). */
+static double d_surf_surf_ortho(const double *x1v, const double *y1v, double z1, const double *x2v, double y2,
+                                const double *z2v)
+{
+    if (g_qs) {
+        g_form = 25;
+        return NAN;
+    }
+    return lp_surf_surf_ortho(x1v, y1v, z1, x2v, y2, z2v);
+}
+static double d_vol_surf(const double *x1v, const double *y1v, const double *z1v, const double *x2v,
+                         const double *y2v, double z2)
+{
+    return g_qs ? qs_vol_surf(x1v, y1v, z1v, x2v, y2v, z2) : lp_vol_surf(x1v, y1v, z1v, x2v, y2v, z2);
+}
+
 double oracle_integ_vol_vol(const double *x1v, const double *y1v, const double *z1v,
                             const double *x2v, const double *y2v, const double *z2v)
 {
@@ -631,13 +849,13 @@ double oracle_integ_vol_vol(const double *x1v, const double *y1v, const double *
     } else if (sum_supp == 5) {
         int is_point_v1 = (sx1 + sy1 + sz1 == 3);
         if (is_point_v1) {
-            if (sx2 == 0) integ = V1 * V2 / a2 * lp_line_point(X2, yc2, zc2, xc1, yc1, zc1);
-            else if (sy2 == 0) integ = V1 * V2 / b2 * lp_line_point(Y2, xc2, zc2, yc1, xc1, zc1);
-            else integ = V1 * V2 / c2 * lp_line_point(Z2, xc2, yc2, zc1, xc1, yc1);
+            if (sx2 == 0) integ = V1 * V2 / a2 * d_line_point(X2, yc2, zc2, xc1, yc1, zc1);
+            else if (sy2 == 0) integ = V1 * V2 / b2 * d_line_point(Y2, xc2, zc2, yc1, xc1, zc1);
+            else integ = V1 * V2 / c2 * d_line_point(Z2, xc2, yc2, zc1, xc1, yc1);
         } else {
-            if (sx1 == 0) integ = V1 * V2 / a1 * lp_line_point(X1, yc1, zc1, xc2, yc2, zc2);
-            else if (sy1 == 0) integ = V1 * V2 / b1 * lp_line_point(Y1, xc1, zc1, yc2, xc2, zc2);
-            else integ = V1 * V2 / c1 * lp_line_point(Z1, xc1, yc1, zc2, xc2, yc2);
+            if (sx1 == 0) integ = V1 * V2 / a1 * d_line_point(X1, yc1, zc1, xc2, yc2, zc2);
+            else if (sy1 == 0) integ = V1 * V2 / b1 * d_line_point(Y1, xc1, zc1, yc2, xc2, zc2);
+            else integ = V1 * V2 / c1 * d_line_point(Z1, xc1, yc1, zc2, xc2, yc2);
         }
     } else if (sum_supp == 4) {
         int is_point_v1 = (sx1 + sy1 + sz1 == 3);
@@ -675,66 +893,66 @@ double oracle_integ_vol_vol(const double *x1v, const double *y1v, const double *
             integ = a2 * b2 * c2 * lp_point_vol(xc2, yc2, zc2, X1, Y1, Z1);
         } else if (is_surf_v1) {
             if (sx1 == 1) {
-                if (sx2 == 0) integ = a1 * b2 * c2 * line_surf_ortho(X2, yc2, zc2, xc1, Y1, Z1);
-                else if (sy2 == 0) integ = a1 * a2 * c2 * line_surf_para(Y1, Z1, xc1, Y2, zc2, xc2);
-                else integ = a1 * a2 * b2 * line_surf_para(Z1, Y1, xc1, Z2, yc2, xc2);
+                if (sx2 == 0) integ = a1 * b2 * c2 * d_line_surf_ortho(X2, yc2, zc2, xc1, Y1, Z1);
+                else if (sy2 == 0) integ = a1 * a2 * c2 * d_line_surf_para(Y1, Z1, xc1, Y2, zc2, xc2);
+                else integ = a1 * a2 * b2 * d_line_surf_para(Z1, Y1, xc1, Z2, yc2, xc2);
             } else if (sy1 == 1) {
-                if (sx2 == 0) integ = b1 * b2 * c2 * line_surf_para(X1, Z1, yc1, X2, zc2, yc2);
-                else if (sy2 == 0) integ = b1 * a2 * c2 * line_surf_ortho(Y2, xc2, zc2, yc1, X1, Z1);
-                else integ = b1 * a2 * b2 * line_surf_para(Z1, X1, yc1, Z2, xc2, yc2);
+                if (sx2 == 0) integ = b1 * b2 * c2 * d_line_surf_para(X1, Z1, yc1, X2, zc2, yc2);
+                else if (sy2 == 0) integ = b1 * a2 * c2 * d_line_surf_ortho(Y2, xc2, zc2, yc1, X1, Z1);
+                else integ = b1 * a2 * b2 * d_line_surf_para(Z1, X1, yc1, Z2, xc2, yc2);
             } else {
-                if (sx2 == 0) integ = c1 * b2 * c2 * line_surf_para(X1, Y1, zc1, X2, yc2, zc2);
-                else if (sy2 == 0) integ = c1 * a2 * c2 * line_surf_para(Y1, X1, zc1, Y2, xc2, zc2);
-                else integ = c1 * a2 * b2 * line_surf_ortho(Z2, xc2, yc2, zc1, X1, Y1);
+                if (sx2 == 0) integ = c1 * b2 * c2 * d_line_surf_para(X1, Y1, zc1, X2, yc2, zc2);
+                else if (sy2 == 0) integ = c1 * a2 * c2 * d_line_surf_para(Y1, X1, zc1, Y2, xc2, zc2);
+                else integ = c1 * a2 * b2 * d_line_surf_ortho(Z2, xc2, yc2, zc1, X1, Y1);
             }
         } else {
             if (sx2 == 1) {
-                if (sx1 == 0) integ = a2 * b1 * c1 * line_surf_ortho(X1, yc1, zc1, xc2, Y2, Z2);
-                else if (sy1 == 0) integ = a2 * a1 * c1 * line_surf_para(Y2, Z2, xc2, Y1, zc1, xc1);
-                else integ = a2 * a1 * b1 * line_surf_para(Z2, Y2, xc2, Z1, yc1, xc1);
+                if (sx1 == 0) integ = a2 * b1 * c1 * d_line_surf_ortho(X1, yc1, zc1, xc2, Y2, Z2);
+                else if (sy1 == 0) integ = a2 * a1 * c1 * d_line_surf_para(Y2, Z2, xc2, Y1, zc1, xc1);
+                else integ = a2 * a1 * b1 * d_line_surf_para(Z2, Y2, xc2, Z1, yc1, xc1);
             } else if (sy2 == 1) {
-                if (sx1 == 0) integ = b2 * b1 * c1 * line_surf_para(X2, Z2, yc2, X1, zc1, yc1);
-                else if (sy1 == 0) integ = b2 * a1 * c1 * line_surf_ortho(Y1, xc1, zc1, yc2, X2, Z2);
-                else integ = b2 * a1 * b1 * line_surf_para(Z2, X2, yc2, Z1, xc1, yc1);
+                if (sx1 == 0) integ = b2 * b1 * c1 * d_line_surf_para(X2, Z2, yc2, X1, zc1, yc1);
+                else if (sy1 == 0) integ = b2 * a1 * c1 * d_line_surf_ortho(Y1, xc1, zc1, yc2, X2, Z2);
+                else integ = b2 * a1 * b1 * d_line_surf_para(Z2, X2, yc2, Z1, xc1, yc1);
             } else {
-                if (sx1 == 0) integ = c2 * b1 * c1 * line_surf_para(X2, Y2, zc2, X1, yc1, zc1);
-                else if (sy1 == 0) integ = c2 * a1 * c1 * line_surf_para(Y2, X2, zc2, Y1, xc1, zc1);
-                else integ = c2 * a1 * b1 * line_surf_ortho(Z1, xc1, yc1, zc2, X2, Y2);
+                if (sx1 == 0) integ = c2 * b1 * c1 * d_line_surf_para(X2, Y2, zc2, X1, yc1, zc1);
+                else if (sy1 == 0) integ = c2 * a1 * c1 * d_line_surf_para(Y2, X2, zc2, Y1, xc1, zc1);
+                else integ = c2 * a1 * b1 * d_line_surf_ortho(Z1, xc1, yc1, zc2, X2, Y2);
             }
         }
     } else if (sum_supp == 2) {
         int is_line_v1 = (sx1 + sy1 + sz1 == 2);
         int is_line_v2 = (sx2 + sy2 + sz2 == 2);
         if (is_line_v1) {
-            if (sx1 == 0) integ = b1 * c1 * lp_line_vol(X2, Y2, Z2, X1, yc1, zc1);
-            else if (sy1 == 0) integ = a1 * c1 * lp_line_vol(Y2, X2, Z2, Y1, xc1, zc1);
-            else integ = a1 * b1 * lp_line_vol(Z2, X2, Y2, Z1, xc1, yc1);
+            if (sx1 == 0) integ = b1 * c1 * d_line_vol(X2, Y2, Z2, X1, yc1, zc1);
+            else if (sy1 == 0) integ = a1 * c1 * d_line_vol(Y2, X2, Z2, Y1, xc1, zc1);
+            else integ = a1 * b1 * d_line_vol(Z2, X2, Y2, Z1, xc1, yc1);
         } else if (is_line_v2) {
-            if (sx2 == 0) integ = b2 * c2 * lp_line_vol(X1, Y1, Z1, X2, yc2, zc2);
-            else if (sy2 == 0) integ = a2 * c2 * lp_line_vol(Y1, X1, Z1, Y2, xc2, zc2);
-            else integ = a2 * b2 * lp_line_vol(Z1, X1, Y1, Z2, xc2, yc2);
+            if (sx2 == 0) integ = b2 * c2 * d_line_vol(X1, Y1, Z1, X2, yc2, zc2);
+            else if (sy2 == 0) integ = a2 * c2 * d_line_vol(Y1, X1, Z1, Y2, xc2, zc2);
+            else integ = a2 * b2 * d_line_vol(Z1, X1, Y1, Z2, xc2, yc2);
         } else {
             if (sx1 == 1) {
-                if (sx2 == 1) integ = a1 * a2 * lp_surf_surf_para(Y1, Z1, xc1, Y2, Z2, xc2);
-                else if (sy2 == 1) integ = a1 * b2 * lp_surf_surf_ortho(Z1, Y1, xc1, Z2, yc2, X2);
-                else integ = a1 * c2 * lp_surf_surf_ortho(Y1, Z1, xc1, Y2, zc2, X2);
+                if (sx2 == 1) integ = a1 * a2 * d_surf_surf_para(Y1, Z1, xc1, Y2, Z2, xc2);
+                else if (sy2 == 1) integ = a1 * b2 * d_surf_surf_ortho(Z1, Y1, xc1, Z2, yc2, X2);
+                else integ = a1 * c2 * d_surf_surf_ortho(Y1, Z1, xc1, Y2, zc2, X2);
             } else if (sy1 == 1) {
-                if (sx2 == 1) integ = b1 * a2 * lp_surf_surf_ortho(Z1, X1, yc1, Z2, xc2, Y2);
-                else if (sy2 == 1) integ = b1 * b2 * lp_surf_surf_para(X1, Z1, yc1, X2, Z2, yc2);
-                else integ = b1 * c2 * lp_surf_surf_ortho(X1, Z1, yc1, X2, zc2, Y2);
+                if (sx2 == 1) integ = b1 * a2 * d_surf_surf_ortho(Z1, X1, yc1, Z2, xc2, Y2);
+                else if (sy2 == 1) integ = b1 * b2 * d_surf_surf_para(X1, Z1, yc1, X2, Z2, yc2);
+                else integ = b1 * c2 * d_surf_surf_ortho(X1, Z1, yc1, X2, zc2, Y2);
             } else {
-                if (sx2 == 1) integ = c1 * a2 * lp_surf_surf_ortho(Y1, X1, zc1, Y2, xc2, Z2);
-                else if (sy2 == 1) integ = c1 * b2 * lp_surf_surf_ortho(X1, Y1, zc1, X2, yc2, Z2);
-                else integ = c1 * c2 * lp_surf_surf_para(X1, Y1, zc1, X2, Y2, zc2);
+                if (sx2 == 1) integ = c1 * a2 * d_surf_surf_ortho(Y1, X1, zc1, Y2, xc2, Z2);
+                else if (sy2 == 1) integ = c1 * b2 * d_surf_surf_ortho(X1, Y1, zc1, X2, yc2, Z2);
+                else integ = c1 * c2 * d_surf_surf_para(X1, Y1, zc1, X2, Y2, zc2);
             }
         }
     } else if (sum_supp == 1) {
-        if (sx1 == 1) integ = a1 * lp_vol_surf(Y2, Z2, X2, Y1, Z1, xc1);
-        else if (sy1 == 1) integ = b1 * lp_vol_surf(X2, Z2, Y2, X1, Z1, yc1);
-        else if (sz1 == 1) integ = c1 * lp_vol_surf(X2, Y2, Z2, X1, Y1, zc1);
-        else if (sx2 == 1) integ = a2 * lp_vol_surf(Y1, Z1, X1, Y2, Z2, xc2);
-        else if (sy2 == 1) integ = b2 * lp_vol_surf(X1, Z1, Y1, X2, Z2, yc2);
-        else if (sz2 == 1) integ = c2 * lp_vol_surf(X1, Y1, Z1, X2, Y2, zc2);
+        if (sx1 == 1) integ = a1 * d_vol_surf(Y2, Z2, X2, Y1, Z1, xc1);
+        else if (sy1 == 1) integ = b1 * d_vol_surf(X2, Z2, Y2, X1, Z1, yc1);
+        else if (sz1 == 1) integ = c1 * d_vol_surf(X2, Y2, Z2, X1, Y1, zc1);
+        else if (sx2 == 1) integ = a2 * d_vol_surf(Y1, Z1, X1, Y2, Z2, xc2);
+        else if (sy2 == 1) integ = b2 * d_vol_surf(X1, Z1, Y1, X2, Z2, yc2);
+        else if (sz2 == 1) integ = c2 * d_vol_surf(X1, Y1, Z1, X2, Y2, zc2);
     } else {
         integ = lp_vol_vol(X1, Y1, Z1, X2, Y2, Z2);
     }
@@ -761,6 +979,19 @@ void oracle_lp_row(const double *src, const double *centers, int64_t N,
         double z2v[2] = {jl_round14(cz - sz2 / 2), jl_round14(cz + sz2 / 2)};
         out[n] = 1e-7 / S * oracle_integ_vol_vol(x1v, y1v, z1v, x2v, y2v, z2v);
     }
+}
+
+/* compute_Lp_Voxels_QS.jl:9-42 (the row of compute_rows_Rcc_Lp.jl:28-32):
+ * the same corner rounding (round_ud(., 14), :3-6) and prefactor; entries
+ * without suppression evaluate compute_Lp_Voxels.jl's dispatcher (the QS
+ * dispatcher calls Integ_vol_vol_fft there, :418), the others the QS forms. */
+void oracle_lp_row_qs(const double *src, const double *centers, int64_t N,
+                      double sx, double sy, double sz, double sx2, double sy2, double sz2,
+                      int dc, double *out)
+{
+    g_qs = 1;
+    oracle_lp_row(src, centers, N, sx, sy, sz, sx2, sy2, sz2, dc, out);
+    g_qs = 0;
 }
 
 /* ---- P classifier: compute_row_P_sup.jl:106-394 ----------------------- */
@@ -955,6 +1186,15 @@ void oracle_lp_row_forms(const double *src, const double *centers, int64_t N,
     }
 }
 
+void oracle_lp_row_qs_forms(const double *src, const double *centers, int64_t N,
+                            double sx, double sy, double sz, double sx2, double sy2, double sz2,
+                            int dc, double *out, int32_t *form)
+{
+    g_qs = 1;
+    oracle_lp_row_forms(src, centers, N, sx, sy, sz, sx2, sy2, sz2, dc, out, form);
+    g_qs = 0;
+}
+
 void oracle_p_row_forms(const double *src, const double *centers, int64_t N,
                         double sx, double sy, double sz, int caser, int caser2, double *out, int32_t *form)
 {
@@ -1002,6 +1242,12 @@ double oracle_lp_self(double l, double W, double T)
  *    13 p_surf_surf_ortho(x1v,y1v,z1,x2v,y2,z2v) 10
  *    14 p_line_surf_ortho(x1v,y1,z1,x2,y2v,z2v) 8
  *    15 line_line_parall (P copy)             8
+ *    19 qs_line_line_sp(x1v,x2v)              4
+ *    20 qs_line_point                         6
+ *    21 qs_line_surf_ortho                    8
+ *    22 qs_line_surf_para                     8
+ *    23 qs_line_vol                           10
+ *    24 qs_vol_surf                           11
  */
 double oracle_closed_form(int id, const double *a)
 {
@@ -1022,6 +1268,12 @@ double oracle_closed_form(int id, const double *a)
     case 13: return p_surf_surf_ortho(a, a + 2, a[4], a + 5, a[7], a + 8);
     case 14: return p_line_surf_ortho(a, a[2], a[3], a[4], a + 5, a + 7);
     case 15: return line_line_parall(a, a[2], a[3], a + 4, a[6], a[7], 1);
+    case 19: return qs_line_line_sp(a, a + 2);
+    case 20: return qs_line_point(a, a[2], a[3], a[4], a[5], a[6]);
+    case 21: return qs_line_surf_ortho(a, a[2], a[3], a[4], a + 5, a + 7);
+    case 22: return qs_line_surf_para(a, a + 2, a[4], a + 5, a[7], a[8]);
+    case 23: return qs_line_vol(a, a + 2, a + 4, a + 6, a[8], a[9]);
+    case 24: return qs_vol_surf(a, a + 2, a + 4, a + 6, a + 8, a[10]);
     }
     return NAN;
 }

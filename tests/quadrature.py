@@ -85,8 +85,9 @@ def _nodes(piece, n, graded):
     return d, w * wd
 
 
-def box_integral(box1, box2, n=24):
-    """\\int_{box1} \\int_{box2} 1/|r1-r2| for degenerate-or-not boxes.
+def box_integral(box1, box2, n=24, beta=0.0):
+    """\\int_{box1} \\int_{box2} 1/|r1-r2| for degenerate-or-not boxes, or
+    with beta != 0 the retarded kernel exp(-j beta |r1-r2|) / |r1-r2|.
 
     box = ((x_lo, x_hi), (y_lo, y_hi), (z_lo, z_hi)); lo == hi collapses an
     axis (surface, segment or point)."""
@@ -98,7 +99,8 @@ def box_integral(box1, box2, n=24):
     cont = [_split_at_zero(p) if graded else p for p in cont]
     total = 0.0
     if not cont:
-        return 1.0 / np.sqrt(d0sq)
+        r = np.sqrt(d0sq)
+        return np.exp(-1j * beta * r) / r if beta else 1.0 / r
     import itertools
     for combo in itertools.product(*cont):
         grids = [_nodes(pc, n, graded) for pc in combo]
@@ -109,6 +111,9 @@ def box_integral(box1, box2, n=24):
         for wi in ws:
             w = w * wi
         with np.errstate(divide="ignore", invalid="ignore"):
-            f = np.where(r2 > 0, w / np.sqrt(r2), 0.0)
+            r = np.sqrt(r2)
+            f = np.where(r2 > 0, w / r, 0.0)
+            if beta:
+                f = f * np.exp(-1j * beta * r)
         total += f.sum()
     return total
