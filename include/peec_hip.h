@@ -83,8 +83,36 @@ int peec_kernels_build(peec_ctx *ctx, const double origin[3], int Nx, int Ny, in
                        double sx, double sy, double sz, double esc_Lp, double esc_P);
 /* Copy one scaled kernel row back (kind 0..2 = Lp x,y,z; 3..8 = P 11, 22,
  * 33, 12, 13, 23).  out holds the compact grid (x fastest); *n receives its
- * length.  Pass out = NULL to query the length. */
+ * length.  Pass out = NULL to query the length.  kind + 16 reads the
+ * retarded path's static row (no escaling), kind + 32 its centre distances
+ * (after peec_kernels_build_rcc). */
 int peec_kernel_row(peec_ctx *ctx, int kind, double *out, int64_t *n);
+
+/* ---- retarded (Rcc) kernels ---------------------------------------------- */
+/* compute_FFT_mutual_coupling_mats(..., QS_Rcc_FW=2, ...)
+ *   = compute_rows_Rcc_P + compute_rows_Rcc_Lp
+ *                         compute_FFT_mutual_coupling_mats.jl:18-27,
+ *                         compute_rows_Rcc_Lp.jl:1-105, compute_rows_Rcc_P.jl:1-64
+ * Builds the quasi-static kernels (as peec_kernels_build) and, per kernel,
+ * the static row of the retarded path -- the Lp rows from the closed forms
+ * of compute_Lp_Voxels_QS.jl, self entry zeroed, no escaling -- and its
+ * centre distances Rcc (compute_Voxels_Rcc.jl, compute_P_vox_Rcc.jl).
+ * From then on every convolution uses the retarded spectra, which
+ * peec_rcc_spectra builds per frequency.  Needs one frequency slot. */
+int peec_kernels_build_rcc(peec_ctx *ctx, const double origin[3], int Nx, int Ny, int Nz,
+                           double sx, double sy, double sz, double esc_Lp, double esc_P);
+
+/* compute_Circulant_Lp_Rcc(rows_Lp, escalings, f) +
+ * compute_Circulant_P_sup_Rcc(rows_P, escalings, f)
+ *                         FFT_solver_QS_S_type.jl:61-64,
+ *                         compute_Circulant_Lp_Rcc.jl:1-25,
+ *                         compute_Circulant_P_sup_Rcc.jl:1-24,
+ *                         build_row_vox_Rcc.jl:1-10
+ * The complex spectra of escaling * row .* exp(-j beta Rcc), beta = 2 pi f
+ * sqrt(eps0 mu0), at frequency freq_hz (Hz, unscaled), for every kernel
+ * (both directions of a half-cell-reduced off-diagonal P block).  Call
+ * after peec_operator_set, and again for each frequency of the sweep. */
+int peec_rcc_spectra(peec_ctx *ctx, double freq_hz);
 
 /* ---- operator (per geometry) -------------------------------------------- */
 /* The sparse data of ComputeMatrixVectorFTT (compute_Matrix_vector_fft.jl:1-81):

@@ -181,7 +181,7 @@ end
 # build_center_air_Voxels.jl: centres at origin + s .* (i - 1/2), Lp centres
 # half a cell further along their axis).
 function compute_FFT_mutual_coupling_mats(circulant_centers, escalings, Nx, Ny, Nz, QS_Rcc_FW, id, chan)
-    QS_Rcc_FW == 1 || error("libpeec_hip builds the quasi-static (QS_Rcc_FW = 1) spectra only")
+    QS_Rcc_FW in (1, 2) || error("libpeec_hip builds the quasi-static (1) and retarded Rcc (2) kernels")
     ctx = peec_ctx(id); peec_sync_stop!(id)
     sx, sy, sz = circulant_centers["sx"], circulant_centers["sy"], circulant_centers["sz"]
     half = (sx / 2, sy / 2, sz / 2)
@@ -189,7 +189,17 @@ function compute_FFT_mutual_coupling_mats(circulant_centers, escalings, Nx, Ny, 
     axis === nothing && error("compute_FFT_mutual_coupling_mats: a 1 x 1 x 1 grid has no edges")
     c = circulant_centers[("Lpx", "Lpy", "Lpz")[axis]][1, :]
     origin = [c[d] - half[d] - (d == axis ? half[d] : 0.0) for d in 1:3]
-    peec_kernels_build(ctx, origin, Nx, Ny, Nz, sx, sy, sz, escalings["Lp"], escalings["P"])
+    if QS_Rcc_FW == 1
+        peec_kernels_build(ctx, origin, Nx, Ny, Nz, sx, sy, sz, escalings["Lp"], escalings["P"])
+    else
+        # compute_rows_Rcc_P + compute_rows_Rcc_Lp: static rows and distances;
+        # the spectra follow per frequency (peec_rcc_spectra)
+        o = Float64[origin[1], origin[2], origin[3]]
+        rc = GC.@preserve o ccall((:peec_kernels_build_rcc, LIBPEEC), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Cint, Cint, Cint, Float64, Float64, Float64, Float64, Float64),
+            ctx, o, Nx, Ny, Nz, sx, sy, sz, escalings["Lp"], escalings["P"])
+        peec_check(rc, "peec_kernels_build_rcc")
+    end
     PEEC_CURRENT[] = ctx
     h = PeecSpectra(ctx, id)
     return h, h
@@ -219,7 +229,7 @@ end
 function FFT_solver_QS_S_type(freq, escalings, incidence_selection, FFTCP, FFTCLp, diagonals, ports,
                               ports_scatter_value, lumped_elements, expansions, GMRES_settings, Zs_info, QS_Rcc_FW,
                               id, chan, commentsEnabled::Bool)
-    QS_Rcc_FW == 1 || error("libpeec_hip runs the quasi-static (QS_Rcc_FW = 1) solver only")
+    QS_Rcc_FW in (1, 2) || error("libpeec_hip runs the quasi-static (1) and retarded Rcc (2) solvers")
     FFTCP isa PeecSpectra || error("FFT_solver_QS_S_type: FFTCP must come from compute_FFT_mutual_coupling_mats")
     ctx = FFTCP.ctx; PEEC_CURRENT[] = ctx
     A = incidence_selection["A"]; Gamma = incidence_selection["Gamma"]
@@ -236,6 +246,10 @@ function FFT_solver_QS_S_type(freq, escalings, incidence_selection, FFTCP, FFTCL
         peec_sync_stop!(id)
         is_stop_requested(id) && return nothing
         wk = 2pi * fs[k]
+        if QS_Rcc_FW == 2   # compute_Circulant_Lp_Rcc / compute_Circulant_P_sup_Rcc (:61-64)
+            peec_check(ccall((:peec_rcc_spectra, LIBPEEC), Cint, (Ptr{Cvoid}, Float64), ctx, fs[k] / escalings["freq"]),
+                       "peec_rcc_spectra")
+        end
         Yle = build_Yle_FFT(lumped_elements, [], ports, escalings, n, wk / escalings["freq"], ports_scatter_value)
         DZ = peec_edge_impedance(diagonals, escalings, Zs_info, wk)
         peec_frequency_set(ctx, wk, DZ, Yle)

@@ -526,8 +526,10 @@ def s2y(S, Zo):
 
 
 def fft_solver_qs(freq, mesh, spec, inner_iter, tol, R0=50.0, stop=lambda: False, record=None, order=None):
-    """FFT_solver_QS_S_type.jl:1-234 (QS_Rcc_FW = 1).  `order`: elimination
-    order for the LU of SS (see lu)."""
+    """FFT_solver_QS_S_type.jl:1-234.  `spec`: the kernel spectra
+    (QS_Rcc_FW = 1), or a function of the frequency in Hz returning them
+    (QS_Rcc_FW = 2: the retarded spectra rebuilt per frequency, :61-64).
+    `order`: elimination order for the LU of SS (see lu)."""
     esc = mesh.escalings
     freq = np.asarray(freq, dtype=np.float64) * esc["freq"]
     tol = np.broadcast_to(np.asarray(tol, dtype=np.float64), freq.shape)
@@ -536,11 +538,13 @@ def fft_solver_qs(freq, mesh, spec, inner_iter, tol, R0=50.0, stop=lambda: False
     P = mesh.port_nodes.shape[0]
     S = np.zeros((P, P, len(freq)), dtype=np.complex128)
     Vrest = np.zeros((m + n + ns, P), dtype=np.complex128)
-    op = Operator(mesh, spec)
+    op = Operator(mesh, None if callable(spec) else spec)
     is_ = np.zeros(n)
     for k in range(len(freq)):
         if stop():
             return None
+        if callable(spec):
+            op.spec = spec(freq[k] / esc["freq"])
         Yle = build_Yle(mesh.lumped_elements, mesh.port_nodes, esc["Yle"], n, w[k] / esc["freq"], R0)
         DZ = frequency_terms(mesh, w[k])
         op.set_frequency(w[k], DZ, Yle, order=order)

@@ -54,6 +54,8 @@ def lib():
         d = ctypes.c_double
         L.peec_kernels_build.argtypes = [vp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, d, d, d, d, d]
         L.peec_kernel_row.argtypes = [vp, ctypes.c_int, dp, i64p]
+        L.peec_kernels_build_rcc.argtypes = [vp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, d, d, d, d, d]
+        L.peec_rcc_spectra.argtypes = [vp, d]
         L.peec_operator_set.argtypes = [vp, i64, i64, i64, ctypes.POINTER(CscF64), ctypes.POINTER(CscF64),
                                         ctypes.POINTER(CscF64), ctypes.POINTER(CscF64), dp]
         L.peec_frequency_set.argtypes = [vp, d, c128p, ctypes.POINTER(CscC128)]

@@ -103,6 +103,16 @@ struct Kernel {
     // for the applications that use it): even kernel (real) and half-cell
     // reduction (complex)
     DevBuf<float> fspec32, pfspec32;
+    // Retarded (Rcc) kernels: the static row without escaling
+    // (compute_rows_Rcc_Lp.jl / compute_rows_Rcc_P.jl) and the centre
+    // distances Rcc, once per geometry; per frequency the complex spectra of
+    // escaling * row .* exp(-j beta Rcc) on the full embedding L, and for a
+    // half-cell reduction one per direction ([0]: parity offset delta,
+    // [1]: -delta; the reduced kernel of a complex row is not conjugate
+    // symmetric), 1/G folded in.
+    DevBuf<double> rrow, rdist;
+    DevBuf<double2_t> rspec64, rpspec64[2];
+    DevBuf<float> rspec32, rpspec32[2];
 };
 
 // One FFT-convolution application of the matvec: scatter unknowns of one
@@ -278,6 +288,11 @@ struct peec_ctx {
     bool kernels_ready = false;
     Kernel kern[9];
     void release_kernels();
+    // retarded (Rcc) kernels (peec_kernels_build_rcc): the convolutions use
+    // the complex spectra of frequency rcc_freq once peec_rcc_spectra has
+    // built them (rcc_ready)
+    bool rcc = false, rcc_ready = false;
+    double rcc_freq = 0.0, rcc_esc_Lp = 1.0, rcc_esc_P = 1.0;
 
     // ---- FFT work buffers (complex, sized for the largest application)
     DevBuf<double2_t> w1, w2;

@@ -104,9 +104,18 @@ void peec_ctx::release_kernels()
         k.poly = false;
         k.pspec64.reset();
         k.pspec32.reset();
+        k.rrow.reset();
+        k.rdist.reset();
+        k.rspec64.reset();
+        k.rspec32.reset();
+        for (int d = 0; d < 2; d++) {
+            k.rpspec64[d].reset();
+            k.rpspec32[d].reset();
+        }
     }
     fft_tune.clear();
     kernels_ready = false;
+    rcc = rcc_ready = false;
 }
 
 // ------------------------------------------------------------------ device
@@ -649,6 +658,57 @@ __global__ void k_embed_poly(const double *__restrict__ row, int nfx, int nfy, i
     out[i] = {v, 0.0};
 }
 
+// escaling * row .* exp(-j beta R) at compact index j (build_row_vox_Rcc.jl:
+// row times the unit phasor, then the escaling, compute_Circulant_*_Rcc.jl)
+__device__ __forceinline__ double2_t rcc_value(const double *__restrict__ row, const double *__restrict__ dist,
+                                               int64_t j, double mbeta, double esc)
+{
+    double s, c;
+    sincos(mbeta * dist[j], &s, &c);
+    const double r = row[j];
+    return {esc * (r * c), esc * (r * s)};
+}
+
+// Even-symmetric embedding of a retarded (complex) row, as k_embed.
+__global__ void k_embed_rcc(const double *__restrict__ row, const double *__restrict__ dist, double mbeta, double esc,
+                            int nx, int ny, int nz, int Lx, int Ly, int Lz, double2_t *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t G = (int64_t)Lx * Ly * Lz;
+    if (i >= G) return;
+    const int x = (int)(i % Lx);
+    const int64_t r = i / Lx;
+    const int y = (int)(r % Ly);
+    const int z = (int)(r / Ly);
+    const int sx = x < nx ? x : (Lx - x < nx ? Lx - x : -1);
+    const int sy = y < ny ? y : (Ly - y < ny ? Ly - y : -1);
+    const int sz = z < nz ? z : (Lz - z < nz ? Lz - z : -1);
+    double2_t v = {0.0, 0.0};
+    if (sx >= 0 && sy >= 0 && sz >= 0) v = rcc_value(row, dist, sx + (int64_t)nx * (sy + (int64_t)ny * sz), mbeta, esc);
+    out[i] = v;
+}
+
+// Half-cell reduction of a retarded row for parity offset (dx, dy, dz), as
+// k_embed_poly.
+__global__ void k_embed_poly_rcc(const double *__restrict__ row, const double *__restrict__ dist, double mbeta,
+                                 double esc, int nfx, int nfy, int nfz, int hx, int hy, int hz, int dx, int dy, int dz,
+                                 int nbx, int nby, int nbz, int Lx, int Ly, int Lz, double2_t *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t G = (int64_t)Lx * Ly * Lz;
+    if (i >= G) return;
+    const int x = (int)(i % Lx);
+    const int64_t r = i / Lx;
+    const int y = (int)(r % Ly);
+    const int z = (int)(r / Ly);
+    const int fx = poly_offset(x, Lx, nbx, hx, dx, nfx);
+    const int fy = poly_offset(y, Ly, nby, hy, dy, nfy);
+    const int fz = poly_offset(z, Lz, nbz, hz, dz, nfz);
+    double2_t v = {0.0, 0.0};
+    if (fx >= 0 && fy >= 0 && fz >= 0) v = rcc_value(row, dist, fx + (int64_t)nfx * (fy + (int64_t)nfy * fz), mbeta, esc);
+    out[i] = v;
+}
+
 __global__ void k_spec_extract_c(const double2_t *__restrict__ g, int64_t G, double scale,
                                  double2_t *__restrict__ s64, float *__restrict__ s32)
 {
@@ -1154,6 +1214,113 @@ int peec_kernels_build_impl(peec_ctx *ctx, const double origin[3], int Nx, int N
     return PEEC_OK;
 }
 
+// ---- retarded (Rcc) kernels
+//
+// compute_rows_Rcc_Lp.jl / compute_rows_Rcc_P.jl (compute_FFT_mutual_coupling_mats.jl
+// with QS_Rcc_FW = 2): per geometry the nine static rows (the Lp rows from
+// compute_Lp_Voxels_QS.jl's closed forms, no escaling, self entry zeroed)
+// and their centre distances.  The quasi-static rows and spectra are built
+// as well: the operator set-up reads them.
+int peec_kernels_build_rcc_impl(peec_ctx *ctx, const double origin[3], int Nx, int Ny, int Nz, double sx, double sy,
+                                double sz, double esc_Lp, double esc_P)
+{
+    int rc = peec_kernels_build_impl(ctx, origin, Nx, Ny, Nz, sx, sy, sz, esc_Lp, esc_P);
+    if (rc) return rc;
+    for (int k = 0; k < 9; k++) {
+        Kernel &K = ctx->kern[k];
+        const int64_t g = K.compact();
+        if (g <= 0) continue;
+        if (!K.rrow.alloc((size_t)g) || !K.rdist.alloc((size_t)g)) {
+            peec_set_error("kernels_build_rcc: out of device memory");
+            return PEEC_ERR_NOMEM;
+        }
+        int64_t n;
+        int dims[3];
+        if ((rc = peec_grid_row_device(ctx->stream, k, origin, Nx, Ny, Nz, sx, sy, sz, 1.0, K.rrow.p, &n, dims, 1)) ||
+            (rc = peec_grid_row_device(ctx->stream, k, origin, Nx, Ny, Nz, sx, sy, sz, 1.0, K.rdist.p, &n, dims, 2)))
+            return rc;
+    }
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->rcc = true;
+    ctx->rcc_ready = false;
+    ctx->rcc_esc_Lp = esc_Lp;
+    ctx->rcc_esc_P = esc_P;
+    return PEEC_OK;
+}
+
+// compute_Circulant_Lp_Rcc.jl / compute_Circulant_P_sup_Rcc.jl at one
+// frequency (Hz): the complex spectrum of escaling * row .* exp(-j beta Rcc)
+// per kernel, beta = 2 pi f sqrt(eps0 mu0) (build_row_vox_Rcc.jl:3-5), on
+// the full embedding and, for a kernel the operator applies through a
+// half-cell reduction, on the reduced box for both directions.
+int peec_rcc_spectra_impl(peec_ctx *ctx, double freq_hz)
+{
+    if (!ctx->rcc || !ctx->kernels_ready) {
+        peec_set_error("peec_rcc_spectra: build the retarded kernels first (peec_kernels_build_rcc)");
+        return PEEC_ERR_STATE;
+    }
+    if (!ctx->op_ready) {
+        peec_set_error("peec_rcc_spectra: set the operator first (peec_operator_set)");
+        return PEEC_ERR_STATE;
+    }
+    const double pi = 3.141592653589793, eps0 = 8.854187816997944e-12, mu0 = 4 * pi * 1e-7;
+    const double beta = 2 * pi * freq_hz * sqrt(eps0 * mu0);
+    int64_t Gmax = 0;
+    for (int k = 0; k < 9; k++) {
+        const Kernel &K = ctx->kern[k];
+        if (K.compact() <= 0) continue;
+        Gmax = std::max(Gmax, K.embedded());
+        if (K.poly) Gmax = std::max(Gmax, (int64_t)K.pL[0] * K.pL[1] * K.pL[2]);
+    }
+    DevBuf<double2_t> grid;
+    if (!grid.alloc((size_t)std::max<int64_t>(Gmax, 1))) {
+        peec_set_error("peec_rcc_spectra: cannot allocate a %lld-point work grid", (long long)Gmax);
+        return PEEC_ERR_NOMEM;
+    }
+    int rc;
+    for (int k = 0; k < 9; k++) {
+        Kernel &K = ctx->kern[k];
+        if (K.compact() <= 0) continue;
+        const double esc = k <= 2 ? ctx->rcc_esc_Lp : ctx->rcc_esc_P;
+        const int64_t G = K.embedded();
+        if (!K.rspec64.alloc((size_t)G) || !K.rspec32.alloc((size_t)(2 * G))) {
+            peec_set_error("peec_rcc_spectra: out of device memory");
+            return PEEC_ERR_NOMEM;
+        }
+        const unsigned nb = (unsigned)((G + 255) / 256);
+        hipLaunchKernelGGL(k_embed_rcc, dim3(nb), dim3(256), 0, ctx->stream, K.rrow.p, K.rdist.p, -beta, esc, K.n[0],
+                           K.n[1], K.n[2], K.L[0], K.L[1], K.L[2], grid.p);
+        PEEC_HIP_TRY(hipGetLastError());
+        if ((rc = fft3_forward_inplace(ctx, grid.p, K.L))) return rc;
+        hipLaunchKernelGGL(k_spec_extract_c, dim3(nb), dim3(256), 0, ctx->stream, grid.p, G, 1.0 / (double)G,
+                           K.rspec64.p, K.rspec32.p);
+        PEEC_HIP_TRY(hipGetLastError());
+        if (!K.poly) continue;
+        const int64_t Gp = (int64_t)K.pL[0] * K.pL[1] * K.pL[2];
+        const unsigned nbp = (unsigned)((Gp + 255) / 256);
+        for (int dir = 0; dir < 2; dir++) {
+            const int sg = dir == 0 ? 1 : -1;
+            if (!K.rpspec64[dir].alloc((size_t)Gp) || !K.rpspec32[dir].alloc((size_t)(2 * Gp))) {
+                peec_set_error("peec_rcc_spectra: out of device memory");
+                return PEEC_ERR_NOMEM;
+            }
+            hipLaunchKernelGGL(k_embed_poly_rcc, dim3(nbp), dim3(256), 0, ctx->stream, K.rrow.p, K.rdist.p, -beta,
+                               esc, K.n[0], K.n[1], K.n[2], kHalfAxes[k][0], kHalfAxes[k][1], kHalfAxes[k][2],
+                               sg * K.delta[0], sg * K.delta[1], sg * K.delta[2], K.pn[0], K.pn[1], K.pn[2], K.pL[0],
+                               K.pL[1], K.pL[2], grid.p);
+            PEEC_HIP_TRY(hipGetLastError());
+            if ((rc = fft3_forward_inplace(ctx, grid.p, K.pL))) return rc;
+            hipLaunchKernelGGL(k_spec_extract_c, dim3(nbp), dim3(256), 0, ctx->stream, grid.p, Gp, 1.0 / (double)Gp,
+                               K.rpspec64[dir].p, K.rpspec32[dir].p);
+            PEEC_HIP_TRY(hipGetLastError());
+        }
+    }
+    PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->rcc_ready = true;
+    ctx->rcc_freq = freq_hz;
+    return PEEC_OK;
+}
+
 int64_t conv_work_elems(const ConvApp &app);
 
 // The five pass descriptors of one convolution application for nrhs
@@ -1172,7 +1339,7 @@ static int conv_passes(peec_ctx *ctx, const ConvApp &app, bool fp32, const doubl
     const int nx = app.n[0], ny = app.n[1], nz = app.n[2];
     const int64_t Lx = app.L[0], Ly = app.L[1], Lz = app.L[2];
     const int64_t s1 = Lx * ny * nz, s2 = Lx * Ly * nz;  // per-rhs grid sizes
-    const double e = fp32 ? 8.0 : 16.0, es = (fp32 ? 4.0 : 8.0) * (app.poly ? 2.0 : 1.0);
+    const double e = fp32 ? 8.0 : 16.0, es = (fp32 ? 4.0 : 8.0) * (app.poly || ctx->rcc ? 2.0 : 1.0);
     const double g = (double)nx * ny * nz, G1 = (double)s1 * nrhs, G2 = (double)s2 * nrhs;
     int rc;
     // 1: x forward with scatter; lines (y, z)
@@ -1208,11 +1375,21 @@ static int conv_passes(peec_ctx *ctx, const ConvApp &app, bool fp32, const doubl
     d3.nrhs = nrhs;
     d3.in_rhs = s2;
     d3.out_rhs = s2;
-    if (app.poly)
-        d3.spec = fp32 ? (const void *)K.pspec32.p : (const void *)K.pspec64.p;
-    else
-        d3.spec = fp32 ? (const void *)K.spec32.p : (const void *)K.spec64.p;
-    d3.spec_conj = app.conj ? 1 : 0;
+    if (ctx->rcc) {
+        // retarded kernels: complex spectra, one per direction of a reduction
+        const int dir = app.conj ? 1 : 0;
+        if (app.poly)
+            d3.spec = fp32 ? (const void *)K.rpspec32[dir].p : (const void *)K.rpspec64[dir].p;
+        else
+            d3.spec = fp32 ? (const void *)K.rspec32.p : (const void *)K.rspec64.p;
+        d3.spec_conj = 0;
+    } else {
+        if (app.poly)
+            d3.spec = fp32 ? (const void *)K.pspec32.p : (const void *)K.pspec64.p;
+        else
+            d3.spec = fp32 ? (const void *)K.spec32.p : (const void *)K.spec64.p;
+        d3.spec_conj = app.conj ? 1 : 0;
+    }
     d3.spec_sb = 1;
     d3.spec_s2 = Lx;
     d3.spec_sj = Lx * Ly;
@@ -1344,7 +1521,12 @@ static FftChoice tuned_choice(const peec_ctx *ctx, int kernel, int pass, bool fp
 
 static const int kProfKind[5] = {PK_FFT_X_FWD, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV};
 static const int kPassMode[5] = {0, 0, 2, 1, 1};
-static int pass_mode(const ConvApp &app, int p) { return p == 2 && app.poly ? 3 : kPassMode[p]; }
+// the z pass multiplies by a complex spectrum for a half-cell reduction and
+// for every retarded kernel
+static int pass_mode(const peec_ctx *ctx, const ConvApp &app, int p)
+{
+    return p == 2 && (app.poly || ctx->rcc) ? 3 : kPassMode[p];
+}
 
 // Tuning-table passes of the fused pipeline: 5 x forward (plane output),
 // 6 the fused y-z pass (variant 0: use the fused pipeline, 1: the five
@@ -1364,7 +1546,7 @@ static int default_yz_rpb(const PassDesc &d)
 // block's LDS stages and barriers then leave HBM idle.
 static bool use_fused(const peec_ctx *ctx, const ConvApp &app, bool fp32, int nrhs)
 {
-    if (!yz_fits(app, fp32)) return false;
+    if (ctx->rcc || !yz_fits(app, fp32)) return false;  // the fused pass reads the static spectra
     const Kernel &K = ctx->kern[app.kernel];
     if (app.poly ? !K.pfspec32.p : !K.fspec32.p) return false;
     return tuned_choice(ctx, app.kernel, kPassYz, fp32, nrhs).variant == 0;
@@ -1393,7 +1575,7 @@ static int conv_apply_fused(peec_ctx *ctx, const ConvApp &app, const double2_t *
             PassDesc &y = cp.d[1];
             y.rpb = ch[1].variant >= 0 ? ch[1].rpb : default_yz_rpb(y);
             const int pj = ctx->prof_begin(PK_FFT_YZ_CONV, cp.bytes[1]);
-            if ((rc = launch_yz(ctx, pass_mode(app, 2), y, cp.src[1], cp.dst[1], app.L[2]))) return rc;
+            if ((rc = launch_yz(ctx, pass_mode(ctx, app, 2), y, cp.src[1], cp.dst[1], app.L[2]))) return rc;
             ctx->prof_end(pj);
         }
     }
@@ -1408,6 +1590,10 @@ int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *ve
                int nrhs, int64_t ld, void *w1, void *w2)
 {
     if (app.compact() <= 0 || app.nnz_in == 0 || app.nnz_out == 0) return PEEC_OK;
+    if (ctx->rcc && !ctx->rcc_ready) {
+        peec_set_error("retarded kernels: build the spectra of a frequency first (peec_rcc_spectra)");
+        return PEEC_ERR_STATE;
+    }
     if (use_fused(ctx, app, fp32, nrhs)) return conv_apply_fused(ctx, app, vec_in, vec_out, nrhs, ld, w1);
     FftChoice ch[5];
     int style[5];
@@ -1424,7 +1610,7 @@ int conv_apply(peec_ctx *ctx, const ConvApp &app, bool fp32, const double2_t *ve
         const int v = ch[p].variant >= 0 ? ch[p].variant : (reg_ok ? kRegVariant : default_variant(d.L, fp32));
         d.rpb = ch[p].variant >= 0 ? ch[p].rpb : default_rpb(d, v == kRegVariant ? 0 : std::max(v, 0));
         const int pi = ctx->prof_begin(kProfKind[p], cp.bytes[p]);
-        if ((rc = launch_pass(ctx, fp32, pass_mode(app, p), d, cp.src[p], cp.dst[p], v))) return rc;
+        if ((rc = launch_pass(ctx, fp32, pass_mode(ctx, app, p), d, cp.src[p], cp.dst[p], v))) return rc;
         ctx->prof_end(pi);
     }
     return PEEC_OK;
@@ -1608,7 +1794,7 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
             for (int p = 0; p < 5 && rc == PEEC_OK; p++) {
                 FftChoice best;
                 float tbest = 1e30f;
-                if ((rc = tune_pass(ctx, fp32, pass_mode(app, p), cp.d[p], cp.src[p], cp.dst[p], cp.bytes[p], p == 2,
+                if ((rc = tune_pass(ctx, fp32, pass_mode(ctx, app, p), cp.d[p], cp.src[p], cp.dst[p], cp.bytes[p], p == 2,
                                     verbose, app.kernel, p, ea, eb, &best, &tbest)))
                     break;
                 ctx->fft_tune[FftTuneKey{app.kernel, p, fp32 ? 1 : 0, nrhs}] = best;
@@ -1618,7 +1804,7 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
             float tsum = t5;
             // the fused pipeline (x forward into planes, fused y-z, x inverse)
             const Kernel &K = ctx->kern[app.kernel];
-            if (yz_fits(app, fp32) && (app.poly ? K.pfspec32.p : K.fspec32.p)) {
+            if (!ctx->rcc && yz_fits(app, fp32) && (app.poly ? K.pfspec32.p : K.fspec32.p)) {
                 const int st3[3] = {0, 0, 0};
                 ConvPasses cf;
                 if ((rc = conv_passes_fused(ctx, app, vin.p, vout.p, nrhs, N, ctx->w1.p, st3, &cf))) break;
@@ -1633,7 +1819,7 @@ extern "C" int peec_fft_tune(peec_ctx *ctx, int nrhs, int precisions, double *re
                     PassDesc d = cf.d[1];
                     d.rpb = r;
                     float t;
-                    if ((rc = time_yz(ctx, pass_mode(app, 2), d, cf.src[1], cf.dst[1], app.L[2], 3, ea, eb, &t))) break;
+                    if ((rc = time_yz(ctx, pass_mode(ctx, app, 2), d, cf.src[1], cf.dst[1], app.L[2], 3, ea, eb, &t))) break;
                     if (verbose)
                         fprintf(stderr, "peec_fft_tune: kernel %d fused y-z Ly %d nz %d Lz %d rpb %d: %.1f us, %.0f GB/s\n",
                                 app.kernel, d.L, d.n2, app.L[2], r, 1e3 * t, cf.bytes[1] / (1e6 * t));
@@ -1739,20 +1925,48 @@ extern "C" int peec_kernels_build(peec_ctx *ctx, const double origin[3], int Nx,
     return peec_kernels_build_impl(ctx, origin, Nx, Ny, Nz, sx, sy, sz, esc_Lp, esc_P);
 }
 
+extern "C" int peec_kernels_build_rcc(peec_ctx *ctx, const double origin[3], int Nx, int Ny, int Nz, double sx,
+                                      double sy, double sz, double esc_Lp, double esc_P)
+{
+    if (!ctx || !origin || Nx < 1 || Ny < 1 || Nz < 1 || !(sx > 0) || !(sy > 0) || !(sz > 0)) {
+        peec_set_error("peec_kernels_build_rcc: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    if (ctx->nslots > 1) {
+        peec_set_error("peec_kernels_build_rcc: the retarded spectra are per frequency; use one frequency slot");
+        return PEEC_ERR_STATE;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    return peec_kernels_build_rcc_impl(ctx, origin, Nx, Ny, Nz, sx, sy, sz, esc_Lp, esc_P);
+}
+
+extern "C" int peec_rcc_spectra(peec_ctx *ctx, double freq_hz)
+{
+    if (!ctx || !(freq_hz >= 0) || !std::isfinite(freq_hz)) {
+        peec_set_error("peec_rcc_spectra: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    return peec_rcc_spectra_impl(ctx, freq_hz);
+}
+
 extern "C" int peec_kernel_row(peec_ctx *ctx, int kind, double *out, int64_t *n)
 {
-    if (!ctx || kind < 0 || kind > 8 || !n) {
+    const int variant = kind >= 0 ? kind / 16 : -1;
+    kind = kind >= 0 ? kind % 16 : kind;
+    if (!ctx || kind < 0 || kind > 8 || variant < 0 || variant > 2 || !n) {
         peec_set_error("peec_kernel_row: invalid argument");
         return PEEC_ERR_ARG;
     }
-    if (!ctx->kernels_ready) {
-        peec_set_error("peec_kernel_row: call peec_kernels_build first");
+    if (!ctx->kernels_ready || (variant > 0 && !ctx->rcc)) {
+        peec_set_error("peec_kernel_row: call peec_kernels_build%s first", variant > 0 ? "_rcc" : "");
         return PEEC_ERR_STATE;
     }
     const Kernel &K = ctx->kern[kind];
     *n = K.compact();
     if (!out || *n <= 0) return PEEC_OK;
-    PEEC_HIP_TRY(hipMemcpyAsync(out, K.row.p, sizeof(double) * (size_t)*n, hipMemcpyDeviceToHost, ctx->stream));
+    const double *src = variant == 0 ? K.row.p : variant == 1 ? K.rrow.p : K.rdist.p;
+    PEEC_HIP_TRY(hipMemcpyAsync(out, src, sizeof(double) * (size_t)*n, hipMemcpyDeviceToHost, ctx->stream));
     PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return PEEC_OK;
 }

@@ -309,8 +309,10 @@ HD double f_two_axis(const double *xv, double xfix, const double *yv, double yfi
     return sol;
 }
 
-// :491-529 / P :438-476  (I C C | I C C)
-template <bool P_COPY>
+// :491-529 / P :438-476 / compute_Lp_Voxels_QS.jl:474-519  (I C C | I C C)
+// COPY 0: the Lp copy, 1: the P copy (x1*log|x1-x2| added outside the
+// (x1-x2)/R factor), 2: the QS copy (added inside it).
+template <int COPY>
 HD double f_line_line_parall(const double *x1v, double y1, double z1,
                                      const double *x2v, double y2, double z2)
 {
@@ -322,8 +324,9 @@ HD double f_line_line_parall(const double *x1v, double y1, double z1,
             int c1 = (c >> 1) & 1, c2 = c & 1;
             double x1 = x1v[c1], x2 = x2v[c2];
             double R = sqrt((x1 - x2) * (x1 - x2));
-            double t1 = (x1 - x2) / R * (x1 - x2 * clog_re(x2 - x1));
-            if (P_COPY) t1 = t1 + x1 * clog_re(x1 - x2);
+            double t1 = COPY == 2 ? (x1 - x2) / R * (x1 - x2 * clog_re(x2 - x1) + x1 * clog_re(x1 - x2))
+                                  : (x1 - x2) / R * (x1 - x2 * clog_re(x2 - x1));
+            if (COPY == 1) t1 = t1 + x1 * clog_re(x1 - x2);
             if (bad(t1)) t1 = 0.0;
             sol = sol + sgn(c1 + c2 + 1) * t1;
         }
@@ -361,7 +364,7 @@ HD double f_line_point(const double *x1v, double y3, double z3, double x1, doubl
 // Form ids used by the classifier tables.
 enum Form : int {
     VOL_VOL, VOL_SURF, SSP_C, SSP_R, SSO_LP, SSO_P, LINE_VOL, LSP, LSO_LP, LSO_P,
-    POINT_VOL, LLO, LLP_LP, LLP_P, POINT_SUP, LINE_POINT
+    POINT_VOL, LLO, LLP_LP, LLP_P, POINT_SUP, LINE_POINT, LLP_QS
 };
 
 // Axis permutations: form axis k reads physical axis PERM[p][k].
@@ -410,8 +413,9 @@ HD double eval_form(const Call &k, const Box &b1, const Box &b2)
     case LSO_P: return f_line_surf_ortho<true>(p[0], pc[1], pc[2], qc[0], q[1], q[2]);
     case POINT_VOL: return f_point_vol(pc[0], pc[1], pc[2], q[0], q[1], q[2]);
     case LLO: return f_two_axis(p[0], qc[0], q[1], pc[1], pc[2] - qc[2], true, false, 1);
-    case LLP_LP: return f_line_line_parall<false>(p[0], pc[1], pc[2], q[0], qc[1], qc[2]);
-    case LLP_P: return f_line_line_parall<true>(p[0], pc[1], pc[2], q[0], qc[1], qc[2]);
+    case LLP_LP: return f_line_line_parall<0>(p[0], pc[1], pc[2], q[0], qc[1], qc[2]);
+    case LLP_P: return f_line_line_parall<1>(p[0], pc[1], pc[2], q[0], qc[1], qc[2]);
+    case LLP_QS: return f_line_line_parall<2>(p[0], pc[1], pc[2], q[0], qc[1], qc[2]);
     case POINT_SUP: return f_two_axis(q[0], pc[0], q[1], pc[1], pc[2] - qc[2], false, false, 0);
     case LINE_POINT: return f_line_point(p[0], pc[1], pc[2], qc[0], qc[1], qc[2]);
     }
@@ -592,7 +596,13 @@ HD int lp_mask(const double *L, const double mx[3], double V1, double V2, double
     return m;
 }
 
-HD double integ_vol_vol(const Box &b1, const Box &b2)
+// qs: compute_Lp_Voxels_QS.jl's copy (the retarded rows).  Its branches
+// differ from compute_Lp_Voxels.jl's in the forms they call; the rows of
+// the circulants compare two alike boxes, whose suppression is symmetric, so
+// only the vol-vol (unchanged: the QS dispatcher calls Integ_vol_vol_fft,
+// :418), parallel-surface (real log), parallel-line and far-field branches
+// are reached, and the others return NaN in this mode.
+HD double integ_vol_vol(const Box &b1, const Box &b2, bool qs = false)
 {
     double mx[3], mn[3];
     axis_gaps(b1, b2, mx, mn);
@@ -607,6 +617,11 @@ HD double integ_vol_vol(const Box &b1, const Box &b2)
     if (pp) {
         double dx = b1.c[0] - b2.c[0], dy = b1.c[1] - b2.c[1], dz = b1.c[2] - b2.c[2];
         return V1 * V2 / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    if (qs && k.form != VOL_VOL) {
+        if (k.form == SSP_C) k.form = SSP_R;
+        else if (k.form == LLP_LP) k.form = LLP_QS;
+        else return __builtin_nan("");
     }
     if (k.form == VOL_VOL) return eval_form(k, b1, b2);
     return pre * eval_form(k, b1, b2);
@@ -700,8 +715,14 @@ HD Box p_face(int caser, double cx, double cy, double cz, double sx, double sy, 
 HD inline double vox_c(double mn, double s, int i1) { return mn + s * (double)(i1 - 1) + s / 2; }
 HD inline double half_c(double mn, double s, int i1) { return mn + s * (double)((i1 + 1) / 2 - 1) + s / 2; }
 
+// variant 0: the static row of the quasi-static path; 1: the static row of
+// the retarded path (compute_rows_Rcc_Lp.jl / compute_rows_Rcc_P.jl: the Lp
+// rows from compute_Lp_Voxels_QS.jl's closed forms); 2: the centre
+// distances Rcc of the retarded path (compute_Voxels_Rcc.jl,
+// compute_P_vox_Rcc.jl).
 struct GridRow {
     int kind;
+    int variant;
     int dims[3];
     double origin[3];
     double s[3];
@@ -784,13 +805,26 @@ HD double grid_row_entry(const GridRow &g, int64_t n)
     grid_center(g, 0, c0);
     grid_center(g, n, c);
     double v;
+    if (g.variant == 2) {
+        // |c0 - c| (Lp centres) or |face(c0, caser) - face(c, caser2)| (P),
+        // LinearAlgebra.norm of the 3-vector: sqrt((d1^2 + d2^2) + d3^2)
+        if (g.kind > 2) {
+            int cs[2];
+            p_cases(g.kind - 3, &cs[0], &cs[1]);
+            int n1 = cs[0] <= 2 ? 1 : (cs[0] <= 4 ? 0 : 2), n2 = cs[1] <= 2 ? 1 : (cs[1] <= 4 ? 0 : 2);
+            c0[n1] = (cs[0] & 1) ? c0[n1] - g.s[n1] / 2 : c0[n1] + g.s[n1] / 2;
+            c[n2] = (cs[1] & 1) ? c[n2] - g.s[n2] / 2 : c[n2] + g.s[n2] / 2;
+        }
+        const double d0 = c0[0] - c[0], d1 = c0[1] - c[1], d2 = c0[2] - c[2];
+        return g.scale * sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+    }
     if (g.kind <= 2) {
         Box b1 = lp_box(c0[0], c0[1], c0[2], g.s[0], g.s[1], g.s[2]);
         Box b2 = lp_box(c[0], c[1], c[2], g.s[0], g.s[1], g.s[2]);
         int d = g.kind;
         double S = d == 0 ? g.s[1] * g.s[2] * g.s[1] * g.s[2]
                  : d == 1 ? g.s[0] * g.s[2] * g.s[0] * g.s[2] : g.s[0] * g.s[1] * g.s[0] * g.s[1];
-        v = 1e-7 / S * integ_vol_vol(b1, b2);
+        v = 1e-7 / S * integ_vol_vol(b1, b2, g.variant == 1);
         if (n == 0) v = 0.0;  // the self entry is carried by DZ (compute_FFT_mutual_coupling_mats.jl:56)
     } else {
         int cs[2];
@@ -942,10 +976,11 @@ static void grid_dims(int kind, int Nx, int Ny, int Nz, int d[3])
 }
 
 static GridRow make_grid_row(int kind, const double origin[3], int Nx, int Ny, int Nz, double sx, double sy,
-                             double sz, double scale)
+                             double sz, double scale, int variant = 0)
 {
     GridRow g;
     g.kind = kind;
+    g.variant = variant;
     grid_dims(kind, Nx, Ny, Nz, g.dims);
     for (int a = 0; a < 3; a++) g.origin[a] = origin[a];
     g.s[0] = sx; g.s[1] = sy; g.s[2] = sz;
@@ -954,9 +989,11 @@ static GridRow make_grid_row(int kind, const double origin[3], int Nx, int Ny, i
 }
 
 int peec_grid_row_device(hipStream_t s, int kind, const double origin[3], int Nx, int Ny, int Nz, double sx,
-                         double sy, double sz, double scale, double *d_out, int64_t *n_out, int dims_out[3])
+                         double sy, double sz, double scale, double *d_out, int64_t *n_out, int dims_out[3],
+                         int variant)
 {
-    GridRow g = make_grid_row(kind, origin, Nx, Ny, Nz, sx, sy, sz, scale);
+    if (variant < 0 || variant > 2) return PEEC_ERR_ARG;
+    GridRow g = make_grid_row(kind, origin, Nx, Ny, Nz, sx, sy, sz, scale, variant);
     int64_t N = (int64_t)g.dims[0] * g.dims[1] * g.dims[2];
     if (n_out) *n_out = N;
     if (dims_out) for (int a = 0; a < 3; a++) dims_out[a] = g.dims[a];
@@ -983,13 +1020,17 @@ extern "C" int peec_test_lp_row_host(const double *src, const double *cen, int64
 
 // Entries idx[0..count) of circulant row `kind` (0..8) on an Nx x Ny x Nz
 // grid, evaluated by k_grid_row's per-entry code (device centre arithmetic
-// included) on the host; returns the row length in *n_out.
+// included) on the host; returns the row length in *n_out.  kind + 16 *
+// variant selects the retarded path's static rows (variant 1) or distances
+// (variant 2).
 extern "C" int peec_test_grid_row_host(int kind, const double *origin, int Nx, int Ny, int Nz, double sx,
                                        double sy, double sz, double scale, const int64_t *idx, int64_t count,
                                        double *out, int64_t *n_out)
 {
-    if (kind < 0 || kind > 8 || !origin || Nx < 1 || Ny < 1 || Nz < 1) return PEEC_ERR_ARG;
-    GridRow g = make_grid_row(kind, origin, Nx, Ny, Nz, sx, sy, sz, scale);
+    const int variant = kind / 16;
+    kind %= 16;
+    if (kind < 0 || kind > 8 || variant > 2 || !origin || Nx < 1 || Ny < 1 || Nz < 1) return PEEC_ERR_ARG;
+    GridRow g = make_grid_row(kind, origin, Nx, Ny, Nz, sx, sy, sz, scale, variant);
     const int64_t N = (int64_t)g.dims[0] * g.dims[1] * g.dims[2];
     if (n_out) *n_out = N;
     for (int64_t i = 0; i < count; i++) {

@@ -590,6 +590,7 @@ extern "C" int peec_operator_set(peec_ctx *ctx, int64_t mx, int64_t my, int64_t 
     }
     if ((rc = yz_prepare(ctx))) return rc;
     ctx->op_ready = true;
+    ctx->rcc_ready = false;  // the half-cell reductions may have changed: rebuild retarded spectra
     return PEEC_OK;
 }
 
@@ -640,6 +641,10 @@ extern "C" int peec_frequency_slots(peec_ctx *ctx, int nslots)
         !ctx->invZ.alloc((size_t)(m * nslots))) {
         peec_set_error("peec_frequency_slots: out of device memory for %d slots", nslots);
         return PEEC_ERR_NOMEM;
+    }
+    if (nslots > 1 && ctx->rcc) {
+        peec_set_error("peec_frequency_slots: the retarded kernels have one set of spectra (one frequency)");
+        return PEEC_ERR_STATE;
     }
     ctx->nslots = nslots;
     ctx->hw.assign((size_t)nslots, 0.0);

@@ -32,9 +32,11 @@ int peec_p_row_device(hipStream_t s, const double src[3], const double *d_center
 // Circulant-grid rows computed straight from the grid description (the
 // centre arithmetic of build_center_P_Voxels.jl / build_centers_Lp_with_air.jl
 // is done on the device).  kind: 0..2 = Lp x/y/z, 3..8 = P 12,34,56,1234,1256,3456.
+// variant 0: static rows of the quasi-static path; 1: static rows of the
+// retarded (Rcc) path; 2: its centre distances
 int peec_grid_row_device(hipStream_t s, int kind, const double origin[3], int Nx, int Ny, int Nz,
                          double sx, double sy, double sz, double scale, double *d_out,
-                         int64_t *n_out, int dims_out[3]);
+                         int64_t *n_out, int dims_out[3], int variant = 0);
 
 // ---------------------------------------------------------------- fft
 struct FftPlan1D {

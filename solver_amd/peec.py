@@ -138,7 +138,34 @@ def compute_FFT_mutual_coupling_mats(origin, Nx, Ny, Nz, sx, sy, sz, escalings, 
     return rc != L.PEEC_STOPPED
 
 
-def kernel_row(kind, ctx=None):
+def compute_rows_Rcc(origin, Nx, Ny, Nz, sx, sy, sz, escalings, ctx=None):
+    """compute_FFT_mutual_coupling_mats.jl:18-27 (QS_Rcc_FW = 2):
+    compute_rows_Rcc_P + compute_rows_Rcc_Lp -- the static rows and centre
+    distances of the retarded kernels, kept on the device inside ctx (with
+    the quasi-static spectra).  From then on the convolutions of ctx use the
+    retarded spectra of the frequency last given to
+    compute_Circulant_Rcc.  Returns False when a stop was requested."""
+    ctx = ctx or default_context()
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    rc = L.check(L.lib().peec_kernels_build_rcc(ctx.handle, L.as_dp(o), int(Nx), int(Ny), int(Nz), sx, sy, sz,
+                                                float(escalings["Lp"]), float(escalings["P"])),
+                 "peec_kernels_build_rcc")
+    return rc != L.PEEC_STOPPED
+
+
+def compute_Circulant_Rcc(freq_hz, ctx=None):
+    """compute_Circulant_Lp_Rcc + compute_Circulant_P_sup_Rcc
+    (FFT_solver_QS_S_type.jl:61-64) at one frequency in Hz: the complex
+    spectra of escaling * row .* exp(-j beta Rcc), on the device."""
+    ctx = ctx or default_context()
+    L.check(L.lib().peec_rcc_spectra(ctx.handle, float(freq_hz)), "peec_rcc_spectra")
+
+
+def kernel_row(kind, ctx=None, variant=0):
+    """Kernel row `kind` (0..8) read back from the device: variant 0 the
+    scaled quasi-static row, 1 the retarded path's static row, 2 its centre
+    distances."""
+    kind = int(kind) + 16 * int(variant)
     ctx = ctx or default_context()
     n = ctypes.c_int64()
     L.check(L.lib().peec_kernel_row(ctx.handle, int(kind), None, ctypes.byref(n)), "peec_kernel_row")

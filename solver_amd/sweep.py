@@ -180,17 +180,22 @@ def lu_factor(SS, order=None):
 class Solver:
     """Holds one geometry on one device context."""
 
-    def __init__(self, mesh, ctx=None, nested_dissection=True, device_lu=False):
+    def __init__(self, mesh, ctx=None, nested_dissection=True, device_lu=False, QS_Rcc_FW=1):
         """device_lu: factor SS on the GPU (peec_lu_factor) instead of with
-        SuperLU on the host; both use the nested-dissection order."""
+        SuperLU on the host; both use the nested-dissection order.
+        QS_Rcc_FW = 2: retarded (Rcc) kernels, whose spectra are rebuilt per
+        frequency (FFT_solver_QS_S_type.jl:61-64)."""
+        if QS_Rcc_FW not in (1, 2):
+            raise ValueError("QS_Rcc_FW must be 1 (quasi-static) or 2 (retarded Rcc)")
         self.mesh = mesh
         self.ctx = ctx or peec.default_context()
         self.device_lu = device_lu
+        self.rcc = QS_Rcc_FW == 2
         self.dissection = ordering.mesh_dissection(mesh) if (nested_dissection or device_lu) else None
         self.order, self.blocks = ((self.dissection.order, self.dissection.block_ptr) if self.dissection is not None
                                    else (None, None))
-        ok = peec.compute_FFT_mutual_coupling_mats(mesh.origin, mesh.Nx, mesh.Ny, mesh.Nz, mesh.sx, mesh.sy,
-                                                   mesh.sz, mesh.escalings, ctx=self.ctx)
+        build = peec.compute_rows_Rcc if self.rcc else peec.compute_FFT_mutual_coupling_mats
+        ok = build(mesh.origin, mesh.Nx, mesh.Ny, mesh.Nz, mesh.sx, mesh.sy, mesh.sz, mesh.escalings, ctx=self.ctx)
         if not ok:
             raise RuntimeError("stopped")
         peec.operator_set(mesh, ctx=self.ctx)
@@ -211,6 +216,8 @@ class Solver:
     def set_frequency(self, w, R0=50.0, sigma=None):
         m = self.mesh
         esc = m.escalings
+        if self.rcc:
+            peec.compute_Circulant_Rcc(w / (2 * np.pi * esc["freq"]), ctx=self.ctx)
         Yle = build_Yle_FFT(m.lumped_elements, m.port_nodes, esc, m.n, w / esc["freq"], R0)
         DZ = DZ_of(m, w, sigma)
         peec.frequency_set(w, DZ, Yle, ctx=self.ctx)
@@ -230,8 +237,10 @@ class Solver:
 
 def FFT_solver_QS_S_type(freq, mesh, inner_iter, tol, ports_scatter_value=50.0, solver=None, record=None,
                          sigma=None, keep=False):
-    """FFT_solver_QS_S_type.jl:1-234 for QS_Rcc_FW = 1.  Returns
-    dict(S, Z, Y, f), or None when a stop was requested."""
+    """FFT_solver_QS_S_type.jl:1-234.  The solver's kernels decide between
+    QS_Rcc_FW = 1 and 2 (Solver(..., QS_Rcc_FW=2) rebuilds the retarded
+    spectra per frequency in set_frequency, :61-64).  Returns dict(S, Z, Y,
+    f), or None when a stop was requested."""
     solver = solver or Solver(mesh)
     esc = mesh.escalings
     freq = np.asarray(freq, dtype=np.float64) * esc["freq"]

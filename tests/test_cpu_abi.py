@@ -123,3 +123,25 @@ def test_deterministic_math_within_one_ulp_of_libm(lib):
     lib.peec_test_probe(0, 2, L.as_dp(y), L.as_dp(xx), n, L.as_dp(o))
     r = np.arctan2(y, xx)
     assert np.max(np.abs(o - r) / np.spacing(np.abs(r))) <= 1.0
+
+
+def test_rcc_row_entries_on_host_match_oracle(lib):
+    """The retarded rows' per-entry device code (static rows from the
+    compute_Lp_Voxels_QS closed forms, centre distances), run on the host
+    through the test hook, against the oracle on a strided sample of the
+    64x64x16 grid."""
+    from solver_amd import _lib as L
+    from oracle import rcc
+    Nx, Ny, Nz, s = 64, 64, 16, (0.25e-3, 0.25e-3, 0.05e-3)
+    org = np.zeros(3)
+    rows = rcc.rcc_rows(org, Nx, Ny, Nz, *s)
+    for k in range(9):
+        for variant in (1, 2):
+            ref = rows[k][variant - 1]
+            idx = np.ascontiguousarray(np.arange(0, ref.size, 37, dtype=np.int64))
+            out = np.empty(idx.size)
+            n = ctypes.c_int64()
+            rc = lib.peec_test_grid_row_host(k + 16 * variant, L.as_dp(org), Nx, Ny, Nz, *s, 1.0,
+                                             idx.ctypes.data_as(L.i64p), idx.size, L.as_dp(out), ctypes.byref(n))
+            assert rc == 0 and n.value == ref.size
+            assert np.array_equal(out, ref[idx]), (k, variant)
