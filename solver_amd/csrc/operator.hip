@@ -85,22 +85,48 @@ __global__ void k_zero(cd *y, int64_t n, int nrhs, int64_t ld)
     y[i + (int64_t)blockIdx.y * ld] = {0.0, 0.0};  // one right-hand side per grid row
 }
 
+// The sparse terms and the permutation scatter / gather run one thread per
+// row and frequency slot (blockIdx.y = slot s; the columns of a batch are
+// grouped by slot, cm.first(s) .. cm.end(s)).  The thread takes the row's
+// index data and the slot's DZ / 1/DZ / w once and gathers kRhsChunk
+// right-hand sides per index, with one accumulator each, so the loads of a
+// chunk are independent.  Each right-hand side's arithmetic is the
+// per-column formula, in the same order.
+constexpr int kRhsChunk = 8;
+
 // Y1 = j w Y1 + DZ .* I + A Phi           (compute_Matrix_vector_fft.jl:29-31)
-// Right-hand side r takes w and DZ of its frequency slot (cm).
 __global__ void k_y1(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const int32_t *ai, const double *av,
                      const double *wv, int64_t m, int64_t phi0, int nrhs, int64_t ld, ColMap cm)
 {
-    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m) return;
-    {
-        const int r = (int)blockIdx.y;  // one right-hand side per grid row
-        const int sl = cm.slot(r);
-        const cd *xr = x + r * ld;
-        cd s = {0.0, 0.0};
-        for (int k = ap[e]; k < ap[e + 1]; k++) s = cadd(s, cscal(xr[phi0 + ai[k]], av[k]));
-        cd v = jw(y[e + r * ld], wv[sl]);
-        v = cadd(v, cmul(DZ[e + sl * m], xr[e]));
-        y[e + r * ld] = cadd(v, s);
+    const int s = (int)blockIdx.y;
+    const int rb = cm.first(s), re = cm.end(s, nrhs);
+    const double w = wv[s];
+    const cd dz = DZ[e + s * m];
+    const int k0 = ap[e], k1 = ap[e + 1];
+    for (int r0 = rb; r0 < re; r0 += kRhsChunk) {
+        const int cnt = min(kRhsChunk, re - r0);
+        const cd *x0 = x + r0 * ld;
+        cd acc[kRhsChunk];
+#pragma unroll
+        for (int j = 0; j < kRhsChunk; j++) acc[j] = {0.0, 0.0};
+        for (int k = k0; k < k1; k++) {
+            const int64_t c = phi0 + ai[k];
+            const double a = av[k];
+#pragma unroll
+            for (int j = 0; j < kRhsChunk; j++)
+                if (j < cnt) acc[j] = cadd(acc[j], cscal(x0[c + j * ld], a));
+        }
+#pragma unroll
+        for (int j = 0; j < kRhsChunk; j++) {
+            if (j < cnt) {
+                const int64_t o = e + (r0 + j) * ld;
+                cd v = jw(y[o], w);
+                v = cadd(v, cmul(dz, x[o]));
+                y[o] = cadd(v, acc[j]);
+            }
+        }
     }
 }
 
@@ -124,24 +150,54 @@ __global__ void k_y3(cd *y, const cd *x, const int32_t *atp, const int32_t *ati,
                      const int32_t *yi, const cd *yv, const int32_t *gp, const int32_t *gi, const double *gv,
                      const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, ColMap cm)
 {
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
-    {
-        const int r = (int)blockIdx.y;  // one right-hand side per grid row
-        const int sl = cm.slot(r);
-        const int32_t *yp = yp0 + sl * (n + 1);
-        const double w = wv[sl];
-        const cd *xr = x + r * ld;
-        cd a = {0.0, 0.0};
-        for (int k = atp[v]; k < atp[v + 1]; k++) a = cadd(a, cscal(xr[ati[k]], atv[k]));
-        cd b = {0.0, 0.0};
-        for (int k = yp[v]; k < yp[v + 1]; k++) b = cadd(b, cmul(yv[k], xr[phi0 + yi[k]]));
-        cd c = {0.0, 0.0};
-        for (int k = gp[v]; k < gp[v + 1]; k++) c = cadd(c, cscal(xr[m + gi[k]], gv[k]));
-        cd t = {-a.x, -a.y};
-        t = cadd(t, b);
-        y[phi0 + v + r * ld] = cadd(t, jw(c, w));
+    const int s = (int)blockIdx.y;
+    const int rb = cm.first(s), re = cm.end(s, nrhs);
+    const int32_t *yp = yp0 + s * (n + 1);
+    const double w = wv[s];
+    const int a0 = atp[v], a1 = atp[v + 1], y0 = yp[v], y1 = yp[v + 1], g0 = gp[v], g1 = gp[v + 1];
+    for (int r0 = rb; r0 < re; r0 += kRhsChunk) {
+        const int cnt = min(kRhsChunk, re - r0);
+        const cd *x0 = x + r0 * ld;
+        cd t[kRhsChunk], acc[kRhsChunk];
+#pragma unroll
+        for (int j = 0; j < kRhsChunk; j++) t[j] = {0.0, 0.0};
+        for (int k = a0; k < a1; k++) {
+            const int64_t c = ati[k];
+            const double a = atv[k];
+#pragma unroll
+            for (int j = 0; j < kRhsChunk; j++)
+                if (j < cnt) t[j] = cadd(t[j], cscal(x0[c + j * ld], a));
+        }
+#pragma unroll
+        for (int j = 0; j < kRhsChunk; j++) {
+            t[j] = {-t[j].x, -t[j].y};
+            acc[j] = {0.0, 0.0};
+        }
+        for (int k = y0; k < y1; k++) {
+            const int64_t c = phi0 + yi[k];
+            const cd a = yv[k];
+#pragma unroll
+            for (int j = 0; j < kRhsChunk; j++)
+                if (j < cnt) acc[j] = cadd(acc[j], cmul(a, x0[c + j * ld]));
+        }
+#pragma unroll
+        for (int j = 0; j < kRhsChunk; j++) {
+            t[j] = cadd(t[j], acc[j]);
+            acc[j] = {0.0, 0.0};
+        }
+        for (int k = g0; k < g1; k++) {
+            const int64_t c = m + gi[k];
+            const double a = gv[k];
+#pragma unroll
+            for (int j = 0; j < kRhsChunk; j++)
+                if (j < cnt) acc[j] = cadd(acc[j], cscal(x0[c + j * ld], a));
+        }
+#pragma unroll
+        for (int j = 0; j < kRhsChunk; j++)
+            if (j < cnt) y[phi0 + v + (r0 + j) * ld] = cadd(t[j], jw(acc[j], w));
     }
 }
 
@@ -152,23 +208,46 @@ __global__ void k_pre(const cd *y, const cd *invZ0, const double *invP, const in
                       const double *Rs, cd *b, const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs,
                       int64_t ld, ColMap cm)
 {
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
-    {
-        const int r = (int)blockIdx.y;  // one right-hand side per grid row
-        const int sl = cm.slot(r);
-        const cd *invZ = invZ0 + sl * m;
-        const double w = wv[sl];
-        const cd *yr = y + r * ld;
-        cd a = {0.0, 0.0};
-        for (int k = atp[v]; k < atp[v + 1]; k++) a = cadd(a, cscal(cmul(invZ[ati[k]], yr[ati[k]]), atv[k]));
-        cd c = {0.0, 0.0};
-        for (int k = gp[v]; k < gp[v + 1]; k++) c = cadd(c, cscal(yr[m + gi[k]], invP[gi[k]] * gv[k]));
-        cd t = csub(a, jw(c, w));
-        t = cadd(t, yr[phi0 + v]);
-        if (Rs) t = cscal(t, Rs[v]);
-        b[pinv[v] + r * n] = t;
+    const int s = (int)blockIdx.y;
+    const int rb = cm.first(s), re = cm.end(s, nrhs);
+    const cd *invZ = invZ0 + s * m;
+    const double w = wv[s];
+    const int a0 = atp[v], a1 = atp[v + 1], g0 = gp[v], g1 = gp[v + 1];
+    const double rs = Rs ? Rs[v] : 1.0;
+    const int64_t dst = pinv[v];
+    for (int r0 = rb; r0 < re; r0 += kRhsChunk) {
+        const int cnt = min(kRhsChunk, re - r0);
+        const cd *y0 = y + r0 * ld;
+        cd a[kRhsChunk], c[kRhsChunk];
+#pragma unroll
+        for (int j = 0; j < kRhsChunk; j++) a[j] = c[j] = {0.0, 0.0};
+        for (int k = a0; k < a1; k++) {
+            const int64_t e = ati[k];
+            const cd iz = invZ[e];
+            const double av = atv[k];
+#pragma unroll
+            for (int j = 0; j < kRhsChunk; j++)
+                if (j < cnt) a[j] = cadd(a[j], cscal(cmul(iz, y0[e + j * ld]), av));
+        }
+        for (int k = g0; k < g1; k++) {
+            const int64_t q = gi[k];
+            const double f = invP[q] * gv[k];
+#pragma unroll
+            for (int j = 0; j < kRhsChunk; j++)
+                if (j < cnt) c[j] = cadd(c[j], cscal(y0[m + q + j * ld], f));
+        }
+#pragma unroll
+        for (int j = 0; j < kRhsChunk; j++) {
+            if (j < cnt) {
+                cd t = csub(a[j], jw(c[j], w));
+                t = cadd(t, y0[phi0 + v + j * ld]);
+                if (Rs) t = cscal(t, rs);
+                b[dst + (r0 + j) * n] = t;
+            }
+        }
     }
 }
 
@@ -178,25 +257,52 @@ __global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ0,
                        const int32_t *ai, const double *av, const int32_t *gtp, const int32_t *gti, const double *gtv,
                        int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, ColMap cm)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t N = m + ns + n;
     if (i >= N) return;
-    {
-        const int r = (int)blockIdx.y;  // one right-hand side per grid row
-        const cd *invZ = invZ0 + cm.slot(r) * m;
-        const cd *zr = z + r * n;
-        cd *yr = y + r * ld;
-        if (i < m) {
-            cd s = {0.0, 0.0};
-            for (int k = ap[i]; k < ap[i + 1]; k++) s = cadd(s, cscal(zr[qinv[ai[k]]], av[k]));
-            yr[i] = cmul(invZ[i], csub(yr[i], s));
-        } else if (i < m + ns) {
-            int64_t s_ = i - m;
-            cd s = {0.0, 0.0};
-            for (int k = gtp[s_]; k < gtp[s_ + 1]; k++) s = cadd(s, cscal(zr[qinv[gti[k]]], gtv[k]));
-            yr[i] = cscal(cadd(yr[i], s), invP[s_]);
+    const int s = (int)blockIdx.y;
+    const int rb = cm.first(s), re = cm.end(s, nrhs);
+    if (i >= m + ns) {
+        const int64_t src = qinv[i - m - ns];
+        for (int r = rb; r < re; r++) y[i + r * ld] = z[src + r * n];
+        return;
+    }
+    const bool edge = i < m;
+    const int64_t s_ = i - m;
+    const int k0 = edge ? ap[i] : gtp[s_], k1 = edge ? ap[i + 1] : gtp[s_ + 1];
+    const int32_t *idx = edge ? ai : gti;
+    const double *val = edge ? av : gtv;
+    for (int r0 = rb; r0 < re; r0 += kRhsChunk) {
+        const int cnt = min(kRhsChunk, re - r0);
+        const cd *z0 = z + r0 * n;
+        cd acc[kRhsChunk];
+#pragma unroll
+        for (int j = 0; j < kRhsChunk; j++) acc[j] = {0.0, 0.0};
+        for (int k = k0; k < k1; k++) {
+            const int64_t c = qinv[idx[k]];
+            const double a = val[k];
+#pragma unroll
+            for (int j = 0; j < kRhsChunk; j++)
+                if (j < cnt) acc[j] = cadd(acc[j], cscal(z0[c + j * n], a));
+        }
+        if (edge) {
+            const cd iz = invZ0[s * m + i];
+#pragma unroll
+            for (int j = 0; j < kRhsChunk; j++) {
+                if (j < cnt) {
+                    const int64_t o = i + (r0 + j) * ld;
+                    y[o] = cmul(iz, csub(y[o], acc[j]));
+                }
+            }
         } else {
-            yr[i] = zr[qinv[i - m - ns]];
+            const double ip = invP[s_];
+#pragma unroll
+            for (int j = 0; j < kRhsChunk; j++) {
+                if (j < cnt) {
+                    const int64_t o = i + (r0 + j) * ld;
+                    y[o] = cscal(cadd(y[o], acc[j]), ip);
+                }
+            }
         }
     }
 }
@@ -971,14 +1077,14 @@ int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, const ColMa
         if (!ctx->vt1.alloc((size_t)(nrhs * n)) || !ctx->vt2.alloc((size_t)(nrhs * n))) return PEEC_ERR_NOMEM;
     }
     double2_t *b = ctx->vt1.p, *z = ctx->vt2.p;
-    hipLaunchKernelGGL(k_pre, dim3(nblk(n), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, ctx->invZ.p, ctx->invP.p, ctx->At.ptr.p,
+    hipLaunchKernelGGL(k_pre, dim3(nblk(n), (unsigned)cm.nsl), dim3(256), 0, ctx->stream, y, ctx->invZ.p, ctx->invP.p, ctx->At.ptr.p,
                        ctx->At.idx.p, ctx->At.val.p, ctx->G.ptr.p, ctx->G.idx.p, ctx->G.val.p, ctx->perm_r.p,
                        ctx->has_Rs ? ctx->Rs.p : nullptr, b, ctx->wv.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
     int rc;
     if ((rc = trsv_solve(ctx, ctx->Lf, true, b, z, nrhs, cm))) return rc;   // L c = b   (c in z)
     if ((rc = trsv_solve(ctx, ctx->Uf, false, z, b, nrhs, cm))) return rc;  // U z' = c  (z' in b)
-    hipLaunchKernelGGL(k_post, dim3(nblk(m + ns + n), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, b, ctx->perm_c.p, ctx->invZ.p,
+    hipLaunchKernelGGL(k_post, dim3(nblk(m + ns + n), (unsigned)cm.nsl), dim3(256), 0, ctx->stream, y, b, ctx->perm_c.p, ctx->invZ.p,
                        ctx->invP.p, ctx->A.ptr.p, ctx->A.idx.p, ctx->A.val.p, ctx->Gt.ptr.p, ctx->Gt.idx.p,
                        ctx->Gt.val.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
@@ -1024,11 +1130,11 @@ static int apply_K_work(peec_ctx *ctx, const double2_t *x, double2_t *y, int nrh
         if (rc) return rc;
     }
     const int64_t phi0 = m + ns;
-    hipLaunchKernelGGL(k_y1, dim3(nblk(m), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, x, ctx->DZ.p, ctx->A.ptr.p, ctx->A.idx.p,
+    hipLaunchKernelGGL(k_y1, dim3(nblk(m), (unsigned)cm.nsl), dim3(256), 0, ctx->stream, y, x, ctx->DZ.p, ctx->A.ptr.p, ctx->A.idx.p,
                        ctx->A.val.p, ctx->wv.p, m, phi0, nrhs, ld, cm);
     hipLaunchKernelGGL(k_y2, dim3(nblk(ns), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, x, ctx->Gt.ptr.p, ctx->Gt.idx.p,
                        ctx->Gt.val.p, m, ns, phi0, nrhs, ld);
-    hipLaunchKernelGGL(k_y3, dim3(nblk(n), (unsigned)nrhs), dim3(256), 0, ctx->stream, y, x, ctx->At.ptr.p, ctx->At.idx.p,
+    hipLaunchKernelGGL(k_y3, dim3(nblk(n), (unsigned)cm.nsl), dim3(256), 0, ctx->stream, y, x, ctx->At.ptr.p, ctx->At.idx.p,
                        ctx->At.val.p, ctx->Yle.ptr.p, ctx->Yle.idx.p, ctx->Yle.val.p, ctx->G.ptr.p, ctx->G.idx.p,
                        ctx->G.val.p, ctx->wv.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
