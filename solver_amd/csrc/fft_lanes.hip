@@ -83,6 +83,26 @@ constexpr CxTw cx_twiddle(int t, int L)
     return {c, -s};
 }
 
+// exp(-2 pi i t / L) for t = 0 .. L-1, evaluated at compile time
+template <int L>
+struct CxTable {
+    double re[L], im[L];
+    constexpr CxTable() : re(), im()
+    {
+        for (int t = 0; t < L; t++) {
+            const CxTw w = cx_twiddle(t, L);
+            re[t] = w.re;
+            im[t] = w.im;
+        }
+    }
+};
+template <typename T, int L>
+__device__ __forceinline__ C2<T> cx_table(int t)
+{
+    static constexpr CxTable<L> tab{};
+    return {(T)tab.re[t], (T)tab.im[t]};
+}
+
 // W_L^E (DIR < 0) or its conjugate as a constant of type T: the template
 // arguments force the front end to evaluate cx_twiddle.
 template <typename T, int DIR, int L, int E>
@@ -92,19 +112,119 @@ __device__ __forceinline__ C2<T> cxc()
     return {(T)w.re, (T)(DIR < 0 ? w.im : -w.im)};
 }
 
+template <typename T>
+__device__ __forceinline__ T fma_t(T a, T b, T c) { return fma(a, b, c); }
+template <>
+__device__ __forceinline__ float fma_t<float>(float a, float b, float c) { return fmaf(a, b, c); }
+
+// a * b and a * conj(b), each part one product and one fused multiply-add
+template <typename T>
+__device__ __forceinline__ C2<T> fmul(C2<T> a, C2<T> b) { return {fma_t(a.x, b.x, -(a.y * b.y)), fma_t(a.x, b.y, a.y * b.x)}; }
+template <typename T>
+__device__ __forceinline__ C2<T> fmulc(C2<T> a, C2<T> b) { return {fma_t(a.x, b.x, a.y * b.y), fma_t(a.y, b.x, -(a.x * b.y))}; }
+
+// a * W_8^1 and a * W_8^3 (DIR < 0) or their conjugates: W_8^1 = h (1 - i),
+// W_8^3 = -h (1 + i), h = 1/sqrt(2)
+template <typename T, int DIR>
+__device__ __forceinline__ C2<T> mul_w8_1(C2<T> a)
+{
+    const T h = (T)0.70710678118654752440084436210485;
+    return DIR < 0 ? C2<T>{(a.x + a.y) * h, (a.y - a.x) * h} : C2<T>{(a.x - a.y) * h, (a.x + a.y) * h};
+}
+template <typename T, int DIR>
+__device__ __forceinline__ C2<T> mul_w8_3(C2<T> a)
+{
+    const T h = (T)0.70710678118654752440084436210485;
+    return DIR < 0 ? C2<T>{(a.y - a.x) * h, -((a.x + a.y) * h)} : C2<T>{-((a.x + a.y) * h), (a.x - a.y) * h};
+}
+
+// a * W_L^E (DIR < 0) or its conjugate for a compile-time exponent; the
+// powers 0, L/4, L/2 and 3L/4 are exact sign changes and swaps, the powers
+// L/8 and 3L/8 the two-product forms above
+template <typename T, int DIR, int L, int E>
+__device__ __forceinline__ C2<T> tw_mul(C2<T> a)
+{
+    constexpr int e = ((E % L) + L) % L;
+    if constexpr (e == 0) return a;
+    else if constexpr (4 * e == L) return rot<T, DIR>(a);
+    else if constexpr (2 * e == L) return C2<T>{-a.x, -a.y};
+    else if constexpr (4 * e == 3 * L) return rot<T, -DIR>(a);
+    else if constexpr (8 * e == L) return mul_w8_1<T, DIR>(a);
+    else if constexpr (8 * e == 3 * L) return mul_w8_3<T, DIR>(a);
+    else return fmul(a, cxc<T, DIR, L, e>());
+}
+
 // v[r] *= W_L^(r BASE), r = 1 .. R-1
 template <typename T, int DIR, int L, int R, int BASE, int r = 1>
 __device__ __forceinline__ void cx_tw_apply(C2<T> (&v)[R])
 {
     if constexpr (BASE > 0 && r < R) {
-        v[r] = cmul(v[r], cxc<T, DIR, L, r * BASE>());
+        v[r] = tw_mul<T, DIR, L, r * BASE>(v[r]);
         cx_tw_apply<T, DIR, L, R, BASE, r + 1>(v);
     }
 }
 
+// butterflies of radix 3, 5 and 8 with fused multiply-adds (radix 2 and 4
+// take additions only)
+template <typename T, int DIR>
+__device__ __forceinline__ void fdft3(C2<T> *v)
+{
+    const T s = (T)0.86602540378443864676372317075294;
+    const C2<T> b = cadd(v[1], v[2]);
+    const C2<T> d = rot<T, DIR>(cscale(csub(v[1], v[2]), s));
+    const C2<T> t = {fma_t((T)-0.5, b.x, v[0].x), fma_t((T)-0.5, b.y, v[0].y)};
+    v[0] = cadd(v[0], b);
+    v[1] = cadd(t, d);
+    v[2] = csub(t, d);
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void fdft5(C2<T> *v)
+{
+    const T c1 = (T)0.30901699437494742410229341718282, c2 = (T)-0.80901699437494742410229341718282;
+    const T s1 = (T)0.95105651629515357211643933337938, s2 = (T)0.58778525229247312916870595463907;
+    const C2<T> b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+    const C2<T> d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+    const C2<T> a0 = v[0];
+    const C2<T> r1 = {fma_t(c2, b2.x, fma_t(c1, b1.x, a0.x)), fma_t(c2, b2.y, fma_t(c1, b1.y, a0.y))};
+    const C2<T> r2 = {fma_t(c1, b2.x, fma_t(c2, b1.x, a0.x)), fma_t(c1, b2.y, fma_t(c2, b1.y, a0.y))};
+    const C2<T> i1 = rot<T, DIR>(C2<T>{fma_t(s2, d2.x, s1 * d1.x), fma_t(s2, d2.y, s1 * d1.y)});
+    const C2<T> i2 = rot<T, DIR>(C2<T>{fma_t(-s1, d2.x, s2 * d1.x), fma_t(-s1, d2.y, s2 * d1.y)});
+    v[0] = cadd(a0, cadd(b1, b2));
+    v[1] = cadd(r1, i1);
+    v[4] = csub(r1, i1);
+    v[2] = cadd(r2, i2);
+    v[3] = csub(r2, i2);
+}
+
+template <typename T, int DIR>
+__device__ __forceinline__ void fdft8(C2<T> *v)
+{
+    C2<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<T, DIR>(e);
+    dft4<T, DIR>(o);
+    o[1] = mul_w8_1<T, DIR>(o[1]);
+    o[2] = rot<T, DIR>(o[2]);
+    o[3] = mul_w8_3<T, DIR>(o[3]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        v[k] = cadd(e[k], o[k]);
+        v[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+template <typename T, int DIR, int R>
+__device__ __forceinline__ void fdftR(C2<T> *v)
+{
+    if constexpr (R == 2) dft2<T, DIR>(v);
+    else if constexpr (R == 3) fdft3<T, DIR>(v);
+    else if constexpr (R == 4) dft4<T, DIR>(v);
+    else if constexpr (R == 5) fdft5<T, DIR>(v);
+    else fdft8<T, DIR>(v);
+}
+
 // DFT of radix R (2, 4 or 8) whose inputs v[R/2 .. R-1] are zero: the
-// operations of dft2 / dft4 / dft8 that do not involve those zeros, in the
-// same order (an x + 0 cannot be folded by the compiler under IEEE rules).
+// operations of the radix-R butterfly that do not involve those zeros
 template <typename T, int DIR, int R>
 __device__ __forceinline__ void dft_halfin(C2<T> *v)
 {
@@ -120,12 +240,9 @@ __device__ __forceinline__ void dft_halfin(C2<T> *v)
         C2<T> e[4] = {v[0], v[2], v[0], v[0]}, o[4] = {v[1], v[3], v[1], v[1]};
         dft_halfin<T, DIR, 4>(e);
         dft_halfin<T, DIR, 4>(o);
-        const T h = (T)0.70710678118654752440084436210485;
-        C2<T> w1 = DIR < 0 ? C2<T>{h, -h} : C2<T>{h, h};
-        C2<T> w3 = DIR < 0 ? C2<T>{-h, -h} : C2<T>{-h, h};
-        o[1] = cmul(o[1], w1);
+        o[1] = mul_w8_1<T, DIR>(o[1]);
         o[2] = rot<T, DIR>(o[2]);
-        o[3] = cmul(o[3], w3);
+        o[3] = mul_w8_3<T, DIR>(o[3]);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             v[k] = cadd(e[k], o[k]);
@@ -151,7 +268,7 @@ __device__ __forceinline__ void cx_butterflies(const C2<T> (&a)[L], C2<T> (&b)[L
 #pragma unroll
             for (int r = 0; r < R; r++) v[r] = a[K + r * nb];
             cx_tw_apply<T, DIR, L, R, kk * stride>(v);
-            dftR<T, DIR, R>(v);
+            fdftR<T, DIR, R>(v);
         }
 #pragma unroll
         for (int r = 0; r < R; r++) b[(K - kk) * R + kk + r * Ns] = v[r];
@@ -174,31 +291,16 @@ __device__ __forceinline__ void cx_stages(C2<T> (&a)[L])
     }
 }
 
-// W_L^(sel E) (DIR < 0) or its conjugate for a lane value 0 <= sel < 2^B:
-// the product over the set bits of sel of the constants W_L^(2^i E), each
-// factor applied by a select between computed values (a select among
-// constants becomes a lookup table in scratch memory).  At most B - 1
-// roundings beyond the constants' own.
-template <typename T, int DIR, int L, int E, int B, int I = 0>
-__device__ __forceinline__ C2<T> tw_lane(int sel, C2<T> w = C2<T>{(T)1, (T)0})
+// a[k2] *= W_L^(q k2) (DIR < 0) or its conjugate, k2 = 1 .. M-1, from the
+// LDS table laid out k2-major: tw[k2 Q] = W_L^(q k2) for this lane's q (one
+// LDS read with an immediate offset per factor)
+template <typename T, int DIR, int M, int Q>
+__device__ __forceinline__ void lane_twiddle(C2<T> (&a)[M], const C2<T> *tw)
 {
-    if constexpr (I < B) {
-        const C2<T> c = cxc<T, DIR, L, (1 << I) * E>();
-        const C2<T> f = I == 0 ? c : cmul(w, c);
-        w = ((sel >> I) & 1) ? f : w;
-        return tw_lane<T, DIR, L, E, B, I + 1>(sel, w);
-    } else {
-        return w;
-    }
-}
-
-// a[k2] *= W_L^(sel k2) (or the conjugate), k2 = 1 .. M-1
-template <typename T, int DIR, int L, int M, int B, int K2 = 1>
-__device__ __forceinline__ void lane_twiddle(C2<T> (&a)[M], int sel)
-{
-    if constexpr (K2 < M) {
-        a[K2] = cmul(a[K2], tw_lane<T, DIR, L, K2, B>(sel));
-        lane_twiddle<T, DIR, L, M, B, K2 + 1>(a, sel);
+#pragma unroll
+    for (int k2 = 1; k2 < M; k2++) {
+        const C2<T> w = tw[k2 * Q];
+        a[k2] = DIR < 0 ? fmul(a[k2], w) : fmulc(a[k2], w);
     }
 }
 
@@ -225,20 +327,31 @@ __device__ __forceinline__ C2<T> xlane2(C2<T> v) { return {xlane<D>(v.x), xlane<
 template <int H>
 constexpr int log2c() { return H <= 1 ? 0 : 1 + log2c<H / 2>(); }
 
-template <typename T>
-__device__ __forceinline__ T fma_t(T a, T b, T c) { return fma(a, b, c); }
-template <>
-__device__ __forceinline__ float fma_t<float>(float a, float b, float c) { return fmaf(a, b, c); }
-
 // p + sg * a with sg = +-1: one rounding, the same as p + a or p - a
 template <typename T>
 __device__ __forceinline__ C2<T> padd(C2<T> p, T sg, C2<T> a) { return {fma_t(sg, a.x, p.x), fma_t(sg, a.y, p.y)}; }
 
+// Twiddles of the exchanges at lane distance H >= 4: wx[log2(H) - 2] =
+// W_(2H)^(q mod H) in the upper lane of a pair, 1 in the lower one.
+template <typename T, int L, int Q>
+struct XTw {
+    static constexpr int N = log2c<Q>() > 2 ? log2c<Q>() - 2 : 1;
+    C2<T> w[N];
+    __device__ __forceinline__ explicit XTw(int q)
+    {
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const int H = 4 << j;
+            w[j] = (H < Q && (q & H)) ? cx_table<T, L>((q & (H - 1)) * (L / (2 * H))) : C2<T>{(T)1, (T)0};
+        }
+    }
+};
+
 // forward exchanges, lane distance H = Q/2, Q/4, .., 1: the lower lane of a
 // pair keeps the sum, the upper one the difference times W_(2H)^(q mod H)
 // (1 for H = 1, 1 or -i for H = 2)
-template <typename T, int M, int H>
-__device__ __forceinline__ void xl_fwd(C2<T> (&a)[M], int q)
+template <typename T, int M, int H, int L, int Q>
+__device__ __forceinline__ void xl_fwd(C2<T> (&a)[M], int q, const XTw<T, L, Q> &xt)
 {
     if constexpr (H >= 1) {
         const bool up = (q & H) != 0;
@@ -254,20 +367,19 @@ __device__ __forceinline__ void xl_fwd(C2<T> (&a)[M], int q)
                 a[m] = mi ? C2<T>{t.y, -t.x} : t;
             }
         } else {
-            const C2<T> w0 = tw_lane<T, -1, 2 * H, 1, log2c<H>()>(q & (H - 1));
-            const C2<T> w = up ? w0 : C2<T>{(T)1, (T)0};
+            const C2<T> w = xt.w[log2c<H>() - 2];
 #pragma unroll
-            for (int m = 0; m < M; m++) a[m] = cmul(padd(xlane2<T, H>(a[m]), sg, a[m]), w);
+            for (int m = 0; m < M; m++) a[m] = fmul(padd(xlane2<T, H>(a[m]), sg, a[m]), w);
         }
-        xl_fwd<T, M, H / 2>(a, q);
+        xl_fwd<T, M, H / 2, L, Q>(a, q, xt);
     }
 }
 
 // inverse exchanges, lane distance H = 1, 2, .., Q/2: the upper lane first
 // takes its value times conj(W_(2H)^(q mod H)) (1 for H = 1, 1 or +i for
 // H = 2), then the lower lane keeps the sum, the upper one the difference
-template <typename T, int M, int Q, int H>
-__device__ __forceinline__ void xl_inv(C2<T> (&a)[M], int q)
+template <typename T, int M, int H, int L, int Q>
+__device__ __forceinline__ void xl_inv(C2<T> (&a)[M], int q, const XTw<T, L, Q> &xt)
 {
     if constexpr (H < Q) {
         const bool up = (q & H) != 0;
@@ -283,15 +395,14 @@ __device__ __forceinline__ void xl_inv(C2<T> (&a)[M], int q)
                 a[m] = padd(xlane2<T, H>(t), sg, t);
             }
         } else {
-            const C2<T> w0 = tw_lane<T, 1, 2 * H, 1, log2c<H>()>(q & (H - 1));
-            const C2<T> w = up ? w0 : C2<T>{(T)1, (T)0};
+            const C2<T> w = xt.w[log2c<H>() - 2];
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                const C2<T> t = cmul(a[m], w);
+                const C2<T> t = fmulc(a[m], w);
                 a[m] = padd(xlane2<T, H>(t), sg, t);
             }
         }
-        xl_inv<T, M, Q, 2 * H>(a, q);
+        xl_inv<T, M, 2 * H, L, Q>(a, q, xt);
     }
 }
 
@@ -315,6 +426,10 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
     constexpr bool HALF = M % 2 == 0;
     constexpr bool CONV = MODE >= 2;
     constexpr uint32_t ES = sizeof(C2<T>);
+    // lane twiddles, k2-major: s_tw[k2 Q + q] = W_L^(q k2)
+    __shared__ C2<T> s_tw[L];
+    for (int t = (int)threadIdx.x; t < L; t += 256) s_tw[t] = cx_table<T, L>((t % Q) * (t / Q));
+    __syncthreads();
     unsigned bx, by, bz;
     xcd_block(bx, by, bz);
     const int q = (int)threadIdx.x % Q;
@@ -323,6 +438,7 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
     const bool live = b0 + lb < d.nb;
     const int l2 = (int)by;
     const int rq = (int)(__brev((unsigned)q) >> (32 - LQ));
+    const XTw<T, L, Q> xt(q);
     // 32-bit lane offsets (bytes) of element 0 of this lane's stride pattern
     const uint32_t in_nat = (uint32_t)(lb * d.in_sb + (int64_t)q * d.in_sj) * ES;
     const uint32_t in_rev = (uint32_t)(lb * d.in_sb + (int64_t)M * rq * d.in_sj) * ES;
@@ -343,51 +459,73 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
             }
         }
     }
-    const int r_end = min(d.nrhs, ((int)bz + 1) * d.rpb);
-    for (int rhs = (int)bz * d.rpb; rhs < r_end; rhs++) {
-        // the lane's index is laundered so that its twiddles are formed here,
-        // not held in registers across the loop
-        int qv = q;
-        asm volatile("" : "+v"(qv));
+    // elements a lane loads per right-hand side: the lower half of its
+    // stride pattern (forward input) or all M (inverse-only input)
+    constexpr int NL = MODE == 1 ? M : (HALF ? M / 2 : M);
+    // the next right-hand side's elements are loaded before this one is
+    // transformed (memory-level parallelism through the whole loop) when
+    // they take at most 48 more VGPRs
+    constexpr bool PREF = NL * (int)sizeof(C2<T>) <= 192;
+    // Element addresses: a wave-uniform pointer stepped by the element
+    // stride (laundered per call, so that the per-element pointers are not
+    // hoisted out of the loop into registers) plus the lane's 32-bit offset.
+    auto load = [&](C2<T>(&v)[NL], int rhs) {
         const char *in = (const char *)(src + (int64_t)b0 * d.in_sb + (int64_t)l2 * d.in_s2 + (int64_t)rhs * d.in_rhs);
+        int64_t step = (MODE == 1 ? 1 : Q) * d.in_sj * (int64_t)ES;
+        asm volatile("" : "+s"(step));
+#pragma unroll
+        for (int i = 0; i < NL; i++, in += step) {
+            const int k = MODE == 1 ? i + M * rq : q + Q * i;
+            v[i] = live && k < d.nin ? ld_at<T>(in, MODE == 1 ? in_rev : in_nat) : C2<T>{(T)0, (T)0};
+        }
+    };
+    const int r_end = min(d.nrhs, ((int)bz + 1) * d.rpb);
+    C2<T> nxt[NL];
+    if (PREF && (int)bz * d.rpb < r_end) load(nxt, (int)bz * d.rpb);
+    for (int rhs = (int)bz * d.rpb; rhs < r_end; rhs++) {
         char *out = (char *)(dst + (int64_t)b0 * d.out_sb + (int64_t)l2 * d.out_s2 + (int64_t)rhs * d.out_rhs);
+        // the table pointer is laundered per right-hand side so that the
+        // twiddles are read from LDS where they are used instead of being
+        // hoisted into registers for the whole loop
+        int twq = q;
+        asm volatile("" : "+v"(twq));
+        const C2<T> *twr = s_tw + twq;
         C2<T> a[M];
-        if (MODE == 1) {
+        if (!PREF) load(nxt, rhs);
 #pragma unroll
-            for (int k2 = 0; k2 < M; k2++) {
-                const int k = k2 + M * rq;
-                a[k2] = live && k < d.nin ? ld_at<T>(in + (int64_t)k2 * d.in_sj * ES, in_rev) : C2<T>{(T)0, (T)0};
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < (HALF ? M / 2 : M); m++) {
-                const int n = q + Q * m;
-                a[m] = live && n < d.nin ? ld_at<T>(in + (int64_t)Q * m * d.in_sj * ES, in_nat) : C2<T>{(T)0, (T)0};
-            }
+        for (int i = 0; i < NL; i++) a[i] = nxt[i];
+        if (PREF && rhs + 1 < r_end) load(nxt, rhs + 1);
+        if (MODE != 1) {
             cx_stages<T, -1, M, 0, HALF>(a);
-            lane_twiddle<T, -1, L, M, LQ>(a, qv);
-            xl_fwd<T, M, Q / 2>(a, qv);
+            lane_twiddle<T, -1, M, Q>(a, twr);
+            xl_fwd<T, M, Q / 2, L, Q>(a, q, xt);
             if (MODE == 0) {
                 if (live) {
+                    int64_t step = d.out_sj * (int64_t)ES;
+                    asm volatile("" : "+s"(step));
 #pragma unroll
-                    for (int k2 = 0; k2 < M; k2++)
-                        if (k2 + M * rq < d.nout) st_at<T>(out + (int64_t)k2 * d.out_sj * ES, out_rev, a[k2]);
+                    for (int k2 = 0; k2 < M; k2++, out += step)
+                        if (k2 + M * rq < d.nout) st_at<T>(out, out_rev, a[k2]);
                 }
                 continue;
             }
 #pragma unroll
             for (int k2 = 0; k2 < M; k2++) {
                 if (MODE == 2) a[k2] = cscale(a[k2], sr[k2]);
-                else a[k2] = cmul(a[k2], C2<T>{sr[k2], si[MODE == 3 ? k2 : 0]});
+                else a[k2] = fmul(a[k2], C2<T>{sr[k2], si[MODE == 3 ? k2 : 0]});
             }
         }
-        xl_inv<T, M, Q, 1>(a, qv);
-        lane_twiddle<T, 1, L, M, LQ>(a, qv);
+        xl_inv<T, M, 1, L, Q>(a, q, xt);
+        // laundered again: the forward twiddles are not kept live until here
+        asm volatile("" : "+v"(twq));
+        lane_twiddle<T, 1, M, Q>(a, s_tw + twq);
         cx_stages<T, 1, M, 0>(a);
         if (live) {
+            int64_t step = Q * d.out_sj * (int64_t)ES;
+            asm volatile("" : "+s"(step));
 #pragma unroll
-            for (int m = 0; m < (HALF ? M / 2 : M); m++)
-                if (q + Q * m < d.nout) st_at<T>(out + (int64_t)Q * m * d.out_sj * ES, out_nat, a[m]);
+            for (int m = 0; m < (HALF ? M / 2 : M); m++, out += step)
+                if (q + Q * m < d.nout) st_at<T>(out, out_nat, a[m]);
         }
     }
 }
@@ -395,10 +533,11 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
 }  // namespace
 
 // (L, Q) pairs k_lines is built for: z convolutions (MODE 2, 3) and y
-// passes (MODE 0, 1) of the workloads' lengths, at most 32 points per lane.
-#define PEEC_LANES_Z(X) X(16, 4) X(32, 4) X(32, 8) X(48, 4) X(48, 8) X(64, 4) X(64, 8) X(64, 16) X(96, 8) \
-    X(96, 16) X(128, 8) X(128, 16)
-#define PEEC_LANES_Y(X) X(128, 8) X(128, 16) X(192, 8) X(192, 16) X(200, 8) X(384, 16) X(384, 32) X(400, 16)
+// passes (MODE 0, 1) of the workloads' lengths, at most 64 points per lane.
+#define PEEC_LANES_Z(X) X(16, 4) X(32, 2) X(32, 4) X(32, 8) X(48, 2) X(48, 4) X(48, 8) X(64, 2) X(64, 4) X(64, 8) \
+    X(64, 16) X(96, 4) X(96, 8) X(96, 16) X(128, 4) X(128, 8) X(128, 16)
+#define PEEC_LANES_Y(X) X(128, 4) X(128, 8) X(128, 16) X(192, 4) X(192, 8) X(192, 16) X(200, 4) X(200, 8) X(384, 8) \
+    X(384, 16) X(384, 32) X(400, 8) X(400, 16)
 
 bool lane_built(int L, int Q, int mode)
 {
