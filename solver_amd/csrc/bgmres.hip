@@ -17,16 +17,19 @@
 // the FP64 residual test ||r|| <= tol ||b|| like the plain path.
 //
 // Right-hand side r uses frequency slot r / rhs_per_slot.  All basis
-// vectors and iterates live in HBM; the host keeps only the small
-// Hessenberg matrices, the rotations and the convergence flags.
+// vectors and iterates live in HBM, and so does the least squares: the
+// Hessenberg columns, their Givens rotations, the rotated residual vector,
+// the back-substitution and the per-column convergence flags are computed
+// by one thread per right-hand side.  The host reads only the nrhs flags
+// after each step, to shrink the next matvec's batch to the columns still
+// iterating (reading them a step late would cost a wasted matvec of every
+// converged column instead).
 #include "context.h"
 
 #include <algorithm>
 #include <cmath>
-#include <complex>
 #include <vector>
 
-typedef std::complex<double> zc;
 
 
 namespace {
@@ -229,6 +232,100 @@ __global__ void k_bsub(cd *r, const cd *b, const cd *y, int64_t total)
     if (i < total) r[i] = {b[i].x - y[i].x, b[i].y - y[i].y};
 }
 
+// ---- Arnoldi least squares on the device (one thread per right-hand side)
+//
+// Complex helpers in FP64.
+__device__ __forceinline__ cd zmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cd zconj(cd a) { return {a.x, -a.y}; }
+__device__ __forceinline__ cd zadd(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ double zabs(cd a) { return hypot(a.x, a.y); }
+// a / b
+__device__ __forceinline__ cd zdiv(cd a, cd b)
+{
+    const double d = b.x * b.x + b.y * b.y;
+    return {(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+}
+
+// Start of a restart cycle: g = beta e_1 for every right-hand side.
+__global__ void k_ls_init(cd *G, int mp1, const double *beta, int nrhs)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrhs) return;
+    cd *g = G + (int64_t)r * mp1;
+    g[0] = {beta[r], 0.0};
+    for (int i = 1; i < mp1; i++) g[i] = {0.0, 0.0};
+}
+
+// Step j of the Givens QR of the Hessenberg matrix (gmres_custom.jl:148-163
+// restated for Gram-Schmidt columns): column j is the sum of the two
+// Gram-Schmidt passes (hbuf rows 0 .. j and m+1 .. m+1+j) with ||w|| below
+// it; the previous rotations are applied, a new one zeroes the subdiagonal,
+// g is rotated, and the right-hand side leaves the iteration (live[r] = 0)
+// when |g_(j+1)| <= thr[r] (the cycle's target times its start residual)
+// or the Krylov space is exhausted (||w|| = 0).
+__global__ void k_ls_step(int j, int m, const cd *hbuf, int hs, const double *hn, cd *H, cd *CS, cd *SN, cd *G,
+                          const double *thr, int32_t *live, int nrhs)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrhs || !live[r]) return;
+    const int mp1 = m + 1;
+    const cd *h = hbuf + (int64_t)r * hs;
+    cd *col = H + ((int64_t)r * m + j) * mp1;
+    cd *cs = CS + (int64_t)r * m, *sn = SN + (int64_t)r * m, *g = G + (int64_t)r * mp1;
+    for (int k = 0; k <= j; k++) col[k] = zadd(h[k], h[mp1 + k]);
+    col[j + 1] = {hn[r], 0.0};
+    for (int k = 0; k < j; k++) {
+        const cd t = zadd(zmul(cs[k], col[k]), zmul(sn[k], col[k + 1]));
+        const cd u = zadd(zmul({-sn[k].x, sn[k].y}, col[k]), zmul(zconj(cs[k]), col[k + 1]));
+        col[k] = t;
+        col[k + 1] = u;
+    }
+    const double a = zabs(col[j]), bb = zabs(col[j + 1]);
+    cd c = {1.0, 0.0}, sv = {0.0, 0.0};
+    if (bb > 0) {
+        const double rr = hypot(a, bb);
+        if (a == 0) {
+            c = {0.0, 0.0};
+            const cd cj = zconj(col[j + 1]);
+            sv = {cj.x / bb, cj.y / bb};
+        } else {
+            c = {a / rr, 0.0};
+            const cd ph = {col[j].x / a, col[j].y / a};
+            const cd t = zmul(ph, zconj(col[j + 1]));
+            sv = {t.x / rr, t.y / rr};
+        }
+    }
+    cs[j] = c;
+    sn[j] = sv;
+    col[j] = zadd(zmul(c, col[j]), zmul(sv, col[j + 1]));
+    col[j + 1] = {0.0, 0.0};
+    const cd gj = g[j];
+    g[j + 1] = zmul({-sv.x, sv.y}, gj);
+    g[j] = zmul(c, gj);
+    if (zabs(g[j + 1]) <= thr[r] || hn[r] == 0.0) live[r] = 0;
+}
+
+// y = R \ g over the kdone[r] leading rows, written into hbuf row r (zero
+// beyond them and for right-hand sides with kdone[r] == 0), ready for
+// x += V y.
+__global__ void k_ls_solve(int m, const cd *H, const cd *G, const int32_t *kdone, cd *hbuf, int hs, int nrhs)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrhs) return;
+    const int mp1 = m + 1, k = kdone[r];
+    const cd *Hr = H + (int64_t)r * m * mp1, *g = G + (int64_t)r * mp1;
+    cd *y = hbuf + (int64_t)r * hs;
+    for (int i = 0; i < hs; i++) y[i] = {0.0, 0.0};
+    for (int i = k - 1; i >= 0; i--) {
+        cd t = g[i];
+        for (int l = i + 1; l < k; l++) {
+            const cd p = zmul(Hr[(int64_t)l * mp1 + i], y[l]);
+            t = {t.x - p.x, t.y - p.y};
+        }
+        y[i] = zdiv(t, Hr[(int64_t)i * mp1 + i]);
+    }
+}
+
 inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
@@ -242,9 +339,19 @@ struct BWork {
     DevBuf<int32_t> cmap;  // compacted columns: col[nrhs], rslot[nrhs], slot_ptr[nslots + 1]
     DevBuf<int32_t> masks; // live[nrhs] (right-hand sides still iterating), second-pass mask[nrhs]
     DevBuf<double> nrm2;   // ||w|| before and after the first Gram-Schmidt pass
+    // least squares of the cycle: Hessenberg columns (m+1) x m, rotations,
+    // rotated g, per-column exit thresholds and steps taken
+    DevBuf<cd> Hd, CSd, SNd, Gd;
+    DevBuf<double> thr;
+    DevBuf<int32_t> kd;
     int alloc()
     {
         const size_t nr = (size_t)nrhs, n = (size_t)N;
+        if (!Hd.alloc(nr * (size_t)m * (size_t)(m + 1)) || !CSd.alloc(nr * (size_t)m) || !SNd.alloc(nr * (size_t)m) ||
+            !Gd.alloc(nr * (size_t)(m + 1)) || !thr.alloc(nr) || !kd.alloc(nr)) {
+            peec_set_error("peec_gmres_batched: out of device memory for the least-squares state");
+            return PEEC_ERR_NOMEM;
+        }
         if (!V.alloc(nr * (size_t)(m + 1) * n) || !B.alloc(nr * n) || !X.alloc(nr * n) || !R.alloc(nr * n) ||
             !W.alloc(nr * n) || !T.alloc(nr * n) || !Wc.alloc(nr * n) || !masks.alloc(2 * nr) || !nrm2.alloc(2 * nr) ||
             !cmap.alloc(2 * nr + (size_t)ctx->nslots + 1) || !hbuf.alloc(nr * 2 * (size_t)(m + 1)) ||
@@ -475,11 +582,8 @@ static int bgmres_solve(peec_ctx *ctx, BWork &W, double tol, int max_cycles, int
     }
     PEEC_HIP_TRY(hipGetLastError());
     const int m = restart, hs = 2 * (m + 1);
-    std::vector<cd> hh((size_t)nrhs * hs);
-    std::vector<double> hn;
-    std::vector<std::vector<zc>> H((size_t)nrhs), G((size_t)nrhs);  // Hessenberg (col-major m+1 x m), rhs g
-    std::vector<std::vector<zc>> cs((size_t)nrhs), sn((size_t)nrhs);
-    std::vector<cd> y((size_t)nrhs * hs);
+    std::vector<int32_t> flags((size_t)nrhs), kdone((size_t)nrhs);
+    std::vector<double> thr((size_t)nrhs);
     int ncycle = 0;
     while (std::any_of(active.begin(), active.end(), [](char c) { return c != 0; }) && ncycle < max_cycles) {
         if (ctx->stop_requested()) return PEEC_STOPPED;
@@ -488,21 +592,25 @@ static int bgmres_solve(peec_ctx *ctx, BWork &W, double tol, int max_cycles, int
         PEEC_HIP_TRY(hipMemcpyAsync(W.nrm.p, beta.data(), (size_t)nrhs * 8, hipMemcpyHostToDevice, ctx->stream));
         hipLaunchKernelGGL(k_bscale, dim3(nb256(N), nrhs), dim3(256), 0, ctx->stream, W.V.p, (int64_t)(m + 1) * N,
                            W.R.p, N, W.nrm.p, N);
+        // g = beta e_1; exit threshold of each column: the final tolerance,
+        // or (mixed) at most inner_tol (default 1e-5) per cycle, times beta
+        hipLaunchKernelGGL(k_ls_init, dim3((nrhs + 63) / 64), dim3(64), 0, ctx->stream, W.Gd.p, m + 1, W.nrm.p, nrhs);
         std::vector<char> inner((size_t)nrhs);
-        std::vector<int> kdone((size_t)nrhs, 0);
-        std::vector<double> eta((size_t)nrhs);
         for (int r = 0; r < nrhs; r++) {
             inner[(size_t)r] = active[(size_t)r];
-            H[(size_t)r].assign((size_t)(m + 1) * m, zc(0, 0));
-            G[(size_t)r].assign((size_t)m + 1, zc(0, 0));
-            G[(size_t)r][0] = beta[(size_t)r];
-            cs[(size_t)r].assign((size_t)m, zc(0, 0));
-            sn[(size_t)r].assign((size_t)m, zc(0, 0));
-            // inner target: the final tolerance, or (mixed) at most 5 digits per cycle
+            flags[(size_t)r] = active[(size_t)r] ? 1 : 0;
+            kdone[(size_t)r] = 0;
             const double target = tol * nb[(size_t)r] / std::max(beta[(size_t)r], 1e-300);
-            eta[(size_t)r] = mixed ? std::max(target, inner_tol > 0 ? inner_tol : 1e-5) : std::max(target, 1e-15);
+            const double eta = mixed ? std::max(target, inner_tol > 0 ? inner_tol : 1e-5) : std::max(target, 1e-15);
+            thr[(size_t)r] = eta * beta[(size_t)r];
         }
-        std::vector<int32_t> live, live_up;
+        PEEC_HIP_TRY(hipMemcpyAsync(W.thr.p, thr.data(), (size_t)nrhs * 8, hipMemcpyHostToDevice, ctx->stream));
+        // the device's live flags (masks[0 .. nrhs)) drive the Gram-Schmidt
+        // passes and the least squares; the host reads them back after each
+        // step to shrink the next matvec's batch
+        PEEC_HIP_TRY(hipMemcpyAsync(W.masks.p, flags.data(), (size_t)nrhs * 4, hipMemcpyHostToDevice, ctx->stream));
+        std::vector<int32_t> live;
+        int jlast = 0;
         for (int j = 0; j < m; j++) {
             // w = A_in v_j for the right-hand sides still iterating (the
             // batch shrinks as they converge)
@@ -512,13 +620,6 @@ static int bgmres_solve(peec_ctx *ctx, BWork &W, double tol, int max_cycles, int
             if ((rc = W.matvec_cols(W.V.p + (int64_t)j * N, (int64_t)(m + 1) * N, live, mixed != 0))) return rc;
             // classical Gram-Schmidt h = V' w, w -= V h on the live right-
             // hand sides, and a second pass where it cancelled too much
-            if (live != live_up) {
-                std::vector<int32_t> lm((size_t)nrhs, 0);
-                for (int32_t r : live) lm[(size_t)r] = 1;
-                PEEC_HIP_TRY(hipMemcpyAsync(W.masks.p, lm.data(), (size_t)nrhs * 4, hipMemcpyHostToDevice,
-                                            ctx->stream));
-                live_up = live;
-            }
             W.norms_device(W.W.p, N, W.nrm2.p);
             W.dot(j + 1, W.W.p, N, 0, W.masks.p);
             W.gemv(W.W.p, N, j + 1, W.hbuf.p, hs, -1.0, W.masks.p);
@@ -527,73 +628,33 @@ static int bgmres_solve(peec_ctx *ctx, BWork &W, double tol, int max_cycles, int
                                W.nrm2.p + nrhs, W.masks.p + nrhs, nrhs);
             W.dot(j + 1, W.W.p, N, m + 1, W.masks.p + nrhs);
             W.gemv(W.W.p, N, j + 1, W.hbuf.p + (m + 1), hs, -1.0, W.masks.p + nrhs);
-            PEEC_HIP_TRY(hipGetLastError());
-            PEEC_HIP_TRY(hipMemcpyAsync(hh.data(), W.hbuf.p, hh.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
-            if ((rc = W.norms(W.W.p, N, hn))) return rc;
+            // v_(j+1) = w / ||w||, then the least-squares step
+            W.norms_device(W.W.p, N, W.nrm.p);
             hipLaunchKernelGGL(k_bscale, dim3(nb256(N), nrhs), dim3(256), 0, ctx->stream,
                                W.V.p + (int64_t)(j + 1) * N, (int64_t)(m + 1) * N, W.W.p, N, W.nrm.p, N);
+            hipLaunchKernelGGL(k_ls_step, dim3((nrhs + 63) / 64), dim3(64), 0, ctx->stream, j, m, W.hbuf.p, hs,
+                               W.nrm.p, W.Hd.p, W.CSd.p, W.SNd.p, W.Gd.p, W.thr.p, W.masks.p, nrhs);
+            PEEC_HIP_TRY(hipGetLastError());
+            PEEC_HIP_TRY(hipMemcpyAsync(flags.data(), W.masks.p, (size_t)nrhs * 4, hipMemcpyDeviceToHost, ctx->stream));
+            PEEC_HIP_TRY(hipStreamSynchronize(ctx->stream));
             bool any = false;
-            for (int r = 0; r < nrhs; r++) {
-                if (!inner[(size_t)r]) continue;
-                std::vector<zc> &Hr = H[(size_t)r];
-                zc *col = Hr.data() + (size_t)j * (m + 1);
-                for (int k = 0; k <= j; k++) {
-                    const cd a = hh[(size_t)r * hs + k], c = hh[(size_t)r * hs + (m + 1) + k];
-                    col[k] = zc(a.x + c.x, a.y + c.y);
-                }
-                col[j + 1] = hn[(size_t)r];
-                // previous rotations, then a new one zeroing col[j + 1]
-                for (int k = 0; k < j; k++) {
-                    const zc t = cs[(size_t)r][k] * col[k] + sn[(size_t)r][k] * col[k + 1];
-                    col[k + 1] = -std::conj(sn[(size_t)r][k]) * col[k] + std::conj(cs[(size_t)r][k]) * col[k + 1];
-                    col[k] = t;
-                }
-                const double a = std::abs(col[j]), bb = std::abs(col[j + 1]);
-                zc c(1, 0), s(0, 0);
-                if (bb > 0) {
-                    const double rr = std::hypot(a, bb);
-                    if (a == 0) {
-                        c = 0;
-                        s = std::conj(col[j + 1]) / bb;
-                    } else {
-                        c = zc(a / rr, 0);
-                        s = (col[j] / a) * std::conj(col[j + 1]) / rr;
-                    }
-                }
-                cs[(size_t)r][j] = c;
-                sn[(size_t)r][j] = s;
-                col[j] = c * col[j] + s * col[j + 1];
-                col[j + 1] = 0;
-                G[(size_t)r][j + 1] = -std::conj(s) * G[(size_t)r][j];
-                G[(size_t)r][j] = c * G[(size_t)r][j];
+            for (int r : live) {
                 kdone[(size_t)r] = j + 1;
                 it[(size_t)r]++;
-                if (std::abs(G[(size_t)r][j + 1]) <= eta[(size_t)r] * beta[(size_t)r] || hn[(size_t)r] == 0.0)
-                    inner[(size_t)r] = 0;
-                else
-                    any = true;
+                inner[(size_t)r] = flags[(size_t)r] != 0;
+                any = any || inner[(size_t)r];
             }
+            jlast = j + 1;
             if (!any) break;
         }
-        // x += V y, y = R \ g per right-hand side (zero where inactive)
-        std::fill(y.begin(), y.end(), cd{0.0, 0.0});
-        for (int r = 0; r < nrhs; r++) {
-            const int k = kdone[(size_t)r];
-            if (!active[(size_t)r] || k == 0) continue;
-            std::vector<zc> yy((size_t)k);
-            const std::vector<zc> &Hr = H[(size_t)r];
-            for (int i = k - 1; i >= 0; i--) {
-                zc t = G[(size_t)r][i];
-                for (int l = i + 1; l < k; l++) t -= Hr[(size_t)l * (m + 1) + i] * yy[(size_t)l];
-                yy[(size_t)i] = t / Hr[(size_t)i * (m + 1) + i];
-            }
-            for (int i = 0; i < k; i++) y[(size_t)r * hs + i] = {yy[(size_t)i].real(), yy[(size_t)i].imag()};
-        }
-        // only basis columns 0 .. kmax-1 were written this cycle (for every
-        // right-hand side, converged ones included)
-        const int kmax = *std::max_element(kdone.begin(), kdone.end());
-        PEEC_HIP_TRY(hipMemcpyAsync(W.hbuf.p, y.data(), y.size() * 16, hipMemcpyHostToDevice, ctx->stream));
-        if (kmax > 0) W.gemv(W.X.p, N, kmax, W.hbuf.p, hs, 1.0);
+        // x += V y with y = R \ g per right-hand side (zero where inactive);
+        // only basis columns 0 .. jlast-1 were written this cycle
+        for (int r = 0; r < nrhs; r++)
+            if (!active[(size_t)r]) kdone[(size_t)r] = 0;
+        PEEC_HIP_TRY(hipMemcpyAsync(W.kd.p, kdone.data(), (size_t)nrhs * 4, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_ls_solve, dim3((nrhs + 63) / 64), dim3(64), 0, ctx->stream, m, W.Hd.p, W.Gd.p, W.kd.p,
+                           W.hbuf.p, hs, nrhs);
+        if (jlast > 0) W.gemv(W.X.p, N, jlast, W.hbuf.p, hs, 1.0);
         PEEC_HIP_TRY(hipGetLastError());
         const std::vector<double> beta_old = beta;
         std::vector<int32_t> act;
