@@ -414,7 +414,10 @@ def lumped_admittance(typ, R, L, C, w):
         def bad(y):
             return np.isnan(y) or np.isinf(y) or abs(y) < 1e-16
 
-        jw = 1j * w
+        # numpy scalars: a zero C or L gives an infinite or NaN term, as in
+        # Julia, which the fallbacks below catch
+        jw = np.complex128(1j * w)
+        R, L, C = np.float64(R), np.float64(L), np.float64(C)
         if typ == 1:
             z = R + jw * L + 1 / (jw * C)
             if np.isnan(z) or np.isinf(z) or abs(z) < 1e-16:

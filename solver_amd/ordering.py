@@ -104,6 +104,10 @@ def nested_dissection(ijk, special=(), leaf=64, cap=64):
 
 
 def mesh_ijk(mesh):
+    """Integer coordinates of the nodes: their lattice points where the
+    mesher records them, else the voxel centres (one node per voxel)."""
+    if getattr(mesh, "node_ijk", None) is not None:
+        return np.asarray(mesh.node_ijk, dtype=np.int64)
     return np.rint((np.asarray(mesh.vox_centers) - np.asarray(mesh.origin)) /
                    np.array([mesh.sx, mesh.sy, mesh.sz]) - 0.5).astype(np.int64)
 

@@ -19,7 +19,10 @@ from . import mesher, ordering, peec
 
 def lumped_admittance(typ, R, L, C, w):
     """get_lumped_elements_admittance_fft (build_Yle_FFT.jl:128-209)."""
-    jw = 1j * w
+    # numpy scalars: a zero C or L gives an infinite or NaN term (not a
+    # Python ZeroDivisionError), which the fallbacks catch
+    jw = np.complex128(1j * w)
+    R, L, C = np.float64(R), np.float64(L), np.float64(C)
     with np.errstate(divide="ignore", invalid="ignore"):
         def ok(y):
             return not (np.isnan(y) or np.isinf(y) or abs(y) < 1e-16)
