@@ -155,10 +155,10 @@ def roofline_leg(workload, nrhs, reps, tune=True):
     complex64, then `reps` batched K x with every launch timed by HIP events
     on the context's stream.  Returns the per-kind table and the roofline
     of the pass kind with the largest time."""
-    from solver_amd import configs, mesher, peec, sweep
+    from solver_amd import configs, peec, sweep
     cfg = configs.CONFIGS[workload]()
     t0 = time.perf_counter()
-    mesh = mesher.mesh(cfg["grid"], cfg["materials"], cfg["cell"], cfg["origin"], cfg["ports"], p_self_fn=gpu_p_self)
+    mesh = configs.mesh(cfg, p_self_fn=gpu_p_self)
     N = mesh.m + mesh.ns + mesh.n
     log("roofline leg %s: N=%d (mesh %.1f s)" % (workload, N, time.perf_counter() - t0))
     ctx = peec.Context(0)
@@ -264,11 +264,11 @@ def main():
         return
 
     from solver_amd import _lib as L
-    from solver_amd import configs, mesher, peec, sweep
+    from solver_amd import configs, peec, sweep
     cfg = configs.CONFIGS[args.workload]()
     t_start = time.perf_counter()
     t0 = time.perf_counter()
-    mesh = mesher.mesh(cfg["grid"], cfg["materials"], cfg["cell"], cfg["origin"], cfg["ports"], p_self_fn=gpu_p_self)
+    mesh = configs.mesh(cfg, p_self_fn=gpu_p_self)
     t_mesh = time.perf_counter() - t0
     P = mesh.port_nodes.shape[0]
     N = mesh.m + mesh.ns + mesh.n
@@ -399,6 +399,7 @@ def main():
         "dtype": "c64 FFT + f64" if mixed else "c128",
         "data": "synthetic (seeded geometry, %s)" % cfg["name"],
         "config": {"workload": cfg["name"], "grid": list(cfg["grid"].shape), "unknowns": int(N), "ports": int(P),
+                   "mesher": "reference (mesher_FFT.jl restatement)",
                    "path": "batched %s" % ("mixed precision (complex64 FFT convolutions, FP64 refinement)" if mixed
                                            else "FP64"),
                    "freq_points_timed": int(nfreq), "freq_points_per_step": args.batch,

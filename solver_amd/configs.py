@@ -4,10 +4,18 @@ Each builder returns a dict with the material grid (0 = empty, k = material
 k), the materials, cell size, port coordinates, frequencies and GMRES
 tolerance.  Ports are given as coordinates (metres) and resolved to voxels
 by the mesher the way find_voxels_port_pp.jl does (nearest voxel centre).
+
+mesh() builds the operator the reference would build for the geometry:
+the reference mesher (voxel_mesher, mesher_FFT.jl) on one grid per
+material.  The adjoint workload carries a conductivity per voxel, which the
+reference's per-material mesher has no place for; it is meshed by
+mesher.mesh, whose conductor slots take per-voxel conductivities.
 """
 import numpy as np
 import scipy.ndimage as ndi
 
+from . import mesher as _mesher
+from . import voxel_mesher
 from .mesher import Material
 
 COPPER = 5.8e7
@@ -173,3 +181,16 @@ CONFIGS = {
     "adjoint_64x64x16": adjoint_64x64x16,
     "large_192x192x48": large_192x192x48,
 }
+
+
+def mesh(cfg, p_self_fn=None):
+    """The operator inputs of a configuration: the reference mesher on one
+    grid per material, or (per-voxel conductivities) mesher.mesh."""
+    if cfg.get("sigma") is not None:
+        return _mesher.mesh(cfg["grid"], cfg["materials"], cfg["cell"], cfg["origin"], cfg["ports"],
+                            p_self_fn=p_self_fn, sigma_grid=cfg["sigma"])
+    g = np.asarray(cfg["grid"])
+    grids = [g == k + 1 for k in range(len(cfg["materials"]))]
+    return voxel_mesher.mesh_request(grids, cfg["materials"], cfg["cell"], cfg["origin"],
+                                     [m.sigmar > 0 for m in cfg["materials"]], cfg["ports"],
+                                     cfg.get("lumped_elements", ()), p_self_fn=p_self_fn)

@@ -1,6 +1,7 @@
 """Write the sweep fixtures: port voltages, S and Z of the oracle's
 quasi-static FFT solver (FFT_solver_QS_S_type.jl:60-232) on the seeded
-benchmark geometries, at three frequencies each.
+benchmark geometries meshed by the reference mesher (configs.mesh), at
+three frequencies each.
 
 Run from the repository root, in the build container:
 
@@ -36,7 +37,7 @@ sys.path.insert(0, ROOT)
 
 from oracle import fft_solver as OF  # noqa: E402
 from oracle import integrals as OI  # noqa: E402
-from solver_amd import configs, mesher, ordering  # noqa: E402
+from solver_amd import configs, ordering  # noqa: E402
 
 R0 = 50.0
 SETS = {
@@ -53,7 +54,7 @@ def p_self(sx, sy, sz):
 def setup(name):
     s = SETS[name]
     cfg = configs.CONFIGS[s["config"]]()
-    M = mesher.mesh(cfg["grid"], cfg["materials"], cfg["cell"], cfg["origin"], cfg["ports"], p_self_fn=p_self)
+    M = configs.mesh(cfg, p_self_fn=p_self)
     fall = np.asarray(cfg["freqs"])
     idx = sorted({int(np.argmin(np.abs(np.log(fall / t)))) for t in s["targets"]})
     return s, cfg, M, fall[idx], np.asarray(idx)

@@ -48,7 +48,7 @@ def z_errors(Z, Zref):
 
 def geometry(name):
     cfg = configs.CONFIGS[name]()
-    M = mesher.mesh(cfg["grid"], cfg["materials"], cfg["cell"], cfg["origin"], cfg["ports"], p_self_fn=pself)
+    M = configs.mesh(cfg, p_self_fn=pself)
     return cfg, M
 
 
