@@ -734,9 +734,10 @@ __global__ void k_spec_extract(const double2_t *__restrict__ g, int64_t G, doubl
 
 // ------------------------------------------------------------------ host side
 // lane-split passes (fft_lanes.hip)
-bool lane_fits(const fftk::PassDesc &d, int Q, int mode);
-bool lane_built(int L, int Q, int mode);
-int launch_lane_pass(peec_ctx *ctx, bool fp32, int mode, fftk::PassDesc &d, const void *src, void *dst, int Q);
+bool lane_fits(const fftk::PassDesc &d, int Q, int mode, bool xl);
+bool lane_built(int L, int Q, int mode, bool xl);
+int launch_lane_pass(peec_ctx *ctx, bool fp32, int mode, fftk::PassDesc &d, const void *src, void *dst, int Q,
+                     bool xl);
 
 struct PassGeom {
     int L, nin, nout, nb, n2;
@@ -936,21 +937,24 @@ static int RegFirstRadix(int L) { return L % 8 == 0 ? 8 : L % 4 == 0 ? 4 : L % 2
 // Launch shape kRegVariant: the register z kernel (fused z passes only).
 static constexpr int kRegVariant = kNumFftVariants;
 // Launch shapes kLaneBase + k: the lane-split kernel with Q = 2 << k lanes
-// per line (strided passes: y and the fused z pass), in fft_lanes.hip.
-static constexpr int kLaneBase = kRegVariant + 1, kNumLaneQ = 5;
-static bool is_lane_variant(int v) { return v >= kLaneBase && v < kLaneBase + kNumLaneQ; }
-static int lane_q(int v) { return 2 << (v - kLaneBase); }
+// per line (strided passes: y and the fused z pass), in fft_lanes.hip, its
+// exchanges across lanes by DPP and swizzles (k < kNumLaneQ) or, for
+// kLaneBase + kNumLaneQ + j, through LDS with Q = 4 << j.
+static constexpr int kLaneBase = kRegVariant + 1, kNumLaneQ = 5, kNumLaneX = 2, kNumLaneV = kNumLaneQ + kNumLaneX;
+static bool is_lane_variant(int v) { return v >= kLaneBase && v < kLaneBase + kNumLaneV; }
+static bool lane_xl(int v) { return v >= kLaneBase + kNumLaneQ; }
+static int lane_q(int v) { return lane_xl(v) ? 4 << (v - kLaneBase - kNumLaneQ) : 2 << (v - kLaneBase); }
 
 static int launch_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const void *src, void *dst, int v = -1)
 {
     if (d.nb <= 0 || d.n2 <= 0 || d.nrhs <= 0) return PEEC_OK;
     if (is_lane_variant(v)) {
         const int Q = lane_q(v);
-        if (!lane_fits(d, Q, mode)) {
+        if (!lane_fits(d, Q, mode, lane_xl(v))) {
             peec_set_error("lane-split FFT pass does not apply (mode %d, length %d, %d lanes)", mode, d.L, Q);
             return PEEC_ERR_ARG;
         }
-        return launch_lane_pass(ctx, fp32, mode, d, src, dst, Q);
+        return launch_lane_pass(ctx, fp32, mode, d, src, dst, Q, lane_xl(v));
     }
     if (v == kRegVariant) {
         // radix-8-first schedule only (the plan the kernel unrolls)
@@ -1692,15 +1696,17 @@ static int tune_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc d, const void 
             best.style = 0;
         }
     }
-    for (int k = 0; k < kNumLaneQ; k++) {
-        const int Q = 2 << k;
-        if (!lane_fits(d, Q, mode)) continue;
+    for (int k = 0; k < kNumLaneV; k++) {
+        const int Q = lane_q(kLaneBase + k);
+        const bool xl = lane_xl(kLaneBase + k);
+        if (!lane_fits(d, Q, mode, xl)) continue;
         d.rpb = default_lane_rpb(d, Q);
         float t;
         if ((rc = time_pass(ctx, fp32, mode, d, src, dst, kLaneBase + k, 3, ea, eb, &t))) return rc;
         if (verbose)
-            fprintf(stderr, "peec_fft_tune: kernel %d pass %d %s L %d: lane-split kernel Q %d rpb %d: %.1f us, %.0f GB/s\n",
-                    kernel, pass, fp32 ? "c64" : "c128", d.L, Q, d.rpb, 1e3 * t, bytes / (1e6 * t));
+            fprintf(stderr, "peec_fft_tune: kernel %d pass %d %s L %d: lane-split kernel Q %d%s rpb %d: %.1f us, %.0f GB/s\n",
+                    kernel, pass, fp32 ? "c64" : "c128", d.L, Q, xl ? " (LDS exchanges)" : "", d.rpb, 1e3 * t,
+                    bytes / (1e6 * t));
         if (t < tbest) {
             tbest = t;
             best.variant = kLaneBase + k;
@@ -1857,7 +1863,7 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
 {
     const bool fused = pass >= kPassXFwdPlanes;
     if (!ctx || kernel < 0 || kernel > 8 || pass < 0 || pass > kPassXInvPlanes || nrhs < 1 || variant < -1 ||
-        variant >= kLaneBase + kNumLaneQ || rpb < 1 || style < 0 || style > 2 || (style != 0 && !fp32) ||
+        variant >= kLaneBase + kNumLaneV || rpb < 1 || style < 0 || style > 2 || (style != 0 && !fp32) ||
         (fused && !fp32) || (variant == kRegVariant && (pass != 2 || style != 0)) ||
         (is_lane_variant(variant) && (pass < 1 || pass > 3 || style != 0)) || (pass == kPassYz && variant > 1)) {
         peec_set_error("peec_fft_set_choice: invalid argument");
@@ -1883,7 +1889,8 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
     const int L = app ? app->L[axis] : 0;
     const bool fits = !app ? false
                       : variant == kRegVariant ? reg_length(L, fp32 != 0) && L % RegFirstRadix(L) == 0
-                      : is_lane_variant(variant) ? L % lane_q(variant) == 0 && lane_built(L, lane_q(variant), pass == 2 ? 2 : 0)
+                      : is_lane_variant(variant) ? L % lane_q(variant) == 0 &&
+                                                       lane_built(L, lane_q(variant), pass == 2 ? 2 : 0, lane_xl(variant))
                       : variant_fits(variant, L, fp32 != 0);
     if (!fits) {
         peec_set_error("peec_fft_set_choice: launch variant %d does not fit length %d", variant, L);

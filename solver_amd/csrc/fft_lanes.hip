@@ -26,13 +26,15 @@ using namespace fftk;
 // DFT -- and ends in natural order.  MODE 0 (forward only) stores from the
 // bit-reversed layout, MODE 1 (inverse only) loads into it.
 //
-// Registers: no LDS and no barriers, so occupancy is set by VGPRs alone.
-// The twiddles of the per-lane DFT are compile-time constants (cx_twiddle,
-// folded into the instructions); the lane-dependent ones are selected from
-// constants every right-hand side instead of being held across the loop;
+// Variant XL replaces the exchanges by an LDS transpose per Q frequencies
+// and Q-point DFTs in registers (xlds), which takes fewer VALU instructions.
+//
+// Registers and LDS: the twiddles of the per-lane DFT are compile-time
+// constants (cx_twiddle, folded into the instructions); the lane twiddles
+// are read from an LDS table filled once per block (the only barrier);
 // addresses are a wave-uniform base plus one 32-bit lane offset.  What stays
-// in registers for the block's rpb right-hand sides is the spectrum.
-
+// in registers for the block's rpb right-hand sides is the spectrum, and
+// the next right-hand side's elements where they fit.
 // exp(-2 pi i t / L) at compile time: octant reduction of t / L, Taylor
 // series of sin and cos on [0, pi/4] (within 2 ulp of a long-double
 // evaluation for every t, L <= 1024).
@@ -406,6 +408,55 @@ __device__ __forceinline__ void xl_inv(C2<T> (&a)[M], int q, const XTw<T, L, Q> 
     }
 }
 
+// The lanes of a line live in one wavefront, and a wavefront's LDS
+// operations complete in order; the wait also keeps the compiler from
+// moving LDS accesses across it.
+__device__ __forceinline__ void lds_wave_fence() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// The Q-point DFTs across the lanes of a line through a wave-private LDS
+// buffer xs (Q rows of Q + 1 entries for this line), Q frequencies k2 at a
+// time.  Forward: for chunk c, lane q writes Y_q[cQ + j] (j < Q) to row j,
+// lane p reads row p -- the Q lanes' values at k2 = cQ + p -- and takes
+// their Q-point DFT, so that slot cQ + k1 holds X[cQ + p + M k1].  Inverse:
+// the mirror image (inverse DFT over k1, row p written, column q read),
+// ending with slot k2 = cQ + j of lane q holding its natural value again.
+template <typename T, int DIR, int M, int Q>
+__device__ __forceinline__ void xlds(C2<T> (&a)[M], C2<T> *xs, int q)
+{
+    constexpr int R = Q + 1;
+#pragma unroll
+    for (int c = 0; c < M / Q; c++) {
+        C2<T> v[Q];
+        if constexpr (DIR < 0) {
+#pragma unroll
+            for (int j = 0; j < Q; j++) xs[j * R + q] = a[c * Q + j];
+            lds_wave_fence();
+#pragma unroll
+            for (int i = 0; i < Q; i++) v[i] = xs[q * R + i];
+            lds_wave_fence();
+            fdftR<T, -1, Q>(v);
+#pragma unroll
+            for (int k1 = 0; k1 < Q; k1++) a[c * Q + k1] = v[k1];
+        } else {
+#pragma unroll
+            for (int k1 = 0; k1 < Q; k1++) v[k1] = a[c * Q + k1];
+            fdftR<T, 1, Q>(v);
+#pragma unroll
+            for (int i = 0; i < Q; i++) xs[q * R + i] = v[i];
+            lds_wave_fence();
+#pragma unroll
+            for (int j = 0; j < Q; j++) a[c * Q + j] = xs[j * R + q];
+            lds_wave_fence();
+        }
+    }
+}
+
+// Frequency offset of register slot s past the lane's first frequency:
+// slot s itself after the DPP exchanges (lane p holds k2 + M rev(p)), and
+// (s / Q) Q + M (s % Q) after the LDS exchanges (lane p holds cQ + p + M k1).
+template <int M, int Q, bool XL>
+__host__ __device__ constexpr int slot_off(int s) { return XL ? (s / Q) * Q + M * (s % Q) : s; }
+
 template <typename T>
 __device__ __forceinline__ C2<T> ld_at(const char *base, uint32_t off) { return *(const C2<T> *)(base + off); }
 template <typename T>
@@ -415,7 +466,7 @@ __device__ __forceinline__ void st_at(char *base, uint32_t off, C2<T> v) { *(C2<
 // blockIdx.y, right-hand sides blockIdx.z * rpb ..; src and dst may alias
 // (every lane loads its line before any lane of it stores).  The host
 // checks that the line offsets fit the 32-bit lane offsets.
-template <typename T, int MODE, int L, int Q>
+template <typename T, int MODE, int L, int Q, bool XL>
 __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<T> *dst)
 {
     constexpr int M = L / Q, LPB = 256 / Q, LQ = log2c<Q>();
@@ -439,17 +490,23 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
     const int l2 = (int)by;
     const int rq = (int)(__brev((unsigned)q) >> (32 - LQ));
     const XTw<T, L, Q> xt(q);
+    // exchange buffer of this lane's line (LDS exchanges)
+    __shared__ C2<T> s_x[XL ? 256 * (Q + 1) : 1];
+    C2<T> *xs = s_x + (XL ? lb * Q * (Q + 1) : 0);
+    // the lane's first frequency: its bit-reversed index times M after the
+    // DPP exchanges, its own index after the LDS exchanges
+    const int lane_k0 = XL ? q : M * rq;
     // 32-bit lane offsets (bytes) of element 0 of this lane's stride pattern
     const uint32_t in_nat = (uint32_t)(lb * d.in_sb + (int64_t)q * d.in_sj) * ES;
-    const uint32_t in_rev = (uint32_t)(lb * d.in_sb + (int64_t)M * rq * d.in_sj) * ES;
+    const uint32_t in_rev = (uint32_t)(lb * d.in_sb + (int64_t)lane_k0 * d.in_sj) * ES;
     const uint32_t out_nat = (uint32_t)(lb * d.out_sb + (int64_t)q * d.out_sj) * ES;
-    const uint32_t out_rev = (uint32_t)(lb * d.out_sb + (int64_t)M * rq * d.out_sj) * ES;
+    const uint32_t out_rev = (uint32_t)(lb * d.out_sb + (int64_t)lane_k0 * d.out_sj) * ES;
     T sr[CONV ? M : 1], si[MODE == 3 ? M : 1];
     if (CONV) {
-        const int64_t sbase = (int64_t)(b0 + lb) * d.spec_sb + (int64_t)l2 * d.spec_s2 + (int64_t)M * rq * d.spec_sj;
+        const int64_t sbase = (int64_t)(b0 + lb) * d.spec_sb + (int64_t)l2 * d.spec_s2 + (int64_t)lane_k0 * d.spec_sj;
 #pragma unroll
         for (int k2 = 0; k2 < M; k2++) {
-            const int64_t s = sbase + (int64_t)k2 * d.spec_sj;
+            const int64_t s = sbase + (int64_t)slot_off<M, Q, XL>(k2) * d.spec_sj;
             if (MODE == 2) {
                 sr[k2] = live ? reinterpret_cast<const T *>(d.spec)[s] : (T)0;
             } else {
@@ -473,10 +530,25 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
         const char *in = (const char *)(src + (int64_t)b0 * d.in_sb + (int64_t)l2 * d.in_s2 + (int64_t)rhs * d.in_rhs);
         int64_t step = (MODE == 1 ? 1 : Q) * d.in_sj * (int64_t)ES;
         asm volatile("" : "+s"(step));
+        if constexpr (MODE == 1 && XL) {
+            // slots at frequencies cQ + q + M k1: Q rows apart within a chunk
+            // of slots, one chunk Q rows further
+            const int64_t sm = M * step, sq = Q * step;
 #pragma unroll
-        for (int i = 0; i < NL; i++, in += step) {
-            const int k = MODE == 1 ? i + M * rq : q + Q * i;
-            v[i] = live && k < d.nin ? ld_at<T>(in, MODE == 1 ? in_rev : in_nat) : C2<T>{(T)0, (T)0};
+            for (int c = 0; c < M / Q; c++) {
+                const char *pk = in + c * sq;
+#pragma unroll
+                for (int k1 = 0; k1 < Q; k1++, pk += sm) {
+                    const int i = c * Q + k1;
+                    v[i] = live && slot_off<M, Q, XL>(i) + lane_k0 < d.nin ? ld_at<T>(pk, in_rev) : C2<T>{(T)0, (T)0};
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NL; i++, in += step) {
+                const int k = MODE == 1 ? i + lane_k0 : q + Q * i;
+                v[i] = live && k < d.nin ? ld_at<T>(in, MODE == 1 ? in_rev : in_nat) : C2<T>{(T)0, (T)0};
+            }
         }
     };
     const int r_end = min(d.nrhs, ((int)bz + 1) * d.rpb);
@@ -498,14 +570,28 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
         if (MODE != 1) {
             cx_stages<T, -1, M, 0, HALF>(a);
             lane_twiddle<T, -1, M, Q>(a, twr);
-            xl_fwd<T, M, Q / 2, L, Q>(a, q, xt);
+            if constexpr (XL) xlds<T, -1, M, Q>(a, xs, q);
+            else xl_fwd<T, M, Q / 2, L, Q>(a, q, xt);
             if (MODE == 0) {
                 if (live) {
                     int64_t step = d.out_sj * (int64_t)ES;
                     asm volatile("" : "+s"(step));
+                    if constexpr (XL) {
+                        const int64_t sm = M * step, sq = Q * step;
 #pragma unroll
-                    for (int k2 = 0; k2 < M; k2++, out += step)
-                        if (k2 + M * rq < d.nout) st_at<T>(out, out_rev, a[k2]);
+                        for (int c = 0; c < M / Q; c++) {
+                            char *pk = out + c * sq;
+#pragma unroll
+                            for (int k1 = 0; k1 < Q; k1++, pk += sm) {
+                                const int i = c * Q + k1;
+                                if (slot_off<M, Q, XL>(i) + lane_k0 < d.nout) st_at<T>(pk, out_rev, a[i]);
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int k2 = 0; k2 < M; k2++, out += step)
+                            if (k2 + lane_k0 < d.nout) st_at<T>(out, out_rev, a[k2]);
+                    }
                 }
                 continue;
             }
@@ -515,7 +601,8 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
                 else a[k2] = fmul(a[k2], C2<T>{sr[k2], si[MODE == 3 ? k2 : 0]});
             }
         }
-        xl_inv<T, M, 1, L, Q>(a, q, xt);
+        if constexpr (XL) xlds<T, 1, M, Q>(a, xs, q);
+        else xl_inv<T, M, 1, L, Q>(a, q, xt);
         // laundered again: the forward twiddles are not kept live until here
         asm volatile("" : "+v"(twq));
         lane_twiddle<T, 1, M, Q>(a, s_tw + twq);
@@ -533,18 +620,26 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
 }  // namespace
 
 // (L, Q) pairs k_lines is built for: z convolutions (MODE 2, 3) and y
-// passes (MODE 0, 1) of the workloads' lengths, at most 64 points per lane.
+// passes (MODE 0, 1) of the workloads' lengths, at most 64 points per lane;
+// with the LDS exchanges (XL) where Q <= 8 divides M = L / Q.
 #define PEEC_LANES_Z(X) X(16, 4) X(32, 2) X(32, 4) X(32, 8) X(48, 2) X(48, 4) X(48, 8) X(64, 2) X(64, 4) X(64, 8) \
     X(64, 16) X(96, 4) X(96, 8) X(96, 16) X(128, 4) X(128, 8) X(128, 16)
 #define PEEC_LANES_Y(X) X(128, 4) X(128, 8) X(128, 16) X(192, 4) X(192, 8) X(192, 16) X(200, 4) X(200, 8) X(384, 8) \
     X(384, 16) X(384, 32) X(400, 8) X(400, 16)
+#define PEEC_LANES_ZX(X) X(32, 4) X(48, 4) X(64, 4) X(64, 8) X(96, 4) X(128, 4) X(128, 8)
+#define PEEC_LANES_YX(X) X(128, 4) X(128, 8) X(192, 4) X(192, 8) X(384, 8)
 
-bool lane_built(int L, int Q, int mode)
+bool lane_built(int L, int Q, int mode, bool xl)
 {
     bool ok = false;
 #define PEEC_LANE_HAS(L_, Q_) ok = ok || (L == L_ && Q == Q_);
-    if (mode >= 2) { PEEC_LANES_Z(PEEC_LANE_HAS) }
-    else { PEEC_LANES_Y(PEEC_LANE_HAS) }
+    if (xl) {
+        if (mode >= 2) { PEEC_LANES_ZX(PEEC_LANE_HAS) }
+        else { PEEC_LANES_YX(PEEC_LANE_HAS) }
+    } else {
+        if (mode >= 2) { PEEC_LANES_Z(PEEC_LANE_HAS) }
+        else { PEEC_LANES_Y(PEEC_LANE_HAS) }
+    }
 #undef PEEC_LANE_HAS
     return ok;
 }
@@ -552,16 +647,16 @@ bool lane_built(int L, int Q, int mode)
 // the lane kernel applies to a pass whose tile axis is contiguous and whose
 // lines are strided, with the forward input (modes 0, 2, 3) and the inverse
 // output (modes 1, 2, 3) in the lower half of the line
-bool lane_fits(const PassDesc &d, int Q, int mode)
+bool lane_fits(const PassDesc &d, int Q, int mode, bool xl)
 {
     // the kernel's lane offsets are 32-bit byte offsets within L line elements
     const int64_t span = ((int64_t)d.L * std::max(d.in_sj, d.out_sj) + 256) * 16;
-    return Q >= 2 && d.L % Q == 0 && lane_built(d.L, Q, mode) && d.in_sb == 1 && d.out_sb == 1 && d.in_sj > 1 &&
+    return Q >= 2 && d.L % Q == 0 && lane_built(d.L, Q, mode, xl) && d.in_sb == 1 && d.out_sb == 1 && d.in_sj > 1 &&
            d.out_sj > 1 && d.nin <= d.L && d.nout <= d.L && span < ((int64_t)1 << 31) &&
            (mode == 1 || 2 * d.nin <= d.L) && (mode == 0 || 2 * d.nout <= d.L);
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, bool XL>
 static int launch_lines_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst, int Q)
 {
     const int rpb = std::max(1, d.rpb), lpb = 256 / Q;
@@ -571,31 +666,43 @@ static int launch_lines_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst
     bool done = false;
 #define PEEC_LANE_LAUNCH(L_, Q_)                                                                                   \
     if (!done && d.L == L_ && Q == Q_) {                                                                           \
-        hipLaunchKernelGGL((k_lines<T, MODE, L_, Q_>), grid, dim3(256), 0, ctx->stream, d, s, o);              \
+        hipLaunchKernelGGL((k_lines<T, MODE, L_, Q_, XL>), grid, dim3(256), 0, ctx->stream, d, s, o);          \
         done = true;                                                                                               \
     }
-    if constexpr (MODE >= 2) { PEEC_LANES_Z(PEEC_LANE_LAUNCH) }
-    else { PEEC_LANES_Y(PEEC_LANE_LAUNCH) }
+    if constexpr (XL) {
+        if constexpr (MODE >= 2) { PEEC_LANES_ZX(PEEC_LANE_LAUNCH) }
+        else { PEEC_LANES_YX(PEEC_LANE_LAUNCH) }
+    } else {
+        if constexpr (MODE >= 2) { PEEC_LANES_Z(PEEC_LANE_LAUNCH) }
+        else { PEEC_LANES_Y(PEEC_LANE_LAUNCH) }
+    }
 #undef PEEC_LANE_LAUNCH
     if (!done) {
-        peec_set_error("lane-split FFT pass: no kernel for length %d with %d lanes per line", d.L, Q);
+        peec_set_error("lane-split FFT pass: no kernel for length %d with %d lanes per line%s", d.L, Q,
+                       XL ? " (LDS exchanges)" : "");
         return PEEC_ERR_ARG;
     }
     PEEC_HIP_TRY(hipGetLastError());
     return PEEC_OK;
 }
 
-
-int launch_lane_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const void *src, void *dst, int Q)
+template <bool XL>
+static int launch_lane_pass_t(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const void *src, void *dst, int Q)
 {
     if (fp32) {
-        if (mode == 0) return launch_lines_t<float, 0>(ctx, d, src, dst, Q);
-        if (mode == 1) return launch_lines_t<float, 1>(ctx, d, src, dst, Q);
-        if (mode == 2) return launch_lines_t<float, 2>(ctx, d, src, dst, Q);
-        return launch_lines_t<float, 3>(ctx, d, src, dst, Q);
+        if (mode == 0) return launch_lines_t<float, 0, XL>(ctx, d, src, dst, Q);
+        if (mode == 1) return launch_lines_t<float, 1, XL>(ctx, d, src, dst, Q);
+        if (mode == 2) return launch_lines_t<float, 2, XL>(ctx, d, src, dst, Q);
+        return launch_lines_t<float, 3, XL>(ctx, d, src, dst, Q);
     }
-    if (mode == 0) return launch_lines_t<double, 0>(ctx, d, src, dst, Q);
-    if (mode == 1) return launch_lines_t<double, 1>(ctx, d, src, dst, Q);
-    if (mode == 2) return launch_lines_t<double, 2>(ctx, d, src, dst, Q);
-    return launch_lines_t<double, 3>(ctx, d, src, dst, Q);
+    if (mode == 0) return launch_lines_t<double, 0, XL>(ctx, d, src, dst, Q);
+    if (mode == 1) return launch_lines_t<double, 1, XL>(ctx, d, src, dst, Q);
+    if (mode == 2) return launch_lines_t<double, 2, XL>(ctx, d, src, dst, Q);
+    return launch_lines_t<double, 3, XL>(ctx, d, src, dst, Q);
+}
+
+int launch_lane_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const void *src, void *dst, int Q, bool xl)
+{
+    return xl ? launch_lane_pass_t<true>(ctx, fp32, mode, d, src, dst, Q)
+              : launch_lane_pass_t<false>(ctx, fp32, mode, d, src, dst, Q);
 }

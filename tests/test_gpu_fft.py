@@ -11,8 +11,8 @@ path bit for bit, also with the 48- and 64-point z kernels.  The complex64
 radix schedules change rounding only: they stay within complex64 accuracy
 of the complex128 result.  The complex128 result itself is held to the
 oracle's K x (1e-12, relative).  The lane-split passes (a line spread over
-Q lanes, cross-lane butterflies) compute the same transforms in another
-operation order: complex128 within 1e-13 of the LDS passes and 1e-12 of
+Q lanes, cross-lane butterflies by DPP and swizzles or through LDS) compute
+the same transforms in another operation order: complex128 within 1e-13 of the LDS passes and 1e-12 of
 the oracle, complex64 within 1e-5 of complex128.
 """
 import numpy as np
@@ -28,6 +28,7 @@ pytestmark = pytest.mark.gpu
 NRHS = 3
 N_VARIANTS = 12  # 11 LDS tile shapes + the register z kernel (fused z pass only)
 LANE_BASE, N_LANE = 12, 5  # lane-split passes with 2, 4, .. 32 lanes per line (y and z passes)
+LANE_XL_BASE = LANE_BASE + N_LANE  # the same with the exchanges through LDS, 4 and 8 lanes per line
 PASS_YZ = 6      # tuning-table pass of the fused y-z pass: variant 0 fused, 1 five passes
 
 
@@ -231,12 +232,12 @@ def lanes_geometry():
     return M, S, ws, X
 
 
-def set_lanes(S, fp32, Q, rpb, nrhs):
-    """Force the lane-split kernel with Q lanes per line on the y and z
-    passes of every kernel where it is built; returns (passes taken,
-    z passes taken)."""
+def set_lanes(S, fp32, Q, rpb, nrhs, xl=False):
+    """Force the lane-split kernel with Q lanes per line (xl: exchanges
+    through LDS) on the y and z passes of every kernel where it is built;
+    returns (passes taken, z passes taken)."""
     took = tz = 0
-    v = LANE_BASE + int(np.log2(Q)) - 1
+    v = LANE_XL_BASE + int(np.log2(Q)) - 2 if xl else LANE_BASE + int(np.log2(Q)) - 1
     for k in range(9):
         if fp32:
             set_fused(S, k, False, nrhs=nrhs)
@@ -258,30 +259,33 @@ def test_lane_split_passes_match(lanes_geometry, fp32):
     y64 = peec.apply_K_batched(X, rps, fp32=False, ctx=S.ctx)
     ref = peec.apply_K_batched(X, rps, fp32=fp32, ctx=S.ctx)
     total = 0
-    for Q in (4, 8, 16):
+    for Q, xl in ((4, False), (8, False), (16, False), (4, True), (8, True)):
         for rpb in (1, 3):
-            took, tz = set_lanes(S, fp32, Q, rpb, nrhs)
+            took, tz = set_lanes(S, fp32, Q, rpb, nrhs, xl)
             if took == 0:
                 continue
             total += took
             Y = peec.apply_K_batched(X, rps, fp32=fp32, ctx=S.ctx)
             e, e64 = rel(Y, ref), rel(Y, y64)
-            print("fp32=%s Q %d rpb %d: lane passes on %d passes (%d z); vs LDS passes %.2e, vs complex128 %.2e"
-                  % (fp32, Q, rpb, took, tz, e, e64), flush=True)
+            print("fp32=%s Q %d%s rpb %d: lane passes on %d passes (%d z); vs LDS passes %.2e, vs complex128 %.2e"
+                  % (fp32, Q, " (LDS exchanges)" if xl else "", rpb, took, tz, e, e64), flush=True)
             assert e64 < (1e-5 if fp32 else 1e-13)
         clear_all(S, nrhs)
     assert total > 0
     if not fp32:
         rows = OF.kernel_rows(M.origin, M.Nx, M.Ny, M.Nz, M.sx, M.sy, M.sz)
         spec = OF.spectra(rows, M.Nx, M.Ny, M.Nz)
-        set_lanes(S, False, 8, 1, nrhs)
-        Y = peec.apply_K_batched(X, rps, fp32=False, ctx=S.ctx)
-        for r in range(nrhs):
-            w = ws[r // rps]
-            op = OF.Operator(M, spec)
-            Yle = OF.build_Yle(M.lumped_elements, M.port_nodes, M.escalings["Yle"], M.n, w / M.escalings["freq"], 50.0)
-            op.set_frequency(w, OF.frequency_terms(M, w), Yle)
-            e = rel(Y[:, r], np.concatenate(op.K(X[:, r])))
-            print("lane passes Q 8, column %d: K x vs oracle %.2e" % (r, e), flush=True)
-            assert e < 1e-12
-        clear_all(S, nrhs)
+        for Q, xl in ((8, False), (4, True)):
+            set_lanes(S, False, Q, 1, nrhs, xl)
+            Y = peec.apply_K_batched(X, rps, fp32=False, ctx=S.ctx)
+            for r in range(nrhs):
+                w = ws[r // rps]
+                op = OF.Operator(M, spec)
+                Yle = OF.build_Yle(M.lumped_elements, M.port_nodes, M.escalings["Yle"], M.n, w / M.escalings["freq"],
+                                   50.0)
+                op.set_frequency(w, OF.frequency_terms(M, w), Yle)
+                e = rel(Y[:, r], np.concatenate(op.K(X[:, r])))
+                print("lane passes Q %d%s, column %d: K x vs oracle %.2e" % (Q, " (LDS exchanges)" if xl else "", r, e),
+                      flush=True)
+                assert e < 1e-12
+            clear_all(S, nrhs)
