@@ -56,7 +56,8 @@ MEASURED_TRAFFIC = {("large_192x192x48", "fft_x_fwd_scatter"): 369.9e6,
                     ("large_192x192x48", "fft_y_inv"): 758.3e6,
                     ("large_192x192x48", "fft_x_inv_gather"): 409.8e6}
 KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv_gather",
-              "trsv_block_L", "trsv_block_U", "lu_factor", "fft_yz_conv", "trsv_panel"]
+              "trsv_block_L", "trsv_block_U", "lu_factor", "fft_yz_conv", "trsv_panel", "mf_small", "mf_gemv",
+              "mf_aux"]
 # kernel symbols behind each profile kind: the LDS pass k_fft_pass<T, MODE,
 # TPL, LPT, LMAX> (MODE 0 forward, 1 inverse, 2/3 forward * spectrum *
 # inverse), the register z kernel k_zconv_reg<T, MODE, L> and the
@@ -68,7 +69,9 @@ KERNEL_SYMBOLS = {"fft_x_fwd_scatter": "k_fft_pass<T, 0, TPL, LPT, LMAX>",
                   "fft_x_inv_gather": "k_fft_pass<T, 1, TPL, LPT, LMAX>", "trsv_gather": "k_trsv_gather4",
                   "trsv_block_L": "k_trsv_block<true>", "trsv_block_U": "k_trsv_block<false>",
                   "lu_factor": "peec_lu_factor (all its kernels)",
-                  "fft_yz_conv": "k_yz_conv<float, MODE, 32, LPT, 256, LZ>", "trsv_panel": "k_trsv_panel"}
+                  "fft_yz_conv": "k_yz_conv<float, MODE, 32, LPT, 256, LZ>", "trsv_panel": "k_trsv_panel",
+                  "mf_small": "k_mf_fwd_small<Q>, k_mf_bwd_small<Q>", "mf_gemv": "k_mf_gemv<Q, GATHER>",
+                  "mf_aux": "k_mf_assemble, k_mf_finish<KIND>"}
 
 
 def log(*a):

@@ -177,7 +177,7 @@ int peec_lu_factor_slot(peec_ctx *ctx, int slot);
 /* Factors the SS of slots slot0 .. slot0 + count - 1 (no counterpart in the
  * reference): the slots go through each launch of the multifrontal schedule
  * together (one copy of the front pool per slot, in groups of at most
- * 48 GiB of pools), so the small fronts near the leaves run as one wide
+ * 80 GiB of pools), so the small fronts near the leaves run as one wide
  * launch instead of one per slot.  Same arithmetic as
  * peec_lu_factor_slot on each slot. */
 int peec_lu_factor_slots(peec_ctx *ctx, int slot0, int count);
@@ -233,6 +233,20 @@ int peec_lu_info(peec_ctx *ctx, int64_t *out8);
  * pivot position). */
 int peec_lu_front(peec_ctx *ctx, int64_t K, int64_t *f, int64_t *s, int64_t *first, peec_c128 *out,
                   int64_t *positions);
+/* Which triangular solves apply the device factorization inside
+ * precond_3_3_vector (compute_Matrix_vector_fft.jl:103-114, where the
+ * reference calls F\ three times with UMFPACK's factors; no counterpart for
+ * the choice itself).  mode 1 (default): multifrontal, front by front along
+ * the assembly tree, reading L11^-1, L21, U12 and U11^-1 of each front in
+ * place from the factorization's pool; the factorization then also forms
+ * the pivot-block inverses.  It applies when one peec_lu_factor_slots call
+ * (or peec_lu_factor with one slot) factored every slot at once, so that
+ * the pool holds all of them; otherwise, and with mode 0, the solves use
+ * the level-scheduled 64-row blocks.  Mode 1 takes effect with the next
+ * such factorization, mode 0 at once.  peec_lu_solve_info reports out4 =
+ * {mode, multifrontal in use, tree height, most GEMV tasks in a launch}. */
+int peec_lu_set_solve(peec_ctx *ctx, int mode);
+int peec_lu_solve_info(peec_ctx *ctx, int64_t *out4);
 
 /* ComputeMatrixVectorFTT(x, w, ...) including the precond_3_3_vector step
  * (compute_Matrix_vector_fft.jl:1-146): y = M^-1 K x, x and y of length

@@ -71,6 +71,8 @@ def lib():
         L.peec_lu_factor_slot.argtypes = [vp, ctypes.c_int]
         L.peec_lu_factor_slots.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.peec_lu_info.argtypes = [vp, i64p]
+        L.peec_lu_set_solve.argtypes = [vp, ctypes.c_int]
+        L.peec_lu_solve_info.argtypes = [vp, i64p]
         L.peec_gmres_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, c128p, ctypes.c_double, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_double, c128p, ip, dp, ip, ip]
         L.peec_precond_rhs_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, c128p]

@@ -15,7 +15,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpeec_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["api.cpp", "integrals.hip", "fft.hip", "fft_lanes.hip", "operator.hip", "trsv.hip", "lu.hip", "gmres.hip", "bgmres.hip"]
+SOURCES = ["api.cpp", "integrals.hip", "fft.hip", "fft_lanes.hip", "operator.hip", "trsv.hip", "mfsolve.hip", "lu.hip", "gmres.hip", "bgmres.hip"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-result", "-I", CSRC]

@@ -140,10 +140,12 @@ struct ConvApp {
 struct LuFrontInfo {
     int64_t off;
     int32_t f, s, first, parent;
+    int64_t inv;  // pool offset of the inverses of the pivot block: L11^-1 (s x s), then U11^-1
 };
 
-// A complex GEMM tile C(m x n) = A(m x k) B(k x n) (mode 0) or
-// C -= A B (mode 1); offsets into the pool, column-major, m, n <= 64.
+// A complex GEMM tile C(m x n) = A(m x k) B(k x n) (mode 0), C -= A B
+// (mode 1) or C = -A B (mode 2); offsets into the pool, column-major,
+// m, n <= 64.
 struct LuGemmItem {
     int64_t a, b, c;
     int32_t lda, ldb, ldc, m, n, k, mode, pad;
@@ -180,10 +182,57 @@ struct LuPlan {
     DevBuf<int4> diag_items;    // (front, scratch slot, first pivot, -)
     DevBuf<LuGemmItem> gemm_items;
     int64_t scratch0 = 0;       // pool offset of the per-slot 64 x 64 inverse pairs
+    // inverses of every front's pivot block (the multifrontal solves): the
+    // 64 x 64 diagonal blocks come from k_lu_diag, the others from GEMM tiles
+    // in inv_sched after the factorization
+    int64_t inv0 = 0, inv_n = 0;
+    std::vector<Launch> inv_sched;
+    DevBuf<LuGemmItem> inv_items;
+    int pool_slot0 = -1, pool_ns = 0;  // slots whose fronts the pool holds now
+    bool pool_inv = false;             // ... with their inverses
     // values of the triangular-solve layouts, gathered from the pool
     DevBuf<int64_t> l_ob_src, l_dense_src, u_ob_src, u_dense_src, u_diag_src;
     DevBuf<int32_t> bad_pivot;  // per slot of a factorization group: 1 = a zero or non-finite pivot
     double flops = 0.0;
+};
+
+// Multifrontal triangular solves (mfsolve.hip) with the device LU's fronts,
+// read in place from the pool: per front its pool offsets and where its
+// vectors live.  u (the update a front passes to its parent, b rows) is kept
+// for every front; w (the front's working rows, f) only for large fronts.
+struct MfFront {
+    int64_t off, inv;    // pool offsets: the front (f x f), its pivot-block inverses
+    int64_t uoff, woff;  // offsets of its u rows / w rows in one right-hand side's u / w area
+    int64_t crow, bptr;  // first contribution row (cptr) / first boundary position (bnd)
+    int32_t f, s, first, pad;
+};
+// A GEMV task of a large front: rows r0 .. r0 + m of an output segment from
+// columns k0 .. k0 + kc of A (pool offset a of element (r0, k0), leading
+// dimension lda) against the same rows of the right-hand side segment.
+struct MfTask {
+    int64_t a;
+    int32_t lda, m, kc, front, r0, k0;
+};
+// Sums the partials of tasks p0 .. p0 + np of one launch for rows r0 .. r0 + m.
+struct MfFin {
+    int32_t front, r0, m, p0, np, pad;
+};
+struct MfPlan {
+    bool ready = false;
+    int H = 0;                           // tree heights
+    int64_t utot = 0, wtot = 0;          // u / w rows per right-hand side
+    int64_t max_tasks = 1;               // most tasks in one GEMV launch
+    std::vector<int64_t> small_ptr;      // per height: fronts small_list[small_ptr[h] ..)
+    std::vector<int64_t> row_ptr;        // per height: large-front rows rows[row_ptr[h] ..)
+    std::vector<int64_t> task_ptr;       // per (height, phase 0..3): tasks[task_ptr[4h + p] ..)
+    std::vector<int64_t> fin_ptr;        // per (height, phase 0..3): fins[fin_ptr[4h + p] ..)
+    std::vector<double> small_fwd_bytes, small_bwd_bytes, task_bytes;  // algorithmic factor bytes per slot
+    DevBuf<MfFront> fr;
+    DevBuf<int32_t> small_list, bnd;
+    DevBuf<int2> rows;                   // (front, row)
+    DevBuf<MfTask> tasks;
+    DevBuf<MfFin> fins;
+    DevBuf<int64_t> cptr, csrc;          // per front row: the u rows of its children that add into it
 };
 
 // Sparse matrix in CSR (0-based, int32 indices) with real or complex values.
@@ -262,7 +311,7 @@ struct ProfRec {
 };
 enum ProfKind {
     PK_FFT_X_FWD = 0, PK_FFT_Y_FWD, PK_FFT_Z_CONV, PK_FFT_Y_INV, PK_FFT_X_INV, PK_TRSV_GATHER, PK_TRSV_BLOCK_L, PK_TRSV_BLOCK_U, PK_LU,
-    PK_FFT_YZ_CONV, PK_TRSV_PANEL, PK_COUNT
+    PK_FFT_YZ_CONV, PK_TRSV_PANEL, PK_MF_SMALL, PK_MF_GEMV, PK_MF_AUX, PK_COUNT
 };
 
 // The batched GMRES workspace, defined in bgmres.hip.
@@ -346,6 +395,12 @@ struct peec_ctx {
     bool has_Rs = false;
     LuPlan lu;                       // device factorization (lu.hip)
     bool lu_layout_from_plan = false;  // Lf/Uf hold the plan's layouts
+    // triangular solves: 1 = multifrontal from the pool when it holds every
+    // slot's fronts and inverses (mf_ready), 0 = the level-scheduled blocks
+    int lu_solve = 1;
+    bool mf_ready = false;
+    MfPlan mf;
+    DevBuf<double2_t> mf_w, mf_u, mf_part;
 
     // ---- vectors
     DevBuf<double2_t> vx, vy, vt1, vt2, vt3;  // N-sized work vectors
@@ -387,6 +442,9 @@ int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std
 int tri_set_slots(TriFactor *f, int nslots);  // reallocates the value sets (contents undefined)
 std::vector<int32_t> tri_chain_blocks(const peec_csc_c128 *L, int cap);
 int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f, const char *what);
+int mf_analyze(peec_ctx *ctx);  // mfsolve.hip: the plan of the multifrontal solves (after the LU plan)
+int mf_prepare(peec_ctx *ctx, int nrhs);  // allocates the solve buffers for nrhs right-hand sides
+int mf_solve(peec_ctx *ctx, bool lower, const double2_t *b, double2_t *x, int nrhs, const ColMap &cm);
 int trsv_solve(peec_ctx *ctx, const TriFactor &f, bool lower, const double2_t *b, double2_t *x, int nrhs,
                const ColMap &cm);
 

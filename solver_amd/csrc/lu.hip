@@ -110,7 +110,8 @@ __global__ void __launch_bounds__(256) k_lu_extend_add(const int2 *items, const 
 
 // LU (no pivoting) of the w x w diagonal block at pivot k0 of a front, in
 // LDS; the factors go back to the front, the inverses of U and of the unit
-// lower L to the item's scratch slot (64 x 64 each, column-major).
+// lower L to the item's scratch slot (64 x 64 each, column-major) and to the
+// diagonal blocks of the front's pivot-block inverses.
 __global__ void __launch_bounds__(256) k_lu_diag(const int4 *items, const LuFrontInfo *fr, cd *pool0, int64_t scratch0,
                                                  int64_t stride)
 {
@@ -175,6 +176,9 @@ __global__ void __launch_bounds__(256) k_lu_diag(const int4 *items, const LuFron
         __syncthreads();
     }
     for (int e = tid; e < 64 * 64; e += 256) Ui[e] = X[e];
+    // the w x w block also goes to the front's U11^-1 (diagonal block at k0)
+    cd *Uinv = pool + F.inv + (int64_t)F.s * F.s + (int64_t)k0 * F.s + k0;
+    for (int e = tid; e < w * w; e += 256) Uinv[(int64_t)(e / w) * F.s + e % w] = X[(e / w) * 64 + e % w];
     __syncthreads();
     // X = L^-1 (unit lower), row by row from the top: X[i][j] = -sum_{k=j..i-1} L[i][k] X[k][j]
     for (int e = tid; e < 64 * 64; e += 256) X[e] = {0.0, 0.0};
@@ -200,6 +204,8 @@ __global__ void __launch_bounds__(256) k_lu_diag(const int4 *items, const LuFron
         __syncthreads();
     }
     for (int e = tid; e < 64 * 64; e += 256) Li[e] = X[e];
+    cd *Linv = pool + F.inv + (int64_t)k0 * F.s + k0;
+    for (int e = tid; e < w * w; e += 256) Linv[(int64_t)(e / w) * F.s + e % w] = X[(e / w) * 64 + e % w];
 }
 
 // GEMM tiles (see LuGemmItem).  256 threads, 4 x 4 outputs each; k in
@@ -253,9 +259,11 @@ __global__ void __launch_bounds__(256) k_lu_gemm(const LuGemmItem *items, cd *po
             const int r = ty + 16 * i, c = tx + 16 * jj;
             if (r < it.m && c < it.n) {
                 cd *p = Cp + (int64_t)c * it.ldc + r;
-                if (it.mode) {
+                if (it.mode == 1) {
                     const cd o = *p;
                     *p = {o.x - ax[i][jj], o.y - ay[i][jj]};
+                } else if (it.mode == 2) {
+                    *p = {-ax[i][jj], -ay[i][jj]};
                 } else {
                     *p = {ax[i][jj], ay[i][jj]};
                 }
@@ -471,6 +479,14 @@ extern "C" int peec_lu_analyze(peec_ctx *ctx, int64_t n, const int64_t *order, i
         P.flops += 8.0 * (s * s * s / 3.0 + s * s * bb + s * bb * bb);
     }
     P.pool_fronts = off;
+    // pivot-block inverses after the fronts: L11^-1 then U11^-1, s x s each
+    P.inv0 = off;
+    for (int64_t K = 0; K < nfront; K++) {
+        LuFrontInfo &F = P.fronts[(size_t)K];
+        F.inv = off;
+        off += 2 * (int64_t)F.s * F.s;
+    }
+    P.inv_n = off - P.inv0;
     // child boundary -> parent local index
     std::vector<int64_t> rel_off((size_t)nfront, 0);
     std::vector<int32_t> rel;
@@ -652,12 +668,89 @@ extern "C" int peec_lu_analyze(peec_ctx *ctx, int64_t n, const int64_t *order, i
             if ((int64_t)gi.size() > t0) P.sched.push_back({2, t0, (int64_t)gi.size() - t0});
         }
     }
-    P.scratch0 = P.pool_fronts;
-    P.pool_total = P.pool_fronts + max_slots * 8192;
+    P.scratch0 = P.inv0 + P.inv_n;
+    P.pool_total = P.scratch0 + max_slots * 8192;
     for (auto &g : gi) {
         if (g.pad == 1) g.b = P.scratch0 + (-1 - g.b) * 8192;         // U^-1 of the slot
         if (g.pad == 2) g.a = P.scratch0 + (-1 - g.a) * 8192 + 4096;  // L^-1 of the slot
         g.pad = 0;
+    }
+    // ---- off-diagonal 64 x 64 blocks of the pivot-block inverses, block row
+    // by block row (k_lu_diag gives the diagonal ones).  X = L11^-1:
+    // X_ij = -Linv_ii sum_{k=j..i-1} L_ik X_kj for j < i, rows i = 1, 2, ...;
+    // X = U11^-1: X_ij = -Uinv_ii sum_{k=i+1..j} U_ik X_kj for j > i, rows
+    // i = nb-2, nb-3, ...  Step t does row t of every L11^-1 and row nb-1-t
+    // of every U11^-1: one launch forms the sums in place, the next
+    // multiplies them by -Linv_ii / -Uinv_ii (B is C, read before written).
+    std::vector<LuGemmItem> ii;
+    P.inv_sched.clear();
+    {
+        int maxnb = 0;
+        for (const auto &F : P.fronts) maxnb = std::max(maxnb, (F.s + 63) / 64);
+        for (int t = 1; t < maxnb; t++) {
+            for (int pass = 0; pass < 2; pass++) {
+                const int64_t a0 = (int64_t)ii.size();
+                for (const auto &F : P.fronts) {
+                    const int s = F.s, nb = (s + 63) / 64;
+                    if (nb <= t) continue;
+                    const int64_t ss = (int64_t)s * s;
+                    {
+                        const int i = t, wi = std::min(64, s - 64 * i);
+                        for (int j = 0; j < i; j++) {
+                            LuGemmItem g{};
+                            g.c = F.inv + (int64_t)64 * j * s + 64 * i;
+                            g.ldc = s;
+                            g.m = wi;
+                            g.n = 64;
+                            if (pass == 0) {
+                                g.a = F.off + (int64_t)64 * j * F.f + 64 * i;
+                                g.lda = F.f;
+                                g.b = F.inv + (int64_t)64 * j * s + 64 * j;
+                                g.ldb = s;
+                                g.k = 64 * (i - j);
+                                g.mode = 0;
+                            } else {
+                                g.a = F.inv + (int64_t)64 * i * s + 64 * i;
+                                g.lda = s;
+                                g.b = g.c;
+                                g.ldb = s;
+                                g.k = wi;
+                                g.mode = 2;
+                            }
+                            ii.push_back(g);
+                        }
+                    }
+                    {
+                        const int i = nb - 1 - t;
+                        for (int j = i + 1; j < nb; j++) {
+                            const int wj = std::min(64, s - 64 * j);
+                            LuGemmItem g{};
+                            g.c = F.inv + ss + (int64_t)64 * j * s + 64 * i;
+                            g.ldc = s;
+                            g.m = 64;
+                            g.n = wj;
+                            if (pass == 0) {
+                                g.a = F.off + (int64_t)64 * (i + 1) * F.f + 64 * i;
+                                g.lda = F.f;
+                                g.b = F.inv + ss + (int64_t)64 * j * s + 64 * (i + 1);
+                                g.ldb = s;
+                                g.k = 64 * j + wj - 64 * (i + 1);
+                                g.mode = 0;
+                            } else {
+                                g.a = F.inv + ss + (int64_t)64 * i * s + 64 * i;
+                                g.lda = s;
+                                g.b = g.c;
+                                g.ldb = s;
+                                g.k = 64;
+                                g.mode = 2;
+                            }
+                            ii.push_back(g);
+                        }
+                    }
+                }
+                if ((int64_t)ii.size() > a0) P.inv_sched.push_back({2, a0, (int64_t)ii.size() - a0});
+            }
+        }
     }
     // ---- layouts of the triangular solves, with pool sources
     auto gpos = [&](int K, int li) -> int32_t {
@@ -702,7 +795,7 @@ extern "C" int peec_lu_analyze(peec_ctx *ctx, int64_t n, const int64_t *order, i
         (rc = upload(P.d_fronts, P.fronts)) || (rc = upload(P.ss_tgt, tgt)) || (rc = upload(P.ss_cptr, cptr)) ||
         (rc = upload(P.ss_cedge, cedge)) || (rc = upload(P.ss_ccoef, ccoef)) || (rc = upload(P.ss_gsum, gsum)) ||
         (rc = upload(P.ea_items, ea)) || (rc = upload(P.rel_off, rel_off)) || (rc = upload(P.rel, rel)) ||
-        (rc = upload(P.diag_items, di)) || (rc = upload(P.gemm_items, gi)))
+        (rc = upload(P.diag_items, di)) || (rc = upload(P.gemm_items, gi)) || (rc = upload(P.inv_items, ii)))
         return rc;
     if (!P.pool.alloc((size_t)P.pool_total)) {
         peec_set_error("peec_lu_analyze: out of device memory for %lld front entries", (long long)P.pool_total);
@@ -717,7 +810,11 @@ extern "C" int peec_lu_analyze(peec_ctx *ctx, int64_t n, const int64_t *order, i
     ctx->slot_lu.assign((size_t)ctx->nslots, 0);
     ctx->lu_layout_from_plan = true;
     P.ready = true;
-    return PEEC_OK;
+    P.pool_slot0 = -1;
+    P.pool_ns = 0;
+    P.pool_inv = false;
+    ctx->mf_ready = false;
+    return mf_analyze(ctx);
 }
 
 // Factor slots slot0 .. slot0 + ns - 1 together: every launch of the
@@ -781,6 +878,16 @@ static int lu_factor_range(peec_ctx *ctx, int slot0, int ns)
         else
             hipLaunchKernelGGL(k_lu_gemm, grid, dim3(256), 0, ctx->stream, P.gemm_items.p + L.first, pool, stride);
     }
+    // the pivot-block inverses of the multifrontal solves, when the pool
+    // holds every slot's fronts (the solves read them there)
+    P.pool_slot0 = slot0;
+    P.pool_ns = ns;
+    P.pool_inv = ctx->lu_solve == 1 && ctx->mf.ready && slot0 == 0 && ns == ctx->nslots;
+    ctx->mf_ready = false;
+    if (P.pool_inv)
+        for (const auto &L : P.inv_sched)
+            hipLaunchKernelGGL(k_lu_gemm, dim3((unsigned)L.count, ny), dim3(256), 0, ctx->stream,
+                               P.inv_items.p + L.first, pool, stride);
     PEEC_HIP_TRY(hipGetLastError());
     // gather into the slots' value sets of the triangular-solve layouts
     TriFactor &Lf = ctx->Lf, &Uf = ctx->Uf;
@@ -828,6 +935,7 @@ static int lu_factor_range(peec_ctx *ctx, int slot0, int ns)
         }
     for (int y = 0; y < ns; y++) ctx->slot_lu[(size_t)(slot0 + y)] = 1;
     ctx->lu_ready = std::all_of(ctx->slot_lu.begin(), ctx->slot_lu.end(), [](char c) { return c != 0; });
+    ctx->mf_ready = P.pool_inv;
     return PEEC_OK;
 }
 
@@ -860,7 +968,7 @@ extern "C" int peec_lu_factor_slot(peec_ctx *ctx, int slot)
 }
 
 // Slots slot0 .. slot0 + count - 1, in groups whose front pools fit kLuBatchBytes.
-static constexpr int64_t kLuBatchBytes = (int64_t)48 << 30;
+static constexpr int64_t kLuBatchBytes = (int64_t)80 << 30;
 extern "C" int peec_lu_factor_slots(peec_ctx *ctx, int slot0, int count)
 {
     if (count < 1) {

@@ -42,6 +42,11 @@ void peec_ctx::release_operator()
     hw.clear();
     lu_layout_from_plan = false;
     lu.ready = false;
+    mf_ready = false;
+    mf.ready = false;
+    mf_w.reset();
+    mf_u.reset();
+    mf_part.reset();
     fft_tune.clear();
     sweep_x0.reset();
     sweep_nports = 0;
@@ -757,6 +762,7 @@ extern "C" int peec_frequency_slots(peec_ctx *ctx, int nslots)
     ctx->hYle.assign((size_t)nslots, peec_ctx::HostCsr());
     ctx->slot_set.assign((size_t)nslots, 0);
     ctx->slot_lu.assign((size_t)nslots, 0);
+    ctx->mf_ready = false;
     ctx->freq_ready = false;
     ctx->lu_ready = false;
     int rc;
@@ -1064,6 +1070,7 @@ extern "C" int peec_lu_set_factors_blocked(peec_ctx *ctx, int64_t n, const peec_
     ctx->slot_lu.assign(1, 1);
     ctx->lu_ready = true;
     ctx->lu_layout_from_plan = false;
+    ctx->mf_ready = false;
     return PEEC_OK;
 }
 
@@ -1082,8 +1089,14 @@ int apply_precond(peec_ctx *ctx, double2_t *y, int nrhs, int64_t ld, const ColMa
                        ctx->has_Rs ? ctx->Rs.p : nullptr, b, ctx->wv.p, m, ns, n, nrhs, ld, cm);
     PEEC_HIP_TRY(hipGetLastError());
     int rc;
-    if ((rc = trsv_solve(ctx, ctx->Lf, true, b, z, nrhs, cm))) return rc;   // L c = b   (c in z)
-    if ((rc = trsv_solve(ctx, ctx->Uf, false, z, b, nrhs, cm))) return rc;  // U z' = c  (z' in b)
+    if (ctx->mf_ready && ctx->lu_solve == 1 && ctx->lu_layout_from_plan) {
+        // multifrontal, from the device factorization's fronts
+        if ((rc = mf_solve(ctx, true, b, z, nrhs, cm))) return rc;
+        if ((rc = mf_solve(ctx, false, z, b, nrhs, cm))) return rc;
+    } else {
+        if ((rc = trsv_solve(ctx, ctx->Lf, true, b, z, nrhs, cm))) return rc;   // L c = b   (c in z)
+        if ((rc = trsv_solve(ctx, ctx->Uf, false, z, b, nrhs, cm))) return rc;  // U z' = c  (z' in b)
+    }
     hipLaunchKernelGGL(k_post, dim3(nblk(m + ns + n), (unsigned)cm.nsl), dim3(256), 0, ctx->stream, y, b, ctx->perm_c.p, ctx->invZ.p,
                        ctx->invP.p, ctx->A.ptr.p, ctx->A.idx.p, ctx->A.val.p, ctx->Gt.ptr.p, ctx->Gt.idx.p,
                        ctx->Gt.val.p, m, ns, n, nrhs, ld, cm);
@@ -1232,6 +1245,7 @@ static int matvec_prepare(peec_ctx *ctx, int nrhs)
         peec_set_error("out of device memory for triangular-solve partial sums");
         return PEEC_ERR_NOMEM;
     }
+    if (ctx->mf_ready) return mf_prepare(ctx, nrhs);
     return PEEC_OK;
 }
 
@@ -1333,6 +1347,33 @@ extern "C" int peec_set_graphs(peec_ctx *ctx, int on)
         return PEEC_ERR_ARG;
     }
     ctx->use_graphs = on != 0;
+    return PEEC_OK;
+}
+
+extern "C" int peec_lu_set_solve(peec_ctx *ctx, int mode)
+{
+    if (!ctx || mode < 0 || mode > 1) {
+        peec_set_error("peec_lu_set_solve: mode must be 0 (level-scheduled blocks) or 1 (multifrontal)");
+        return PEEC_ERR_ARG;
+    }
+    ctx->lu_solve = mode;
+    // the multifrontal solve needs the pivot-block inverses of the current
+    // factorization: mode 1 takes effect with the next one that covers
+    // every slot, mode 0 at once
+    ctx->mf_ready = mode == 1 && ctx->mf_ready;
+    return PEEC_OK;
+}
+
+extern "C" int peec_lu_solve_info(peec_ctx *ctx, int64_t *out4)
+{
+    if (!ctx || !out4) {
+        peec_set_error("peec_lu_solve_info: invalid argument");
+        return PEEC_ERR_ARG;
+    }
+    out4[0] = ctx->lu_solve;
+    out4[1] = ctx->mf_ready && ctx->lu_solve == 1 && ctx->lu_layout_from_plan ? 1 : 0;
+    out4[2] = ctx->mf.ready ? ctx->mf.H : 0;
+    out4[3] = ctx->mf.ready ? ctx->mf.max_tasks : 0;
     return PEEC_OK;
 }
 

@@ -452,6 +452,24 @@ def lu_info(ctx=None):
     return dict(zip(keys, (int(v) for v in out)))
 
 
+def lu_set_solve(mode, ctx=None):
+    """Triangular solves of the preconditioner (peec_lu_set_solve): 1 =
+    multifrontal from the device factorization's fronts (default; takes
+    effect with the next factorization of every slot at once), 0 = the
+    level-scheduled 64-row blocks (at once)."""
+    ctx = ctx or default_context()
+    L.check(L.lib().peec_lu_set_solve(ctx.handle, int(mode)), "peec_lu_set_solve")
+
+
+def lu_solve_info(ctx=None):
+    """{mode, multifrontal (in use), height (assembly tree), max_tasks}
+    (peec_lu_solve_info)."""
+    ctx = ctx or default_context()
+    out = (ctypes.c_int64 * 4)()
+    L.check(L.lib().peec_lu_solve_info(ctx.handle, out), "peec_lu_solve_info")
+    return dict(zip(("mode", "multifrontal", "height", "max_tasks"), (int(v) for v in out)))
+
+
 def lu_front(K, ctx=None):
     """(s, positions, F): front K of the last device factorization -- its
     pivot count, the factor position of each row/column, and the f x f

@@ -129,9 +129,12 @@ def pself_fn():
 def test_batched_lu_equals_per_slot():
     """peec_lu_factor_slots (all slots through each launch of the
     multifrontal schedule together) gives the same factors, bit for bit,
-    as factoring the slots one at a time: M^-1 x on every slot agrees."""
+    as factoring the slots one at a time: M^-1 x on every slot agrees.
+    Both apply the factors through the level-scheduled solves (the
+    multifrontal solves need every slot factored at once)."""
     M = small_mesh()
     S = sweep.Solver(M, ctx=peec.Context(0), device_lu=True)
+    peec.lu_set_solve(0, ctx=S.ctx)
     fs = [1e7, 3e8, 2e9, 9e9]
     ws = [2 * np.pi * f * M.escalings["freq"] for f in fs]
     peec.frequency_slots(len(ws), ctx=S.ctx)

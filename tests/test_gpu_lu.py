@@ -207,3 +207,111 @@ def test_umfpack_convention_factors_with_pivoting_and_row_scaling():
               flush=True)
         assert e <= max(1e-12, 10 * cond * np.finfo(float).eps)
     S.ctx.close()
+
+
+def _layered_solver(nx, ny, nz, seed=3):
+    from solver_amd import configs, mesher
+    from test_gpu_operator import pself
+    cell = (0.25e-3, 0.25e-3, 0.05e-3)
+    g, mats, _ = configs._layered(nx, ny, nz, cell, 4, 0.45, 4.4, 0.02, seed=seed)
+    M = mesher.mesh(g, mats, cell, (0.0, 0.0, 0.0), configs._x_face_ports(g, cell, 4), p_self_fn=pself)
+    return M, sweep.Solver(M, device_lu=True)
+
+
+def _set_slots(M, S, fs):
+    from solver_amd import peec
+    ws = [2 * np.pi * f * M.escalings["freq"] for f in fs]
+    peec.frequency_slots(len(ws), ctx=S.ctx)
+    for slot, w in enumerate(ws):
+        Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+        peec.frequency_set_slot(slot, w, sweep.DZ_of(M, w), Yle, ctx=S.ctx)
+    return ws
+
+
+def _profiled_precond(S, X, rps):
+    """M^-1 X and the launch count per profile kind (11: multifrontal
+    small fronts, 12: GEMV tasks, 13: assembly / finishing)."""
+    import ctypes
+    from solver_amd import _lib, peec
+    nk = 14
+    ms, by, cnt = (ctypes.c_double * nk)(), (ctypes.c_double * nk)(), (ctypes.c_int64 * nk)()
+    _lib.check(_lib.lib().peec_profile(S.ctx.handle, 1, ms, by, cnt, nk), "peec_profile")
+    Y = peec.apply_K_batched(X, rps, precond=2, ctx=S.ctx)
+    _lib.check(_lib.lib().peec_profile(S.ctx.handle, 0, ms, by, cnt, nk), "peec_profile")
+    return Y, list(cnt)
+
+
+@pytest.mark.parametrize("nxyz,rps,fs", [((12, 10, 6), 9, [2e8, 4e9]), ((12, 10, 6), 3, [2e8, 4e9, 1e7]),
+                                         ((32, 30, 8), 8, [1e7, 3e9]), ((32, 30, 8), 1, [5e8])])
+def test_multifrontal_solve_matches_oracle(nxyz, rps, fs):
+    """The multifrontal solves (every slot factored at once, so the front
+    pool holds them all with their pivot-block inverses): M^-1 x per slot
+    matches the oracle's precond_3_3_vector column by column, to the
+    max(1e-12, cond(SS) eps) bar of the level-scheduled solves.  rps 9: a
+    pass of 8 right-hand sides and a remainder of 1 per slot; 3: the 4-wide
+    kernels; 1: the 1-wide ones.  12 x 10 x 6 has only small fronts (one
+    fused launch per tree height); 32 x 30 x 8 has separators of more than
+    64 pivots, whose pivots go through the GEMV tasks (and their
+    L11^-1 / U11^-1 from the block-row inversion)."""
+    from solver_amd import peec
+    M, S = _layered_solver(*nxyz)
+    ws = _set_slots(M, S, fs)
+    peec.lu_factor_slots(0, len(ws), ctx=S.ctx)
+    info = peec.lu_solve_info(ctx=S.ctx)
+    print("solve info", info, flush=True)
+    assert info["multifrontal"] == 1
+    N = M.m + M.ns + M.n
+    rng = np.random.default_rng(23)
+    X = rng.normal(size=(N, rps * len(ws))) + 1j * rng.normal(size=(N, rps * len(ws)))
+    Y, cnt = _profiled_precond(S, X, rps)
+    print("launches per kind", cnt, flush=True)
+    assert cnt[11] > 0 and cnt[5] == 0 and cnt[10] == 0
+    if nxyz[0] > 12:
+        assert cnt[12] > 0 and cnt[13] > 0
+    rows = OF.kernel_rows(M.origin, M.Nx, M.Ny, M.Nz, M.sx, M.sy, M.sz)
+    spec = OF.spectra(rows, M.Nx, M.Ny, M.Nz)
+    m, ns = M.m, M.ns
+    for slot, w in enumerate(ws):
+        op = OF.Operator(M, spec)
+        Yle = OF.build_Yle(M.lumped_elements, M.port_nodes, M.escalings["Yle"], M.n, w / M.escalings["freq"], 50.0)
+        op.set_frequency(w, OF.frequency_terms(M, w), Yle, order=S.order)
+        # 1-norm condition estimate from the oracle's factors; SS is complex
+        # symmetric, so SS^-H v = conj(SS^-1 conj(v))
+        F = op.F
+        inv = spla.LinearOperator(op.SS.shape, matvec=lambda v: F.solve(np.asarray(v, dtype=complex)),
+                                  rmatvec=lambda v: np.conj(F.solve(np.conj(np.asarray(v, dtype=complex)))),
+                                  dtype=complex)
+        cond = spla.onenormest(op.SS) * spla.onenormest(inv)
+        tol = max(1e-12, cond * np.finfo(float).eps)
+        for c in range(slot * rps, (slot + 1) * rps):
+            x = X[:, c]
+            e = rel(Y[:, c], op.precond(x[:m], x[m:m + ns], x[m + ns:]))
+            print("slot %d column %d: M^-1 x vs oracle %.2e (tol %.1e)" % (slot, c, e, tol), flush=True)
+            assert e < tol
+
+
+def test_multifrontal_matches_level_solve():
+    """The same factors through both solves (mode 0: level-scheduled
+    blocks, mode 1: multifrontal): M^-1 x agrees to 1e-11 relative, and the
+    multifrontal solve gives bitwise the same result when run again."""
+    from solver_amd import peec
+    M, S = _layered_solver(32, 30, 8, seed=5)
+    ws = _set_slots(M, S, [3e7, 1e9, 8e9])
+    N = M.m + M.ns + M.n
+    rng = np.random.default_rng(29)
+    X = rng.normal(size=(N, 8 * len(ws))) + 1j * rng.normal(size=(N, 8 * len(ws)))
+    peec.lu_set_solve(0, ctx=S.ctx)
+    peec.lu_factor_slots(0, len(ws), ctx=S.ctx)
+    assert peec.lu_solve_info(ctx=S.ctx)["multifrontal"] == 0
+    Y0 = peec.apply_K_batched(X, 8, precond=2, ctx=S.ctx)
+    peec.lu_set_solve(1, ctx=S.ctx)
+    assert peec.lu_solve_info(ctx=S.ctx)["multifrontal"] == 0  # needs a factorization with the inverses
+    peec.lu_factor_slots(0, len(ws), ctx=S.ctx)
+    assert peec.lu_solve_info(ctx=S.ctx)["multifrontal"] == 1
+    Y1 = peec.apply_K_batched(X, 8, precond=2, ctx=S.ctx)
+    Y2 = peec.apply_K_batched(X, 8, precond=2, ctx=S.ctx)
+    e = max(rel(Y1[:, c], Y0[:, c]) for c in range(X.shape[1]))
+    print("multifrontal vs level-scheduled: max column rel %.2e; repeat bitwise %s" % (e, np.array_equal(Y1, Y2)),
+          flush=True)
+    assert np.array_equal(Y1, Y2)
+    assert e < 1e-11
