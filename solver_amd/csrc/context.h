@@ -190,6 +190,7 @@ struct LuPlan {
     DevBuf<LuGemmItem> inv_items;
     int pool_slot0 = -1, pool_ns = 0;  // slots whose fronts the pool holds now
     bool pool_inv = false;             // ... with their inverses
+    bool layouts_deferred = false;     // Lf/Uf values not gathered from the pool yet
     // values of the triangular-solve layouts, gathered from the pool
     DevBuf<int64_t> l_ob_src, l_dense_src, u_ob_src, u_dense_src, u_diag_src;
     DevBuf<int32_t> bad_pivot;  // per slot of a factorization group: 1 = a zero or non-finite pivot
@@ -207,8 +208,8 @@ struct MfFront {
     int32_t f, s, first, pad;
 };
 // A GEMV task of a large front: rows r0 .. r0 + m of an output segment from
-// columns k0 .. k0 + kc of A (pool offset a of element (r0, k0), leading
-// dimension lda) against the same rows of the right-hand side segment.
+// kc columns of A (pool offset a of its first element, leading dimension
+// lda) against rows k0 .. k0 + kc of the front's working rows w.
 struct MfTask {
     int64_t a;
     int32_t lda, m, kc, front, r0, k0;
@@ -442,6 +443,7 @@ int tri_layout(int64_t n, bool lower, const std::vector<int32_t> &blk, const std
 int tri_set_slots(TriFactor *f, int nslots);  // reallocates the value sets (contents undefined)
 std::vector<int32_t> tri_chain_blocks(const peec_csc_c128 *L, int cap);
 int tri_build(const peec_csc_c128 *T, bool lower, const std::vector<int32_t> &blk, TriFactor *f, const char *what);
+int lu_gather_deferred(peec_ctx *ctx);  // lu.hip: Lf/Uf values from the pool if a factorization skipped them
 int mf_analyze(peec_ctx *ctx);  // mfsolve.hip: the plan of the multifrontal solves (after the LU plan)
 int mf_prepare(peec_ctx *ctx, int nrhs);  // allocates the solve buffers for nrhs right-hand sides
 int mf_solve(peec_ctx *ctx, bool lower, const double2_t *b, double2_t *x, int nrhs, const ColMap &cm);

@@ -813,8 +813,45 @@ extern "C" int peec_lu_analyze(peec_ctx *ctx, int64_t n, const int64_t *order, i
     P.pool_slot0 = -1;
     P.pool_ns = 0;
     P.pool_inv = false;
+    P.layouts_deferred = false;
     ctx->mf_ready = false;
     return mf_analyze(ctx);
+}
+
+// The factor values of slots slot0 .. slot0 + ns - 1 (pool copies 0 ..
+// ns - 1) into the level-scheduled layouts: U's inverse diagonal always
+// (the pivot check reads it), the off-block and dense values when `all`.
+static void lu_gather_values(peec_ctx *ctx, int slot0, int ns, bool all)
+{
+    LuPlan &P = ctx->lu;
+    TriFactor &Lf = ctx->Lf, &Uf = ctx->Uf;
+    for (int y = 0; y < ns; y++) {
+        const int slot = slot0 + y;
+        const cd *pl = P.pool.p + (int64_t)y * P.pool_total;
+        if (all) {
+            hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Lf.ob_n)), dim3(256), 0, ctx->stream, P.l_ob_src.p,
+                               Lf.ob_val.p + slot * Lf.ob_n, Lf.ob_n, pl, 0);
+            hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Lf.dense_n)), dim3(256), 0, ctx->stream, P.l_dense_src.p,
+                               Lf.dense.p + slot * Lf.dense_n, Lf.dense_n, pl, 0);
+            hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Uf.ob_n)), dim3(256), 0, ctx->stream, P.u_ob_src.p,
+                               Uf.ob_val.p + slot * Uf.ob_n, Uf.ob_n, pl, 0);
+            hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Uf.dense_n)), dim3(256), 0, ctx->stream, P.u_dense_src.p,
+                               Uf.dense.p + slot * Uf.dense_n, Uf.dense_n, pl, 0);
+        }
+        hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(P.n)), dim3(256), 0, ctx->stream, P.u_diag_src.p,
+                           Uf.diag.p + slot * P.n, P.n, pl, 1);
+    }
+}
+
+int lu_gather_deferred(peec_ctx *ctx)
+{
+    LuPlan &P = ctx->lu;
+    if (!P.layouts_deferred) return PEEC_OK;
+    PEEC_HIP_TRY(hipSetDevice(ctx->device));
+    lu_gather_values(ctx, P.pool_slot0, P.pool_ns, true);
+    PEEC_HIP_TRY(hipGetLastError());
+    P.layouts_deferred = false;
+    return PEEC_OK;
 }
 
 // Factor slots slot0 .. slot0 + ns - 1 together: every launch of the
@@ -824,6 +861,12 @@ static int lu_factor_range(peec_ctx *ctx, int slot0, int ns)
 {
     LuPlan &P = ctx->lu;
     const int64_t stride = P.pool_total;
+    // a factorization that does not cover every slot overwrites pool copies
+    // that may still hold the only values of other slots: gather them first
+    if (P.layouts_deferred && !(slot0 == 0 && ns == ctx->nslots)) {
+        int rc = lu_gather_deferred(ctx);
+        if (rc) return rc;
+    }
     if (P.pool.n < (size_t)(stride * ns) && !P.pool.alloc((size_t)(stride * ns))) {
         peec_set_error("peec_lu_factor: out of device memory for %d x %lld front entries", ns, (long long)stride);
         return PEEC_ERR_NOMEM;
@@ -896,20 +939,10 @@ static int lu_factor_range(peec_ctx *ctx, int slot0, int ns)
                        slot0 + ns - 1);
         return PEEC_ERR_STATE;
     }
-    for (int y = 0; y < ns; y++) {
-        const int slot = slot0 + y;
-        const cd *pl = pool + (int64_t)y * stride;
-        hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Lf.ob_n)), dim3(256), 0, ctx->stream, P.l_ob_src.p,
-                           Lf.ob_val.p + slot * Lf.ob_n, Lf.ob_n, pl, 0);
-        hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Lf.dense_n)), dim3(256), 0, ctx->stream, P.l_dense_src.p,
-                           Lf.dense.p + slot * Lf.dense_n, Lf.dense_n, pl, 0);
-        hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Uf.ob_n)), dim3(256), 0, ctx->stream, P.u_ob_src.p,
-                           Uf.ob_val.p + slot * Uf.ob_n, Uf.ob_n, pl, 0);
-        hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(Uf.dense_n)), dim3(256), 0, ctx->stream, P.u_dense_src.p,
-                           Uf.dense.p + slot * Uf.dense_n, Uf.dense_n, pl, 0);
-        hipLaunchKernelGGL(k_lu_gather, dim3(grid1d(P.n)), dim3(256), 0, ctx->stream, P.u_diag_src.p,
-                           Uf.diag.p + slot * P.n, P.n, pl, 1);
-    }
+    // (the multifrontal solves read the pool itself: the level-scheduled
+    // layouts are gathered only if peec_lu_set_solve(0) asks for them)
+    P.layouts_deferred = P.pool_inv;
+    lu_gather_values(ctx, slot0, ns, !P.layouts_deferred);
     PEEC_HIP_TRY(hipGetLastError());
     // no pivoting: a zero or non-finite pivot makes the factors unusable
     if (P.bad_pivot.n < (size_t)ns && !P.bad_pivot.alloc((size_t)std::max(ns, 64))) {

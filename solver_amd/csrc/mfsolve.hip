@@ -268,16 +268,32 @@ __global__ void __launch_bounds__(256) k_mf_assemble(const int2 *rows, int64_t n
     }
 }
 
+// x at the boundary of the large fronts of one height, into their w rows
+// s .. f (the backward sweep's right-hand side of U12 x_B).
+__global__ void __launch_bounds__(256) k_mf_gather_b(const int2 *rows, int64_t nrows, const MfFront *fr,
+                                                     const int32_t *bnd, const cd *x, cd *wbuf, int64_t wtot,
+                                                     int64_t n, int nrhs, ColMap cm)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= nrows) return;
+    const int slot = (int)blockIdx.y;
+    const int2 fj = rows[e];
+    const MfFront F = fr[fj.x];
+    if (fj.y < F.s) return;
+    const int64_t pos = bnd[F.bptr + fj.y - F.s];
+    const int rend = cm.end(slot, nrhs);
+    for (int64_t r = cm.first(slot); r < rend; r++) wbuf[r * wtot + F.woff + fj.y] = x[r * n + pos];
+}
+
 // GEMV tasks of the large fronts: partial sums of rows r0 .. r0 + m of
-// A (columns k0 .. k0 + kc) times the right-hand side rows k0 .. k0 + kc:
-// the front's w area (GATHER false) or x at the front's boundary positions
-// (GATHER true).  The slot's right-hand sides go Q per pass; their rows of B
-// are staged in LDS and each of the four wavefronts takes a quarter of the
-// columns, one row per lane, with 8 loads of A in flight.
-template <int Q, bool GATHER>
+// A (kc columns) times rows k0 .. k0 + kc of the front's working rows w.
+// The slot's right-hand sides go Q per pass; their rows of w are staged in
+// LDS and each of the four wavefronts takes a quarter of the columns, one
+// row per lane, with 8 loads of A in flight.
+template <int Q>
 __global__ void __launch_bounds__(256) k_mf_gemv(const MfTask *tasks, const MfFront *fr, const cd *pool0,
-                                                 int64_t stride, const cd *bsrc, int64_t bstride, const int32_t *bnd,
-                                                 cd *part, int nrhs, ColMap cm)
+                                                 int64_t stride, const cd *wbuf, int64_t wtot, cd *part, int nrhs,
+                                                 ColMap cm)
 {
     __shared__ cd sb[kMfKc * Q];  // B rows [k][c]; then the four wavefronts' sums [wave][row][c]
     const int slot = (int)blockIdx.y;
@@ -297,8 +313,7 @@ __global__ void __launch_bounds__(256) k_mf_gemv(const MfTask *tasks, const MfFr
         for (int e = threadIdx.x; e < t.kc * Q; e += 256) {
             const int k = e / Q, c = e % Q;
             // columns past the pass read its last one (in bounds) and store 0
-            const int64_t col = (int64_t)(r0 + min(c, nc - 1)) * bstride;
-            const cd v = GATHER ? bsrc[col + bnd[F.bptr + t.k0 + k]] : bsrc[col + F.woff + t.k0 + k];
+            const cd v = wbuf[(int64_t)(r0 + min(c, nc - 1)) * wtot + F.woff + t.k0 + k];
             const bool keep = c < nc;
             sb[e] = {keep ? v.x : 0.0, keep ? v.y : 0.0};
         }
@@ -536,7 +551,7 @@ int mf_analyze(peec_ctx *ctx)
                         tk.kc = kc;
                         tk.front = K;
                         tk.r0 = r0;
-                        tk.k0 = k0;
+                        tk.k0 = ph == 2 ? s + k0 : k0;  // first row of w it multiplies
                         tasks.push_back(tk);
                         M.task_bytes[(size_t)(h * 4 + ph)] += 16.0 * m * kc;
                     }
@@ -595,24 +610,17 @@ int mf_solve(peec_ctx *ctx, bool lower, const double2_t *b, double2_t *x, int nr
     const hipStream_t st = ctx->stream;
     const unsigned ny = (unsigned)nsl;
     const double vbytes = 16.0 * nrhs;  // one row of every right-hand side
-    auto gemv = [&](int h, int ph, const cd *bsrc, int64_t bstride, bool gather) {
+    auto gemv = [&](int h, int ph) {
         const int64_t t0 = M.task_ptr[(size_t)(h * 4 + ph)], nt = M.task_ptr[(size_t)(h * 4 + ph) + 1] - t0;
         if (nt <= 0) return;
         int pi = ctx->prof_begin(PK_MF_GEMV, M.task_bytes[(size_t)(h * 4 + ph)] * nsl + (double)nt * 64 * 2 * vbytes);
         const MfTask *tk = M.tasks.p + t0;
         const dim3 grid((unsigned)nt, ny);
-#define MF_GEMV(QQ, GG)                                                                                              \
-    hipLaunchKernelGGL((k_mf_gemv<QQ, GG>), grid, dim3(256), 0, st, tk, M.fr.p, pool, stride, bsrc, bstride, M.bnd.p, \
-                       part, nrhs, cm)
-        if (gather) {
-            if (Q == 8) MF_GEMV(8, true);
-            else if (Q == 4) MF_GEMV(4, true);
-            else MF_GEMV(1, true);
-        } else {
-            if (Q == 8) MF_GEMV(8, false);
-            else if (Q == 4) MF_GEMV(4, false);
-            else MF_GEMV(1, false);
-        }
+#define MF_GEMV(QQ)                                                                                                  \
+    hipLaunchKernelGGL((k_mf_gemv<QQ>), grid, dim3(256), 0, st, tk, M.fr.p, pool, stride, w, M.wtot, part, nrhs, cm)
+        if (Q == 8) MF_GEMV(8);
+        else if (Q == 4) MF_GEMV(4);
+        else MF_GEMV(1);
 #undef MF_GEMV
         ctx->prof_end(pi);
     };
@@ -667,15 +675,19 @@ int mf_solve(peec_ctx *ctx, bool lower, const double2_t *b, double2_t *x, int nr
             hipLaunchKernelGGL(k_mf_assemble, dim3((unsigned)((nr + 255) / 256), ny), dim3(256), 0, st, M.rows.p + r0,
                                nr, M.fr.p, M.cptr.p, M.csrc.p, b, w, M.wtot, u, M.utot, n, nrhs, cm);
             ctx->prof_end(pi);
-            gemv(h, 0, w, M.wtot, false);  // L11^-1 w_P
-            finish(h, 0);                  // y -> x and w
-            gemv(h, 1, w, M.wtot, false);  // L21 y
-            finish(h, 1);                  // u = w_B - L21 y
+            gemv(h, 0);    // L11^-1 w_P
+            finish(h, 0);  // y -> x and w
+            gemv(h, 1);    // L21 y
+            finish(h, 1);  // u = w_B - L21 y
         } else {
-            gemv(h, 2, x, n, true);        // U12 x_B
-            finish(h, 2);                  // t = c_P - U12 x_B
-            gemv(h, 3, w, M.wtot, false);  // U11^-1 t
-            finish(h, 3);                  // x_P
+            int pi = ctx->prof_begin(PK_MF_AUX, (double)nr * 2 * vbytes);
+            hipLaunchKernelGGL(k_mf_gather_b, dim3((unsigned)((nr + 255) / 256), ny), dim3(256), 0, st, M.rows.p + r0,
+                               nr, M.fr.p, M.bnd.p, x, w, M.wtot, n, nrhs, cm);
+            ctx->prof_end(pi);
+            gemv(h, 2);    // U12 x_B
+            finish(h, 2);  // t = c_P - U12 x_B
+            gemv(h, 3);    // U11^-1 t
+            finish(h, 3);  // x_P
         }
     }
     PEEC_HIP_TRY(hipGetLastError());

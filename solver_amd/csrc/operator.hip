@@ -1359,8 +1359,10 @@ extern "C" int peec_lu_set_solve(peec_ctx *ctx, int mode)
     ctx->lu_solve = mode;
     // the multifrontal solve needs the pivot-block inverses of the current
     // factorization: mode 1 takes effect with the next one that covers
-    // every slot, mode 0 at once
+    // every slot, mode 0 at once (with the layouts' values gathered from the
+    // pool if the factorization left them to the multifrontal solve)
     ctx->mf_ready = mode == 1 && ctx->mf_ready;
+    if (mode == 0) return lu_gather_deferred(ctx);
     return PEEC_OK;
 }
 
