@@ -20,7 +20,8 @@
 //      pivots -- the 64 x 64 diagonal block by one workgroup in LDS, which
 //      also forms its two triangular inverses; the panel's L and U blocks
 //      as GEMMs with those inverses; the trailing update as 64 x 64 GEMM
-//      tiles -- so the dense work of every level is spread over all CUs;
+//      tiles, blocked over groups of kLuGroup pivots -- so the dense work of
+//      every level is spread over all CUs;
 //   3. the factors are gathered into the blocked layout of the triangular
 //      solves (trsv.hip).
 //
@@ -301,6 +302,10 @@ __global__ void k_lu_clear_flags(int32_t *bad, int n)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) bad[i] = 0;
 }
+
+// Pivots per group of the blocked trailing update (a multiple of the
+// 64-pivot panel).
+constexpr int kLuGroup = 256;
 
 // Blocks of 256 for a grid-stride kernel over n elements: enough to fill
 // the chip, never more than kMaxBlocks1d (n > 2^32 included).
@@ -644,26 +649,44 @@ extern "C" int peec_lu_analyze(peec_ctx *ctx, int64_t n, const int64_t *order, i
                 }
             }
             if ((int64_t)gi.size() > g0) P.sched.push_back({2, g0, (int64_t)gi.size() - g0});
-            // trailing update
+            // trailing update.  Pivots go in groups of kLuGroup: inside a
+            // group a panel updates only what the group's later panels read
+            // (rows below it in the group's columns, and the group's rows
+            // right of the group); the group's last panel then applies the
+            // whole group's L and U panels to the rest in one pass of
+            // k = group width, so the large trailing matrices are read and
+            // written once per group instead of once per 64 pivots
             const int64_t t0 = (int64_t)gi.size();
             for (auto &a : act) {
                 const LuFrontInfo &F = P.fronts[(size_t)a.first];
-                const int w = std::min(64, F.s - k0), rest = F.f - k0 - w;
-                for (int c = 0; c < rest; c += 64)
-                    for (int r = 0; r < rest; r += 64) {
-                        LuGemmItem g{};
-                        g.a = F.off + (int64_t)k0 * F.f + (k0 + w + r);
-                        g.lda = F.f;
-                        g.b = F.off + (int64_t)(k0 + w + c) * F.f + k0;
-                        g.ldb = F.f;
-                        g.c = F.off + (int64_t)(k0 + w + c) * F.f + (k0 + w + r);
-                        g.ldc = F.f;
-                        g.m = std::min(64, rest - r);
-                        g.n = std::min(64, rest - c);
-                        g.k = w;
-                        g.mode = 1;
-                        gi.push_back(g);
-                    }
+                const int w = std::min(64, F.s - k0), next = k0 + w;
+                const int g0 = k0 / kLuGroup * kLuGroup, gend = std::min(g0 + kLuGroup, F.s);
+                // C(r0.., c0..) -= L(r0.., kb .. kb + kk) U(kb .. kb + kk, c0..)
+                auto tile = [&](int r0, int c0, int m, int n, int kb, int kk) {
+                    LuGemmItem g{};
+                    g.a = F.off + (int64_t)kb * F.f + r0;
+                    g.lda = F.f;
+                    g.b = F.off + (int64_t)c0 * F.f + kb;
+                    g.ldb = F.f;
+                    g.c = F.off + (int64_t)c0 * F.f + r0;
+                    g.ldc = F.f;
+                    g.m = m;
+                    g.n = n;
+                    g.k = kk;
+                    g.mode = 1;
+                    gi.push_back(g);
+                };
+                if (next < gend) {
+                    for (int c = next; c < gend; c += 64)
+                        for (int r = next; r < F.f; r += 64) tile(r, c, std::min(64, F.f - r), std::min(64, gend - c), k0, w);
+                    for (int c = gend; c < F.f; c += 64)
+                        for (int r = next; r < gend; r += 64)
+                            tile(r, c, std::min(64, gend - r), std::min(64, F.f - c), k0, w);
+                } else {
+                    for (int c = gend; c < F.f; c += 64)
+                        for (int r = gend; r < F.f; r += 64)
+                            tile(r, c, std::min(64, F.f - r), std::min(64, F.f - c), g0, gend - g0);
+                }
             }
             if ((int64_t)gi.size() > t0) P.sched.push_back({2, t0, (int64_t)gi.size() - t0});
         }
