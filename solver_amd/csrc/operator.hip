@@ -92,7 +92,9 @@ __global__ void k_zero(cd *y, int64_t n, int nrhs, int64_t ld)
 
 // The sparse terms and the permutation scatter / gather run one thread per
 // row and frequency slot (blockIdx.y = slot s; the columns of a batch are
-// grouped by slot, cm.first(s) .. cm.end(s)).  The thread takes the row's
+// grouped by slot, cm.first(s) .. cm.end(s)).  Rows go to the XCDs in
+// contiguous ranges (xcd_tile): neighbouring rows gather the same cache
+// lines of the other sections, which then stay in one L2.  The thread takes the row's
 // index data and the slot's DZ / 1/DZ / w once and gathers kRhsChunk
 // right-hand sides per index, with one accumulator each, so the loads of a
 // chunk are independent.  Each right-hand side's arithmetic is the
@@ -103,7 +105,7 @@ constexpr int kRhsChunk = 8;
 __global__ void k_y1(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const int32_t *ai, const double *av,
                      const double *wv, int64_t m, int64_t phi0, int nrhs, int64_t ld, ColMap cm)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (e >= m) return;
     const int s = (int)blockIdx.y;
     const int rb = cm.first(s), re = cm.end(s, nrhs);
@@ -139,7 +141,7 @@ __global__ void k_y1(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const 
 __global__ void k_y2(cd *y, const cd *x, const int32_t *gp, const int32_t *gi, const double *gv, int64_t m,
                      int64_t ns, int64_t phi0, int nrhs, int64_t ld)
 {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t s = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     {
         const int r = (int)blockIdx.y;  // one right-hand side per grid row
@@ -155,7 +157,7 @@ __global__ void k_y3(cd *y, const cd *x, const int32_t *atp, const int32_t *ati,
                      const int32_t *yi, const cd *yv, const int32_t *gp, const int32_t *gi, const double *gv,
                      const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, ColMap cm)
 {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
     const int s = (int)blockIdx.y;
@@ -213,7 +215,7 @@ __global__ void k_pre(const cd *y, const cd *invZ0, const double *invP, const in
                       const double *Rs, cd *b, const double *wv, int64_t m, int64_t ns, int64_t n, int nrhs,
                       int64_t ld, ColMap cm)
 {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
     const int s = (int)blockIdx.y;
@@ -262,7 +264,7 @@ __global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ0,
                        const int32_t *ai, const double *av, const int32_t *gtp, const int32_t *gti, const double *gtv,
                        int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, ColMap cm)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int64_t N = m + ns + n;
     if (i >= N) return;
     const int s = (int)blockIdx.y;
@@ -327,7 +329,7 @@ __global__ void k_post(cd *y, const cd *z, const int32_t *qinv, const cd *invZ0,
 __global__ void k_y1h(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const int32_t *ai, const double *av,
                       const double *wv, int64_t m, int64_t phi0, int nrhs, int64_t ld, ColMap cm)
 {
-    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t e = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (e >= m) return;
     {
         const int r = (int)blockIdx.y;  // one right-hand side per grid row
@@ -346,7 +348,7 @@ __global__ void k_y1h(cd *y, const cd *x, const cd *DZ, const int32_t *ap, const
 __global__ void k_y2h(cd *y, const cd *x, const int32_t *gp, const int32_t *gi, const double *gv, const double *wv,
                       int64_t m, int64_t ns, int64_t phi0, int nrhs, int64_t ld, ColMap cm)
 {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t s = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     {
         const int r = (int)blockIdx.y;  // one right-hand side per grid row
@@ -362,7 +364,7 @@ __global__ void k_y3h(cd *y, const cd *x, const int32_t *atp, const int32_t *ati
                       const int32_t *yp0, const int32_t *yi, const cd *yv, const int32_t *gp, const int32_t *gi,
                       const double *gv, int64_t m, int64_t ns, int64_t n, int nrhs, int64_t ld, ColMap cm)
 {
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t v = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (v >= n) return;
     const int64_t phi0 = m + ns;
     {
