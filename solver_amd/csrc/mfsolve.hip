@@ -351,6 +351,85 @@ __global__ void __launch_bounds__(256) k_mf_gemv(const MfTask *tasks, const MfFr
     }
 }
 
+// The same tasks for passes of 8 right-hand sides on the matrix cores.  The
+// complex product is the real one C' = A' B' with A' = [Ar Ai] and, for the
+// rows of A' holding column k's real / imaginary parts, B' rows [Br Bi] /
+// [-Bi Br] (16 columns: the 8 right-hand sides' real parts, then their
+// imaginary parts).  Wavefront w owns rows 16 w .. 16 w + 15 of the task over
+// all its columns.  Lane l loads the whole complex A[16 w + l % 16][4 s + l / 16]
+// and feeds its real part to one v_mfma_f64_16x16x4f64 and its imaginary part
+// to the next (lane l holds A'[l % 16][l / 16], B'[l / 16][l % 16] and
+// D[l / 16 + 4 j][l % 16]): four complex columns per pair of instructions, no
+// wavefront waits on another, and the LDS serves only the staged rows of w.
+typedef double v4d __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_mf_gemv8m(const MfTask *tasks, const MfFront *fr, const cd *pool0,
+                                                   int64_t stride, const cd *wbuf, int64_t wtot, cd *part, int nrhs,
+                                                   ColMap cm)
+{
+    __shared__ cd sb[kMfKc * 8];
+    const int slot = (int)blockIdx.y;
+    const MfTask t = tasks[blockIdx.x];
+    const MfFront F = fr[t.front];
+    const int rbeg = cm.first(slot), rend = cm.end(slot, nrhs);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, c16 = lane & 15;
+    // A fragment: row 16 w + l % 16 (rows past m read row m - 1 and are not
+    // written), column 4 s + q
+    const int arow = min(wave * 16 + c16, t.m - 1);
+    const cd *Ap = pool0 + (int64_t)slot * stride + t.a + arow;
+    const int64_t lda = t.lda;
+    // B' fragment: right-hand side c16 & 7, output part c16 >> 3 (0 real,
+    // 1 imaginary); with A's real part B' = [Br Bi], with its imaginary part
+    // B' = [-Bi Br]
+    const int bc = c16 & 7;
+    const bool im_out = c16 >= 8;
+    double *pout = reinterpret_cast<double *>(part + (int64_t)blockIdx.x * 64 * nrhs);
+    const int ksteps = (t.kc + 3) / 4;
+    for (int r0 = rbeg; r0 < rend; r0 += 8) {
+        const int nc = min(8, rend - r0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < t.kc * 8; e += 256) {
+            const int k = e >> 3, c = e & 7;
+            const cd v = wbuf[(int64_t)(r0 + min(c, nc - 1)) * wtot + F.woff + t.k0 + k];
+            const bool keep = c < nc;
+            sb[e] = {keep ? v.x : 0.0, keep ? v.y : 0.0};
+        }
+        __syncthreads();
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+        int st = 0;
+        for (; st + 4 <= ksteps; st += 4) {
+            cd a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int kk = 4 * (st + u) + q;
+                a[u] = kk < t.kc ? Ap[kk * lda] : cd{0.0, 0.0};
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int kk = 4 * (st + u) + q;
+                b[u] = kk < t.kc ? sb[kk * 8 + bc] : cd{0.0, 0.0};
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].x, im_out ? b[u].y : b[u].x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].y, im_out ? b[u].x : -b[u].y, acc, 0, 0, 0);
+            }
+        }
+        for (; st < ksteps; st++) {
+            const int kk = 4 * st + q;
+            const cd a = kk < t.kc ? Ap[kk * lda] : cd{0.0, 0.0};
+            const cd b = kk < t.kc ? sb[kk * 8 + bc] : cd{0.0, 0.0};
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, im_out ? b.y : b.x, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, im_out ? b.x : -b.y, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int row = wave * 16 + q + 4 * j;
+            if (row < t.m && bc < nc) pout[((int64_t)row * nrhs + r0 + bc) * 2 + (c16 >> 3)] = acc[j];
+        }
+    }
+}
+
 // Sums of a large front's task partials, in task order, and what they make:
 // KIND 0: y (forward pivots) -> out and w;  1: u = w_B - sum -> u area;
 // KIND 2: t = in - sum (backward pivots) -> w;  3: x -> out.
@@ -618,7 +697,8 @@ int mf_solve(peec_ctx *ctx, bool lower, const double2_t *b, double2_t *x, int nr
         const dim3 grid((unsigned)nt, ny);
 #define MF_GEMV(QQ)                                                                                                  \
     hipLaunchKernelGGL((k_mf_gemv<QQ>), grid, dim3(256), 0, st, tk, M.fr.p, pool, stride, w, M.wtot, part, nrhs, cm)
-        if (Q == 8) MF_GEMV(8);
+        if (Q == 8)
+            hipLaunchKernelGGL(k_mf_gemv8m, grid, dim3(256), 0, st, tk, M.fr.p, pool, stride, w, M.wtot, part, nrhs, cm);
         else if (Q == 4) MF_GEMV(4);
         else MF_GEMV(1);
 #undef MF_GEMV
