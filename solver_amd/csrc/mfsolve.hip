@@ -268,6 +268,172 @@ __global__ void __launch_bounds__(256) k_mf_assemble(const int2 *rows, int64_t n
     }
 }
 
+// D += A (16 rows from Ar, columns 0 .. kend, leading dimension lda) times
+// B (rows 0 .. kend of an LDS block of 8 right-hand sides, [k][c]), on the
+// matrix cores as in k_mf_gemv8m: lane l loads A[l % 16][4 s + l / 16] whole
+// and feeds its real and imaginary parts to two v_mfma_f64_16x16x4f64.
+// Ar already points at the lane's row (clamped by the caller).
+typedef double v4d __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void mf_mfma_rows(v4d &acc, const cd *Ar, int64_t lda, const cd *sB, int kend, int q,
+                                             int bc, bool im_out)
+{
+    const int ksteps = (kend + 3) / 4;
+    int st = 0;
+    for (; st + 4 <= ksteps; st += 4) {
+        cd a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int kk = 4 * (st + u) + q;
+            a[u] = kk < kend ? Ar[kk * lda] : cd{0.0, 0.0};
+            b[u] = kk < kend ? sB[kk * 8 + bc] : cd{0.0, 0.0};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].x, im_out ? b[u].y : b[u].x, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].y, im_out ? b[u].x : -b[u].y, acc, 0, 0, 0);
+        }
+    }
+    for (; st < ksteps; st++) {
+        const int kk = 4 * st + q;
+        const cd a = kk < kend ? Ar[kk * lda] : cd{0.0, 0.0};
+        const cd b = kk < kend ? sB[kk * 8 + bc] : cd{0.0, 0.0};
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, im_out ? b.y : b.x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, im_out ? b.x : -b.y, acc, 0, 0, 0);
+    }
+}
+
+// Small fronts, passes of 8 right-hand sides, on the matrix cores: the same
+// steps as k_mf_fwd_small / k_mf_bwd_small, each product as 16-row blocks
+// (wavefront w takes rows 16 w .. 16 w + 15 of a 64-row block).  Lane l
+// holds row 16 w + l / 16 + 4 j, right-hand side l % 8, real (l % 16 < 8) or
+// imaginary part of the result.
+__global__ void __launch_bounds__(256) k_mf_fwd_small8m(const int32_t *list, const MfFront *fr, const cd *pool0,
+                                                        int64_t stride, const int64_t *cptr, const int64_t *csrc,
+                                                        const cd *in, cd *out, cd *ubuf, int64_t utot, int64_t n,
+                                                        int nrhs, ColMap cm)
+{
+    __shared__ cd sw[64 * 8];
+    __shared__ cd sy[64 * 8];
+    const int slot = (int)blockIdx.y;
+    const MfFront F = fr[list[blockIdx.x]];
+    const cd *pool = pool0 + (int64_t)slot * stride;
+    const int s = F.s, b = F.f - F.s;
+    const int rbeg = cm.first(slot), rend = cm.end(slot, nrhs);
+    const int i = threadIdx.x & 63, cg = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4, c16 = lane & 15, bc = c16 & 7;
+    const int part = c16 >> 3;
+    const bool im_out = part == 1;
+    double *outd = reinterpret_cast<double *>(out);
+    double *ud = reinterpret_cast<double *>(ubuf);
+    double *syd = reinterpret_cast<double *>(sy);
+    for (int r0 = rbeg; r0 < rend; r0 += 8) {
+        const int nc = min(8, rend - r0);
+        // w at the pivots: r plus the children's updates
+        for (int c = cg; c < 8; c += 4) {
+            cd w = {0.0, 0.0};
+            if (i < s && c < nc) {
+                const int64_t r = r0 + c;
+                w = in[r * n + F.first + i];
+                for (int64_t e = cptr[F.crow + i]; e < cptr[F.crow + i + 1]; e++) {
+                    const cd v = ubuf[r * utot + csrc[e]];
+                    w.x += v.x;
+                    w.y += v.y;
+                }
+            }
+            sw[i * 8 + c] = w;
+        }
+        __syncthreads();
+        // y = L11^-1 w (stored whole: zeros above the diagonal)
+        if (16 * wave < s) {
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
+            mf_mfma_rows(acc, pool + F.inv + min(16 * wave + c16, s - 1), s, sw, s, q, bc, im_out);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int row = 16 * wave + q + 4 * j;
+                const bool live = row < s && bc < nc;
+                if (live) outd[((int64_t)(r0 + bc) * n + F.first + row) * 2 + part] = acc[j];
+                syd[(row * 8 + bc) * 2 + part] = live ? acc[j] : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) syd[((16 * wave + q + 4 * j) * 8 + bc) * 2 + part] = 0.0;
+        }
+        __syncthreads();
+        // u = w_B - L21 y, 64 boundary rows at a time
+        for (int jb = 0; jb < b; jb += 64) {
+            if (jb + 16 * wave >= b) continue;  // wave-uniform: no barrier inside
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
+            mf_mfma_rows(acc, pool + F.off + F.s + min(jb + 16 * wave + c16, b - 1), F.f, sy, s, q, bc, im_out);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int row = jb + 16 * wave + q + 4 * j;
+                if (row >= b || bc >= nc) continue;
+                double w = 0.0;
+                for (int64_t e = cptr[F.crow + s + row]; e < cptr[F.crow + s + row + 1]; e++)
+                    w += ud[((int64_t)(r0 + bc) * utot + csrc[e]) * 2 + part];
+                ud[((int64_t)(r0 + bc) * utot + F.uoff + row) * 2 + part] = w - acc[j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_mf_bwd_small8m(const int32_t *list, const MfFront *fr, const cd *pool0,
+                                                        int64_t stride, const int32_t *bnd, const cd *in, cd *out,
+                                                        int64_t n, int nrhs, ColMap cm)
+{
+    __shared__ cd sx[kMfSmallB * 8];  // x at the boundary
+    __shared__ cd st[64 * 8];
+    const int slot = (int)blockIdx.y;
+    const MfFront F = fr[list[blockIdx.x]];
+    const cd *pool = pool0 + (int64_t)slot * stride;
+    const int s = F.s, b = F.f - F.s;
+    const int rbeg = cm.first(slot), rend = cm.end(slot, nrhs);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4, c16 = lane & 15, bc = c16 & 7;
+    const int part = c16 >> 3;
+    const bool im_out = part == 1;
+    const bool rows_here = 16 * wave < s;  // wave-uniform
+    const int arow = min(16 * wave + c16, s - 1);
+    const double *ind = reinterpret_cast<const double *>(in);
+    double *outd = reinterpret_cast<double *>(out);
+    double *std_ = reinterpret_cast<double *>(st);
+    for (int r0 = rbeg; r0 < rend; r0 += 8) {
+        const int nc = min(8, rend - r0);
+        for (int e = threadIdx.x; e < b * 8; e += 256) {
+            const int k = e >> 3, c = e & 7;
+            sx[e] = c < nc ? out[(int64_t)(r0 + c) * n + bnd[F.bptr + k]] : cd{0.0, 0.0};
+        }
+        __syncthreads();
+        // t = c_P - U12 x_B
+        if (rows_here) {
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
+            mf_mfma_rows(acc, pool + F.off + (int64_t)s * F.f + arow, F.f, sx, b, q, bc, im_out);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int row = 16 * wave + q + 4 * j;
+                double t = 0.0;
+                if (row < s && bc < nc) t = ind[((int64_t)(r0 + bc) * n + F.first + row) * 2 + part] - acc[j];
+                std_[(row * 8 + bc) * 2 + part] = t;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) std_[((16 * wave + q + 4 * j) * 8 + bc) * 2 + part] = 0.0;
+        }
+        __syncthreads();
+        // x_P = U11^-1 t (stored whole: zeros below the diagonal)
+        if (rows_here) {
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
+            mf_mfma_rows(acc, pool + F.inv + (int64_t)s * s + arow, s, st, s, q, bc, im_out);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int row = 16 * wave + q + 4 * j;
+                if (row < s && bc < nc) outd[((int64_t)(r0 + bc) * n + F.first + row) * 2 + part] = acc[j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // x at the boundary of the large fronts of one height, into their w rows
 // s .. f (the backward sweep's right-hand side of U12 x_B).
 __global__ void __launch_bounds__(256) k_mf_gather_b(const int2 *rows, int64_t nrows, const MfFront *fr,
@@ -361,7 +527,6 @@ __global__ void __launch_bounds__(256) k_mf_gemv(const MfTask *tasks, const MfFr
 // to the next (lane l holds A'[l % 16][l / 16], B'[l / 16][l % 16] and
 // D[l / 16 + 4 j][l % 16]): four complex columns per pair of instructions, no
 // wavefront waits on another, and the LDS serves only the staged rows of w.
-typedef double v4d __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) k_mf_gemv8m(const MfTask *tasks, const MfFront *fr, const cd *pool0,
                                                    int64_t stride, const cd *wbuf, int64_t wtot, cd *part, int nrhs,
                                                    ColMap cm)
@@ -731,7 +896,9 @@ int mf_solve(peec_ctx *ctx, bool lower, const double2_t *b, double2_t *x, int nr
 #define MF_FWD(QQ)                                                                                                   \
     hipLaunchKernelGGL(k_mf_fwd_small<QQ>, grid, dim3(256), 0, st, ls, M.fr.p, pool, stride, M.cptr.p, M.csrc.p, b, x, \
                        u, M.utot, n, nrhs, cm)
-                if (Q == 8) MF_FWD(8);
+                if (Q == 8)
+                    hipLaunchKernelGGL(k_mf_fwd_small8m, grid, dim3(256), 0, st, ls, M.fr.p, pool, stride, M.cptr.p,
+                                       M.csrc.p, b, x, u, M.utot, n, nrhs, cm);
                 else if (Q == 4) MF_FWD(4);
                 else MF_FWD(1);
 #undef MF_FWD
@@ -741,7 +908,9 @@ int mf_solve(peec_ctx *ctx, bool lower, const double2_t *b, double2_t *x, int nr
 #define MF_BWD(QQ)                                                                                                   \
     hipLaunchKernelGGL(k_mf_bwd_small<QQ>, grid, dim3(256), 0, st, ls, M.fr.p, pool, stride, M.bnd.p, b, x, n, nrhs, \
                        cm)
-                if (Q == 8) MF_BWD(8);
+                if (Q == 8)
+                    hipLaunchKernelGGL(k_mf_bwd_small8m, grid, dim3(256), 0, st, ls, M.fr.p, pool, stride, M.bnd.p, b,
+                                       x, n, nrhs, cm);
                 else if (Q == 4) MF_BWD(4);
                 else MF_BWD(1);
 #undef MF_BWD
