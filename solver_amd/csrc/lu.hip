@@ -209,65 +209,99 @@ __global__ void __launch_bounds__(256) k_lu_diag(const int4 *items, const LuFron
     for (int e = tid; e < w * w; e += 256) Linv[(int64_t)(e / w) * F.s + e % w] = X[(e / w) * 64 + e % w];
 }
 
-// GEMM tiles (see LuGemmItem).  256 threads, 4 x 4 outputs each; k in
-// chunks of 16 staged through LDS.  A tile may alias C (rows of C read as
-// A) or B may alias C: every read of the tile happens before its write.
+// GEMM tiles (see LuGemmItem) on the matrix cores.  The complex product is
+// the real one C' = A' B' with A' = [Ar Ai] and, for the rows of A' holding
+// column k's real / imaginary parts, B' rows [Br Bi] / [-Bi Br]; each
+// v_mfma_f64_16x16x4f64 takes 16 rows of A against 8 complex columns of B
+// (lane l holds A'[l % 16][l / 16], B'[l / 16][l % 16] and
+// D[l / 16 + 4 j][l % 16]).  Wavefront w owns rows 16 w .. 16 w + 15 of
+// the tile and all 64 columns (8 accumulator blocks).  k goes in chunks of
+// 16: B's chunk is staged through LDS, A is read straight into registers
+// (lane l: row 16 w + l % 16, columns k0 + 4 g + l / 16), and both of the
+// next chunk are loaded while this one is multiplied.  A may alias C (a
+// wavefront reads only its own rows of A, before it writes them) and B may
+// alias C (every read of B ends before the last barrier).
+typedef double v4d __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) k_lu_gemm(const LuGemmItem *items, cd *pool0, int64_t stride)
 {
     cd *pool = pool0 + (int64_t)blockIdx.y * stride;
-    __shared__ cd As[16][65];
-    __shared__ cd Bs[16][65];
+    __shared__ cd Bs[16][66];
     const LuGemmItem it = items[blockIdx.x];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    const cd *Ap = pool + it.a, *Bp = pool + it.b;
-    cd *Cp = pool + it.c;
-    double ax[4][4], ay[4][4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int q = lane >> 4, c16 = lane & 15, bc = c16 & 7;
+    const bool im_out = c16 >= 8;
+    const bool rows_here = 16 * wave < it.m;  // wave-uniform
+    const cd *Ap = pool + it.a + min(16 * wave + c16, it.m - 1);
+    const cd *Bp = pool + it.b;
+    auto load_a = [&](int k0, cd (&a)[4]) {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+        for (int g = 0; g < 4; g++) {
+            const int kk = k0 + 4 * g + q;
+            a[g] = rows_here && kk < it.k ? Ap[(int64_t)kk * it.lda] : cd{0.0, 0.0};
+        }
+    };
+    // B staging: thread tid takes elements e = tid + 256 i of the 16 x 64
+    // chunk, k fastest (kb = e & 15, column e >> 4)
+    auto load_b = [&](int k0, cd (&bv)[4]) {
 #pragma unroll
-        for (int jj = 0; jj < 4; jj++) ax[i][jj] = ay[i][jj] = 0.0;
+        for (int i = 0; i < 4; i++) {
+            const int e = tid + 256 * i, kb = e & 15, c = e >> 4;
+            bv[i] = c < it.n && k0 + kb < it.k ? Bp[(int64_t)c * it.ldb + k0 + kb] : cd{0.0, 0.0};
+        }
+    };
+    v4d acc[8];
+#pragma unroll
+    for (int bl = 0; bl < 8; bl++) acc[bl] = {0.0, 0.0, 0.0, 0.0};
+    cd a[4], bv[4];
+    load_a(0, a);
+    load_b(0, bv);
     for (int k0 = 0; k0 < it.k; k0 += 16) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int e = tid + 256 * q;
-            const int i = e & 63, kk = e >> 6;
-            As[kk][i] = (i < it.m && k0 + kk < it.k) ? Ap[(int64_t)(k0 + kk) * it.lda + i] : cd{0.0, 0.0};
-            const int kb = e & 15, c = e >> 4;
-            Bs[kb][c] = (c < it.n && k0 + kb < it.k) ? Bp[(int64_t)c * it.ldb + k0 + kb] : cd{0.0, 0.0};
+        for (int i = 0; i < 4; i++) {
+            const int e = tid + 256 * i;
+            Bs[e & 15][e >> 4] = bv[i];
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < 16; kk++) {
-            cd a[4], b[4];
+        cd an[4], bn[4];
+        if (k0 + 16 < it.k) {
+            load_a(k0 + 16, an);
+            load_b(k0 + 16, bn);
+        }
+        if (rows_here) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = As[kk][ty + 16 * i];
+            for (int g = 0; g < 4; g++)
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) b[jj] = Bs[kk][tx + 16 * jj];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int jj = 0; jj < 4; jj++) {
-                    ax[i][jj] += a[i].x * b[jj].x - a[i].y * b[jj].y;
-                    ay[i][jj] += a[i].x * b[jj].y + a[i].y * b[jj].x;
+                for (int bl = 0; bl < 8; bl++) {
+                    const cd b = Bs[4 * g + q][8 * bl + bc];
+                    acc[bl] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[g].x, im_out ? b.y : b.x, acc[bl], 0, 0, 0);
+                    acc[bl] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[g].y, im_out ? b.x : -b.y, acc[bl], 0, 0, 0);
                 }
         }
         __syncthreads();
+        if (k0 + 16 < it.k) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                a[i] = an[i];
+                bv[i] = bn[i];
+            }
+        }
     }
+    if (!rows_here) return;
+    double *Cd = reinterpret_cast<double *>(pool + it.c);
+    const int part = c16 >> 3;
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int bl = 0; bl < 8; bl++)
 #pragma unroll
-        for (int jj = 0; jj < 4; jj++) {
-            const int r = ty + 16 * i, c = tx + 16 * jj;
+        for (int j = 0; j < 4; j++) {
+            const int r = 16 * wave + q + 4 * j, c = 8 * bl + bc;
             if (r < it.m && c < it.n) {
-                cd *p = Cp + (int64_t)c * it.ldc + r;
-                if (it.mode == 1) {
-                    const cd o = *p;
-                    *p = {o.x - ax[i][jj], o.y - ay[i][jj]};
-                } else if (it.mode == 2) {
-                    *p = {-ax[i][jj], -ay[i][jj]};
-                } else {
-                    *p = {ax[i][jj], ay[i][jj]};
-                }
+                double *p = Cd + ((int64_t)c * it.ldc + r) * 2 + part;
+                if (it.mode == 1)
+                    *p -= acc[bl][j];
+                else if (it.mode == 2)
+                    *p = -acc[bl][j];
+                else
+                    *p = acc[bl][j];
             }
         }
 }
