@@ -50,11 +50,11 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of a roofline-leg pass (FETCH_SIZE + WRITE_SIZE,
 # scaled by a copy kernel of known size), measured with rocprofv3 --pmc on
 # `bench.py --roofline-only`; the summaries and commands are in profiles/.
-MEASURED_TRAFFIC = {("large_192x192x48", "fft_x_fwd_scatter"): 369.9e6,
-                    ("large_192x192x48", "fft_y_fwd"): 722.1e6,
-                    ("large_192x192x48", "fft_z_conv"): 1142.8e6,
-                    ("large_192x192x48", "fft_y_inv"): 758.3e6,
-                    ("large_192x192x48", "fft_x_inv_gather"): 409.8e6}
+MEASURED_TRAFFIC = {("large_192x192x48", "fft_x_fwd_scatter"): 364.6e6,
+                    ("large_192x192x48", "fft_y_fwd"): 729.4e6,
+                    ("large_192x192x48", "fft_z_conv"): 1126.7e6,
+                    ("large_192x192x48", "fft_y_inv"): 731.1e6,
+                    ("large_192x192x48", "fft_x_inv_gather"): 410.3e6}
 KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv_gather",
               "trsv_block_L", "trsv_block_U", "lu_factor", "fft_yz_conv", "trsv_panel", "mf_small", "mf_gemv",
               "mf_aux"]
