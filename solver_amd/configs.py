@@ -183,14 +183,17 @@ CONFIGS = {
 }
 
 
-def mesh(cfg, p_self_fn=None):
+def mesh(cfg, p_self_fn=None, renumber=True):
     """The operator inputs of a configuration: the reference mesher on one
-    grid per material, or (per-voxel conductivities) mesher.mesh."""
+    grid per material (its edges renumbered x-fastest in each direction,
+    voxel_mesher.renumber_edges, unless renumber is false), or (per-voxel
+    conductivities) mesher.mesh."""
     if cfg.get("sigma") is not None:
         return _mesher.mesh(cfg["grid"], cfg["materials"], cfg["cell"], cfg["origin"], cfg["ports"],
                             p_self_fn=p_self_fn, sigma_grid=cfg["sigma"])
     g = np.asarray(cfg["grid"])
     grids = [g == k + 1 for k in range(len(cfg["materials"]))]
-    return voxel_mesher.mesh_request(grids, cfg["materials"], cfg["cell"], cfg["origin"],
-                                     [m.sigmar > 0 for m in cfg["materials"]], cfg["ports"],
-                                     cfg.get("lumped_elements", ()), p_self_fn=p_self_fn)
+    M = voxel_mesher.mesh_request(grids, cfg["materials"], cfg["cell"], cfg["origin"],
+                                  [m.sigmar > 0 for m in cfg["materials"]], cfg["ports"],
+                                  cfg.get("lumped_elements", ()), p_self_fn=p_self_fn)
+    return voxel_mesher.renumber_edges(M)[0] if renumber else M

@@ -541,3 +541,39 @@ def _surfaces(M, grids, ext, nidx):
     M.N1 = int(np.count_nonzero(maps["12"]))
     M.N2 = int(np.count_nonzero(maps["34"]))
     M.N3 = int(np.count_nonzero(maps["56"]))
+
+
+def renumber_edges(M):
+    """The same mesh with each direction's edges in x-fastest order of their
+    lower node's lattice position (z slowest).  The reference mesher lists
+    the y edges y-fastest and the z edges z-fastest (mesher_FFT.jl), so the
+    nodes of consecutive y / z edges lie a grid row / plane apart; in
+    x-fastest order they are neighbours, like the nodes themselves, and the
+    gathers between edge and node values (A, A') read contiguous runs.
+
+    The operator is the same up to the order of its edge unknowns: the rows
+    of A and of every per-edge array, and the columns of each direction's
+    Lp position map, move together; nodes, surfaces and ports keep their
+    numbers, so port impedances are unchanged.  Returns the mesh and the
+    permutation (new edge k is old edge perm[k])."""
+    A = M.A.tocsr()
+    ijk = np.asarray(M.node_ijk)
+    bounds = (0, M.mx, M.mx + M.my, M.m)
+    perm = []
+    for d in range(3):
+        lo, hi = bounds[d], bounds[d + 1]
+        rows = np.arange(lo, hi)
+        lower = np.minimum.reduceat(A.indices[A.indptr[lo]:A.indptr[hi]], A.indptr[lo:hi] - A.indptr[lo])
+        p = ijk[lower]
+        perm.append(lo + np.lexsort((p[:, 0], p[:, 1], p[:, 2])))
+        assert np.all(np.diff(A.indptr[lo:hi + 1]) > 0)
+    perm = np.concatenate(perm)
+    M.A = A[perm].tocsc()
+    for k in ("Zs", "diag_Lp", "diag_R", "diag_Cd", "li_border", "li_mat"):
+        v = getattr(M, k, None)
+        if v is not None:
+            setattr(M, k, np.asarray(v)[perm])
+    M.mat_map_Lp = [sp.csc_matrix(M.mat_map_Lp[d])[:, perm[bounds[d]:bounds[d + 1]] - bounds[d]]
+                    for d in range(3)]
+    M.surface_edges = np.nonzero(np.abs(M.Zs) != 0)[0]
+    return M, perm

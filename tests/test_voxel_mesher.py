@@ -227,3 +227,41 @@ def test_request_rejects_unlisted_material():
     d["solverInput"]["materials"][0]["name"] = "other"
     with pytest.raises(ValueError):
         RQ.parse(d)
+
+
+def test_renumbered_edges_are_the_same_operator_in_x_fastest_order():
+    """voxel_mesher.renumber_edges (applied by configs.mesh) only relabels
+    the edges: each direction's edges keep their direction group and move
+    to x-fastest order of their lower node; A's rows, every per-edge array
+    and the Lp position map's columns move together, so A'A, the set of
+    (Lp position, edge data) pairs and the ports are unchanged."""
+    import copy
+    import scipy.sparse as sp
+    from solver_amd import configs
+    cfg = configs.two_material_64x64x16()
+    g = np.asarray(cfg["grid"])[:24, :20, :8]
+    grids = [g == k + 1 for k in range(len(cfg["materials"]))]
+    M0 = VM.mesh_request(grids, cfg["materials"], cfg["cell"], cfg["origin"],
+                         [m.sigmar > 0 for m in cfg["materials"]], [])
+    M, perm = VM.renumber_edges(copy.deepcopy(M0))
+    assert sorted(perm.tolist()) == list(range(M0.m))
+    b = (0, M0.mx, M0.mx + M0.my, M0.m)
+    for d in range(3):
+        assert np.all((perm[b[d]:b[d + 1]] >= b[d]) & (perm[b[d]:b[d + 1]] < b[d + 1]))
+    A0, A1 = sp.csr_matrix(M0.A), sp.csr_matrix(M.A)
+    assert (A1 - A0[perm]).nnz == 0
+    assert abs((A0.T @ A0) - (A1.T @ A1)).max() == 0
+    for k in ("Zs", "diag_Lp", "diag_R", "diag_Cd", "li_border", "li_mat"):
+        assert np.array_equal(np.asarray(getattr(M, k)), np.asarray(getattr(M0, k))[perm]), k
+    # the surface edges are the same edges under their new numbers
+    assert np.array_equal(np.sort(perm[M.surface_edges]), M0.surface_edges)
+    ijk = np.asarray(M.node_ijk)
+    for d in range(3):
+        P0 = sp.csc_matrix(M0.mat_map_Lp[d])
+        P1 = sp.csc_matrix(M.mat_map_Lp[d])
+        local = perm[b[d]:b[d + 1]] - b[d]
+        assert (P1 - P0[:, local]).nnz == 0
+        # x-fastest: the lower nodes' lattice keys increase along the group
+        lower = np.array([A1.indices[A1.indptr[e]:A1.indptr[e + 1]].min() for e in range(b[d], b[d + 1])])
+        key = ijk[lower][:, 2] * 10 ** 8 + ijk[lower][:, 1] * 10 ** 4 + ijk[lower][:, 0]
+        assert np.all(np.diff(key) > 0)
