@@ -4,8 +4,9 @@
 // zero-padded grid (compute_Matrix_vector_fft.jl:14-65, 149-161) and builds
 // the kernel spectra with fft(store_circulant_fft(row)) (compute_FFT_mutual_
 // coupling_mats.jl:57-125, 130-155).  Here:
-//   - each axis uses its own smooth length L >= 2n-1 (2^a 3^b 5^c); any such
-//     length gives the same linear convolution as the reference's 2n;
+//   - each axis uses its own smooth length L >= 2n-2 (2^a 3^b 5^c); any such
+//     length gives the same linear convolution as the reference's 2n (the
+//     kernels are even, or half-cell reductions spanning 2n-2 lags);
 //   - the even-symmetric embedding has a real spectrum, stored as a real
 //     array (FP64 and FP32 copies) with 1/(Lx Ly Lz) folded in;
 //   - one convolution = five passes over HBM: x-FFT (fused with the 0/1
@@ -601,7 +602,8 @@ __global__ void k_spec_planes(const T *__restrict__ in, int Lx, int Ly, int Lz, 
 
 // Even-symmetric embedding of a compact row into an L grid
 // (store_circulant_fft, compute_FFT_mutual_coupling_mats.jl:130-155, at
-// lengths L >= 2n - 1 instead of 2n).
+// lengths L >= 2n - 2 instead of 2n: at 2n - 2 the offsets n-1 and -(n-1)
+// share index n - 1, where both hold row(n-1)).
 __global__ void k_embed(const double *__restrict__ row, int nx, int ny, int nz, int Lx, int Ly, int Lz,
                         double2_t *__restrict__ out)
 {
@@ -629,14 +631,25 @@ __global__ void k_embed(const double *__restrict__ row, int nx, int ny, int nz, 
 // even, so it is embedded as a general circulant (offset m at m mod L,
 // m = -(nb-1) .. nb-1) and its spectrum is complex.  Axes without half
 // cells keep the even embedding (delta = 0, f = |m|).
+//
+// At L = 2 nb - 2 the offsets nb-1 and -(nb-1) share i = L/2.  On a half-cell
+// axis the parity-0 positions number nb and the parity-1 ones nb - 1, so a
+// block with delta = +1 (-1) uses the offsets -(nb-1) .. nb-2 (-(nb-2) ..
+// nb-1): 2 nb - 2 of them, and the one it does not use lies outside the row
+// (|2m + delta| = 2 nb - 1 = nf).  With delta = 0, or without half cells,
+// both give the same value.  So i = L/2 takes whichever lies inside.
 __device__ __forceinline__ int poly_offset(int i, int L, int nb, int half, int delta, int nf)
 {
-    int m;
-    if (i <= nb - 1) m = i;
-    else if (L - i <= nb - 1) m = i - L;
-    else return -1;
-    const int f = half ? abs(2 * m + delta) : abs(m);
-    return f < nf ? f : -1;
+    if (i <= nb - 1) {
+        const int f = half ? abs(2 * i + delta) : i;
+        if (f < nf) return f;
+    }
+    if (L - i <= nb - 1) {
+        const int m = i - L;
+        const int f = half ? abs(2 * m + delta) : -m;
+        if (f < nf) return f;
+    }
+    return -1;
 }
 
 __global__ void k_embed_poly(const double *__restrict__ row, int nfx, int nfy, int nfz, int hx, int hy, int hz,
@@ -822,10 +835,16 @@ static int launch_pass_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst,
 }
 #undef PEEC_FFT_CASE
 
+// Shortest circulant of a compact extent n: 2n - 2.  The kernels are even
+// along every axis the operator convolves in full (lags -(n-1) and n-1 share
+// index n-1 and hold the same value), and a half-cell reduction spans 2n - 2
+// lags (poly_offset), so any length >= 2n - 2 gives the linear convolution.
+int fft_embed_min(int n) { return n <= 1 ? 1 : 2 * n - 2; }
+
 // z lengths the register kernel is built for (complex64; complex128 up to
 // 32), and the z embedding length of a compact extent n: the smallest of
-// them >= 2n - 1 (any length >= 2n - 1 embeds the convolution), else the
-// smallest multiple of 16 of the form 16 * 2^a 3^b 5^c.
+// them >= 2n - 2, else the smallest multiple of 16 of the form
+// 16 * 2^a 3^b 5^c.
 static const int kRegLengths[] = {8, 16, 32, 48, 64};
 static bool reg_length(int L, bool fp32)
 {
@@ -836,10 +855,10 @@ static bool reg_length(int L, bool fp32)
 int fft_z_length(int n)
 {
     for (int l : kRegLengths)
-        if (l >= 2 * n - 1) return l;
+        if (l >= fft_embed_min(n)) return l;
     // longer z lines: a multiple of 16 (16 * 2^a 3^b 5^c), so that the lane
     // kernels can split them over 8 or 16 lanes with an even share per lane
-    for (int L = 2 * n - 1;; L++)
+    for (int L = fft_embed_min(n);; L++)
         if (L % 16 == 0 && fft_smooth_length(L / 16) == L / 16) return L;
 }
 
@@ -1136,7 +1155,7 @@ int poly_spectrum_build(peec_ctx *ctx, int k, const int delta[3])
     Kernel &K = ctx->kern[k];
     for (int a = 0; a < 3; a++) {
         K.pn[a] = kHalfAxes[k][a] ? (K.n[a] + 1) / 2 : K.n[a];
-        K.pL[a] = a == 2 ? fft_z_length(K.pn[a]) : fft_smooth_length(2 * K.pn[a] - 1);
+        K.pL[a] = a == 2 ? fft_z_length(K.pn[a]) : fft_smooth_length(fft_embed_min(K.pn[a]));
         if (!register_fft_length(ctx, K.pL[a])) {
             peec_set_error("operator_set: twiddle table allocation failed");
             return PEEC_ERR_NOMEM;
@@ -1173,7 +1192,7 @@ int peec_kernels_build_impl(peec_ctx *ctx, const double origin[3], int Nx, int N
         Kernel &K = ctx->kern[k];
         kernel_dims(k, Nx, Ny, Nz, K.n);
         for (int a = 0; a < 3; a++)
-            K.L[a] = K.n[a] > 0 ? (a == 2 ? fft_z_length(K.n[a]) : fft_smooth_length(2 * K.n[a] - 1)) : 0;
+            K.L[a] = K.n[a] > 0 ? (a == 2 ? fft_z_length(K.n[a]) : fft_smooth_length(fft_embed_min(K.n[a]))) : 0;
         if (K.compact() > 0) Gmax = std::max(Gmax, K.embedded());
     }
     // register every FFT length before the first launch: the twiddle table

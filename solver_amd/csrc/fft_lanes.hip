@@ -471,9 +471,10 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
 {
     constexpr int M = L / Q, LPB = 256 / Q, LQ = log2c<Q>();
     // forward inputs and inverse outputs live in the lower half of the line
-    // (nin, nout <= L/2, checked on the host): with M even that is m < M/2
-    // in every lane, so the zero half is never loaded or transformed and
-    // the unused half of the inverse is never formed
+    // and element L/2 (nin, nout <= L/2 + 1, checked on the host): with M
+    // even the lower half is m < M/2 in every lane, so the zero half is
+    // never loaded or transformed and the unused half of the inverse is
+    // never formed
     constexpr bool HALF = M % 2 == 0;
     constexpr bool CONV = MODE >= 2;
     constexpr uint32_t ES = sizeof(C2<T>);
@@ -501,6 +502,19 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
     const uint32_t in_rev = (uint32_t)(lb * d.in_sb + (int64_t)lane_k0 * d.in_sj) * ES;
     const uint32_t out_nat = (uint32_t)(lb * d.out_sb + (int64_t)q * d.out_sj) * ES;
     const uint32_t out_rev = (uint32_t)(lb * d.out_sb + (int64_t)lane_k0 * d.out_sj) * ES;
+    // Lines of n = L/2 + 1 elements (the circulant embedding of length
+    // 2n - 2, exact for an even kernel and for a half-cell reduction, whose
+    // 2n - 2 lags fit it): element L/2 lies outside the lower half the
+    // lanes transform.  Forward, it adds x[L/2] (-1)^k to every frequency k
+    // (with M and Q even, the parity of k is that of the slot after the lane
+    // exchanges and that of the lane after the LDS exchanges).  Inverse,
+    // y[L/2] is lane 0's element m = M/2, formed from its M-point
+    // coefficients.
+    const bool ext_in = HALF && MODE != 1 && d.nin > L / 2;
+    const bool ext_out = HALF && MODE != 0 && d.nout > L / 2;
+    const uint32_t in_mid = (uint32_t)(lb * d.in_sb + (int64_t)(L / 2) * d.in_sj) * ES;
+    const uint32_t out_mid = (uint32_t)(lb * d.out_sb + (int64_t)(L / 2) * d.out_sj) * ES;
+    auto odd_slot = [&](int i) { return XL ? (q & 1) != 0 : (i & 1) != 0; };
     T sr[CONV ? M : 1], si[MODE == 3 ? M : 1];
     if (CONV) {
         const int64_t sbase = (int64_t)(b0 + lb) * d.spec_sb + (int64_t)l2 * d.spec_s2 + (int64_t)lane_k0 * d.spec_sj;
@@ -526,8 +540,9 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
     // Element addresses: a wave-uniform pointer stepped by the element
     // stride (laundered per call, so that the per-element pointers are not
     // hoisted out of the loop into registers) plus the lane's 32-bit offset.
-    auto load = [&](C2<T>(&v)[NL], int rhs) {
+    auto load = [&](C2<T>(&v)[NL], C2<T> &ve, int rhs) {
         const char *in = (const char *)(src + (int64_t)b0 * d.in_sb + (int64_t)l2 * d.in_s2 + (int64_t)rhs * d.in_rhs);
+        ve = ext_in && live ? ld_at<T>(in, in_mid) : C2<T>{(T)0, (T)0};
         int64_t step = (MODE == 1 ? 1 : Q) * d.in_sj * (int64_t)ES;
         asm volatile("" : "+s"(step));
         if constexpr (MODE == 1 && XL) {
@@ -552,8 +567,8 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
         }
     };
     const int r_end = min(d.nrhs, ((int)bz + 1) * d.rpb);
-    C2<T> nxt[NL];
-    if (PREF && (int)bz * d.rpb < r_end) load(nxt, (int)bz * d.rpb);
+    C2<T> nxt[NL], nxt_e;
+    if (PREF && (int)bz * d.rpb < r_end) load(nxt, nxt_e, (int)bz * d.rpb);
     for (int rhs = (int)bz * d.rpb; rhs < r_end; rhs++) {
         char *out = (char *)(dst + (int64_t)b0 * d.out_sb + (int64_t)l2 * d.out_s2 + (int64_t)rhs * d.out_rhs);
         // the table pointer is laundered per right-hand side so that the
@@ -563,15 +578,21 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
         asm volatile("" : "+v"(twq));
         const C2<T> *twr = s_tw + twq;
         C2<T> a[M];
-        if (!PREF) load(nxt, rhs);
+        if (!PREF) load(nxt, nxt_e, rhs);
 #pragma unroll
         for (int i = 0; i < NL; i++) a[i] = nxt[i];
-        if (PREF && rhs + 1 < r_end) load(nxt, rhs + 1);
+        const C2<T> xe = nxt_e;
+        if (PREF && rhs + 1 < r_end) load(nxt, nxt_e, rhs + 1);
         if (MODE != 1) {
             cx_stages<T, -1, M, 0, HALF>(a);
             lane_twiddle<T, -1, M, Q>(a, twr);
             if constexpr (XL) xlds<T, -1, M, Q>(a, xs, q);
             else xl_fwd<T, M, Q / 2, L, Q>(a, q, xt);
+            if (ext_in) {
+                const C2<T> xn = {-xe.x, -xe.y};
+#pragma unroll
+                for (int i = 0; i < M; i++) a[i] = cadd(a[i], odd_slot(i) ? xn : xe);
+            }
             if (MODE == 0) {
                 if (live) {
                     int64_t step = d.out_sj * (int64_t)ES;
@@ -603,11 +624,19 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
         }
         if constexpr (XL) xlds<T, 1, M, Q>(a, xs, q);
         else xl_inv<T, M, 1, L, Q>(a, q, xt);
+        // element L/2 is m = M/2 of lane 0, whose lane twiddles are 1: the
+        // (-1)^k2-weighted sum of its M-point coefficients
+        C2<T> ye = {(T)0, (T)0};
+        if (ext_out) {
+#pragma unroll
+            for (int k2 = 0; k2 < M; k2++) ye = (k2 & 1) ? csub(ye, a[k2]) : cadd(ye, a[k2]);
+        }
         // laundered again: the forward twiddles are not kept live until here
         asm volatile("" : "+v"(twq));
         lane_twiddle<T, 1, M, Q>(a, s_tw + twq);
         cx_stages<T, 1, M, 0>(a);
         if (live) {
+            if (ext_out && q == 0) st_at<T>(out, out_mid, ye);
             int64_t step = Q * d.out_sj * (int64_t)ES;
             asm volatile("" : "+s"(step));
 #pragma unroll
@@ -646,14 +675,17 @@ bool lane_built(int L, int Q, int mode, bool xl)
 
 // the lane kernel applies to a pass whose tile axis is contiguous and whose
 // lines are strided, with the forward input (modes 0, 2, 3) and the inverse
-// output (modes 1, 2, 3) in the lower half of the line
+// output (modes 1, 2, 3) in the lower half of the line and element L/2
 bool lane_fits(const PassDesc &d, int Q, int mode, bool xl)
 {
     // the kernel's lane offsets are 32-bit byte offsets within L line elements
     const int64_t span = ((int64_t)d.L * std::max(d.in_sj, d.out_sj) + 256) * 16;
+    // an even share per lane (M = L / Q) carries a line of L/2 + 1 elements
+    // (its last one outside the lanes' lower halves); an odd share L/2
+    const int lim = d.L / 2 + ((d.L / std::max(Q, 1)) % 2 == 0 ? 1 : 0);
     return Q >= 2 && d.L % Q == 0 && lane_built(d.L, Q, mode, xl) && d.in_sb == 1 && d.out_sb == 1 && d.in_sj > 1 &&
            d.out_sj > 1 && d.nin <= d.L && d.nout <= d.L && span < ((int64_t)1 << 31) &&
-           (mode == 1 || 2 * d.nin <= d.L) && (mode == 0 || 2 * d.nout <= d.L);
+           (mode == 1 || d.nin <= lim) && (mode == 0 || d.nout <= lim);
 }
 
 template <typename T, int MODE, bool XL>
