@@ -747,8 +747,8 @@ __global__ void k_spec_extract(const double2_t *__restrict__ g, int64_t G, doubl
 
 // ------------------------------------------------------------------ host side
 // lane-split passes (fft_lanes.hip)
-bool lane_fits(const fftk::PassDesc &d, int Q, int mode, bool xl);
-bool lane_built(int L, int Q, int mode, bool xl);
+bool lane_fits(const fftk::PassDesc &d, int Q, int mode, bool xl, bool fp32);
+bool lane_built(int L, int Q, int mode, bool xl, bool map);
 int launch_lane_pass(peec_ctx *ctx, bool fp32, int mode, fftk::PassDesc &d, const void *src, void *dst, int Q,
                      bool xl);
 
@@ -969,7 +969,7 @@ static int launch_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc &d, const vo
     if (d.nb <= 0 || d.n2 <= 0 || d.nrhs <= 0) return PEEC_OK;
     if (is_lane_variant(v)) {
         const int Q = lane_q(v);
-        if (!lane_fits(d, Q, mode, lane_xl(v))) {
+        if (!lane_fits(d, Q, mode, lane_xl(v), fp32)) {
             peec_set_error("lane-split FFT pass does not apply (mode %d, length %d, %d lanes)", mode, d.L, Q);
             return PEEC_ERR_ARG;
         }
@@ -1718,7 +1718,7 @@ static int tune_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc d, const void 
     for (int k = 0; k < kNumLaneV; k++) {
         const int Q = lane_q(kLaneBase + k);
         const bool xl = lane_xl(kLaneBase + k);
-        if (!lane_fits(d, Q, mode, xl)) continue;
+        if (!lane_fits(d, Q, mode, xl, fp32)) continue;
         d.rpb = default_lane_rpb(d, Q);
         float t;
         if ((rc = time_pass(ctx, fp32, mode, d, src, dst, kLaneBase + k, 3, ea, eb, &t))) return rc;
@@ -1884,7 +1884,7 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
     if (!ctx || kernel < 0 || kernel > 8 || pass < 0 || pass > kPassXInvPlanes || nrhs < 1 || variant < -1 ||
         variant >= kLaneBase + kNumLaneV || rpb < 1 || style < 0 || style > 2 || (style != 0 && !fp32) ||
         (fused && !fp32) || (variant == kRegVariant && (pass != 2 || style != 0)) ||
-        (is_lane_variant(variant) && (pass < 1 || pass > 3 || style != 0)) || (pass == kPassYz && variant > 1)) {
+        (is_lane_variant(variant) && (pass == kPassYz || style != 0)) || (pass == kPassYz && variant > 1)) {
         peec_set_error("peec_fft_set_choice: invalid argument");
         return PEEC_ERR_ARG;
     }
@@ -1908,8 +1908,9 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
     const int L = app ? app->L[axis] : 0;
     const bool fits = !app ? false
                       : variant == kRegVariant ? reg_length(L, fp32 != 0) && L % RegFirstRadix(L) == 0
-                      : is_lane_variant(variant) ? L % lane_q(variant) == 0 &&
-                                                       lane_built(L, lane_q(variant), pass == 2 ? 2 : 0, lane_xl(variant))
+                      : is_lane_variant(variant) ? (axis != 0 || fp32 != 0) && L % lane_q(variant) == 0 &&
+                                                       lane_built(L, lane_q(variant), axis == 2 ? 2 : 0, lane_xl(variant),
+                                                                  axis == 0)
                       : variant_fits(variant, L, fp32 != 0);
     if (!fits) {
         peec_set_error("peec_fft_set_choice: launch variant %d does not fit length %d", variant, L);

@@ -5,6 +5,7 @@
 #include "fft_common.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace {
 
@@ -466,7 +467,15 @@ __device__ __forceinline__ void st_at(char *base, uint32_t off, C2<T> v) { *(C2<
 // blockIdx.y, right-hand sides blockIdx.z * rpb ..; src and dst may alias
 // (every lane loads its line before any lane of it stores).  The host
 // checks that the line offsets fit the 32-bit lane offsets.
-template <typename T, int MODE, int L, int Q, bool XL>
+//
+// MAP: the x passes, whose lines are contiguous.  MODE 0 reads its elements
+// from the FP64 unknown vector through the scatter map (d.map_in), MODE 1
+// adds its outputs into it through the gather map (d.map_out); the unknown
+// indices of a lane's elements are read once per block.  The Q lanes of a
+// line then read (MODE 0) and update (MODE 1) Q consecutive unknowns per
+// instruction where the map runs along x, and with the LDS exchanges (XL)
+// also write (MODE 0) and read (MODE 1) Q consecutive frequencies.
+template <typename T, int MODE, int L, int Q, bool XL, bool MAP>
 __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<T> *dst)
 {
     constexpr int M = L / Q, LPB = 256 / Q, LQ = log2c<Q>();
@@ -515,6 +524,23 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
     const uint32_t in_mid = (uint32_t)(lb * d.in_sb + (int64_t)(L / 2) * d.in_sj) * ES;
     const uint32_t out_mid = (uint32_t)(lb * d.out_sb + (int64_t)(L / 2) * d.out_sj) * ES;
     auto odd_slot = [&](int i) { return XL ? (q & 1) != 0 : (i & 1) != 0; };
+    // MAP: unknown index of element n = q + Q i (and of element L/2), -1
+    // where the position holds no unknown or lies past the line
+    constexpr int NMAP = MAP ? (HALF ? M / 2 : M) : 1;
+    int32_t mi[NMAP], mext = -1;
+    if constexpr (MAP) {
+        const int32_t *map = MODE == 0 ? d.map_in : d.map_out;
+        const int64_t sb = MODE == 0 ? d.in_sb : d.out_sb, s2 = MODE == 0 ? d.in_s2 : d.out_s2;
+        const int64_t sj = MODE == 0 ? d.in_sj : d.out_sj;
+        const int nlim = MODE == 0 ? d.nin : d.nout;
+        const int32_t *mp = map + (int64_t)(b0 + lb) * sb + (int64_t)l2 * s2;
+#pragma unroll
+        for (int i = 0; i < NMAP; i++) {
+            const int n = q + Q * i;
+            mi[i] = live && n < nlim ? mp[(int64_t)n * sj] : -1;
+        }
+        if (live && HALF && L / 2 < nlim) mext = mp[(int64_t)(L / 2) * sj];
+    }
     T sr[CONV ? M : 1], si[MODE == 3 ? M : 1];
     if (CONV) {
         const int64_t sbase = (int64_t)(b0 + lb) * d.spec_sb + (int64_t)l2 * d.spec_s2 + (int64_t)lane_k0 * d.spec_sj;
@@ -541,6 +567,17 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
     // stride (laundered per call, so that the per-element pointers are not
     // hoisted out of the loop into registers) plus the lane's 32-bit offset.
     auto load = [&](C2<T>(&v)[NL], C2<T> &ve, int rhs) {
+        if constexpr (MAP && MODE == 0) {
+            const double2_t *vb = d.vec_in + (int64_t)rhs * d.vec_in_ld;
+#pragma unroll
+            for (int i = 0; i < NL; i++) {
+                const double2_t x = mi[i] >= 0 ? vb[mi[i]] : double2_t{0.0, 0.0};
+                v[i] = {(T)x.x, (T)x.y};
+            }
+            const double2_t x = ext_in && mext >= 0 ? vb[mext] : double2_t{0.0, 0.0};
+            ve = {(T)x.x, (T)x.y};
+            return;
+        }
         const char *in = (const char *)(src + (int64_t)b0 * d.in_sb + (int64_t)l2 * d.in_s2 + (int64_t)rhs * d.in_rhs);
         ve = ext_in && live ? ld_at<T>(in, in_mid) : C2<T>{(T)0, (T)0};
         int64_t step = (MODE == 1 ? 1 : Q) * d.in_sj * (int64_t)ES;
@@ -635,6 +672,20 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
         asm volatile("" : "+v"(twq));
         lane_twiddle<T, 1, M, Q>(a, s_tw + twq);
         cx_stages<T, 1, M, 0>(a);
+        if constexpr (MAP && MODE == 1) {
+            double2_t *vb = d.vec_out + (int64_t)rhs * d.vec_out_ld;
+            auto acc = [&](int32_t u, C2<T> v) {
+                double2_t t = vb[u];
+                t.x += (double)v.x;
+                t.y += (double)v.y;
+                vb[u] = t;
+            };
+#pragma unroll
+            for (int m = 0; m < NMAP; m++)
+                if (mi[m] >= 0) acc(mi[m], a[m]);
+            if (ext_out && q == 0 && mext >= 0) acc(mext, ye);
+            continue;
+        }
         if (live) {
             if (ext_out && q == 0) st_at<T>(out, out_mid, ye);
             int64_t step = Q * d.out_sj * (int64_t)ES;
@@ -650,19 +701,26 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
 
 // (L, Q) pairs k_lines is built for: z convolutions (MODE 2, 3) and y
 // passes (MODE 0, 1) of the workloads' lengths, at most 64 points per lane;
-// with the LDS exchanges (XL) where Q <= 8 divides M = L / Q.
+// with the LDS exchanges (XL) where Q <= 8 divides M = L / Q; the x passes
+// with their maps (complex64 only).
 #define PEEC_LANES_Z(X) X(16, 4) X(32, 2) X(32, 4) X(32, 8) X(48, 2) X(48, 4) X(48, 8) X(64, 2) X(64, 4) X(64, 8) \
     X(64, 16) X(96, 4) X(96, 8) X(96, 16) X(128, 4) X(128, 8) X(128, 16)
 #define PEEC_LANES_Y(X) X(128, 4) X(128, 8) X(128, 16) X(192, 4) X(192, 8) X(192, 16) X(200, 4) X(200, 8) X(384, 8) \
     X(384, 16) X(384, 32) X(400, 8) X(400, 16)
 #define PEEC_LANES_ZX(X) X(32, 4) X(48, 4) X(64, 4) X(64, 8) X(96, 4) X(128, 4) X(128, 8)
 #define PEEC_LANES_YX(X) X(128, 4) X(128, 8) X(192, 4) X(192, 8) X(384, 8)
+#define PEEC_LANES_XM(X) X(128, 8) X(128, 16) X(192, 8) X(192, 16) X(384, 16) X(384, 32)
+#define PEEC_LANES_XMX(X) X(128, 8) X(192, 8) X(384, 8)
 
-bool lane_built(int L, int Q, int mode, bool xl)
+bool lane_built(int L, int Q, int mode, bool xl, bool map)
 {
     bool ok = false;
 #define PEEC_LANE_HAS(L_, Q_) ok = ok || (L == L_ && Q == Q_);
-    if (xl) {
+    if (map) {
+        if (mode >= 2) return false;
+        if (xl) { PEEC_LANES_XMX(PEEC_LANE_HAS) }
+        else { PEEC_LANES_XM(PEEC_LANE_HAS) }
+    } else if (xl) {
         if (mode >= 2) { PEEC_LANES_ZX(PEEC_LANE_HAS) }
         else { PEEC_LANES_YX(PEEC_LANE_HAS) }
     } else {
@@ -673,19 +731,29 @@ bool lane_built(int L, int Q, int mode, bool xl)
     return ok;
 }
 
-// the lane kernel applies to a pass whose tile axis is contiguous and whose
-// lines are strided, with the forward input (modes 0, 2, 3) and the inverse
-// output (modes 1, 2, 3) in the lower half of the line and element L/2
-bool lane_fits(const PassDesc &d, int Q, int mode, bool xl)
+// whether the pass is an x pass with its map (forward scatter, inverse gather)
+static bool lane_map(const PassDesc &d, int mode)
 {
-    // the kernel's lane offsets are 32-bit byte offsets within L line elements
-    const int64_t span = ((int64_t)d.L * std::max(d.in_sj, d.out_sj) + 256) * 16;
+    return (mode == 0 && d.map_in != nullptr) || (mode == 1 && d.map_out != nullptr);
+}
+
+// The lane kernel applies to a pass whose tile axis is contiguous and whose
+// lines are strided, or to an x pass with its map (complex64); the forward
+// input (modes 0, 2, 3) and the inverse output (modes 1, 2, 3) lie in the
+// lower half of the line and element L/2.
+bool lane_fits(const PassDesc &d, int Q, int mode, bool xl, bool fp32)
+{
+    const bool map = lane_map(d, mode);
+    // the kernel's lane offsets are 32-bit byte offsets within a tile
+    const int64_t sb = std::max(d.in_sb, d.out_sb), sj = std::max(d.in_sj, d.out_sj);
+    const int64_t span = ((int64_t)(256 / std::max(Q, 1)) * sb + (int64_t)d.L * sj + 256) * 16;
     // an even share per lane (M = L / Q) carries a line of L/2 + 1 elements
     // (its last one outside the lanes' lower halves); an odd share L/2
     const int lim = d.L / 2 + ((d.L / std::max(Q, 1)) % 2 == 0 ? 1 : 0);
-    return Q >= 2 && d.L % Q == 0 && lane_built(d.L, Q, mode, xl) && d.in_sb == 1 && d.out_sb == 1 && d.in_sj > 1 &&
-           d.out_sj > 1 && d.nin <= d.L && d.nout <= d.L && span < ((int64_t)1 << 31) &&
-           (mode == 1 || d.nin <= lim) && (mode == 0 || d.nout <= lim);
+    const bool layout = map ? fp32 && (mode == 0 ? d.vec_in != nullptr : d.vec_out != nullptr)
+                            : d.in_sb == 1 && d.out_sb == 1 && d.in_sj > 1 && d.out_sj > 1 && !d.map_in && !d.map_out;
+    return Q >= 2 && d.L % Q == 0 && lane_built(d.L, Q, mode, xl, map) && layout && d.nin <= d.L &&
+           d.nout <= d.L && span < ((int64_t)1 << 31) && (mode == 1 || d.nin <= lim) && (mode == 0 || d.nout <= lim);
 }
 
 template <typename T, int MODE, bool XL>
@@ -695,13 +763,24 @@ static int launch_lines_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst
     dim3 grid((unsigned)((d.nb + lpb - 1) / lpb), (unsigned)d.n2, (unsigned)((d.nrhs + rpb - 1) / rpb));
     const C2<T> *s = (const C2<T> *)src;
     C2<T> *o = (C2<T> *)dst;
+    const bool map = lane_map(d, MODE);
     bool done = false;
 #define PEEC_LANE_LAUNCH(L_, Q_)                                                                                   \
     if (!done && d.L == L_ && Q == Q_) {                                                                           \
-        hipLaunchKernelGGL((k_lines<T, MODE, L_, Q_, XL>), grid, dim3(256), 0, ctx->stream, d, s, o);          \
+        hipLaunchKernelGGL((k_lines<T, MODE, L_, Q_, XL, false>), grid, dim3(256), 0, ctx->stream, d, s, o);   \
         done = true;                                                                                               \
     }
-    if constexpr (XL) {
+#define PEEC_LANE_LAUNCH_MAP(L_, Q_)                                                                               \
+    if (!done && d.L == L_ && Q == Q_) {                                                                           \
+        hipLaunchKernelGGL((k_lines<T, MODE, L_, Q_, XL, true>), grid, dim3(256), 0, ctx->stream, d, s, o);    \
+        done = true;                                                                                               \
+    }
+    if (map) {
+        if constexpr (MODE < 2 && std::is_same<T, float>::value) {
+            if constexpr (XL) { PEEC_LANES_XMX(PEEC_LANE_LAUNCH_MAP) }
+            else { PEEC_LANES_XM(PEEC_LANE_LAUNCH_MAP) }
+        }
+    } else if constexpr (XL) {
         if constexpr (MODE >= 2) { PEEC_LANES_ZX(PEEC_LANE_LAUNCH) }
         else { PEEC_LANES_YX(PEEC_LANE_LAUNCH) }
     } else {
@@ -709,9 +788,10 @@ static int launch_lines_t(peec_ctx *ctx, PassDesc &d, const void *src, void *dst
         else { PEEC_LANES_Y(PEEC_LANE_LAUNCH) }
     }
 #undef PEEC_LANE_LAUNCH
+#undef PEEC_LANE_LAUNCH_MAP
     if (!done) {
-        peec_set_error("lane-split FFT pass: no kernel for length %d with %d lanes per line%s", d.L, Q,
-                       XL ? " (LDS exchanges)" : "");
+        peec_set_error("lane-split FFT pass: no kernel for length %d with %d lanes per line%s%s", d.L, Q,
+                       XL ? " (LDS exchanges)" : "", map ? " (x pass with map)" : "");
         return PEEC_ERR_ARG;
     }
     PEEC_HIP_TRY(hipGetLastError());

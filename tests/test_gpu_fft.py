@@ -289,3 +289,66 @@ def test_lane_split_passes_match(lanes_geometry, fp32):
                       flush=True)
                 assert e < 1e-12
             clear_all(S, nrhs)
+
+
+@pytest.fixture(scope="module")
+def xlanes_geometry():
+    """96 x 10 x 6: 192-point x lines, where the x passes can run the
+    lane-split kernel with their scatter and gather maps."""
+    cell = (0.25e-3, 0.25e-3, 0.05e-3)
+    g, mats, _ = configs._layered(96, 10, 6, cell, 4, 0.45, 4.4, 0.02, seed=11)
+    ports = configs._x_face_ports(g, cell, 4)
+    M = mesher.mesh(g, mats, cell, (0.0, 0.0, 0.0), ports, p_self_fn=pself)
+    S = sweep.Solver(M, ctx=peec.Context(0), device_lu=True)
+    ws = [2 * np.pi * f * M.escalings["freq"] for f in (3e7, 8e9)]
+    peec.frequency_slots(2, ctx=S.ctx)
+    for slot, w in enumerate(ws):
+        Yle = sweep.build_Yle_FFT(M.lumped_elements, M.port_nodes, M.escalings, M.n, w / M.escalings["freq"], 50.0)
+        peec.frequency_set_slot(slot, w, sweep.DZ_of(M, w), Yle, ctx=S.ctx)
+    N = M.m + M.ns + M.n
+    rng = np.random.default_rng(23)
+    X = rng.normal(size=(N, 4)) + 1j * rng.normal(size=(N, 4))
+    return M, S, ws, X
+
+
+def test_x_pass_lanes_match(xlanes_geometry):
+    """Every lane-split x pass built for 192 points (scatter MODE 0 and
+    gather MODE 1, lane exchanges and LDS exchanges, 1 and 3 right-hand
+    sides per block) against complex128 and the oracle's K x."""
+    M, S, ws, X = xlanes_geometry
+    nrhs, rps = X.shape[1], 2
+    clear_all(S, nrhs)
+    y64 = peec.apply_K_batched(X, rps, fp32=False, ctx=S.ctx)
+    ref = peec.apply_K_batched(X, rps, fp32=True, ctx=S.ctx)
+    rows = OF.kernel_rows(M.origin, M.Nx, M.Ny, M.Nz, M.sx, M.sy, M.sz)
+    op = OF.Operator(M, OF.spectra(rows, M.Nx, M.Ny, M.Nz))
+    yo = np.empty_like(X)
+    for r in range(nrhs):
+        w = ws[r // rps]
+        Yle = OF.build_Yle(M.lumped_elements, M.port_nodes, M.escalings["Yle"], M.n, w / M.escalings["freq"], 50.0)
+        op.set_frequency(w, OF.frequency_terms(M, w), Yle)
+        yo[:, r] = np.concatenate(op.K(X[:, r]))
+    assert rel(y64, yo) < 1e-12
+    total = 0
+    for Q, xl in ((8, False), (16, False), (8, True)):
+        v = LANE_XL_BASE + int(np.log2(Q)) - 2 if xl else LANE_BASE + int(np.log2(Q)) - 1
+        for rpb in (1, 3):
+            took = 0
+            for k in range(9):
+                set_fused(S, k, False, nrhs=nrhs)
+                for p in (0, 4):
+                    try:
+                        peec.fft_set_choice(k, p, True, nrhs, v, rpb, 0, ctx=S.ctx)
+                        took += 1
+                    except _lib.PeecError:
+                        peec.fft_set_choice(k, p, True, nrhs, -1, ctx=S.ctx)
+            if took == 0:
+                continue
+            total += took
+            Y = peec.apply_K_batched(X, rps, fp32=True, ctx=S.ctx)
+            e, e64, eo = rel(Y, ref), rel(Y, y64), rel(Y, yo)
+            print("x passes Q %d%s rpb %d on %d passes: vs LDS x passes %.2e, vs complex128 %.2e, vs oracle %.2e"
+                  % (Q, " (LDS exchanges)" if xl else "", rpb, took, e, e64, eo), flush=True)
+            assert e64 < 1e-5 and eo < 1e-5
+        clear_all(S, nrhs)
+    assert total >= 12
