@@ -50,27 +50,30 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of a roofline-leg pass (FETCH_SIZE + WRITE_SIZE,
 # scaled by a copy kernel of known size), measured with rocprofv3 --pmc on
 # `bench.py --roofline-only`; the summaries and commands are in profiles/.
-MEASURED_TRAFFIC = {("large_192x192x48", "fft_x_fwd_scatter"): 364.6e6,
-                    ("large_192x192x48", "fft_y_fwd"): 729.4e6,
-                    ("large_192x192x48", "fft_z_conv"): 1126.7e6,
-                    ("large_192x192x48", "fft_y_inv"): 731.1e6,
-                    ("large_192x192x48", "fft_x_inv_gather"): 410.3e6}
+MEASURED_TRAFFIC = {("large_192x192x48", "fft_x_fwd_scatter"): 330.9e6,
+                    ("large_192x192x48", "fft_y_fwd"): 693.9e6,
+                    ("large_192x192x48", "fft_z_conv"): 1002.2e6,
+                    ("large_192x192x48", "fft_y_inv"): 699.3e6,
+                    ("large_192x192x48", "fft_x_inv_gather"): 348.3e6}
 KIND_NAMES = ["fft_x_fwd_scatter", "fft_y_fwd", "fft_z_conv", "fft_y_inv", "fft_x_inv_gather", "trsv_gather",
               "trsv_block_L", "trsv_block_U", "lu_factor", "fft_yz_conv", "trsv_panel", "mf_small", "mf_gemv",
               "mf_aux"]
 # kernel symbols behind each profile kind: the LDS pass k_fft_pass<T, MODE,
 # TPL, LPT, LMAX> (MODE 0 forward, 1 inverse, 2/3 forward * spectrum *
 # inverse), the register z kernel k_zconv_reg<T, MODE, L> and the
-# lane-split pass k_lines<T, MODE, L, Q>, as the tuner chooses per pass
-KERNEL_SYMBOLS = {"fft_x_fwd_scatter": "k_fft_pass<T, 0, TPL, LPT, LMAX>",
+# lane-split pass k_lines<T, MODE, L, Q, XL, MAP> (MAP: the x passes with
+# their scatter / gather maps), as the tuner chooses per pass
+KERNEL_SYMBOLS = {"fft_x_fwd_scatter": "k_lines<float, 0, L, Q, XL, true> or k_fft_pass<T, 0, TPL, LPT, LMAX> (tuned)",
                   "fft_y_fwd": "k_lines<T, 0, L, Q> or k_fft_pass<T, 0, TPL, LPT, LMAX> (tuned)",
                   "fft_z_conv": "k_lines<T, 2|3, L, Q>, k_zconv_reg<T, 2|3, L> or k_fft_pass<T, 2|3, ...> (tuned)",
                   "fft_y_inv": "k_lines<T, 1, L, Q> or k_fft_pass<T, 1, TPL, LPT, LMAX> (tuned)",
-                  "fft_x_inv_gather": "k_fft_pass<T, 1, TPL, LPT, LMAX>", "trsv_gather": "k_trsv_gather4",
+                  "fft_x_inv_gather": "k_lines<float, 1, L, Q, XL, true> or k_fft_pass<T, 1, TPL, LPT, LMAX> (tuned)",
+                  "trsv_gather": "k_trsv_gather4",
                   "trsv_block_L": "k_trsv_block<true>", "trsv_block_U": "k_trsv_block<false>",
                   "lu_factor": "peec_lu_factor (all its kernels)",
                   "fft_yz_conv": "k_yz_conv<float, MODE, 32, LPT, 256, LZ>", "trsv_panel": "k_trsv_panel",
-                  "mf_small": "k_mf_fwd_small<Q>, k_mf_bwd_small<Q>", "mf_gemv": "k_mf_gemv<Q, GATHER>",
+                  "mf_small": "k_mf_fwd_small8m, k_mf_bwd_small8m (or k_mf_fwd_small<Q>, k_mf_bwd_small<Q>)",
+                  "mf_gemv": "k_mf_gemv8m (or k_mf_gemv<Q>)",
                   "mf_aux": "k_mf_assemble, k_mf_finish<KIND>"}
 
 
