@@ -1,7 +1,8 @@
 // Lane-split FFT passes: strided lines (the y passes and the fused z
-// convolution) transformed in registers with Q lanes per line and
-// cross-lane butterflies.  Part of the circulant convolution pipeline of
-// fft.hip, which picks these kernels through its autotuner.
+// convolution) and the contiguous x lines with their scatter / gather maps,
+// transformed in registers with Q lanes per line and cross-lane
+// butterflies.  Part of the circulant convolution pipeline of fft.hip,
+// which picks these kernels through its autotuner.
 #include "fft_common.h"
 
 #include <algorithm>
