@@ -542,7 +542,12 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
         }
         if (live && HALF && L / 2 < nlim) mext = mp[(int64_t)(L / 2) * sj];
     }
-    T sr[CONV ? M : 1], si[MODE == 3 ? M : 1];
+    // MODE 3 with the lane exchanges and at most 48 KiB of it per block: the
+    // complex spectrum waits in LDS, lane-private and k2-major (conflict-free
+    // 8-byte reads), instead of in 2M VGPRs, so that three waves fit a SIMD
+    constexpr bool SPEC_LDS = MODE == 3 && !XL && M * 256 * (int)sizeof(C2<T>) <= 48 * 1024;
+    __shared__ C2<T> s_spec[SPEC_LDS ? M * 256 : 1];
+    T sr[CONV && !SPEC_LDS ? M : 1], si[MODE == 3 && !SPEC_LDS ? M : 1];
     if (CONV) {
         const int64_t sbase = (int64_t)(b0 + lb) * d.spec_sb + (int64_t)l2 * d.spec_s2 + (int64_t)lane_k0 * d.spec_sj;
 #pragma unroll
@@ -552,8 +557,12 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
                 sr[k2] = live ? reinterpret_cast<const T *>(d.spec)[s] : (T)0;
             } else {
                 const C2<T> c = live ? reinterpret_cast<const C2<T> *>(d.spec)[s] : C2<T>{(T)0, (T)0};
-                sr[k2] = c.x;
-                si[MODE == 3 ? k2 : 0] = d.spec_conj ? -c.y : c.y;
+                if constexpr (SPEC_LDS) {
+                    s_spec[k2 * 256 + (int)threadIdx.x] = {c.x, d.spec_conj ? -c.y : c.y};
+                } else {
+                    sr[SPEC_LDS ? 0 : k2] = c.x;
+                    si[SPEC_LDS ? 0 : k2] = d.spec_conj ? -c.y : c.y;
+                }
             }
         }
     }
@@ -654,10 +663,18 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc d, const C2<T> *src, C2<
                 }
                 continue;
             }
+            if constexpr (SPEC_LDS) {
+                // laundered per right-hand side: read where used, not hoisted
+                int so = (int)threadIdx.x;
+                asm volatile("" : "+v"(so));
 #pragma unroll
-            for (int k2 = 0; k2 < M; k2++) {
-                if (MODE == 2) a[k2] = cscale(a[k2], sr[k2]);
-                else a[k2] = fmul(a[k2], C2<T>{sr[k2], si[MODE == 3 ? k2 : 0]});
+                for (int k2 = 0; k2 < M; k2++) a[k2] = fmul(a[k2], s_spec[k2 * 256 + so]);
+            } else {
+#pragma unroll
+                for (int k2 = 0; k2 < M; k2++) {
+                    if (MODE == 2) a[k2] = cscale(a[k2], sr[k2]);
+                    else a[k2] = fmul(a[k2], C2<T>{sr[k2], si[MODE == 3 ? k2 : 0]});
+                }
             }
         }
         if constexpr (XL) xlds<T, 1, M, Q>(a, xs, q);
