@@ -340,7 +340,11 @@ int peec_fft_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int
 /* Sets the choice of one pass by hand (a tuning table kept from an earlier
  * run, or a test of every launch shape): variant -1 restores the default
  * (for pass 6: the five passes).  Non-zero radix schedules, and passes
- * 5..7, are accepted for complex64 only. */
+ * 5..7, are accepted for complex64 only.  Variants 12..16 (2..32 lanes per
+ * line, lane exchanges) and 17..18 (4 and 8 lanes, LDS exchanges) are the
+ * lane-split kernel: on the y and z passes in both precisions, on the x
+ * passes (0, 4, 5, 7, with their maps) in complex64, for the lengths it is
+ * built for. */
 int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int variant, int rpb, int style);
 
 /* K x (precond = 0), M^-1 K x (precond = 1) or M^-1 x (precond = 2, the
