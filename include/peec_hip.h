@@ -343,8 +343,8 @@ int peec_fft_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int
  * 5..7, are accepted for complex64 only.  Variants 12..16 (2..32 lanes per
  * line, lane exchanges) and 17..18 (4 and 8 lanes, LDS exchanges) are the
  * lane-split kernel: on the y and z passes in both precisions, on the x
- * passes (0, 4, 5, 7, with their maps) in complex64, for the lengths it is
- * built for. */
+ * passes 0 and 4 (with their maps) in complex64, for the lengths it is
+ * built for; not on passes 5..7. */
 int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32, int nrhs, int variant, int rpb, int style);
 
 /* K x (precond = 0), M^-1 K x (precond = 1) or M^-1 x (precond = 2, the

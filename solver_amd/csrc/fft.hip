@@ -1715,7 +1715,9 @@ static int tune_pass(peec_ctx *ctx, bool fp32, int mode, PassDesc d, const void 
             best.style = 0;
         }
     }
-    for (int k = 0; k < kNumLaneV; k++) {
+    // (the x passes into and out of planes, passes 5 and 7, keep the LDS
+    // kernel: the lane kernel is tested on the five-pass layout only)
+    for (int k = 0; k < kNumLaneV && pass != kPassXFwdPlanes && pass != kPassXInvPlanes; k++) {
         const int Q = lane_q(kLaneBase + k);
         const bool xl = lane_xl(kLaneBase + k);
         if (!lane_fits(d, Q, mode, xl, fp32)) continue;
@@ -1884,7 +1886,7 @@ extern "C" int peec_fft_set_choice(peec_ctx *ctx, int kernel, int pass, int fp32
     if (!ctx || kernel < 0 || kernel > 8 || pass < 0 || pass > kPassXInvPlanes || nrhs < 1 || variant < -1 ||
         variant >= kLaneBase + kNumLaneV || rpb < 1 || style < 0 || style > 2 || (style != 0 && !fp32) ||
         (fused && !fp32) || (variant == kRegVariant && (pass != 2 || style != 0)) ||
-        (is_lane_variant(variant) && (pass == kPassYz || style != 0)) || (pass == kPassYz && variant > 1)) {
+        (is_lane_variant(variant) && (fused || style != 0)) || (pass == kPassYz && variant > 1)) {
         peec_set_error("peec_fft_set_choice: invalid argument");
         return PEEC_ERR_ARG;
     }
